@@ -1,0 +1,34 @@
+# Builds the reference's two single-threaded CPU counters straight from their
+# sources under /root/reference into oracle/_ref/ (git-ignored). They are
+# checkers only: tests/golden/make_golden.py runs them to write fixtures, and
+# bench.py may time them as the "reference" CPU baseline. Flags follow the
+# reference notebooks (RunBenchmark.ipynb cells 5-6: g++ -O3 -std=c++17).
+#
+#   make -f oracle/ref.mk            (from the repo root)
+#
+# Nothing here is copied into the repository; the sources are compiled where
+# they lie.
+
+REF      ?= /root/reference
+OUT      ?= oracle/_ref
+CXX      ?= g++
+CXXFLAGS ?= -O3 -std=c++17
+
+REWRITTEN_SRC := $(REF)/Rewritten_CPP/main.cpp $(REF)/Rewritten_CPP/RdfIndexes.cpp \
+                 $(REF)/Rewritten_CPP/RuleParser.cpp $(REF)/Rewritten_CPP/FinalRule.cpp \
+                 $(REF)/Rewritten_CPP/SupportCounting.cpp
+CPU01_SRC     := $(REF)/CPU_01_nonparallel/main.cpp $(REF)/CPU_01_nonparallel/RdfIndexes.cpp \
+                 $(REF)/CPU_01_nonparallel/RuleParser.cpp $(REF)/CPU_01_nonparallel/FinalRule.cpp \
+                 $(REF)/CPU_01_nonparallel/SupportCounting.cpp
+
+all: $(OUT)/rewritten $(OUT)/cpu01
+
+$(OUT)/rewritten: $(REWRITTEN_SRC)
+	@mkdir -p $(OUT)
+	$(CXX) $(CXXFLAGS) -I$(REF)/Rewritten_CPP $(REWRITTEN_SRC) -o $@
+
+$(OUT)/cpu01: $(CPU01_SRC)
+	@mkdir -p $(OUT)
+	$(CXX) $(CXXFLAGS) -I$(REF)/CPU_01_nonparallel $(CPU01_SRC) -o $@
+
+.PHONY: all

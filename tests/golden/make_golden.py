@@ -1,0 +1,185 @@
+"""Writes the golden fixtures under tests/golden/ by running the reference's
+own CPU counters, compiled from their sources by oracle/ref.mk into
+oracle/_ref/ (Rewritten_CPP -> rewritten, CPU_01_nonparallel -> cpu01).
+
+Each fixture is data only: the generator name and arguments that rebuild the
+graph and rules from a seed (plus a sha256 of the generated triples, or the
+triples themselves for tiny graphs), the rule lines fed to the reference, and
+the integers the two reference programs printed for every rule.
+
+    python tests/golden/make_golden.py            # all fixture sets
+    python tests/golden/make_golden.py tiny c1    # selected sets
+
+Run in the build container only (it needs /root/reference); the GPU box and
+the test suite only read the committed JSON.
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from masterthesis_gpu_acceleration_amd import workloads as W  # noqa: E402
+
+REF_DIR = os.path.join(ROOT, "oracle", "_ref")
+
+
+def graph_sha(g) -> str:
+    h = hashlib.sha256()
+    for a in (g.s, g.p, g.o):
+        h.update(np.ascontiguousarray(a, np.int32).tobytes())
+    return h.hexdigest()
+
+
+def rule_to_json(rule):
+    head, body = rule
+    enc = lambda a: [list(a[0]), int(a[1]), list(a[2])]  # noqa: E731
+    return {"head": enc(head), "body": [enc(a) for a in body]}
+
+
+def rule_from_json(d):
+    dec = lambda a: ((a[0][0], a[0][1]), int(a[1]), (a[2][0], a[2][1]))  # noqa: E731
+    return (dec(d["head"]), [dec(a) for a in d["body"]])
+
+
+def run_cpu01(ttl, rules_txt, timeout):
+    t = time.time()
+    out = subprocess.run([os.path.join(REF_DIR, "cpu01"), ttl, rules_txt, "--json"],
+                         capture_output=True, text=True, timeout=timeout, check=True).stdout
+    dt = time.time() - t
+    rows = json.loads(out)
+    return [[r["support"], r["headSize"], r["headSupport"], r["bodySize"]] for r in rows], dt
+
+
+_RW = re.compile(r"support=(-?\d+)\s+headCoverage=\S+\s+bodySize=(-?\d+)")
+
+
+def run_rewritten(ttl, rules_txt, timeout):
+    t = time.time()
+    out = subprocess.run([os.path.join(REF_DIR, "rewritten"), ttl, rules_txt],
+                         capture_output=True, text=True, timeout=timeout, check=True).stdout
+    dt = time.time() - t
+    return [[int(a), int(b)] for a, b in _RW.findall(out)], dt
+
+
+def make_fixture(name, graph, rules, generator=None, inline_triples=False, timeout=1800):
+    """Runs both reference programs on (graph, rules). Rules with 8 body atoms
+    are withheld from Rewritten_CPP: its used-triple buffer holds 8 entries
+    (Rewritten_CPP/SupportCounting.hpp:20) and support needs 9."""
+    with tempfile.TemporaryDirectory() as d:
+        ttl = os.path.join(d, "g.ttl")
+        W.write_ttl(graph, ttl)
+        allr = os.path.join(d, "all.txt")
+        W.write_rules(rules, allr)
+        cpu01, t_cpu01 = run_cpu01(ttl, allr, timeout)
+        rw_idx = [i for i, r in enumerate(rules) if len(r[1]) <= 7]
+        rwf = os.path.join(d, "rw.txt")
+        W.write_rules([rules[i] for i in rw_idx], rwf)
+        rw_rows, t_rw = run_rewritten(ttl, rwf, timeout)
+    assert len(cpu01) == len(rules), (len(cpu01), len(rules))
+    assert len(rw_rows) == len(rw_idx)
+    rewritten = [None] * len(rules)
+    for i, row in zip(rw_idx, rw_rows):
+        rewritten[i] = row
+    fx = {
+        "name": name,
+        "generator": generator,
+        "n_entities": graph.n_entities,
+        "n_predicates": graph.n_predicates,
+        "n_triples": graph.n_triples,
+        "graph_sha256": graph_sha(graph),
+        "rules": [rule_to_json(r) for r in rules],
+        "rules_text": [W.rule_text(r) for r in rules],
+        "cpu01": cpu01,            # [support, headSize, headSupport, bodySize] per rule
+        "rewritten": rewritten,    # [support, bodySize] per rule, null if withheld
+        "ref_seconds": {"cpu01": round(t_cpu01, 3), "rewritten": round(t_rw, 3)},
+    }
+    if inline_triples:
+        fx["triples"] = [graph.s.tolist(), graph.p.tolist(), graph.o.tolist()]
+    path = os.path.join(HERE, name + ".json")
+    with open(path, "w") as f:
+        json.dump(fx, f, separators=(",", ":"))
+    print("%-22s rules=%5d triples=%8d cpu01 %.2fs rewritten %.2fs -> %s" % (
+        name, len(rules), graph.n_triples, t_cpu01, t_rw, os.path.relpath(path, ROOT)))
+
+
+# ---------------------------------------------------------------- fixture sets
+
+def fx_tiny():
+    """Tiny dense graphs with self-loops and random rules exercising
+    constants, reflexive heads, repeated and disconnected atoms, 1-8 atoms."""
+    for gi in range(6):
+        g = W.tiny_random(1000 + gi, n_entities=7 + gi % 3, n_predicates=2 + gi % 2,
+                          n_triples=24 + 6 * gi)
+        rng = np.random.default_rng(2000 + gi)
+        rules = []
+        for k in range(60):
+            nb = 1 + k % 8
+            rules.append(W.random_rule(rng, g.n_entities, g.n_predicates, nb,
+                                       n_vars=3 + k % 3, p_const=0.12,
+                                       closed=(k % 7 != 3), reflexive_head=(k % 11 == 5)))
+        make_fixture("tiny_%d" % gi, g, rules, inline_triples=True)
+
+
+def fx_small_paths():
+    kw = dict(n_entities=2000, n_predicates=10, n_triples=12000, n_rules=40)
+    w = W.small_paths(**kw)
+    make_fixture("small_paths_2k", w.graph, w.rules, {"fn": "small_paths", "kwargs": kw})
+
+
+def fx_c1():
+    w = W.small_paths()
+    make_fixture("small_paths_full", w.graph, w.rules, {"fn": "small_paths", "kwargs": {}})
+
+
+def fx_dbpedia_small():
+    kw = dict(n_entities=5000, n_predicates=60, n_triples=20000, n_rules=400)
+    w = W.dbpedia_shaped(**kw)
+    make_fixture("dbpedia_shaped_5k", w.graph, w.rules, {"fn": "dbpedia_shaped", "kwargs": kw})
+
+
+def fx_c2():
+    w = W.dbpedia_shaped()
+    make_fixture("dbpedia_shaped_full", w.graph, w.rules, {"fn": "dbpedia_shaped", "kwargs": {}})
+
+
+def fx_dense_small():
+    kw = dict(n_entities=300, n_predicates=6, n_triples=6000, n_rules=24)
+    w = W.dense_conf(**kw)
+    make_fixture("dense_conf_300", w.graph, w.rules, {"fn": "dense_conf", "kwargs": kw})
+
+
+def fx_hub_small():
+    kw = dict(n_entities=20000, n_predicates=16, n_triples=120000, n_hubs=10, n_hub_preds=4,
+              n_rules=16)
+    w = W.hub_fanout(**kw)
+    make_fixture("hub_fanout_20k", w.graph, w.rules, {"fn": "hub_fanout", "kwargs": kw})
+
+
+SETS = {
+    "tiny": fx_tiny,
+    "paths2k": fx_small_paths,
+    "c1": fx_c1,
+    "db5k": fx_dbpedia_small,
+    "c2": fx_c2,
+    "dense300": fx_dense_small,
+    "hub20k": fx_hub_small,
+}
+
+if __name__ == "__main__":
+    subprocess.check_call(["make", "-s", "-f", os.path.join(ROOT, "oracle", "ref.mk"),
+                           "REF=/root/reference", "OUT=" + REF_DIR], cwd=ROOT)
+    names = sys.argv[1:] or list(SETS)
+    for n in names:
+        SETS[n]()

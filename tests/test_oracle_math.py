@@ -1,0 +1,132 @@
+"""Pins the CPU oracle by brute force and closed forms, independently of the
+reference programs: exhaustive search over all variable assignments on tiny
+graphs, and set/matrix identities for 1- and 2-atom rules."""
+import itertools
+
+import numpy as np
+import pytest
+
+from masterthesis_gpu_acceleration_amd import workloads as W
+from oracle import Oracle
+
+
+def brute(g, rule):
+    """Direct restatement of the definitions by exhaustive assignment."""
+    T = set(zip(g.s.tolist(), g.p.tolist(), g.o.tolist()))
+    head, body = rule
+    vars_ = W.rule_vars(rule)
+    ents = range(g.n_entities)
+
+    def val(t, a):
+        return a[t[1]] if t[0] == "v" else t[1]
+
+    def body_ok(a, extra_used):
+        trip = [(val(x[0], a), x[1], val(x[2], a)) for x in body]
+        if any(t not in T for t in trip):
+            return False
+        used = trip + extra_used
+        return len(set(used)) == len(used)
+
+    hp = head[1]
+    head_triples = sorted((s, o) for (s, p, o) in T if p == hp)
+    support = 0
+    bvars = [v for v in vars_ if v not in {t[1] for t in (head[0], head[2]) if t[0] == "v"}]
+    for (s, o) in head_triples:
+        a = {}
+        ok = True
+        for t, x in ((head[0], s), (head[2], o)):
+            if t[0] == "c":
+                ok &= t[1] == x
+            elif t[1] in a:
+                ok &= a[t[1]] == x
+            else:
+                a[t[1]] = x
+        if not ok or len(set(a.values())) != len(a):
+            continue
+        found = False
+        for combo in itertools.permutations([e for e in ents if e not in a.values()], len(bvars)):
+            b = dict(a)
+            b.update(zip(bvars, combo))
+            if body_ok(b, [(s, hp, o)]):
+                found = True
+                break
+        support += found
+    pairs = set()
+    hv = [t for t in (head[0], head[2])]
+    inbody = {t[1] for x in body for t in (x[0], x[2]) if t[0] == "v"}
+    if all(t[0] == "v" and t[1] in inbody for t in hv):
+        bv = [v for v in vars_ if v in inbody]
+        for combo in itertools.permutations(ents, len(bv)):
+            b = dict(zip(bv, combo))
+            if body_ok(b, []):
+                pairs.add((b[hv[0][1]], b[hv[1][1]]))
+    return support, len(pairs)
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_oracle_equals_brute_force(seed):
+    g = W.tiny_random(500 + seed, n_entities=5 + seed % 2, n_predicates=2, n_triples=18 + seed)
+    rng = np.random.default_rng(seed)
+    rules = []
+    for k in range(40):
+        rules.append(W.random_rule(rng, g.n_entities, g.n_predicates, 1 + k % 4, n_vars=3 + k % 2,
+                                   p_const=0.15, closed=(k % 5 != 0), reflexive_head=(k % 9 == 4)))
+    res = Oracle(g.s, g.p, g.o).eval(W.encode_rules(rules))
+    for i, r in enumerate(rules):
+        sup, pairs = brute(g, r)
+        assert res[i][0] == sup, (i, r, res[i], sup)
+        assert res[i][3] == pairs, (i, r, res[i], pairs)
+        assert res[i][4] == (pairs if sup > 0 else 0)
+
+
+def _rand_graph(seed, n=40, P=3, m=500):
+    return W.tiny_random(seed, n_entities=n, n_predicates=P, n_triples=m)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_closed_forms_one_atom(seed):
+    g = _rand_graph(seed)
+    T = {}
+    for s, p, o in zip(g.s.tolist(), g.p.tolist(), g.o.tolist()):
+        T.setdefault(p, set()).add((s, o))
+    p, q = 0, 1
+    a, b = W.V("a"), W.V("b")
+    rules = [((a, p, b), [(a, q, b)]),          # same orientation, p != q
+             ((a, p, b), [(b, q, a)]),          # inverse
+             ((a, p, b), [(a, p, b)]),          # body atom == head atom
+             ((a, p, b), [(a, q, b), (a, q, b)])]  # repeated body atom
+    res = Oracle(g.s, g.p, g.o).eval(W.encode_rules(rules))
+    pq = {(s, o) for (s, o) in T[p] & T[q] if s != o}
+    inv = {(s, o) for (s, o) in T[p] if (o, s) in T[q] and s != o}
+    q_pairs = {(s, o) for (s, o) in T[q] if s != o}
+    assert res[0][0] == len(pq) and res[0][3] == len(q_pairs)
+    assert res[1][0] == len(inv) and res[1][3] == len({(o, s) for (s, o) in q_pairs})
+    assert res[2][0] == 0 and res[2][3] == len({(s, o) for (s, o) in T[p] if s != o})
+    assert res[3][0] == 0 and res[3][3] == 0
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_closed_form_two_atom_path(seed):
+    """bodyPairs of (?a q1 ?c) ^ (?c q2 ?b) = #{(a,b): a != b, exists c not in
+    {a,b} with q1(a,c), q2(c,b)} -- a masked Boolean matrix product."""
+    g = _rand_graph(100 + seed, n=30, P=2, m=400)
+    n = g.n_entities
+    A = np.zeros((2, n, n), np.int64)
+    A[g.p, g.s, g.o] = 1
+    Q1 = A[0].copy()
+    Q2 = A[1].copy()
+    np.fill_diagonal(Q1, 0)   # c != a
+    np.fill_diagonal(Q2, 0)   # c != b
+    M = Q1 @ Q2
+    np.fill_diagonal(M, 0)    # a != b
+    expect = int((M > 0).sum())
+    a, b, c = W.V("a"), W.V("b"), W.V("c")
+    rule = ((a, 0, b), [(a, 0, c), (c, 1, b)])
+    res = Oracle(g.s, g.p, g.o).eval(W.encode_rules([rule]))
+    assert res[0][3] == expect
+    # support: head triples (a,b) in q1 with a != b and M[a,b] > 0, minus the
+    # case where the witness path reuses the head triple: (a,0,c) == (a,0,b)
+    # needs c == b, which the c-mask already excludes.
+    H = A[0].copy()
+    np.fill_diagonal(H, 0)
+    assert res[0][0] == int(((H > 0) & (M > 0)).sum())
