@@ -1,0 +1,148 @@
+/* gpu_rules_api.h -- C ABI of libgpu_rules.so, the MI355X (gfx950) rule
+ * support / confidence counting library.
+ *
+ * The first five declarations are the reference's interface, unchanged in
+ * name, signature, struct layout and return conventions, so the reference's
+ * JNA binding (GPU_03/GpuRulesLib.scala:23-58, Native.load("gpu_rules"))
+ * binds this library as it is. Everything after them is additive.
+ *
+ * Self-contained as plain C (the reference header relies on the includer
+ * for size_t; this one includes <stddef.h> and <stdint.h>).
+ */
+#ifndef GPU_RULES_API_H
+#define GPU_RULES_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Result of one rule (replaces GPU_03/gpu_rules_api.h:21-30; same fields,
+ * order and 56-byte layout). Counts are the library's exact 64-bit counts
+ * truncated to int, as the reference does (GPU_03/gpu_rules_lib.cu:435,525). */
+typedef struct {
+    int    support;       /* head triples whose body has an injective grounding        */
+    int    headSize;      /* distinct triples of the head predicate                     */
+    int    headSupport;   /* head groundings, constant-aware                            */
+    double headCoverage;  /* support / headSize                                         */
+    int    bodySize;      /* distinct (headS, headO) body bindings; 0 when support == 0 */
+    double confidence;    /* support / bodySize, 0 when bodySize == 0                   */
+    double supportMs;     /* device time of the support launches (ms)                   */
+    double confidenceMs;  /* device time of the body-size launches (ms)                 */
+} GpuRuleResult;
+
+/* Replaces GPU_03/gpu_rules_api.h:43 (impl. GPU_03/gpu_rules_lib.cu:246-325).
+ * Parses a Turtle file, builds the per-predicate SPO/POS adjacency and keeps
+ * it resident in HBM. Returns an opaque context, or NULL (message on stderr
+ * prefixed "[gpu_rules]") if the file cannot be read, no GPU is present or
+ * HBM is short. */
+void* gpu_load_graph(const char* ttlPath);
+
+/* Replaces GPU_03/gpu_rules_api.h:59 (impl. GPU_03/gpu_rules_lib.cu:330-543).
+ * Evaluates one rule line "( ?a <p> ?c ) ^ ... => ( ?a <q> ?b )" with one
+ * support launch and (if support > 0) one body-size launch, each followed by
+ * a device sync and a device-to-host copy. *out is zeroed first. Returns -1
+ * if an argument is NULL or the rule cannot be parsed, else 0 -- including
+ * rules with an unsupported shape (body outside 1..8 atoms, > 16 variables),
+ * which report headSize/headSupport and zero measures. Not thread-safe per
+ * context. */
+int gpu_evaluate_rule(void* ctx, const char* ruleText, GpuRuleResult* out);
+
+/* Replaces GPU_03/gpu_rules_api.h:72 (impl. GPU_03/gpu_rules_lib.cu:550-565).
+ * Same as gpu_evaluate_rule for one rule given as an RDFRules JSON object
+ * {"body":[{"subject":{"type":..,"value":..},"predicate":..,"object":{..}}],"head":{..}}. */
+int gpu_evaluate_rule_json(void* ctx, const char* ruleJson, GpuRuleResult* out);
+
+/* Replaces GPU_03/gpu_rules_api.h:79 (impl. GPU_03/gpu_rules_lib.cu:570-574).
+ * Releases the context and all its device memory. NULL-safe. */
+void gpu_free_graph(void* ctx);
+
+/* Replaces GPU_03/gpu_rules_api.h:87 (impl. GPU_03/gpu_rules_lib.cu:237-241).
+ * Free and total HBM of the current device; 0 on success, -1 if no device. */
+int gpu_query_vram(size_t* freeBytesOut, size_t* totalBytesOut);
+
+/* ------------------------------------------------------------------------
+ * Additive API (no counterpart in the reference).
+ * ------------------------------------------------------------------------ */
+
+/* Structured rule: words[0] = number of body atoms, words[1..3] = head
+ * (s, p, o), words[4+3j .. 6+3j] = body atom j (s, p, o), j < 8. A term
+ * >= 0 is a term id of the graph; a term < 0 is variable -(t+1). */
+#define GPU_RULE_WORDS 28
+typedef struct {
+    int32_t words[GPU_RULE_WORDS];
+} GpuRuleDesc;
+
+/* Exact 64-bit result of one rule in a batch. */
+typedef struct {
+    int64_t support;
+    int64_t headSize;
+    int64_t headSupport;
+    int64_t bodySize;      /* distinct body pairs, 0 when support == 0 (reference convention) */
+    int64_t bodyPairs;     /* distinct body pairs regardless of support; -1 if not computed   */
+    double  headCoverage;
+    double  confidence;
+    int32_t status;        /* 0 ok, 1 unsupported shape, 2 parse error */
+    int32_t reserved;
+} GpuRuleResult64;
+
+/* Batch flags. */
+#define GPU_RULES_UNGATED   1  /* also count body pairs of rules with support 0 (bodyPairs) */
+#define GPU_RULES_PER_RULE  2  /* evaluate rule by rule with the per-rule launch sequence    */
+
+/* Loads a graph from term-id triples: subject, predicate and object ids in
+ * [0, nTerms). Predicates and entities share the id space, as in the
+ * reference's IdMapper. Rule text can then only name terms by id-less
+ * syntax, so such graphs are used with the structured entry points. */
+void* gpu_load_graph_triples(const int32_t* s, const int32_t* p, const int32_t* o, int64_t n,
+                             int64_t nTerms);
+
+/* Evaluates n structured rules in one batched pass (persistent thread-mode
+ * kernel over all rules' seeds, wave-mode kernel over escalated seeds,
+ * captured in a HIP graph and replayed while the batch shape is unchanged).
+ * Returns 0, or -1 on a NULL argument / device error. */
+int gpu_evaluate_rules(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags, GpuRuleResult64* out);
+
+/* Same for rule lines (isJson = 0) or JSON objects (isJson = 1). A rule that
+ * fails to parse gets status 2; the others are evaluated. */
+int gpu_evaluate_rules_text(void* ctx, const char* const* rules, int32_t n, int32_t isJson, int32_t flags,
+                            GpuRuleResult64* out);
+
+/* Device time of the last gpu_evaluate_rules* call (ms) and its phases. */
+int gpu_last_batch_stats(void* ctx, double* totalMs, double* supportMs, double* pairsMs, int64_t* heavySeeds);
+
+/* Tuning knobs: "budget" (thread-mode candidates per seed before a seed is
+ * moved to wave mode), "thread_waves_per_cu", "wave_waves_per_cu",
+ * "use_graph" (0/1). Returns 0, or -1 for an unknown key. */
+int gpu_set_tuning(void* ctx, const char* key, int64_t value);
+int64_t gpu_get_tuning(void* ctx, const char* key);
+
+/* ---- host-side introspection (no device work; usable without a GPU) ---- */
+
+/* Parses a Turtle file into a host-only graph handle. */
+void* gpu_host_load_turtle(const char* ttlPath);
+void gpu_host_free(void* host);
+/* Term count, predicate count and distinct triple count of a graph handle
+ * (host handle or full context). */
+int gpu_host_graph_info(void* handle, int64_t* nTerms, int32_t* nPredicates, int64_t* nTriples);
+/* Term id of a term string as the Turtle reader stores it (IRIs without
+ * brackets), or -1. */
+int64_t gpu_host_term_id(void* handle, const char* term);
+/* Parses a rule line (isJson = 0) or JSON object into GPU_RULE_WORDS words.
+ * Unknown constants get ids >= the graph's term count. 0 or -1. */
+int gpu_host_parse_rule(void* handle, const char* rule, int32_t isJson, int32_t* words);
+/* Head figures and plan facts of a structured rule: info[0] headSize,
+ * info[1] headSupport, info[2] status, info[3] support launched (0/1),
+ * info[4] body pairs launched (0/1), info[5] support seeds, info[6] pair seeds. */
+int gpu_host_plan_info(void* handle, const int32_t* words, int64_t* info);
+
+/* Library version string. */
+const char* gpu_rules_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPU_RULES_API_H */

@@ -1,0 +1,66 @@
+"""Builds libgpu_rules.so (host C++17 + HIP kernels for gfx950) in-tree.
+
+    python -m masterthesis_gpu_acceleration_amd.build
+
+Objects go to masterthesis_gpu_acceleration_amd/_build/, the library to
+masterthesis_gpu_acceleration_amd/libgpu_rules.so (both git-ignored; the
+library travels with the tree).
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+OBJ = os.path.join(PKG, "_build")
+LIB = os.path.join(PKG, "libgpu_rules.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("GPU_RULES_ARCH", "gfx950")
+
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result", "-I/opt/rocm/include", "-I" + os.path.join(ROOT, "include"),
+          "-I" + CSRC]
+# Device code: gfx950 only; packed-FP32 instruction selection off.
+HIPFLAGS = ["-x", "hip", "--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-fno-gpu-rdc"]
+
+SOURCES = ["graph_host.cpp", "rules_host.cpp", "planner.cpp", "engine.cpp", "kernels.hip"]
+
+
+def _compile(src: str) -> str:
+    path = os.path.join(CSRC, src)
+    obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
+    flags = COMMON + (HIPFLAGS if src.endswith(".hip") else ["-x", "c++", "-D__HIP_PLATFORM_AMD__"])
+    cmd = [HIPCC] + flags + ["-c", path, "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("compile failed: " + src)
+    if r.stderr.strip():
+        noise = [l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l and l.strip()]
+        if noise:
+            sys.stderr.write("\n".join(noise) + "\n")
+    return obj
+
+
+def build(verbose: bool = True) -> str:
+    os.makedirs(OBJ, exist_ok=True)
+    with ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
+        objs = list(ex.map(_compile, SOURCES))
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("link failed")
+    os.replace(tmp, LIB)
+    if verbose:
+        print("built", os.path.relpath(LIB, ROOT))
+    return LIB
+
+
+if __name__ == "__main__":
+    build()

@@ -1,0 +1,801 @@
+// engine.cpp -- contexts, device residency and the C ABI of libgpu_rules.so.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/gpu_rules_api.h"
+#include "graph_host.hpp"
+#include "kernels.hpp"
+#include "planner.hpp"
+#include "rules_host.hpp"
+
+namespace gr {
+
+namespace {
+
+constexpr uint32_t kMagicHost = 0x47524831;  // "GRH1"
+constexpr uint32_t kMagicCtx = 0x47524331;   // "GRC1"
+constexpr uint32_t kMagicBatch = 0x47524231; // "GRB1"
+
+void log_err(const char* what, hipError_t e) {
+    std::fprintf(stderr, "[gpu_rules] %s: %s\n", what, hipGetErrorString(e));
+}
+
+#define GR_HIP(call)                                   \
+    do {                                               \
+        hipError_t e_ = (call);                        \
+        if (e_ != hipSuccess) {                        \
+            log_err(#call, e_);                        \
+            return false;                              \
+        }                                              \
+    } while (0)
+
+struct Handle {
+    uint32_t magic;
+};
+
+struct HostHandle : Handle {
+    HostGraph g;
+    std::unique_ptr<TermResolver> tr;
+};
+
+// Device buffer that only grows.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    bool ensure(size_t need) {
+        if (need <= bytes) return true;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        size_t b = std::max<size_t>(need, 256);
+        hipError_t e = hipMalloc(&p, b);
+        if (e != hipSuccess) {
+            log_err("hipMalloc", e);
+            p = nullptr;
+            return false;
+        }
+        bytes = b;
+        return true;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+struct Context;
+
+// A compiled, device-resident batch of rules.
+struct Batch : Handle {
+    Context* ctx = nullptr;
+    int32_t n = 0;
+    int32_t flags = 0;
+    std::vector<CompiledRule> comp;
+    std::vector<int32_t> status;
+    // chunk lists and heavy-queue segments per (phase, body length)
+    uint32_t nChunks[2][kMaxBody + 1] = {};
+    uint32_t chunkOff[2][kMaxBody + 1] = {};
+    uint32_t heavyCap[2][kMaxBody + 1] = {};
+    uint64_t heavyOff[2][kMaxBody + 1] = {};
+    uint32_t capRules = 0;
+    DevBuf plans, wplans, refs, wrefs, chunks, counts, heavy, ctl;
+    unsigned long long* hCounts = nullptr;   // pinned
+    uint32_t* hCtl = nullptr;                // pinned
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    double lastMs = 0, lastSupMs = 0, lastPairMs = 0;
+    int64_t lastHeavy = 0;
+    ~Batch();
+};
+
+struct Context : Handle {
+    HostGraph g;
+    std::unique_ptr<TermResolver> tr;
+    Tuning tune;
+    int device = 0;
+    int numCUs = 256;
+    int threadWavesPerCU = 16;
+    int waveWavesPerCU = 8;
+    bool useGraph = true;
+    hipStream_t st = nullptr;
+    hipEvent_t ev[4] = {};
+    DevBuf rows, col, coo, keys;
+    DevGraph dg{};
+    // per-rule launch scratch
+    DevBuf one;       // [0] count (u64); +64: u32 control words; +128: ConflictRefs
+    DevBuf heavy1;
+    uint32_t heavy1Cap = 0;
+    unsigned long long* hOne = nullptr;   // pinned
+    // wave-mode dedup rows
+    DevBuf dedup;
+    int64_t rowWords = 0;
+    int waveSlots = 0;
+    Batch* lastBatch = nullptr;
+    double lastMs = 0, lastSupMs = 0, lastPairMs = 0;
+    int64_t lastHeavy = 0;
+
+    int waveBlocks() const { return std::max(1, waveSlots / 4); }
+    int threadBlocks() const { return std::max(1, numCUs * threadWavesPerCU / 4); }
+
+    bool upload();
+    bool ensure_dedup();
+    ~Context();
+};
+
+Batch::~Batch() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    plans.release(); wplans.release(); refs.release(); wrefs.release(); chunks.release(); counts.release();
+    heavy.release(); ctl.release();
+    if (hCounts) (void)hipHostFree(hCounts);
+    if (hCtl) (void)hipHostFree(hCtl);
+    if (ctx && ctx->lastBatch == this) ctx->lastBatch = nullptr;
+}
+
+Context::~Context() {
+    if (st) (void)hipStreamSynchronize(st);
+    rows.release(); col.release(); coo.release(); keys.release();
+    one.release(); heavy1.release(); dedup.release();
+    if (hOne) (void)hipHostFree(hOne);
+    for (auto& e : ev) if (e) (void)hipEventDestroy(e);
+    if (st) (void)hipStreamDestroy(st);
+}
+
+template <class T>
+bool put(DevBuf& b, const std::vector<T>& v, hipStream_t st) {
+    if (!b.ensure(std::max<size_t>(v.size(), 1) * sizeof(T))) return false;
+    if (!v.empty()) GR_HIP(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+    return true;
+}
+
+bool Context::upload() {
+    GR_HIP(hipGetDevice(&device));
+    hipDeviceProp_t prop;
+    GR_HIP(hipGetDeviceProperties(&prop, device));
+    numCUs = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    GR_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    for (auto& e : ev) GR_HIP(hipEventCreate(&e));
+    GR_HIP(hipHostMalloc((void**)&hOne, 64, hipHostMallocDefault));
+
+    const int32_t P = g.P();
+    const int64_t nE = std::max<int64_t>(g.nE, 1);
+    const int64_t T = g.T();
+    // dense rows: {begin,end} (absolute entries of the merged arrays) for
+    // every (dir, pred, term id)
+    std::vector<RowSpan> hrows((size_t)2 * P * nE, RowSpan{0, 0});
+    for (int d = 0; d < 2; ++d) {
+        const std::vector<int32_t>& key = d == 0 ? g.spoKey : g.posKey;
+        for (int32_t p = 0; p < P; ++p) {
+            RowSpan* base = hrows.data() + ((size_t)d * P + p) * nE;
+            for (int32_t e = g.predOff[p]; e < g.predOff[p + 1]; ++e) {
+                RowSpan& r = base[key[e]];
+                const int32_t abs = (int32_t)(d * T + e);
+                if (r.end == r.begin) r.begin = abs;
+                r.end = abs + 1;
+            }
+        }
+    }
+    if (!put(rows, hrows, st)) return false;
+    std::vector<RowSpan>().swap(hrows);
+    std::vector<int32_t> m;
+    m.reserve((size_t)2 * T);
+    m.insert(m.end(), g.spoVal.begin(), g.spoVal.end());
+    m.insert(m.end(), g.posVal.begin(), g.posVal.end());
+    if (!put(col, m, st)) return false;
+    GR_HIP(hipStreamSynchronize(st));
+    m.clear();
+    m.insert(m.end(), g.spoKey.begin(), g.spoKey.end());
+    m.insert(m.end(), g.posKey.begin(), g.posKey.end());
+    if (!put(coo, m, st)) return false;
+    GR_HIP(hipStreamSynchronize(st));
+    m.clear();
+    m.insert(m.end(), g.keysS.begin(), g.keysS.end());
+    m.insert(m.end(), g.keysO.begin(), g.keysO.end());
+    if (!put(keys, m, st)) return false;
+    GR_HIP(hipStreamSynchronize(st));
+    dg.rows = rows.as<RowSpan>();
+    dg.col = col.as<int32_t>();
+    dg.coo = coo.as<int32_t>();
+    dg.keys = keys.as<int32_t>();
+    if (!one.ensure(512)) return false;
+    GR_HIP(hipMemset(one.p, 0, 512));
+    return ensure_dedup();
+}
+
+bool Context::ensure_dedup() {
+    rowWords = (g.nE + 31) / 32;
+    if (rowWords < 1) rowWords = 1;
+    waveSlots = numCUs * waveWavesPerCU;
+    // keep the dedup rows within 2 GiB
+    const int64_t maxSlots = std::max<int64_t>(4, (int64_t(2) << 30) / (rowWords * 4));
+    if (waveSlots > maxSlots) waveSlots = (int)(maxSlots / 4 * 4);
+    const size_t bytes = (size_t)waveSlots * rowWords * 4;
+    if (dedup.bytes < bytes) {
+        if (!dedup.ensure(bytes)) return false;
+        GR_HIP(hipMemset(dedup.p, 0, dedup.bytes));
+    }
+    return true;
+}
+
+// --------------------------------------------------------- per-rule path
+
+bool has_conflicts(const Plan& P) {
+    for (int L = 0; L < P.nlev; ++L)
+        if (level_conflicts(P.lev[L].flags)) return true;
+    return false;
+}
+
+// One plan: thread-mode launch over all seeds, wave-mode launch over the
+// escalated ones, sync, copy the count back.
+bool run_one(Context& C, const Plan& P, const ConflictRefs& R, const Plan& PW, const ConflictRefs& RW, int64_t items,
+             uint64_t& count, double& ms, int64_t& heavySeeds) {
+    count = 0;
+    ms = 0;
+    if (items <= 0) return true;
+    if ((uint64_t)items > C.heavy1Cap) {
+        if (!C.heavy1.ensure((size_t)items * sizeof(HeavyItem))) return false;
+        C.heavy1Cap = (uint32_t)items;
+    }
+    unsigned long long* d_count = C.one.as<unsigned long long>();
+    uint32_t* ctl = reinterpret_cast<uint32_t*>(C.one.as<char>() + 64);
+    ConflictRefs* d_refs = reinterpret_cast<ConflictRefs*>(C.one.as<char>() + 128);
+    const bool conf = has_conflicts(P) || has_conflicts(PW);
+    if (conf) {
+        ConflictRefs both[2] = {R, RW};
+        GR_HIP(hipMemcpyAsync(d_refs, both, sizeof(both), hipMemcpyHostToDevice, C.st));
+    }
+    launch_fill_u64(d_count, 1, 0, C.st);
+    launch_fill_u32(ctl, 4, 0, C.st);
+    GR_HIP(hipEventRecord(C.ev[0], C.st));
+    const int64_t waves = (items + kWave - 1) / kWave;
+    const int blocks = (int)std::min<int64_t>((waves + 3) / 4, C.threadBlocks());
+    OneArgs oa{conf ? d_refs : nullptr, (uint32_t)items, C.heavy1Cap, d_count, C.heavy1.as<HeavyItem>(), ctl + 0};
+    launch_thread_one(P, oa, C.dg, blocks, C.st);
+    WaveArgs wa{nullptr, conf ? d_refs + 1 : nullptr, C.heavy1.as<HeavyItem>(), ctl + 0, C.heavy1Cap, 0, ctl + 1,
+                d_count, C.dedup.as<uint32_t>(), C.rowWords};
+    launch_wave_one(PW, wa, C.dg, C.waveBlocks(), C.st);
+    GR_HIP(hipGetLastError());
+    GR_HIP(hipEventRecord(C.ev[1], C.st));
+    GR_HIP(hipMemcpyAsync(C.hOne, C.one.p, 80, hipMemcpyDeviceToHost, C.st));
+    GR_HIP(hipStreamSynchronize(C.st));
+    float f = 0;
+    GR_HIP(hipEventElapsedTime(&f, C.ev[0], C.ev[1]));
+    ms = f;
+    count = C.hOne[0];
+    heavySeeds += reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(C.hOne) + 64)[0];
+    return true;
+}
+
+void finish_measures(GpuRuleResult64& r) {
+    r.headCoverage = r.headSize > 0 ? (double)r.support / (double)r.headSize : 0.0;
+    r.confidence = r.bodySize > 0 ? (double)r.support / (double)r.bodySize : 0.0;
+}
+
+bool eval_rule_plain(Context& C, const RuleIR& r, bool gated, GpuRuleResult64& res, double& supMs, double& confMs) {
+    std::memset(&res, 0, sizeof(res));
+    res.bodyPairs = -1;
+    supMs = confMs = 0;
+    CompiledRule cr = compile_rule(r, C.g, C.tune);
+    res.headSize = cr.headSize;
+    res.headSupport = cr.headSupport;
+    res.status = cr.status;
+    if (cr.status != kRuleOk) return true;
+    int64_t heavy = 0;
+    uint64_t sup = 0;
+    if (cr.supportLaunch && !run_one(C, cr.sup, cr.supRefs, cr.supWave, cr.supWaveRefs, cr.supItems, sup, supMs, heavy))
+        return false;
+    res.support = (int64_t)sup;
+    if (!gated || sup > 0) {
+        uint64_t pairs = 0;
+        if (cr.pairsLaunch &&
+            !run_one(C, cr.pairs, cr.pairsRefs, cr.pairsWave, cr.pairsWaveRefs, cr.pairItems, pairs, confMs, heavy))
+            return false;
+        res.bodyPairs = (int64_t)pairs;
+    }
+    res.bodySize = res.support > 0 && res.bodyPairs > 0 ? res.bodyPairs : 0;
+    finish_measures(res);
+    C.lastHeavy = heavy;
+    return true;
+}
+
+// ------------------------------------------------------------ batch path
+
+constexpr int kCtlWords = 2 * (kMaxBody + 1) * 4;   // per (phase, N): thread cursor, heavy count, wave cursor
+
+bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const std::vector<int32_t>& parseStatus,
+                 int32_t flags) {
+    B.ctx = &C;
+    B.n = (int32_t)rules.size();
+    B.flags = flags;
+    B.comp.assign(rules.size(), CompiledRule{});
+    B.status = parseStatus;
+    const bool gated = !(flags & GPU_RULES_UNGATED);
+    const size_t np = std::max<size_t>(2 * rules.size(), 1);
+    std::vector<Plan> plans(np), wplans(np);
+    std::vector<ConflictRefs> refs(np), wrefs(np);
+    std::vector<Chunk> lists[2][kMaxBody + 1];
+    uint64_t items[2][kMaxBody + 1] = {};
+    auto add = [&](int ph, size_t i, const Plan& P, const ConflictRefs& R, const Plan& W, const ConflictRefs& RW,
+                   int64_t n) {
+        const size_t k = 2 * i + ph;
+        plans[k] = P;
+        wplans[k] = W;
+        refs[k] = R;
+        wrefs[k] = RW;
+        plans[k].rule = wplans[k].rule = (int32_t)i;
+        const int N = P.nlev;
+        for (int64_t b = 0; b < n; b += kWave)
+            lists[ph][N].push_back(Chunk{(uint32_t)k, (uint32_t)b, (uint32_t)std::min<int64_t>(kWave, n - b),
+                                         (ph == 1 && gated) ? 1u : 0u});
+        items[ph][N] += (uint64_t)n;
+    };
+    for (size_t i = 0; i < rules.size(); ++i) {
+        if (B.status[i] != kRuleOk) continue;
+        CompiledRule& cr = B.comp[i];
+        cr = compile_rule(rules[i], C.g, C.tune);
+        B.status[i] = cr.status;
+        if (cr.status != kRuleOk) continue;
+        if (cr.supportLaunch && cr.supItems > 0)
+            add(0, i, cr.sup, cr.supRefs, cr.supWave, cr.supWaveRefs, cr.supItems);
+        if (cr.pairsLaunch && cr.pairItems > 0)
+            add(1, i, cr.pairs, cr.pairsRefs, cr.pairsWave, cr.pairsWaveRefs, cr.pairItems);
+    }
+    std::vector<Chunk> all;
+    uint64_t heavyTotal = 0;
+    for (int ph = 0; ph < 2; ++ph)
+        for (int N = 1; N <= kMaxBody; ++N) {
+            B.chunkOff[ph][N] = (uint32_t)all.size();
+            B.nChunks[ph][N] = (uint32_t)lists[ph][N].size();
+            all.insert(all.end(), lists[ph][N].begin(), lists[ph][N].end());
+            B.heavyOff[ph][N] = heavyTotal;
+            B.heavyCap[ph][N] = (uint32_t)items[ph][N];
+            heavyTotal += items[ph][N];
+        }
+    if (all.empty()) all.push_back(Chunk{0, 0, 0, 0});
+    B.capRules = std::max<uint32_t>((uint32_t)rules.size(), 1);
+    hipStream_t st = C.st;
+    if (!put(B.plans, plans, st) || !put(B.wplans, wplans, st) || !put(B.refs, refs, st) ||
+        !put(B.wrefs, wrefs, st) || !put(B.chunks, all, st))
+        return false;
+    if (!B.counts.ensure((size_t)2 * B.capRules * 8) ||
+        !B.heavy.ensure((size_t)std::max<uint64_t>(heavyTotal, 1) * sizeof(HeavyItem)) || !B.ctl.ensure(kCtlWords * 4))
+        return false;
+    GR_HIP(hipHostMalloc((void**)&B.hCounts, (size_t)2 * B.capRules * 8, hipHostMallocDefault));
+    GR_HIP(hipHostMalloc((void**)&B.hCtl, kCtlWords * 4, hipHostMallocDefault));
+    GR_HIP(hipStreamSynchronize(st));
+    return true;
+}
+
+// The device pass of a batch (what the HIP graph captures): per phase, one
+// persistent thread-mode kernel per body length, then one wave-mode kernel
+// per body length over the seeds the first escalated.
+void batch_enqueue(Context& C, Batch& B, bool timed) {
+    hipStream_t st = C.st;
+    uint32_t* ctl = B.ctl.as<uint32_t>();
+    unsigned long long* counts = B.counts.as<unsigned long long>();
+    launch_fill_u64(counts, 2 * (int64_t)B.capRules, 0, st);
+    launch_fill_u32(ctl, kCtlWords, 0, st);
+    if (timed) (void)hipEventRecord(C.ev[0], st);
+    for (int ph = 0; ph < 2; ++ph) {
+        for (int N = 1; N <= kMaxBody; ++N) {
+            const uint32_t nc = B.nChunks[ph][N];
+            if (!nc) continue;
+            uint32_t* c = ctl + (ph * (kMaxBody + 1) + N) * 4;
+            ThreadArgs ta{B.plans.as<Plan>(), B.refs.as<ConflictRefs>(), B.chunks.as<Chunk>() + B.chunkOff[ph][N], nc,
+                          B.heavyCap[ph][N], c + 0, counts, B.heavy.as<HeavyItem>() + B.heavyOff[ph][N], c + 1};
+            const int blocks = std::min<int>(C.threadBlocks(), (int)((nc + 3) / 4));
+            launch_thread_batch(ta, C.dg, std::max(blocks, 1), st);
+        }
+        for (int N = 1; N <= kMaxBody; ++N) {
+            if (!B.nChunks[ph][N]) continue;
+            uint32_t* c = ctl + (ph * (kMaxBody + 1) + N) * 4;
+            WaveArgs wa{B.wplans.as<Plan>(), B.wrefs.as<ConflictRefs>(), B.heavy.as<HeavyItem>() + B.heavyOff[ph][N],
+                        c + 1, B.heavyCap[ph][N], 0, c + 2, counts, C.dedup.as<uint32_t>(), C.rowWords};
+            launch_wave_batch(wa, C.dg, C.waveBlocks(), st);
+        }
+        if (timed) (void)hipEventRecord(C.ev[1 + ph], st);
+    }
+    (void)hipMemcpyAsync(B.hCounts, counts, (size_t)2 * B.capRules * 8, hipMemcpyDeviceToHost, st);
+    (void)hipMemcpyAsync(B.hCtl, ctl, kCtlWords * 4, hipMemcpyDeviceToHost, st);
+}
+
+bool batch_run(Context& C, Batch& B, GpuRuleResult64* out) {
+    const bool graphMode = C.useGraph;
+    if (graphMode) {
+        if (!B.exec) {
+            GR_HIP(hipStreamBeginCapture(C.st, hipStreamCaptureModeThreadLocal));
+            batch_enqueue(C, B, false);
+            GR_HIP(hipStreamEndCapture(C.st, &B.graph));
+            GR_HIP(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
+        }
+        GR_HIP(hipEventRecord(C.ev[0], C.st));
+        GR_HIP(hipGraphLaunch(B.exec, C.st));
+        GR_HIP(hipEventRecord(C.ev[3], C.st));
+        GR_HIP(hipStreamSynchronize(C.st));
+        float f = 0;
+        (void)hipEventElapsedTime(&f, C.ev[0], C.ev[3]);
+        B.lastMs = f;
+        B.lastSupMs = B.lastPairMs = 0;
+    } else {
+        batch_enqueue(C, B, true);
+        GR_HIP(hipEventRecord(C.ev[3], C.st));
+        GR_HIP(hipStreamSynchronize(C.st));
+        float a = 0, b = 0, c = 0;
+        (void)hipEventElapsedTime(&a, C.ev[0], C.ev[1]);
+        (void)hipEventElapsedTime(&b, C.ev[1], C.ev[2]);
+        (void)hipEventElapsedTime(&c, C.ev[0], C.ev[3]);
+        B.lastSupMs = a;
+        B.lastPairMs = b;
+        B.lastMs = c;
+    }
+    GR_HIP(hipGetLastError());
+    B.lastHeavy = 0;
+    for (int ph = 0; ph < 2; ++ph)
+        for (int N = 1; N <= kMaxBody; ++N) B.lastHeavy += B.hCtl[(ph * (kMaxBody + 1) + N) * 4 + 1];
+    const bool gated = !(B.flags & GPU_RULES_UNGATED);
+    for (int32_t i = 0; i < B.n; ++i) {
+        GpuRuleResult64& r = out[i];
+        std::memset(&r, 0, sizeof(r));
+        const CompiledRule& cr = B.comp[i];
+        r.status = B.status[i];
+        r.headSize = cr.headSize;
+        r.headSupport = cr.headSupport;
+        r.bodyPairs = -1;
+        if (r.status != kRuleOk) continue;
+        r.support = cr.supportLaunch ? (int64_t)B.hCounts[2 * i] : 0;
+        if (!gated || r.support > 0) r.bodyPairs = cr.pairsLaunch ? (int64_t)B.hCounts[2 * i + 1] : 0;
+        r.bodySize = r.support > 0 && r.bodyPairs > 0 ? r.bodyPairs : 0;
+        finish_measures(r);
+    }
+    C.lastMs = B.lastMs;
+    C.lastSupMs = B.lastSupMs;
+    C.lastPairMs = B.lastPairMs;
+    C.lastHeavy = B.lastHeavy;
+    return true;
+}
+
+// ------------------------------------------------------------ handles
+
+Context* as_ctx(void* h) {
+    if (!h || static_cast<Handle*>(h)->magic != kMagicCtx) return nullptr;
+    return static_cast<Context*>(h);
+}
+
+const HostGraph* as_graph(void* h) {
+    if (!h) return nullptr;
+    uint32_t m = static_cast<Handle*>(h)->magic;
+    if (m == kMagicCtx) return &static_cast<Context*>(h)->g;
+    if (m == kMagicHost) return &static_cast<HostHandle*>(h)->g;
+    return nullptr;
+}
+
+TermResolver* as_resolver(void* h) {
+    if (!h) return nullptr;
+    uint32_t m = static_cast<Handle*>(h)->magic;
+    if (m == kMagicCtx) return static_cast<Context*>(h)->tr.get();
+    if (m == kMagicHost) return static_cast<HostHandle*>(h)->tr.get();
+    return nullptr;
+}
+
+Context* new_context_from(HostGraph&& g) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        std::fprintf(stderr, "[gpu_rules] no GPU device available\n");
+        return nullptr;
+    }
+    auto* C = new Context();
+    C->magic = kMagicCtx;
+    C->g = std::move(g);
+    C->tr.reset(new TermResolver(C->g));
+    // HBM estimate, as the reference checks before uploading (GPU_03/gpu_rules_lib.cu:222-279)
+    size_t freeB = 0, totalB = 0;
+    if (hipMemGetInfo(&freeB, &totalB) == hipSuccess) {
+        const size_t need = (size_t)2 * C->g.P() * std::max<int64_t>(C->g.nE, 1) * 8 + (size_t)C->g.T() * 16 +
+                            (size_t)(C->g.keysS.size() + C->g.keysO.size()) * 4 + (size_t)2048 * (C->g.nE / 8 + 4) +
+                            (64u << 20);
+        if (need > freeB) {
+            std::fprintf(stderr, "[gpu_rules] not enough HBM: need ~%zu MiB, %zu MiB free (of %zu MiB)\n",
+                         need >> 20, freeB >> 20, totalB >> 20);
+            delete C;
+            return nullptr;
+        }
+    }
+    if (!C->upload()) {
+        delete C;
+        return nullptr;
+    }
+    return C;
+}
+
+}  // namespace
+}  // namespace gr
+
+using namespace gr;
+
+// ===================================================================== ABI
+
+extern "C" {
+
+const char* gpu_rules_version(void) { return "gpu_rules 1.0 (gfx950)"; }
+
+int gpu_query_vram(size_t* freeBytesOut, size_t* totalBytesOut) {
+    if (!freeBytesOut || !totalBytesOut) return -1;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return -1;
+    return hipMemGetInfo(freeBytesOut, totalBytesOut) == hipSuccess ? 0 : -1;
+}
+
+void* gpu_load_graph(const char* ttlPath) {
+    if (!ttlPath) return nullptr;
+    try {
+        HostGraph g;
+        std::string err;
+        if (!load_turtle(ttlPath, g, err)) {
+            std::fprintf(stderr, "[gpu_rules] failed to parse TTL: %s (%s)\n", ttlPath, err.c_str());
+            return nullptr;
+        }
+        return new_context_from(std::move(g));
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "[gpu_rules] gpu_load_graph error: %s\n", e.what());
+        return nullptr;
+    }
+}
+
+void* gpu_load_graph_triples(const int32_t* s, const int32_t* p, const int32_t* o, int64_t n, int64_t nTerms) {
+    if ((!s || !p || !o) && n > 0) return nullptr;
+    try {
+        for (int64_t i = 0; i < n; ++i) {
+            if (s[i] < 0 || p[i] < 0 || o[i] < 0 || s[i] >= nTerms || p[i] >= nTerms || o[i] >= nTerms) {
+                std::fprintf(stderr, "[gpu_rules] triple %lld has a term id outside [0, %lld)\n", (long long)i,
+                             (long long)nTerms);
+                return nullptr;
+            }
+        }
+        HostGraph g;
+        std::vector<int32_t> S(s, s + n), P(p, p + n), O(o, o + n);
+        build_graph(g, S, P, O, nTerms);
+        return new_context_from(std::move(g));
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "[gpu_rules] gpu_load_graph_triples error: %s\n", e.what());
+        return nullptr;
+    }
+}
+
+void gpu_free_graph(void* ctx) {
+    Context* C = as_ctx(ctx);
+    if (!C) return;
+    delete C->lastBatch;
+    C->lastBatch = nullptr;
+    delete C;
+}
+
+static int eval_one_abi(Context* C, const RuleIR& r, GpuRuleResult* out) {
+    GpuRuleResult64 res;
+    double supMs = 0, confMs = 0;
+    if (!eval_rule_plain(*C, r, true, res, supMs, confMs)) return -1;
+    out->support = (int)res.support;
+    out->headSize = (int)res.headSize;
+    out->headSupport = (int)res.headSupport;
+    out->bodySize = (int)res.bodySize;
+    // measures from the int fields, as FinalRule::setMeasures/setConfidence do
+    // (GPU_03/FinalRule.cpp:19-40)
+    out->headCoverage = out->headSize > 0 ? (double)out->support / (double)out->headSize : 0.0;
+    out->confidence = out->bodySize > 0 ? (double)out->support / (double)out->bodySize : 0.0;
+    out->supportMs = supMs;
+    out->confidenceMs = confMs;
+    return 0;
+}
+
+int gpu_evaluate_rule(void* ctx, const char* ruleText, GpuRuleResult* out) {
+    Context* C = as_ctx(ctx);
+    if (!C || !ruleText || !out) return -1;
+    std::memset(out, 0, sizeof(*out));
+    try {
+        RuleIR r = parse_rule_text(ruleText, *C->tr);
+        return eval_one_abi(C, r, out);
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "[gpu_rules] gpu_evaluate_rule error: %s\n", e.what());
+        return -1;
+    }
+}
+
+int gpu_evaluate_rule_json(void* ctx, const char* ruleJson, GpuRuleResult* out) {
+    Context* C = as_ctx(ctx);
+    if (!C || !ruleJson || !out) return -1;
+    std::memset(out, 0, sizeof(*out));
+    try {
+        RuleIR r = parse_rule_json(ruleJson, *C->tr);
+        return eval_one_abi(C, r, out);
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "[gpu_rules] gpu_evaluate_rule_json error: %s\n", e.what());
+        return -1;
+    }
+}
+
+static int eval_batch(Context* C, std::vector<RuleIR>& rules, std::vector<int32_t>& st, int32_t flags,
+                      GpuRuleResult64* out) {
+    if (flags & GPU_RULES_PER_RULE) {
+        const bool gated = !(flags & GPU_RULES_UNGATED);
+        double tot = 0, ts = 0, tp = 0;
+        int64_t heavy = 0;
+        for (size_t i = 0; i < rules.size(); ++i) {
+            if (st[i] != kRuleOk) {
+                std::memset(&out[i], 0, sizeof(out[i]));
+                out[i].status = st[i];
+                out[i].bodyPairs = -1;
+                continue;
+            }
+            double a = 0, b = 0;
+            if (!eval_rule_plain(*C, rules[i], gated, out[i], a, b)) return -1;
+            ts += a; tp += b; tot += a + b;
+            heavy += C->lastHeavy;
+        }
+        C->lastMs = tot; C->lastSupMs = ts; C->lastPairMs = tp; C->lastHeavy = heavy;
+        return 0;
+    }
+    delete C->lastBatch;
+    C->lastBatch = nullptr;
+    auto* B = new Batch();
+    B->magic = kMagicBatch;
+    if (!batch_build(*C, *B, rules, st, flags) || !batch_run(*C, *B, out)) {
+        delete B;
+        return -1;
+    }
+    C->lastBatch = B;
+    return 0;
+}
+
+int gpu_evaluate_rules(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags, GpuRuleResult64* out) {
+    Context* C = as_ctx(ctx);
+    if (!C || (!rules && n > 0) || (!out && n > 0) || n < 0) return -1;
+    try {
+        std::vector<RuleIR> rs(n);
+        std::vector<int32_t> st(n, kRuleOk);
+        for (int32_t i = 0; i < n; ++i) rs[i] = rule_from_words(rules[i].words);
+        return eval_batch(C, rs, st, flags, out);
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "[gpu_rules] gpu_evaluate_rules error: %s\n", e.what());
+        return -1;
+    }
+}
+
+int gpu_evaluate_rules_text(void* ctx, const char* const* rules, int32_t n, int32_t isJson, int32_t flags,
+                            GpuRuleResult64* out) {
+    Context* C = as_ctx(ctx);
+    if (!C || (!rules && n > 0) || (!out && n > 0) || n < 0) return -1;
+    try {
+        std::vector<RuleIR> rs(n);
+        std::vector<int32_t> st(n, kRuleOk);
+        for (int32_t i = 0; i < n; ++i) {
+            try {
+                if (!rules[i]) throw ParseError("null rule");
+                rs[i] = isJson ? parse_rule_json(rules[i], *C->tr) : parse_rule_text(rules[i], *C->tr);
+            } catch (const std::exception& e) {
+                st[i] = kRuleError;
+            }
+        }
+        return eval_batch(C, rs, st, flags, out);
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "[gpu_rules] gpu_evaluate_rules_text error: %s\n", e.what());
+        return -1;
+    }
+}
+
+int gpu_last_batch_stats(void* ctx, double* totalMs, double* supportMs, double* pairsMs, int64_t* heavySeeds) {
+    Context* C = as_ctx(ctx);
+    if (!C) return -1;
+    if (totalMs) *totalMs = C->lastMs;
+    if (supportMs) *supportMs = C->lastSupMs;
+    if (pairsMs) *pairsMs = C->lastPairMs;
+    if (heavySeeds) *heavySeeds = C->lastHeavy;
+    return 0;
+}
+
+int gpu_set_tuning(void* ctx, const char* key, int64_t value) {
+    Context* C = as_ctx(ctx);
+    if (!C || !key) return -1;
+    std::string k(key);
+    if (k == "budget") C->tune.budget = (int32_t)std::max<int64_t>(1, value);
+    else if (k == "thread_waves_per_cu") C->threadWavesPerCU = (int)std::max<int64_t>(1, value);
+    else if (k == "wave_waves_per_cu") {
+        C->waveWavesPerCU = (int)std::max<int64_t>(1, value);
+        delete C->lastBatch;
+        C->lastBatch = nullptr;
+        if (!C->ensure_dedup()) return -1;
+    } else if (k == "use_graph") C->useGraph = value != 0;
+    else return -1;
+    if (C->lastBatch) { delete C->lastBatch; C->lastBatch = nullptr; }
+    return 0;
+}
+
+int64_t gpu_get_tuning(void* ctx, const char* key) {
+    Context* C = as_ctx(ctx);
+    if (!C || !key) return -1;
+    std::string k(key);
+    if (k == "budget") return C->tune.budget;
+    if (k == "thread_waves_per_cu") return C->threadWavesPerCU;
+    if (k == "wave_waves_per_cu") return C->waveWavesPerCU;
+    if (k == "use_graph") return C->useGraph ? 1 : 0;
+    if (k == "num_cus") return C->numCUs;
+    if (k == "wave_slots") return C->waveSlots;
+    return -1;
+}
+
+// ---------------------------------------------------------- host handles
+
+void* gpu_host_load_turtle(const char* ttlPath) {
+    if (!ttlPath) return nullptr;
+    try {
+        auto* H = new HostHandle();
+        H->magic = kMagicHost;
+        std::string err;
+        if (!load_turtle(ttlPath, H->g, err)) {
+            std::fprintf(stderr, "[gpu_rules] failed to parse TTL: %s (%s)\n", ttlPath, err.c_str());
+            delete H;
+            return nullptr;
+        }
+        H->tr.reset(new TermResolver(H->g));
+        return H;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "[gpu_rules] gpu_host_load_turtle error: %s\n", e.what());
+        return nullptr;
+    }
+}
+
+void gpu_host_free(void* host) {
+    if (host && static_cast<Handle*>(host)->magic == kMagicHost) delete static_cast<HostHandle*>(host);
+}
+
+int gpu_host_graph_info(void* handle, int64_t* nTerms, int32_t* nPredicates, int64_t* nTriples) {
+    const HostGraph* g = as_graph(handle);
+    if (!g) return -1;
+    if (nTerms) *nTerms = g->nE;
+    if (nPredicates) *nPredicates = g->P();
+    if (nTriples) *nTriples = g->T();
+    return 0;
+}
+
+int64_t gpu_host_term_id(void* handle, const char* term) {
+    const HostGraph* g = as_graph(handle);
+    if (!g || !term) return -1;
+    return g->terms.find(term);
+}
+
+int gpu_host_parse_rule(void* handle, const char* rule, int32_t isJson, int32_t* words) {
+    TermResolver* tr = as_resolver(handle);
+    if (!tr || !rule || !words) return -1;
+    try {
+        RuleIR r = isJson ? parse_rule_json(rule, *tr) : parse_rule_text(rule, *tr);
+        if (r.body.size() > (size_t)kMaxBody) return -1;
+        rule_to_words(r, words);
+        return 0;
+    } catch (const std::exception&) {
+        return -1;
+    }
+}
+
+int gpu_host_plan_info(void* handle, const int32_t* words, int64_t* info) {
+    const HostGraph* g = as_graph(handle);
+    if (!g || !words || !info) return -1;
+    Tuning t;
+    CompiledRule cr = compile_rule(rule_from_words(words), *g, t);
+    info[0] = cr.headSize;
+    info[1] = cr.headSupport;
+    info[2] = cr.status;
+    info[3] = cr.supportLaunch;
+    info[4] = cr.pairsLaunch;
+    info[5] = cr.supItems;
+    info[6] = cr.pairItems;
+    return 0;
+}
+
+}  // extern "C"

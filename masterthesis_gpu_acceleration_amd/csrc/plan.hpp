@@ -1,0 +1,117 @@
+// plan.hpp -- plain-old-data shared by the host planner and the HIP kernels.
+//
+// A rule is compiled into two Plans: a SUPPORT plan (head variables bound by
+// each seed head triple, existence search over the body) and a PAIRS plan
+// (head subject bound by each seed key, all head-object bindings counted
+// once). Each plan is a list of Levels, one per body atom, in execution
+// order. Variables live in "slots" numbered in binding order, so a level
+// reads only slots bound before it and binds at most two new ones.
+//
+// Term references: r >= 0 is slot r; r < 0 is the constant term id -(r+1).
+#pragma once
+#include <cstdint>
+
+#ifndef __HIPCC__
+#ifndef __host__
+#define __host__
+#define __device__
+#endif
+#endif
+
+namespace gr {
+
+constexpr int kMaxBody  = 8;    // body atoms per rule (GPU_03/gpu_common.cuh:25-27)
+constexpr int kMaxSlots = 16;   // distinct variables per rule (same source)
+constexpr int kWave     = 64;   // CDNA wavefront width
+
+enum LevelKind : int32_t {
+    kCheck = 0,   // both terms bound: binary search of one row
+    kExt   = 1,   // key bound, other term a new variable: enumerate one row
+    kScan  = 2,   // neither bound: enumerate all triples of the predicate
+};
+
+enum Dir : int32_t { kSPO = 0, kPOS = 1 };
+
+enum PlanKind : int32_t { kSupport = 0, kPairs = 1 };
+
+// Level descriptor: 32 bytes, so a plan's levels stay in scalar registers.
+// flags: bits 0-1 kind, bit 2 dir, bits 3-7 nbound (variable slots bound
+// before this level), bits 8-16 conflict mask (bit 8+j: the ground triple
+// may repeat level j's; bit 16: may repeat the head triple).
+// CHECK/EXT: rows index = rowBase + key value, rowBase = lo | hi << 32.
+// SCAN: entries [lo, hi) of the merged col/coo arrays.
+struct alignas(16) Level {
+    int32_t flags;
+    int32_t key;   // term ref of the key position (CHECK, EXT)
+    int32_t val;   // CHECK: term ref searched in the row; EXT: slot bound; SCAN: slot of the key position
+    int32_t val2;  // SCAN: slot of the other position (== val for (?x p ?x))
+    int32_t lo;
+    int32_t hi;
+    int32_t pad0, pad1;
+};
+
+__host__ __device__ constexpr int32_t level_kind(int32_t f) { return f & 3; }
+__host__ __device__ constexpr int32_t level_dir(int32_t f) { return (f >> 2) & 1; }
+__host__ __device__ constexpr int32_t level_nbound(int32_t f) { return (f >> 3) & 31; }
+__host__ __device__ constexpr int32_t level_conflicts(int32_t f) { return (f >> 8) & 511; }
+__host__ __device__ constexpr int32_t make_level_flags(int32_t kind, int32_t dir, int32_t nbound, int32_t conf) {
+    return kind | (dir << 2) | (nbound << 3) | (conf << 8);
+}
+
+struct alignas(16) Plan {
+    int32_t nlev;        // number of levels (= body atoms)
+    int32_t kind;        // PlanKind
+    int32_t rule;        // rule index within its batch (result slot)
+    int32_t seedDir;     // SUPPORT: orientation of the seed entries (kSPO: key = subject)
+    int32_t seedBegin;   // SUPPORT: entries [seedBegin, seedEnd) of the merged col/coo arrays;
+    int32_t seedEnd;     // PAIRS: entries of the merged keys array
+    int32_t hsRef;       // SUPPORT: head subject term ref; PAIRS: slot of the head subject (0)
+    int32_t hoRef;       // SUPPORT: head object term ref; PAIRS: slot of the head object
+    int32_t pairLevel;   // PAIRS: level binding the head object; -1 if bound by the seed
+    int32_t budget;      // thread-mode candidates per seed before escalation
+    int32_t pad0, pad1;
+    Level lev[kMaxBody];
+};
+
+// Subject/object term refs of each level and of the head, read only when a
+// level's conflict mask is non-zero (rules with constants).
+struct alignas(16) ConflictRefs {
+    int32_t sref[kMaxBody];
+    int32_t oref[kMaxBody];
+    int32_t hs, ho, pad0, pad1;
+};
+
+// One unit of thread-mode work: up to 64 consecutive seeds of one plan.
+struct alignas(16) Chunk {
+    uint32_t plan;       // index into the plan table
+    uint32_t begin;      // first seed (relative to the plan's seed range)
+    uint32_t count;      // seeds in this chunk (<= 64)
+    uint32_t gate;       // 1: skip if the rule's support is 0 (PAIRS chunks only)
+};
+
+// A seed escalated from thread mode to wave mode.
+struct alignas(8) HeavyItem {
+    uint32_t plan;
+    uint32_t item;
+};
+
+// Row extent {begin, end} of one (direction, predicate, key) in the merged
+// col/coo arrays. One 8-byte word, so a row lookup is one 64-bit load.
+struct alignas(8) RowSpan {
+    int32_t begin;
+    int32_t end;
+};
+
+// Device-resident graph. Direction d's entries of predicate p occupy
+// [d*T + predOff[p], d*T + predOff[p+1]) of col/coo; SPO entries are sorted
+// by (s, o) within a predicate (coo = s, col = o), POS by (o, s) (coo = o,
+// col = s). keys holds the distinct subjects then the distinct objects of
+// every predicate.
+struct DevGraph {
+    const RowSpan* rows;   // [2][P][nE]
+    const int32_t* col;    // [2T]
+    const int32_t* coo;    // [2T]
+    const int32_t* keys;
+};
+
+}  // namespace gr

@@ -1,0 +1,296 @@
+// planner.cpp -- rule -> device plans.
+//
+// Both plans evaluate the same injective-grounding semantics as the
+// reference's CPU counters (Rewritten_CPP/SupportCounting.cpp,
+// CPU_01_nonparallel/SupportCounting.cpp); the atom order only changes the
+// work, never the result. The order is greedy: the atom with the most bound
+// terms first, then the smallest expected candidate list -- the same idea as
+// the reference's chooseBestAtom / order_body (GPU_03/gpu_rules_lib.cu:93-176),
+// fixed per rule so the device needs no scoring.
+#include "planner.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <limits>
+
+namespace gr {
+
+namespace {
+
+enum Eq { kNever, kMaybe, kAlways };
+
+Eq term_eq(const TermIR& a, const TermIR& b) {
+    if (a.var && b.var) return a.v == b.v ? kAlways : kNever;   // distinct variables never share a value
+    if (!a.var && !b.var) return a.v == b.v ? kAlways : kNever;
+    return kMaybe;                                               // a variable may equal a constant
+}
+
+// Can the ground triples of two atoms coincide? (same predicate assumed)
+Eq atom_eq(const AtomIR& a, const AtomIR& b) {
+    Eq s = term_eq(a.s, b.s), o = term_eq(a.o, b.o);
+    if (s == kNever || o == kNever) return kNever;
+    if (s == kAlways && o == kAlways) return kAlways;
+    return kMaybe;
+}
+
+struct Builder {
+    const RuleIR& r;
+    const HostGraph& g;
+    int32_t slotOf[64];
+    int32_t nslots = 0;
+
+    Builder(const RuleIR& r_, const HostGraph& g_) : r(r_), g(g_) {
+        for (auto& s : slotOf) s = -1;
+    }
+    bool bound(const TermIR& t) const { return !t.var || slotOf[t.v] >= 0; }
+    int32_t ref(const TermIR& t) const { return t.var ? slotOf[t.v] : -(t.v + 1); }
+    int32_t bind(int32_t var) {
+        if (slotOf[var] < 0) slotOf[var] = nslots++;
+        return slotOf[var];
+    }
+
+    double cost(const AtomIR& a, int nb2, bool sB) const {
+        int32_t p = g.dense_pred(a.pterm);
+        if (p < 0) return 0;
+        if (nb2 == 2) return 0;
+        if (nb2 == 0) return (double)g.nnz(p) + 1.0;
+        int dir = sB ? 0 : 1;
+        const TermIR& k = sB ? a.s : a.o;
+        if (!k.var) {
+            if (k.v >= g.nE) return 0;
+            auto rr = g.row(dir, p, k.v);
+            return rr.second - rr.first;
+        }
+        int32_t nk = g.nkeys(dir, p);
+        return nk ? (double)g.nnz(p) / nk : 0;
+    }
+
+    // Greedy order from the current bound set (does not bind anything).
+    std::vector<int> order() const {
+        int nb = (int)r.body.size();
+        std::vector<int> out;
+        std::vector<char> used(nb, 0);
+        bool vb[64];
+        for (int v = 0; v < 64; ++v) vb[v] = slotOf[v] >= 0;
+        auto isb = [&](const TermIR& t) { return !t.var || vb[t.v]; };
+        for (int step = 0; step < nb; ++step) {
+            int best = -1, bestNb = -1;
+            double bestCost = 0;
+            for (int i = 0; i < nb; ++i) {
+                if (used[i]) continue;
+                const AtomIR& a = r.body[i];
+                bool sB = isb(a.s), oB = isb(a.o);
+                int nb2 = (int)sB + (int)oB;
+                double c = cost(a, nb2, sB);
+                if (best < 0 || nb2 > bestNb || (nb2 == bestNb && c < bestCost)) {
+                    best = i; bestNb = nb2; bestCost = c;
+                }
+            }
+            used[best] = 1;
+            out.push_back(best);
+            const AtomIR& a = r.body[best];
+            if (a.s.var) vb[a.s.v] = true;
+            if (a.o.var) vb[a.o.v] = true;
+        }
+        return out;
+    }
+
+    // Builds levels for `ord`, binding slots as it goes. `withHead` adds
+    // head-triple comparisons (support plans).
+    void levels(const std::vector<int>& ord, Plan& P, ConflictRefs& R, bool withHead) {
+        P.nlev = (int32_t)ord.size();
+        const int64_t T = g.T();
+        for (size_t L = 0; L < ord.size(); ++L) {
+            const AtomIR& a = r.body[ord[L]];
+            Level lv{};
+            const int32_t p = g.dense_pred(a.pterm);
+            const int32_t nbound = nslots;
+            bool sB = bound(a.s), oB = bound(a.o);
+            int32_t kind, dir;
+            if (sB && oB) {
+                kind = kCheck; dir = kSPO; lv.key = ref(a.s); lv.val = ref(a.o);
+            } else if (sB) {
+                kind = kExt; dir = kSPO; lv.key = ref(a.s); lv.val = bind(a.o.v);
+            } else if (oB) {
+                kind = kExt; dir = kPOS; lv.key = ref(a.o); lv.val = bind(a.s.v);
+            } else {
+                kind = kScan; dir = kSPO; lv.key = 0;
+                lv.val = bind(a.s.v);
+                lv.val2 = bind(a.o.v);   // same slot when the atom is (?x p ?x)
+            }
+            if (kind == kScan) {
+                lv.lo = g.predOff[p];
+                lv.hi = g.predOff[p + 1];
+            } else {
+                const int64_t base = ((int64_t)dir * g.P() + p) * g.nE;
+                lv.lo = (int32_t)(uint32_t)(base & 0xffffffffLL);
+                lv.hi = (int32_t)(base >> 32);
+            }
+            (void)T;
+            R.sref[L] = ref(a.s);
+            R.oref[L] = ref(a.o);
+            int32_t m = 0;
+            for (size_t j = 0; j < L; ++j) {
+                const AtomIR& b = r.body[ord[j]];
+                if (b.pterm == a.pterm && atom_eq(a, b) != kNever) m |= 1 << j;
+            }
+            if (withHead && r.head.pterm == a.pterm && atom_eq(a, r.head) != kNever) m |= 1 << 8;
+            lv.flags = make_level_flags(kind, dir, nbound, m);
+            P.lev[L] = lv;
+        }
+    }
+};
+
+bool const_ok(const TermIR& t, const HostGraph& g) { return t.var || (t.v >= 0 && t.v < g.nE); }
+
+// Moves the first non-CHECK atom of `ord` to the front (CHECK atoms bind
+// nothing, so slots and kinds are unchanged). Returns false if every level
+// is a CHECK.
+bool wave_order(const Plan& thread, std::vector<int>& ord) {
+    int f = -1;
+    for (int L = 0; L < thread.nlev; ++L)
+        if (level_kind(thread.lev[L].flags) != kCheck) { f = L; break; }
+    if (f < 0) return false;
+    int a = ord[f];
+    ord.erase(ord.begin() + f);
+    ord.insert(ord.begin(), a);
+    return true;
+}
+
+}  // namespace
+
+CompiledRule compile_rule(const RuleIR& r, const HostGraph& g, const Tuning& t) {
+    CompiledRule c;
+    const int nb = (int)r.body.size();
+    const int32_t hp = g.dense_pred(r.head.pterm);
+
+    // head size and constant-aware head support
+    // (Rewritten_CPP/SupportCounting.cpp:142-201, CPU_01_nonparallel/SupportCounting.cpp:474-507)
+    if (hp >= 0) {
+        c.headSize = g.nnz(hp);
+        const TermIR &hs = r.head.s, &ho = r.head.o;
+        if (hs.var && ho.var) {
+            c.headSupport = c.headSize;
+        } else if (!hs.var && ho.var) {
+            if (const_ok(hs, g)) { auto rr = g.row(0, hp, hs.v); c.headSupport = rr.second - rr.first; }
+        } else if (hs.var && !ho.var) {
+            if (const_ok(ho, g)) { auto rr = g.row(1, hp, ho.v); c.headSupport = rr.second - rr.first; }
+        } else {
+            c.headSupport = const_ok(hs, g) && const_ok(ho, g) && g.has(hp, hs.v, ho.v) ? 1 : 0;
+        }
+    }
+    // unsupported shapes: reported with head figures only (GPU_03/gpu_rules_lib.cu:352-357)
+    if (nb < 1 || nb > kMaxBody || r.nvars > kMaxSlots) {
+        c.status = kRuleSkipped;
+        return c;
+    }
+
+    bool bodyMatchable = true;
+    bool repeatedBody = false;
+    for (int i = 0; i < nb; ++i) {
+        const AtomIR& a = r.body[i];
+        if (g.dense_pred(a.pterm) < 0 || !const_ok(a.s, g) || !const_ok(a.o, g)) bodyMatchable = false;
+        for (int j = 0; j < i; ++j)
+            if (r.body[j].pterm == a.pterm && atom_eq(a, r.body[j]) == kAlways) repeatedBody = true;
+    }
+    bool bodyIsHead = false;
+    for (const AtomIR& a : r.body)
+        if (a.pterm == r.head.pterm && atom_eq(a, r.head) == kAlways) bodyIsHead = true;
+
+    // ---------------------------------------------------------- support
+    c.supportLaunch = bodyMatchable && !repeatedBody && !bodyIsHead && hp >= 0 && c.headSupport > 0;
+    if (c.supportLaunch) {
+        Builder B(r, g);
+        Plan& P = c.sup;
+        P.kind = kSupport;
+        P.budget = t.budget;
+        const TermIR &hs = r.head.s, &ho = r.head.o;
+        if (hs.var) B.bind(hs.v);
+        if (ho.var) B.bind(ho.v);
+        P.hsRef = B.ref(hs);
+        P.hoRef = B.ref(ho);
+        if (!hs.var) {
+            auto rr = g.row(0, hp, hs.v);
+            P.seedDir = kSPO; P.seedBegin = rr.first; P.seedEnd = rr.second;
+        } else if (!ho.var) {
+            auto rr = g.row(1, hp, ho.v);
+            P.seedDir = kPOS; P.seedBegin = (int32_t)(g.T() + rr.first); P.seedEnd = (int32_t)(g.T() + rr.second);
+        } else {
+            P.seedDir = kSPO; P.seedBegin = g.predOff[hp]; P.seedEnd = g.predOff[hp + 1];
+        }
+        P.pairLevel = -1;
+        Builder B0 = B;
+        std::vector<int> ord = B.order();
+        B.levels(ord, P, c.supRefs, true);
+        c.supRefs.hs = P.hsRef;
+        c.supRefs.ho = P.hoRef;
+        c.supItems = P.seedEnd - P.seedBegin;
+        c.supWave = P;
+        c.supWaveRefs = c.supRefs;
+        if (wave_order(P, ord)) {
+            B0.levels(ord, c.supWave, c.supWaveRefs, true);
+            c.supWaveOk = true;
+        }
+    }
+
+    // ------------------------------------------------------- body pairs
+    bool headVars = r.head.s.var && r.head.o.var;
+    bool sIn = false, oIn = false;
+    for (const AtomIR& a : r.body)
+        for (const TermIR* x : {&a.s, &a.o}) {
+            if (x->var && headVars && x->v == r.head.s.v) sIn = true;
+            if (x->var && headVars && x->v == r.head.o.v) oIn = true;
+        }
+    c.pairsLaunch = bodyMatchable && !repeatedBody && headVars && sIn && oIn;
+    if (c.pairsLaunch) {
+        // seed keys: the head-subject side of the body atom with the fewest
+        // distinct keys there (CPU_01_nonparallel/SupportCounting.cpp:577-605
+        // seeds from the smallest such atom)
+        int32_t bestDir = -1, bestP = -1, bestN = std::numeric_limits<int32_t>::max();
+        for (const AtomIR& a : r.body) {
+            int32_t p = g.dense_pred(a.pterm);
+            if (a.s.var && a.s.v == r.head.s.v && g.nkeys(0, p) < bestN) { bestDir = 0; bestP = p; bestN = g.nkeys(0, p); }
+            if (a.o.var && a.o.v == r.head.s.v && g.nkeys(1, p) < bestN) { bestDir = 1; bestP = p; bestN = g.nkeys(1, p); }
+        }
+        Builder B(r, g);
+        Plan& P = c.pairs;
+        P.kind = kPairs;
+        P.budget = t.budget;
+        B.bind(r.head.s.v);
+        P.hsRef = 0;
+        P.seedDir = bestDir;
+        const int32_t kbase = bestDir == 0 ? 0 : (int32_t)g.keysS.size();
+        const std::vector<int32_t>& off = bestDir == 0 ? g.keysSOff : g.keysOOff;
+        P.seedBegin = kbase + off[bestP];
+        P.seedEnd = kbase + off[bestP + 1];
+        Builder B0 = B;
+        std::vector<int> ord = B.order();
+        B.levels(ord, P, c.pairsRefs, false);
+        auto finish = [&](Plan& Q, const Builder& b) {
+            Q.hoRef = b.slotOf[r.head.o.v];
+            Q.pairLevel = -1;
+            if (r.head.o.v != r.head.s.v)
+                for (int L = 0; L < Q.nlev; ++L) {
+                    const Level& lv = Q.lev[L];
+                    const int32_t kd = level_kind(lv.flags);
+                    if ((kd == kExt && lv.val == Q.hoRef) ||
+                        (kd == kScan && (lv.val == Q.hoRef || lv.val2 == Q.hoRef))) {
+                        Q.pairLevel = L;
+                        break;
+                    }
+                }
+        };
+        finish(P, B);
+        c.pairItems = P.seedEnd - P.seedBegin;
+        c.pairsWave = P;
+        c.pairsWaveRefs = c.pairsRefs;
+        if (wave_order(P, ord)) {
+            B0.levels(ord, c.pairsWave, c.pairsWaveRefs, false);
+            finish(c.pairsWave, B0);
+            c.pairsWaveOk = true;
+        }
+    }
+    return c;
+}
+
+}  // namespace gr

@@ -1,0 +1,36 @@
+// planner.hpp -- compiles a rule into device plans.
+#pragma once
+#include <cstdint>
+
+#include "graph_host.hpp"
+#include "plan.hpp"
+#include "rules_host.hpp"
+
+namespace gr {
+
+enum RuleStatus : int32_t {
+    kRuleOk = 0,        // counted on the device (or statically zero)
+    kRuleSkipped = 1,   // unsupported shape: body not in [1,8] atoms or > 16 variables
+    kRuleError = 2,     // parse error
+};
+
+struct Tuning {
+    int32_t budget = 4096;       // thread-mode candidates per seed before escalation
+};
+
+struct CompiledRule {
+    int32_t status = kRuleOk;
+    int64_t headSize = 0;
+    int64_t headSupport = 0;
+    bool supportLaunch = false;  // false: support is statically 0
+    bool pairsLaunch = false;    // false: body pairs statically 0
+    Plan sup{}, supWave{};       // thread order / wave order (level 0 is the split level)
+    Plan pairs{}, pairsWave{};
+    ConflictRefs supRefs{}, supWaveRefs{}, pairsRefs{}, pairsWaveRefs{};
+    bool supWaveOk = false, pairsWaveOk = false;
+    int64_t supItems = 0, pairItems = 0;
+};
+
+CompiledRule compile_rule(const RuleIR& r, const HostGraph& g, const Tuning& t);
+
+}  // namespace gr

@@ -1,0 +1,214 @@
+"""ctypes binding of libgpu_rules.so.
+
+`GpuRulesLib` mirrors the reference's JNA binding (GPU_03/GpuRulesLib.scala:
+23-58: the GpuRuleResult structure in the same field order, and the
+gpu_load_graph / gpu_evaluate_rule / gpu_free_graph / gpu_query_vram calls),
+with the same C calling convention JNA uses, plus the library's additive
+entry points. `Engine` is a small convenience layer over it for term-id
+graphs and structured rules.
+
+There is no fallback: if the library is missing or no GPU is present, the
+calls fail loudly (RuntimeError / NULL context).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("GPU_RULES_LIB", os.path.join(PKG, "libgpu_rules.so"))
+
+RULE_WORDS = 28
+UNGATED = 1
+PER_RULE = 2
+
+
+class GpuRuleResult(ctypes.Structure):
+    """GPU_03/gpu_rules_api.h:21-30 (56 bytes)."""
+    _fields_ = [("support", ctypes.c_int), ("headSize", ctypes.c_int), ("headSupport", ctypes.c_int),
+                ("headCoverage", ctypes.c_double), ("bodySize", ctypes.c_int), ("confidence", ctypes.c_double),
+                ("supportMs", ctypes.c_double), ("confidenceMs", ctypes.c_double)]
+
+
+class GpuRuleResult64(ctypes.Structure):
+    _fields_ = [("support", ctypes.c_int64), ("headSize", ctypes.c_int64), ("headSupport", ctypes.c_int64),
+                ("bodySize", ctypes.c_int64), ("bodyPairs", ctypes.c_int64), ("headCoverage", ctypes.c_double),
+                ("confidence", ctypes.c_double), ("status", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+RESULT64_DTYPE = np.dtype([("support", "<i8"), ("headSize", "<i8"), ("headSupport", "<i8"), ("bodySize", "<i8"),
+                           ("bodyPairs", "<i8"), ("headCoverage", "<f8"), ("confidence", "<f8"),
+                           ("status", "<i4"), ("reserved", "<i4")])
+assert RESULT64_DTYPE.itemsize == ctypes.sizeof(GpuRuleResult64)
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "gpu_load_graph": (ctypes.c_void_p, [ctypes.c_char_p]),
+    "gpu_evaluate_rule": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(GpuRuleResult)]),
+    "gpu_evaluate_rule_json": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(GpuRuleResult)]),
+    "gpu_free_graph": (None, [ctypes.c_void_p]),
+    "gpu_query_vram": (ctypes.c_int, [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]),
+    "gpu_load_graph_triples": (ctypes.c_void_p, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                                 ctypes.c_int64]),
+    "gpu_evaluate_rules": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                                          ctypes.c_void_p]),
+    "gpu_evaluate_rules_text": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_char_p), ctypes.c_int32,
+                                               ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p]),
+    "gpu_last_batch_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
+                                            ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                            ctypes.POINTER(ctypes.c_int64)]),
+    "gpu_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int64]),
+    "gpu_get_tuning": (ctypes.c_int64, [ctypes.c_void_p, ctypes.c_char_p]),
+    "gpu_host_load_turtle": (ctypes.c_void_p, [ctypes.c_char_p]),
+    "gpu_host_free": (None, [ctypes.c_void_p]),
+    "gpu_host_graph_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64),
+                                           ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64)]),
+    "gpu_host_term_id": (ctypes.c_int64, [ctypes.c_void_p, ctypes.c_char_p]),
+    "gpu_host_parse_rule": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int32, ctypes.c_void_p]),
+    "gpu_host_plan_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "gpu_rules_version": (ctypes.c_char_p, []),
+}
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Loads libgpu_rules.so and declares every entry point."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError("libgpu_rules.so is not built (%s); run masterthesis_gpu_acceleration_amd.build" % path)
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_names():
+    return list(_SIGS)
+
+
+class GpuRulesLib:
+    """The reference binding's surface (GpuRulesLib.scala:46-116): load a
+    graph once, evaluate rules, free. evaluate_rule raises on a non-zero
+    return code, as the Scala wrapper does (:107-112)."""
+
+    def __init__(self, path: str = LIB_PATH):
+        self.lib = load_library(path)
+
+    def query_vram(self):
+        f = ctypes.c_size_t()
+        t = ctypes.c_size_t()
+        rc = self.lib.gpu_query_vram(ctypes.byref(f), ctypes.byref(t))
+        return rc, f.value, t.value
+
+    def load_graph(self, ttl_path: str):
+        ctx = self.lib.gpu_load_graph(ttl_path.encode())
+        if not ctx:
+            raise RuntimeError("gpu_load_graph failed for " + ttl_path)
+        return ctx
+
+    def evaluate_rule(self, ctx, rule_text: str) -> GpuRuleResult:
+        out = GpuRuleResult()
+        rc = self.lib.gpu_evaluate_rule(ctx, rule_text.encode(), ctypes.byref(out))
+        if rc != 0:
+            raise RuntimeError("gpu_evaluate_rule returned %d" % rc)
+        return out
+
+    def evaluate_rule_json(self, ctx, rule_json: str) -> GpuRuleResult:
+        out = GpuRuleResult()
+        rc = self.lib.gpu_evaluate_rule_json(ctx, rule_json.encode(), ctypes.byref(out))
+        if rc != 0:
+            raise RuntimeError("gpu_evaluate_rule_json returned %d" % rc)
+        return out
+
+    def free_graph(self, ctx):
+        self.lib.gpu_free_graph(ctx)
+
+
+class Engine:
+    """A resident graph plus batched / per-rule evaluation of structured
+    rules (int32 records of RULE_WORDS words, see workloads.encode_rules)."""
+
+    def __init__(self, ctx, lib=None):
+        self.lib = lib or load_library()
+        self.ctx = ctx
+
+    @classmethod
+    def from_ttl(cls, path: str):
+        lib = load_library()
+        ctx = lib.gpu_load_graph(path.encode())
+        if not ctx:
+            raise RuntimeError("gpu_load_graph failed for " + path)
+        return cls(ctx, lib)
+
+    @classmethod
+    def from_triples(cls, s, p, o, n_terms: int):
+        lib = load_library()
+        s = np.ascontiguousarray(s, np.int32)
+        p = np.ascontiguousarray(p, np.int32)
+        o = np.ascontiguousarray(o, np.int32)
+        ctx = lib.gpu_load_graph_triples(s.ctypes.data, p.ctypes.data, o.ctypes.data, int(s.shape[0]), int(n_terms))
+        if not ctx:
+            raise RuntimeError("gpu_load_graph_triples failed (no GPU, HBM short, or bad ids)")
+        return cls(ctx, lib)
+
+    @classmethod
+    def from_graph(cls, g):
+        """workloads.Graph: entity i -> term i, predicate k -> term n_entities + k."""
+        return cls.from_triples(g.s, g.p.astype(np.int64) + g.n_entities, g.o, g.n_entities + g.n_predicates)
+
+    def evaluate(self, rule_words: np.ndarray, flags: int = 0) -> np.ndarray:
+        w = np.ascontiguousarray(rule_words, np.int32)
+        n = int(w.shape[0])
+        out = np.zeros(n, RESULT64_DTYPE)
+        rc = self.lib.gpu_evaluate_rules(self.ctx, w.ctypes.data, n, int(flags), out.ctypes.data)
+        if rc != 0:
+            raise RuntimeError("gpu_evaluate_rules returned %d" % rc)
+        return out
+
+    def evaluate_text(self, rules, is_json: bool = False, flags: int = 0) -> np.ndarray:
+        n = len(rules)
+        arr = (ctypes.c_char_p * n)(*[r.encode() for r in rules])
+        out = np.zeros(n, RESULT64_DTYPE)
+        rc = self.lib.gpu_evaluate_rules_text(self.ctx, arr, n, int(is_json), int(flags), out.ctypes.data)
+        if rc != 0:
+            raise RuntimeError("gpu_evaluate_rules_text returned %d" % rc)
+        return out
+
+    def last_stats(self):
+        t, s, p = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        h = ctypes.c_int64()
+        self.lib.gpu_last_batch_stats(self.ctx, ctypes.byref(t), ctypes.byref(s), ctypes.byref(p), ctypes.byref(h))
+        return {"total_ms": t.value, "support_ms": s.value, "pairs_ms": p.value, "heavy_seeds": h.value}
+
+    def set_tuning(self, key: str, value: int):
+        if self.lib.gpu_set_tuning(self.ctx, key.encode(), int(value)) != 0:
+            raise KeyError(key)
+
+    def get_tuning(self, key: str) -> int:
+        return int(self.lib.gpu_get_tuning(self.ctx, key.encode()))
+
+    def close(self):
+        if self.ctx:
+            self.lib.gpu_free_graph(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def rule_words_for_graph(rules, g):
+    """Structured rules over a workloads.Graph loaded with Engine.from_graph."""
+    from . import workloads as W
+    pred_id = np.arange(g.n_predicates, dtype=np.int64) + g.n_entities
+    return W.encode_rules(rules, None, pred_id)

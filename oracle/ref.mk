@@ -21,7 +21,9 @@ CPU01_SRC     := $(REF)/CPU_01_nonparallel/main.cpp $(REF)/CPU_01_nonparallel/Rd
                  $(REF)/CPU_01_nonparallel/RuleParser.cpp $(REF)/CPU_01_nonparallel/FinalRule.cpp \
                  $(REF)/CPU_01_nonparallel/SupportCounting.cpp
 
-all: $(OUT)/rewritten $(OUT)/cpu01
+FRONT_SRC     := $(REF)/GPU_03/RdfIndexes.cpp $(REF)/GPU_03/RuleParser.cpp $(REF)/GPU_03/FinalRule.cpp
+
+all: $(OUT)/rewritten $(OUT)/cpu01 $(OUT)/frontend_probe
 
 $(OUT)/rewritten: $(REWRITTEN_SRC)
 	@mkdir -p $(OUT)
@@ -30,5 +32,11 @@ $(OUT)/rewritten: $(REWRITTEN_SRC)
 $(OUT)/cpu01: $(CPU01_SRC)
 	@mkdir -p $(OUT)
 	$(CXX) $(CXXFLAGS) -I$(REF)/CPU_01_nonparallel $(CPU01_SRC) -o $@
+
+# the reference's Turtle reader and rule parser behind a probe main of this
+# project (oracle/ref_frontend_probe.cpp)
+$(OUT)/frontend_probe: oracle/ref_frontend_probe.cpp $(FRONT_SRC)
+	@mkdir -p $(OUT)
+	$(CXX) $(CXXFLAGS) -I$(REF)/GPU_03 oracle/ref_frontend_probe.cpp $(FRONT_SRC) -o $@
 
 .PHONY: all
