@@ -1,0 +1,229 @@
+#!/usr/bin/env python3
+"""Benchmark of the rule support / body-pair counting path on one MI355X.
+
+Workload (default): the DBpedia-shaped configuration -- 206,000 triples over
+50,000 entities and 300 predicates (predicate frequency Zipf s=1.0,
+per-predicate out-degree lognormal(0,1)), 10,000 closed rules of 1-3 body
+atoms -- generated from a fixed seed (the reference's datasets are not
+available, so the data is synthetic).
+
+A step is one evaluation of all rules: exact support and distinct body-pair
+count for every rule (the body pairs of every rule are counted, also when
+support is 0, i.e. the work CPU_01_nonparallel does), from a batch compiled
+once and kept resident in HBM, replayed as a HIP graph. The JSON line also
+reports:
+  * per_rule: the same rules through the per-rule launch sequence of the
+    reference library (one support launch and one body-size launch per rule,
+    each followed by a sync and a device-to-host copy), with p50/p99 latency;
+  * roofline: algorithmic bytes of the dominant kernel (host traffic model of
+    the same plans) over its event-timed duration, against 8 TB/s;
+  * cpu_baseline: the reference's CPU_01_nonparallel program (built from the
+    reference sources into oracle/_ref/, single-threaded) on a bounded sample
+    of the same rules, or the project's CPU oracle if that build is absent.
+
+Usage: python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from masterthesis_gpu_acceleration_amd import lib as L  # noqa: E402
+from masterthesis_gpu_acceleration_amd import workloads as W  # noqa: E402
+
+METRIC = "rules/s & body-pairs/s (batched vs per-rule launch); p50/p99 rule latency"
+HBM_PEAK_GBPS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def pct(x, q):
+    x = np.sort(np.asarray(x, np.float64))
+    if x.size == 0:
+        return 0.0
+    k = max(0, min(x.size - 1, int(np.ceil(q * x.size)) - 1))
+    return float(x[k])
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(wl, rules, sample):
+    """Reference CPU_01_nonparallel (single thread) on the first `sample`
+    rules; rule compute = sum of its per-rule support + confidence times, as
+    the reference notebooks count it (RunBenchmark.ipynb; thesis p.43)."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "cpu01")
+    sub = rules[:sample]
+    if os.path.exists(exe):
+        with tempfile.TemporaryDirectory() as d:
+            ttl = os.path.join(d, "g.ttl")
+            rf = os.path.join(d, "r.txt")
+            W.write_ttl(wl.graph, ttl)
+            W.write_rules(sub, rf)
+            t = time.time()
+            out = subprocess.run([exe, ttl, rf, "--json"], capture_output=True, text=True, timeout=600, check=True)
+            wall = time.time() - t
+        rows = json.loads(out.stdout)
+        ms = sum(r["supportMs"] + r["confidenceMs"] for r in rows)
+        return {"value": round(len(rows) / (ms / 1000.0), 2), "unit": "rules/s", "cores": 1, "kind": "reference",
+                "sample": "first %d of the %d rules, full graph; CPU_01_nonparallel rule compute %.1f ms "
+                          "(wall incl. Turtle load %.1f s); host %s, nproc %d" % (
+                              len(rows), len(rules), ms, wall, cpu_model(), os.cpu_count() or 0)}
+    from oracle import Oracle  # CPU oracle (the port) when the reference build is absent
+    g = wl.graph
+    o = Oracle(g.s, g.p, g.o)
+    words = W.encode_rules(sub)
+    t = time.time()
+    o.eval(words)
+    dt = time.time() - t
+    return {"value": round(len(sub) / dt, 2), "unit": "rules/s", "cores": 1, "kind": "port",
+            "sample": "first %d of the %d rules, full graph; project CPU oracle; host %s, nproc %d" % (
+                len(sub), len(rules), cpu_model(), os.cpu_count() or 0)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-sample", type=int, default=2000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-per-rule", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (rules are independent; see "
+              "DESIGN.md for how the work would shard). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    t0 = time.time()
+    wl = W.dbpedia_shaped()
+    g = wl.graph
+    rules = wl.rules
+    log("workload %s: %d triples, %d entities, %d predicates, %d rules (%.1fs)" % (
+        wl.name, g.n_triples, g.n_entities, g.n_predicates, len(rules), time.time() - t0))
+    words = L.rule_words_for_graph(rules, g)
+    t = time.time()
+    eng = L.Engine.from_graph(g)
+    log("graph resident in HBM (%.2fs)" % (time.time() - t))
+
+    flags = L.UNGATED
+    t = time.time()
+    batch = eng.batch(words, flags)
+    build_s = time.time() - t
+    log("batch compiled and uploaded (%.3fs)" % build_s)
+    for _ in range(args.warmup):
+        res = batch.run()
+    log("warmup done")
+    dev_ms = []
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        res = batch.run()
+        dev_ms.append(eng.last_stats()["total_ms"])
+    wall = time.perf_counter() - t
+    ms_step = wall * 1000.0 / args.steps
+    res = res.copy()
+    log("timed %d steps: %.3f ms/step (device %.3f ms)" % (args.steps, ms_step, float(np.mean(dev_ms))))
+    n = len(rules)
+    pairs = int(res["bodyPairs"][res["bodyPairs"] > 0].sum())
+    rules_per_s = n / (ms_step / 1000.0)
+
+    # end to end: compile + upload + run of a fresh batch
+    t = time.perf_counter()
+    eng.evaluate(words, flags)
+    e2e_s = time.perf_counter() - t
+
+    # roofline of the dominant kernel: eager pass with per-kernel events
+    eng.set_tuning("use_graph", 0)
+    kms = np.zeros(4)
+    reps = 3
+    for _ in range(reps):
+        eng.evaluate(words, flags)
+        kms += eng.kernel_ms()
+    kms /= reps
+    eng.set_tuning("use_graph", 1)
+    traffic = eng.traffic(words)
+    names = ["k_thread_batch (support)", "k_wave_batch (support)", "k_thread_batch (body pairs)",
+             "k_wave_batch (body pairs)"]
+    dom = int(np.argmax(kms))
+    phase = dom // 2
+    bytes_phase = int(traffic[:, phase].sum())
+    achieved = bytes_phase / (kms[dom] * 1e-3) / 1e9 if kms[dom] > 0 else 0.0
+    log("kernels ms: %s" % ", ".join("%s=%.3f" % (a, b) for a, b in zip(names, kms)))
+
+    out = {
+        "metric": METRIC,
+        "value": round(rules_per_s, 1),
+        "unit": "rules/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": "dbpedia_shaped: %d triples, %d entities, %d predicates, %d rules of 1-3 atoms, "
+                               "exact support + distinct body pairs for every rule, resident batch, HIP graph "
+                               "replay" % (g.n_triples, g.n_entities, g.n_predicates, n),
+                   "rules": n, "triples": g.n_triples, "seed": hex(W.SEED_BASE + 2)},
+        "body_pairs_per_s": round(pairs / (ms_step / 1000.0), 1),
+        "body_pairs_total": pairs,
+        "rules_with_support": int((res["support"] > 0).sum()),
+        "e2e_rules_per_s": round(n / e2e_s, 1),
+        "batch_build_s": round(build_s, 4),
+        "latency_ms": {"batched_p50": round(pct([x for x in dev_ms], 0.5), 4),
+                       "batched_p99": round(pct([x for x in dev_ms], 0.99), 4)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBPS, 5), "traffic": None,
+                     "kernel": names[dom], "kernel_ms": round(float(kms[dom]), 4),
+                     "algorithmic_bytes": bytes_phase, "kernel_ms_all": [round(float(x), 4) for x in kms]},
+    }
+
+    if not args.no_per_rule:
+        t = time.perf_counter()
+        pr = eng.evaluate(words, flags | L.PER_RULE)
+        pr_wall = time.perf_counter() - t
+        lat = pr["latencyUs"].astype(np.float64) / 1000.0
+        same = bool(np.array_equal(pr["support"], res["support"]) and np.array_equal(pr["bodyPairs"], res["bodyPairs"]))
+        out["per_rule"] = {"rules_per_s": round(n / pr_wall, 1), "wall_s": round(pr_wall, 3),
+                           "p50_ms": round(pct(lat, 0.5), 4), "p99_ms": round(pct(lat, 0.99), 4),
+                           "same_counts_as_batched": same}
+        out["speedup_vs_per_rule"] = round(rules_per_s / (n / pr_wall), 2)
+        out["latency_ms"]["per_rule_p50"] = out["per_rule"]["p50_ms"]
+        out["latency_ms"]["per_rule_p99"] = out["per_rule"]["p99_ms"]
+        log("per-rule launch: %.1f rules/s, p50 %.3f ms p99 %.3f ms, same counts: %s" % (
+            n / pr_wall, out["per_rule"]["p50_ms"], out["per_rule"]["p99_ms"], same))
+
+    if not args.no_cpu:
+        t = time.time()
+        out["cpu_baseline"] = cpu_baseline(wl, rules, args.cpu_sample)
+        log("cpu baseline %.1fs: %s" % (time.time() - t, out["cpu_baseline"]["value"]))
+
+    batch.close()
+    eng.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

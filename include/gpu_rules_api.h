@@ -85,7 +85,7 @@ typedef struct {
     double  headCoverage;
     double  confidence;
     int32_t status;        /* 0 ok, 1 unsupported shape, 2 parse error */
-    int32_t reserved;
+    int32_t latencyUs;     /* GPU_RULES_PER_RULE: host wall time of this rule's launches (us) */
 } GpuRuleResult64;
 
 /* Batch flags. */
@@ -110,8 +110,28 @@ int gpu_evaluate_rules(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t f
 int gpu_evaluate_rules_text(void* ctx, const char* const* rules, int32_t n, int32_t isJson, int32_t flags,
                             GpuRuleResult64* out);
 
+/* A compiled batch kept resident in HBM (plans, seed chunks, counters), so
+ * the same rules can be evaluated again without host work: each
+ * gpu_batch_run replays the batch's HIP graph (or launches eagerly when the
+ * context's "use_graph" is 0) and fills out[n]. The batch must be freed
+ * before its context. Flags as for gpu_evaluate_rules (GPU_RULES_UNGATED). */
+void* gpu_batch_create(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags);
+int gpu_batch_run(void* batch, GpuRuleResult64* out);
+void gpu_batch_free(void* batch);
+
 /* Device time of the last gpu_evaluate_rules* call (ms) and its phases. */
 int gpu_last_batch_stats(void* ctx, double* totalMs, double* supportMs, double* pairsMs, int64_t* heavySeeds);
+
+/* Device time of the four batched kernels of the last eager (non-graph)
+ * batch: support thread mode, support wave mode, pairs thread mode, pairs
+ * wave mode (ms). Zero after a graph replay. */
+int gpu_last_kernel_ms(void* ctx, double* ms4);
+
+/* Host model of the HBM bytes each rule's searches read (the same plans,
+ * walked serially): out[4*i] support bytes, out[4*i+1] body-pair bytes,
+ * out[4*i+2], out[4*i+3] search levels entered. Works on a context or a
+ * host handle; does no device work. */
+int gpu_plan_traffic(void* handle, const GpuRuleDesc* rules, int32_t n, int64_t* out);
 
 /* Tuning knobs: "budget" (thread-mode candidates per seed before a seed is
  * moved to wave mode), "thread_waves_per_cu", "wave_waves_per_cu",

@@ -27,7 +27,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result", "-I/opt/r
 HIPFLAGS = ["-x", "hip", "--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-fno-gpu-rdc"]
 
-SOURCES = ["graph_host.cpp", "rules_host.cpp", "planner.cpp", "engine.cpp", "kernels.hip"]
+SOURCES = ["graph_host.cpp", "rules_host.cpp", "planner.cpp", "traffic.cpp", "engine.cpp", "kernels.hip"]
 
 
 def _compile(src: str) -> str:

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -14,6 +15,7 @@
 #include "kernels.hpp"
 #include "planner.hpp"
 #include "rules_host.hpp"
+#include "traffic.hpp"
 
 namespace gr {
 
@@ -81,17 +83,20 @@ struct Batch : Handle {
     int32_t flags = 0;
     std::vector<CompiledRule> comp;
     std::vector<int32_t> status;
-    // chunk lists and heavy-queue segments per (phase, body length)
-    uint32_t nChunks[2][kMaxBody + 1] = {};
-    uint32_t chunkOff[2][kMaxBody + 1] = {};
-    uint32_t heavyCap[2][kMaxBody + 1] = {};
-    uint64_t heavyOff[2][kMaxBody + 1] = {};
+    // chunk lists and heavy-queue segments per phase (0 support, 1 pairs)
+    uint32_t nChunks[2] = {};
+    uint32_t chunkOff[2] = {};
+    uint32_t heavyCap[2] = {};
+    uint64_t heavyOff[2] = {};
     uint32_t capRules = 0;
+    double kernelMs[4] = {};
     DevBuf plans, wplans, refs, wrefs, chunks, counts, heavy, ctl;
     unsigned long long* hCounts = nullptr;   // pinned
     uint32_t* hCtl = nullptr;                // pinned
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    const void* capturedDedup = nullptr;   // context scratch the graph was captured with
+    int capturedBlocks[2] = {0, 0};
     double lastMs = 0, lastSupMs = 0, lastPairMs = 0;
     int64_t lastHeavy = 0;
     ~Batch();
@@ -107,7 +112,7 @@ struct Context : Handle {
     int waveWavesPerCU = 8;
     bool useGraph = true;
     hipStream_t st = nullptr;
-    hipEvent_t ev[4] = {};
+    hipEvent_t ev[6] = {};
     DevBuf rows, col, coo, keys;
     DevGraph dg{};
     // per-rule launch scratch
@@ -121,6 +126,7 @@ struct Context : Handle {
     int waveSlots = 0;
     Batch* lastBatch = nullptr;
     double lastMs = 0, lastSupMs = 0, lastPairMs = 0;
+    double lastKernelMs[4] = {};
     int64_t lastHeavy = 0;
 
     int waveBlocks() const { return std::max(1, waveSlots / 4); }
@@ -164,7 +170,7 @@ bool Context::upload() {
     numCUs = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     GR_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     for (auto& e : ev) GR_HIP(hipEventCreate(&e));
-    GR_HIP(hipHostMalloc((void**)&hOne, 64, hipHostMallocDefault));
+    GR_HIP(hipHostMalloc((void**)&hOne, 512, hipHostMallocDefault));
 
     const int32_t P = g.P();
     const int64_t nE = std::max<int64_t>(g.nE, 1);
@@ -309,7 +315,7 @@ bool eval_rule_plain(Context& C, const RuleIR& r, bool gated, GpuRuleResult64& r
 
 // ------------------------------------------------------------ batch path
 
-constexpr int kCtlWords = 2 * (kMaxBody + 1) * 4;   // per (phase, N): thread cursor, heavy count, wave cursor
+constexpr int kCtlWords = 8;   // per phase: thread cursor, heavy count, wave cursor, spare
 
 bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const std::vector<int32_t>& parseStatus,
                  int32_t flags) {
@@ -322,8 +328,8 @@ bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const s
     const size_t np = std::max<size_t>(2 * rules.size(), 1);
     std::vector<Plan> plans(np), wplans(np);
     std::vector<ConflictRefs> refs(np), wrefs(np);
-    std::vector<Chunk> lists[2][kMaxBody + 1];
-    uint64_t items[2][kMaxBody + 1] = {};
+    std::vector<Chunk> lists[2];
+    uint64_t items[2] = {0, 0};
     auto add = [&](int ph, size_t i, const Plan& P, const ConflictRefs& R, const Plan& W, const ConflictRefs& RW,
                    int64_t n) {
         const size_t k = 2 * i + ph;
@@ -332,11 +338,10 @@ bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const s
         refs[k] = R;
         wrefs[k] = RW;
         plans[k].rule = wplans[k].rule = (int32_t)i;
-        const int N = P.nlev;
         for (int64_t b = 0; b < n; b += kWave)
-            lists[ph][N].push_back(Chunk{(uint32_t)k, (uint32_t)b, (uint32_t)std::min<int64_t>(kWave, n - b),
-                                         (ph == 1 && gated) ? 1u : 0u});
-        items[ph][N] += (uint64_t)n;
+            lists[ph].push_back(Chunk{(uint32_t)k, (uint32_t)b, (uint32_t)std::min<int64_t>(kWave, n - b),
+                                      (ph == 1 && gated) ? 1u : 0u});
+        items[ph] += (uint64_t)n;
     };
     for (size_t i = 0; i < rules.size(); ++i) {
         if (B.status[i] != kRuleOk) continue;
@@ -351,15 +356,14 @@ bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const s
     }
     std::vector<Chunk> all;
     uint64_t heavyTotal = 0;
-    for (int ph = 0; ph < 2; ++ph)
-        for (int N = 1; N <= kMaxBody; ++N) {
-            B.chunkOff[ph][N] = (uint32_t)all.size();
-            B.nChunks[ph][N] = (uint32_t)lists[ph][N].size();
-            all.insert(all.end(), lists[ph][N].begin(), lists[ph][N].end());
-            B.heavyOff[ph][N] = heavyTotal;
-            B.heavyCap[ph][N] = (uint32_t)items[ph][N];
-            heavyTotal += items[ph][N];
-        }
+    for (int ph = 0; ph < 2; ++ph) {
+        B.chunkOff[ph] = (uint32_t)all.size();
+        B.nChunks[ph] = (uint32_t)lists[ph].size();
+        all.insert(all.end(), lists[ph].begin(), lists[ph].end());
+        B.heavyOff[ph] = heavyTotal;
+        B.heavyCap[ph] = (uint32_t)items[ph];
+        heavyTotal += items[ph];
+    }
     if (all.empty()) all.push_back(Chunk{0, 0, 0, 0});
     B.capRules = std::max<uint32_t>((uint32_t)rules.size(), 1);
     hipStream_t st = C.st;
@@ -376,8 +380,9 @@ bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const s
 }
 
 // The device pass of a batch (what the HIP graph captures): per phase, one
-// persistent thread-mode kernel per body length, then one wave-mode kernel
-// per body length over the seeds the first escalated.
+// persistent thread-mode kernel over all chunks, then one wave-mode kernel
+// over the seeds it escalated. PAIRS chunks of rules whose support is 0 are
+// skipped on the device when the batch is gated.
 void batch_enqueue(Context& C, Batch& B, bool timed) {
     hipStream_t st = C.st;
     uint32_t* ctl = B.ctl.as<uint32_t>();
@@ -386,23 +391,21 @@ void batch_enqueue(Context& C, Batch& B, bool timed) {
     launch_fill_u32(ctl, kCtlWords, 0, st);
     if (timed) (void)hipEventRecord(C.ev[0], st);
     for (int ph = 0; ph < 2; ++ph) {
-        for (int N = 1; N <= kMaxBody; ++N) {
-            const uint32_t nc = B.nChunks[ph][N];
-            if (!nc) continue;
-            uint32_t* c = ctl + (ph * (kMaxBody + 1) + N) * 4;
-            ThreadArgs ta{B.plans.as<Plan>(), B.refs.as<ConflictRefs>(), B.chunks.as<Chunk>() + B.chunkOff[ph][N], nc,
-                          B.heavyCap[ph][N], c + 0, counts, B.heavy.as<HeavyItem>() + B.heavyOff[ph][N], c + 1};
+        const uint32_t nc = B.nChunks[ph];
+        uint32_t* c = ctl + ph * 4;
+        if (nc) {
+            ThreadArgs ta{B.plans.as<Plan>(), B.refs.as<ConflictRefs>(), B.chunks.as<Chunk>() + B.chunkOff[ph], nc,
+                          B.heavyCap[ph], c + 0, counts, B.heavy.as<HeavyItem>() + B.heavyOff[ph], c + 1};
             const int blocks = std::min<int>(C.threadBlocks(), (int)((nc + 3) / 4));
             launch_thread_batch(ta, C.dg, std::max(blocks, 1), st);
         }
-        for (int N = 1; N <= kMaxBody; ++N) {
-            if (!B.nChunks[ph][N]) continue;
-            uint32_t* c = ctl + (ph * (kMaxBody + 1) + N) * 4;
-            WaveArgs wa{B.wplans.as<Plan>(), B.wrefs.as<ConflictRefs>(), B.heavy.as<HeavyItem>() + B.heavyOff[ph][N],
-                        c + 1, B.heavyCap[ph][N], 0, c + 2, counts, C.dedup.as<uint32_t>(), C.rowWords};
+        if (timed) (void)hipEventRecord(C.ev[1 + 2 * ph], st);
+        if (nc) {
+            WaveArgs wa{B.wplans.as<Plan>(), B.wrefs.as<ConflictRefs>(), B.heavy.as<HeavyItem>() + B.heavyOff[ph],
+                        c + 1, B.heavyCap[ph], 0, c + 2, counts, C.dedup.as<uint32_t>(), C.rowWords};
             launch_wave_batch(wa, C.dg, C.waveBlocks(), st);
         }
-        if (timed) (void)hipEventRecord(C.ev[1 + ph], st);
+        if (timed) (void)hipEventRecord(C.ev[2 + 2 * ph], st);
     }
     (void)hipMemcpyAsync(B.hCounts, counts, (size_t)2 * B.capRules * 8, hipMemcpyDeviceToHost, st);
     (void)hipMemcpyAsync(B.hCtl, ctl, kCtlWords * 4, hipMemcpyDeviceToHost, st);
@@ -411,7 +414,18 @@ void batch_enqueue(Context& C, Batch& B, bool timed) {
 bool batch_run(Context& C, Batch& B, GpuRuleResult64* out) {
     const bool graphMode = C.useGraph;
     if (graphMode) {
+        // re-capture if the context's scratch or launch shape changed since
+        if (B.exec && (B.capturedDedup != C.dedup.p || B.capturedBlocks[0] != C.threadBlocks() ||
+                       B.capturedBlocks[1] != C.waveBlocks())) {
+            (void)hipGraphExecDestroy(B.exec);
+            (void)hipGraphDestroy(B.graph);
+            B.exec = nullptr;
+            B.graph = nullptr;
+        }
         if (!B.exec) {
+            B.capturedDedup = C.dedup.p;
+            B.capturedBlocks[0] = C.threadBlocks();
+            B.capturedBlocks[1] = C.waveBlocks();
             GR_HIP(hipStreamBeginCapture(C.st, hipStreamCaptureModeThreadLocal));
             batch_enqueue(C, B, false);
             GR_HIP(hipStreamEndCapture(C.st, &B.graph));
@@ -419,28 +433,27 @@ bool batch_run(Context& C, Batch& B, GpuRuleResult64* out) {
         }
         GR_HIP(hipEventRecord(C.ev[0], C.st));
         GR_HIP(hipGraphLaunch(B.exec, C.st));
-        GR_HIP(hipEventRecord(C.ev[3], C.st));
+        GR_HIP(hipEventRecord(C.ev[5], C.st));
         GR_HIP(hipStreamSynchronize(C.st));
         float f = 0;
-        (void)hipEventElapsedTime(&f, C.ev[0], C.ev[3]);
+        (void)hipEventElapsedTime(&f, C.ev[0], C.ev[5]);
         B.lastMs = f;
         B.lastSupMs = B.lastPairMs = 0;
+        for (double& k : B.kernelMs) k = 0;
     } else {
         batch_enqueue(C, B, true);
-        GR_HIP(hipEventRecord(C.ev[3], C.st));
+        GR_HIP(hipEventRecord(C.ev[5], C.st));
         GR_HIP(hipStreamSynchronize(C.st));
-        float a = 0, b = 0, c = 0;
-        (void)hipEventElapsedTime(&a, C.ev[0], C.ev[1]);
-        (void)hipEventElapsedTime(&b, C.ev[1], C.ev[2]);
-        (void)hipEventElapsedTime(&c, C.ev[0], C.ev[3]);
-        B.lastSupMs = a;
-        B.lastPairMs = b;
-        B.lastMs = c;
+        float t[5] = {0, 0, 0, 0, 0};
+        for (int k = 0; k < 4; ++k) (void)hipEventElapsedTime(&t[k], C.ev[k], C.ev[k + 1]);
+        (void)hipEventElapsedTime(&t[4], C.ev[0], C.ev[5]);
+        for (int k = 0; k < 4; ++k) B.kernelMs[k] = t[k];
+        B.lastSupMs = t[0] + t[1];
+        B.lastPairMs = t[2] + t[3];
+        B.lastMs = t[4];
     }
     GR_HIP(hipGetLastError());
-    B.lastHeavy = 0;
-    for (int ph = 0; ph < 2; ++ph)
-        for (int N = 1; N <= kMaxBody; ++N) B.lastHeavy += B.hCtl[(ph * (kMaxBody + 1) + N) * 4 + 1];
+    B.lastHeavy = (int64_t)B.hCtl[1] + B.hCtl[5];
     const bool gated = !(B.flags & GPU_RULES_UNGATED);
     for (int32_t i = 0; i < B.n; ++i) {
         GpuRuleResult64& r = out[i];
@@ -460,6 +473,7 @@ bool batch_run(Context& C, Batch& B, GpuRuleResult64* out) {
     C.lastSupMs = B.lastSupMs;
     C.lastPairMs = B.lastPairMs;
     C.lastHeavy = B.lastHeavy;
+    for (int k = 0; k < 4; ++k) C.lastKernelMs[k] = B.kernelMs[k];
     return true;
 }
 
@@ -635,7 +649,11 @@ static int eval_batch(Context* C, std::vector<RuleIR>& rules, std::vector<int32_
                 continue;
             }
             double a = 0, b = 0;
+            const auto t0 = std::chrono::steady_clock::now();
             if (!eval_rule_plain(*C, rules[i], gated, out[i], a, b)) return -1;
+            const auto t1 = std::chrono::steady_clock::now();
+            out[i].latencyUs = (int32_t)std::min<int64_t>(
+                std::chrono::duration_cast<std::chrono::microseconds>(t1 - t0).count(), INT32_MAX);
             ts += a; tp += b; tot += a + b;
             heavy += C->lastHeavy;
         }
@@ -690,6 +708,37 @@ int gpu_evaluate_rules_text(void* ctx, const char* const* rules, int32_t n, int3
     }
 }
 
+void* gpu_batch_create(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags) {
+    Context* C = as_ctx(ctx);
+    if (!C || (!rules && n > 0) || n < 0) return nullptr;
+    try {
+        std::vector<RuleIR> rs(n);
+        std::vector<int32_t> st(n, kRuleOk);
+        for (int32_t i = 0; i < n; ++i) rs[i] = rule_from_words(rules[i].words);
+        auto* B = new Batch();
+        B->magic = kMagicBatch;
+        if (!batch_build(*C, *B, rs, st, flags & GPU_RULES_UNGATED)) {
+            delete B;
+            return nullptr;
+        }
+        return B;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "[gpu_rules] gpu_batch_create error: %s\n", e.what());
+        return nullptr;
+    }
+}
+
+int gpu_batch_run(void* batch, GpuRuleResult64* out) {
+    if (!batch || !out || static_cast<Handle*>(batch)->magic != kMagicBatch) return -1;
+    Batch* B = static_cast<Batch*>(batch);
+    if (!B->ctx) return -1;
+    return batch_run(*B->ctx, *B, out) ? 0 : -1;
+}
+
+void gpu_batch_free(void* batch) {
+    if (batch && static_cast<Handle*>(batch)->magic == kMagicBatch) delete static_cast<Batch*>(batch);
+}
+
 int gpu_last_batch_stats(void* ctx, double* totalMs, double* supportMs, double* pairsMs, int64_t* heavySeeds) {
     Context* C = as_ctx(ctx);
     if (!C) return -1;
@@ -698,6 +747,41 @@ int gpu_last_batch_stats(void* ctx, double* totalMs, double* supportMs, double* 
     if (pairsMs) *pairsMs = C->lastPairMs;
     if (heavySeeds) *heavySeeds = C->lastHeavy;
     return 0;
+}
+
+int gpu_last_kernel_ms(void* ctx, double* ms4) {
+    Context* C = as_ctx(ctx);
+    if (!C || !ms4) return -1;
+    for (int k = 0; k < 4; ++k) ms4[k] = C->lastKernelMs[k];
+    return 0;
+}
+
+int gpu_plan_traffic(void* handle, const GpuRuleDesc* rules, int32_t n, int64_t* out) {
+    const HostGraph* g = as_graph(handle);
+    if (!g || (!rules && n > 0) || (!out && n > 0)) return -1;
+    try {
+        Tuning t;
+        for (int32_t i = 0; i < n; ++i) {
+            CompiledRule cr = compile_rule(rule_from_words(rules[i].words), *g, t);
+            int64_t* o = out + 4 * i;
+            o[0] = o[1] = o[2] = o[3] = 0;
+            if (cr.status != kRuleOk) continue;
+            if (cr.supportLaunch) {
+                TrafficEstimate e = plan_traffic(cr.sup, cr.supRefs, *g, cr.supItems);
+                o[0] = e.bytes;
+                o[2] = e.nodes;
+            }
+            if (cr.pairsLaunch) {
+                TrafficEstimate e = plan_traffic(cr.pairs, cr.pairsRefs, *g, cr.pairItems);
+                o[1] = e.bytes;
+                o[3] = e.nodes;
+            }
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "[gpu_rules] gpu_plan_traffic error: %s\n", e.what());
+        return -1;
+    }
 }
 
 int gpu_set_tuning(void* ctx, const char* key, int64_t value) {

@@ -1,0 +1,155 @@
+// traffic.cpp -- algorithmic HBM bytes of a plan.
+//
+// Walks the device plan seed by seed on the host, in the order a thread-mode
+// lane would (same levels, same early exits, same pair-level settling), and
+// counts the bytes the search must read from the graph arrays:
+//   seed              8 B (SUPPORT: coo + col of the head entry) or 4 B (PAIRS key)
+//   row lookup        8 B per EXT/CHECK level entered
+//   CHECK probe       4 B per binary-search step (plus the final compare)
+//   EXT candidate     4 B (col)
+//   SCAN candidate    8 B (coo + col)
+// It returns no counts to callers; the counting path is the device's.
+#include <algorithm>
+#include <cstdint>
+#include <vector>
+
+#include "graph_host.hpp"
+#include "plan.hpp"
+#include "traffic.hpp"
+
+namespace gr {
+
+namespace {
+
+struct Walker {
+    const Plan& P;
+    const ConflictRefs& R;
+    const HostGraph& g;
+    int64_t T;
+    int64_t bytes = 0;
+    int64_t nodes = 0;
+    int v[kMaxSlots];
+    std::vector<int> set;   // distinct head objects of the current PAIRS seed
+
+    int col(int64_t e) const { return e < T ? g.spoVal[e] : g.posVal[e - T]; }
+    int coo(int64_t e) const { return e < T ? g.spoKey[e] : g.posKey[e - T]; }
+    int tv(int r) const { return r >= 0 ? v[r] : -(r + 1); }
+
+    bool repeats(int L, int mask, int s, int o) const {
+        for (int j = 0; j < kMaxBody; ++j)
+            if ((mask & (1 << j)) && tv(R.sref[j]) == s && tv(R.oref[j]) == o) return true;
+        return (mask & 256) && tv(R.hs) == s && tv(R.ho) == o;
+        (void)L;
+    }
+
+    void range(int L, int64_t& lo, int64_t& hi) {
+        const Level& lv = P.lev[L];
+        const int kind = level_kind(lv.flags);
+        if (kind == kScan) { lo = lv.lo; hi = lv.hi; return; }
+        const int64_t base = (int64_t)(uint32_t)lv.lo | ((int64_t)lv.hi << 32);
+        const int64_t per = (int64_t)g.P() * g.nE;
+        const int dir = (int)(base / per);
+        const int pred = (int)((base % per) / g.nE);
+        auto r = g.row(dir, pred, tv(lv.key));
+        bytes += 8;
+        lo = dir * T + r.first;
+        hi = dir * T + r.second;
+        if (kind == kCheck) {
+            const int target = tv(lv.val);
+            int64_t l = lo, h = hi;
+            while (l < h) {
+                const int64_t m = (l + h) >> 1;
+                bytes += 4;
+                if (col(m) < target) l = m + 1; else h = m;
+            }
+            if (l < hi) bytes += 4;
+            if (l < hi && col(l) == target) { lo = l; hi = l + 1; }
+            else { lo = hi = 0; }
+        }
+    }
+
+    bool bind(int L, int64_t i) {
+        const Level& lv = P.lev[L];
+        const int f = lv.flags, kind = level_kind(f), nb = level_nbound(f);
+        const int x = col(i);
+        int k = 0;
+        auto taken = [&](int y) { for (int j = 0; j < nb; ++j) if (v[j] == y) return true; return false; };
+        if (kind == kExt) {
+            bytes += 4;
+            if (taken(x)) return false;
+            v[lv.val] = x;
+        } else if (kind == kScan) {
+            bytes += 8;
+            k = coo(i);
+            if (lv.val == lv.val2) {
+                if (k != x || taken(k)) return false;
+                v[lv.val] = k;
+            } else {
+                if (k == x || taken(k) || taken(x)) return false;
+                v[lv.val] = k;
+                v[lv.val2] = x;
+            }
+        }
+        const int conf = level_conflicts(f);
+        if (conf) {
+            if (kind != kScan) k = tv(lv.key);
+            const bool spo = level_dir(f) == kSPO;
+            if (repeats(L, conf, spo ? k : x, spo ? x : k)) return false;
+        }
+        return true;
+    }
+
+    // Returns true when a complete grounding was found (existence mode).
+    bool dfs(int L, bool pairs) {
+        ++nodes;
+        int64_t lo, hi;
+        range(L, lo, hi);
+        for (int64_t i = lo; i < hi; ++i) {
+            if (!bind(L, i)) continue;
+            if (pairs && L == P.pairLevel) {
+                const int ho = tv(P.hoRef);
+                if (std::find(set.begin(), set.end(), ho) != set.end()) continue;
+                if (L + 1 == P.nlev || dfs(L + 1, false)) set.push_back(ho);
+                continue;
+            }
+            if (L + 1 == P.nlev) {
+                if (!pairs) return true;
+                continue;
+            }
+            if (dfs(L + 1, pairs) && !pairs) return true;
+        }
+        return false;
+    }
+
+    void seed(int64_t item) {
+        for (int& x : v) x = -1;
+        set.clear();
+        if (P.kind == kSupport) {
+            bytes += 8;
+            const int64_t e = P.seedBegin + item;
+            const int key = coo(e), val = col(e);
+            const int s = P.seedDir == kSPO ? key : val, o = P.seedDir == kSPO ? val : key;
+            if (P.hsRef < 0 && -(P.hsRef + 1) != s) return;
+            if (P.hoRef < 0 && -(P.hoRef + 1) != o) return;
+            if (P.hsRef >= 0 && P.hoRef >= 0 && ((P.hsRef == P.hoRef) != (s == o))) return;
+            if (P.hsRef >= 0) v[P.hsRef] = s;
+            if (P.hoRef >= 0) v[P.hoRef] = o;
+            dfs(0, false);
+            return;
+        }
+        bytes += 4;
+        const int64_t ks = P.seedBegin + item;
+        v[0] = ks < (int64_t)g.keysS.size() ? g.keysS[ks] : g.keysO[ks - g.keysS.size()];
+        dfs(0, P.pairLevel >= 0);
+    }
+};
+
+}  // namespace
+
+TrafficEstimate plan_traffic(const Plan& P, const ConflictRefs& R, const HostGraph& g, int64_t items) {
+    Walker w{P, R, g, g.T()};
+    for (int64_t i = 0; i < items; ++i) w.seed(i);
+    return TrafficEstimate{w.bytes, w.nodes};
+}
+
+}  // namespace gr

@@ -35,12 +35,12 @@ class GpuRuleResult(ctypes.Structure):
 class GpuRuleResult64(ctypes.Structure):
     _fields_ = [("support", ctypes.c_int64), ("headSize", ctypes.c_int64), ("headSupport", ctypes.c_int64),
                 ("bodySize", ctypes.c_int64), ("bodyPairs", ctypes.c_int64), ("headCoverage", ctypes.c_double),
-                ("confidence", ctypes.c_double), ("status", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("confidence", ctypes.c_double), ("status", ctypes.c_int32), ("latencyUs", ctypes.c_int32)]
 
 
 RESULT64_DTYPE = np.dtype([("support", "<i8"), ("headSize", "<i8"), ("headSupport", "<i8"), ("bodySize", "<i8"),
                            ("bodyPairs", "<i8"), ("headCoverage", "<f8"), ("confidence", "<f8"),
-                           ("status", "<i4"), ("reserved", "<i4")])
+                           ("status", "<i4"), ("latencyUs", "<i4")])
 assert RESULT64_DTYPE.itemsize == ctypes.sizeof(GpuRuleResult64)
 
 # name -> (restype, argtypes)
@@ -59,6 +59,11 @@ _SIGS = {
     "gpu_last_batch_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
                                             ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                             ctypes.POINTER(ctypes.c_int64)]),
+    "gpu_batch_create": (ctypes.c_void_p, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
+    "gpu_batch_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "gpu_batch_free": (None, [ctypes.c_void_p]),
+    "gpu_last_kernel_ms": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "gpu_plan_traffic": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]),
     "gpu_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int64]),
     "gpu_get_tuning": (ctypes.c_int64, [ctypes.c_void_p, ctypes.c_char_p]),
     "gpu_host_load_turtle": (ctypes.c_void_p, [ctypes.c_char_p]),
@@ -182,11 +187,29 @@ class Engine:
             raise RuntimeError("gpu_evaluate_rules_text returned %d" % rc)
         return out
 
+    def batch(self, rule_words: np.ndarray, flags: int = 0) -> "ResidentBatch":
+        return ResidentBatch(self, rule_words, flags)
+
     def last_stats(self):
         t, s, p = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
         h = ctypes.c_int64()
         self.lib.gpu_last_batch_stats(self.ctx, ctypes.byref(t), ctypes.byref(s), ctypes.byref(p), ctypes.byref(h))
         return {"total_ms": t.value, "support_ms": s.value, "pairs_ms": p.value, "heavy_seeds": h.value}
+
+    def kernel_ms(self):
+        """[support thread, support wave, pairs thread, pairs wave] of the last
+        eager batch (ms)."""
+        out = np.zeros(4, np.float64)
+        self.lib.gpu_last_kernel_ms(self.ctx, out.ctypes.data)
+        return out
+
+    def traffic(self, rule_words: np.ndarray) -> np.ndarray:
+        """Per rule: [support bytes, pair bytes, support nodes, pair nodes]."""
+        w = np.ascontiguousarray(rule_words, np.int32)
+        out = np.zeros((w.shape[0], 4), np.int64)
+        if self.lib.gpu_plan_traffic(self.ctx, w.ctypes.data, int(w.shape[0]), out.ctypes.data) != 0:
+            raise RuntimeError("gpu_plan_traffic failed")
+        return out
 
     def set_tuning(self, key: str, value: int):
         if self.lib.gpu_set_tuning(self.ctx, key.encode(), int(value)) != 0:
@@ -199,6 +222,36 @@ class Engine:
         if self.ctx:
             self.lib.gpu_free_graph(self.ctx)
             self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class ResidentBatch:
+    """gpu_batch_create / gpu_batch_run / gpu_batch_free: rules compiled once,
+    kept in HBM, evaluated by replaying the batch's HIP graph."""
+
+    def __init__(self, engine: Engine, rule_words: np.ndarray, flags: int = 0):
+        self.engine = engine
+        w = np.ascontiguousarray(rule_words, np.int32)
+        self.n = int(w.shape[0])
+        self.h = engine.lib.gpu_batch_create(engine.ctx, w.ctypes.data, self.n, int(flags))
+        if not self.h:
+            raise RuntimeError("gpu_batch_create failed")
+        self.out = np.zeros(self.n, RESULT64_DTYPE)
+
+    def run(self) -> np.ndarray:
+        if self.engine.lib.gpu_batch_run(self.h, self.out.ctypes.data) != 0:
+            raise RuntimeError("gpu_batch_run failed")
+        return self.out
+
+    def close(self):
+        if self.h:
+            self.engine.lib.gpu_batch_free(self.h)
+            self.h = None
 
     def __del__(self):
         try:

@@ -2,7 +2,7 @@
 // against GPU_03/RdfIndexes.cpp, RuleParser.cpp and FinalRule.cpp where they
 // lie (see oracle/ref.mk). Test infrastructure only: it prints, for a Turtle
 // file and a rule file,
-//   T <id> <term>                       every term in id order
+//   T <id> <term>                       every term in id order (\\, \\n, \\r, \\t escaped)
 //   N <terms> <predicates> <triples>    graph totals
 //   R <nbody> <hs> <hp> <ho> (<s> <p> <o>)*   each rule, terms as
 //                                       v<id> (variable) or c<id> (constant)
@@ -12,6 +12,19 @@
 
 #include "RdfIndexes.hpp"
 #include "RuleParser.hpp"
+
+// Terms may contain newlines or tabs (long literals): print them escaped.
+static std::string esc(const std::string& s) {
+    std::string o;
+    for (char c : s) {
+        if (c == '\\') o += "\\\\";
+        else if (c == '\n') o += "\\n";
+        else if (c == '\r') o += "\\r";
+        else if (c == '\t') o += "\\t";
+        else o += c;
+    }
+    return o;
+}
 
 static std::string term(const Term& t) { return (t.isVariable() ? "v" : "c") + std::to_string(t.value); }
 
@@ -40,7 +53,7 @@ int main(int argc, char** argv) {
             }
         }
     }
-    for (size_t i = 0; i < nTerms; ++i) std::cout << "T " << i << " " << idx.mapper.getValue((int)i) << "\n";
+    for (size_t i = 0; i < nTerms; ++i) std::cout << "T " << i << " " << esc(idx.mapper.getValue((int)i)) << "\n";
     std::cout << "N " << nTerms << " " << idx.predIndexes.size() << " " << triples << "\n";
     return 0;
 }

@@ -167,7 +167,70 @@ def fx_hub_small():
     make_fixture("hub_fanout_20k", w.graph, w.rules, {"fn": "hub_fanout", "kwargs": kw})
 
 
+FRONTEND_CASES = [
+    ("""@prefix ex: <https://ex.org/> .
+PREFIX rdf: <http://www.w3.org/1999/02/22-rdf-syntax-ns#>
+# comment line
+ex:a ex:knows ex:b , ex:c ;
+     a ex:Person .
+<https://ex.org/b> ex:age 42 ; ex:name "Bob"@en .
+_:n1 ex:knows [] .
+ex:c ex:label "x y"^^<http://www.w3.org/2001/XMLSchema#string> ; ex:flag true .
+ex:a ex:knows ex:b .
+""", ["( ?x ex:knows ?z ) ^ ( ?z <https://ex.org/knows> ?y ) => ( ?x ex:knows ?y ) | note",
+      "( ex:a ex:knows ?b ) => ( ?b ex:knows ex:a )",
+      "( ?a rdf:type ex:Person ) ^ ( ?a ex:knows ?b ) => ( ?b ex:age ?c )",
+      "( ?a ex:name \"Bob\"@en ) => ( ?a ex:knows ?a )",
+      "( ?a unknown:thing ?b ) => ( ?a ex:knows ?b )",
+      "( ?a ex:knows ?b ) ( ?b",
+      "=> ( ?a ex:knows ?b )"]),
+    ("""@prefix : <http://d.org/> .
+@prefix foaf: <http://xmlns.com/foaf/0.1/> .
+@base <http://base.org/> .
+:s1 foaf:knows :s2, :s3; foaf:age -3.5e2 .
+:s2 foaf:nick 'single' ; foaf:bio \"\"\"multi
+line\"\"\" .
+:s3 foaf:knows :s1 ; foaf:knows :s1.
+[ ] foaf:knows :s1 .
+:s4 foaf:mbox <mailto:x@y.z> ; foaf:title "esc\\"q\\n" ; foaf:ok false .
+""", ["( ?a foaf:knows ?b ) ^ ( ?b foaf:knows ?c ) => ( ?a foaf:knows ?c )",
+      "( :s1 foaf:knows ?x ) => ( ?x foaf:knows :s1 )",
+      "( ?a <http://xmlns.com/foaf/0.1/age> ?n ) => ( ?a foaf:nick ?n )"]),
+]
+
+
+def fx_frontend():
+    """Term ids and parsed rules from the reference's own Turtle reader and
+    rule parser (GPU_03 sources behind oracle/ref_frontend_probe.cpp)."""
+    out = []
+    with tempfile.TemporaryDirectory() as d:
+        for k, (ttl, rules) in enumerate(FRONTEND_CASES):
+            tp = os.path.join(d, "g%d.ttl" % k)
+            rp = os.path.join(d, "r%d.txt" % k)
+            open(tp, "w").write(ttl)
+            open(rp, "w").write("\n".join(rules) + "\n")
+            txt = subprocess.run([os.path.join(REF_DIR, "frontend_probe"), tp, rp], capture_output=True, text=True,
+                                 check=True).stdout
+            terms, parsed, totals = [], [], None
+            for line in txt.splitlines():
+                if line.startswith("T "):
+                    _, i, t = line.split(" ", 2)
+                    assert int(i) == len(terms)
+                    terms.append(t.encode().decode("unicode_escape") if "\\" in t else t)
+                elif line.startswith("R "):
+                    parsed.append(line[2:].split())
+                elif line.startswith("N "):
+                    totals = [int(x) for x in line.split()[1:]]
+            out.append({"ttl": ttl, "rules": rules, "terms": terms, "parsed": parsed, "totals": totals})
+    os.makedirs(os.path.join(HERE, "frontend"), exist_ok=True)
+    path = os.path.join(HERE, "frontend", "probe.json")
+    with open(path, "w") as f:
+        json.dump(out, f, indent=1)
+    print("frontend cases=%d -> %s" % (len(out), os.path.relpath(path, ROOT)))
+
+
 SETS = {
+    "frontend": fx_frontend,
     "tiny": fx_tiny,
     "paths2k": fx_small_paths,
     "c1": fx_c1,

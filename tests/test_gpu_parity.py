@@ -31,8 +31,8 @@ def say(*a):
 def engines():
     cache = {}
     yield cache
-    for e in cache.values():
-        e.close()
+    for entry in cache.values():
+        entry[0].close()
 
 
 def engine_for(cache, name):
@@ -152,20 +152,24 @@ def test_text_abi(name):
 
 
 def test_graph_replay_is_stable():
-    """The batched pass runs from a captured HIP graph; replays interleaved
-    with eager work must all agree."""
+    """A resident batch replays its captured HIP graph; every replay,
+    interleaved with eager batches and host synchronisation, must agree with
+    the eager result."""
     say("graph-replay")
     fx, g, rules = goldens.load("dbpedia_shaped_5k")
     eng = L.Engine.from_graph(g)
     try:
         words = L.rule_words_for_graph(rules, g)
-        ref = None
-        for k in range(4):
-            eng.set_tuning("use_graph", k % 2)
-            res = eng.evaluate(words, L.UNGATED)
-            if ref is None:
-                ref = res.copy()
-            assert np.array_equal(res, ref), k
-        check(fx, rules, ref, False, "replay")
+        eng.set_tuning("use_graph", 0)
+        eager = eng.evaluate(words, L.UNGATED).copy()
+        check(fx, rules, eager, False, "eager")
+        eng.set_tuning("use_graph", 1)
+        b = eng.batch(words, L.UNGATED)
+        for k in range(5):
+            res = b.run()
+            assert np.array_equal(res, eager), k
+            other = eng.evaluate(words[: 50 + 10 * k], 0)       # eager-built batch in between
+            check(fx, rules[: 50 + 10 * k], other, True, "interleaved")
+        b.close()
     finally:
         eng.close()
