@@ -108,12 +108,23 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=2000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-per-rule", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run the streaming copy yardstick over this many bytes (counter scaling) and exit")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (rules are independent; see "
               "DESIGN.md for how the work would shard). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_probe:
+        g = W.tiny_random(1)
+        eng = L.Engine.from_graph(g)
+        ms = eng.lib.gpu_copy_probe(eng.ctx, int(args.copy_probe) // 16 * 16)
+        print(json.dumps({"copy_probe_bytes": int(args.copy_probe) // 16 * 16, "ms": ms,
+                          "GBps": (2 * args.copy_probe / (ms * 1e-3) / 1e9) if ms > 0 else None}), flush=True)
+        eng.close()
+        return
 
     t0 = time.time()
     wl = W.dbpedia_shaped()

@@ -133,6 +133,11 @@ int gpu_last_kernel_ms(void* ctx, double* ms4);
  * host handle; does no device work. */
 int gpu_plan_traffic(void* handle, const GpuRuleDesc* rules, int32_t n, int64_t* out);
 
+/* Measurement yardstick: copies `bytes` (multiple of 16) between two fresh
+ * device buffers with a plain streaming kernel (k_copy_u32) and returns its
+ * device time in ms (-1 on error). Used to scale hardware byte counters. */
+double gpu_copy_probe(void* ctx, int64_t bytes);
+
 /* Tuning knobs: "budget" (thread-mode candidates per seed before a seed is
  * moved to wave mode), "thread_waves_per_cu", "wave_waves_per_cu",
  * "use_graph" (0/1). Returns 0, or -1 for an unknown key. */

@@ -113,7 +113,7 @@ struct Context : Handle {
     bool useGraph = true;
     hipStream_t st = nullptr;
     hipEvent_t ev[6] = {};
-    DevBuf rows, col, coo, keys;
+    DevBuf blocks, rowStart, col, coo, keys;
     DevGraph dg{};
     // per-rule launch scratch
     DevBuf one;       // [0] count (u64); +64: u32 control words; +128: ConflictRefs
@@ -149,7 +149,7 @@ Batch::~Batch() {
 
 Context::~Context() {
     if (st) (void)hipStreamSynchronize(st);
-    rows.release(); col.release(); coo.release(); keys.release();
+    blocks.release(); rowStart.release(); col.release(); coo.release(); keys.release();
     one.release(); heavy1.release(); dedup.release();
     if (hOne) (void)hipHostFree(hOne);
     for (auto& e : ev) if (e) (void)hipEventDestroy(e);
@@ -175,23 +175,33 @@ bool Context::upload() {
     const int32_t P = g.P();
     const int64_t nE = std::max<int64_t>(g.nE, 1);
     const int64_t T = g.T();
-    // dense rows: {begin,end} (absolute entries of the merged arrays) for
-    // every (dir, pred, term id)
-    std::vector<RowSpan> hrows((size_t)2 * P * nE, RowSpan{0, 0});
+    // key index: per (dir, pred) one block per 64 term ids; rows numbered in
+    // (dir, pred, key) order, rowStart[r] = first merged entry of row r
+    const int64_t nB = g.nBlocks();
+    std::vector<KeyBlock> hb((size_t)2 * P * nB, KeyBlock{0, 0, 0});
+    std::vector<int32_t> rs;
+    rs.reserve(g.keysS.size() + g.keysO.size() + 1);
+    uint32_t row = 0;
     for (int d = 0; d < 2; ++d) {
         const std::vector<int32_t>& key = d == 0 ? g.spoKey : g.posKey;
         for (int32_t p = 0; p < P; ++p) {
-            RowSpan* base = hrows.data() + ((size_t)d * P + p) * nE;
+            KeyBlock* base = hb.data() + ((size_t)d * P + p) * nB;
+            int64_t nextBlock = 0;   // blocks before this one already have their rank
             for (int32_t e = g.predOff[p]; e < g.predOff[p + 1]; ++e) {
-                RowSpan& r = base[key[e]];
-                const int32_t abs = (int32_t)(d * T + e);
-                if (r.end == r.begin) r.begin = abs;
-                r.end = abs + 1;
+                if (e != g.predOff[p] && key[e] == key[e - 1]) continue;
+                const int64_t blk = key[e] >> 6;
+                for (; nextBlock <= blk; ++nextBlock) base[nextBlock].rank = row;
+                base[blk].bits |= 1ull << (key[e] & 63);
+                rs.push_back((int32_t)(d * T + e));
+                ++row;
             }
+            for (; nextBlock < nB; ++nextBlock) base[nextBlock].rank = row;
         }
     }
-    if (!put(rows, hrows, st)) return false;
-    std::vector<RowSpan>().swap(hrows);
+    rs.push_back((int32_t)(2 * T));
+    if (!put(blocks, hb, st) || !put(rowStart, rs, st)) return false;
+    GR_HIP(hipStreamSynchronize(st));
+    std::vector<KeyBlock>().swap(hb);
     std::vector<int32_t> m;
     m.reserve((size_t)2 * T);
     m.insert(m.end(), g.spoVal.begin(), g.spoVal.end());
@@ -208,7 +218,8 @@ bool Context::upload() {
     m.insert(m.end(), g.keysO.begin(), g.keysO.end());
     if (!put(keys, m, st)) return false;
     GR_HIP(hipStreamSynchronize(st));
-    dg.rows = rows.as<RowSpan>();
+    dg.blocks = blocks.as<KeyBlock>();
+    dg.rowStart = rowStart.as<int32_t>();
     dg.col = col.as<int32_t>();
     dg.coo = coo.as<int32_t>();
     dg.keys = keys.as<int32_t>();
@@ -262,7 +273,7 @@ bool run_one(Context& C, const Plan& P, const ConflictRefs& R, const Plan& PW, c
     launch_fill_u64(d_count, 1, 0, C.st);
     launch_fill_u32(ctl, 4, 0, C.st);
     GR_HIP(hipEventRecord(C.ev[0], C.st));
-    const int64_t waves = (items + kWave - 1) / kWave;
+    const int64_t waves = (items + kChunkSeeds - 1) / kChunkSeeds;
     const int blocks = (int)std::min<int64_t>((waves + 3) / 4, C.threadBlocks());
     OneArgs oa{conf ? d_refs : nullptr, (uint32_t)items, C.heavy1Cap, d_count, C.heavy1.as<HeavyItem>(), ctl + 0};
     launch_thread_one(P, oa, C.dg, blocks, C.st);
@@ -338,8 +349,8 @@ bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const s
         refs[k] = R;
         wrefs[k] = RW;
         plans[k].rule = wplans[k].rule = (int32_t)i;
-        for (int64_t b = 0; b < n; b += kWave)
-            lists[ph].push_back(Chunk{(uint32_t)k, (uint32_t)b, (uint32_t)std::min<int64_t>(kWave, n - b),
+        for (int64_t b = 0; b < n; b += kChunkSeeds)
+            lists[ph].push_back(Chunk{(uint32_t)k, (uint32_t)b, (uint32_t)std::min<int64_t>(kChunkSeeds, n - b),
                                       (ph == 1 && gated) ? 1u : 0u});
         items[ph] += (uint64_t)n;
     };
@@ -513,7 +524,7 @@ Context* new_context_from(HostGraph&& g) {
     // HBM estimate, as the reference checks before uploading (GPU_03/gpu_rules_lib.cu:222-279)
     size_t freeB = 0, totalB = 0;
     if (hipMemGetInfo(&freeB, &totalB) == hipSuccess) {
-        const size_t need = (size_t)2 * C->g.P() * std::max<int64_t>(C->g.nE, 1) * 8 + (size_t)C->g.T() * 16 +
+        const size_t need = (size_t)2 * C->g.P() * C->g.nBlocks() * 16 + (size_t)C->g.T() * 16 +
                             (size_t)(C->g.keysS.size() + C->g.keysO.size()) * 4 + (size_t)2048 * (C->g.nE / 8 + 4) +
                             (64u << 20);
         if (need > freeB) {
@@ -747,6 +758,23 @@ int gpu_last_batch_stats(void* ctx, double* totalMs, double* supportMs, double* 
     if (pairsMs) *pairsMs = C->lastPairMs;
     if (heavySeeds) *heavySeeds = C->lastHeavy;
     return 0;
+}
+
+double gpu_copy_probe(void* ctx, int64_t bytes) {
+    Context* C = as_ctx(ctx);
+    if (!C || bytes < 16) return -1;
+    DevBuf a, b;
+    if (!a.ensure((size_t)bytes) || !b.ensure((size_t)bytes)) return -1;
+    if (hipMemsetAsync(a.p, 1, (size_t)bytes, C->st) != hipSuccess) return -1;
+    (void)hipEventRecord(C->ev[0], C->st);
+    launch_copy(a.p, b.p, bytes, C->st);
+    (void)hipEventRecord(C->ev[1], C->st);
+    if (hipStreamSynchronize(C->st) != hipSuccess) return -1;
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, C->ev[0], C->ev[1]);
+    a.release();
+    b.release();
+    return ms;
 }
 
 int gpu_last_kernel_ms(void* ctx, double* ms4) {

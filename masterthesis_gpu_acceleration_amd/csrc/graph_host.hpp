@@ -1,6 +1,7 @@
 // graph_host.hpp -- host-side knowledge graph: term dictionary, Turtle
 // reader and the predicate-grouped adjacency arrays uploaded to HBM.
 #pragma once
+#include <algorithm>
 #include <cstdint>
 #include <string>
 #include <unordered_map>
@@ -37,6 +38,7 @@ struct HostGraph {
     std::vector<int32_t> keysO, keysOOff;  // distinct objects per predicate
 
     int32_t P() const { return (int32_t)predTerm.size(); }
+    int64_t nBlocks() const { return (std::max<int64_t>(nE, 1) + 63) / 64; }   // key-index blocks per (dir, pred)
     int64_t T() const { return (int64_t)spoKey.size(); }
     int32_t nnz(int32_t p) const { return predOff[p + 1] - predOff[p]; }
     int32_t nkeys(int dir, int32_t p) const {

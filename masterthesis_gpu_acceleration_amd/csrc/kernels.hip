@@ -9,15 +9,17 @@
 //     cursor, end) at its own depth. Frames and the plan live in LDS, the
 //     variable slots in registers, so the control flow stays flat and the
 //     register count does not grow with the body length.
-//   * Thread mode: one lane per seed, with a work budget. A seed that uses up
-//     its budget, or (PAIRS) whose distinct head objects outgrow the lane's
-//     register set, is appended to the heavy queue and its partial work
-//     discarded.
-//   * Wave mode: one wave64 per heavy seed. Lane j starts level 0 at entry
-//     j and steps by 64; deeper levels are searched per lane. SUPPORT seeds
-//     stop as soon as __any shows a complete grounding; PAIRS seeds mark head
-//     objects in a per-wave bitmap row whose popcount is the seed's count (the
-//     same atomic that reads a word clears it).
+//   * Thread mode: one lane per seed, with a work budget; a wave works through
+//     a chunk of up to kChunkSeeds seeds of one plan and a lane takes the next
+//     seed as soon as its own is done. A seed that uses up its budget, or
+//     (PAIRS) whose distinct head objects outgrow the lane's register set, is
+//     appended to the heavy queue and its partial work discarded.
+//   * Wave mode: one wave64 per heavy seed. Lanes take level-0 entries one at
+//     a time from a shared LDS cursor and search below them on their own.
+//     SUPPORT seeds stop as soon as __any shows a complete grounding; PAIRS
+//     seeds mark head objects in a per-wave bitmap row; the words that were
+//     set are listed in LDS, so counting (popcount) and clearing the row
+//     touch only those words.
 //
 // Semantics, as the reference CPU counters define them
 // (Rewritten_CPP/SupportCounting.cpp:299-479, CPU_01_nonparallel/
@@ -38,15 +40,23 @@ constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr int kPlanWords = (int)(sizeof(Plan) / 4);
 
+constexpr int kTouchCap = 256;   // dedup-row words listed per wave-mode seed
+
 struct Frame {
     int i;     // next entry
     int end;   // one past the last entry
 };
 
-// Per-wave LDS: the plan being evaluated and one frame per (level, lane).
+// Per-wave LDS: the plan being evaluated, one frame per (level, lane), and
+// the wave-mode seed state (level-0 cursor, touched dedup-row words).
 struct WaveLds {
     Plan plan;
     Frame frame[kMaxBody][kWave];
+    int next0;
+    int end0;
+    uint32_t nTouched;
+    uint32_t pad;
+    uint32_t touched[kTouchCap];
 };
 
 static_assert(sizeof(Plan) % 4 == 0, "plan must be word-sized");
@@ -117,15 +127,21 @@ struct RegSet {
 };
 
 // Distinct head objects of one seed, one bit per term id (wave mode). The
-// load bypasses the vector L1 so it sees the other lanes' atomics.
+// load bypasses the vector L1 so it sees the other lanes' atomics. The lane
+// whose atomic finds a word still zero lists that word in LDS.
 struct BitRow {
     uint32_t* row;
+    WaveLds* W;
     __device__ __forceinline__ bool seen(int x) const {
         const uint32_t w = __hip_atomic_load(row + (x >> 5), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return (w >> (x & 31)) & 1u;
     }
     __device__ __forceinline__ bool insert(int x) {
-        atomicOr(row + (x >> 5), 1u << (x & 31));
+        const uint32_t old = atomicOr(row + (x >> 5), 1u << (x & 31));
+        if (old == 0) {
+            const uint32_t k = atomicAdd(&W->nTouched, 1u);
+            if (k < (uint32_t)kTouchCap) W->touched[k] = (uint32_t)(x >> 5);
+        }
         return true;
     }
 };
@@ -136,9 +152,14 @@ __device__ __forceinline__ Frame enter_level(const Level& lv, const DevGraph& G,
     if (kind == kScan) return Frame{lv.lo, lv.hi};
     const int keyv = term_value(v, lv.key);
     const int64_t base = (int64_t)(uint32_t)lv.lo | ((int64_t)lv.hi << 32);
-    const RowSpan rs = G.rows[base + keyv];
-    if (kind == kExt) return Frame{rs.begin, rs.end};
-    const int at = find_in_row(G.col, rs.begin, rs.end, term_value(v, lv.val));
+    const KeyBlock kb = G.blocks[base + (keyv >> 6)];
+    const uint64_t bit = 1ull << (keyv & 63);
+    if (!(kb.bits & bit)) return Frame{0, 0};
+    const uint32_t r = kb.rank + (uint32_t)__popcll(kb.bits & (bit - 1));
+    const int begin = G.rowStart[r];
+    const int end = G.rowStart[r + 1];
+    if (kind == kExt) return Frame{begin, end};
+    const int at = find_in_row(G.col, begin, end, term_value(v, lv.val));
     return at < 0 ? Frame{0, 0} : Frame{at, at + 1};
 }
 
@@ -213,67 +234,6 @@ __device__ __forceinline__ void copy_plan(WaveLds& W, const Plan* src) {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Outcome of one lane's search.
-struct Outcome {
-    uint32_t count;   // support: 0/1; pairs: distinct head objects (thread mode)
-    bool heavy;       // budget or register set exhausted
-};
-
-// The search. `active` lanes start at level 0 with frame[0] set; level-0
-// entries advance by `step0` (1 in thread mode, 64 in wave mode).
-// `waveExit`: stop the whole wave when any lane finds a grounding (wave-mode
-// existence). Sink: RegSet (thread) or BitRow (wave).
-template <class Sink>
-__device__ __forceinline__ Outcome search(WaveLds& W, const DevGraph& G, const ConflictRefs* R, int (&v)[kMaxSlots],
-                                         bool active, int step0, int budget, bool waveExit, Sink& sink) {
-    const int lane = lane_id();
-    const Plan& P = W.plan;
-    const int nlev = P.nlev;
-    const bool pairs = P.kind == kPairs && P.pairLevel >= 0;
-    const int pl = P.pairLevel;
-    int d = 0;
-    bool found = false, heavy = false;
-    while (__any(active)) {
-        if (active) {
-            const Frame fr = W.frame[d][lane];
-            if (fr.i >= fr.end) {
-                if (d == 0) active = false;
-                else --d;
-            } else {
-                W.frame[d][lane].i = fr.i + (d == 0 ? step0 : 1);
-                if (--budget < 0) {
-                    heavy = true;
-                    active = false;
-                } else {
-                    const Level lv = P.lev[d];
-                    bool ok = bind_entry(lv, G, R, v, fr.i);
-                    if (ok && pairs && d == pl) ok = !sink.seen(term_value(v, P.hoRef));
-                    if (ok) {
-                        if (d + 1 == nlev) {
-                            if (pairs) {
-                                if (!sink.insert(term_value(v, P.hoRef))) {
-                                    heavy = true;
-                                    active = false;
-                                } else {
-                                    d = pl;   // head object settled: next candidate at the pair level
-                                }
-                            } else {
-                                found = true;
-                                active = false;
-                            }
-                        } else {
-                            ++d;
-                            W.frame[d][lane] = enter_level(P.lev[d], G, v);
-                        }
-                    }
-                }
-            }
-        }
-        if (waveExit && __any(found)) break;
-    }
-    return Outcome{found ? 1u : 0u, heavy};
-}
-
 // ------------------------------------------------------------ helpers
 
 __device__ __forceinline__ uint64_t wave_sum(uint64_t x) {
@@ -303,42 +263,172 @@ __device__ __forceinline__ void push_heavy(bool heavy, uint32_t plan, uint32_t i
     }
 }
 
-// Thread mode over `count` (<= 64) seeds starting at `begin` of the plan in W.
-__device__ __forceinline__ Outcome thread_seeds(WaveLds& W, const DevGraph& G, const ConflictRefs* R, uint32_t begin,
-                                                uint32_t count, uint32_t& item) {
+// Thread mode over seeds [begin, begin + count) of the plan in W. Every lane
+// runs its own depth-first search; a lane whose seed is finished takes the
+// next one (ballot prefix count), so no lane waits for the slowest seed.
+// Returns this lane's count; escalated seeds go to the heavy queue.
+__device__ __forceinline__ uint64_t thread_chunk(WaveLds& W, const DevGraph& G, const ConflictRefs* R,
+                                                 uint32_t planIdx, uint32_t begin, uint32_t count, HeavyItem* hq,
+                                                 uint32_t* hCount, uint32_t hCap) {
     const int lane = lane_id();
     const Plan& P = W.plan;
+    const int nlev = P.nlev;
+    const bool pairs = P.kind == kPairs && P.pairLevel >= 0;
+    const int pl = P.pairLevel;
+    const uint64_t below = (1ull << lane) - 1ull;
     int v[kMaxSlots];
-    item = begin + (uint32_t)lane;
-    bool active = (uint32_t)lane < count && bind_seed(P, G, (int)item, v);
-    if (active) W.frame[0][lane] = enter_level(P.lev[0], G, v);
     RegSet set;
     set.n = 0;
-    Outcome o = search(W, G, R, v, active, 1, P.budget, false, set);
-    if (P.kind == kPairs && P.pairLevel >= 0) o.count = (uint32_t)set.n;
-    if (o.heavy) o.count = 0;
-    return o;
+    int d = 0, budget = 0;
+    bool active = false;
+    uint32_t item = 0, next = 0;
+    uint64_t mine = 0;
+    for (;;) {
+        const bool idle = !active;
+        const uint64_t want = __ballot(idle);
+        if (want && next < count) {
+            const uint32_t off = next + (uint32_t)__popcll(want & below);
+            if (idle && off < count) {
+                item = begin + off;
+                active = bind_seed(P, G, (int)item, v);
+                if (active) {
+                    W.frame[0][lane] = enter_level(P.lev[0], G, v);
+                    d = 0;
+                    budget = P.budget;
+                    set.n = 0;
+                }
+            }
+            next += (uint32_t)__popcll(want);
+        }
+        if (!__any(active)) {
+            if (next >= count) break;
+            continue;
+        }
+        bool heavyNow = false;
+        if (active) {
+            const Frame fr = W.frame[d][lane];
+            if (fr.i >= fr.end) {
+                if (d == 0) {
+                    active = false;
+                    if (pairs) mine += (uint64_t)set.n;
+                } else {
+                    --d;
+                }
+            } else {
+                W.frame[d][lane].i = fr.i + 1;
+                if (--budget < 0) {
+                    heavyNow = true;
+                    active = false;
+                } else {
+                    bool ok = bind_entry(P.lev[d], G, R, v, fr.i);
+                    if (ok && pairs && d == pl) ok = !set.seen(term_value(v, P.hoRef));
+                    if (ok) {
+                        if (d + 1 == nlev) {
+                            if (pairs) {
+                                if (!set.insert(term_value(v, P.hoRef))) {
+                                    heavyNow = true;
+                                    active = false;
+                                } else {
+                                    d = pl;   // head object settled: next candidate at the pair level
+                                }
+                            } else {
+                                mine += 1;
+                                active = false;
+                            }
+                        } else {
+                            ++d;
+                            W.frame[d][lane] = enter_level(P.lev[d], G, v);
+                        }
+                    }
+                }
+            }
+        }
+        push_heavy(heavyNow, planIdx, item, hq, hCount, hCap);
+    }
+    return mine;
 }
 
-// Wave mode: the whole wave on one seed.
+// Wave mode: the whole wave on one seed. Level-0 entries are handed out one
+// at a time from an LDS cursor.
 __device__ __forceinline__ uint64_t wave_seed(WaveLds& W, const DevGraph& G, const ConflictRefs* R, uint32_t item,
                                               uint32_t* row, int64_t rowWords) {
     const int lane = lane_id();
     const Plan& P = W.plan;
+    const int nlev = P.nlev;
+    const bool pairs = P.kind == kPairs && P.pairLevel >= 0;
+    const int pl = P.pairLevel;
     int v[kMaxSlots];
     if (!bind_seed(P, G, (int)item, v)) return 0;
-    Frame f0 = enter_level(P.lev[0], G, v);
-    f0.i += lane;
-    W.frame[0][lane] = f0;
-    const bool pairs = P.kind == kPairs && P.pairLevel >= 0;
-    BitRow br{row};
-    const Outcome o = search(W, G, R, v, f0.i < f0.end, kWave, 0x7fffffff, !pairs, br);
-    if (!pairs) return __any(o.count) ? 1 : 0;
+    const Frame f0 = enter_level(P.lev[0], G, v);
+    if (f0.i >= f0.end) return 0;
+    if (lane == 0) {
+        W.next0 = f0.i;
+        W.end0 = f0.end;
+        W.nTouched = 0;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int vs[kMaxSlots] = {v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7],
+                               v[8], v[9], v[10], v[11], v[12], v[13], v[14], v[15]};
+    BitRow br{row, &W};
+    int d = 0;
+    bool active = true, found = false;
+    // take the first level-0 entry
+    {
+        const int i0 = atomicAdd(&W.next0, 1);
+        if (i0 < f0.end) W.frame[0][lane] = Frame{i0, i0 + 1};
+        else active = false;
+    }
+    while (__any(active)) {
+        if (active) {
+            const Frame fr = W.frame[d][lane];
+            if (fr.i >= fr.end) {
+                if (d == 0) {
+                    const int i0 = atomicAdd(&W.next0, 1);
+                    if (i0 < f0.end) {
+#pragma unroll
+                        for (int j = 0; j < kMaxSlots; ++j) v[j] = vs[j];
+                        W.frame[0][lane] = Frame{i0, i0 + 1};
+                    } else {
+                        active = false;
+                    }
+                } else {
+                    --d;
+                }
+            } else {
+                W.frame[d][lane].i = fr.i + 1;
+                bool ok = bind_entry(P.lev[d], G, R, v, fr.i);
+                if (ok && pairs && d == pl) ok = !br.seen(term_value(v, P.hoRef));
+                if (ok) {
+                    if (d + 1 == nlev) {
+                        if (pairs) {
+                            br.insert(term_value(v, P.hoRef));
+                            d = pl;
+                        } else {
+                            found = true;
+                            active = false;
+                        }
+                    } else {
+                        ++d;
+                        W.frame[d][lane] = enter_level(P.lev[d], G, v);
+                    }
+                }
+            }
+        }
+        if (!pairs && __any(found)) return 1;
+    }
+    if (!pairs) return 0;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    const uint32_t nt = W.nTouched;
     uint64_t c = 0;
-    for (int64_t w = lane; w < rowWords; w += kWave) c += (uint64_t)__popc(atomicAnd(row + w, 0u));
+    if (nt <= (uint32_t)kTouchCap) {
+        for (uint32_t k = lane; k < nt; k += kWave) c += (uint64_t)__popc(atomicAnd(row + W.touched[k], 0u));
+    } else {
+        for (int64_t w = lane; w < rowWords; w += kWave) c += (uint64_t)__popc(atomicAnd(row + w, 0u));
+    }
     return wave_sum(c);
 }
 
@@ -360,10 +450,9 @@ __global__ void __launch_bounds__(kBlock) k_thread_batch(ThreadArgs a, DevGraph 
             copy_plan(W, a.plans + ch.plan);
             loaded = ch.plan;
         }
-        uint32_t item;
-        const Outcome o = thread_seeds(W, G, a.refs + ch.plan, ch.begin, ch.count, item);
-        push_heavy(o.heavy, ch.plan, item, a.heavy, a.heavyCount, a.heavyCap);
-        const uint64_t tot = wave_sum(o.count);
+        const uint64_t mine = thread_chunk(W, G, a.refs + ch.plan, ch.plan, ch.begin, ch.count, a.heavy,
+                                           a.heavyCount, a.heavyCap);
+        const uint64_t tot = wave_sum(mine);
         if (lane == 0 && tot) atomicAdd(a.counts + 2 * W.plan.rule + W.plan.kind, (unsigned long long)tot);
     }
 }
@@ -376,13 +465,11 @@ __global__ void __launch_bounds__(kBlock) k_thread_one(Plan P, OneArgs a, DevGra
     const int lane = lane_id();
     copy_plan(W, &P);
     const uint32_t waves = (gridDim.x * blockDim.x) / kWave;
-    for (uint32_t base = ((blockIdx.x * blockDim.x + threadIdx.x) / kWave) * kWave; base < a.nItems;
-         base += waves * kWave) {
-        uint32_t item;
-        const uint32_t cnt = a.nItems - base < (uint32_t)kWave ? a.nItems - base : (uint32_t)kWave;
-        const Outcome o = thread_seeds(W, G, a.refs, base, cnt, item);
-        push_heavy(o.heavy, 0u, item, a.heavy, a.heavyCount, a.heavyCap);
-        const uint64_t tot = wave_sum(o.count);
+    for (uint32_t base = ((blockIdx.x * blockDim.x + threadIdx.x) / kWave) * kChunkSeeds; base < a.nItems;
+         base += waves * kChunkSeeds) {
+        const uint32_t cnt = a.nItems - base < (uint32_t)kChunkSeeds ? a.nItems - base : (uint32_t)kChunkSeeds;
+        const uint64_t mine = thread_chunk(W, G, a.refs, 0u, base, cnt, a.heavy, a.heavyCount, a.heavyCap);
+        const uint64_t tot = wave_sum(mine);
         if (lane == 0 && tot) atomicAdd(a.count, (unsigned long long)tot);
     }
 }
@@ -427,6 +514,12 @@ __global__ void __launch_bounds__(kBlock) k_wave_one(Plan P, WaveArgs a, DevGrap
     }
 }
 
+// Plain streaming copy: the yardstick that scales hardware byte counters.
+__global__ void k_copy_u32(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n16) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n16; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 __global__ void k_fill_u64(unsigned long long* p, int64_t n, unsigned long long v) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         p[i] = v;
@@ -449,6 +542,12 @@ void launch_wave_batch(const WaveArgs& a, const DevGraph& G, int blocks, hipStre
 }
 void launch_wave_one(const Plan& P, const WaveArgs& a, const DevGraph& G, int blocks, hipStream_t s) {
     hipLaunchKernelGGL(k_wave_one, dim3(blocks), dim3(kBlock), 0, s, P, a, G);
+}
+
+void launch_copy(const void* src, void* dst, int64_t bytes, hipStream_t s) {
+    const int64_t n16 = bytes / 16;
+    hipLaunchKernelGGL(k_copy_u32, dim3(2048), dim3(256), 0, s, static_cast<const uint4*>(src),
+                       static_cast<uint4*>(dst), n16);
 }
 
 static int fill_blocks(int64_t n) {

@@ -23,6 +23,7 @@ namespace gr {
 constexpr int kMaxBody  = 8;    // body atoms per rule (GPU_03/gpu_common.cuh:25-27)
 constexpr int kMaxSlots = 16;   // distinct variables per rule (same source)
 constexpr int kWave     = 64;   // CDNA wavefront width
+constexpr int kChunkSeeds = 1024;   // seeds per thread-mode work unit
 
 enum LevelKind : int32_t {
     kCheck = 0,   // both terms bound: binary search of one row
@@ -38,7 +39,8 @@ enum PlanKind : int32_t { kSupport = 0, kPairs = 1 };
 // flags: bits 0-1 kind, bit 2 dir, bits 3-7 nbound (variable slots bound
 // before this level), bits 8-16 conflict mask (bit 8+j: the ground triple
 // may repeat level j's; bit 16: may repeat the head triple).
-// CHECK/EXT: rows index = rowBase + key value, rowBase = lo | hi << 32.
+// CHECK/EXT: key-index block of term id k = blockBase + (k >> 6), with
+// blockBase = lo | hi << 32 (the (direction, predicate) block run).
 // SCAN: entries [lo, hi) of the merged col/coo arrays.
 struct alignas(16) Level {
     int32_t flags;
@@ -81,11 +83,11 @@ struct alignas(16) ConflictRefs {
     int32_t hs, ho, pad0, pad1;
 };
 
-// One unit of thread-mode work: up to 64 consecutive seeds of one plan.
+// One unit of thread-mode work: up to kChunkSeeds consecutive seeds of one plan.
 struct alignas(16) Chunk {
     uint32_t plan;       // index into the plan table
     uint32_t begin;      // first seed (relative to the plan's seed range)
-    uint32_t count;      // seeds in this chunk (<= 64)
+    uint32_t count;      // seeds in this chunk (<= kChunkSeeds)
     uint32_t gate;       // 1: skip if the rule's support is 0 (PAIRS chunks only)
 };
 
@@ -95,23 +97,29 @@ struct alignas(8) HeavyItem {
     uint32_t item;
 };
 
-// Row extent {begin, end} of one (direction, predicate, key) in the merged
-// col/coo arrays. One 8-byte word, so a row lookup is one 64-bit load.
-struct alignas(8) RowSpan {
-    int32_t begin;
-    int32_t end;
+// Key index of one (direction, predicate): one 16-byte block per 64
+// consecutive term ids -- `bits` marks the ids that have a row, `rank` is
+// the global row number of the block's first marked id. Row r spans entries
+// [rowStart[r], rowStart[r+1]) of the merged col/coo arrays. Rows are
+// numbered in (direction, predicate, key) order, the order of `keys`, so
+// rowStart also gives every row's key list position.
+struct alignas(16) KeyBlock {
+    uint32_t rank;
+    uint32_t pad;
+    uint64_t bits;
 };
 
 // Device-resident graph. Direction d's entries of predicate p occupy
 // [d*T + predOff[p], d*T + predOff[p+1]) of col/coo; SPO entries are sorted
 // by (s, o) within a predicate (coo = s, col = o), POS by (o, s) (coo = o,
 // col = s). keys holds the distinct subjects then the distinct objects of
-// every predicate.
+// every predicate (row r's key is keys[r]).
 struct DevGraph {
-    const RowSpan* rows;   // [2][P][nE]
-    const int32_t* col;    // [2T]
-    const int32_t* coo;    // [2T]
-    const int32_t* keys;
+    const KeyBlock* blocks;   // [2][P][ceil(nE/64)]
+    const int32_t* rowStart;  // [rows + 1]
+    const int32_t* col;       // [2T]
+    const int32_t* coo;       // [2T]
+    const int32_t* keys;      // [rows]
 };
 
 }  // namespace gr

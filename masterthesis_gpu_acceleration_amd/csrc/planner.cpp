@@ -122,7 +122,7 @@ struct Builder {
                 lv.lo = g.predOff[p];
                 lv.hi = g.predOff[p + 1];
             } else {
-                const int64_t base = ((int64_t)dir * g.P() + p) * g.nE;
+                const int64_t base = ((int64_t)dir * g.P() + p) * g.nBlocks();
                 lv.lo = (int32_t)(uint32_t)(base & 0xffffffffLL);
                 lv.hi = (int32_t)(base >> 32);
             }

@@ -4,7 +4,8 @@
 // lane would (same levels, same early exits, same pair-level settling), and
 // counts the bytes the search must read from the graph arrays:
 //   seed              8 B (SUPPORT: coo + col of the head entry) or 4 B (PAIRS key)
-//   row lookup        8 B per EXT/CHECK level entered
+//   row lookup        16 B key-index block per EXT/CHECK level entered,
+//                     plus 8 B of row bounds when the key has a row
 //   CHECK probe       4 B per binary-search step (plus the final compare)
 //   EXT candidate     4 B (col)
 //   SCAN candidate    8 B (coo + col)
@@ -47,11 +48,12 @@ struct Walker {
         const int kind = level_kind(lv.flags);
         if (kind == kScan) { lo = lv.lo; hi = lv.hi; return; }
         const int64_t base = (int64_t)(uint32_t)lv.lo | ((int64_t)lv.hi << 32);
-        const int64_t per = (int64_t)g.P() * g.nE;
+        const int64_t per = (int64_t)g.P() * g.nBlocks();
         const int dir = (int)(base / per);
-        const int pred = (int)((base % per) / g.nE);
+        const int pred = (int)((base % per) / g.nBlocks());
         auto r = g.row(dir, pred, tv(lv.key));
-        bytes += 8;
+        bytes += 16;
+        if (r.second > r.first) bytes += 8;
         lo = dir * T + r.first;
         hi = dir * T + r.second;
         if (kind == kCheck) {

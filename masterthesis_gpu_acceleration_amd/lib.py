@@ -62,6 +62,7 @@ _SIGS = {
     "gpu_batch_create": (ctypes.c_void_p, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "gpu_batch_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "gpu_batch_free": (None, [ctypes.c_void_p]),
+    "gpu_copy_probe": (ctypes.c_double, [ctypes.c_void_p, ctypes.c_int64]),
     "gpu_last_kernel_ms": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "gpu_plan_traffic": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]),
     "gpu_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int64]),

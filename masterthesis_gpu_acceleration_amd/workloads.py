@@ -352,25 +352,80 @@ def _sample_guided_rule(rng, adj: _Adjacency, shape, pcdf, n_predicates, head_fr
     return ((V("a"), int(_sample_cdf(rng, pcdf, 1)[0]), V("b")), body)
 
 
+def _plant_templates(rng, n_entities, n_predicates, pcdf, n_templates, n_triples):
+    """Rule regularities planted into a graph, as mined rules reflect them:
+    each template is a closed body shape over {a,b,c,d} with fixed body and
+    head predicates; each planted grounding adds the body triples and, with
+    the template's confidence, the head triple. Returns (templates, s, p, o)."""
+    templates = []
+    for _ in range(n_templates):
+        k = int(rng.choice([1, 2, 3], p=[0.15, 0.35, 0.5]))
+        shapes = _DB_SHAPES[k]
+        shape = shapes[int(rng.integers(len(shapes)))]
+        preds = [int(x) for x in _sample_cdf(rng, pcdf, k)]
+        orient = [bool(rng.integers(2)) for _ in range(k)]
+        head = int(_sample_cdf(rng, pcdf, 1)[0])
+        conf = float(rng.uniform(0.2, 0.9))
+        templates.append((shape, preds, orient, head, conf))
+    S, P, O = [], [], []
+    per = max(1, n_triples // (n_templates * 3))
+    for shape, preds, orient, head, conf in templates:
+        for _ in range(per):
+            val = {}
+            for x in ("a", "b", "c", "d"):
+                val[x] = int(rng.integers(n_entities))
+            if len(set(val.values())) < 4:
+                continue
+            for (x, y), q, xs in zip(shape, preds, orient):
+                a, b = (val[x], val[y]) if xs else (val[y], val[x])
+                S.append(a); P.append(q); O.append(b)
+            if rng.random() < conf:
+                S.append(val["a"]); P.append(head); O.append(val["b"])
+    return templates, np.asarray(S, np.int64), np.asarray(P, np.int64), np.asarray(O, np.int64)
+
+
 def dbpedia_shaped(seed=SEED_BASE + 2, n_entities=50000, n_predicates=300, n_triples=206000,
-                   n_rules=10000, atom_mix=(164, 1240, 8596)) -> Workload:
+                   n_rules=10000, atom_mix=(164, 1240, 8596), planted_share=0.3, n_templates=400,
+                   template_rules=0.5) -> Workload:
     """Zipf(1.0) predicate frequencies, lognormal(0,1) out-degrees; closed
     rules of 1-3 body atoms over {?a,?b,?c,?d} in the DBpedia run's length mix
-    (RunBenchmark_DBpedia.ipynb: 164 x 1, 1240 x 2, 8596 x 3 atoms)."""
+    (RunBenchmark_DBpedia.ipynb: 164 x 1, 1240 x 2, 8596 x 3 atoms).
+    `planted_share` of the triples realise `n_templates` planted rule
+    regularities (mined rules reflect such regularities); `template_rules`
+    of the rules are drawn from the templates, the rest are grounded random
+    walks with a head predicate taken from the graph where one connects the
+    head variables."""
     rng = np.random.default_rng(seed)
+    pcdf = _zipf_cdf(n_predicates, 1.0)
+    n_planted = int(n_triples * planted_share)
+    templates, ps, pp, po = _plant_templates(rng, n_entities, n_predicates, pcdf, n_templates, n_planted)
     s, p, o = _graph_zipf_lognormal(rng, n_entities, n_predicates, n_triples, 1.0, 1.0, 0.0)
+    s = np.concatenate([ps, s.astype(np.int64)])
+    p = np.concatenate([pp, p.astype(np.int64)])
+    o = np.concatenate([po, o.astype(np.int64)])
+    s, p, o = _dedup_keep_order(s, p, o, n_entities, n_predicates)
+    s, p, o = s[:n_triples].astype(np.int32), p[:n_triples].astype(np.int32), o[:n_triples].astype(np.int32)
     g = Graph(s, p, o, n_entities, n_predicates, "dbpedia_shaped")
     adj = _Adjacency(g)
-    pcdf = _zipf_cdf(n_predicates, 1.0)
     tot = sum(atom_mix)
     counts = [int(round(n_rules * m / tot)) for m in atom_mix]
     counts[-1] = n_rules - sum(counts[:-1])
+    by_len = {1: [], 2: [], 3: []}
+    for t in templates:
+        by_len[len(t[0])].append(t)
     rules = []
     for k, cnt in zip((1, 2, 3), counts):
         shapes = _DB_SHAPES[k]
         for _ in range(cnt):
-            shape = shapes[int(rng.integers(len(shapes)))]
-            rules.append(_sample_guided_rule(rng, adj, shape, pcdf, n_predicates))
+            if by_len[k] and rng.random() < template_rules:
+                shape, preds, orient, head, _conf = by_len[k][int(rng.integers(len(by_len[k])))]
+                body = [_atom(V(x), q, V(y), xs) for (x, y), q, xs in zip(shape, preds, orient)]
+                if rng.random() < 0.2:   # a variant: another head predicate
+                    head = int(_sample_cdf(rng, pcdf, 1)[0])
+                rules.append(((V("a"), head, V("b")), body))
+            else:
+                shape = shapes[int(rng.integers(len(shapes)))]
+                rules.append(_sample_guided_rule(rng, adj, shape, pcdf, n_predicates))
     order = rng.permutation(len(rules))
     return Workload(g, [rules[i] for i in order], "dbpedia_shaped")
 
