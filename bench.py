@@ -108,6 +108,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=2000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-per-rule", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run the streaming copy yardstick over this many bytes (counter scaling) and exit")
     args = ap.parse_args()
@@ -138,6 +139,14 @@ def main():
     log("graph resident in HBM (%.2fs)" % (time.time() - t))
 
     flags = L.UNGATED
+    if not args.no_autotune:
+        t = time.time()
+        tl = eng.autotune(words, flags)
+        best = tl[np.argmin(tl[:, 3])]
+        log("autotune (%.1fs, %d trials): budget %d, thread waves/CU %d, wave waves/CU %d -> %.3f ms" % (
+            time.time() - t, len(tl), best[0], best[1], best[2], best[3]))
+        for r in tl:
+            log("   trial budget %6d tw %2d ww %2d  %.3f ms  escalated %d" % tuple(r))
     t = time.time()
     batch = eng.batch(words, flags)
     build_s = time.time() - t
@@ -179,7 +188,8 @@ def main():
     phase = dom // 2
     bytes_phase = int(traffic[:, phase].sum())
     achieved = bytes_phase / (kms[dom] * 1e-3) / 1e9 if kms[dom] > 0 else 0.0
-    log("kernels ms: %s" % ", ".join("%s=%.3f" % (a, b) for a, b in zip(names, kms)))
+    heavy = eng.last_stats()["heavy_seeds"]
+    log("kernels ms: %s; escalated seeds %d" % (", ".join("%s=%.3f" % (a, b) for a, b in zip(names, kms)), heavy))
 
     out = {
         "metric": METRIC,
@@ -201,6 +211,8 @@ def main():
         "body_pairs_per_s": round(pairs / (ms_step / 1000.0), 1),
         "body_pairs_total": pairs,
         "rules_with_support": int((res["support"] > 0).sum()),
+        "escalated_seeds": int(heavy),
+        "tuning": {k: eng.get_tuning(k) for k in ("budget", "thread_waves_per_cu", "wave_waves_per_cu")},
         "e2e_rules_per_s": round(n / e2e_s, 1),
         "batch_build_s": round(build_s, 4),
         "latency_ms": {"batched_p50": round(pct([x for x in dev_ms], 0.5), 4),

@@ -138,6 +138,16 @@ int gpu_plan_traffic(void* handle, const GpuRuleDesc* rules, int32_t n, int64_t*
  * device time in ms (-1 on error). Used to scale hardware byte counters. */
 double gpu_copy_probe(void* ctx, int64_t bytes);
 
+/* Autotuner: times the given rules (a sample of the expected workload)
+ * under candidate settings -- thread-mode work budget per seed (which
+ * decides how much work stays one-lane-per-seed and how much moves to
+ * wave64 mode), thread-mode and wave-mode waves per CU -- by coordinate
+ * descent, and keeps the fastest in the context. If `log` is given it
+ * receives up to logCap rows of (budget, thread waves/CU, wave waves/CU, ms,
+ * escalated seeds); *logLen gets the row count. 0 or -1. */
+int gpu_autotune(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags, double* log, int32_t logCap,
+                 int32_t* logLen);
+
 /* Tuning knobs: "budget" (thread-mode candidates per seed before a seed is
  * moved to wave mode), "thread_waves_per_cu", "wave_waves_per_cu",
  * "use_graph" (0/1). Returns 0, or -1 for an unknown key. */

@@ -273,7 +273,7 @@ bool run_one(Context& C, const Plan& P, const ConflictRefs& R, const Plan& PW, c
     launch_fill_u64(d_count, 1, 0, C.st);
     launch_fill_u32(ctl, 4, 0, C.st);
     GR_HIP(hipEventRecord(C.ev[0], C.st));
-    const int64_t waves = (items + kChunkSeeds - 1) / kChunkSeeds;
+    const int64_t waves = (items + kWave - 1) / kWave;
     const int blocks = (int)std::min<int64_t>((waves + 3) / 4, C.threadBlocks());
     OneArgs oa{conf ? d_refs : nullptr, (uint32_t)items, C.heavy1Cap, d_count, C.heavy1.as<HeavyItem>(), ctl + 0};
     launch_thread_one(P, oa, C.dg, blocks, C.st);
@@ -775,6 +775,74 @@ double gpu_copy_probe(void* ctx, int64_t bytes) {
     a.release();
     b.release();
     return ms;
+}
+
+int gpu_autotune(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags, double* log, int32_t logCap,
+                 int32_t* logLen) {
+    Context* C = as_ctx(ctx);
+    if (!C || (!rules && n > 0) || n <= 0) return -1;
+    try {
+        std::vector<RuleIR> rs(n);
+        std::vector<int32_t> st(n, kRuleOk);
+        for (int32_t i = 0; i < n; ++i) rs[i] = rule_from_words(rules[i].words);
+        const bool graphWas = C->useGraph;
+        C->useGraph = false;
+        int32_t nlog = 0;
+        // one trial: compile with the current knobs, run eagerly, keep the best of 2
+        auto trial = [&](double& ms) -> bool {
+            Batch B;
+            B.magic = kMagicBatch;
+            if (!batch_build(*C, B, rs, st, flags & GPU_RULES_UNGATED)) return false;
+            std::vector<GpuRuleResult64> out(n);
+            double best = 1e30;
+            for (int k = 0; k < 2; ++k) {
+                if (!batch_run(*C, B, out.data())) return false;
+                best = std::min(best, B.lastMs);
+            }
+            ms = best;
+            if (log && nlog < logCap) {
+                double* r = log + 5 * nlog;
+                r[0] = C->tune.budget; r[1] = C->threadWavesPerCU; r[2] = C->waveWavesPerCU; r[3] = ms;
+                r[4] = (double)B.lastHeavy;
+            }
+            ++nlog;
+            return true;
+        };
+        // coordinate descent: budget, then thread-mode waves, then wave-mode waves
+        const int32_t budgets[] = {256, 1024, 4096, 16384, 65536};
+        const int tw[] = {8, 16, 32};
+        const int ww[] = {4, 8, 16};
+        double bestMs = 1e30, ms = 0;
+        int32_t bestBudget = C->tune.budget;
+        for (int32_t b : budgets) {
+            C->tune.budget = b;
+            if (!trial(ms)) return -1;
+            if (ms < bestMs) { bestMs = ms; bestBudget = b; }
+        }
+        C->tune.budget = bestBudget;
+        int bestTw = C->threadWavesPerCU;
+        for (int t : tw) {
+            C->threadWavesPerCU = t;
+            if (!trial(ms)) return -1;
+            if (ms < bestMs) { bestMs = ms; bestTw = t; }
+        }
+        C->threadWavesPerCU = bestTw;
+        int bestWw = C->waveWavesPerCU;
+        for (int w : ww) {
+            C->waveWavesPerCU = w;
+            if (!C->ensure_dedup()) return -1;
+            if (!trial(ms)) return -1;
+            if (ms < bestMs) { bestMs = ms; bestWw = w; }
+        }
+        C->waveWavesPerCU = bestWw;
+        if (!C->ensure_dedup()) return -1;
+        C->useGraph = graphWas;
+        if (logLen) *logLen = std::min(nlog, logCap);
+        return 0;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "[gpu_rules] gpu_autotune error: %s\n", e.what());
+        return -1;
+    }
 }
 
 int gpu_last_kernel_ms(void* ctx, double* ms4) {

@@ -40,7 +40,8 @@ constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr int kPlanWords = (int)(sizeof(Plan) / 4);
 
-constexpr int kTouchCap = 256;   // dedup-row words listed per wave-mode seed
+constexpr int kTouchCap = 256;      // dedup-row words listed per wave-mode seed
+constexpr int kOneChunkSeeds = 64;  // seeds per wave in per-rule launches (one rule fills the GPU)
 
 struct Frame {
     int i;     // next entry
@@ -465,9 +466,9 @@ __global__ void __launch_bounds__(kBlock) k_thread_one(Plan P, OneArgs a, DevGra
     const int lane = lane_id();
     copy_plan(W, &P);
     const uint32_t waves = (gridDim.x * blockDim.x) / kWave;
-    for (uint32_t base = ((blockIdx.x * blockDim.x + threadIdx.x) / kWave) * kChunkSeeds; base < a.nItems;
-         base += waves * kChunkSeeds) {
-        const uint32_t cnt = a.nItems - base < (uint32_t)kChunkSeeds ? a.nItems - base : (uint32_t)kChunkSeeds;
+    for (uint32_t base = ((blockIdx.x * blockDim.x + threadIdx.x) / kWave) * kOneChunkSeeds; base < a.nItems;
+         base += waves * kOneChunkSeeds) {
+        const uint32_t cnt = a.nItems - base < (uint32_t)kOneChunkSeeds ? a.nItems - base : (uint32_t)kOneChunkSeeds;
         const uint64_t mine = thread_chunk(W, G, a.refs, 0u, base, cnt, a.heavy, a.heavyCount, a.heavyCap);
         const uint64_t tot = wave_sum(mine);
         if (lane == 0 && tot) atomicAdd(a.count, (unsigned long long)tot);

@@ -63,6 +63,8 @@ _SIGS = {
     "gpu_batch_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "gpu_batch_free": (None, [ctypes.c_void_p]),
     "gpu_copy_probe": (ctypes.c_double, [ctypes.c_void_p, ctypes.c_int64]),
+    "gpu_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p,
+                                    ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
     "gpu_last_kernel_ms": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "gpu_plan_traffic": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]),
     "gpu_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int64]),
@@ -196,6 +198,17 @@ class Engine:
         h = ctypes.c_int64()
         self.lib.gpu_last_batch_stats(self.ctx, ctypes.byref(t), ctypes.byref(s), ctypes.byref(p), ctypes.byref(h))
         return {"total_ms": t.value, "support_ms": s.value, "pairs_ms": p.value, "heavy_seeds": h.value}
+
+    def autotune(self, rule_words: np.ndarray, flags: int = 0):
+        """Runs the library's autotuner on these rules; returns its trial log
+        as rows of (budget, thread waves/CU, wave waves/CU, ms, escalated)."""
+        w = np.ascontiguousarray(rule_words, np.int32)
+        log = np.zeros((64, 5), np.float64)
+        n = ctypes.c_int32(0)
+        if self.lib.gpu_autotune(self.ctx, w.ctypes.data, int(w.shape[0]), int(flags), log.ctypes.data, 64,
+                                 ctypes.byref(n)) != 0:
+            raise RuntimeError("gpu_autotune failed")
+        return log[: n.value]
 
     def kernel_ms(self):
         """[support thread, support wave, pairs thread, pairs wave] of the last
