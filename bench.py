@@ -67,6 +67,22 @@ def cpu_model():
     return "unknown"
 
 
+WORKLOADS = {
+    # name: (generator, CPU-baseline sample size in rules)
+    "dbpedia_shaped": (lambda: W.dbpedia_shaped(), 2000),
+    "small_paths": (lambda: W.small_paths(), 10),
+    "hub_fanout": (lambda: W.hub_fanout(), 3),
+    "dense_conf": (lambda: W.dense_conf(), 8),
+}
+
+
+def describe(wl):
+    g = wl.graph
+    nb = [len(r[1]) for r in wl.rules]
+    return "%s: %d triples, %d entities, %d predicates, %d rules of %d-%d atoms" % (
+        wl.name, g.n_triples, g.n_entities, g.n_predicates, len(wl.rules), min(nb), max(nb))
+
+
 def cpu_baseline(wl, rules, sample):
     """Reference CPU_01_nonparallel (single thread) on the first `sample`
     rules; rule compute = sum of its per-rule support + confidence times, as
@@ -80,7 +96,12 @@ def cpu_baseline(wl, rules, sample):
             W.write_ttl(wl.graph, ttl)
             W.write_rules(sub, rf)
             t = time.time()
-            out = subprocess.run([exe, ttl, rf, "--json"], capture_output=True, text=True, timeout=600, check=True)
+            try:
+                out = subprocess.run([exe, ttl, rf, "--json"], capture_output=True, text=True, timeout=180,
+                                     check=True)
+            except subprocess.TimeoutExpired:
+                return {"value": None, "unit": "rules/s", "cores": 1, "kind": "reference",
+                        "sample": "first %d rules did not finish within 180 s on one core" % len(sub)}
             wall = time.time() - t
         rows = json.loads(out.stdout)
         ms = sum(r["supportMs"] + r["confidenceMs"] for r in rows)
@@ -105,7 +126,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cpu-sample", type=int, default=2000)
+    ap.add_argument("--workload", default="dbpedia_shaped", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-sample", type=int, default=0, help="rules in the CPU-baseline sample (0: per workload)")
+    ap.add_argument("--gated", action="store_true", help="body pairs only for rules with support > 0")
+    ap.add_argument("--serve", action="store_true",
+                    help="serving mode: Poisson request stream over the high-fan-out graph")
+    ap.add_argument("--requests", type=int, default=100000)
+    ap.add_argument("--rate", type=float, default=20000.0, help="requests per second (serving mode)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-per-rule", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
@@ -127,18 +154,21 @@ def main():
         eng.close()
         return
 
+    if args.serve:
+        return serve_mode(args)
+
     t0 = time.time()
-    wl = W.dbpedia_shaped()
+    gen, sample = WORKLOADS[args.workload]
+    wl = gen()
     g = wl.graph
     rules = wl.rules
-    log("workload %s: %d triples, %d entities, %d predicates, %d rules (%.1fs)" % (
-        wl.name, g.n_triples, g.n_entities, g.n_predicates, len(rules), time.time() - t0))
+    log("workload %s (%.1fs)" % (describe(wl), time.time() - t0))
     words = L.rule_words_for_graph(rules, g)
     t = time.time()
     eng = L.Engine.from_graph(g)
     log("graph resident in HBM (%.2fs)" % (time.time() - t))
 
-    flags = L.UNGATED
+    flags = 0 if args.gated else L.UNGATED
     if not args.no_autotune:
         t = time.time()
         tl = eng.autotune(words, flags)
@@ -204,10 +234,10 @@ def main():
         "vs_baseline": None,
         "dtype": "int32",
         "data": "synthetic",
-        "config": {"workload": "dbpedia_shaped: %d triples, %d entities, %d predicates, %d rules of 1-3 atoms, "
-                               "exact support + distinct body pairs for every rule, resident batch, HIP graph "
-                               "replay" % (g.n_triples, g.n_entities, g.n_predicates, n),
-                   "rules": n, "triples": g.n_triples, "seed": hex(W.SEED_BASE + 2)},
+        "config": {"workload": describe(wl) + (", exact support + distinct body pairs for %s, resident batch, "
+                                                "HIP graph replay" % ("rules with support > 0" if args.gated
+                                                                      else "every rule")),
+                   "rules": n, "triples": g.n_triples},
         "body_pairs_per_s": round(pairs / (ms_step / 1000.0), 1),
         "body_pairs_total": pairs,
         "rules_with_support": int((res["support"] > 0).sum()),
@@ -240,10 +270,54 @@ def main():
 
     if not args.no_cpu:
         t = time.time()
-        out["cpu_baseline"] = cpu_baseline(wl, rules, args.cpu_sample)
+        out["cpu_baseline"] = cpu_baseline(wl, rules, args.cpu_sample or sample)
         log("cpu baseline %.1fs: %s" % (time.time() - t, out["cpu_baseline"]["value"]))
 
     batch.close()
+    eng.close()
+    print(json.dumps(out), flush=True)
+
+
+def serve_mode(args):
+    """Serving: a Poisson stream of rule requests (1-8 atoms, predicates
+    Zipf s=1.0) against the resident high-fan-out graph, through the
+    library's continuous-batching server; reports rules/s and p50/p99 latency
+    from arrival to result."""
+    from masterthesis_gpu_acceleration_amd import serving as S
+    t0 = time.time()
+    wl = W.hub_fanout()
+    g = wl.graph
+    arrivals, reqs = W.serving_requests(graph=g, n_requests=args.requests, rate_per_s=args.rate)
+    log("serving workload: %s; %d requests at %.0f/s (%.1fs)" % (describe(wl), len(reqs), args.rate,
+                                                                 time.time() - t0))
+    words = L.rule_words_for_graph(reqs, g)
+    eng = L.Engine.from_graph(g)
+    flags = 0     # reference semantics: body size only for rules with support > 0
+    eng.evaluate(words[:256], flags)     # warm the kernels and the dedup rows
+    srv = S.Server(eng, max_batch=2048, max_in_flight=3, flags=flags)
+    try:
+        r = S.run_stream(srv, words, arrivals)
+        st = srv.stats()
+    finally:
+        srv.stop()
+    lat = r["latency_ms"]
+    comp = r["completions"]
+    out = {
+        "metric": METRIC, "value": round(r["rules_per_s"], 1), "unit": "rules/s", "n_gpus": 1,
+        "steps": 1, "warmup": 0, "ms_per_step": round(r["span_s"] * 1000.0, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+        "config": {"workload": "serving: Poisson stream of %d rule requests (1-8 atoms, predicates Zipf s=1.0) "
+                               "at %.0f/s over %s; support for every request, body pairs when support > 0"
+                               % (len(reqs), args.rate, describe(wl)),
+                   "requests": len(reqs), "rate_per_s": args.rate},
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4),
+                       "max": round(float(lat.max()), 4)},
+        "serving": st,
+        "mean_batch": round(float(comp["batchSize"].mean()), 1),
+        "rules_with_support": int((comp["result"]["support"] > 0).sum()),
+    }
+    log("serving: %.0f rules/s, p50 %.3f ms, p99 %.3f ms, %d batches" % (
+        r["rules_per_s"], out["latency_ms"]["p50"], out["latency_ms"]["p99"], st["batches"]))
     eng.close()
     print(json.dumps(out), flush=True)
 

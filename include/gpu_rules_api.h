@@ -154,6 +154,42 @@ int gpu_autotune(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags, 
 int gpu_set_tuning(void* ctx, const char* key, int64_t value);
 int64_t gpu_get_tuning(void* ctx, const char* key);
 
+/* ---- serving: continuous batching over a resident graph ----
+ *
+ * A server owns a scheduler thread and maxInFlight execution slots (stream +
+ * dedup rows each). Whenever a slot is free it takes the pending requests in
+ * arrival order (up to maxBatch), admits those that fit the HBM budget left
+ * by the batches in flight (memBudgetBytes; <= 0 means half the free HBM),
+ * and launches them as one batch; batches in different slots overlap on the
+ * device. A request that cannot fit even on an idle device completes with
+ * status 3. Results are exact per rule, independent of batching. While a
+ * server runs, its context must not be used by other calls; stop it before
+ * gpu_free_graph. */
+typedef struct {
+    uint64_t tag;              /* the caller's request id */
+    GpuRuleResult64 result;
+    int64_t submitNs;          /* gpu_monotonic_ns() clock */
+    int64_t admitNs;           /* batch launched */
+    int64_t completeNs;        /* result available to gpu_serve_poll */
+    int32_t batchSize;         /* requests in the batch it ran in */
+    int32_t pad;
+} GpuServeCompletion;
+
+void* gpu_serve_start(void* ctx, int32_t maxBatch, int32_t maxInFlight, int64_t memBudgetBytes, int32_t flags);
+/* Queues n requests; returns n, or -1 if the server is stopping. */
+int gpu_serve_submit(void* srv, const GpuRuleDesc* rules, const uint64_t* tags, int32_t n);
+/* Moves up to maxOut completions into out, waiting up to waitMs for the
+ * first; returns how many. */
+int gpu_serve_poll(void* srv, GpuServeCompletion* out, int32_t maxOut, int32_t waitMs);
+/* stats[8]: submitted, admitted, completed, batches, refused, deferred for
+ * memory, largest batch, peak batches in flight. Returns 0, 1 if a batch
+ * failed on the device, -1 on a bad handle. */
+int gpu_serve_stats(void* srv, int64_t* stats);
+/* Finishes queued and in-flight work, joins the scheduler, frees the server. */
+void gpu_serve_stop(void* srv);
+/* The monotonic clock the completion timestamps use (ns). */
+int64_t gpu_monotonic_ns(void);
+
 /* ---- host-side introspection (no device work; usable without a GPU) ---- */
 
 /* Parses a Turtle file into a host-only graph handle. */

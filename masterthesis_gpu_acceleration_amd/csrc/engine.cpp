@@ -10,158 +10,10 @@
 #include <string>
 #include <vector>
 
-#include "../../include/gpu_rules_api.h"
-#include "graph_host.hpp"
-#include "kernels.hpp"
-#include "planner.hpp"
-#include "rules_host.hpp"
+#include "engine_internal.hpp"
 #include "traffic.hpp"
 
 namespace gr {
-
-namespace {
-
-constexpr uint32_t kMagicHost = 0x47524831;  // "GRH1"
-constexpr uint32_t kMagicCtx = 0x47524331;   // "GRC1"
-constexpr uint32_t kMagicBatch = 0x47524231; // "GRB1"
-
-void log_err(const char* what, hipError_t e) {
-    std::fprintf(stderr, "[gpu_rules] %s: %s\n", what, hipGetErrorString(e));
-}
-
-#define GR_HIP(call)                                   \
-    do {                                               \
-        hipError_t e_ = (call);                        \
-        if (e_ != hipSuccess) {                        \
-            log_err(#call, e_);                        \
-            return false;                              \
-        }                                              \
-    } while (0)
-
-struct Handle {
-    uint32_t magic;
-};
-
-struct HostHandle : Handle {
-    HostGraph g;
-    std::unique_ptr<TermResolver> tr;
-};
-
-// Device buffer that only grows.
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    bool ensure(size_t need) {
-        if (need <= bytes) return true;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-        size_t b = std::max<size_t>(need, 256);
-        hipError_t e = hipMalloc(&p, b);
-        if (e != hipSuccess) {
-            log_err("hipMalloc", e);
-            p = nullptr;
-            return false;
-        }
-        bytes = b;
-        return true;
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    template <class T> T* as() const { return static_cast<T*>(p); }
-};
-
-struct Context;
-
-// A compiled, device-resident batch of rules.
-struct Batch : Handle {
-    Context* ctx = nullptr;
-    int32_t n = 0;
-    int32_t flags = 0;
-    std::vector<CompiledRule> comp;
-    std::vector<int32_t> status;
-    // chunk lists and heavy-queue segments per phase (0 support, 1 pairs)
-    uint32_t nChunks[2] = {};
-    uint32_t chunkOff[2] = {};
-    uint32_t heavyCap[2] = {};
-    uint64_t heavyOff[2] = {};
-    uint32_t capRules = 0;
-    double kernelMs[4] = {};
-    DevBuf plans, wplans, refs, wrefs, chunks, counts, heavy, ctl;
-    unsigned long long* hCounts = nullptr;   // pinned
-    uint32_t* hCtl = nullptr;                // pinned
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    const void* capturedDedup = nullptr;   // context scratch the graph was captured with
-    int capturedBlocks[2] = {0, 0};
-    double lastMs = 0, lastSupMs = 0, lastPairMs = 0;
-    int64_t lastHeavy = 0;
-    ~Batch();
-};
-
-struct Context : Handle {
-    HostGraph g;
-    std::unique_ptr<TermResolver> tr;
-    Tuning tune;
-    int device = 0;
-    int numCUs = 256;
-    int threadWavesPerCU = 16;
-    int waveWavesPerCU = 8;
-    bool useGraph = true;
-    hipStream_t st = nullptr;
-    hipEvent_t ev[6] = {};
-    DevBuf blocks, rowStart, col, coo, keys;
-    DevGraph dg{};
-    // per-rule launch scratch
-    DevBuf one;       // [0] count (u64); +64: u32 control words; +128: ConflictRefs
-    DevBuf heavy1;
-    uint32_t heavy1Cap = 0;
-    unsigned long long* hOne = nullptr;   // pinned
-    // wave-mode dedup rows
-    DevBuf dedup;
-    int64_t rowWords = 0;
-    int waveSlots = 0;
-    Batch* lastBatch = nullptr;
-    double lastMs = 0, lastSupMs = 0, lastPairMs = 0;
-    double lastKernelMs[4] = {};
-    int64_t lastHeavy = 0;
-
-    int waveBlocks() const { return std::max(1, waveSlots / 4); }
-    int threadBlocks() const { return std::max(1, numCUs * threadWavesPerCU / 4); }
-
-    bool upload();
-    bool ensure_dedup();
-    ~Context();
-};
-
-Batch::~Batch() {
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
-    plans.release(); wplans.release(); refs.release(); wrefs.release(); chunks.release(); counts.release();
-    heavy.release(); ctl.release();
-    if (hCounts) (void)hipHostFree(hCounts);
-    if (hCtl) (void)hipHostFree(hCtl);
-    if (ctx && ctx->lastBatch == this) ctx->lastBatch = nullptr;
-}
-
-Context::~Context() {
-    if (st) (void)hipStreamSynchronize(st);
-    blocks.release(); rowStart.release(); col.release(); coo.release(); keys.release();
-    one.release(); heavy1.release(); dedup.release();
-    if (hOne) (void)hipHostFree(hOne);
-    for (auto& e : ev) if (e) (void)hipEventDestroy(e);
-    if (st) (void)hipStreamDestroy(st);
-}
-
-template <class T>
-bool put(DevBuf& b, const std::vector<T>& v, hipStream_t st) {
-    if (!b.ensure(std::max<size_t>(v.size(), 1) * sizeof(T))) return false;
-    if (!v.empty()) GR_HIP(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
-    return true;
-}
 
 bool Context::upload() {
     GR_HIP(hipGetDevice(&device));
@@ -173,7 +25,6 @@ bool Context::upload() {
     GR_HIP(hipHostMalloc((void**)&hOne, 512, hipHostMallocDefault));
 
     const int32_t P = g.P();
-    const int64_t nE = std::max<int64_t>(g.nE, 1);
     const int64_t T = g.T();
     // key index: per (dir, pred) one block per 64 term ids; rows numbered in
     // (dir, pred, key) order, rowStart[r] = first merged entry of row r
@@ -329,7 +180,7 @@ bool eval_rule_plain(Context& C, const RuleIR& r, bool gated, GpuRuleResult64& r
 constexpr int kCtlWords = 8;   // per phase: thread cursor, heavy count, wave cursor, spare
 
 bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const std::vector<int32_t>& parseStatus,
-                 int32_t flags) {
+                 int32_t flags, hipStream_t st) {
     B.ctx = &C;
     B.n = (int32_t)rules.size();
     B.flags = flags;
@@ -377,15 +228,19 @@ bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const s
     }
     if (all.empty()) all.push_back(Chunk{0, 0, 0, 0});
     B.capRules = std::max<uint32_t>((uint32_t)rules.size(), 1);
-    hipStream_t st = C.st;
     if (!put(B.plans, plans, st) || !put(B.wplans, wplans, st) || !put(B.refs, refs, st) ||
         !put(B.wrefs, wrefs, st) || !put(B.chunks, all, st))
         return false;
     if (!B.counts.ensure((size_t)2 * B.capRules * 8) ||
         !B.heavy.ensure((size_t)std::max<uint64_t>(heavyTotal, 1) * sizeof(HeavyItem)) || !B.ctl.ensure(kCtlWords * 4))
         return false;
-    GR_HIP(hipHostMalloc((void**)&B.hCounts, (size_t)2 * B.capRules * 8, hipHostMallocDefault));
-    GR_HIP(hipHostMalloc((void**)&B.hCtl, kCtlWords * 4, hipHostMallocDefault));
+    if (B.hCountsCap < (size_t)2 * B.capRules) {
+        if (B.hCounts) (void)hipHostFree(B.hCounts);
+        B.hCounts = nullptr;
+        GR_HIP(hipHostMalloc((void**)&B.hCounts, (size_t)2 * B.capRules * 8, hipHostMallocDefault));
+        B.hCountsCap = (size_t)2 * B.capRules;
+    }
+    if (!B.hCtl) GR_HIP(hipHostMalloc((void**)&B.hCtl, kCtlWords * 4, hipHostMallocDefault));
     GR_HIP(hipStreamSynchronize(st));
     return true;
 }
@@ -394,13 +249,13 @@ bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const s
 // persistent thread-mode kernel over all chunks, then one wave-mode kernel
 // over the seeds it escalated. PAIRS chunks of rules whose support is 0 are
 // skipped on the device when the batch is gated.
-void batch_enqueue(Context& C, Batch& B, bool timed) {
-    hipStream_t st = C.st;
+void batch_enqueue(Context& C, Batch& B, const ExecTarget& t) {
+    hipStream_t st = t.st;
     uint32_t* ctl = B.ctl.as<uint32_t>();
     unsigned long long* counts = B.counts.as<unsigned long long>();
     launch_fill_u64(counts, 2 * (int64_t)B.capRules, 0, st);
     launch_fill_u32(ctl, kCtlWords, 0, st);
-    if (timed) (void)hipEventRecord(C.ev[0], st);
+    if (t.ev) (void)hipEventRecord(t.ev[0], st);
     for (int ph = 0; ph < 2; ++ph) {
         const uint32_t nc = B.nChunks[ph];
         uint32_t* c = ctl + ph * 4;
@@ -410,16 +265,35 @@ void batch_enqueue(Context& C, Batch& B, bool timed) {
             const int blocks = std::min<int>(C.threadBlocks(), (int)((nc + 3) / 4));
             launch_thread_batch(ta, C.dg, std::max(blocks, 1), st);
         }
-        if (timed) (void)hipEventRecord(C.ev[1 + 2 * ph], st);
+        if (t.ev) (void)hipEventRecord(t.ev[1 + 2 * ph], st);
         if (nc) {
             WaveArgs wa{B.wplans.as<Plan>(), B.wrefs.as<ConflictRefs>(), B.heavy.as<HeavyItem>() + B.heavyOff[ph],
-                        c + 1, B.heavyCap[ph], 0, c + 2, counts, C.dedup.as<uint32_t>(), C.rowWords};
-            launch_wave_batch(wa, C.dg, C.waveBlocks(), st);
+                        c + 1, B.heavyCap[ph], 0, c + 2, counts, t.dedup, t.rowWords};
+            launch_wave_batch(wa, C.dg, t.waveBlocks, st);
         }
-        if (timed) (void)hipEventRecord(C.ev[2 + 2 * ph], st);
+        if (t.ev) (void)hipEventRecord(t.ev[2 + 2 * ph], st);
     }
     (void)hipMemcpyAsync(B.hCounts, counts, (size_t)2 * B.capRules * 8, hipMemcpyDeviceToHost, st);
     (void)hipMemcpyAsync(B.hCtl, ctl, kCtlWords * 4, hipMemcpyDeviceToHost, st);
+}
+
+void batch_collect(Batch& B, GpuRuleResult64* out) {
+    B.lastHeavy = (int64_t)B.hCtl[1] + B.hCtl[5];
+    const bool gated = !(B.flags & GPU_RULES_UNGATED);
+    for (int32_t i = 0; i < B.n; ++i) {
+        GpuRuleResult64& r = out[i];
+        std::memset(&r, 0, sizeof(r));
+        const CompiledRule& cr = B.comp[i];
+        r.status = B.status[i];
+        r.headSize = cr.headSize;
+        r.headSupport = cr.headSupport;
+        r.bodyPairs = -1;
+        if (r.status != kRuleOk) continue;
+        r.support = cr.supportLaunch ? (int64_t)B.hCounts[2 * i] : 0;
+        if (!gated || r.support > 0) r.bodyPairs = cr.pairsLaunch ? (int64_t)B.hCounts[2 * i + 1] : 0;
+        r.bodySize = r.support > 0 && r.bodyPairs > 0 ? r.bodyPairs : 0;
+        finish_measures(r);
+    }
 }
 
 bool batch_run(Context& C, Batch& B, GpuRuleResult64* out) {
@@ -438,7 +312,7 @@ bool batch_run(Context& C, Batch& B, GpuRuleResult64* out) {
             B.capturedBlocks[0] = C.threadBlocks();
             B.capturedBlocks[1] = C.waveBlocks();
             GR_HIP(hipStreamBeginCapture(C.st, hipStreamCaptureModeThreadLocal));
-            batch_enqueue(C, B, false);
+            batch_enqueue(C, B, ExecTarget{C.st, C.dedup.as<uint32_t>(), C.rowWords, C.waveBlocks(), nullptr});
             GR_HIP(hipStreamEndCapture(C.st, &B.graph));
             GR_HIP(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
         }
@@ -452,7 +326,7 @@ bool batch_run(Context& C, Batch& B, GpuRuleResult64* out) {
         B.lastSupMs = B.lastPairMs = 0;
         for (double& k : B.kernelMs) k = 0;
     } else {
-        batch_enqueue(C, B, true);
+        batch_enqueue(C, B, ExecTarget{C.st, C.dedup.as<uint32_t>(), C.rowWords, C.waveBlocks(), C.ev});
         GR_HIP(hipEventRecord(C.ev[5], C.st));
         GR_HIP(hipStreamSynchronize(C.st));
         float t[5] = {0, 0, 0, 0, 0};
@@ -464,22 +338,7 @@ bool batch_run(Context& C, Batch& B, GpuRuleResult64* out) {
         B.lastMs = t[4];
     }
     GR_HIP(hipGetLastError());
-    B.lastHeavy = (int64_t)B.hCtl[1] + B.hCtl[5];
-    const bool gated = !(B.flags & GPU_RULES_UNGATED);
-    for (int32_t i = 0; i < B.n; ++i) {
-        GpuRuleResult64& r = out[i];
-        std::memset(&r, 0, sizeof(r));
-        const CompiledRule& cr = B.comp[i];
-        r.status = B.status[i];
-        r.headSize = cr.headSize;
-        r.headSupport = cr.headSupport;
-        r.bodyPairs = -1;
-        if (r.status != kRuleOk) continue;
-        r.support = cr.supportLaunch ? (int64_t)B.hCounts[2 * i] : 0;
-        if (!gated || r.support > 0) r.bodyPairs = cr.pairsLaunch ? (int64_t)B.hCounts[2 * i + 1] : 0;
-        r.bodySize = r.support > 0 && r.bodyPairs > 0 ? r.bodyPairs : 0;
-        finish_measures(r);
-    }
+    batch_collect(B, out);
     C.lastMs = B.lastMs;
     C.lastSupMs = B.lastSupMs;
     C.lastPairMs = B.lastPairMs;
@@ -541,7 +400,6 @@ Context* new_context_from(HostGraph&& g) {
     return C;
 }
 
-}  // namespace
 }  // namespace gr
 
 using namespace gr;
@@ -675,7 +533,7 @@ static int eval_batch(Context* C, std::vector<RuleIR>& rules, std::vector<int32_
     C->lastBatch = nullptr;
     auto* B = new Batch();
     B->magic = kMagicBatch;
-    if (!batch_build(*C, *B, rules, st, flags) || !batch_run(*C, *B, out)) {
+    if (!batch_build(*C, *B, rules, st, flags, C->st) || !batch_run(*C, *B, out)) {
         delete B;
         return -1;
     }
@@ -728,7 +586,7 @@ void* gpu_batch_create(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t f
         for (int32_t i = 0; i < n; ++i) rs[i] = rule_from_words(rules[i].words);
         auto* B = new Batch();
         B->magic = kMagicBatch;
-        if (!batch_build(*C, *B, rs, st, flags & GPU_RULES_UNGATED)) {
+        if (!batch_build(*C, *B, rs, st, flags & GPU_RULES_UNGATED, C->st)) {
             delete B;
             return nullptr;
         }
@@ -792,7 +650,7 @@ int gpu_autotune(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags, 
         auto trial = [&](double& ms) -> bool {
             Batch B;
             B.magic = kMagicBatch;
-            if (!batch_build(*C, B, rs, st, flags & GPU_RULES_UNGATED)) return false;
+            if (!batch_build(*C, B, rs, st, flags & GPU_RULES_UNGATED, C->st)) return false;
             std::vector<GpuRuleResult64> out(n);
             double best = 1e30;
             for (int k = 0; k < 2; ++k) {

@@ -1,0 +1,184 @@
+// engine_internal.hpp -- library-internal types shared by the C ABI
+// (engine.cpp) and the serving loop (serve.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstdio>
+#include <memory>
+#include <vector>
+
+#include "../../include/gpu_rules_api.h"
+#include "graph_host.hpp"
+#include "kernels.hpp"
+#include "planner.hpp"
+#include "rules_host.hpp"
+
+namespace gr {
+
+
+constexpr uint32_t kMagicHost = 0x47524831;  // "GRH1"
+constexpr uint32_t kMagicCtx = 0x47524331;   // "GRC1"
+constexpr uint32_t kMagicBatch = 0x47524231; // "GRB1"
+
+inline void log_err(const char* what, hipError_t e) {
+    std::fprintf(stderr, "[gpu_rules] %s: %s\n", what, hipGetErrorString(e));
+}
+
+#define GR_HIP(call)                                   \
+    do {                                               \
+        hipError_t e_ = (call);                        \
+        if (e_ != hipSuccess) {                        \
+            log_err(#call, e_);                        \
+            return false;                              \
+        }                                              \
+    } while (0)
+
+struct Handle {
+    uint32_t magic;
+};
+
+struct HostHandle : Handle {
+    HostGraph g;
+    std::unique_ptr<TermResolver> tr;
+};
+
+// Device buffer that only grows.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    bool ensure(size_t need) {
+        if (need <= bytes) return true;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        size_t b = std::max<size_t>(need, 256);
+        hipError_t e = hipMalloc(&p, b);
+        if (e != hipSuccess) {
+            log_err("hipMalloc", e);
+            p = nullptr;
+            return false;
+        }
+        bytes = b;
+        return true;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+struct Context;
+
+// A compiled, device-resident batch of rules.
+struct Batch : Handle {
+    Context* ctx = nullptr;
+    int32_t n = 0;
+    int32_t flags = 0;
+    std::vector<CompiledRule> comp;
+    std::vector<int32_t> status;
+    // chunk lists and heavy-queue segments per phase (0 support, 1 pairs)
+    uint32_t nChunks[2] = {};
+    uint32_t chunkOff[2] = {};
+    uint32_t heavyCap[2] = {};
+    uint64_t heavyOff[2] = {};
+    uint32_t capRules = 0;
+    double kernelMs[4] = {};
+    DevBuf plans, wplans, refs, wrefs, chunks, counts, heavy, ctl;
+    unsigned long long* hCounts = nullptr;   // pinned
+    size_t hCountsCap = 0;                   // words
+    uint32_t* hCtl = nullptr;                // pinned
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const void* capturedDedup = nullptr;   // context scratch the graph was captured with
+    int capturedBlocks[2] = {0, 0};
+    double lastMs = 0, lastSupMs = 0, lastPairMs = 0;
+    int64_t lastHeavy = 0;
+    ~Batch();
+};
+
+struct Context : Handle {
+    HostGraph g;
+    std::unique_ptr<TermResolver> tr;
+    Tuning tune;
+    int device = 0;
+    int numCUs = 256;
+    int threadWavesPerCU = 16;
+    int waveWavesPerCU = 8;
+    bool useGraph = true;
+    hipStream_t st = nullptr;
+    hipEvent_t ev[6] = {};
+    DevBuf blocks, rowStart, col, coo, keys;
+    DevGraph dg{};
+    // per-rule launch scratch
+    DevBuf one;       // [0] count (u64); +64: u32 control words; +128: ConflictRefs
+    DevBuf heavy1;
+    uint32_t heavy1Cap = 0;
+    unsigned long long* hOne = nullptr;   // pinned
+    // wave-mode dedup rows
+    DevBuf dedup;
+    int64_t rowWords = 0;
+    int waveSlots = 0;
+    Batch* lastBatch = nullptr;
+    double lastMs = 0, lastSupMs = 0, lastPairMs = 0;
+    double lastKernelMs[4] = {};
+    int64_t lastHeavy = 0;
+
+    int waveBlocks() const { return std::max(1, waveSlots / 4); }
+    int threadBlocks() const { return std::max(1, numCUs * threadWavesPerCU / 4); }
+
+    bool upload();
+    bool ensure_dedup();
+    ~Context();
+};
+
+inline Batch::~Batch() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    plans.release(); wplans.release(); refs.release(); wrefs.release(); chunks.release(); counts.release();
+    heavy.release(); ctl.release();
+    if (hCounts) (void)hipHostFree(hCounts);
+    if (hCtl) (void)hipHostFree(hCtl);
+    if (ctx && ctx->lastBatch == this) ctx->lastBatch = nullptr;
+}
+
+inline Context::~Context() {
+    if (st) (void)hipStreamSynchronize(st);
+    blocks.release(); rowStart.release(); col.release(); coo.release(); keys.release();
+    one.release(); heavy1.release(); dedup.release();
+    if (hOne) (void)hipHostFree(hOne);
+    for (auto& e : ev) if (e) (void)hipEventDestroy(e);
+    if (st) (void)hipStreamDestroy(st);
+}
+
+template <class T>
+inline bool put(DevBuf& b, const std::vector<T>& v, hipStream_t st) {
+    if (!b.ensure(std::max<size_t>(v.size(), 1) * sizeof(T))) return false;
+    if (!v.empty()) GR_HIP(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+    return true;
+}
+
+// Where a batch's device pass runs: stream, wave-mode dedup rows, and
+// (optionally) six timing events.
+struct ExecTarget {
+    hipStream_t st;
+    uint32_t* dedup;
+    int64_t rowWords;
+    int waveBlocks;
+    hipEvent_t* ev;   // nullptr: untimed
+};
+
+// Compiles rules into B and uploads plans and chunk lists on stream st.
+bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const std::vector<int32_t>& status,
+                 int32_t flags, hipStream_t st);
+// Enqueues B's device pass (counter fills, kernels, result copies) on t.st.
+void batch_enqueue(Context& C, Batch& B, const ExecTarget& t);
+// Fills out[B.n] from B's host-side counters after its pass completed.
+void batch_collect(Batch& B, GpuRuleResult64* out);
+// Builds-if-needed, runs (graph replay or eager) and collects on the context's stream.
+bool batch_run(Context& C, Batch& B, GpuRuleResult64* out);
+
+}  // namespace gr

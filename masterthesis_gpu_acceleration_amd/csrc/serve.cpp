@@ -1,0 +1,348 @@
+// serve.cpp -- continuous batching of rule requests over a resident graph.
+//
+// A scheduler thread owns all device work of a server. Requests are queued
+// by gpu_serve_submit from any thread. Whenever an execution slot is free,
+// the scheduler takes the pending requests (up to maxBatch, in arrival
+// order), admits as many as fit the HBM budget left by the batches in flight
+// (seed queues, plans, chunk lists, counters), compiles them into one batch
+// and launches its device pass on the slot's own stream with the slot's own
+// dedup rows -- so batches from different slots overlap on the device while
+// new requests keep arriving. Finished slots are found by event query; their
+// results go to a completion queue read by gpu_serve_poll.
+//
+// Counts are exact integers computed per rule, so a rule's result does not
+// depend on which batch or slot it ran in.
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <thread>
+
+#include "engine_internal.hpp"
+
+namespace gr {
+
+namespace {
+
+constexpr uint32_t kMagicServer = 0x47525331;   // "GRS1"
+
+int64_t mono_ns() {
+    return std::chrono::duration_cast<std::chrono::nanoseconds>(
+               std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+struct Request {
+    uint64_t tag;
+    GpuRuleDesc rule;
+    int64_t submitNs;
+};
+
+struct Slot {
+    hipStream_t st = nullptr;
+    hipEvent_t done = nullptr;
+    DevBuf dedup;
+    int64_t rowWords = 0;
+    int waveSlots = 0;
+    Batch batch;
+    std::vector<Request> reqs;
+    std::vector<GpuRuleResult64> out;
+    int64_t reserved = 0;
+    int64_t admitNs = 0;
+    bool busy = false;
+    ~Slot() {
+        if (st) (void)hipStreamSynchronize(st);
+        dedup.release();
+        if (done) (void)hipEventDestroy(done);
+        if (st) (void)hipStreamDestroy(st);
+    }
+};
+
+// Bytes a request needs on the device while in flight: seed queue entries
+// for both phases, two plans and two conflict tables per phase, chunk
+// descriptors, counters.
+int64_t request_bytes(const CompiledRule& cr) {
+    int64_t items = 0, chunks = 0;
+    if (cr.supportLaunch) {
+        items += cr.supItems;
+        chunks += (cr.supItems + kChunkSeeds - 1) / kChunkSeeds;
+    }
+    if (cr.pairsLaunch) {
+        items += cr.pairItems;
+        chunks += (cr.pairItems + kChunkSeeds - 1) / kChunkSeeds;
+    }
+    return items * (int64_t)sizeof(HeavyItem) + chunks * (int64_t)sizeof(Chunk) +
+           4 * (int64_t)(sizeof(Plan) + sizeof(ConflictRefs)) + 16;
+}
+
+struct Server : Handle {
+    Context* C = nullptr;
+    int32_t maxBatch = 1024;
+    int32_t flags = 0;
+    int64_t budget = 0;      // bytes for in-flight batches
+    int64_t reservedNow = 0;
+    std::vector<std::unique_ptr<Slot>> slots;
+    std::mutex mu;
+    std::condition_variable cv;      // new requests / stop
+    std::condition_variable doneCv;  // new completions
+    std::deque<Request> pending;
+    std::deque<GpuServeCompletion> done;
+    std::thread th;
+    std::atomic<bool> stopping{false};
+    std::atomic<bool> failed{false};
+    // statistics
+    std::atomic<int64_t> nSubmitted{0}, nAdmitted{0}, nCompleted{0}, nBatches{0}, nRefused{0}, nDeferred{0};
+    std::atomic<int64_t> maxBatchSeen{0}, inFlightPeak{0};
+
+    bool init(int32_t maxInFlight, int64_t memBudget) {
+        if (memBudget <= 0) {
+            size_t f = 0, t = 0;
+            if (hipMemGetInfo(&f, &t) != hipSuccess) return false;
+            memBudget = (int64_t)(f / 2);
+        }
+        budget = memBudget;
+        for (int i = 0; i < maxInFlight; ++i) {
+            auto S = std::make_unique<Slot>();
+            GR_HIP(hipStreamCreateWithFlags(&S->st, hipStreamNonBlocking));
+            GR_HIP(hipEventCreateWithFlags(&S->done, hipEventDisableTiming));
+            S->rowWords = C->rowWords;
+            S->waveSlots = C->waveSlots;
+            const size_t bytes = (size_t)S->waveSlots * S->rowWords * 4;
+            if (!S->dedup.ensure(bytes)) return false;
+            GR_HIP(hipMemset(S->dedup.p, 0, bytes));
+            S->batch.magic = kMagicBatch;
+            S->batch.ctx = C;
+            budget -= (int64_t)bytes;
+            slots.push_back(std::move(S));
+        }
+        if (budget <= 0) {
+            std::fprintf(stderr, "[gpu_rules] serve: HBM budget smaller than the slots' dedup rows\n");
+            return false;
+        }
+        th = std::thread([this] { loop(); });
+        return true;
+    }
+
+    void complete(Request& r, const GpuRuleResult64& res, int64_t admitNs, int32_t batchSize) {
+        GpuServeCompletion c;
+        std::memset(&c, 0, sizeof(c));
+        c.tag = r.tag;
+        c.result = res;
+        c.submitNs = r.submitNs;
+        c.admitNs = admitNs;
+        c.completeNs = mono_ns();
+        c.batchSize = batchSize;
+        std::lock_guard<std::mutex> g(mu);
+        done.push_back(c);
+        nCompleted++;
+    }
+
+    // Finished slots -> completion queue.
+    int reap() {
+        int busy = 0;
+        for (auto& S : slots) {
+            if (!S->busy) continue;
+            hipError_t q = hipEventQuery(S->done);
+            if (q == hipErrorNotReady) { ++busy; continue; }
+            if (q != hipSuccess) {
+                log_err("serve: batch failed", q);
+                failed = true;
+            }
+            S->out.resize(S->reqs.size());
+            batch_collect(S->batch, S->out.data());
+            for (size_t i = 0; i < S->reqs.size(); ++i) {
+                if (q != hipSuccess) S->out[i].status = kRuleError;
+                complete(S->reqs[i], S->out[i], S->admitNs, (int32_t)S->reqs.size());
+            }
+            S->reqs.clear();
+            reservedNow -= S->reserved;
+            S->reserved = 0;
+            S->busy = false;
+            doneCv.notify_all();
+        }
+        return busy;
+    }
+
+    // Pending requests -> a batch on a free slot. Returns true if launched.
+    bool admit() {
+        Slot* S = nullptr;
+        for (auto& s : slots) if (!s->busy) { S = s.get(); break; }
+        if (!S) return false;
+        std::vector<Request> take;
+        {
+            std::lock_guard<std::mutex> g(mu);
+            while (!pending.empty() && (int32_t)take.size() < maxBatch) {
+                take.push_back(pending.front());
+                pending.pop_front();
+            }
+        }
+        if (take.empty()) return false;
+        std::vector<RuleIR> rules;
+        std::vector<int32_t> status;
+        std::vector<Request> admitted, back;
+        int64_t need = 0;
+        const bool idle = reservedNow == 0;
+        for (auto& r : take) {
+            if (!back.empty()) { back.push_back(r); continue; }
+            RuleIR ir = rule_from_words(r.rule.words);
+            const CompiledRule cr = compile_rule(ir, C->g, C->tune);
+            const int64_t b = request_bytes(cr);
+            if (reservedNow + need + b > budget) {
+                if (idle && admitted.empty()) {   // can never fit: refuse it
+                    GpuRuleResult64 res;
+                    std::memset(&res, 0, sizeof(res));
+                    res.status = 3;
+                    res.bodyPairs = -1;
+                    complete(r, res, mono_ns(), 0);
+                    nRefused++;
+                    continue;
+                }
+                back.push_back(r);   // wait for memory to free up
+                continue;
+            }
+            need += b;
+            rules.push_back(std::move(ir));
+            status.push_back(kRuleOk);
+            admitted.push_back(r);
+        }
+        if (!back.empty()) {
+            nDeferred += (int64_t)back.size();
+            std::lock_guard<std::mutex> g(mu);
+            for (auto it = back.rbegin(); it != back.rend(); ++it) pending.push_front(*it);
+        }
+        if (admitted.empty()) return false;
+        if (!batch_build(*C, S->batch, rules, status, flags, S->st)) {
+            failed = true;
+            for (auto& r : admitted) {
+                GpuRuleResult64 res;
+                std::memset(&res, 0, sizeof(res));
+                res.status = kRuleError;
+                complete(r, res, mono_ns(), 0);
+            }
+            return false;
+        }
+        batch_enqueue(*C, S->batch, ExecTarget{S->st, S->dedup.as<uint32_t>(), S->rowWords,
+                                                std::max(1, S->waveSlots / 4), nullptr});
+        (void)hipEventRecord(S->done, S->st);
+        S->reqs = std::move(admitted);
+        S->reserved = need;
+        S->admitNs = mono_ns();
+        S->busy = true;
+        reservedNow += need;
+        nAdmitted += (int64_t)S->reqs.size();
+        nBatches++;
+        if ((int64_t)S->reqs.size() > maxBatchSeen) maxBatchSeen = (int64_t)S->reqs.size();
+        int64_t inflight = 0;
+        for (auto& s : slots) inflight += s->busy ? 1 : 0;
+        if (inflight > inFlightPeak) inFlightPeak = inflight;
+        return true;
+    }
+
+    void loop() {
+        for (;;) {
+            const int busy = reap();
+            bool launched = false;
+            while (admit()) launched = true;
+            bool havePending;
+            {
+                std::lock_guard<std::mutex> g(mu);
+                havePending = !pending.empty();
+            }
+            if (stopping && !havePending && busy == 0) {
+                bool anyBusy = false;
+                for (auto& s : slots) anyBusy |= s->busy;
+                if (!anyBusy) break;
+            }
+            if (launched) continue;
+            if (busy > 0 || havePending) {
+                std::this_thread::sleep_for(std::chrono::microseconds(20));
+            } else {
+                std::unique_lock<std::mutex> lk(mu);
+                cv.wait_for(lk, std::chrono::milliseconds(2), [this] { return !pending.empty() || stopping.load(); });
+            }
+        }
+    }
+};
+
+Server* as_server(void* h) {
+    if (!h || static_cast<Handle*>(h)->magic != kMagicServer) return nullptr;
+    return static_cast<Server*>(h);
+}
+
+}  // namespace
+}  // namespace gr
+
+using namespace gr;
+
+extern "C" {
+
+int64_t gpu_monotonic_ns(void) { return mono_ns(); }
+
+void* gpu_serve_start(void* ctx, int32_t maxBatch, int32_t maxInFlight, int64_t memBudgetBytes, int32_t flags) {
+    if (!ctx || static_cast<Handle*>(ctx)->magic != kMagicCtx) return nullptr;
+    auto* S = new Server();
+    S->magic = kMagicServer;
+    S->C = static_cast<Context*>(ctx);
+    S->maxBatch = maxBatch > 0 ? maxBatch : 1024;
+    S->flags = flags & GPU_RULES_UNGATED;
+    if (!S->init(maxInFlight > 0 ? maxInFlight : 3, memBudgetBytes)) {
+        delete S;
+        return nullptr;
+    }
+    return S;
+}
+
+int gpu_serve_submit(void* srv, const GpuRuleDesc* rules, const uint64_t* tags, int32_t n) {
+    Server* S = as_server(srv);
+    if (!S || (!rules && n > 0) || (!tags && n > 0) || n < 0) return -1;
+    if (S->stopping) return -1;
+    const int64_t now = mono_ns();
+    {
+        std::lock_guard<std::mutex> g(S->mu);
+        for (int32_t i = 0; i < n; ++i) S->pending.push_back(Request{tags[i], rules[i], now});
+    }
+    S->nSubmitted += n;
+    S->cv.notify_one();
+    return n;
+}
+
+int gpu_serve_poll(void* srv, GpuServeCompletion* out, int32_t maxOut, int32_t waitMs) {
+    Server* S = as_server(srv);
+    if (!S || (!out && maxOut > 0)) return -1;
+    std::unique_lock<std::mutex> lk(S->mu);
+    if (S->done.empty() && waitMs > 0)
+        S->doneCv.wait_for(lk, std::chrono::milliseconds(waitMs), [S] { return !S->done.empty(); });
+    int32_t k = 0;
+    while (k < maxOut && !S->done.empty()) {
+        out[k++] = S->done.front();
+        S->done.pop_front();
+    }
+    return k;
+}
+
+int gpu_serve_stats(void* srv, int64_t* stats) {
+    Server* S = as_server(srv);
+    if (!S || !stats) return -1;
+    stats[0] = S->nSubmitted;
+    stats[1] = S->nAdmitted;
+    stats[2] = S->nCompleted;
+    stats[3] = S->nBatches;
+    stats[4] = S->nRefused;
+    stats[5] = S->nDeferred;
+    stats[6] = S->maxBatchSeen;
+    stats[7] = S->inFlightPeak;
+    return S->failed ? 1 : 0;
+}
+
+void gpu_serve_stop(void* srv) {
+    Server* S = as_server(srv);
+    if (!S) return;
+    S->stopping = true;
+    S->cv.notify_all();
+    if (S->th.joinable()) S->th.join();
+    delete S;
+}
+
+}  // extern "C"

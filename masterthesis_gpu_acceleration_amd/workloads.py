@@ -433,19 +433,19 @@ def dbpedia_shaped(seed=SEED_BASE + 2, n_entities=50000, n_predicates=300, n_tri
 # ---------------------------------------------------------------- hub_fanout
 
 def hub_fanout(seed=SEED_BASE + 3, n_entities=500000, n_predicates=60, n_triples=6800000,
-               n_hubs=50, hub_share=0.37, exponent=2.1, n_hub_preds=12, n_rules=50,
+               n_hubs=50, exponent=2.1, n_hub_preds=50, hub_obj_p=0.97, hub_subj_p=0.01, n_rules=50,
                min_atoms=2, max_atoms=4) -> Workload:
-    """Power-law degrees (Chung-Lu weights, exponent 2.1) plus `n_hubs` hub
-    entities holding `hub_share` of all objects of the hub predicates; rules
-    of 2-4 atoms whose paths pass through hub predicates."""
+    """Power-law degrees (Chung-Lu weights, exponent 2.1) for the tail, plus
+    `n_hubs` hub entities: objects of the `n_hub_preds` hub predicates are a
+    (uniformly chosen) hub with probability hub_obj_p, so every hub ends with
+    a degree around 10^5 at the default sizes; rules of 2-4 atoms are paths
+    grounded through a hub."""
     rng = np.random.default_rng(seed)
     rank = rng.permutation(n_entities)
     hubs = rank[:n_hubs]
     alpha = 1.0 / (exponent - 1.0)
     wcdf = _zipf_cdf(n_entities - n_hubs, alpha)
     tail = rank[n_hubs:]
-    hub_w = np.arange(1, n_hubs + 1, dtype=np.float64) ** -0.5
-    hcdf = np.cumsum(hub_w) / hub_w.sum()
     hub_preds = np.arange(n_hub_preds)
 
     def draw(k):
@@ -453,17 +453,15 @@ def hub_fanout(seed=SEED_BASE + 3, n_entities=500000, n_predicates=60, n_triples
         pp = rng.integers(0, n_predicates, k)
         op = tail[_sample_cdf(rng, wcdf, k)]
         is_hub_pred = pp < n_hub_preds
-        to_hub = is_hub_pred & (rng.random(k) < hub_share * n_predicates / n_hub_preds)
-        op = np.where(to_hub, hubs[_sample_cdf(rng, hcdf, k)], op)
-        # a few hub -> tail edges so hubs also fan out as subjects
-        from_hub = (~to_hub) & is_hub_pred & (rng.random(k) < 0.02)
-        sp = np.where(from_hub, hubs[_sample_cdf(rng, hcdf, k)], sp)
+        to_hub = is_hub_pred & (rng.random(k) < hub_obj_p)
+        op = np.where(to_hub, hubs[rng.integers(0, n_hubs, k)], op)
+        from_hub = is_hub_pred & ~to_hub & (rng.random(k) < hub_subj_p / max(1e-9, 1.0 - hub_obj_p))
+        sp = np.where(from_hub, hubs[rng.integers(0, n_hubs, k)], sp)
         return sp, pp, op
 
     s, p, o = _fill_distinct(rng, n_triples, n_entities, n_predicates, draw)
     g = Graph(s, p, o, n_entities, n_predicates, "hub_fanout")
     adj = _Adjacency(g)
-    pcdf = np.cumsum(np.ones(n_predicates)) / n_predicates
     rules = []
     for i in range(n_rules):
         k = int(rng.integers(min_atoms, max_atoms + 1))

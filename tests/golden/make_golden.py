@@ -161,7 +161,7 @@ def fx_dense_small():
 
 
 def fx_hub_small():
-    kw = dict(n_entities=20000, n_predicates=16, n_triples=120000, n_hubs=10, n_hub_preds=4,
+    kw = dict(n_entities=20000, n_predicates=16, n_triples=40000, n_hubs=20, n_hub_preds=12,
               n_rules=16)
     w = W.hub_fanout(**kw)
     make_fixture("hub_fanout_20k", w.graph, w.rules, {"fn": "hub_fanout", "kwargs": kw})
