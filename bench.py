@@ -76,6 +76,18 @@ WORKLOADS = {
 }
 
 
+def pmc_traffic(workload, kernel):
+    """HBM bytes per dispatch of `kernel` from the committed counter summary
+    (rocprofv3 FETCH_SIZE and WRITE_SIZE runs, scaled by the copy yardstick),
+    or None when that workload has not been profiled."""
+    p = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    try:
+        d = json.load(open(p))[workload]
+        return int(d["kernels"][kernel]["hbm_bytes"]), d["kernels"][kernel]["l2_hit"], d["source"]
+    except (OSError, KeyError, ValueError):
+        return None, None, None
+
+
 def describe(wl):
     g = wl.graph
     nb = [len(r[1]) for r in wl.rules]
@@ -136,6 +148,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-per-rule", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tune", default="", help="BUDGET,THREAD_WAVES_PER_CU,WAVE_WAVES_PER_CU: fixed tuning, no autotune")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run the streaming copy yardstick over this many bytes (counter scaling) and exit")
     args = ap.parse_args()
@@ -169,7 +182,13 @@ def main():
     log("graph resident in HBM (%.2fs)" % (time.time() - t))
 
     flags = 0 if args.gated else L.UNGATED
-    if not args.no_autotune:
+    if args.tune:
+        bud, tw, ww = (int(x) for x in args.tune.split(","))
+        eng.set_tuning("budget", bud)
+        eng.set_tuning("thread_waves_per_cu", tw)
+        eng.set_tuning("wave_waves_per_cu", ww)
+        log("fixed tuning: budget %d, thread waves/CU %d, wave waves/CU %d" % (bud, tw, ww))
+    elif not args.no_autotune:
         t = time.time()
         tl = eng.autotune(words, flags)
         best = tl[np.argmin(tl[:, 3])]
@@ -212,13 +231,13 @@ def main():
     kms /= reps
     eng.set_tuning("use_graph", 1)
     traffic = eng.traffic(words)
-    names = ["k_thread_batch (support)", "k_wave_batch (support)", "k_thread_batch (body pairs)",
-             "k_wave_batch (body pairs)"]
+    names = ["k_thread_batch<0>", "k_wave_batch<0>", "k_thread_batch<1>", "k_wave_batch<1>"]   # <plan kind>
     dom = int(np.argmax(kms))
     phase = dom // 2
     bytes_phase = int(traffic[:, phase].sum())
     achieved = bytes_phase / (kms[dom] * 1e-3) / 1e9 if kms[dom] > 0 else 0.0
     heavy = eng.last_stats()["heavy_seeds"]
+    hbm, l2hit, src = pmc_traffic(args.workload + ("_gated" if args.gated else ""), names[dom])
     log("kernels ms: %s; escalated seeds %d" % (", ".join("%s=%.3f" % (a, b) for a, b in zip(names, kms)), heavy))
 
     out = {
@@ -248,8 +267,8 @@ def main():
         "latency_ms": {"batched_p50": round(pct([x for x in dev_ms], 0.5), 4),
                        "batched_p99": round(pct([x for x in dev_ms], 0.99), 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBPS, 5), "traffic": None,
-                     "kernel": names[dom], "kernel_ms": round(float(kms[dom]), 4),
+                     "frac": round(achieved / HBM_PEAK_GBPS, 5), "traffic": hbm,
+                     "traffic_source": src, "l2_hit": l2hit, "kernel": names[dom], "kernel_ms": round(float(kms[dom]), 4),
                      "algorithmic_bytes": bytes_phase, "kernel_ms_all": [round(float(x), 4) for x in kms]},
     }
 

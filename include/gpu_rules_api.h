@@ -134,8 +134,9 @@ int gpu_last_kernel_ms(void* ctx, double* ms4);
 int gpu_plan_traffic(void* handle, const GpuRuleDesc* rules, int32_t n, int64_t* out);
 
 /* Measurement yardstick: copies `bytes` (multiple of 16) between two fresh
- * device buffers with a plain streaming kernel (k_copy_u32) and returns its
- * device time in ms (-1 on error). Used to scale hardware byte counters. */
+ * device buffers with a plain streaming kernel (k_copy_u32): one untimed
+ * pass, then the best device time in ms of five timed passes (-1 on error).
+ * Used to scale hardware byte counters. */
 double gpu_copy_probe(void* ctx, int64_t bytes);
 
 /* Autotuner: times the given rules (a sample of the expected workload)

@@ -267,14 +267,16 @@ __device__ __forceinline__ void push_heavy(bool heavy, uint32_t plan, uint32_t i
 // Thread mode over seeds [begin, begin + count) of the plan in W. Every lane
 // runs its own depth-first search; a lane whose seed is finished takes the
 // next one (ballot prefix count), so no lane waits for the slowest seed.
-// Returns this lane's count; escalated seeds go to the heavy queue.
+// Returns this lane's count; escalated seeds go to the heavy queue. K is the
+// plan kind of every plan the launch sees (kSupport or kPairs).
+template <int K>
 __device__ __forceinline__ uint64_t thread_chunk(WaveLds& W, const DevGraph& G, const ConflictRefs* R,
                                                  uint32_t planIdx, uint32_t begin, uint32_t count, HeavyItem* hq,
                                                  uint32_t* hCount, uint32_t hCap) {
     const int lane = lane_id();
     const Plan& P = W.plan;
     const int nlev = P.nlev;
-    const bool pairs = P.kind == kPairs && P.pairLevel >= 0;
+    const bool pairs = K == kPairs && P.pairLevel >= 0;
     const int pl = P.pairLevel;
     const uint64_t below = (1ull << lane) - 1ull;
     int v[kMaxSlots];
@@ -351,12 +353,13 @@ __device__ __forceinline__ uint64_t thread_chunk(WaveLds& W, const DevGraph& G, 
 
 // Wave mode: the whole wave on one seed. Level-0 entries are handed out one
 // at a time from an LDS cursor.
+template <int K>
 __device__ __forceinline__ uint64_t wave_seed(WaveLds& W, const DevGraph& G, const ConflictRefs* R, uint32_t item,
                                               uint32_t* row, int64_t rowWords) {
     const int lane = lane_id();
     const Plan& P = W.plan;
     const int nlev = P.nlev;
-    const bool pairs = P.kind == kPairs && P.pairLevel >= 0;
+    const bool pairs = K == kPairs && P.pairLevel >= 0;
     const int pl = P.pairLevel;
     int v[kMaxSlots];
     if (!bind_seed(P, G, (int)item, v)) return 0;
@@ -437,6 +440,7 @@ __device__ __forceinline__ uint64_t wave_seed(WaveLds& W, const DevGraph& G, con
 
 // Persistent thread mode over a chunk list: each wave repeatedly takes <= 64
 // seeds of one plan, one per lane.
+template <int K>
 __global__ void __launch_bounds__(kBlock) k_thread_batch(ThreadArgs a, DevGraph G) {
     __shared__ WaveLds lds[kWavesPerBlock];
     WaveLds& W = lds[wave_in_block()];
@@ -451,15 +455,16 @@ __global__ void __launch_bounds__(kBlock) k_thread_batch(ThreadArgs a, DevGraph 
             copy_plan(W, a.plans + ch.plan);
             loaded = ch.plan;
         }
-        const uint64_t mine = thread_chunk(W, G, a.refs + ch.plan, ch.plan, ch.begin, ch.count, a.heavy,
+        const uint64_t mine = thread_chunk<K>(W, G, a.refs + ch.plan, ch.plan, ch.begin, ch.count, a.heavy,
                                            a.heavyCount, a.heavyCap);
         const uint64_t tot = wave_sum(mine);
-        if (lane == 0 && tot) atomicAdd(a.counts + 2 * W.plan.rule + W.plan.kind, (unsigned long long)tot);
+        if (lane == 0 && tot) atomicAdd(a.counts + 2 * W.plan.rule + K, (unsigned long long)tot);
     }
 }
 
 // One plan passed by value in the kernel arguments (per-rule launches);
 // waves stride over its seeds.
+template <int K>
 __global__ void __launch_bounds__(kBlock) k_thread_one(Plan P, OneArgs a, DevGraph G) {
     __shared__ WaveLds lds[kWavesPerBlock];
     WaveLds& W = lds[wave_in_block()];
@@ -469,13 +474,14 @@ __global__ void __launch_bounds__(kBlock) k_thread_one(Plan P, OneArgs a, DevGra
     for (uint32_t base = ((blockIdx.x * blockDim.x + threadIdx.x) / kWave) * kOneChunkSeeds; base < a.nItems;
          base += waves * kOneChunkSeeds) {
         const uint32_t cnt = a.nItems - base < (uint32_t)kOneChunkSeeds ? a.nItems - base : (uint32_t)kOneChunkSeeds;
-        const uint64_t mine = thread_chunk(W, G, a.refs, 0u, base, cnt, a.heavy, a.heavyCount, a.heavyCap);
+        const uint64_t mine = thread_chunk<K>(W, G, a.refs, 0u, base, cnt, a.heavy, a.heavyCount, a.heavyCap);
         const uint64_t tot = wave_sum(mine);
         if (lane == 0 && tot) atomicAdd(a.count, (unsigned long long)tot);
     }
 }
 
 // Persistent wave mode over the heavy queue.
+template <int K>
 __global__ void __launch_bounds__(kBlock) k_wave_batch(WaveArgs a, DevGraph G) {
     __shared__ WaveLds lds[kWavesPerBlock];
     WaveLds& W = lds[wave_in_block()];
@@ -493,11 +499,12 @@ __global__ void __launch_bounds__(kBlock) k_wave_batch(WaveArgs a, DevGraph G) {
             copy_plan(W, a.plans + it.plan);
             loaded = it.plan;
         }
-        const uint64_t c = wave_seed(W, G, a.refs + it.plan, it.item, row, a.rowWords);
-        if (lane == 0 && c) atomicAdd(a.counts + 2 * W.plan.rule + W.plan.kind, (unsigned long long)c);
+        const uint64_t c = wave_seed<K>(W, G, a.refs + it.plan, it.item, row, a.rowWords);
+        if (lane == 0 && c) atomicAdd(a.counts + 2 * W.plan.rule + K, (unsigned long long)c);
     }
 }
 
+template <int K>
 __global__ void __launch_bounds__(kBlock) k_wave_one(Plan P, WaveArgs a, DevGraph G) {
     __shared__ WaveLds lds[kWavesPerBlock];
     WaveLds& W = lds[wave_in_block()];
@@ -510,7 +517,7 @@ __global__ void __launch_bounds__(kBlock) k_wave_one(Plan P, WaveArgs a, DevGrap
     for (;;) {
         const uint32_t h = wave_take(a.cursor);
         if (h >= n) break;
-        const uint64_t c = wave_seed(W, G, a.refs, a.heavy[h].item, row, a.rowWords);
+        const uint64_t c = wave_seed<K>(W, G, a.refs, a.heavy[h].item, row, a.rowWords);
         if (lane == 0 && c) atomicAdd(a.counts, (unsigned long long)c);
     }
 }
@@ -532,17 +539,23 @@ __global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v) {
 
 // ---------------------------------------------------------------- launchers
 
-void launch_thread_batch(const ThreadArgs& a, const DevGraph& G, int blocks, hipStream_t s) {
-    hipLaunchKernelGGL(k_thread_batch, dim3(blocks), dim3(kBlock), 0, s, a, G);
+// One instantiation per plan kind, so a profile names the phase and the
+// SUPPORT kernels carry no distinct-set code.
+void launch_thread_batch(int kind, const ThreadArgs& a, const DevGraph& G, int blocks, hipStream_t s) {
+    if (kind == kSupport) hipLaunchKernelGGL(k_thread_batch<kSupport>, dim3(blocks), dim3(kBlock), 0, s, a, G);
+    else hipLaunchKernelGGL(k_thread_batch<kPairs>, dim3(blocks), dim3(kBlock), 0, s, a, G);
 }
-void launch_thread_one(const Plan& P, const OneArgs& a, const DevGraph& G, int blocks, hipStream_t s) {
-    hipLaunchKernelGGL(k_thread_one, dim3(blocks), dim3(kBlock), 0, s, P, a, G);
+void launch_thread_one(int kind, const Plan& P, const OneArgs& a, const DevGraph& G, int blocks, hipStream_t s) {
+    if (kind == kSupport) hipLaunchKernelGGL(k_thread_one<kSupport>, dim3(blocks), dim3(kBlock), 0, s, P, a, G);
+    else hipLaunchKernelGGL(k_thread_one<kPairs>, dim3(blocks), dim3(kBlock), 0, s, P, a, G);
 }
-void launch_wave_batch(const WaveArgs& a, const DevGraph& G, int blocks, hipStream_t s) {
-    hipLaunchKernelGGL(k_wave_batch, dim3(blocks), dim3(kBlock), 0, s, a, G);
+void launch_wave_batch(int kind, const WaveArgs& a, const DevGraph& G, int blocks, hipStream_t s) {
+    if (kind == kSupport) hipLaunchKernelGGL(k_wave_batch<kSupport>, dim3(blocks), dim3(kBlock), 0, s, a, G);
+    else hipLaunchKernelGGL(k_wave_batch<kPairs>, dim3(blocks), dim3(kBlock), 0, s, a, G);
 }
-void launch_wave_one(const Plan& P, const WaveArgs& a, const DevGraph& G, int blocks, hipStream_t s) {
-    hipLaunchKernelGGL(k_wave_one, dim3(blocks), dim3(kBlock), 0, s, P, a, G);
+void launch_wave_one(int kind, const Plan& P, const WaveArgs& a, const DevGraph& G, int blocks, hipStream_t s) {
+    if (kind == kSupport) hipLaunchKernelGGL(k_wave_one<kSupport>, dim3(blocks), dim3(kBlock), 0, s, P, a, G);
+    else hipLaunchKernelGGL(k_wave_one<kPairs>, dim3(blocks), dim3(kBlock), 0, s, P, a, G);
 }
 
 void launch_copy(const void* src, void* dst, int64_t bytes, hipStream_t s) {

@@ -45,10 +45,11 @@ struct WaveArgs {
     int64_t rowWords;
 };
 
-void launch_thread_batch(const ThreadArgs& a, const DevGraph& G, int blocks, hipStream_t s);
-void launch_thread_one(const Plan& P, const OneArgs& a, const DevGraph& G, int blocks, hipStream_t s);
-void launch_wave_batch(const WaveArgs& a, const DevGraph& G, int blocks, hipStream_t s);
-void launch_wave_one(const Plan& P, const WaveArgs& a, const DevGraph& G, int blocks, hipStream_t s);
+// `kind` is the plan kind (kSupport / kPairs) of every plan the launch sees.
+void launch_thread_batch(int kind, const ThreadArgs& a, const DevGraph& G, int blocks, hipStream_t s);
+void launch_thread_one(int kind, const Plan& P, const OneArgs& a, const DevGraph& G, int blocks, hipStream_t s);
+void launch_wave_batch(int kind, const WaveArgs& a, const DevGraph& G, int blocks, hipStream_t s);
+void launch_wave_one(int kind, const Plan& P, const WaveArgs& a, const DevGraph& G, int blocks, hipStream_t s);
 void launch_copy(const void* src, void* dst, int64_t bytes, hipStream_t s);
 void launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t s);
 void launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);

@@ -504,7 +504,6 @@ def _sample_hub_rule(rng, adj, shape, hub_preds, n_predicates, hubs):
             atoms[j] = _atom(V(names[j]), q, V(names[j + 1]), src_is_subject)
         if not ok:
             continue
-        preds = adj.pairs().get((val["a"], val["b"]), []) if False else []
         hp = int(hub_preds[int(rng.integers(len(hub_preds)))]) if rng.random() < 0.5 \
             else int(rng.integers(n_predicates))
         return ((V("a"), hp, V("b")), atoms)
