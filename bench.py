@@ -143,8 +143,10 @@ def main():
     ap.add_argument("--gated", action="store_true", help="body pairs only for rules with support > 0")
     ap.add_argument("--serve", action="store_true",
                     help="serving mode: Poisson request stream over the high-fan-out graph")
-    ap.add_argument("--requests", type=int, default=100000)
-    ap.add_argument("--rate", type=float, default=20000.0, help="requests per second (serving mode)")
+    ap.add_argument("--requests", type=int, default=6000, help="requests in the serving stream")
+    ap.add_argument("--rate", type=float, default=150.0, help="requests per second (serving mode)")
+    ap.add_argument("--max-work", type=float, default=1e8,
+                    help="serving: drop requests whose estimated search work exceeds this (0: keep all)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-per-rule", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
@@ -306,9 +308,11 @@ def serve_mode(args):
     t0 = time.time()
     wl = W.hub_fanout()
     g = wl.graph
-    arrivals, reqs = W.serving_requests(graph=g, n_requests=args.requests, rate_per_s=args.rate)
-    log("serving workload: %s; %d requests at %.0f/s (%.1fs)" % (describe(wl), len(reqs), args.rate,
-                                                                 time.time() - t0))
+    arrivals, reqs = W.serving_requests(graph=g, n_requests=args.requests, rate_per_s=args.rate,
+                                        max_work=args.max_work or None)
+    kept = getattr(W.serving_requests, "last_kept", {})
+    log("serving workload: %s; %d requests at %.0f/s, kept fraction per path length %s (%.1fs)" % (
+        describe(wl), len(reqs), args.rate, kept, time.time() - t0))
     words = L.rule_words_for_graph(reqs, g)
     eng = L.Engine.from_graph(g)
     flags = 0     # reference semantics: body size only for rules with support > 0
@@ -325,10 +329,12 @@ def serve_mode(args):
         "metric": METRIC, "value": round(r["rules_per_s"], 1), "unit": "rules/s", "n_gpus": 1,
         "steps": 1, "warmup": 0, "ms_per_step": round(r["span_s"] * 1000.0, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic",
-        "config": {"workload": "serving: Poisson stream of %d rule requests (1-8 atoms, predicates Zipf s=1.0) "
-                               "at %.0f/s over %s; support for every request, body pairs when support > 0"
-                               % (len(reqs), args.rate, describe(wl)),
-                   "requests": len(reqs), "rate_per_s": args.rate},
+        "config": {"workload": "serving: Poisson stream of %d rule requests (1-8 atoms, predicates Zipf s=1.0, "
+                               "paths with estimated search work above %.0e redrawn) at %.0f/s over %s; support for "
+                               "every request, body pairs when support > 0"
+                               % (len(reqs), args.max_work, args.rate, describe(wl)),
+                   "requests": len(reqs), "rate_per_s": args.rate, "max_work": args.max_work,
+                   "kept_fraction_by_path_length": {str(k): round(v, 4) for k, v in kept.items()}},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4),
                        "max": round(float(lat.max()), 4)},
         "serving": st,

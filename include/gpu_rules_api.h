@@ -195,6 +195,8 @@ int64_t gpu_monotonic_ns(void);
 
 /* Parses a Turtle file into a host-only graph handle. */
 void* gpu_host_load_turtle(const char* ttlPath);
+/* Host-only graph handle from term-id triples (as gpu_load_graph_triples). */
+void* gpu_host_load_triples(const int32_t* s, const int32_t* p, const int32_t* o, int64_t n, int64_t nTerms);
 void gpu_host_free(void* host);
 /* Term count, predicate count and distinct triple count of a graph handle
  * (host handle or full context). */

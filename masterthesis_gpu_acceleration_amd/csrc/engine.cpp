@@ -8,6 +8,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "engine_internal.hpp"
@@ -720,21 +721,29 @@ int gpu_plan_traffic(void* handle, const GpuRuleDesc* rules, int32_t n, int64_t*
     if (!g || (!rules && n > 0) || (!out && n > 0)) return -1;
     try {
         Tuning t;
+        std::vector<CompiledRule> crs(std::max<int32_t>(n, 0));
+        std::vector<TrafficJob> jobs;
+        std::vector<int64_t> slot;   // out index of each job
         for (int32_t i = 0; i < n; ++i) {
-            CompiledRule cr = compile_rule(rule_from_words(rules[i].words), *g, t);
-            int64_t* o = out + 4 * i;
-            o[0] = o[1] = o[2] = o[3] = 0;
+            crs[i] = compile_rule(rule_from_words(rules[i].words), *g, t);
+            const CompiledRule& cr = crs[i];
+            for (int k = 0; k < 4; ++k) out[4 * i + k] = 0;
             if (cr.status != kRuleOk) continue;
             if (cr.supportLaunch) {
-                TrafficEstimate e = plan_traffic(cr.sup, cr.supRefs, *g, cr.supItems);
-                o[0] = e.bytes;
-                o[2] = e.nodes;
+                jobs.push_back(TrafficJob{&cr.sup, &cr.supRefs, cr.supItems});
+                slot.push_back(4 * (int64_t)i);
             }
             if (cr.pairsLaunch) {
-                TrafficEstimate e = plan_traffic(cr.pairs, cr.pairsRefs, *g, cr.pairItems);
-                o[1] = e.bytes;
-                o[3] = e.nodes;
+                jobs.push_back(TrafficJob{&cr.pairs, &cr.pairsRefs, cr.pairItems});
+                slot.push_back(4 * (int64_t)i + 1);
             }
+        }
+        std::vector<TrafficEstimate> est;
+        const unsigned hw = std::thread::hardware_concurrency();
+        plan_traffic_many(jobs, *g, est, (int)std::min<unsigned>(hw ? hw : 1, 16));
+        for (size_t j = 0; j < jobs.size(); ++j) {
+            out[slot[j]] = est[j].bytes;
+            out[slot[j] + 2] = est[j].nodes;
         }
         return 0;
     } catch (const std::exception& e) {
@@ -790,6 +799,28 @@ void* gpu_host_load_turtle(const char* ttlPath) {
         return H;
     } catch (const std::exception& e) {
         std::fprintf(stderr, "[gpu_rules] gpu_host_load_turtle error: %s\n", e.what());
+        return nullptr;
+    }
+}
+
+void* gpu_host_load_triples(const int32_t* s, const int32_t* p, const int32_t* o, int64_t n, int64_t nTerms) {
+    if ((!s || !p || !o) && n > 0) return nullptr;
+    try {
+        for (int64_t i = 0; i < n; ++i) {
+            if (s[i] < 0 || p[i] < 0 || o[i] < 0 || s[i] >= nTerms || p[i] >= nTerms || o[i] >= nTerms) {
+                std::fprintf(stderr, "[gpu_rules] triple %lld has a term id outside [0, %lld)\n", (long long)i,
+                             (long long)nTerms);
+                return nullptr;
+            }
+        }
+        auto* H = new HostHandle();
+        H->magic = kMagicHost;
+        std::vector<int32_t> S(s, s + n), P(p, p + n), O(o, o + n);
+        build_graph(H->g, S, P, O, nTerms);
+        H->tr.reset(new TermResolver(H->g));
+        return H;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "[gpu_rules] gpu_host_load_triples error: %s\n", e.what());
         return nullptr;
     }
 }

@@ -11,7 +11,9 @@
 //   SCAN candidate    8 B (coo + col)
 // It returns no counts to callers; the counting path is the device's.
 #include <algorithm>
+#include <atomic>
 #include <cstdint>
+#include <thread>
 #include <vector>
 
 #include "graph_host.hpp"
@@ -30,7 +32,21 @@ struct Walker {
     int64_t bytes = 0;
     int64_t nodes = 0;
     int v[kMaxSlots];
-    std::vector<int> set;   // distinct head objects of the current PAIRS seed
+    // distinct head objects of the current PAIRS seed: one bit per term id,
+    // with the words set listed so a seed clears only those
+    std::vector<uint64_t>& bits;
+    std::vector<int64_t>& touched;
+
+    bool seen(int x) const { return (bits[x >> 6] >> (x & 63)) & 1u; }
+    void insert(int x) {
+        uint64_t& w = bits[x >> 6];
+        if (!w) touched.push_back(x >> 6);
+        w |= 1ull << (x & 63);
+    }
+    void clear_set() {
+        for (int64_t w : touched) bits[w] = 0;
+        touched.clear();
+    }
 
     int col(int64_t e) const { return e < T ? g.spoVal[e] : g.posVal[e - T]; }
     int coo(int64_t e) const { return e < T ? g.spoKey[e] : g.posKey[e - T]; }
@@ -110,8 +126,8 @@ struct Walker {
             if (!bind(L, i)) continue;
             if (pairs && L == P.pairLevel) {
                 const int ho = tv(P.hoRef);
-                if (std::find(set.begin(), set.end(), ho) != set.end()) continue;
-                if (L + 1 == P.nlev || dfs(L + 1, false)) set.push_back(ho);
+                if (seen(ho)) continue;
+                if (L + 1 == P.nlev || dfs(L + 1, false)) insert(ho);
                 continue;
             }
             if (L + 1 == P.nlev) {
@@ -125,7 +141,7 @@ struct Walker {
 
     void seed(int64_t item) {
         for (int& x : v) x = -1;
-        set.clear();
+        clear_set();
         if (P.kind == kSupport) {
             bytes += 8;
             const int64_t e = P.seedBegin + item;
@@ -149,9 +165,51 @@ struct Walker {
 }  // namespace
 
 TrafficEstimate plan_traffic(const Plan& P, const ConflictRefs& R, const HostGraph& g, int64_t items) {
-    Walker w{P, R, g, g.T()};
+    std::vector<uint64_t> bits((size_t)g.nBlocks(), 0);
+    std::vector<int64_t> touched;
+    Walker w{P, R, g, g.T(), 0, 0, {}, bits, touched};
     for (int64_t i = 0; i < items; ++i) w.seed(i);
     return TrafficEstimate{w.bytes, w.nodes};
+}
+
+void plan_traffic_many(const std::vector<TrafficJob>& jobs, const HostGraph& g, std::vector<TrafficEstimate>& out,
+                       int threads) {
+    constexpr int64_t kSeedsPerTask = 512;
+    struct Task {
+        size_t job;
+        int64_t begin, end;
+    };
+    std::vector<Task> tasks;
+    for (size_t j = 0; j < jobs.size(); ++j)
+        for (int64_t b = 0; b < jobs[j].items; b += kSeedsPerTask)
+            tasks.push_back(Task{j, b, std::min(jobs[j].items, b + kSeedsPerTask)});
+    std::vector<std::atomic<int64_t>> bytes(jobs.size()), nodes(jobs.size());
+    for (size_t j = 0; j < jobs.size(); ++j) {
+        bytes[j] = 0;
+        nodes[j] = 0;
+    }
+    std::atomic<size_t> next{0};
+    auto work = [&]() {
+        std::vector<uint64_t> bits((size_t)g.nBlocks(), 0);
+        std::vector<int64_t> touched;
+        for (;;) {
+            const size_t t = next.fetch_add(1);
+            if (t >= tasks.size()) break;
+            const TrafficJob& J = jobs[tasks[t].job];
+            Walker w{*J.plan, *J.refs, g, g.T(), 0, 0, {}, bits, touched};
+            for (int64_t i = tasks[t].begin; i < tasks[t].end; ++i) w.seed(i);
+            w.clear_set();
+            bytes[tasks[t].job] += w.bytes;
+            nodes[tasks[t].job] += w.nodes;
+        }
+    };
+    threads = std::max(1, std::min<int>(threads, (int)std::max<size_t>(tasks.size(), 1)));
+    std::vector<std::thread> pool;
+    for (int k = 1; k < threads; ++k) pool.emplace_back(work);
+    work();
+    for (auto& th : pool) th.join();
+    out.assign(jobs.size(), TrafficEstimate{0, 0});
+    for (size_t j = 0; j < jobs.size(); ++j) out[j] = TrafficEstimate{bytes[j].load(), nodes[j].load()};
 }
 
 }  // namespace gr

@@ -1,6 +1,7 @@
 // traffic.hpp -- host model of the bytes a plan's search reads from HBM.
 #pragma once
 #include <cstdint>
+#include <vector>
 
 #include "graph_host.hpp"
 #include "plan.hpp"
@@ -13,5 +14,14 @@ struct TrafficEstimate {
 };
 
 TrafficEstimate plan_traffic(const Plan& P, const ConflictRefs& R, const HostGraph& g, int64_t items);
+
+// Many plans at once on `threads` host threads (seeds split into tasks).
+struct TrafficJob {
+    const Plan* plan;
+    const ConflictRefs* refs;
+    int64_t items;
+};
+void plan_traffic_many(const std::vector<TrafficJob>& jobs, const HostGraph& g, std::vector<TrafficEstimate>& out,
+                       int threads);
 
 }  // namespace gr

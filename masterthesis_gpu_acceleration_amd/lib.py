@@ -70,6 +70,8 @@ _SIGS = {
     "gpu_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int64]),
     "gpu_get_tuning": (ctypes.c_int64, [ctypes.c_void_p, ctypes.c_char_p]),
     "gpu_host_load_turtle": (ctypes.c_void_p, [ctypes.c_char_p]),
+    "gpu_host_load_triples": (ctypes.c_void_p, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                                ctypes.c_int64]),
     "gpu_host_free": (None, [ctypes.c_void_p]),
     "gpu_host_graph_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64),
                                            ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64)]),
@@ -138,6 +140,39 @@ class GpuRulesLib:
 
     def free_graph(self, ctx):
         self.lib.gpu_free_graph(ctx)
+
+
+class HostGraph:
+    """Host-only graph handle (no device): planner and traffic model."""
+
+    def __init__(self, g):
+        self.lib = load_library()
+        s = np.ascontiguousarray(g.s, np.int32)
+        p = np.ascontiguousarray(g.p.astype(np.int64) + g.n_entities, np.int32)
+        o = np.ascontiguousarray(g.o, np.int32)
+        self.h = self.lib.gpu_host_load_triples(s.ctypes.data, p.ctypes.data, o.ctypes.data, int(s.shape[0]),
+                                                int(g.n_entities + g.n_predicates))
+        if not self.h:
+            raise RuntimeError("gpu_host_load_triples failed")
+
+    def traffic(self, rule_words: np.ndarray) -> np.ndarray:
+        """Per rule: [support bytes, pair bytes, support nodes, pair nodes]."""
+        w = np.ascontiguousarray(rule_words, np.int32)
+        out = np.zeros((w.shape[0], 4), np.int64)
+        if self.lib.gpu_plan_traffic(self.h, w.ctypes.data, int(w.shape[0]), out.ctypes.data) != 0:
+            raise RuntimeError("gpu_plan_traffic failed")
+        return out
+
+    def close(self):
+        if self.h:
+            self.lib.gpu_host_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Engine:

@@ -60,6 +60,138 @@ class Workload:
 
 # ---------------------------------------------------------------- helpers
 
+class _PathWork:
+    """Estimates of the search work of path-shaped rules, used to keep
+    generated workloads within a stated bound.
+
+    Step (p, d) moves from a variable u to w over predicate p: d = 0 when u
+    is the subject (w ranges over u's p-objects), d = 1 when u is the object.
+    deg[2p+d][u] is the number of such moves from u. For a path of steps
+    s1..sk from every start entity, the number of groundings (with repeats
+    allowed, so an upper bound on the injective count) is
+    1' A1 A2 ... Ak 1; work(x, s, y) = sum over moves u->w of step s of
+    deg[x][u] * deg[y][w] gives the 3-step count in one pass over s's
+    triples, and walk() gives exact path counts of any length by repeated
+    sparse products."""
+
+    def __init__(self, g: Graph):
+        self.g = g
+        n = g.n_entities
+        P = g.n_predicates
+        order = np.argsort(g.p, kind="stable")
+        self.s = g.s[order].astype(np.int64)
+        self.o = g.o[order].astype(np.int64)
+        self.ptr = np.searchsorted(g.p[order], np.arange(P + 1))
+        self.nnz = np.diff(self.ptr).astype(np.float64)
+        self.deg = np.zeros((2 * P, n), np.float32)
+        for p in range(P):
+            a, b = self.ptr[p], self.ptr[p + 1]
+            self.deg[2 * p] = np.bincount(self.s[a:b], minlength=n)
+            self.deg[2 * p + 1] = np.bincount(self.o[a:b], minlength=n)
+        self._dot = None
+        self._work = {}
+
+    def moves(self, step):
+        p, d = step >> 1, step & 1
+        a, b = self.ptr[p], self.ptr[p + 1]
+        return (self.s[a:b], self.o[a:b]) if d == 0 else (self.o[a:b], self.s[a:b])
+
+    def dot(self, x, y):
+        if self._dot is None:
+            self._dot = (self.deg.astype(np.float64) @ self.deg.T.astype(np.float64))
+        return float(self._dot[x, y])
+
+    def work(self, x, step, y):
+        if step not in self._work:
+            u, w = self.moves(step)
+            self._work[step] = self.deg[:, u].astype(np.float64) @ self.deg[:, w].T.astype(np.float64)
+        return float(self._work[step][x, y])
+
+    def work_fast(self, head_pred, steps) -> float:
+        """_rule_work for paths of at most 3 steps from the dot / work
+        tables (no pass over the triples once the tables are built)."""
+        k = len(steps)
+        h = head_pred
+        if k == 1:
+            return float(max(self.nnz[steps[0] >> 1], self.nnz[h]))
+        if k == 2:
+            s1, s2 = steps
+            return max(self.dot(s1 ^ 1, s2), self.dot(2 * h, s1), self.dot(2 * h + 1, s2 ^ 1))
+        s1, s2, s3 = steps
+        return max(self.work(s1 ^ 1, s2, s3), self.work(2 * h, s1, s2), self.work(2 * h + 1, s3 ^ 1, s2 ^ 1))
+
+    def tables(self):
+        """dot[x, y] and the full work[x, s, y] table (float32; one
+        product per predicate, the reversed step is the transpose)."""
+        if getattr(self, "_table", None) is None:
+            P2 = self.deg.shape[0]
+            tab = np.zeros((P2, P2, P2), np.float32)
+            for p in range(P2 // 2):
+                u, w = self.moves(2 * p)
+                m = self.deg[:, u] @ self.deg[:, w].T
+                tab[:, 2 * p, :] = m
+                tab[:, 2 * p + 1, :] = m.T
+            self._table = tab
+            self._dot32 = (self.deg @ self.deg.T).astype(np.float64)
+        return self._dot32, self._table
+
+    def work_many(self, heads, steps, plen):
+        """work_fast over arrays: heads[m], steps[m, 3] (only the first
+        plen[i] used), plen[m] in 1..3."""
+        dot, tab = self.tables()
+        h = np.asarray(heads, np.int64)
+        s1, s2, s3 = (steps[:, j].astype(np.int64) for j in range(3))
+        w1 = np.maximum(self.nnz[s1 >> 1], self.nnz[h])
+        w2 = np.maximum.reduce([dot[s1 ^ 1, s2], dot[2 * h, s1], dot[2 * h + 1, s2 ^ 1]])
+        w3 = np.maximum.reduce([tab[s1 ^ 1, s2, s3], tab[2 * h, s1, s2], tab[2 * h + 1, s3 ^ 1, s2 ^ 1]])
+        return np.where(plen == 1, w1, np.where(plen == 2, w2, w3.astype(np.float64)))
+
+    def walk(self, steps, start=None):
+        """Number of walks along `steps` (from every entity, or weighted by
+        `start`), one sparse product per step."""
+        v = np.ones(self.g.n_entities) if start is None else start.astype(np.float64)
+        for st in steps:
+            u, w = self.moves(st)
+            v = np.bincount(w, weights=v[u], minlength=self.g.n_entities)
+        return float(v.sum())
+
+
+def _path_steps(body, names):
+    """Steps of a path-shaped body walked from names[0]: atom (x p y) met at x
+    is (p, 0), met at y is (p, 1). Atoms off the path (closing atoms) are
+    left out."""
+    steps = []
+    left = list(body)
+    cur = names[0]
+    for nxt in names[1:]:
+        for j, (x, p, y) in enumerate(left):
+            if x == V(cur) and y == V(nxt):
+                steps.append(2 * p)
+                break
+            if y == V(cur) and x == V(nxt):
+                steps.append(2 * p + 1)
+                break
+        else:
+            return None
+        left.pop(j)
+        cur = nxt
+    return steps
+
+
+def _rule_work(pw: _PathWork, head_pred: int, steps) -> float:
+    """Largest of: body groundings from every start (the body-pair search),
+    and the per-head-triple search from either end of the path (support)."""
+    k = len(steps)
+    h = head_pred
+    if k == 1:
+        return max(pw.nnz[steps[0] >> 1], pw.nnz[h])
+    back = [s ^ 1 for s in reversed(steps)]
+    pairs = pw.walk(steps)
+    sup_a = pw.walk(steps[:-1], pw.deg[2 * h])
+    sup_b = pw.walk(back[:-1], pw.deg[2 * h + 1])
+    return max(pairs, sup_a, sup_b)
+
+
 def _zipf_cdf(n: int, s: float) -> np.ndarray:
     w = np.arange(1, n + 1, dtype=np.float64) ** (-s)
     c = np.cumsum(w)
@@ -434,12 +566,21 @@ def dbpedia_shaped(seed=SEED_BASE + 2, n_entities=50000, n_predicates=300, n_tri
 
 def hub_fanout(seed=SEED_BASE + 3, n_entities=500000, n_predicates=60, n_triples=6800000,
                n_hubs=50, exponent=2.1, n_hub_preds=50, hub_obj_p=0.97, hub_subj_p=0.01, n_rules=50,
-               min_atoms=2, max_atoms=4) -> Workload:
+               min_atoms=2, max_atoms=4, max_path=3, max_work=1e9, tries=64) -> Workload:
     """Power-law degrees (Chung-Lu weights, exponent 2.1) for the tail, plus
     `n_hubs` hub entities: objects of the `n_hub_preds` hub predicates are a
     (uniformly chosen) hub with probability hub_obj_p, so every hub ends with
     a degree around 10^5 at the default sizes; rules of 2-4 atoms are paths
-    grounded through a hub."""
+    grounded through a hub (at most `max_path` atoms long; longer rules add
+    atoms that close over two path variables).
+
+    A path through a hub with both neighbouring atoms on hub predicates has
+    about deg(hub)^2 groundings per hub, and every further atom multiplies
+    that by a degree-weighted fan-out, so most 3-4 atom walks have 10^10 to
+    10^12 groundings. A rule is kept when its estimated search work
+    (_rule_work: body groundings from every start, and the support search
+    from either end) is at most `max_work`; up to `tries` walks are drawn per
+    rule and the cheapest is kept when none qualifies."""
     rng = np.random.default_rng(seed)
     rank = rng.permutation(n_entities)
     hubs = rank[:n_hubs]
@@ -462,18 +603,32 @@ def hub_fanout(seed=SEED_BASE + 3, n_entities=500000, n_predicates=60, n_triples
     s, p, o = _fill_distinct(rng, n_triples, n_entities, n_predicates, draw)
     g = Graph(s, p, o, n_entities, n_predicates, "hub_fanout")
     adj = _Adjacency(g)
+    pw = _PathWork(g)
     rules = []
     for i in range(n_rules):
         k = int(rng.integers(min_atoms, max_atoms + 1))
-        names = ["a"] + ["c%d" % j for j in range(1, k)] + ["b"]
-        shape = [(names[j], names[j + 1]) for j in range(k)]
-        rules.append(_sample_hub_rule(rng, adj, shape, hub_preds, n_predicates, hubs))
+        plen = min(k, max_path)
+        names = ["a"] + ["c%d" % j for j in range(1, plen)] + ["b"]
+        shape = [(names[j], names[j + 1]) for j in range(plen)]
+        best, best_w = None, None
+        for _ in range(tries):
+            r = _sample_hub_rule(rng, adj, shape, hub_preds, n_predicates, hubs, n_close=k - plen)
+            steps = _path_steps(r[1], names)
+            w = _rule_work(pw, r[0][1], steps) if steps is not None else float("inf")
+            if best_w is None or w < best_w:
+                best, best_w = r, w
+            if w <= max_work:
+                break
+        rules.append(best)
     return Workload(g, rules, "hub_fanout")
 
 
-def _sample_hub_rule(rng, adj, shape, hub_preds, n_predicates, hubs):
+def _sample_hub_rule(rng, adj, shape, hub_preds, n_predicates, hubs, n_close=0):
     """Path rule grounded through one hub: the walk starts at a hub and
-    extends both ways, so at least one body atom sits on a hub predicate."""
+    extends both ways, so at least one body atom sits on a hub predicate.
+    `n_close` more atoms join two non-adjacent path variables, over a
+    predicate that links their entities in the sampled grounding when one
+    does."""
     g = adj.g
     hubset = set(int(h) for h in hubs)
     for _ in range(64):
@@ -504,11 +659,32 @@ def _sample_hub_rule(rng, adj, shape, hub_preds, n_predicates, hubs):
             atoms[j] = _atom(V(names[j]), q, V(names[j + 1]), src_is_subject)
         if not ok:
             continue
+        for _ in range(n_close):
+            i, j = sorted(int(x) for x in rng.choice(len(names), size=2, replace=False))
+            if j - i < 2 and len(names) > 2:
+                i, j = 0, len(names) - 1
+            atoms.append(_closing_atom(rng, adj, names[i], names[j], val, n_predicates))
         hp = int(hub_preds[int(rng.integers(len(hub_preds)))]) if rng.random() < 0.5 \
             else int(rng.integers(n_predicates))
         return ((V("a"), hp, V("b")), atoms)
     body = [(V(x), int(rng.integers(n_predicates)), V(y)) for x, y in shape]
     return ((V("a"), int(rng.integers(n_predicates)), V("b")), body)
+
+
+def _closing_atom(rng, adj, x, y, val, n_predicates):
+    """Atom between variables x and y: a predicate of a triple joining their
+    sampled entities (either direction) if there is one, else a random one."""
+    g = adj.g
+    out_t, in_t = adj.incident(val[x])
+    fwd = out_t[g.o[out_t] == val[y]]
+    bwd = in_t[g.s[in_t] == val[y]]
+    if fwd.shape[0] + bwd.shape[0]:
+        k = int(rng.integers(fwd.shape[0] + bwd.shape[0]))
+        if k < fwd.shape[0]:
+            return (V(x), int(g.p[fwd[k]]), V(y))
+        return (V(y), int(g.p[bwd[k - fwd.shape[0]]]), V(x))
+    p = int(rng.integers(n_predicates))
+    return (V(x), p, V(y)) if rng.random() < 0.5 else (V(y), p, V(x))
 
 
 # ---------------------------------------------------------------- dense_conf
@@ -621,28 +797,63 @@ def is_closed(rule) -> bool:
 # ----------------------------------------------------------- serving stream
 
 def serving_requests(seed=SEED_BASE + 5, graph: Graph | None = None, n_requests=100000,
-                     rate_per_s=20000.0, min_atoms=1, max_atoms=8, pred_s=1.0):
+                     rate_per_s=20000.0, min_atoms=1, max_atoms=8, pred_s=1.0, max_work=None):
     """Poisson arrivals at `rate_per_s`; rule lengths uniform in
     [min_atoms, max_atoms]; predicates Zipf(pred_s). Returns (arrival_times_s,
-    rules). Longer rules chain a path a -> b and close extra atoms over bound
-    variables, as in dense_conf."""
+    rules). A rule's body is a path a -> b of min(k, 3) atoms (random
+    directions) plus atoms closing over two of its variables; the head is
+    (?a p ?b).
+
+    With `max_work` (needs `graph`), a path whose estimated search work
+    (_PathWork.work_many) exceeds the bound is dropped and another path of
+    the same length drawn, as a rule miner's pruning bound keeps such rules
+    out of the stream; the lengths stay uniform. The kept fraction per path
+    length is left in serving_requests.last_kept."""
     rng = np.random.default_rng(seed)
     n_pred = graph.n_predicates if graph is not None else 60
     pcdf = _zipf_cdf(n_pred, pred_s)
     gaps = rng.exponential(1.0 / rate_per_s, n_requests)
     t = np.cumsum(gaps)
+    k = rng.integers(min_atoms, max_atoms + 1, n_requests)
+    plen = np.minimum(k, 3)
+    pw = _PathWork(graph) if (max_work is not None and graph is not None) else None
+    pool = {}
+    kept = {}
+    for L in (1, 2, 3):
+        need = int((plen == L).sum())
+        got_p, got_f, got_h = [], [], []
+        have, drawn = 0, 0
+        while have < need:
+            m = max(1024, 2 * (need - have))
+            preds = _sample_cdf(rng, pcdf, 3 * m).reshape(m, 3)
+            fwd = rng.integers(0, 2, (m, 3)).astype(bool)
+            head = _sample_cdf(rng, pcdf, m)
+            drawn += m
+            if pw is not None:
+                ok = pw.work_many(head, 2 * preds + (~fwd).astype(np.int64), np.full(m, L)) <= max_work
+                preds, fwd, head = preds[ok], fwd[ok], head[ok]
+            got_p.append(preds)
+            got_f.append(fwd)
+            got_h.append(head)
+            have += preds.shape[0]
+        if need:
+            pool[L] = (np.concatenate(got_p), np.concatenate(got_f), np.concatenate(got_h))
+            kept[L] = have / drawn
+    used = {1: 0, 2: 0, 3: 0}
     rules = []
-    for _ in range(n_requests):
-        k = int(rng.integers(min_atoms, max_atoms + 1))
-        plen = min(k, 3)
-        names = ["a"] + ["c%d" % j for j in range(1, plen)] + ["b"]
-        body = [_atom(V(names[j]), int(_sample_cdf(rng, pcdf, 1)[0]), V(names[j + 1]),
-                      bool(rng.integers(2))) for j in range(plen)]
-        while len(body) < k:
+    for i in range(n_requests):
+        L = int(plen[i])
+        preds, fwd, head = pool[L]
+        j = used[L]
+        used[L] += 1
+        names = ["a"] + ["c%d" % q for q in range(1, L)] + ["b"]
+        body = [_atom(V(names[q]), int(preds[j, q]), V(names[q + 1]), bool(fwd[j, q])) for q in range(L)]
+        while len(body) < int(k[i]):
             x, y = rng.choice(len(names), size=2, replace=False)
             body.append(_atom(V(names[x]), int(_sample_cdf(rng, pcdf, 1)[0]), V(names[y]),
                               bool(rng.integers(2))))
-        rules.append(((V("a"), int(_sample_cdf(rng, pcdf, 1)[0]), V("b")), body))
+        rules.append(((V("a"), int(head[j]), V("b")), body))
+    serving_requests.last_kept = kept
     return t, rules
 
 
