@@ -319,7 +319,8 @@ def serve_mode(args):
     eng.evaluate(words[:256], flags)     # warm the kernels and the dedup rows
     srv = S.Server(eng, max_batch=2048, max_in_flight=3, flags=flags)
     try:
-        r = S.run_stream(srv, words, arrivals)
+        r = S.run_stream(srv, words, arrivals,
+                         progress=lambda sub, done: log("serving: %d submitted, %d completed" % (sub, done)))
         st = srv.stats()
     finally:
         srv.stop()

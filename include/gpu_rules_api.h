@@ -127,6 +127,13 @@ int gpu_last_batch_stats(void* ctx, double* totalMs, double* supportMs, double* 
  * wave mode (ms). Zero after a graph replay. */
 int gpu_last_kernel_ms(void* ctx, double* ms4);
 
+/* Sort-unique dedup of the last evaluation: device time of the radix sort +
+ * unique pass (eager batches; 0 after a graph replay), keys the wave-mode
+ * kernels emitted, and the buffers' current key capacity. 0 or -1. The
+ * counterpart of the reference's overflow path (Thrust sort + unique,
+ * GPU_03/gpu_rules_lib.cu:42-51,508-511), here an additive symbol. */
+int gpu_last_dedup_stats(void* ctx, double* sortMs, int64_t* keysEmitted, int64_t* sortCap);
+
 /* Host model of the HBM bytes each rule's searches read (the same plans,
  * walked serially): out[4*i] support bytes, out[4*i+1] body-pair bytes,
  * out[4*i+2], out[4*i+3] search levels entered. Works on a context or a
@@ -151,7 +158,12 @@ int gpu_autotune(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags, 
 
 /* Tuning knobs: "budget" (thread-mode candidates per seed before a seed is
  * moved to wave mode), "thread_waves_per_cu", "wave_waves_per_cu",
- * "use_graph" (0/1). Returns 0, or -1 for an unknown key. */
+ * "use_graph" (0/1), "dedup" (body-pair dedup of wave-mode seeds: 0 auto --
+ * bitmap rows unless one nE-bit row per CU exceeds 8 GiB, then sort-unique;
+ * 1 bitmap rows; 2 sort-unique), "sort_cap" (initial key capacity of the
+ * sort-unique buffers, >= 4096; they grow on overflow). Returns 0, or -1 for
+ * an unknown key or value. gpu_get_tuning also reads "num_cus",
+ * "wave_slots", "sort_active" (0/1) and "sort_cap". */
 int gpu_set_tuning(void* ctx, const char* key, int64_t value);
 int64_t gpu_get_tuning(void* ctx, const char* key);
 

@@ -21,6 +21,8 @@ bool Context::upload() {
     hipDeviceProp_t prop;
     GR_HIP(hipGetDeviceProperties(&prop, device));
     numCUs = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    if (prop.sharedMemPerBlock > 0) ldsPerBlock = prop.sharedMemPerBlock;
+    ldsPerCU = prop.maxSharedMemoryPerMultiProcessor > 0 ? prop.maxSharedMemoryPerMultiProcessor : ldsPerBlock;
     GR_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     for (auto& e : ev) GR_HIP(hipEventCreate(&e));
     GR_HIP(hipHostMalloc((void**)&hOne, 512, hipHostMallocDefault));
@@ -83,11 +85,25 @@ bool Context::upload() {
 bool Context::ensure_dedup() {
     rowWords = (g.nE + 31) / 32;
     if (rowWords < 1) rowWords = 1;
+    const int64_t rowBytes = rowWords * 4;
+    rowsFit = rowBytes * numCUs <= kRowBudget;
+    const size_t blockLds = wave_static_lds() + (size_t)rowBytes + 1024;
+    ldsFits = blockLds <= ldsPerBlock;
     waveSlots = numCUs * waveWavesPerCU;
-    // keep the dedup rows within 2 GiB
-    const int64_t maxSlots = std::max<int64_t>(4, (int64_t(2) << 30) / (rowWords * 4));
-    if (waveSlots > maxSlots) waveSlots = (int)(maxSlots / 4 * 4);
-    const size_t bytes = (size_t)waveSlots * rowWords * 4;
+    const int mode = pairsDedup();
+    if (mode == kDedupLds) {
+        prepare_wave_kernels((size_t)rowBytes);
+        const int perCU = std::max<int>(1, (int)(ldsPerCU / blockLds));   // resident bitmaps per CU
+        waveSlots = numCUs * std::min(waveWavesPerCU, perCU);
+    }
+    // LDS bitmaps and the sort path never touch the HBM rows; one keeps the kernels' pointer valid
+    size_t bytes = (size_t)rowBytes;
+    if (mode == kDedupRows) {
+        // keep the dedup rows within kRowBudget
+        const int64_t maxSlots = std::max<int64_t>(4, kRowBudget / rowBytes);
+        if (waveSlots > maxSlots) waveSlots = (int)(maxSlots / 4 * 4);
+        bytes = (size_t)waveSlots * rowBytes;
+    }
     if (dedup.bytes < bytes) {
         if (!dedup.ensure(bytes)) return false;
         GR_HIP(hipMemset(dedup.p, 0, dedup.bytes));
@@ -105,6 +121,9 @@ bool has_conflicts(const Plan& P) {
 
 // One plan: thread-mode launch over all seeds, wave-mode launch over the
 // escalated ones, sync, copy the count back.
+// With the sort-unique dedup path, a body-pair plan's wave-mode launch
+// emits keys and the sort-unique pass counts them; a pass that emitted more
+// keys than the buffer holds is redone with a larger one.
 bool run_one(Context& C, const Plan& P, const ConflictRefs& R, const Plan& PW, const ConflictRefs& RW, int64_t items,
              uint64_t& count, double& ms, int64_t& heavySeeds) {
     count = 0;
@@ -115,33 +134,58 @@ bool run_one(Context& C, const Plan& P, const ConflictRefs& R, const Plan& PW, c
         C.heavy1Cap = (uint32_t)items;
     }
     unsigned long long* d_count = C.one.as<unsigned long long>();
-    uint32_t* ctl = reinterpret_cast<uint32_t*>(C.one.as<char>() + 64);
+    uint32_t* ctl = reinterpret_cast<uint32_t*>(C.one.as<char>() + 64);   // heavy count, wave cursor, keys emitted
     ConflictRefs* d_refs = reinterpret_cast<ConflictRefs*>(C.one.as<char>() + 128);
     const bool conf = has_conflicts(P) || has_conflicts(PW);
     if (conf) {
         ConflictRefs both[2] = {R, RW};
         GR_HIP(hipMemcpyAsync(d_refs, both, sizeof(both), hipMemcpyHostToDevice, C.st));
     }
-    launch_fill_u64(d_count, 1, 0, C.st);
-    launch_fill_u32(ctl, 4, 0, C.st);
-    GR_HIP(hipEventRecord(C.ev[0], C.st));
-    const int64_t waves = (items + kWave - 1) / kWave;
-    const int blocks = (int)std::min<int64_t>((waves + 3) / 4, C.threadBlocks());
-    OneArgs oa{conf ? d_refs : nullptr, (uint32_t)items, C.heavy1Cap, d_count, C.heavy1.as<HeavyItem>(), ctl + 0};
-    launch_thread_one(P.kind, P, oa, C.dg, blocks, C.st);
-    WaveArgs wa{nullptr, conf ? d_refs + 1 : nullptr, C.heavy1.as<HeavyItem>(), ctl + 0, C.heavy1Cap, 0, ctl + 1,
-                d_count, C.dedup.as<uint32_t>(), C.rowWords};
-    launch_wave_one(PW.kind, PW, wa, C.dg, C.waveBlocks(), C.st);
-    GR_HIP(hipGetLastError());
-    GR_HIP(hipEventRecord(C.ev[1], C.st));
-    GR_HIP(hipMemcpyAsync(C.hOne, C.one.p, 80, hipMemcpyDeviceToHost, C.st));
-    GR_HIP(hipStreamSynchronize(C.st));
-    float f = 0;
-    GR_HIP(hipEventElapsedTime(&f, C.ev[0], C.ev[1]));
-    ms = f;
-    count = C.hOne[0];
-    heavySeeds += reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(C.hOne) + 64)[0];
-    return true;
+    const int mode = PW.kind == kPairs ? C.pairsDedup() : kDedupRows;
+    const bool sort = mode == kDedupSort;
+    if (sort && !C.sortb.ensure(1)) return false;
+    const int hoBits = bits_for((uint64_t)C.g.nE);
+    const int hBits = bits_for((uint64_t)items);
+    for (;;) {
+        launch_fill_u64(d_count, 1, 0, C.st);
+        launch_fill_u32(ctl, 4, 0, C.st);
+        GR_HIP(hipEventRecord(C.ev[0], C.st));
+        const int64_t waves = (items + kWave - 1) / kWave;
+        const int blocks = (int)std::min<int64_t>((waves + 3) / 4, C.threadBlocks());
+        OneArgs oa{conf ? d_refs : nullptr, (uint32_t)items, C.heavy1Cap, d_count, C.heavy1.as<HeavyItem>(), ctl + 0};
+        launch_thread_one(P.kind, P, oa, C.dg, blocks, C.st);
+        SortSink sink{};
+        if (sort) sink = SortSink{C.sortb.keys[0].as<unsigned long long>(), ctl + 2, C.sortb.cap, hoBits};
+        WaveArgs wa{nullptr, conf ? d_refs + 1 : nullptr, C.heavy1.as<HeavyItem>(), ctl + 0, C.heavy1Cap, 0, ctl + 1,
+                    d_count, C.dedup.as<uint32_t>(), C.rowWords, sink};
+        launch_wave_one(PW.kind, mode, PW, wa, C.dg, C.waveBlocks(), C.st);
+        if (sort) {
+            SortJob j{{C.sortb.keys[0].as<unsigned long long>(), C.sortb.keys[1].as<unsigned long long>()},
+                      ctl + 2, C.sortb.cap, sort_max_tiles(C.sortb.cap), C.sortb.hist.as<uint32_t>(),
+                      hoBits + hBits, hoBits, nullptr, nullptr, d_count, C.sortBlocks()};
+            launch_sort_unique(j, C.st);
+        }
+        GR_HIP(hipGetLastError());
+        GR_HIP(hipEventRecord(C.ev[1], C.st));
+        GR_HIP(hipMemcpyAsync(C.hOne, C.one.p, 80, hipMemcpyDeviceToHost, C.st));
+        GR_HIP(hipStreamSynchronize(C.st));
+        const uint32_t* hctl = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(C.hOne) + 64);
+        if (sort) C.lastEmitted = hctl[2];
+        if (sort && hctl[2] > C.sortb.cap) {
+            if (C.sortb.cap >= kSortCapMax) {
+                std::fprintf(stderr, "[gpu_rules] body-pair keys of one rule exceed %u\n", kSortCapMax);
+                return false;
+            }
+            if (!C.sortb.ensure(hctl[2])) return false;
+            continue;
+        }
+        float f = 0;
+        GR_HIP(hipEventElapsedTime(&f, C.ev[0], C.ev[1]));
+        ms = f;
+        count = C.hOne[0];
+        heavySeeds += hctl[0];
+        return true;
+    }
 }
 
 void finish_measures(GpuRuleResult64& r) {
@@ -268,9 +312,21 @@ void batch_enqueue(Context& C, Batch& B, const ExecTarget& t) {
         }
         if (t.ev) (void)hipEventRecord(t.ev[1 + 2 * ph], st);
         if (nc) {
+            const bool sort = ph == kPairs && t.pairsDedup == kDedupSort && t.sort;
+            const int hoBits = bits_for((uint64_t)C.g.nE);
+            SortSink sink{};
+            if (sort) sink = SortSink{t.sort->keys[0].as<unsigned long long>(), c + 3, t.sort->cap, hoBits};
             WaveArgs wa{B.wplans.as<Plan>(), B.wrefs.as<ConflictRefs>(), B.heavy.as<HeavyItem>() + B.heavyOff[ph],
-                        c + 1, B.heavyCap[ph], 0, c + 2, counts, t.dedup, t.rowWords};
-            launch_wave_batch(ph, wa, C.dg, t.waveBlocks, st);
+                        c + 1, B.heavyCap[ph], 0, c + 2, counts, t.dedup, t.rowWords, sink};
+            launch_wave_batch(ph, ph == kPairs ? t.pairsDedup : kDedupRows, wa, C.dg, t.waveBlocks, st);
+            if (sort) {
+                if (t.ev) (void)hipEventRecord(t.ev[6], st);
+                SortJob j{{t.sort->keys[0].as<unsigned long long>(), t.sort->keys[1].as<unsigned long long>()},
+                          c + 3, t.sort->cap, sort_max_tiles(t.sort->cap), t.sort->hist.as<uint32_t>(),
+                          hoBits + bits_for(B.heavyCap[ph]), hoBits, B.heavy.as<HeavyItem>() + B.heavyOff[ph],
+                          B.wplans.as<Plan>(), counts, t.sortBlocks};
+                launch_sort_unique(j, st);
+            }
         }
         if (t.ev) (void)hipEventRecord(t.ev[2 + 2 * ph], st);
     }
@@ -278,8 +334,11 @@ void batch_enqueue(Context& C, Batch& B, const ExecTarget& t) {
     (void)hipMemcpyAsync(B.hCtl, ctl, kCtlWords * 4, hipMemcpyDeviceToHost, st);
 }
 
+int64_t batch_emitted(const Batch& B) { return B.hCtl ? (int64_t)B.hCtl[7] : 0; }
+
 void batch_collect(Batch& B, GpuRuleResult64* out) {
     B.lastHeavy = (int64_t)B.hCtl[1] + B.hCtl[5];
+    B.lastEmitted = batch_emitted(B);
     const bool gated = !(B.flags & GPU_RULES_UNGATED);
     for (int32_t i = 0; i < B.n; ++i) {
         GpuRuleResult64& r = out[i];
@@ -299,51 +358,86 @@ void batch_collect(Batch& B, GpuRuleResult64* out) {
 
 bool batch_run(Context& C, Batch& B, GpuRuleResult64* out) {
     const bool graphMode = C.useGraph;
-    if (graphMode) {
-        // re-capture if the context's scratch or launch shape changed since
-        if (B.exec && (B.capturedDedup != C.dedup.p || B.capturedBlocks[0] != C.threadBlocks() ||
-                       B.capturedBlocks[1] != C.waveBlocks())) {
-            (void)hipGraphExecDestroy(B.exec);
-            (void)hipGraphDestroy(B.graph);
-            B.exec = nullptr;
-            B.graph = nullptr;
-        }
-        if (!B.exec) {
-            B.capturedDedup = C.dedup.p;
-            B.capturedBlocks[0] = C.threadBlocks();
-            B.capturedBlocks[1] = C.waveBlocks();
-            GR_HIP(hipStreamBeginCapture(C.st, hipStreamCaptureModeThreadLocal));
-            batch_enqueue(C, B, ExecTarget{C.st, C.dedup.as<uint32_t>(), C.rowWords, C.waveBlocks(), nullptr});
-            GR_HIP(hipStreamEndCapture(C.st, &B.graph));
-            GR_HIP(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
-        }
-        GR_HIP(hipEventRecord(C.ev[0], C.st));
-        GR_HIP(hipGraphLaunch(B.exec, C.st));
-        GR_HIP(hipEventRecord(C.ev[5], C.st));
-        GR_HIP(hipStreamSynchronize(C.st));
-        float f = 0;
-        (void)hipEventElapsedTime(&f, C.ev[0], C.ev[5]);
-        B.lastMs = f;
-        B.lastSupMs = B.lastPairMs = 0;
-        for (double& k : B.kernelMs) k = 0;
-    } else {
-        batch_enqueue(C, B, ExecTarget{C.st, C.dedup.as<uint32_t>(), C.rowWords, C.waveBlocks(), C.ev});
-        GR_HIP(hipEventRecord(C.ev[5], C.st));
-        GR_HIP(hipStreamSynchronize(C.st));
-        float t[5] = {0, 0, 0, 0, 0};
-        for (int k = 0; k < 4; ++k) (void)hipEventElapsedTime(&t[k], C.ev[k], C.ev[k + 1]);
-        (void)hipEventElapsedTime(&t[4], C.ev[0], C.ev[5]);
-        for (int k = 0; k < 4; ++k) B.kernelMs[k] = t[k];
-        B.lastSupMs = t[0] + t[1];
-        B.lastPairMs = t[2] + t[3];
-        B.lastMs = t[4];
+    SortBufs* sb = nullptr;
+    const int mode = C.pairsDedup();
+    if (mode == kDedupSort && B.nChunks[kPairs]) {
+        if (!C.sortb.ensure(1)) return false;
+        sb = &C.sortb;
     }
-    GR_HIP(hipGetLastError());
+    for (;;) {
+        if (graphMode) {
+            // re-capture if the context's scratch or launch shape changed since
+            const void* sp = sb ? sb->keys[0].p : nullptr;
+            const uint32_t sc = sb ? sb->cap : 0;
+            if (B.exec && (B.capturedDedup != C.dedup.p || B.capturedBlocks[0] != C.threadBlocks() ||
+                           B.capturedBlocks[1] != C.waveBlocks() || B.capturedSort != sp || B.capturedSortCap != sc ||
+                           B.capturedMode != mode)) {
+                (void)hipGraphExecDestroy(B.exec);
+                (void)hipGraphDestroy(B.graph);
+                B.exec = nullptr;
+                B.graph = nullptr;
+            }
+            if (!B.exec) {
+                B.capturedDedup = C.dedup.p;
+                B.capturedBlocks[0] = C.threadBlocks();
+                B.capturedBlocks[1] = C.waveBlocks();
+                B.capturedSort = sp;
+                B.capturedSortCap = sc;
+                B.capturedMode = mode;
+                GR_HIP(hipStreamBeginCapture(C.st, hipStreamCaptureModeThreadLocal));
+                batch_enqueue(C, B, ExecTarget{C.st, C.dedup.as<uint32_t>(), C.rowWords, C.waveBlocks(), nullptr, sb,
+                                               C.sortBlocks(), mode});
+                GR_HIP(hipStreamEndCapture(C.st, &B.graph));
+                GR_HIP(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
+            }
+            GR_HIP(hipEventRecord(C.ev[0], C.st));
+            GR_HIP(hipGraphLaunch(B.exec, C.st));
+            GR_HIP(hipEventRecord(C.ev[5], C.st));
+            GR_HIP(hipStreamSynchronize(C.st));
+            float f = 0;
+            (void)hipEventElapsedTime(&f, C.ev[0], C.ev[5]);
+            B.lastMs = f;
+            B.lastSupMs = B.lastPairMs = B.lastSortMs = 0;
+            for (double& k : B.kernelMs) k = 0;
+        } else {
+            batch_enqueue(C, B, ExecTarget{C.st, C.dedup.as<uint32_t>(), C.rowWords, C.waveBlocks(), C.ev, sb,
+                                           C.sortBlocks(), mode});
+            GR_HIP(hipEventRecord(C.ev[5], C.st));
+            GR_HIP(hipStreamSynchronize(C.st));
+            float t[5] = {0, 0, 0, 0, 0};
+            for (int k = 0; k < 4; ++k) (void)hipEventElapsedTime(&t[k], C.ev[k], C.ev[k + 1]);
+            (void)hipEventElapsedTime(&t[4], C.ev[0], C.ev[5]);
+            B.lastSortMs = 0;
+            if (sb) {   // pairs wave kernel alone, then the sort-unique pass
+                float w = 0, so = 0;
+                (void)hipEventElapsedTime(&w, C.ev[3], C.ev[6]);
+                (void)hipEventElapsedTime(&so, C.ev[6], C.ev[4]);
+                t[3] = w;
+                B.lastSortMs = so;
+            }
+            for (int k = 0; k < 4; ++k) B.kernelMs[k] = t[k];
+            B.lastSupMs = t[0] + t[1];
+            B.lastPairMs = t[2] + t[3] + (float)B.lastSortMs;
+            B.lastMs = t[4];
+        }
+        GR_HIP(hipGetLastError());
+        if (sb && batch_emitted(B) > (int64_t)sb->cap) {   // keys overflowed: larger buffer, run again
+            if (sb->cap >= kSortCapMax) {
+                std::fprintf(stderr, "[gpu_rules] body-pair keys of one batch exceed %u\n", kSortCapMax);
+                return false;
+            }
+            if (!sb->ensure((uint64_t)batch_emitted(B))) return false;
+            continue;
+        }
+        break;
+    }
     batch_collect(B, out);
     C.lastMs = B.lastMs;
     C.lastSupMs = B.lastSupMs;
     C.lastPairMs = B.lastPairMs;
+    C.lastSortMs = B.lastSortMs;
     C.lastHeavy = B.lastHeavy;
+    C.lastEmitted = sb ? B.lastEmitted : 0;
     for (int k = 0; k < 4; ++k) C.lastKernelMs[k] = B.kernelMs[k];
     return true;
 }
@@ -709,6 +803,15 @@ int gpu_autotune(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags, 
     }
 }
 
+int gpu_last_dedup_stats(void* ctx, double* sortMs, int64_t* keysEmitted, int64_t* sortCap) {
+    Context* C = as_ctx(ctx);
+    if (!C) return -1;
+    if (sortMs) *sortMs = C->lastSortMs;
+    if (keysEmitted) *keysEmitted = C->lastEmitted;
+    if (sortCap) *sortCap = C->sortb.cap;
+    return 0;
+}
+
 int gpu_last_kernel_ms(void* ctx, double* ms4) {
     Context* C = as_ctx(ctx);
     if (!C || !ms4) return -1;
@@ -764,7 +867,18 @@ int gpu_set_tuning(void* ctx, const char* key, int64_t value) {
         C->lastBatch = nullptr;
         if (!C->ensure_dedup()) return -1;
     } else if (k == "use_graph") C->useGraph = value != 0;
-    else return -1;
+    else if (k == "dedup") {
+        if (value < 0 || value > 3) return -1;
+        C->dedupMode = (int)value;
+        delete C->lastBatch;
+        C->lastBatch = nullptr;
+        if (!C->ensure_dedup()) return -1;
+    } else if (k == "sort_cap") {
+        // start the sort-unique buffers at this many keys (they still grow on overflow)
+        if (value < kSortTile || value > (int64_t)kSortCapMax) return -1;
+        C->sortb.release();
+        if (!C->sortb.ensure((uint64_t)value, (uint64_t)value)) return -1;
+    } else return -1;
     if (C->lastBatch) { delete C->lastBatch; C->lastBatch = nullptr; }
     return 0;
 }
@@ -779,6 +893,10 @@ int64_t gpu_get_tuning(void* ctx, const char* key) {
     if (k == "use_graph") return C->useGraph ? 1 : 0;
     if (k == "num_cus") return C->numCUs;
     if (k == "wave_slots") return C->waveSlots;
+    if (k == "dedup") return C->dedupMode;
+    if (k == "sort_active") return C->sortActive() ? 1 : 0;
+    if (k == "pairs_dedup") return C->pairsDedup();
+    if (k == "sort_cap") return C->sortb.cap;
     return -1;
 }
 

@@ -14,6 +14,7 @@
 #include "kernels.hpp"
 #include "planner.hpp"
 #include "rules_host.hpp"
+#include "sort_unique.hpp"
 
 namespace gr {
 
@@ -71,6 +72,37 @@ struct DevBuf {
     template <class T> T* as() const { return static_cast<T*>(p); }
 };
 
+// Buffers of the sort-unique dedup path: two key arrays (LSD ping-pong) of
+// `cap` keys each and the radix histogram. The capacity is a power of two;
+// it doubles (at least) whenever a pass emitted more keys than it holds, up
+// to kSortCapMax keys (32 GiB of key arrays -- sized for a 288 GB device).
+constexpr uint32_t kSortCapMin = 1u << 22;
+constexpr uint32_t kSortCapMax = 1u << 31;
+struct SortBufs {
+    DevBuf keys[2], hist;
+    uint32_t cap = 0;
+    // Capacity for at least `want` keys (clamped to kSortCapMax); the first
+    // allocation holds at least `first` keys.
+    bool ensure(uint64_t want, uint64_t first = kSortCapMin) {
+        uint64_t c = cap ? cap : kSortTile;
+        while ((c < want || (!cap && c < first)) && c < kSortCapMax) c <<= 1;
+        if (c == cap) return true;
+        if (!keys[0].ensure(c * 8) || !keys[1].ensure(c * 8) || !hist.ensure(sort_hist_words((uint32_t)c) * 4)) {
+            release();
+            return false;
+        }
+        cap = (uint32_t)c;
+        return true;
+    }
+    void release() {
+        keys[0].release();
+        keys[1].release();
+        hist.release();
+        cap = 0;
+    }
+    size_t bytes() const { return keys[0].bytes + keys[1].bytes + hist.bytes; }
+};
+
 struct Context;
 
 // A compiled, device-resident batch of rules.
@@ -94,9 +126,13 @@ struct Batch : Handle {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     const void* capturedDedup = nullptr;   // context scratch the graph was captured with
+    const void* capturedSort = nullptr;
+    uint32_t capturedSortCap = 0;
+    int capturedMode = -1;
     int capturedBlocks[2] = {0, 0};
-    double lastMs = 0, lastSupMs = 0, lastPairMs = 0;
+    double lastMs = 0, lastSupMs = 0, lastPairMs = 0, lastSortMs = 0;
     int64_t lastHeavy = 0;
+    int64_t lastEmitted = 0;   // sort path: keys the last pass emitted
     ~Batch();
 };
 
@@ -109,8 +145,12 @@ struct Context : Handle {
     int threadWavesPerCU = 16;
     int waveWavesPerCU = 8;
     bool useGraph = true;
+    int dedupMode = 0;        // body-pair dedup: 0 auto, 1 HBM bitmap rows, 2 sort-unique, 3 LDS bitmap
+    bool rowsFit = true;      // HBM bitmap rows fit kRowBudget at one wave per CU
+    bool ldsFits = false;     // one nE-bit bitmap fits a wave-mode block's LDS
+    size_t ldsPerBlock = 65536, ldsPerCU = 65536;
     hipStream_t st = nullptr;
-    hipEvent_t ev[6] = {};
+    hipEvent_t ev[8] = {};
     DevBuf blocks, rowStart, col, coo, keys;
     DevGraph dg{};
     // per-rule launch scratch
@@ -122,13 +162,23 @@ struct Context : Handle {
     DevBuf dedup;
     int64_t rowWords = 0;
     int waveSlots = 0;
+    SortBufs sortb;   // sort-unique dedup buffers (allocated when that path is used)
     Batch* lastBatch = nullptr;
-    double lastMs = 0, lastSupMs = 0, lastPairMs = 0;
+    double lastMs = 0, lastSupMs = 0, lastPairMs = 0, lastSortMs = 0;
     double lastKernelMs[4] = {};
     int64_t lastHeavy = 0;
+    int64_t lastEmitted = 0;
 
-    int waveBlocks() const { return std::max(1, waveSlots / 4); }
+    int waveBlocks() const { return std::max(1, waveSlots); }   // one wave per wave-mode block
     int threadBlocks() const { return std::max(1, numCUs * threadWavesPerCU / 4); }
+    int sortBlocks() const { return std::max(1, numCUs * 4); }
+    bool sortActive() const { return dedupMode == 2 || (dedupMode == 0 && !rowsFit && !ldsFits); }
+    // DedupMode of the body-pair wave-mode kernels
+    int pairsDedup() const {
+        if (sortActive()) return kDedupSort;
+        if ((dedupMode == 0 || dedupMode == 3) && ldsFits) return kDedupLds;
+        return kDedupRows;
+    }
 
     bool upload();
     bool ensure_dedup();
@@ -148,7 +198,7 @@ inline Batch::~Batch() {
 inline Context::~Context() {
     if (st) (void)hipStreamSynchronize(st);
     blocks.release(); rowStart.release(); col.release(); coo.release(); keys.release();
-    one.release(); heavy1.release(); dedup.release();
+    one.release(); heavy1.release(); dedup.release(); sortb.release();
     if (hOne) (void)hipHostFree(hOne);
     for (auto& e : ev) if (e) (void)hipEventDestroy(e);
     if (st) (void)hipStreamDestroy(st);
@@ -161,15 +211,26 @@ inline bool put(DevBuf& b, const std::vector<T>& v, hipStream_t st) {
     return true;
 }
 
-// Where a batch's device pass runs: stream, wave-mode dedup rows, and
-// (optionally) six timing events.
+// Where a batch's device pass runs: stream, wave-mode dedup rows or (when
+// `sort` is set) the sort-unique buffers, and (optionally) timing events
+// ev[0..4] around the four kernels, ev[6] before the sort-unique pass.
 struct ExecTarget {
     hipStream_t st;
     uint32_t* dedup;
     int64_t rowWords;
     int waveBlocks;
     hipEvent_t* ev;   // nullptr: untimed
+    SortBufs* sort;   // the sort-unique buffers (pairsDedup == kDedupSort)
+    int sortBlocks;
+    int pairsDedup;   // DedupMode of the body-pair wave-mode kernel
 };
+
+// Bytes of bitmap rows allowed before body-pair dedup switches to the
+// sort-unique path (auto mode).
+constexpr int64_t kRowBudget = int64_t(8) << 30;
+
+// Keys emitted by B's last pass (sort path), from its host control copy.
+int64_t batch_emitted(const Batch& B);
 
 // Compiles rules into B and uploads plans and chunk lists on stream st.
 bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const std::vector<int32_t>& status,

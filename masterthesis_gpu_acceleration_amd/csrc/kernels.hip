@@ -16,10 +16,13 @@
 //     appended to the heavy queue and its partial work discarded.
 //   * Wave mode: one wave64 per heavy seed. Lanes take level-0 entries one at
 //     a time from a shared LDS cursor and search below them on their own.
-//     SUPPORT seeds stop as soon as __any shows a complete grounding; PAIRS
-//     seeds mark head objects in a per-wave bitmap row; the words that were
-//     set are listed in LDS, so counting (popcount) and clearing the row
-//     touch only those words.
+//     SUPPORT seeds stop as soon as __any shows a complete grounding. PAIRS
+//     seeds dedup their head objects one of two ways: (rows) a per-wave
+//     bitmap row whose set words are listed in LDS, so counting (popcount)
+//     and clearing touch only those words; (sort) keys (seed, object)
+//     appended to an HBM buffer -- with a small LDS cache of objects already
+//     emitted for the seed, to prune repeats -- and counted afterwards by the
+//     radix sort + unique pass of sort_unique.hip.
 //
 // Semantics, as the reference CPU counters define them
 // (Rewritten_CPP/SupportCounting.cpp:299-479, CPU_01_nonparallel/
@@ -41,22 +44,34 @@ constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr int kPlanWords = (int)(sizeof(Plan) / 4);
 
 constexpr int kTouchCap = 256;      // dedup-row words listed per wave-mode seed
+constexpr int kCacheSlots = 512;    // sort path: LDS cache of emitted head objects per wave (direct mapped)
 constexpr int kOneChunkSeeds = 64;  // seeds per wave in per-rule launches (one rule fills the GPU)
+constexpr int kLeafBatch = 8;       // last-level candidates a lane tries per step of its own search (wave mode)
+constexpr int kLeafBatchT = 4;      // the same in thread mode (keeps it within 128 VGPRs: 4 waves per SIMD)
 
 struct Frame {
     int i;     // next entry
     int end;   // one past the last entry
 };
 
-// Per-wave LDS: the plan being evaluated, one frame per (level, lane), and
-// the wave-mode seed state (level-0 cursor, touched dedup-row words).
-struct WaveLds {
+// Per-wave LDS of thread mode: the plan being evaluated and one frame per
+// (level, lane).
+struct ThreadLds {
     Plan plan;
     Frame frame[kMaxBody][kWave];
-    int next0;
-    int end0;
+};
+
+// Wave mode adds the seed's shared pool -- a frame at level poolLevel whose
+// entries [poolNext, poolEnd) any idle lane may take, with the bindings
+// poolV that hold above that level -- and the dedup words touched so far.
+struct WaveLds : ThreadLds {
+    int poolV[kMaxSlots];
+    int poolLevel;
+    int poolNext;
+    int poolEnd;
+    uint32_t poolGen;
     uint32_t nTouched;
-    uint32_t pad;
+    uint32_t pad[3];
     uint32_t touched[kTouchCap];
 };
 
@@ -127,26 +142,6 @@ struct RegSet {
     }
 };
 
-// Distinct head objects of one seed, one bit per term id (wave mode). The
-// load bypasses the vector L1 so it sees the other lanes' atomics. The lane
-// whose atomic finds a word still zero lists that word in LDS.
-struct BitRow {
-    uint32_t* row;
-    WaveLds* W;
-    __device__ __forceinline__ bool seen(int x) const {
-        const uint32_t w = __hip_atomic_load(row + (x >> 5), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        return (w >> (x & 31)) & 1u;
-    }
-    __device__ __forceinline__ bool insert(int x) {
-        const uint32_t old = atomicOr(row + (x >> 5), 1u << (x & 31));
-        if (old == 0) {
-            const uint32_t k = atomicAdd(&W->nTouched, 1u);
-            if (k < (uint32_t)kTouchCap) W->touched[k] = (uint32_t)(x >> 5);
-        }
-        return true;
-    }
-};
-
 // Entry range of a level under the current bindings.
 __device__ __forceinline__ Frame enter_level(const Level& lv, const DevGraph& G, const int (&v)[kMaxSlots]) {
     const int kind = level_kind(lv.flags);
@@ -166,12 +161,12 @@ __device__ __forceinline__ Frame enter_level(const Level& lv, const DevGraph& G,
 
 // Binds entry i of a level. False if the candidate violates injectivity or
 // repeats an earlier ground triple.
-__device__ __forceinline__ bool bind_entry(const Level& lv, const DevGraph& G, const ConflictRefs* R,
-                                           int (&v)[kMaxSlots], int i) {
+// x is col[i], loaded by the caller.
+__device__ __forceinline__ bool bind_entry_x(const Level& lv, const DevGraph& G, const ConflictRefs* R,
+                                             int (&v)[kMaxSlots], int i, int x) {
     const int f = lv.flags;
     const int kind = level_kind(f);
     const int nbound = level_nbound(f);
-    const int x = G.col[i];
     int k = 0;
     if (kind == kExt) {
         if (taken(v, nbound, x)) return false;
@@ -194,6 +189,11 @@ __device__ __forceinline__ bool bind_entry(const Level& lv, const DevGraph& G, c
         if (repeats(R, v, conf, spo ? k : x, spo ? x : k)) return false;
     }
     return true;
+}
+
+__device__ __forceinline__ bool bind_entry(const Level& lv, const DevGraph& G, const ConflictRefs* R,
+                                           int (&v)[kMaxSlots], int i) {
+    return bind_entry_x(lv, G, R, v, i, G.col[i]);
 }
 
 // Binds the head of a SUPPORT seed; false if the head triple cannot match.
@@ -226,7 +226,7 @@ __device__ __forceinline__ bool bind_seed(const Plan& P, const DevGraph& G, int 
     return true;
 }
 
-__device__ __forceinline__ void copy_plan(WaveLds& W, const Plan* src) {
+__device__ __forceinline__ void copy_plan(ThreadLds& W, const Plan* src) {
     const int32_t* s = reinterpret_cast<const int32_t*>(src);
     int32_t* d = reinterpret_cast<int32_t*>(&W.plan);
     for (int w = lane_id(); w < kPlanWords; w += kWave) d[w] = s[w];
@@ -270,7 +270,7 @@ __device__ __forceinline__ void push_heavy(bool heavy, uint32_t plan, uint32_t i
 // Returns this lane's count; escalated seeds go to the heavy queue. K is the
 // plan kind of every plan the launch sees (kSupport or kPairs).
 template <int K>
-__device__ __forceinline__ uint64_t thread_chunk(WaveLds& W, const DevGraph& G, const ConflictRefs* R,
+__device__ __forceinline__ uint64_t thread_chunk(ThreadLds& W, const DevGraph& G, const ConflictRefs* R,
                                                  uint32_t planIdx, uint32_t begin, uint32_t count, HeavyItem* hq,
                                                  uint32_t* hCount, uint32_t hCap) {
     const int lane = lane_id();
@@ -317,6 +317,40 @@ __device__ __forceinline__ uint64_t thread_chunk(WaveLds& W, const DevGraph& G, 
                 } else {
                     --d;
                 }
+            } else if (d + 1 == nlev && fr.end - fr.i > 1) {
+                // last level: up to kLeafBatchT candidates per step, loaded together
+                const int take = fr.end - fr.i < kLeafBatchT ? fr.end - fr.i : kLeafBatchT;
+                W.frame[d][lane].i = fr.i + take;
+                budget -= take;
+                if (budget < 0) {
+                    heavyNow = true;
+                    active = false;
+                } else {
+                    int xs[kLeafBatchT];
+#pragma unroll
+                    for (int k = 0; k < kLeafBatchT; ++k) xs[k] = k < take ? G.col[fr.i + k] : 0;
+                    bool stop = false;
+#pragma unroll
+                    for (int k = 0; k < kLeafBatchT; ++k) {
+                        if (stop || k >= take || !bind_entry_x(P.lev[d], G, R, v, fr.i + k, xs[k])) continue;
+                        if (!pairs) {
+                            mine += 1;
+                            active = false;
+                            stop = true;
+                            continue;
+                        }
+                        const int ho = term_value(v, P.hoRef);
+                        if (d == pl && set.seen(ho)) continue;
+                        if (!set.insert(ho)) {
+                            heavyNow = true;
+                            active = false;
+                            stop = true;
+                        } else if (d != pl) {
+                            d = pl;   // head object settled: next candidate at the pair level
+                            stop = true;
+                        }
+                    }
+                }
             } else {
                 W.frame[d][lane].i = fr.i + 1;
                 if (--budget < 0) {
@@ -353,87 +387,312 @@ __device__ __forceinline__ uint64_t thread_chunk(WaveLds& W, const DevGraph& G, 
 
 // Wave mode: the whole wave on one seed. Level-0 entries are handed out one
 // at a time from an LDS cursor.
-template <int K>
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+// Head-object dedup of one PAIRS seed in wave mode.
+//   kDedupRows: the wave's bitmap row of nE bits in HBM (atomicOr);
+//   kDedupLds:  the same bitmap in the wave's LDS;
+//   kDedupSort: nothing is marked; the caller emits keys to HBM, and a small
+//               direct-mapped LDS cache of emitted objects prunes repeats.
+// For the bitmaps, the lane whose atomicOr finds a word still zero lists it
+// in LDS so that clearing touches only those words; insert returns 1 to the
+// one lane whose call set the bit, so the lanes' sums are the distinct count.
+template <int D>
+struct Dedup {
+    uint32_t* bits;
+    WaveLds* W;
+    int* cache;
+    __device__ __forceinline__ bool seen(int x) const {
+        if (D == kDedupSort) return cache[x & (kCacheSlots - 1)] == x;
+        uint32_t w;
+        if (D == kDedupRows) w = __hip_atomic_load(bits + (x >> 5), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else w = bits[x >> 5];
+        return (w >> (x & 31)) & 1u;
+    }
+    __device__ __forceinline__ uint32_t insert(int x) {
+        if (D == kDedupSort) {
+            cache[x & (kCacheSlots - 1)] = x;
+            return 0;
+        }
+        const uint32_t bit = 1u << (x & 31);
+        const uint32_t old = atomicOr(bits + (x >> 5), bit);
+        if (old == 0) {
+            const uint32_t k = atomicAdd(&W->nTouched, 1u);
+            if (k < (uint32_t)kTouchCap) W->touched[k] = (uint32_t)(x >> 5);
+        }
+        return (old & bit) ? 0u : 1u;
+    }
+    __device__ __forceinline__ void clear_word(uint32_t w) {
+        if (D == kDedupRows) atomicAnd(bits + w, 0u);
+        else bits[w] = 0u;
+    }
+};
+
+// Sort path: wave-aggregated append of this step's emitted head objects as
+// keys keyHi | object. Called by the whole wave.
+__device__ __forceinline__ void emit_keys(const SortSink& sink, unsigned long long keyHi, int emit) {
+    const uint64_t em = __ballot(emit >= 0);
+    if (em == 0) return;
+    uint32_t base = 0;
+    if (lane_id() == 0) base = atomicAdd(sink.count, (uint32_t)__popcll(em));
+    base = __shfl(base, 0, kWave);
+    if (emit >= 0) {
+        const uint32_t pos = base + (uint32_t)__popcll(em & ((1ull << lane_id()) - 1ull));
+        if (pos < sink.cap) sink.keys[pos] = keyHi | (unsigned long long)(uint32_t)emit;
+    }
+}
+
+constexpr int kLeafUnroll = 8;   // last-level pool entries an idle lane takes per draw
+
+// Wave mode: the whole wave on one seed, with a shared pool of work.
+//   * The pool starts as the seed's level-0 row. Each step, the idle lanes
+//     draw consecutive pool entries (coalesced): from a last-level pool each
+//     takes kLeafUnroll entries and finishes them on the spot; from any other
+//     level each takes one entry, binds it on top of the pool's bindings and
+//     searches below it on its own (frames in LDS), as in thread mode.
+//   * When the pool is empty and some lanes are idle, the busy lane with the
+//     most untried entries in its shallowest open frame donates the rest of
+//     that frame (with its bindings) as the new pool. Long rows at any depth
+//     -- hub rows -- are thus spread over the wave instead of scanned by one
+//     lane.
+//   * SUPPORT seeds end at the first complete grounding (__any). PAIRS seeds
+//     count each head object once (Dedup); a lane whose head object is
+//     already counted drops the rest of that subtree.
+// Returns the seed's count (SUPPORT: 0/1; PAIRS: distinct head objects, or 0
+// on the sort path, which counts later).
+template <int K, int D>
 __device__ __forceinline__ uint64_t wave_seed(WaveLds& W, const DevGraph& G, const ConflictRefs* R, uint32_t item,
-                                              uint32_t* row, int64_t rowWords) {
+                                              Dedup<D>& dd, int64_t rowWords, const SortSink& sink,
+                                              unsigned long long keyHi) {
     const int lane = lane_id();
     const Plan& P = W.plan;
     const int nlev = P.nlev;
+    const int last = nlev - 1;
     const bool pairs = K == kPairs && P.pairLevel >= 0;
     const int pl = P.pairLevel;
+    const uint64_t below = (1ull << lane) - 1ull;
     int v[kMaxSlots];
     if (!bind_seed(P, G, (int)item, v)) return 0;
     const Frame f0 = enter_level(P.lev[0], G, v);
     if (f0.i >= f0.end) return 0;
+    if (lane < kMaxSlots) W.poolV[lane] = term_value(v, lane);
     if (lane == 0) {
-        W.next0 = f0.i;
-        W.end0 = f0.end;
+        W.poolLevel = 0;
+        W.poolNext = f0.i;
+        W.poolEnd = f0.end;
+        W.poolGen = W.poolGen + 1u;
         W.nTouched = 0;
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const int vs[kMaxSlots] = {v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7],
-                               v[8], v[9], v[10], v[11], v[12], v[13], v[14], v[15]};
-    BitRow br{row, &W};
-    int d = 0;
-    bool active = true, found = false;
-    // take the first level-0 entry
-    {
-        const int i0 = atomicAdd(&W.next0, 1);
-        if (i0 < f0.end) W.frame[0][lane] = Frame{i0, i0 + 1};
-        else active = false;
-    }
-    while (__any(active)) {
-        if (active) {
-            const Frame fr = W.frame[d][lane];
-            if (fr.i >= fr.end) {
-                if (d == 0) {
-                    const int i0 = atomicAdd(&W.next0, 1);
-                    if (i0 < f0.end) {
+    if (D == kDedupSort && pairs)
+        for (int k = lane; k < kCacheSlots; k += kWave) dd.cache[k] = -1;
+    wave_sync();
+    bool busy = false, found = false;
+    int d = 0, base = 0;            // own search: current level, first level it owns
+    uint32_t gen = 0xffffffffu;     // pool whose bindings v holds (idle lanes)
+    uint64_t mine = 0;
+    for (;;) {
+        // ---- A: idle lanes draw from the pool
+        const uint64_t idleM = __ballot(!busy);
+        const int pn = uniform(W.poolNext), pe = uniform(W.poolEnd), plv = uniform(W.poolLevel);
+        const uint32_t pg = (uint32_t)uniform((int)W.poolGen);
+        if (idleM != 0 && pn < pe) {
+            const int n = __popcll(idleM);
+            const int r = __popcll(idleM & below);
+            int next = pe;
+            bool drop = false;
+            if (pairs && pl < plv) {   // every entry shares a head object bound above the pool
+                const int hr = P.hoRef;
+                const int hov = hr >= 0 ? W.poolV[hr] : -(hr + 1);
+                drop = dd.seen(uniform(hov));
+            }
+            if (!drop) {
+                if (!busy && gen != pg) {
 #pragma unroll
-                        for (int j = 0; j < kMaxSlots; ++j) v[j] = vs[j];
-                        W.frame[0][lane] = Frame{i0, i0 + 1};
-                    } else {
-                        active = false;
+                    for (int j = 0; j < kMaxSlots; ++j) v[j] = W.poolV[j];
+                    gen = pg;
+                }
+                if (plv == last) {
+                    next = pn + n * kLeafUnroll < pe ? pn + n * kLeafUnroll : pe;
+                    int xs[kLeafUnroll];
+#pragma unroll
+                    for (int k = 0; k < kLeafUnroll; ++k) {
+                        const int i = pn + r + n * k;
+                        xs[k] = (!busy && i < pe) ? G.col[i] : 0;
+                    }
+                    bool hit = false;
+#pragma unroll
+                    for (int k = 0; k < kLeafUnroll; ++k) {
+                        const int i = pn + r + n * k;
+                        int emit = -1;
+                        if (!busy && i < pe && bind_entry_x(P.lev[last], G, R, v, i, xs[k])) {
+                            if (!pairs) {
+                                found = true;
+                            } else {
+                                const int ho = term_value(v, P.hoRef);
+                                hit = true;
+                                if (!dd.seen(ho)) {
+                                    mine += dd.insert(ho);
+                                    emit = ho;
+                                }
+                            }
+                        }
+                        if (D == kDedupSort && pairs) emit_keys(sink, keyHi, emit);
+                    }
+                    // head object bound above the pool and now counted: the rest adds nothing
+                    if (pairs && pl < last && __any(hit)) next = pe;
+                } else {
+                    next = pn + n < pe ? pn + n : pe;
+                    const int i = pn + r;
+                    if (!busy && i < pe) {
+                        bool ok = bind_entry(P.lev[plv], G, R, v, i);
+                        if (ok && pairs && plv == pl) ok = !dd.seen(term_value(v, P.hoRef));
+                        if (ok) {
+                            d = base = plv + 1;
+                            W.frame[d][lane] = enter_level(P.lev[d], G, v);
+                            busy = true;
+                        }
+                    }
+                }
+            }
+            wave_sync();
+            if (lane == 0) W.poolNext = next;
+        }
+        if (!pairs && __any(found)) return 1;
+        // ---- B: busy lanes advance their own search by one step
+        int emit = -1;
+        int emits[kLeafBatch];
+#pragma unroll
+        for (int k = 0; k < kLeafBatch; ++k) emits[k] = -1;
+        if (busy) {
+            if (pairs && pl < base && dd.seen(term_value(v, P.hoRef))) {
+                busy = false;   // head object counted by another lane
+            } else {
+                const Frame fr = W.frame[d][lane];
+                if (fr.i >= fr.end) {
+                    if (d == base) busy = false;
+                    else --d;
+                } else if (d == last && fr.end - fr.i > 1) {
+                    // last level: up to kLeafBatch candidates per step, loaded together
+                    const int take = fr.end - fr.i < kLeafBatch ? fr.end - fr.i : kLeafBatch;
+                    W.frame[d][lane].i = fr.i + take;
+                    int xs[kLeafBatch];
+#pragma unroll
+                    for (int k = 0; k < kLeafBatch; ++k) xs[k] = k < take ? G.col[fr.i + k] : 0;
+                    bool stop = false;
+#pragma unroll
+                    for (int k = 0; k < kLeafBatch; ++k) {
+                        if (stop || k >= take || !bind_entry_x(P.lev[d], G, R, v, fr.i + k, xs[k])) continue;
+                        if (!pairs) {
+                            found = true;
+                            busy = false;
+                            stop = true;
+                            continue;
+                        }
+                        const int ho = term_value(v, P.hoRef);
+                        if (d == pl) {
+                            if (!dd.seen(ho)) {
+                                mine += dd.insert(ho);
+                                emits[k] = ho;
+                            }
+                        } else {
+                            mine += dd.insert(ho);
+                            emits[k] = ho;
+                            if (pl >= base) d = pl;   // head object settled: next candidate at the pair level
+                            else busy = false;
+                            stop = true;
+                        }
                     }
                 } else {
-                    --d;
-                }
-            } else {
-                W.frame[d][lane].i = fr.i + 1;
-                bool ok = bind_entry(P.lev[d], G, R, v, fr.i);
-                if (ok && pairs && d == pl) ok = !br.seen(term_value(v, P.hoRef));
-                if (ok) {
-                    if (d + 1 == nlev) {
-                        if (pairs) {
-                            br.insert(term_value(v, P.hoRef));
-                            d = pl;
+                    W.frame[d][lane].i = fr.i + 1;
+                    bool ok = bind_entry(P.lev[d], G, R, v, fr.i);
+                    if (ok && pairs && d == pl) ok = !dd.seen(term_value(v, P.hoRef));
+                    if (ok) {
+                        if (d == last) {
+                            if (pairs) {
+                                const int ho = term_value(v, P.hoRef);
+                                mine += dd.insert(ho);
+                                emit = ho;
+                                if (pl >= base) d = pl;   // head object settled: next candidate at the pair level
+                                else busy = false;
+                            } else {
+                                found = true;
+                                busy = false;
+                            }
                         } else {
-                            found = true;
-                            active = false;
+                            ++d;
+                            W.frame[d][lane] = enter_level(P.lev[d], G, v);
                         }
-                    } else {
-                        ++d;
-                        W.frame[d][lane] = enter_level(P.lev[d], G, v);
                     }
                 }
             }
         }
+        if (D == kDedupSort && pairs) {
+            emit_keys(sink, keyHi, emit);
+#pragma unroll
+            for (int k = 0; k < kLeafBatch; ++k) emit_keys(sink, keyHi, emits[k]);
+        }
         if (!pairs && __any(found)) return 1;
+        // ---- C: done, or donate work to the idle lanes
+        wave_sync();
+        const uint64_t busyM = __ballot(busy);
+        const bool poolEmpty = uniform(W.poolNext) >= uniform(W.poolEnd);
+        if (busyM == 0 && poolEmpty) break;
+        if (busyM != ~0ull && poolEmpty) {
+            int rem = 0, lvl = 0;
+            if (busy) {
+                for (int l = base; l <= d; ++l) {
+                    const Frame f = W.frame[l][lane];
+                    if (f.end > f.i) {
+                        rem = f.end - f.i;
+                        lvl = l;
+                        break;
+                    }
+                }
+            }
+            uint64_t key = ((uint64_t)(uint32_t)rem << 8) | (uint64_t)lane;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const uint64_t o = __shfl_xor(key, off, kWave);
+                key = o > key ? o : key;
+            }
+            const int donor = (int)(key & 255u);
+            if ((key >> 8) != 0) {
+                if (lane == donor) {
+                    const Frame f = W.frame[lvl][lane];
+                    W.poolLevel = lvl;
+                    W.poolNext = f.i;
+                    W.poolEnd = f.end;
+                    W.poolGen = W.poolGen + 1u;
+#pragma unroll
+                    for (int j = 0; j < kMaxSlots; ++j) W.poolV[j] = v[j];
+                    W.frame[lvl][lane].end = f.i;
+                }
+                wave_sync();
+            }
+        }
     }
-    if (!pairs) return 0;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    const uint32_t nt = W.nTouched;
-    uint64_t c = 0;
-    if (nt <= (uint32_t)kTouchCap) {
-        for (uint32_t k = lane; k < nt; k += kWave) c += (uint64_t)__popc(atomicAnd(row + W.touched[k], 0u));
+    if (!pairs || D == kDedupSort) return 0;
+    if (D == kDedupRows) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     } else {
-        for (int64_t w = lane; w < rowWords; w += kWave) c += (uint64_t)__popc(atomicAnd(row + w, 0u));
+        wave_sync();
     }
-    return wave_sum(c);
+    const uint32_t nt = W.nTouched;
+    if (nt <= (uint32_t)kTouchCap) {
+        for (uint32_t k = lane; k < nt; k += kWave) dd.clear_word(W.touched[k]);
+    } else {
+        for (int64_t w = lane; w < rowWords; w += kWave) dd.clear_word((uint32_t)w);
+    }
+    if (D == kDedupLds) wave_sync();
+    return wave_sum(mine);
 }
 
 // ------------------------------------------------------------- kernels
@@ -442,8 +701,8 @@ __device__ __forceinline__ uint64_t wave_seed(WaveLds& W, const DevGraph& G, con
 // seeds of one plan, one per lane.
 template <int K>
 __global__ void __launch_bounds__(kBlock) k_thread_batch(ThreadArgs a, DevGraph G) {
-    __shared__ WaveLds lds[kWavesPerBlock];
-    WaveLds& W = lds[wave_in_block()];
+    __shared__ ThreadLds lds[kWavesPerBlock];
+    ThreadLds& W = lds[wave_in_block()];
     const int lane = lane_id();
     uint32_t loaded = 0xffffffffu;
     for (;;) {
@@ -466,8 +725,8 @@ __global__ void __launch_bounds__(kBlock) k_thread_batch(ThreadArgs a, DevGraph 
 // waves stride over its seeds.
 template <int K>
 __global__ void __launch_bounds__(kBlock) k_thread_one(Plan P, OneArgs a, DevGraph G) {
-    __shared__ WaveLds lds[kWavesPerBlock];
-    WaveLds& W = lds[wave_in_block()];
+    __shared__ ThreadLds lds[kWavesPerBlock];
+    ThreadLds& W = lds[wave_in_block()];
     const int lane = lane_id();
     copy_plan(W, &P);
     const uint32_t waves = (gridDim.x * blockDim.x) / kWave;
@@ -480,14 +739,34 @@ __global__ void __launch_bounds__(kBlock) k_thread_one(Plan P, OneArgs a, DevGra
     }
 }
 
-// Persistent wave mode over the heavy queue.
-template <int K>
-__global__ void __launch_bounds__(kBlock) k_wave_batch(WaveArgs a, DevGraph G) {
-    __shared__ WaveLds lds[kWavesPerBlock];
-    WaveLds& W = lds[wave_in_block()];
+// Dedup state of one wave-mode wave: D selects the bitmap in HBM (this
+// wave's row), the bitmap in LDS (dynamic shared memory, cleared once here)
+// or the sort path's LDS cache.
+template <int D>
+__device__ __forceinline__ Dedup<D> wave_dedup(WaveLds& W, int* cache, uint32_t* dyn, const WaveArgs& a) {
+    Dedup<D> dd;
+    dd.W = &W;
+    dd.cache = cache;
+    dd.bits = nullptr;
+    if (D == kDedupRows) dd.bits = a.rows + (int64_t)blockIdx.x * a.rowWords;
+    if (D == kDedupLds) {
+        dd.bits = dyn;
+        for (int64_t w = lane_id(); w < a.rowWords; w += kWave) dyn[w] = 0u;
+    }
+    if (lane_id() == 0) W.poolGen = 0;
+    wave_sync();
+    return dd;
+}
+
+// Persistent wave mode over the heavy queue: one wave per block, one heavy
+// seed at a time.
+template <int K, int D>
+__global__ void __launch_bounds__(kWave) k_wave_batch(WaveArgs a, DevGraph G) {
+    __shared__ WaveLds W;
+    __shared__ int cache[D == kDedupSort ? kCacheSlots : 1];
+    extern __shared__ uint32_t dyn[];
     const int lane = lane_id();
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-    uint32_t* row = a.rows + (int64_t)wave * a.rowWords;
+    Dedup<D> dd = wave_dedup<D>(W, cache, dyn, a);
     uint32_t n = *a.heavyCount;
     if (n > a.heavyCap) n = a.heavyCap;
     uint32_t loaded = 0xffffffffu;
@@ -499,25 +778,27 @@ __global__ void __launch_bounds__(kBlock) k_wave_batch(WaveArgs a, DevGraph G) {
             copy_plan(W, a.plans + it.plan);
             loaded = it.plan;
         }
-        const uint64_t c = wave_seed<K>(W, G, a.refs + it.plan, it.item, row, a.rowWords);
+        const uint64_t c = wave_seed<K, D>(W, G, a.refs + it.plan, it.item, dd, a.rowWords, a.sink,
+                                           (unsigned long long)h << a.sink.hoBits);
         if (lane == 0 && c) atomicAdd(a.counts + 2 * W.plan.rule + K, (unsigned long long)c);
     }
 }
 
-template <int K>
-__global__ void __launch_bounds__(kBlock) k_wave_one(Plan P, WaveArgs a, DevGraph G) {
-    __shared__ WaveLds lds[kWavesPerBlock];
-    WaveLds& W = lds[wave_in_block()];
+template <int K, int D>
+__global__ void __launch_bounds__(kWave) k_wave_one(Plan P, WaveArgs a, DevGraph G) {
+    __shared__ WaveLds W;
+    __shared__ int cache[D == kDedupSort ? kCacheSlots : 1];
+    extern __shared__ uint32_t dyn[];
     const int lane = lane_id();
     copy_plan(W, &P);
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-    uint32_t* row = a.rows + (int64_t)wave * a.rowWords;
+    Dedup<D> dd = wave_dedup<D>(W, cache, dyn, a);
     uint32_t n = *a.heavyCount;
     if (n > a.heavyCap) n = a.heavyCap;
     for (;;) {
         const uint32_t h = wave_take(a.cursor);
         if (h >= n) break;
-        const uint64_t c = wave_seed<K>(W, G, a.refs, a.heavy[h].item, row, a.rowWords);
+        const uint64_t c = wave_seed<K, D>(W, G, a.refs, a.heavy[h].item, dd, a.rowWords, a.sink,
+                                           (unsigned long long)h << a.sink.hoBits);
         if (lane == 0 && c) atomicAdd(a.counts, (unsigned long long)c);
     }
 }
@@ -549,13 +830,44 @@ void launch_thread_one(int kind, const Plan& P, const OneArgs& a, const DevGraph
     if (kind == kSupport) hipLaunchKernelGGL(k_thread_one<kSupport>, dim3(blocks), dim3(kBlock), 0, s, P, a, G);
     else hipLaunchKernelGGL(k_thread_one<kPairs>, dim3(blocks), dim3(kBlock), 0, s, P, a, G);
 }
-void launch_wave_batch(int kind, const WaveArgs& a, const DevGraph& G, int blocks, hipStream_t s) {
-    if (kind == kSupport) hipLaunchKernelGGL(k_wave_batch<kSupport>, dim3(blocks), dim3(kBlock), 0, s, a, G);
-    else hipLaunchKernelGGL(k_wave_batch<kPairs>, dim3(blocks), dim3(kBlock), 0, s, a, G);
+// Wave-mode launches: one wave per block; kDedupLds blocks carry their
+// bitmap (rowWords words) in dynamic LDS.
+template <int D>
+static size_t wave_dyn_lds(const WaveArgs& a) {
+    return D == kDedupLds ? (size_t)a.rowWords * 4 : 0;
 }
-void launch_wave_one(int kind, const Plan& P, const WaveArgs& a, const DevGraph& G, int blocks, hipStream_t s) {
-    if (kind == kSupport) hipLaunchKernelGGL(k_wave_one<kSupport>, dim3(blocks), dim3(kBlock), 0, s, P, a, G);
-    else hipLaunchKernelGGL(k_wave_one<kPairs>, dim3(blocks), dim3(kBlock), 0, s, P, a, G);
+
+void launch_wave_batch(int kind, int dedup, const WaveArgs& a, const DevGraph& G, int blocks, hipStream_t s) {
+    if (kind == kSupport)
+        hipLaunchKernelGGL((k_wave_batch<kSupport, kDedupRows>), dim3(blocks), dim3(kWave), 0, s, a, G);
+    else if (dedup == kDedupSort)
+        hipLaunchKernelGGL((k_wave_batch<kPairs, kDedupSort>), dim3(blocks), dim3(kWave), 0, s, a, G);
+    else if (dedup == kDedupLds)
+        hipLaunchKernelGGL((k_wave_batch<kPairs, kDedupLds>), dim3(blocks), dim3(kWave), wave_dyn_lds<kDedupLds>(a),
+                           s, a, G);
+    else
+        hipLaunchKernelGGL((k_wave_batch<kPairs, kDedupRows>), dim3(blocks), dim3(kWave), 0, s, a, G);
+}
+void launch_wave_one(int kind, int dedup, const Plan& P, const WaveArgs& a, const DevGraph& G, int blocks,
+                     hipStream_t s) {
+    if (kind == kSupport)
+        hipLaunchKernelGGL((k_wave_one<kSupport, kDedupRows>), dim3(blocks), dim3(kWave), 0, s, P, a, G);
+    else if (dedup == kDedupSort)
+        hipLaunchKernelGGL((k_wave_one<kPairs, kDedupSort>), dim3(blocks), dim3(kWave), 0, s, P, a, G);
+    else if (dedup == kDedupLds)
+        hipLaunchKernelGGL((k_wave_one<kPairs, kDedupLds>), dim3(blocks), dim3(kWave), wave_dyn_lds<kDedupLds>(a), s,
+                           P, a, G);
+    else
+        hipLaunchKernelGGL((k_wave_one<kPairs, kDedupRows>), dim3(blocks), dim3(kWave), 0, s, P, a, G);
+}
+
+size_t wave_static_lds() { return sizeof(WaveLds) + kCacheSlots * sizeof(int); }
+
+void prepare_wave_kernels(size_t dynLds) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wave_batch<kPairs, kDedupLds>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)dynLds);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wave_one<kPairs, kDedupLds>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)dynLds);
 }
 
 void launch_copy(const void* src, void* dst, int64_t bytes, hipStream_t s) {

@@ -46,6 +46,9 @@ struct Slot {
     DevBuf dedup;
     int64_t rowWords = 0;
     int waveSlots = 0;
+    SortBufs sortb;          // sort-unique dedup path (when the context uses it)
+    bool sort = false;
+    int mode = kDedupRows;   // DedupMode of the body-pair wave-mode kernel
     Batch batch;
     std::vector<Request> reqs;
     std::vector<GpuRuleResult64> out;
@@ -55,6 +58,7 @@ struct Slot {
     ~Slot() {
         if (st) (void)hipStreamSynchronize(st);
         dedup.release();
+        sortb.release();
         if (done) (void)hipEventDestroy(done);
         if (st) (void)hipStreamDestroy(st);
     }
@@ -109,9 +113,16 @@ struct Server : Handle {
             GR_HIP(hipEventCreateWithFlags(&S->done, hipEventDisableTiming));
             S->rowWords = C->rowWords;
             S->waveSlots = C->waveSlots;
-            const size_t bytes = (size_t)S->waveSlots * S->rowWords * 4;
+            S->mode = C->pairsDedup();
+            S->sort = S->mode == kDedupSort;
+            // only the HBM-row mode uses the rows; one keeps the kernels' pointer valid otherwise
+            size_t bytes = (size_t)(S->mode == kDedupRows ? S->waveSlots : 1) * S->rowWords * 4;
             if (!S->dedup.ensure(bytes)) return false;
             GR_HIP(hipMemset(S->dedup.p, 0, bytes));
+            if (S->sort) {
+                if (!S->sortb.ensure(1)) return false;
+                bytes += S->sortb.bytes();
+            }
             S->batch.magic = kMagicBatch;
             S->batch.ctx = C;
             budget -= (int64_t)bytes;
@@ -149,6 +160,17 @@ struct Server : Handle {
             if (q != hipSuccess) {
                 log_err("serve: batch failed", q);
                 failed = true;
+            } else if (S->sort && batch_emitted(S->batch) > (int64_t)S->sortb.cap && S->sortb.cap < kSortCapMax) {
+                // body-pair keys overflowed the slot's sort buffers: grow them and run the batch again
+                const size_t was = S->sortb.bytes();
+                if (S->sortb.ensure((uint64_t)batch_emitted(S->batch))) {
+                    budget -= (int64_t)(S->sortb.bytes() - was);
+                    launch(*S);
+                    ++busy;
+                    continue;
+                }
+                failed = true;
+                q = hipErrorOutOfMemory;
             }
             S->out.resize(S->reqs.size());
             batch_collect(S->batch, S->out.data());
@@ -163,6 +185,13 @@ struct Server : Handle {
             doneCv.notify_all();
         }
         return busy;
+    }
+
+    // The slot's batch -> its stream, with the slot's dedup rows or sort buffers.
+    void launch(Slot& S) {
+        batch_enqueue(*C, S.batch, ExecTarget{S.st, S.dedup.as<uint32_t>(), S.rowWords, std::max(1, S.waveSlots),
+                                              nullptr, S.sort ? &S.sortb : nullptr, C->sortBlocks(), S.mode});
+        (void)hipEventRecord(S.done, S.st);
     }
 
     // Pending requests -> a batch on a free slot. Returns true if launched.
@@ -223,9 +252,7 @@ struct Server : Handle {
             }
             return false;
         }
-        batch_enqueue(*C, S->batch, ExecTarget{S->st, S->dedup.as<uint32_t>(), S->rowWords,
-                                                std::max(1, S->waveSlots / 4), nullptr});
-        (void)hipEventRecord(S->done, S->st);
+        launch(*S);
         S->reqs = std::move(admitted);
         S->reserved = need;
         S->admitNs = mono_ns();

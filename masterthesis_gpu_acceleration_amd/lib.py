@@ -66,6 +66,8 @@ _SIGS = {
     "gpu_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p,
                                     ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
     "gpu_last_kernel_ms": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "gpu_last_dedup_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
+                                            ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
     "gpu_plan_traffic": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]),
     "gpu_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int64]),
     "gpu_get_tuning": (ctypes.c_int64, [ctypes.c_void_p, ctypes.c_char_p]),
@@ -233,6 +235,13 @@ class Engine:
         h = ctypes.c_int64()
         self.lib.gpu_last_batch_stats(self.ctx, ctypes.byref(t), ctypes.byref(s), ctypes.byref(p), ctypes.byref(h))
         return {"total_ms": t.value, "support_ms": s.value, "pairs_ms": p.value, "heavy_seeds": h.value}
+
+    def dedup_stats(self):
+        """Sort-unique dedup of the last evaluation: radix sort + unique time
+        (eager batches), keys emitted, key capacity of the buffers."""
+        ms, n, cap = ctypes.c_double(), ctypes.c_int64(), ctypes.c_int64()
+        self.lib.gpu_last_dedup_stats(self.ctx, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(cap))
+        return {"sort_ms": ms.value, "keys_emitted": n.value, "sort_cap": cap.value}
 
     def autotune(self, rule_words: np.ndarray, flags: int = 0):
         """Runs the library's autotuner on these rules; returns its trial log

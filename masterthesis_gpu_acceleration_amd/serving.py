@@ -92,11 +92,14 @@ def now_ns() -> int:
     return int(_lib().gpu_monotonic_ns())
 
 
-def run_stream(server: Server, rule_words: np.ndarray, arrivals_s: np.ndarray, lead_s: float = 0.005) -> dict:
+def run_stream(server: Server, rule_words: np.ndarray, arrivals_s: np.ndarray, lead_s: float = 0.005,
+               progress=None, progress_every_s: float = 10.0) -> dict:
     """Submits request i at time t0 + arrivals_s[i] (pacing on the host),
     collects every completion, and returns per-request latencies (ms, from
-    the scheduled arrival) with throughput."""
+    the scheduled arrival) with throughput. `progress(submitted, completed)`
+    is called about every `progress_every_s` seconds."""
     n = int(rule_words.shape[0])
+    last_report = time.time()
     t0 = now_ns() + int(lead_s * 1e9)
     sched = t0 + (np.asarray(arrivals_s, np.float64) * 1e9).astype(np.int64)
     done = np.zeros(n, COMPLETION_DTYPE)
@@ -105,6 +108,9 @@ def run_stream(server: Server, rule_words: np.ndarray, arrivals_s: np.ndarray, l
     ndone = 0
     tags = np.arange(n, dtype=np.uint64)
     while ndone < n:
+        if progress is not None and time.time() - last_report >= progress_every_s:
+            progress(i, ndone)
+            last_report = time.time()
         now = now_ns()
         j = int(np.searchsorted(sched, now, side="right"))
         if j > i:

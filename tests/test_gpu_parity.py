@@ -73,22 +73,30 @@ def test_golden_per_rule(engines, name):
     check(fx, rules, eng.evaluate(words, L.PER_RULE | L.UNGATED), False, name + "/per-rule")
 
 
-@pytest.mark.parametrize("name", ["tiny_2", "small_paths_2k", "dbpedia_shaped_5k"])
-def test_forced_wave_mode(engines, name):
+@pytest.mark.parametrize("name", ["tiny_2", "small_paths_2k", "dense_conf_300", "dbpedia_shaped_5k"])
+@pytest.mark.parametrize("dedup", [1, 3])
+def test_forced_wave_mode(engines, name, dedup):
     """Budget 1: every seed with more than one candidate is escalated, so the
-    wave-mode kernels (ballot exit, bitmap rows) carry the counts."""
-    say("wave-mode", name)
+    wave-mode kernels (shared pool, work donation, ballot exit, head-object
+    bitmaps in HBM rows (dedup 1) or in LDS (dedup 3)) carry the counts."""
+    say("wave-mode", name, dedup)
     eng, fx, g, rules = engine_for(engines, name)
     words = L.rule_words_for_graph(rules, g)
     eng.set_tuning("budget", 1)
+    eng.set_tuning("dedup", dedup)
     try:
+        assert eng.get_tuning("pairs_dedup") == {1: 0, 3: 2}[dedup]
         res = eng.evaluate(words, L.UNGATED)
         assert eng.last_stats()["heavy_seeds"] > 0
         check(fx, rules, res, False, name + "/wave")
+        eng.set_tuning("use_graph", 0)
+        check(fx, rules, eng.evaluate(words, L.UNGATED), False, name + "/wave-eager")
         res = eng.evaluate(words, L.UNGATED | L.PER_RULE)
         check(fx, rules, res, False, name + "/wave-per-rule")
     finally:
         eng.set_tuning("budget", 4096)
+        eng.set_tuning("dedup", 0)
+        eng.set_tuning("use_graph", 1)
 
 
 @pytest.mark.parametrize("seed", range(6))
