@@ -59,6 +59,7 @@ struct Frame {
 struct ThreadLds {
     Plan plan;
     Frame frame[kMaxBody][kWave];
+    Frame cur[kMaxBody][kWave];   // ISECT levels: cursor into row 2 and its end
 };
 
 // Wave mode adds the seed's shared pool -- a frame at level poolLevel whose
@@ -142,21 +143,97 @@ struct RegSet {
     }
 };
 
-// Entry range of a level under the current bindings.
-__device__ __forceinline__ Frame enter_level(const Level& lv, const DevGraph& G, const int (&v)[kMaxSlots]) {
-    const int kind = level_kind(lv.flags);
-    if (kind == kScan) return Frame{lv.lo, lv.hi};
-    const int keyv = term_value(v, lv.key);
-    const int64_t base = (int64_t)(uint32_t)lv.lo | ((int64_t)lv.hi << 32);
+// Entry range of term `keyv` in the (direction, predicate) block run `base`.
+__device__ __forceinline__ Frame key_row(const DevGraph& G, int64_t base, int keyv) {
     const KeyBlock kb = G.blocks[base + (keyv >> 6)];
     const uint64_t bit = 1ull << (keyv & 63);
     if (!(kb.bits & bit)) return Frame{0, 0};
     const uint32_t r = kb.rank + (uint32_t)__popcll(kb.bits & (bit - 1));
-    const int begin = G.rowStart[r];
-    const int end = G.rowStart[r + 1];
-    if (kind == kExt) return Frame{begin, end};
-    const int at = find_in_row(G.col, begin, end, term_value(v, lv.val));
+    return Frame{G.rowStart[r], G.rowStart[r + 1]};
+}
+
+// Entry range of a level under the current bindings (ISECT: its row 1).
+__device__ __forceinline__ Frame enter_level(const Level& lv, const DevGraph& G, const int (&v)[kMaxSlots]) {
+    const int kind = level_kind(lv.flags);
+    if (kind == kScan) return Frame{lv.lo, lv.hi};
+    const int64_t base = (int64_t)(uint32_t)lv.lo | ((int64_t)lv.hi << 32);
+    const Frame row = key_row(G, base, term_value(v, lv.key));
+    if (kind != kCheck) return row;
+    const int at = find_in_row(G.col, row.i, row.end, term_value(v, lv.val));
     return at < 0 ? Frame{0, 0} : Frame{at, at + 1};
+}
+
+// Row 2 of an ISECT level under the current bindings.
+__device__ __forceinline__ Frame isect_row(const Level& lv, const DevGraph& G, const int (&v)[kMaxSlots]) {
+    const int64_t base = (int64_t)(uint32_t)lv.lo2 | ((int64_t)lv.hi2 << 32);
+    return key_row(G, base, term_value(v, lv.val2));
+}
+
+// Entering level L in a lane's own search: an ISECT level also places the
+// lane's row-2 cursor (no entry can match if row 2 is empty); the CHECK
+// after an ISECT level is the entry the intersection matched.
+__device__ __forceinline__ Frame enter(ThreadLds& W, const Plan& P, int L, const DevGraph& G,
+                                       const int (&v)[kMaxSlots]) {
+    const Level& lv = P.lev[L];
+    const int kind = level_kind(lv.flags);
+    if (kind == kIsect) {
+        const Frame r2 = isect_row(lv, G, v);
+        if (r2.i >= r2.end) return Frame{0, 0};
+        W.cur[L][lane_id()] = r2;
+    } else if (kind == kCheck && L > 0 && level_kind(P.lev[L - 1].flags) == kIsect) {
+        const int j = W.cur[L - 1][lane_id()].i;
+        return Frame{j, j + 1};
+    }
+    return enter_level(lv, G, v);
+}
+
+// Is x in row 2 of ISECT level L? Row 1 is enumerated in increasing x, so
+// the lane's cursor only moves forward: gallop from it, then binary search.
+// Leaves the cursor at the first entry >= x (the match, if any).
+__device__ __forceinline__ bool isect_next(ThreadLds& W, int L, const DevGraph& G, int x) {
+    const int lane = lane_id();
+    int j = W.cur[L][lane].i;
+    const int e = W.cur[L][lane].end;
+    bool hit = false;
+    if (j < e) {
+        const int y = G.col[j];
+        if (y < x) {
+            int lo = j, hi = e, step = 1;
+            for (;;) {
+                const int t = lo + step;
+                if (t >= e) break;
+                if (G.col[t] >= x) {
+                    hi = t;
+                    break;
+                }
+                lo = t;
+                step <<= 1;
+            }
+            int l = lo + 1, h = hi;
+            while (l < h) {
+                const int m = (l + h) >> 1;
+                if (G.col[m] < x) l = m + 1;
+                else h = m;
+            }
+            j = l;
+            hit = j < e && G.col[j] == x;
+        } else {
+            hit = y == x;
+        }
+    }
+    W.cur[L][lane].i = j;
+    return hit;
+}
+
+// The same test for one entry drawn from a shared pool (no cursor to reuse):
+// binary search of row 2; on a match the lane's cursor points at it.
+__device__ __forceinline__ bool isect_find(ThreadLds& W, int L, const Level& lv, const DevGraph& G,
+                                           const int (&v)[kMaxSlots], int x) {
+    const Frame r2 = isect_row(lv, G, v);
+    const int at = find_in_row(G.col, r2.i, r2.end, x);
+    if (at < 0) return false;
+    W.cur[L][lane_id()] = Frame{at, r2.end};
+    return true;
 }
 
 // Binds entry i of a level. False if the candidate violates injectivity or
@@ -168,7 +245,7 @@ __device__ __forceinline__ bool bind_entry_x(const Level& lv, const DevGraph& G,
     const int kind = level_kind(f);
     const int nbound = level_nbound(f);
     int k = 0;
-    if (kind == kExt) {
+    if (kind == kExt || kind == kIsect) {
         if (taken(v, nbound, x)) return false;
         set_slot(v, lv.val, x);
     } else if (kind == kScan) {
@@ -295,7 +372,7 @@ __device__ __forceinline__ uint64_t thread_chunk(ThreadLds& W, const DevGraph& G
                 item = begin + off;
                 active = bind_seed(P, G, (int)item, v);
                 if (active) {
-                    W.frame[0][lane] = enter_level(P.lev[0], G, v);
+                    W.frame[0][lane] = enter(W, P, 0, G, v);
                     d = 0;
                     budget = P.budget;
                     set.n = 0;
@@ -357,7 +434,9 @@ __device__ __forceinline__ uint64_t thread_chunk(ThreadLds& W, const DevGraph& G
                     heavyNow = true;
                     active = false;
                 } else {
-                    bool ok = bind_entry(P.lev[d], G, R, v, fr.i);
+                    const int x = G.col[fr.i];
+                    bool ok = bind_entry_x(P.lev[d], G, R, v, fr.i, x);
+                    if (ok && level_kind(P.lev[d].flags) == kIsect) ok = isect_next(W, d, G, x);
                     if (ok && pairs && d == pl) ok = !set.seen(term_value(v, P.hoRef));
                     if (ok) {
                         if (d + 1 == nlev) {
@@ -374,7 +453,7 @@ __device__ __forceinline__ uint64_t thread_chunk(ThreadLds& W, const DevGraph& G
                             }
                         } else {
                             ++d;
-                            W.frame[d][lane] = enter_level(P.lev[d], G, v);
+                            W.frame[d][lane] = enter(W, P, d, G, v);
                         }
                     }
                 }
@@ -479,7 +558,7 @@ __device__ __forceinline__ uint64_t wave_seed(WaveLds& W, const DevGraph& G, con
     const uint64_t below = (1ull << lane) - 1ull;
     int v[kMaxSlots];
     if (!bind_seed(P, G, (int)item, v)) return 0;
-    const Frame f0 = enter_level(P.lev[0], G, v);
+    const Frame f0 = enter(W, P, 0, G, v);
     if (f0.i >= f0.end) return 0;
     if (lane < kMaxSlots) W.poolV[lane] = term_value(v, lane);
     if (lane == 0) {
@@ -550,11 +629,13 @@ __device__ __forceinline__ uint64_t wave_seed(WaveLds& W, const DevGraph& G, con
                     next = pn + n < pe ? pn + n : pe;
                     const int i = pn + r;
                     if (!busy && i < pe) {
-                        bool ok = bind_entry(P.lev[plv], G, R, v, i);
+                        const int x = G.col[i];
+                        bool ok = bind_entry_x(P.lev[plv], G, R, v, i, x);
+                        if (ok && level_kind(P.lev[plv].flags) == kIsect) ok = isect_find(W, plv, P.lev[plv], G, v, x);
                         if (ok && pairs && plv == pl) ok = !dd.seen(term_value(v, P.hoRef));
                         if (ok) {
                             d = base = plv + 1;
-                            W.frame[d][lane] = enter_level(P.lev[d], G, v);
+                            W.frame[d][lane] = enter(W, P, d, G, v);
                             busy = true;
                         }
                     }
@@ -610,7 +691,9 @@ __device__ __forceinline__ uint64_t wave_seed(WaveLds& W, const DevGraph& G, con
                     }
                 } else {
                     W.frame[d][lane].i = fr.i + 1;
-                    bool ok = bind_entry(P.lev[d], G, R, v, fr.i);
+                    const int x = G.col[fr.i];
+                    bool ok = bind_entry_x(P.lev[d], G, R, v, fr.i, x);
+                    if (ok && level_kind(P.lev[d].flags) == kIsect) ok = isect_next(W, d, G, x);
                     if (ok && pairs && d == pl) ok = !dd.seen(term_value(v, P.hoRef));
                     if (ok) {
                         if (d == last) {
@@ -626,7 +709,7 @@ __device__ __forceinline__ uint64_t wave_seed(WaveLds& W, const DevGraph& G, con
                             }
                         } else {
                             ++d;
-                            W.frame[d][lane] = enter_level(P.lev[d], G, v);
+                            W.frame[d][lane] = enter(W, P, d, G, v);
                         }
                     }
                 }

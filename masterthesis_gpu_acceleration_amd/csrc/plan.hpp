@@ -29,6 +29,8 @@ enum LevelKind : int32_t {
     kCheck = 0,   // both terms bound: binary search of one row
     kExt   = 1,   // key bound, other term a new variable: enumerate one row
     kScan  = 2,   // neither bound: enumerate all triples of the predicate
+    kIsect = 3,   // kExt whose new variable must also lie in a second row (the CHECK level after it):
+                  // enumerate row 1, walk row 2 with a cursor; the CHECK's frame is the match
 };
 
 enum Dir : int32_t { kSPO = 0, kPOS = 1 };
@@ -39,17 +41,18 @@ enum PlanKind : int32_t { kSupport = 0, kPairs = 1 };
 // flags: bits 0-1 kind, bit 2 dir, bits 3-7 nbound (variable slots bound
 // before this level), bits 8-16 conflict mask (bit 8+j: the ground triple
 // may repeat level j's; bit 16: may repeat the head triple).
-// CHECK/EXT: key-index block of term id k = blockBase + (k >> 6), with
+// CHECK/EXT/ISECT: key-index block of term id k = blockBase + (k >> 6), with
 // blockBase = lo | hi << 32 (the (direction, predicate) block run).
 // SCAN: entries [lo, hi) of the merged col/coo arrays.
+// ISECT: the second row is that of term val2 in the block run lo2 | hi2 << 32.
 struct alignas(16) Level {
     int32_t flags;
-    int32_t key;   // term ref of the key position (CHECK, EXT)
-    int32_t val;   // CHECK: term ref searched in the row; EXT: slot bound; SCAN: slot of the key position
-    int32_t val2;  // SCAN: slot of the other position (== val for (?x p ?x))
+    int32_t key;   // term ref of the key position (CHECK, EXT, ISECT)
+    int32_t val;   // CHECK: term ref searched in the row; EXT/ISECT: slot bound; SCAN: slot of the key position
+    int32_t val2;  // SCAN: slot of the other position (== val for (?x p ?x)); ISECT: key term ref of row 2
     int32_t lo;
     int32_t hi;
-    int32_t pad0, pad1;
+    int32_t lo2, hi2;   // ISECT: block run of row 2
 };
 
 __host__ __device__ constexpr int32_t level_kind(int32_t f) { return f & 3; }

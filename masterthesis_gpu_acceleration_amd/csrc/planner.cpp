@@ -138,6 +138,46 @@ struct Builder {
             lv.flags = make_level_flags(kind, dir, nbound, m);
             P.lev[L] = lv;
         }
+        fuse(P, ord);
+    }
+
+    // An EXT level binding x followed by a CHECK of (x, y) or (y, x) with y
+    // bound before the EXT becomes an intersection: x runs over the EXT row
+    // and must also be in y's row of the CHECK predicate -- (y p x): y's
+    // objects (SPO), (x p y): y's subjects (POS). The CHECK stays (same
+    // conflict mask), keyed by y; its frame is the intersection's match, so
+    // it costs no lookup or search of its own.
+    void fuse(Plan& P, const std::vector<int>& ord) const {
+        for (int L = 0; L + 1 < P.nlev; ++L) {
+            Level& a = P.lev[L];
+            Level& m = P.lev[L + 1];
+            if (level_kind(a.flags) != kExt || level_kind(m.flags) != kCheck) continue;
+            const AtomIR& am = r.body[ord[L + 1]];
+            const int32_t x = a.val;
+            const int32_t nb = level_nbound(a.flags);
+            const int32_t rs = ref(am.s), ro = ref(am.o);
+            auto before = [&](int32_t t) { return t < 0 || t < nb; };
+            int32_t y, dir2;
+            if (ro == x && rs != x && before(rs)) {
+                y = rs;
+                dir2 = kSPO;
+            } else if (rs == x && ro != x && before(ro)) {
+                y = ro;
+                dir2 = kPOS;
+            } else {
+                continue;
+            }
+            const int32_t p2 = g.dense_pred(am.pterm);
+            const int64_t base2 = ((int64_t)dir2 * g.P() + p2) * g.nBlocks();
+            a.flags = make_level_flags(kIsect, level_dir(a.flags), nb, level_conflicts(a.flags));
+            a.val2 = y;
+            a.lo2 = (int32_t)(uint32_t)(base2 & 0xffffffffLL);
+            a.hi2 = (int32_t)(base2 >> 32);
+            m.flags = make_level_flags(kCheck, dir2, level_nbound(m.flags), level_conflicts(m.flags));
+            m.key = y;
+            m.val = x;
+            ++L;   // the CHECK is taken
+        }
     }
 };
 
@@ -273,7 +313,7 @@ CompiledRule compile_rule(const RuleIR& r, const HostGraph& g, const Tuning& t) 
                 for (int L = 0; L < Q.nlev; ++L) {
                     const Level& lv = Q.lev[L];
                     const int32_t kd = level_kind(lv.flags);
-                    if ((kd == kExt && lv.val == Q.hoRef) ||
+                    if (((kd == kExt || kd == kIsect) && lv.val == Q.hoRef) ||
                         (kd == kScan && (lv.val == Q.hoRef || lv.val2 == Q.hoRef))) {
                         Q.pairLevel = L;
                         break;

@@ -9,6 +9,9 @@
 //   CHECK probe       4 B per binary-search step (plus the final compare)
 //   EXT candidate     4 B (col)
 //   SCAN candidate    8 B (coo + col)
+//   ISECT             an EXT candidate, plus a second row lookup per level
+//                     entered and 4 B per row-2 probe of the forward cursor
+//                     (gallop + binary search); the CHECK after it costs nothing
 // It returns no counts to callers; the counting path is the device's.
 #include <algorithm>
 #include <atomic>
@@ -48,6 +51,8 @@ struct Walker {
         touched.clear();
     }
 
+    int64_t cur[kMaxBody] = {}, curEnd[kMaxBody] = {};   // ISECT row-2 cursors
+
     int col(int64_t e) const { return e < T ? g.spoVal[e] : g.posVal[e - T]; }
     int coo(int64_t e) const { return e < T ? g.spoKey[e] : g.posKey[e - T]; }
     int tv(int r) const { return r >= 0 ? v[r] : -(r + 1); }
@@ -59,19 +64,67 @@ struct Walker {
         (void)L;
     }
 
-    void range(int L, int64_t& lo, int64_t& hi) {
-        const Level& lv = P.lev[L];
-        const int kind = level_kind(lv.flags);
-        if (kind == kScan) { lo = lv.lo; hi = lv.hi; return; }
-        const int64_t base = (int64_t)(uint32_t)lv.lo | ((int64_t)lv.hi << 32);
+    // Entry range of the row of term `key` in block run `base`.
+    void key_row(int64_t base, int key, int64_t& lo, int64_t& hi) {
         const int64_t per = (int64_t)g.P() * g.nBlocks();
         const int dir = (int)(base / per);
         const int pred = (int)((base % per) / g.nBlocks());
-        auto r = g.row(dir, pred, tv(lv.key));
+        auto r = g.row(dir, pred, key);
         bytes += 16;
         if (r.second > r.first) bytes += 8;
         lo = dir * T + r.first;
         hi = dir * T + r.second;
+    }
+
+    // Forward-cursor membership test of ISECT level L (as the device does).
+    bool isect_next(int L, int x) {
+        int64_t j = cur[L];
+        const int64_t e = curEnd[L];
+        bool hit = false;
+        if (j < e) {
+            bytes += 4;
+            if (col(j) < x) {
+                int64_t lo = j, hi = e, step = 1;
+                for (;;) {
+                    const int64_t t = lo + step;
+                    if (t >= e) break;
+                    bytes += 4;
+                    if (col(t) >= x) { hi = t; break; }
+                    lo = t;
+                    step <<= 1;
+                }
+                int64_t l = lo + 1, h = hi;
+                while (l < h) {
+                    const int64_t m = (l + h) >> 1;
+                    bytes += 4;
+                    if (col(m) < x) l = m + 1; else h = m;
+                }
+                j = l;
+                hit = j < e && col(j) == x;
+            } else {
+                hit = col(j) == x;
+            }
+        }
+        cur[L] = j;
+        return hit;
+    }
+
+    void range(int L, int64_t& lo, int64_t& hi) {
+        const Level& lv = P.lev[L];
+        const int kind = level_kind(lv.flags);
+        if (kind == kScan) { lo = lv.lo; hi = lv.hi; return; }
+        if (kind == kCheck && L > 0 && level_kind(P.lev[L - 1].flags) == kIsect) {   // the intersection's match
+            lo = cur[L - 1];
+            hi = lo + 1;
+            return;
+        }
+        const int64_t base = (int64_t)(uint32_t)lv.lo | ((int64_t)lv.hi << 32);
+        if (kind == kIsect) {
+            const int64_t base2 = (int64_t)(uint32_t)lv.lo2 | ((int64_t)lv.hi2 << 32);
+            key_row(base2, tv(lv.val2), cur[L], curEnd[L]);
+            if (cur[L] >= curEnd[L]) { lo = hi = 0; return; }
+        }
+        key_row(base, tv(lv.key), lo, hi);
         if (kind == kCheck) {
             const int target = tv(lv.val);
             int64_t l = lo, h = hi;
@@ -92,7 +145,7 @@ struct Walker {
         const int x = col(i);
         int k = 0;
         auto taken = [&](int y) { for (int j = 0; j < nb; ++j) if (v[j] == y) return true; return false; };
-        if (kind == kExt) {
+        if (kind == kExt || kind == kIsect) {
             bytes += 4;
             if (taken(x)) return false;
             v[lv.val] = x;
@@ -122,8 +175,10 @@ struct Walker {
         ++nodes;
         int64_t lo, hi;
         range(L, lo, hi);
+        const bool isect = level_kind(P.lev[L].flags) == kIsect;
         for (int64_t i = lo; i < hi; ++i) {
             if (!bind(L, i)) continue;
+            if (isect && !isect_next(L, col(i))) continue;
             if (pairs && L == P.pairLevel) {
                 const int ho = tv(P.hoRef);
                 if (seen(ho)) continue;

@@ -181,3 +181,34 @@ def test_graph_replay_is_stable():
         b.close()
     finally:
         eng.close()
+
+
+@pytest.mark.skip(reason="cross-wave seed splitting is switched off: its launches ran past the time limit on the "
+                         "GPU (tiny_5, dedup 1) twice without a cause shown; the wave kernels run without it")
+@pytest.mark.parametrize("name", ["tiny_5", "small_paths_2k", "dense_conf_300", "dbpedia_shaped_5k"])
+@pytest.mark.parametrize("dedup", [1, 2, 3])
+def test_forced_cross_wave_split(engines, name, dedup):
+    """split_after 1 and budget 1: wave-mode seeds are split across waves as
+    soon as any wave is idle -- tasks, shared group rows (or sort keys),
+    group found flags -- and every count stays exact."""
+    say("split", name, dedup)
+    eng, fx, g, rules = engine_for(engines, name)
+    words = L.rule_words_for_graph(rules, g)
+    eng.set_tuning("budget", 1)
+    eng.set_tuning("dedup", dedup)
+    eng.set_tuning("split_after", 1)
+    try:
+        splits = 0
+        for flags, graph in ((L.UNGATED, 1), (L.UNGATED, 0), (0, 1), (L.UNGATED | L.PER_RULE, 1)):
+            eng.set_tuning("use_graph", graph)
+            res = eng.evaluate(words, flags)
+            check(fx, rules, res, not (flags & L.UNGATED), "%s/split%d/%d" % (name, dedup, flags))
+            splits += eng.get_tuning("last_splits")
+        say("splits", splits)
+        if name in ("dense_conf_300", "dbpedia_shaped_5k"):
+            assert splits > 0
+    finally:
+        eng.set_tuning("budget", 4096)
+        eng.set_tuning("dedup", 0)
+        eng.set_tuning("split_after", 256)
+        eng.set_tuning("use_graph", 1)
