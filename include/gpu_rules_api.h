@@ -198,6 +198,9 @@ int gpu_serve_poll(void* srv, GpuServeCompletion* out, int32_t maxOut, int32_t w
  * memory, largest batch, peak batches in flight. Returns 0, 1 if a batch
  * failed on the device, -1 on a bad handle. */
 int gpu_serve_stats(void* srv, int64_t* stats);
+/* HBM one execution slot holds for the server's life (bitmap rows, sort and
+ * giant-seed buffers), outside the per-request budget; -1 on a bad handle. */
+int64_t gpu_serve_slot_bytes(void* srv);
 /* Finishes queued and in-flight work, joins the scheduler, frees the server. */
 void gpu_serve_stop(void* srv);
 /* The monotonic clock the completion timestamps use (ns). */

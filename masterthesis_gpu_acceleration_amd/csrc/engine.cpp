@@ -108,7 +108,25 @@ bool Context::ensure_dedup() {
         if (!dedup.ensure(bytes)) return false;
         GR_HIP(hipMemset(dedup.p, 0, dedup.bytes));
     }
-    return true;
+    return giants.ensure(rowWords);
+}
+
+void giant_prepare(GiantBufs& gb, int after, GiantArgs& ga, hipStream_t s) {
+    uint32_t* ctl = gb.ctl.as<uint32_t>();
+    launch_fill_u32(ctl, 4, 0, s);
+    launch_fill_u32(gb.remaining.as<uint32_t>(), kGiantCap, 0, s);
+    launch_fill_u32(gb.found.as<uint32_t>(), kGiantCap, 0, s);
+    ga.after = after;
+    ga.cap = kGiantCap;
+    ga.list = gb.list.as<HeavyItem>();
+    ga.heavyIdx = gb.heavyIdx.as<uint32_t>();
+    ga.count = ctl + 0;
+    ga.subs = gb.subs.as<SubItem>();
+    ga.subCount = ctl + 1;
+    ga.subCursor = ctl + 2;
+    ga.remaining = gb.remaining.as<uint32_t>();
+    ga.found = gb.found.as<uint32_t>();
+    ga.rows = gb.rows.as<uint32_t>();
 }
 
 // --------------------------------------------------------- per-rule path
@@ -286,6 +304,8 @@ bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const s
         B.hCountsCap = (size_t)2 * B.capRules;
     }
     if (!B.hCtl) GR_HIP(hipHostMalloc((void**)&B.hCtl, kCtlWords * 4, hipHostMallocDefault));
+    if (!B.hGiant) GR_HIP(hipHostMalloc((void**)&B.hGiant, 8, hipHostMallocDefault));
+    B.hGiant[0] = B.hGiant[1] = 0;
     GR_HIP(hipStreamSynchronize(st));
     return true;
 }
@@ -318,7 +338,13 @@ void batch_enqueue(Context& C, Batch& B, const ExecTarget& t) {
             if (sort) sink = SortSink{t.sort->keys[0].as<unsigned long long>(), c + 3, t.sort->cap, hoBits};
             WaveArgs wa{B.wplans.as<Plan>(), B.wrefs.as<ConflictRefs>(), B.heavy.as<HeavyItem>() + B.heavyOff[ph],
                         c + 1, B.heavyCap[ph], 0, c + 2, counts, t.dedup, t.rowWords, sink};
-            launch_wave_batch(ph, ph == kPairs ? t.pairsDedup : kDedupRows, wa, C.dg, t.waveBlocks, st);
+            const int mode = ph == kPairs ? t.pairsDedup : kDedupRows;
+            giant_prepare(*t.giants, t.giantAfter, wa.giant, st);
+            launch_wave_batch(ph, mode, wa, C.dg, t.waveBlocks, st);
+            if (t.giantAfter > 0) {
+                launch_giants(ph, mode, wa, C.dg, t.waveBlocks, st);
+                (void)hipMemcpyAsync(B.hGiant + ph, wa.giant.count, 4, hipMemcpyDeviceToHost, st);
+            }
             if (sort) {
                 if (t.ev) (void)hipEventRecord(t.ev[6], st);
                 SortJob j{{t.sort->keys[0].as<unsigned long long>(), t.sort->keys[1].as<unsigned long long>()},
@@ -365,13 +391,15 @@ bool batch_run(Context& C, Batch& B, GpuRuleResult64* out) {
         sb = &C.sortb;
     }
     for (;;) {
+        B.hGiant[0] = B.hGiant[1] = 0;
         if (graphMode) {
             // re-capture if the context's scratch or launch shape changed since
             const void* sp = sb ? sb->keys[0].p : nullptr;
             const uint32_t sc = sb ? sb->cap : 0;
             if (B.exec && (B.capturedDedup != C.dedup.p || B.capturedBlocks[0] != C.threadBlocks() ||
                            B.capturedBlocks[1] != C.waveBlocks() || B.capturedSort != sp || B.capturedSortCap != sc ||
-                           B.capturedMode != mode)) {
+                           B.capturedMode != mode || B.capturedGiants != C.giants.rows.p ||
+                           B.capturedGiantAfter != C.giantAfter)) {
                 (void)hipGraphExecDestroy(B.exec);
                 (void)hipGraphDestroy(B.graph);
                 B.exec = nullptr;
@@ -384,9 +412,11 @@ bool batch_run(Context& C, Batch& B, GpuRuleResult64* out) {
                 B.capturedSort = sp;
                 B.capturedSortCap = sc;
                 B.capturedMode = mode;
+                B.capturedGiants = C.giants.rows.p;
+                B.capturedGiantAfter = C.giantAfter;
                 GR_HIP(hipStreamBeginCapture(C.st, hipStreamCaptureModeThreadLocal));
                 batch_enqueue(C, B, ExecTarget{C.st, C.dedup.as<uint32_t>(), C.rowWords, C.waveBlocks(), nullptr, sb,
-                                               C.sortBlocks(), mode});
+                                               C.sortBlocks(), mode, &C.giants, C.giantAfter});
                 GR_HIP(hipStreamEndCapture(C.st, &B.graph));
                 GR_HIP(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
             }
@@ -401,7 +431,7 @@ bool batch_run(Context& C, Batch& B, GpuRuleResult64* out) {
             for (double& k : B.kernelMs) k = 0;
         } else {
             batch_enqueue(C, B, ExecTarget{C.st, C.dedup.as<uint32_t>(), C.rowWords, C.waveBlocks(), C.ev, sb,
-                                           C.sortBlocks(), mode});
+                                           C.sortBlocks(), mode, &C.giants, C.giantAfter});
             GR_HIP(hipEventRecord(C.ev[5], C.st));
             GR_HIP(hipStreamSynchronize(C.st));
             float t[5] = {0, 0, 0, 0, 0};
@@ -438,6 +468,7 @@ bool batch_run(Context& C, Batch& B, GpuRuleResult64* out) {
     C.lastSortMs = B.lastSortMs;
     C.lastHeavy = B.lastHeavy;
     C.lastEmitted = sb ? B.lastEmitted : 0;
+    C.lastGiants = (int64_t)B.hGiant[0] + B.hGiant[1];
     for (int k = 0; k < 4; ++k) C.lastKernelMs[k] = B.kernelMs[k];
     return true;
 }
@@ -873,6 +904,8 @@ int gpu_set_tuning(void* ctx, const char* key, int64_t value) {
         delete C->lastBatch;
         C->lastBatch = nullptr;
         if (!C->ensure_dedup()) return -1;
+    } else if (k == "giant_after") {
+        C->giantAfter = (int)std::max<int64_t>(0, value);
     } else if (k == "sort_cap") {
         // start the sort-unique buffers at this many keys (they still grow on overflow)
         if (value < kSortTile || value > (int64_t)kSortCapMax) return -1;
@@ -896,6 +929,8 @@ int64_t gpu_get_tuning(void* ctx, const char* key) {
     if (k == "dedup") return C->dedupMode;
     if (k == "sort_active") return C->sortActive() ? 1 : 0;
     if (k == "pairs_dedup") return C->pairsDedup();
+    if (k == "giant_after") return C->giantAfter;
+    if (k == "last_giants") return C->lastGiants;
     if (k == "sort_cap") return C->sortb.cap;
     return -1;
 }

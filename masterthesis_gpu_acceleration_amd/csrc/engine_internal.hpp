@@ -103,6 +103,38 @@ struct SortBufs {
     size_t bytes() const { return keys[0].bytes + keys[1].bytes + hist.bytes; }
 };
 
+// Buffers of the giant-seed pass for one execution target (the context or
+// a serving slot): the giant list, sub-items, per-giant counters and found
+// flags, and per-giant HBM bitmap rows (all zero between uses: the last
+// sub-item of a giant clears its row). Counters, remaining counts and flags
+// are zeroed by fill kernels before each wave-mode phase.
+constexpr uint32_t kGiantCap = 512;
+struct GiantBufs {
+    DevBuf list, heavyIdx, subs, ctl, remaining, found, rows;
+    int64_t rowWords = 0;
+    bool ensure(int64_t words) {
+        words = std::max<int64_t>(words, 1);
+        if (rows.p && words == rowWords) return true;
+        release();
+        const size_t rowBytes = (size_t)kGiantCap * (size_t)words * 4;
+        if (!list.ensure(kGiantCap * sizeof(HeavyItem)) || !heavyIdx.ensure(kGiantCap * 4) ||
+            !subs.ensure((size_t)kGiantCap * kSubsPerGiant * sizeof(SubItem)) || !ctl.ensure(16) ||
+            !remaining.ensure(kGiantCap * 4) || !found.ensure(kGiantCap * 4) || !rows.ensure(rowBytes))
+            return false;
+        if (hipMemset(rows.p, 0, rowBytes) != hipSuccess) return false;
+        rowWords = words;
+        return true;
+    }
+    void release() {
+        list.release(); heavyIdx.release(); subs.release(); ctl.release(); remaining.release(); found.release();
+        rows.release();
+        rowWords = 0;
+    }
+    size_t bytes() const {
+        return list.bytes + heavyIdx.bytes + subs.bytes + ctl.bytes + remaining.bytes + found.bytes + rows.bytes;
+    }
+};
+
 struct Context;
 
 // A compiled, device-resident batch of rules.
@@ -129,10 +161,13 @@ struct Batch : Handle {
     const void* capturedSort = nullptr;
     uint32_t capturedSortCap = 0;
     int capturedMode = -1;
+    const void* capturedGiants = nullptr;
+    int capturedGiantAfter = -1;
     int capturedBlocks[2] = {0, 0};
     double lastMs = 0, lastSupMs = 0, lastPairMs = 0, lastSortMs = 0;
     int64_t lastHeavy = 0;
     int64_t lastEmitted = 0;   // sort path: keys the last pass emitted
+    uint32_t* hGiant = nullptr;   // pinned: giants listed in each phase of the last pass
     ~Batch();
 };
 
@@ -163,6 +198,9 @@ struct Context : Handle {
     int64_t rowWords = 0;
     int waveSlots = 0;
     SortBufs sortb;   // sort-unique dedup buffers (allocated when that path is used)
+    GiantBufs giants; // giant-seed pass
+    int giantAfter = 2048;   // wave-mode steps on one seed before it is listed as a giant (0: never)
+    int64_t lastGiants = 0;
     Batch* lastBatch = nullptr;
     double lastMs = 0, lastSupMs = 0, lastPairMs = 0, lastSortMs = 0;
     double lastKernelMs[4] = {};
@@ -192,13 +230,14 @@ inline Batch::~Batch() {
     heavy.release(); ctl.release();
     if (hCounts) (void)hipHostFree(hCounts);
     if (hCtl) (void)hipHostFree(hCtl);
+    if (hGiant) (void)hipHostFree(hGiant);
     if (ctx && ctx->lastBatch == this) ctx->lastBatch = nullptr;
 }
 
 inline Context::~Context() {
     if (st) (void)hipStreamSynchronize(st);
     blocks.release(); rowStart.release(); col.release(); coo.release(); keys.release();
-    one.release(); heavy1.release(); dedup.release(); sortb.release();
+    one.release(); heavy1.release(); dedup.release(); sortb.release(); giants.release();
     if (hOne) (void)hipHostFree(hOne);
     for (auto& e : ev) if (e) (void)hipEventDestroy(e);
     if (st) (void)hipStreamDestroy(st);
@@ -223,7 +262,13 @@ struct ExecTarget {
     SortBufs* sort;   // the sort-unique buffers (pairsDedup == kDedupSort)
     int sortBlocks;
     int pairsDedup;   // DedupMode of the body-pair wave-mode kernel
+    GiantBufs* giants;
+    int giantAfter;
 };
+
+// Zeroes the giant pass's counters and flags for one wave-mode phase and
+// fills a GiantArgs for it (after == 0: no seed is listed).
+void giant_prepare(GiantBufs& gb, int after, GiantArgs& ga, hipStream_t s);
 
 // Bytes of bitmap rows allowed before body-pair dedup switches to the
 // sort-unique path (auto mode).

@@ -473,6 +473,10 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 __device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ uint32_t bcast(uint32_t x) { return (uint32_t)__shfl((int)x, 0, kWave); }
+__device__ __forceinline__ uint32_t ld_rlx(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // Head-object dedup of one PAIRS seed in wave mode.
 //   kDedupRows: the wave's bitmap row of nE bits in HBM (atomicOr);
@@ -484,13 +488,15 @@ __device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirs
 // one lane whose call set the bit, so the lanes' sums are the distinct count.
 template <int D>
 struct Dedup {
-    uint32_t* bits;
+    uint32_t* bits;     // the wave's own bitmap: its HBM row (kDedupRows) or LDS (kDedupLds)
+    uint32_t* shared;   // a giant's HBM row, shared by its sub-items (then used instead of bits), or null
     WaveLds* W;
     int* cache;
     __device__ __forceinline__ bool seen(int x) const {
         if (D == kDedupSort) return cache[x & (kCacheSlots - 1)] == x;
         uint32_t w;
-        if (D == kDedupRows) w = __hip_atomic_load(bits + (x >> 5), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (shared) w = __hip_atomic_load(shared + (x >> 5), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else if (D == kDedupRows) w = __hip_atomic_load(bits + (x >> 5), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         else w = bits[x >> 5];
         return (w >> (x & 31)) & 1u;
     }
@@ -500,6 +506,7 @@ struct Dedup {
             return 0;
         }
         const uint32_t bit = 1u << (x & 31);
+        if (shared) return (atomicOr(shared + (x >> 5), bit) & bit) ? 0u : 1u;
         const uint32_t old = atomicOr(bits + (x >> 5), bit);
         if (old == 0) {
             const uint32_t k = atomicAdd(&W->nTouched, 1u);
@@ -543,12 +550,18 @@ constexpr int kLeafUnroll = 8;   // last-level pool entries an idle lane takes p
 //   * SUPPORT seeds end at the first complete grounding (__any). PAIRS seeds
 //     count each head object once (Dedup); a lane whose head object is
 //     already counted drops the rest of that subtree.
-// Returns the seed's count (SUPPORT: 0/1; PAIRS: distinct head objects, or 0
-// on the sort path, which counts later).
+//   * A seed (not a sub-item) still running after ga.after steps is listed
+//     as a giant and abandoned (`aborted`; its partial count is dropped).
+//     A SUPPORT sub-item shares its giant's `found` flag: it polls it and
+//     counts only if it is the one to set it.
+// The pool (poolLevel/Next/End/V) is set up by the caller, all lanes idle.
+// Returns the count this wave adds (SUPPORT: 0/1; PAIRS: distinct head
+// objects it marked first, or 0 on the sort path, which counts later).
 template <int K, int D>
-__device__ __forceinline__ uint64_t wave_seed(WaveLds& W, const DevGraph& G, const ConflictRefs* R, uint32_t item,
-                                              Dedup<D>& dd, int64_t rowWords, const SortSink& sink,
-                                              unsigned long long keyHi) {
+__device__ __forceinline__ uint64_t wave_run(WaveLds& W, const DevGraph& G, const ConflictRefs* R, Dedup<D>& dd,
+                                             int64_t rowWords, const SortSink& sink, unsigned long long keyHi,
+                                             const GiantArgs& ga, bool mayGiant, HeavyItem self, uint32_t heavyIdx,
+                                             uint32_t* foundFlag, bool& aborted) {
     const int lane = lane_id();
     const Plan& P = W.plan;
     const int nlev = P.nlev;
@@ -557,24 +570,14 @@ __device__ __forceinline__ uint64_t wave_seed(WaveLds& W, const DevGraph& G, con
     const int pl = P.pairLevel;
     const uint64_t below = (1ull << lane) - 1ull;
     int v[kMaxSlots];
-    if (!bind_seed(P, G, (int)item, v)) return 0;
-    const Frame f0 = enter(W, P, 0, G, v);
-    if (f0.i >= f0.end) return 0;
-    if (lane < kMaxSlots) W.poolV[lane] = term_value(v, lane);
-    if (lane == 0) {
-        W.poolLevel = 0;
-        W.poolNext = f0.i;
-        W.poolEnd = f0.end;
-        W.poolGen = W.poolGen + 1u;
-        W.nTouched = 0;
-    }
-    if (D == kDedupSort && pairs)
-        for (int k = lane; k < kCacheSlots; k += kWave) dd.cache[k] = -1;
-    wave_sync();
-    bool busy = false, found = false;
+#pragma unroll
+    for (int j = 0; j < kMaxSlots; ++j) v[j] = -1;
+    bool busy = false, found = false, stopped = false;
+    aborted = false;
     int d = 0, base = 0;            // own search: current level, first level it owns
     uint32_t gen = 0xffffffffu;     // pool whose bindings v holds (idle lanes)
     uint64_t mine = 0;
+    int steps = 0;
     for (;;) {
         // ---- A: idle lanes draw from the pool
         const uint64_t idleM = __ballot(!busy);
@@ -644,7 +647,7 @@ __device__ __forceinline__ uint64_t wave_seed(WaveLds& W, const DevGraph& G, con
             wave_sync();
             if (lane == 0) W.poolNext = next;
         }
-        if (!pairs && __any(found)) return 1;
+        if (!pairs && __any(found)) break;
         // ---- B: busy lanes advance their own search by one step
         int emit = -1;
         int emits[kLeafBatch];
@@ -720,12 +723,36 @@ __device__ __forceinline__ uint64_t wave_seed(WaveLds& W, const DevGraph& G, con
 #pragma unroll
             for (int k = 0; k < kLeafBatch; ++k) emit_keys(sink, keyHi, emits[k]);
         }
-        if (!pairs && __any(found)) return 1;
-        // ---- C: done, or donate work to the idle lanes
+        if (!pairs && __any(found)) break;
+        // ---- C: done, hand the seed over as a giant, or donate work to the idle lanes
         wave_sync();
         const uint64_t busyM = __ballot(busy);
         const bool poolEmpty = uniform(W.poolNext) >= uniform(W.poolEnd);
         if (busyM == 0 && poolEmpty) break;
+        ++steps;
+        if (mayGiant && steps >= ga.after) {
+            mayGiant = false;   // one try per seed
+            uint32_t slot = 0xffffffffu;
+            if (lane == 0) {
+                const uint32_t gi = atomicAdd(ga.count, 1u);
+                if (gi < ga.cap) {
+                    ga.list[gi] = self;
+                    ga.heavyIdx[gi] = heavyIdx;
+                    slot = gi;
+                }
+            }
+            if (bcast(slot) != 0xffffffffu) {
+                aborted = true;
+                break;
+            }
+        }
+        if (foundFlag && (steps & 31) == 0) {   // another sub-item of this giant found a grounding
+            uint32_t f = lane == 0 ? ld_rlx(foundFlag) : 0u;
+            if (bcast(f)) {
+                stopped = true;
+                break;
+            }
+        }
         if (busyM != ~0ull && poolEmpty) {
             int rem = 0, lvl = 0;
             if (busy) {
@@ -760,14 +787,16 @@ __device__ __forceinline__ uint64_t wave_seed(WaveLds& W, const DevGraph& G, con
             }
         }
     }
-    if (!pairs || D == kDedupSort) return 0;
-    if (D == kDedupRows) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    } else {
-        wave_sync();
+    if (!pairs) {
+        if (aborted || stopped || !__any(found)) return 0;
+        if (!foundFlag) return 1;
+        uint32_t win = 0;
+        if (lane == 0) win = atomicCAS(foundFlag, 0u, 1u) == 0u ? 1u : 0u;
+        return bcast(win);
     }
+    if (D == kDedupSort) return 0;
+    // clear the wave's own bitmap (a sub-item marked only its giant's row: nothing listed)
+    wave_sync();
     const uint32_t nt = W.nTouched;
     if (nt <= (uint32_t)kTouchCap) {
         for (uint32_t k = lane; k < nt; k += kWave) dd.clear_word(W.touched[k]);
@@ -775,7 +804,36 @@ __device__ __forceinline__ uint64_t wave_seed(WaveLds& W, const DevGraph& G, con
         for (int64_t w = lane; w < rowWords; w += kWave) dd.clear_word((uint32_t)w);
     }
     if (D == kDedupLds) wave_sync();
-    return wave_sum(mine);
+    const uint64_t total = wave_sum(mine);
+    return aborted ? 0 : total;
+}
+
+// Sets up the pool of heavy seed `item` (its level-0 row) and runs it.
+template <int K, int D>
+__device__ __forceinline__ uint64_t wave_seed(WaveLds& W, const DevGraph& G, const ConflictRefs* R, uint32_t item,
+                                              Dedup<D>& dd, int64_t rowWords, const SortSink& sink,
+                                              unsigned long long keyHi, const GiantArgs& ga, uint32_t planIdx,
+                                              uint32_t heavyIdx) {
+    const int lane = lane_id();
+    const Plan& P = W.plan;
+    int v[kMaxSlots];
+    if (!bind_seed(P, G, (int)item, v)) return 0;
+    const Frame f0 = enter(W, P, 0, G, v);
+    if (f0.i >= f0.end) return 0;
+    if (lane < kMaxSlots) W.poolV[lane] = term_value(v, lane);
+    if (lane == 0) {
+        W.poolLevel = 0;
+        W.poolNext = f0.i;
+        W.poolEnd = f0.end;
+        W.poolGen = W.poolGen + 1u;
+        W.nTouched = 0;
+    }
+    if (D == kDedupSort && K == kPairs)
+        for (int k = lane; k < kCacheSlots; k += kWave) dd.cache[k] = -1;
+    wave_sync();
+    bool aborted = false;
+    return wave_run<K, D>(W, G, R, dd, rowWords, sink, keyHi, ga, ga.after > 0, HeavyItem{planIdx, item}, heavyIdx,
+                          nullptr, aborted);
 }
 
 // ------------------------------------------------------------- kernels
@@ -831,6 +889,7 @@ __device__ __forceinline__ Dedup<D> wave_dedup(WaveLds& W, int* cache, uint32_t*
     dd.W = &W;
     dd.cache = cache;
     dd.bits = nullptr;
+    dd.shared = nullptr;
     if (D == kDedupRows) dd.bits = a.rows + (int64_t)blockIdx.x * a.rowWords;
     if (D == kDedupLds) {
         dd.bits = dyn;
@@ -862,7 +921,7 @@ __global__ void __launch_bounds__(kWave) k_wave_batch(WaveArgs a, DevGraph G) {
             loaded = it.plan;
         }
         const uint64_t c = wave_seed<K, D>(W, G, a.refs + it.plan, it.item, dd, a.rowWords, a.sink,
-                                           (unsigned long long)h << a.sink.hoBits);
+                                           (unsigned long long)h << a.sink.hoBits, a.giant, it.plan, h);
         if (lane == 0 && c) atomicAdd(a.counts + 2 * W.plan.rule + K, (unsigned long long)c);
     }
 }
@@ -881,8 +940,157 @@ __global__ void __launch_bounds__(kWave) k_wave_one(Plan P, WaveArgs a, DevGraph
         const uint32_t h = wave_take(a.cursor);
         if (h >= n) break;
         const uint64_t c = wave_seed<K, D>(W, G, a.refs, a.heavy[h].item, dd, a.rowWords, a.sink,
-                                           (unsigned long long)h << a.sink.hoBits);
+                                           (unsigned long long)h << a.sink.hoBits, a.giant, 0u, h);
         if (lane == 0 && c) atomicAdd(a.counts, (unsigned long long)c);
+    }
+}
+
+// ------------------------------------------------------------ giant seeds
+
+// One wave per listed giant: binds the seed again and writes its sub-items
+// -- kSubsPerGiant ranges of its level-0 row, or, when that row is shorter
+// than kSubMinRow, up to 4 ranges of the level-1 row below each level-0
+// entry (one lane per entry), with that entry's bindings -- and records
+// how many it wrote. Capacity is never short: cap * kSubsPerGiant slots.
+__global__ void __launch_bounds__(kWave) k_giant_expand(GiantArgs ga, const Plan* plans, const ConflictRefs* refs,
+                                                        DevGraph G) {
+    __shared__ ThreadLds W;
+    const int lane = lane_id();
+    uint32_t n = *ga.count;
+    if (n > ga.cap) n = ga.cap;
+    for (uint32_t g = blockIdx.x; g < n; g += gridDim.x) {
+        const HeavyItem it = ga.list[g];
+        const uint32_t hv = ga.heavyIdx[g];
+        copy_plan(W, plans + it.plan);
+        const Plan& P = W.plan;
+        int v[kMaxSlots];
+        uint32_t total = 0;
+        if (bind_seed(P, G, (int)it.item, v)) {
+            const Frame f0 = enter(W, P, 0, G, v);
+            const int len0 = f0.end - f0.i;
+            if (len0 >= kSubMinRow || (len0 > 0 && P.nlev == 1)) {
+                const int parts = len0 < kSubsPerGiant ? len0 : kSubsPerGiant;
+                const int chunk = (len0 + parts - 1) / parts;
+                const int items = (len0 + chunk - 1) / chunk;
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(ga.subCount, (uint32_t)items);
+                base = bcast(base);
+                for (int c = lane; c < items; c += kWave) {
+                    SubItem& S = ga.subs[base + c];
+                    S.giant = g;
+                    S.plan = it.plan;
+                    S.level = 0;
+                    S.begin = f0.i + c * chunk;
+                    S.end = f0.i + (c + 1) * chunk < f0.end ? f0.i + (c + 1) * chunk : f0.end;
+                    S.heavy = hv;
+#pragma unroll
+                    for (int j = 0; j < kMaxSlots; ++j) S.v[j] = term_value(v, j);
+                }
+                total = (uint32_t)items;
+            } else if (len0 > 0) {
+                // short level-0 row (< kSubMinRow <= 64 entries): one lane per entry, cut its level-1 row
+                int mine = 0, chunk = 1;
+                Frame f1{0, 0};
+                if (lane < len0) {
+                    const int i = f0.i + lane;
+                    const int x = G.col[i];
+                    bool ok = bind_entry_x(P.lev[0], G, refs + it.plan, v, i, x);
+                    if (ok && level_kind(P.lev[0].flags) == kIsect) ok = isect_find(W, 0, P.lev[0], G, v, x);
+                    if (ok) {
+                        f1 = enter(W, P, 1, G, v);
+                        const int len1 = f1.end - f1.i;
+                        if (len1 > 0) {
+                            const int parts = len1 < 4 ? len1 : 4;
+                            chunk = (len1 + parts - 1) / parts;
+                            mine = (len1 + chunk - 1) / chunk;
+                        }
+                    }
+                }
+                // exclusive prefix of the lanes' item counts
+                int incl = mine;
+#pragma unroll
+                for (int off = 1; off < kWave; off <<= 1) {
+                    const int y = __shfl_up(incl, off, kWave);
+                    if (lane >= off) incl += y;
+                }
+                const int all = __shfl(incl, kWave - 1, kWave);
+                uint32_t base = 0;
+                if (lane == 0 && all > 0) base = atomicAdd(ga.subCount, (uint32_t)all);
+                base = bcast(base) + (uint32_t)(incl - mine);
+                for (int c = 0; c < mine; ++c) {
+                    SubItem& S = ga.subs[base + c];
+                    S.giant = g;
+                    S.plan = it.plan;
+                    S.level = 1;
+                    S.begin = f1.i + c * chunk;
+                    S.end = f1.i + (c + 1) * chunk < f1.end ? f1.i + (c + 1) * chunk : f1.end;
+                    S.heavy = hv;
+#pragma unroll
+                    for (int j = 0; j < kMaxSlots; ++j) S.v[j] = term_value(v, j);
+                }
+                total = (uint32_t)all;
+            }
+        }
+        if (lane == 0) ga.remaining[g] = total;
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// The sub-items of all giants, over all waves: each one is a pool (its range
+// and bindings) run by wave_run against the giant's shared row or found
+// flag. The sub-item that finishes a giant's last clears its row.
+template <int K, int D>
+__global__ void __launch_bounds__(kWave) k_wave_sub(WaveArgs a, DevGraph G) {
+    __shared__ WaveLds W;
+    __shared__ int cache[D == kDedupSort ? kCacheSlots : 1];
+    const int lane = lane_id();
+    Dedup<D> dd = wave_dedup<D>(W, cache, nullptr, a);
+    const GiantArgs& ga = a.giant;
+    uint32_t n = *ga.subCount;
+    const uint32_t capSubs = ga.cap * (uint32_t)kSubsPerGiant;
+    if (n > capSubs) n = capSubs;
+    uint32_t loaded = 0xffffffffu;
+    for (;;) {
+        const uint32_t si = wave_take(ga.subCursor);
+        if (si >= n) break;
+        const SubItem& S = ga.subs[si];
+        const uint32_t g = (uint32_t)uniform((int)S.giant);
+        const uint32_t plan = (uint32_t)uniform((int)S.plan);
+        const uint32_t hv = (uint32_t)uniform((int)S.heavy);
+        if (plan != loaded) {
+            copy_plan(W, a.plans + plan);
+            loaded = plan;
+        }
+        // existence searches (SUPPORT, and PAIRS plans whose head object is bound by the seed) count a giant once
+        uint32_t* foundFlag = (K == kSupport || W.plan.pairLevel < 0) ? ga.found + g : nullptr;
+        uint32_t skip = 0;
+        if (foundFlag && lane == 0) skip = ld_rlx(foundFlag);
+        uint64_t c = 0;
+        if (!bcast(skip)) {
+            if (lane < kMaxSlots) W.poolV[lane] = S.v[lane];
+            if (lane == 0) {
+                W.poolLevel = S.level;
+                W.poolNext = S.begin;
+                W.poolEnd = S.end;
+                W.poolGen = W.poolGen + 1u;
+                W.nTouched = 0;
+            }
+            if (D == kDedupSort && K == kPairs)
+                for (int k = lane; k < kCacheSlots; k += kWave) cache[k] = -1;
+            wave_sync();
+            dd.shared = (K == kPairs && D != kDedupSort) ? ga.rows + (int64_t)g * a.rowWords : nullptr;
+            bool aborted = false;
+            c = wave_run<K, D>(W, G, a.refs + plan, dd, a.rowWords, a.sink, (unsigned long long)hv << a.sink.hoBits,
+                               ga, false, HeavyItem{plan, 0u}, hv, foundFlag, aborted);
+            dd.shared = nullptr;
+        }
+        if (lane == 0 && c) atomicAdd(a.counts + 2 * W.plan.rule + K, (unsigned long long)c);
+        uint32_t last = 0;
+        if (lane == 0) last = atomicSub(ga.remaining + g, 1u) == 1u ? 1u : 0u;
+        if (bcast(last) && K == kPairs && D != kDedupSort) {
+            uint32_t* row = ga.rows + (int64_t)g * a.rowWords;
+            for (int64_t w = lane; w < a.rowWords; w += kWave) row[w] = 0u;   // read again only by later launches
+        }
     }
 }
 
@@ -951,6 +1159,17 @@ void prepare_wave_kernels(size_t dynLds) {
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)dynLds);
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wave_one<kPairs, kDedupLds>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)dynLds);
+}
+
+void launch_giants(int kind, int dedup, const WaveArgs& a, const DevGraph& G, int blocks, hipStream_t s) {
+    const int eb = blocks < 1024 ? blocks : 1024;
+    hipLaunchKernelGGL(k_giant_expand, dim3(eb), dim3(kWave), 0, s, a.giant, a.plans, a.refs, G);
+    if (kind == kSupport)
+        hipLaunchKernelGGL((k_wave_sub<kSupport, kDedupRows>), dim3(blocks), dim3(kWave), 0, s, a, G);
+    else if (dedup == kDedupSort)
+        hipLaunchKernelGGL((k_wave_sub<kPairs, kDedupSort>), dim3(blocks), dim3(kWave), 0, s, a, G);
+    else
+        hipLaunchKernelGGL((k_wave_sub<kPairs, kDedupRows>), dim3(blocks), dim3(kWave), 0, s, a, G);
 }
 
 void launch_copy(const void* src, void* dst, int64_t bytes, hipStream_t s) {

@@ -49,6 +49,42 @@ struct SortSink {
     int32_t hoBits;
 };
 
+// Giant seeds. A wave-mode seed still running after `after` steps is
+// abandoned (its partial count dropped) and listed; an expansion kernel
+// cuts each listed seed into sub-items -- ranges of its level-0 row, or of
+// its level-1 rows when the level-0 row is short, each with the bindings
+// above -- and a sub-item kernel runs them on all waves. Sub-items of one
+// giant mark one HBM bitmap row (PAIRS; atomicOr results count each head
+// object once) or one found flag (SUPPORT); the sub-item that finishes last
+// clears the row. Nothing waits on another wave.
+struct alignas(16) SubItem {
+    uint32_t giant;      // index in the giant list
+    uint32_t plan;       // plan index (wave-order table)
+    int32_t level;       // entries [begin, end) of this level
+    int32_t begin;
+    int32_t end;
+    uint32_t heavy;      // heavy-queue index of the seed (sort-path keys)
+    int32_t pad[2];
+    int32_t v[kMaxSlots];   // bindings above `level`
+};
+
+constexpr int kSubsPerGiant = 256;   // sub-items one giant is cut into, at most
+constexpr int kSubMinRow = 64;       // a level-0 row at least this long is cut at level 0
+
+struct GiantArgs {
+    int32_t after;           // wave-mode steps before a seed is listed as a giant (0: never)
+    uint32_t cap;            // giants listed at most (more are run to the end as usual)
+    HeavyItem* list;         // [cap] giant seeds (plan, item)
+    uint32_t* heavyIdx;      // [cap] their heavy-queue index
+    uint32_t* count;         // giants listed (may pass cap)
+    SubItem* subs;           // [cap * kSubsPerGiant]
+    uint32_t* subCount;
+    uint32_t* subCursor;
+    uint32_t* remaining;     // [cap] sub-items of each giant not yet finished
+    uint32_t* found;         // [cap] SUPPORT: a sub-item found a grounding
+    uint32_t* rows;          // [cap] HBM bitmap rows (rowWords each), all zero between uses
+};
+
 // Wave-mode pass over the heavy queue.
 struct WaveArgs {
     const Plan* plans;            // wave-order plans (unused by the by-value kernel)
@@ -62,6 +98,7 @@ struct WaveArgs {
     uint32_t* rows;               // per-wave dedup rows, all zero between seeds (kDedupRows)
     int64_t rowWords;
     SortSink sink;                // kDedupSort
+    GiantArgs giant;              // batch launches; after == 0 for per-rule launches
 };
 
 // `kind` is the plan kind (kSupport / kPairs) of every plan the launch sees.
@@ -75,6 +112,9 @@ void launch_wave_one(int kind, int dedup, const Plan& P, const WaveArgs& a, cons
 // request for the kDedupLds kernels.
 size_t wave_static_lds();
 void prepare_wave_kernels(size_t dynLds);
+// The giant-seed pass that follows a wave-mode batch launch: expansion of
+// the listed giants, then their sub-items (same `a` as the wave launch).
+void launch_giants(int kind, int dedup, const WaveArgs& a, const DevGraph& G, int blocks, hipStream_t s);
 void launch_copy(const void* src, void* dst, int64_t bytes, hipStream_t s);
 void launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t s);
 void launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);

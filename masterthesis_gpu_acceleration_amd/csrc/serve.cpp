@@ -47,6 +47,7 @@ struct Slot {
     int64_t rowWords = 0;
     int waveSlots = 0;
     SortBufs sortb;          // sort-unique dedup path (when the context uses it)
+    GiantBufs giants;        // giant-seed pass
     bool sort = false;
     int mode = kDedupRows;   // DedupMode of the body-pair wave-mode kernel
     Batch batch;
@@ -59,6 +60,7 @@ struct Slot {
         if (st) (void)hipStreamSynchronize(st);
         dedup.release();
         sortb.release();
+        giants.release();
         if (done) (void)hipEventDestroy(done);
         if (st) (void)hipStreamDestroy(st);
     }
@@ -87,6 +89,7 @@ struct Server : Handle {
     int32_t flags = 0;
     int64_t budget = 0;      // bytes for in-flight batches
     int64_t reservedNow = 0;
+    int64_t slotBytes = 0;   // HBM each slot holds for its whole life (dedup, sort and giant buffers)
     std::vector<std::unique_ptr<Slot>> slots;
     std::mutex mu;
     std::condition_variable cv;      // new requests / stop
@@ -107,6 +110,7 @@ struct Server : Handle {
             memBudget = (int64_t)(f / 2);
         }
         budget = memBudget;
+        slotBytes = 0;
         for (int i = 0; i < maxInFlight; ++i) {
             auto S = std::make_unique<Slot>();
             GR_HIP(hipStreamCreateWithFlags(&S->st, hipStreamNonBlocking));
@@ -123,9 +127,12 @@ struct Server : Handle {
                 if (!S->sortb.ensure(1)) return false;
                 bytes += S->sortb.bytes();
             }
+            if (!S->giants.ensure(S->rowWords)) return false;
+            bytes += S->giants.bytes();
             S->batch.magic = kMagicBatch;
             S->batch.ctx = C;
             budget -= (int64_t)bytes;
+            slotBytes = (int64_t)bytes;
             slots.push_back(std::move(S));
         }
         if (budget <= 0) {
@@ -190,7 +197,8 @@ struct Server : Handle {
     // The slot's batch -> its stream, with the slot's dedup rows or sort buffers.
     void launch(Slot& S) {
         batch_enqueue(*C, S.batch, ExecTarget{S.st, S.dedup.as<uint32_t>(), S.rowWords, std::max(1, S.waveSlots),
-                                              nullptr, S.sort ? &S.sortb : nullptr, C->sortBlocks(), S.mode});
+                                              nullptr, S.sort ? &S.sortb : nullptr, C->sortBlocks(), S.mode,
+                                              &S.giants, C->giantAfter});
         (void)hipEventRecord(S.done, S.st);
     }
 
@@ -361,6 +369,11 @@ int gpu_serve_stats(void* srv, int64_t* stats) {
     stats[6] = S->maxBatchSeen;
     stats[7] = S->inFlightPeak;
     return S->failed ? 1 : 0;
+}
+
+int64_t gpu_serve_slot_bytes(void* srv) {
+    Server* S = as_server(srv);
+    return S ? S->slotBytes : -1;
 }
 
 void gpu_serve_stop(void* srv) {

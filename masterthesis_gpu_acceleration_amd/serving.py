@@ -25,6 +25,7 @@ _SIGS = {
     "gpu_serve_poll": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "gpu_serve_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "gpu_serve_stop": (None, [ctypes.c_void_p]),
+    "gpu_serve_slot_bytes": (ctypes.c_int64, [ctypes.c_void_p]),
     "gpu_monotonic_ns": (ctypes.c_int64, []),
 }
 L._SIGS.update(_SIGS)
@@ -75,6 +76,10 @@ class Server:
         d = dict(zip(STAT_NAMES, (int(x) for x in s)))
         d["device_failure"] = rc == 1
         return d
+
+    def slot_bytes(self) -> int:
+        """HBM each execution slot holds for the server's life."""
+        return int(self.lib.gpu_serve_slot_bytes(self.h))
 
     def stop(self):
         if self.h:

@@ -212,3 +212,61 @@ def test_forced_cross_wave_split(engines, name, dedup):
         eng.set_tuning("dedup", 0)
         eng.set_tuning("split_after", 256)
         eng.set_tuning("use_graph", 1)
+
+
+@pytest.mark.parametrize("name", ["tiny_5", "small_paths_2k", "dense_conf_300", "dbpedia_shaped_5k"])
+@pytest.mark.parametrize("dedup", [1, 2, 3])
+def test_forced_giant_seeds(engines, name, dedup):
+    """giant_after 1 and budget 1: nearly every wave-mode seed is abandoned
+    after one step and listed as a giant, cut into sub-items (level-0 or
+    level-1 ranges) that all waves run against the giant's shared bitmap
+    row, sort keys or found flag; every count stays exact."""
+    say("giants", name, dedup)
+    eng, fx, g, rules = engine_for(engines, name)
+    words = L.rule_words_for_graph(rules, g)
+    eng.set_tuning("budget", 1)
+    eng.set_tuning("dedup", dedup)
+    eng.set_tuning("giant_after", 1)
+    try:
+        giants = 0
+        for flags, graph in ((L.UNGATED, 1), (L.UNGATED, 0), (0, 1), (L.UNGATED, 1)):
+            eng.set_tuning("use_graph", graph)
+            res = eng.evaluate(words, flags)
+            check(fx, rules, res, not (flags & L.UNGATED), "%s/giant%d/%d" % (name, dedup, flags))
+            giants += eng.get_tuning("last_giants")
+        say("giants listed", giants)
+        if name in ("dense_conf_300", "dbpedia_shaped_5k"):
+            assert giants > 0
+    finally:
+        eng.set_tuning("budget", 4096)
+        eng.set_tuning("dedup", 0)
+        eng.set_tuning("giant_after", 2048)
+        eng.set_tuning("use_graph", 1)
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_giant_seeds_random_rules(seed):
+    """Random rules with constants, reflexive heads, repeated and disconnected
+    atoms: with every wave-mode seed turned into a giant, counts equal the
+    oracle's."""
+    say("giants-random", seed)
+    g = W.tiny_random(9300 + seed, n_entities=12 + seed, n_predicates=3, n_triples=80 + 10 * seed)
+    rng = np.random.default_rng(300 + seed)
+    rules = [W.random_rule(rng, g.n_entities, g.n_predicates, 1 + k % 8, n_vars=3 + k % 4, p_const=0.15,
+                           closed=(k % 6 != 1), reflexive_head=(k % 13 == 3)) for k in range(120)]
+    exp = Oracle(g.s, g.p, g.o).eval(W.encode_rules(rules))
+    eng = L.Engine.from_graph(g)
+    try:
+        words = L.rule_words_for_graph(rules, g)
+        eng.set_tuning("budget", 1)
+        eng.set_tuning("giant_after", 1)
+        for dedup in (1, 3):
+            eng.set_tuning("dedup", dedup)
+            res = eng.evaluate(words, L.UNGATED)
+            for i in range(len(rules)):
+                tag = (seed, dedup, i, W.rule_text(rules[i]))
+                assert res[i]["support"] == exp[i][0], tag
+                assert res[i]["bodyPairs"] == exp[i][3], tag
+                assert res[i]["bodySize"] == exp[i][4], tag
+    finally:
+        eng.close()

@@ -77,20 +77,18 @@ def test_alone_and_mixed_match_batch(setup):
 def test_memory_admission(setup):
     say("serve admission")
     eng, words, ref = setup
-    # HBM bitmap rows per slot, so the budget below is the slots' rows plus
-    # room for about one small request
-    eng.set_tuning("dedup", 1)
+    # budget: what the two slots hold for their life plus room for about one small request
+    probe = S.Server(eng, max_batch=64, max_in_flight=2)
+    slot = probe.slot_bytes()
+    probe.stop()
+    assert slot > 0
+    srv = S.Server(eng, max_batch=64, max_in_flight=2, mem_budget=2 * slot + 200_000)
     try:
-        rows = eng.get_tuning("wave_slots") * ((5060 + 31) // 32) * 4 * 2
-        srv = S.Server(eng, max_batch=64, max_in_flight=2, mem_budget=rows + 200_000)
-        try:
-            srv.submit(words[:80], np.arange(80, dtype=np.uint64))
-            out = collect(srv, 80)
-            st = srv.stats()
-        finally:
-            srv.stop()
+        srv.submit(words[:80], np.arange(80, dtype=np.uint64))
+        out = collect(srv, 80)
+        st = srv.stats()
     finally:
-        eng.set_tuning("dedup", 0)
+        srv.stop()
     ok = out["result"]["status"] == 0
     refused = out["result"]["status"] == 3
     assert (ok | refused).all()
