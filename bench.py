@@ -150,7 +150,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-per-rule", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--tune", default="", help="BUDGET,THREAD_WAVES_PER_CU,WAVE_WAVES_PER_CU: fixed tuning, no autotune")
+    ap.add_argument("--tune", default="",
+                    help="BUDGET,THREAD_WAVES_PER_CU,WAVE_WAVES_PER_CU[,DEDUP[,GIANT_AFTER]]: fixed tuning, no autotune")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run the streaming copy yardstick over this many bytes (counter scaling) and exit")
     args = ap.parse_args()
@@ -185,11 +186,10 @@ def main():
 
     flags = 0 if args.gated else L.UNGATED
     if args.tune:
-        bud, tw, ww = (int(x) for x in args.tune.split(","))
-        eng.set_tuning("budget", bud)
-        eng.set_tuning("thread_waves_per_cu", tw)
-        eng.set_tuning("wave_waves_per_cu", ww)
-        log("fixed tuning: budget %d, thread waves/CU %d, wave waves/CU %d" % (bud, tw, ww))
+        knobs = [int(x) for x in args.tune.split(",")]
+        for key, val in zip(("budget", "thread_waves_per_cu", "wave_waves_per_cu", "dedup", "giant_after"), knobs):
+            eng.set_tuning(key, val)
+        log("fixed tuning: %s" % args.tune)
     elif not args.no_autotune:
         t = time.time()
         tl = eng.autotune(words, flags)
@@ -197,7 +197,7 @@ def main():
         log("autotune (%.1fs, %d trials): budget %d, thread waves/CU %d, wave waves/CU %d -> %.3f ms" % (
             time.time() - t, len(tl), best[0], best[1], best[2], best[3]))
         for r in tl:
-            log("   trial budget %6d tw %2d ww %2d  %.3f ms  escalated %d" % tuple(r))
+            log("   trial budget %6d tw %2d ww %2d  %.3f ms  escalated %d  pairs dedup %d  giant_after %d" % tuple(r))
     t = time.time()
     batch = eng.batch(words, flags)
     build_s = time.time() - t
@@ -263,7 +263,8 @@ def main():
         "body_pairs_total": pairs,
         "rules_with_support": int((res["support"] > 0).sum()),
         "escalated_seeds": int(heavy),
-        "tuning": {k: eng.get_tuning(k) for k in ("budget", "thread_waves_per_cu", "wave_waves_per_cu")},
+        "tuning": {k: eng.get_tuning(k) for k in ("budget", "thread_waves_per_cu", "wave_waves_per_cu", "pairs_dedup",
+                                                  "giant_after", "wave_slots")},
         "e2e_rules_per_s": round(n / e2e_s, 1),
         "batch_build_s": round(build_s, 4),
         "latency_ms": {"batched_p50": round(pct([x for x in dev_ms], 0.5), 4),

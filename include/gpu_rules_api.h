@@ -149,21 +149,27 @@ double gpu_copy_probe(void* ctx, int64_t bytes);
 /* Autotuner: times the given rules (a sample of the expected workload)
  * under candidate settings -- thread-mode work budget per seed (which
  * decides how much work stays one-lane-per-seed and how much moves to
- * wave64 mode), thread-mode and wave-mode waves per CU -- by coordinate
- * descent, and keeps the fastest in the context. If `log` is given it
- * receives up to logCap rows of (budget, thread waves/CU, wave waves/CU, ms,
- * escalated seeds); *logLen gets the row count. 0 or -1. */
+ * wave64 mode), thread-mode and wave-mode waves per CU, head-object bitmaps
+ * in HBM rows or LDS (auto dedup only), and the steps before a wave-mode
+ * seed is cut into sub-items -- by coordinate descent, and keeps the fastest
+ * in the context. If `log` is given it receives up to logCap rows of
+ * (budget, thread waves/CU, wave waves/CU, ms, escalated seeds, pairs dedup
+ * mode, giant_after); *logLen gets the row count. 0 or -1. */
 int gpu_autotune(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags, double* log, int32_t logCap,
                  int32_t* logLen);
 
 /* Tuning knobs: "budget" (thread-mode candidates per seed before a seed is
  * moved to wave mode), "thread_waves_per_cu", "wave_waves_per_cu",
+ * "giant_after" (wave-mode steps before a seed is cut into sub-items that
+ * all waves share; 0 never),
  * "use_graph" (0/1), "dedup" (body-pair dedup of wave-mode seeds: 0 auto --
  * bitmap rows unless one nE-bit row per CU exceeds 8 GiB, then sort-unique;
  * 1 bitmap rows; 2 sort-unique), "sort_cap" (initial key capacity of the
  * sort-unique buffers, >= 4096; they grow on overflow). Returns 0, or -1 for
  * an unknown key or value. gpu_get_tuning also reads "num_cus",
- * "wave_slots", "sort_active" (0/1) and "sort_cap". */
+ * "wave_slots", "sort_active" (0/1), "sort_cap", "pairs_dedup" (the
+ * DedupMode in use: 0 HBM rows, 1 sort, 2 LDS) and "last_giants" (seeds cut
+ * into sub-items by the last batch). */
 int gpu_set_tuning(void* ctx, const char* key, int64_t value);
 int64_t gpu_get_tuning(void* ctx, const char* key);
 

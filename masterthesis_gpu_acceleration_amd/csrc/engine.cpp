@@ -89,6 +89,7 @@ bool Context::ensure_dedup() {
     rowsFit = rowBytes * numCUs <= kRowBudget;
     const size_t blockLds = wave_static_lds() + (size_t)rowBytes + 1024;
     ldsFits = blockLds <= ldsPerBlock;
+    ldsPreferred = ldsFits && ldsPerCU / blockLds >= (size_t)kLdsMinWaves;
     waveSlots = numCUs * waveWavesPerCU;
     const int mode = pairsDedup();
     if (mode == kDedupLds) {
@@ -111,8 +112,10 @@ bool Context::ensure_dedup() {
     return giants.ensure(rowWords);
 }
 
-void giant_prepare(GiantBufs& gb, int after, GiantArgs& ga, hipStream_t s) {
+void giant_prepare(GiantBufs& gb, int after, GiantArgs& ga, hipStream_t s, int phase) {
     uint32_t* ctl = gb.ctl.as<uint32_t>();
+    ga.stats = gb.stats.as<unsigned long long>() + phase * kWaveStats;
+    launch_fill_u64(ga.stats, kWaveStats, 0, s);
     launch_fill_u32(ctl, 4, 0, s);
     launch_fill_u32(gb.remaining.as<uint32_t>(), kGiantCap, 0, s);
     launch_fill_u32(gb.found.as<uint32_t>(), kGiantCap, 0, s);
@@ -305,6 +308,8 @@ bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const s
     }
     if (!B.hCtl) GR_HIP(hipHostMalloc((void**)&B.hCtl, kCtlWords * 4, hipHostMallocDefault));
     if (!B.hGiant) GR_HIP(hipHostMalloc((void**)&B.hGiant, 8, hipHostMallocDefault));
+    if (!B.hWaveStats) GR_HIP(hipHostMalloc((void**)&B.hWaveStats, 2 * kWaveStats * 8, hipHostMallocDefault));
+    std::memset(B.hWaveStats, 0, 2 * kWaveStats * 8);
     B.hGiant[0] = B.hGiant[1] = 0;
     GR_HIP(hipStreamSynchronize(st));
     return true;
@@ -339,12 +344,14 @@ void batch_enqueue(Context& C, Batch& B, const ExecTarget& t) {
             WaveArgs wa{B.wplans.as<Plan>(), B.wrefs.as<ConflictRefs>(), B.heavy.as<HeavyItem>() + B.heavyOff[ph],
                         c + 1, B.heavyCap[ph], 0, c + 2, counts, t.dedup, t.rowWords, sink};
             const int mode = ph == kPairs ? t.pairsDedup : kDedupRows;
-            giant_prepare(*t.giants, t.giantAfter, wa.giant, st);
+            giant_prepare(*t.giants, t.giantAfter, wa.giant, st, ph);
             launch_wave_batch(ph, mode, wa, C.dg, t.waveBlocks, st);
             if (t.giantAfter > 0) {
                 launch_giants(ph, mode, wa, C.dg, t.waveBlocks, st);
                 (void)hipMemcpyAsync(B.hGiant + ph, wa.giant.count, 4, hipMemcpyDeviceToHost, st);
             }
+            (void)hipMemcpyAsync(B.hWaveStats + ph * kWaveStats, wa.giant.stats, kWaveStats * 8,
+                                 hipMemcpyDeviceToHost, st);
             if (sort) {
                 if (t.ev) (void)hipEventRecord(t.ev[6], st);
                 SortJob j{{t.sort->keys[0].as<unsigned long long>(), t.sort->keys[1].as<unsigned long long>()},
@@ -469,6 +476,7 @@ bool batch_run(Context& C, Batch& B, GpuRuleResult64* out) {
     C.lastHeavy = B.lastHeavy;
     C.lastEmitted = sb ? B.lastEmitted : 0;
     C.lastGiants = (int64_t)B.hGiant[0] + B.hGiant[1];
+    for (int k = 0; k < 2 * kWaveStats; ++k) C.lastWaveStats[k] = B.hWaveStats[k];
     for (int k = 0; k < 4; ++k) C.lastKernelMs[k] = B.kernelMs[k];
     return true;
 }
@@ -790,9 +798,9 @@ int gpu_autotune(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags, 
             }
             ms = best;
             if (log && nlog < logCap) {
-                double* r = log + 5 * nlog;
+                double* r = log + 7 * nlog;
                 r[0] = C->tune.budget; r[1] = C->threadWavesPerCU; r[2] = C->waveWavesPerCU; r[3] = ms;
-                r[4] = (double)B.lastHeavy;
+                r[4] = (double)B.lastHeavy; r[5] = C->pairsDedup(); r[6] = C->giantAfter;
             }
             ++nlog;
             return true;
@@ -825,6 +833,26 @@ int gpu_autotune(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags, 
         }
         C->waveWavesPerCU = bestWw;
         if (!C->ensure_dedup()) return -1;
+        // head-object bitmaps in HBM rows or in LDS (when they fit), unless fixed by the caller
+        if (C->dedupMode == 0 && C->ldsFits && !C->sortActive()) {
+            int bestMode = 0;
+            for (int m : {1, 3}) {
+                C->dedupMode = m;
+                if (!C->ensure_dedup()) return -1;
+                if (!trial(ms)) return -1;
+                if (ms < bestMs) { bestMs = ms; bestMode = m; }
+            }
+            C->dedupMode = bestMode;
+            if (!C->ensure_dedup()) return -1;
+        }
+        // steps before a wave-mode seed is cut into sub-items
+        int bestGa = C->giantAfter;
+        for (int gaf : {512, 2048, 8192}) {
+            C->giantAfter = gaf;
+            if (!trial(ms)) return -1;
+            if (ms < bestMs) { bestMs = ms; bestGa = gaf; }
+        }
+        C->giantAfter = bestGa;
         C->useGraph = graphWas;
         if (logLen) *logLen = std::min(nlog, logCap);
         return 0;
@@ -931,6 +959,12 @@ int64_t gpu_get_tuning(void* ctx, const char* key) {
     if (k == "pairs_dedup") return C->pairsDedup();
     if (k == "giant_after") return C->giantAfter;
     if (k == "last_giants") return C->lastGiants;
+    // wave-mode counters of the last batch: wave_<phase>_<seeds|steps|donations|leaf_draws|draws>
+    static const char* kStatNames[kWaveStats] = {"seeds", "steps", "donations", "leaf_draws", "draws"};
+    for (int ph = 0; ph < 2; ++ph)
+        for (int q = 0; q < kWaveStats; ++q)
+            if (k == std::string(ph ? "wave_pairs_" : "wave_support_") + kStatNames[q])
+                return (int64_t)C->lastWaveStats[ph * kWaveStats + q];
     if (k == "sort_cap") return C->sortb.cap;
     return -1;
 }

@@ -110,7 +110,7 @@ struct SortBufs {
 // are zeroed by fill kernels before each wave-mode phase.
 constexpr uint32_t kGiantCap = 512;
 struct GiantBufs {
-    DevBuf list, heavyIdx, subs, ctl, remaining, found, rows;
+    DevBuf list, heavyIdx, subs, ctl, remaining, found, rows, stats;
     int64_t rowWords = 0;
     bool ensure(int64_t words) {
         words = std::max<int64_t>(words, 1);
@@ -119,7 +119,8 @@ struct GiantBufs {
         const size_t rowBytes = (size_t)kGiantCap * (size_t)words * 4;
         if (!list.ensure(kGiantCap * sizeof(HeavyItem)) || !heavyIdx.ensure(kGiantCap * 4) ||
             !subs.ensure((size_t)kGiantCap * kSubsPerGiant * sizeof(SubItem)) || !ctl.ensure(16) ||
-            !remaining.ensure(kGiantCap * 4) || !found.ensure(kGiantCap * 4) || !rows.ensure(rowBytes))
+            !remaining.ensure(kGiantCap * 4) || !found.ensure(kGiantCap * 4) || !rows.ensure(rowBytes) ||
+            !stats.ensure(2 * kWaveStats * 8))
             return false;
         if (hipMemset(rows.p, 0, rowBytes) != hipSuccess) return false;
         rowWords = words;
@@ -127,7 +128,7 @@ struct GiantBufs {
     }
     void release() {
         list.release(); heavyIdx.release(); subs.release(); ctl.release(); remaining.release(); found.release();
-        rows.release();
+        rows.release(); stats.release();
         rowWords = 0;
     }
     size_t bytes() const {
@@ -168,6 +169,7 @@ struct Batch : Handle {
     int64_t lastHeavy = 0;
     int64_t lastEmitted = 0;   // sort path: keys the last pass emitted
     uint32_t* hGiant = nullptr;   // pinned: giants listed in each phase of the last pass
+    unsigned long long* hWaveStats = nullptr;   // pinned: kWaveStats counters per phase of the last pass
     ~Batch();
 };
 
@@ -183,6 +185,7 @@ struct Context : Handle {
     int dedupMode = 0;        // body-pair dedup: 0 auto, 1 HBM bitmap rows, 2 sort-unique, 3 LDS bitmap
     bool rowsFit = true;      // HBM bitmap rows fit kRowBudget at one wave per CU
     bool ldsFits = false;     // one nE-bit bitmap fits a wave-mode block's LDS
+    bool ldsPreferred = false;  // ... and still leaves room for kLdsMinWaves wave-mode waves per CU
     size_t ldsPerBlock = 65536, ldsPerCU = 65536;
     hipStream_t st = nullptr;
     hipEvent_t ev[8] = {};
@@ -201,6 +204,7 @@ struct Context : Handle {
     GiantBufs giants; // giant-seed pass
     int giantAfter = 2048;   // wave-mode steps on one seed before it is listed as a giant (0: never)
     int64_t lastGiants = 0;
+    unsigned long long lastWaveStats[2 * kWaveStats] = {};
     Batch* lastBatch = nullptr;
     double lastMs = 0, lastSupMs = 0, lastPairMs = 0, lastSortMs = 0;
     double lastKernelMs[4] = {};
@@ -214,7 +218,7 @@ struct Context : Handle {
     // DedupMode of the body-pair wave-mode kernels
     int pairsDedup() const {
         if (sortActive()) return kDedupSort;
-        if ((dedupMode == 0 || dedupMode == 3) && ldsFits) return kDedupLds;
+        if ((dedupMode == 0 && ldsPreferred) || (dedupMode == 3 && ldsFits)) return kDedupLds;
         return kDedupRows;
     }
 
@@ -231,6 +235,7 @@ inline Batch::~Batch() {
     if (hCounts) (void)hipHostFree(hCounts);
     if (hCtl) (void)hipHostFree(hCtl);
     if (hGiant) (void)hipHostFree(hGiant);
+    if (hWaveStats) (void)hipHostFree(hWaveStats);
     if (ctx && ctx->lastBatch == this) ctx->lastBatch = nullptr;
 }
 
@@ -268,11 +273,16 @@ struct ExecTarget {
 
 // Zeroes the giant pass's counters and flags for one wave-mode phase and
 // fills a GiantArgs for it (after == 0: no seed is listed).
-void giant_prepare(GiantBufs& gb, int after, GiantArgs& ga, hipStream_t s);
+void giant_prepare(GiantBufs& gb, int after, GiantArgs& ga, hipStream_t s, int phase);
 
 // Bytes of bitmap rows allowed before body-pair dedup switches to the
 // sort-unique path (auto mode).
 constexpr int64_t kRowBudget = int64_t(8) << 30;
+// Auto mode keeps head-object bitmaps in LDS only while that leaves at
+// least this many wave-mode waves per CU: wave mode is latency-bound, and on
+// the 500k-term graph LDS bitmaps (2 waves per CU) ran 2.4x slower than HBM
+// rows (16 waves per CU).
+constexpr int kLdsMinWaves = 8;
 
 // Keys emitted by B's last pass (sort path), from its host control copy.
 int64_t batch_emitted(const Batch& B);

@@ -518,7 +518,43 @@ struct Dedup {
         if (D == kDedupRows) atomicAnd(bits + w, 0u);
         else bits[w] = 0u;
     }
+    // The bitmap word holding x (bitmap modes), for tests batched ahead of the marks.
+    __device__ __forceinline__ uint32_t word(int x) const {
+        if (shared) return __hip_atomic_load(shared + (x >> 5), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (D == kDedupRows) return __hip_atomic_load(bits + (x >> 5), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return bits[x >> 5];
+    }
 };
+
+// Marks the head objects of a batch of last-level groundings (ho[k] >= 0;
+// -1: none). The word tests are issued together, then the marks together,
+// so a batch costs two memory round trips instead of two per candidate.
+// Returns the objects this lane marked first; on the sort path emit[k] gets
+// the objects to emit.
+template <int D, int U>
+__device__ __forceinline__ uint32_t mark_batch(Dedup<D>& dd, const int (&ho)[U], int (&emit)[U]) {
+    uint32_t n = 0;
+    if (D == kDedupSort) {
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            emit[k] = -1;
+            if (ho[k] >= 0 && !dd.seen(ho[k])) {
+                dd.insert(ho[k]);
+                emit[k] = ho[k];
+            }
+        }
+        return 0;
+    }
+    uint32_t w[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) w[k] = ho[k] >= 0 ? dd.word(ho[k]) : 0xffffffffu;
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+        emit[k] = -1;
+        if (ho[k] >= 0 && !((w[k] >> (ho[k] & 31)) & 1u)) n += dd.insert(ho[k]);
+    }
+    return n;
+}
 
 // Sort path: wave-aggregated append of this step's emitted head objects as
 // keys keyHi | object. Called by the whole wave.
@@ -578,6 +614,7 @@ __device__ __forceinline__ uint64_t wave_run(WaveLds& W, const DevGraph& G, cons
     uint32_t gen = 0xffffffffu;     // pool whose bindings v holds (idle lanes)
     uint64_t mine = 0;
     int steps = 0;
+    uint32_t nDonate = 0, nLeafDraw = 0, nDraw = 0;
     for (;;) {
         // ---- A: idle lanes draw from the pool
         const uint64_t idleM = __ballot(!busy);
@@ -600,6 +637,7 @@ __device__ __forceinline__ uint64_t wave_run(WaveLds& W, const DevGraph& G, cons
                     gen = pg;
                 }
                 if (plv == last) {
+                    ++nLeafDraw;
                     next = pn + n * kLeafUnroll < pe ? pn + n * kLeafUnroll : pe;
                     int xs[kLeafUnroll];
 #pragma unroll
@@ -608,27 +646,36 @@ __device__ __forceinline__ uint64_t wave_run(WaveLds& W, const DevGraph& G, cons
                         xs[k] = (!busy && i < pe) ? G.col[i] : 0;
                     }
                     bool hit = false;
+                    int hos[kLeafUnroll];
 #pragma unroll
                     for (int k = 0; k < kLeafUnroll; ++k) {
                         const int i = pn + r + n * k;
-                        int emit = -1;
+                        hos[k] = -1;
                         if (!busy && i < pe && bind_entry_x(P.lev[last], G, R, v, i, xs[k])) {
-                            if (!pairs) {
-                                found = true;
-                            } else {
-                                const int ho = term_value(v, P.hoRef);
-                                hit = true;
-                                if (!dd.seen(ho)) {
-                                    mine += dd.insert(ho);
-                                    emit = ho;
-                                }
-                            }
+                            if (!pairs) found = true;
+                            else hos[k] = term_value(v, P.hoRef);
                         }
-                        if (D == kDedupSort && pairs) emit_keys(sink, keyHi, emit);
+                    }
+                    if (pairs) {
+                        if (pl < last) {   // one head object for the whole pool: mark it once
+#pragma unroll
+                            for (int k = 1; k < kLeafUnroll; ++k)
+                                if (hos[0] < 0) hos[0] = hos[k];
+#pragma unroll
+                            for (int k = 1; k < kLeafUnroll; ++k) hos[k] = -1;
+                        }
+                        int emits[kLeafUnroll];
+                        mine += mark_batch<D, kLeafUnroll>(dd, hos, emits);
+                        hit = hos[0] >= 0;
+                        if (D == kDedupSort) {
+#pragma unroll
+                            for (int k = 0; k < kLeafUnroll; ++k) emit_keys(sink, keyHi, emits[k]);
+                        }
                     }
                     // head object bound above the pool and now counted: the rest adds nothing
                     if (pairs && pl < last && __any(hit)) next = pe;
                 } else {
+                    ++nDraw;
                     next = pn + n < pe ? pn + n : pe;
                     const int i = pn + r;
                     if (!busy && i < pe) {
@@ -668,29 +715,34 @@ __device__ __forceinline__ uint64_t wave_run(WaveLds& W, const DevGraph& G, cons
                     int xs[kLeafBatch];
 #pragma unroll
                     for (int k = 0; k < kLeafBatch; ++k) xs[k] = k < take ? G.col[fr.i + k] : 0;
-                    bool stop = false;
+                    int hos[kLeafBatch];
+                    bool any = false;
 #pragma unroll
                     for (int k = 0; k < kLeafBatch; ++k) {
-                        if (stop || k >= take || !bind_entry_x(P.lev[d], G, R, v, fr.i + k, xs[k])) continue;
-                        if (!pairs) {
+                        hos[k] = -1;
+                        if (k < take && bind_entry_x(P.lev[d], G, R, v, fr.i + k, xs[k])) {
+                            hos[k] = pairs ? term_value(v, P.hoRef) : 0;
+                            any = true;
+                        }
+                    }
+                    if (!pairs) {
+                        if (any) {
                             found = true;
                             busy = false;
-                            stop = true;
-                            continue;
                         }
-                        const int ho = term_value(v, P.hoRef);
-                        if (d == pl) {
-                            if (!dd.seen(ho)) {
-                                mine += dd.insert(ho);
-                                emits[k] = ho;
+                    } else {
+                        if (d != pl) {   // head object bound above: one grounding settles it
+#pragma unroll
+                            for (int k = 1; k < kLeafBatch; ++k)
+                                if (hos[0] < 0) hos[0] = hos[k];
+#pragma unroll
+                            for (int k = 1; k < kLeafBatch; ++k) hos[k] = -1;
+                            if (any) {
+                                if (pl >= base) d = pl;   // head object settled: next candidate at the pair level
+                                else busy = false;
                             }
-                        } else {
-                            mine += dd.insert(ho);
-                            emits[k] = ho;
-                            if (pl >= base) d = pl;   // head object settled: next candidate at the pair level
-                            else busy = false;
-                            stop = true;
                         }
+                        mine += mark_batch<D, kLeafBatch>(dd, hos, emits);
                     }
                 } else {
                     W.frame[d][lane].i = fr.i + 1;
@@ -773,6 +825,7 @@ __device__ __forceinline__ uint64_t wave_run(WaveLds& W, const DevGraph& G, cons
             }
             const int donor = (int)(key & 255u);
             if ((key >> 8) != 0) {
+                ++nDonate;
                 if (lane == donor) {
                     const Frame f = W.frame[lvl][lane];
                     W.poolLevel = lvl;
@@ -786,6 +839,13 @@ __device__ __forceinline__ uint64_t wave_run(WaveLds& W, const DevGraph& G, cons
                 wave_sync();
             }
         }
+    }
+    if (ga.stats && lane == 0) {
+        atomicAdd(ga.stats + 0, 1ull);
+        atomicAdd(ga.stats + 1, (unsigned long long)steps);
+        atomicAdd(ga.stats + 2, (unsigned long long)nDonate);
+        atomicAdd(ga.stats + 3, (unsigned long long)nLeafDraw);
+        atomicAdd(ga.stats + 4, (unsigned long long)nDraw);
     }
     if (!pairs) {
         if (aborted || stopped || !__any(found)) return 0;

@@ -83,7 +83,12 @@ struct GiantArgs {
     uint32_t* remaining;     // [cap] sub-items of each giant not yet finished
     uint32_t* found;         // [cap] SUPPORT: a sub-item found a grounding
     uint32_t* rows;          // [cap] HBM bitmap rows (rowWords each), all zero between uses
+    unsigned long long* stats;   // kWaveStats counters of the launch (wave and sub-item kernels), or null
 };
+
+// Wave-mode counters of one launch: seeds and sub-items run, loop steps,
+// donations, pool draws at the last level, pool draws above it.
+constexpr int kWaveStats = 5;
 
 // Wave-mode pass over the heavy queue.
 struct WaveArgs {

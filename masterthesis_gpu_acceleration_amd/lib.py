@@ -245,9 +245,10 @@ class Engine:
 
     def autotune(self, rule_words: np.ndarray, flags: int = 0):
         """Runs the library's autotuner on these rules; returns its trial log
-        as rows of (budget, thread waves/CU, wave waves/CU, ms, escalated)."""
+        as rows of (budget, thread waves/CU, wave waves/CU, ms, escalated,
+        pairs dedup mode, giant_after)."""
         w = np.ascontiguousarray(rule_words, np.int32)
-        log = np.zeros((64, 5), np.float64)
+        log = np.zeros((64, 7), np.float64)
         n = ctypes.c_int32(0)
         if self.lib.gpu_autotune(self.ctx, w.ctypes.data, int(w.shape[0]), int(flags), log.ctypes.data, 64,
                                  ctypes.byref(n)) != 0:
