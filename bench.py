@@ -147,6 +147,8 @@ def main():
     ap.add_argument("--rate", type=float, default=150.0, help="requests per second (serving mode)")
     ap.add_argument("--max-work", type=float, default=1e8,
                     help="serving: drop requests whose estimated search work exceeds this (0: keep all)")
+    ap.add_argument("--compare-dedup", action="store_true",
+                    help="also time the batch under each body-pair dedup strategy (default for dense_conf)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-per-rule", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
@@ -274,6 +276,31 @@ def main():
                      "traffic_source": src, "l2_hit": l2hit, "kernel": names[dom], "kernel_ms": round(float(kms[dom]), 4),
                      "algorithmic_bytes": bytes_phase, "kernel_ms_all": [round(float(x), 4) for x in kms]},
     }
+
+    if args.compare_dedup or args.workload == "dense_conf":
+        # body-pair dedup strategies on the same batch (eager passes, best of 3)
+        cmp = {}
+        eng.set_tuning("use_graph", 0)
+        for mode, label in ((1, "bitmap_hbm_rows"), (3, "bitmap_lds"), (2, "sort_unique")):
+            eng.set_tuning("dedup", mode)
+            if mode == 3 and eng.get_tuning("pairs_dedup") != 2:
+                continue   # the bitmap does not fit a block's LDS
+            eng.evaluate(words, flags)
+            times = []
+            for _ in range(3):
+                t = time.perf_counter()
+                r = eng.evaluate(words, flags)
+                times.append(time.perf_counter() - t)
+            k = eng.kernel_ms()
+            ds = eng.dedup_stats()
+            same = bool(np.array_equal(r["bodyPairs"], res["bodyPairs"]) and np.array_equal(r["support"], res["support"]))
+            cmp[label] = {"ms": round(min(times) * 1e3, 3), "pairs_wave_ms": round(float(k[3]), 3),
+                          "sort_ms": round(ds["sort_ms"], 3), "keys_emitted": int(ds["keys_emitted"]),
+                          "same_counts": same}
+            log("dedup %s: %s" % (label, cmp[label]))
+        eng.set_tuning("dedup", 0)
+        eng.set_tuning("use_graph", 1)
+        out["dedup_compare"] = cmp
 
     if not args.no_per_rule:
         t = time.perf_counter()

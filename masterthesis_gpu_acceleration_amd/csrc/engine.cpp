@@ -258,6 +258,8 @@ bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const s
     std::vector<ConflictRefs> refs(np), wrefs(np);
     std::vector<Chunk> lists[2];
     uint64_t items[2] = {0, 0};
+    struct Job { int ph; size_t k; int64_t n; };
+    std::vector<Job> jobs;
     auto add = [&](int ph, size_t i, const Plan& P, const ConflictRefs& R, const Plan& W, const ConflictRefs& RW,
                    int64_t n) {
         const size_t k = 2 * i + ph;
@@ -266,9 +268,7 @@ bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const s
         refs[k] = R;
         wrefs[k] = RW;
         plans[k].rule = wplans[k].rule = (int32_t)i;
-        for (int64_t b = 0; b < n; b += kChunkSeeds)
-            lists[ph].push_back(Chunk{(uint32_t)k, (uint32_t)b, (uint32_t)std::min<int64_t>(kChunkSeeds, n - b),
-                                      (ph == 1 && gated) ? 1u : 0u});
+        jobs.push_back(Job{ph, k, n});
         items[ph] += (uint64_t)n;
     };
     for (size_t i = 0; i < rules.size(); ++i) {
@@ -281,6 +281,20 @@ bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const s
             add(0, i, cr.sup, cr.supRefs, cr.supWave, cr.supWaveRefs, cr.supItems);
         if (cr.pairsLaunch && cr.pairItems > 0)
             add(1, i, cr.pairs, cr.pairsRefs, cr.pairsWave, cr.pairsWaveRefs, cr.pairItems);
+    }
+    // Seeds per thread-mode work unit: 1024 (fewer plan loads and cursor
+    // atomics) unless that leaves fewer than two units per thread-mode wave,
+    // down to 64 -- a batch of few rules with few, costly seeds each still
+    // spreads over the whole GPU.
+    const uint64_t waves = (uint64_t)C.numCUs * (uint64_t)C.threadWavesPerCU;
+    for (int ph = 0; ph < 2; ++ph) {
+        int64_t chunk = kChunkSeeds;
+        while (chunk > kMinChunkSeeds && items[ph] / (uint64_t)chunk < 2 * waves) chunk >>= 1;
+        for (const Job& j : jobs)
+            if (j.ph == ph)
+                for (int64_t b = 0; b < j.n; b += chunk)
+                    lists[ph].push_back(Chunk{(uint32_t)j.k, (uint32_t)b, (uint32_t)std::min<int64_t>(chunk, j.n - b),
+                                              (ph == 1 && gated) ? 1u : 0u});
     }
     std::vector<Chunk> all;
     uint64_t heavyTotal = 0;
@@ -783,16 +797,17 @@ int gpu_autotune(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags, 
         std::vector<int32_t> st(n, kRuleOk);
         for (int32_t i = 0; i < n; ++i) rs[i] = rule_from_words(rules[i].words);
         const bool graphWas = C->useGraph;
-        C->useGraph = false;
+        C->useGraph = true;
         int32_t nlog = 0;
-        // one trial: compile with the current knobs, run eagerly, keep the best of 2
+        // one trial: compile with the current knobs, capture the HIP graph, best of 3 replays
         auto trial = [&](double& ms) -> bool {
             Batch B;
             B.magic = kMagicBatch;
             if (!batch_build(*C, B, rs, st, flags & GPU_RULES_UNGATED, C->st)) return false;
             std::vector<GpuRuleResult64> out(n);
+            if (!batch_run(*C, B, out.data())) return false;
             double best = 1e30;
-            for (int k = 0; k < 2; ++k) {
+            for (int k = 0; k < 3; ++k) {
                 if (!batch_run(*C, B, out.data())) return false;
                 best = std::min(best, B.lastMs);
             }

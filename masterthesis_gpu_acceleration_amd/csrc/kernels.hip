@@ -597,7 +597,8 @@ template <int K, int D>
 __device__ __forceinline__ uint64_t wave_run(WaveLds& W, const DevGraph& G, const ConflictRefs* R, Dedup<D>& dd,
                                              int64_t rowWords, const SortSink& sink, unsigned long long keyHi,
                                              const GiantArgs& ga, bool mayGiant, HeavyItem self, uint32_t heavyIdx,
-                                             uint32_t* foundFlag, bool& aborted) {
+                                             uint32_t* foundFlag, bool& aborted,
+                                             unsigned long long (&stats)[kWaveStats]) {
     const int lane = lane_id();
     const Plan& P = W.plan;
     const int nlev = P.nlev;
@@ -840,13 +841,11 @@ __device__ __forceinline__ uint64_t wave_run(WaveLds& W, const DevGraph& G, cons
             }
         }
     }
-    if (ga.stats && lane == 0) {
-        atomicAdd(ga.stats + 0, 1ull);
-        atomicAdd(ga.stats + 1, (unsigned long long)steps);
-        atomicAdd(ga.stats + 2, (unsigned long long)nDonate);
-        atomicAdd(ga.stats + 3, (unsigned long long)nLeafDraw);
-        atomicAdd(ga.stats + 4, (unsigned long long)nDraw);
-    }
+    stats[0] += 1;
+    stats[1] += (unsigned long long)steps;
+    stats[2] += nDonate;
+    stats[3] += nLeafDraw;
+    stats[4] += nDraw;
     if (!pairs) {
         if (aborted || stopped || !__any(found)) return 0;
         if (!foundFlag) return 1;
@@ -873,7 +872,7 @@ template <int K, int D>
 __device__ __forceinline__ uint64_t wave_seed(WaveLds& W, const DevGraph& G, const ConflictRefs* R, uint32_t item,
                                               Dedup<D>& dd, int64_t rowWords, const SortSink& sink,
                                               unsigned long long keyHi, const GiantArgs& ga, uint32_t planIdx,
-                                              uint32_t heavyIdx) {
+                                              uint32_t heavyIdx, unsigned long long (&stats)[kWaveStats]) {
     const int lane = lane_id();
     const Plan& P = W.plan;
     int v[kMaxSlots];
@@ -893,7 +892,7 @@ __device__ __forceinline__ uint64_t wave_seed(WaveLds& W, const DevGraph& G, con
     wave_sync();
     bool aborted = false;
     return wave_run<K, D>(W, G, R, dd, rowWords, sink, keyHi, ga, ga.after > 0, HeavyItem{planIdx, item}, heavyIdx,
-                          nullptr, aborted);
+                          nullptr, aborted, stats);
 }
 
 // ------------------------------------------------------------- kernels
@@ -960,6 +959,13 @@ __device__ __forceinline__ Dedup<D> wave_dedup(WaveLds& W, int* cache, uint32_t*
     return dd;
 }
 
+// Adds a wave's counters to the launch's (once per wave).
+__device__ __forceinline__ void flush_stats(const GiantArgs& ga, const unsigned long long (&stats)[kWaveStats]) {
+    if (ga.stats && lane_id() == 0)
+        for (int q = 0; q < kWaveStats; ++q)
+            if (stats[q]) atomicAdd(ga.stats + q, stats[q]);
+}
+
 // Persistent wave mode over the heavy queue: one wave per block, one heavy
 // seed at a time.
 template <int K, int D>
@@ -972,6 +978,7 @@ __global__ void __launch_bounds__(kWave) k_wave_batch(WaveArgs a, DevGraph G) {
     uint32_t n = *a.heavyCount;
     if (n > a.heavyCap) n = a.heavyCap;
     uint32_t loaded = 0xffffffffu;
+    unsigned long long stats[kWaveStats] = {0, 0, 0, 0, 0};
     for (;;) {
         const uint32_t h = wave_take(a.cursor);
         if (h >= n) break;
@@ -981,9 +988,10 @@ __global__ void __launch_bounds__(kWave) k_wave_batch(WaveArgs a, DevGraph G) {
             loaded = it.plan;
         }
         const uint64_t c = wave_seed<K, D>(W, G, a.refs + it.plan, it.item, dd, a.rowWords, a.sink,
-                                           (unsigned long long)h << a.sink.hoBits, a.giant, it.plan, h);
+                                           (unsigned long long)h << a.sink.hoBits, a.giant, it.plan, h, stats);
         if (lane == 0 && c) atomicAdd(a.counts + 2 * W.plan.rule + K, (unsigned long long)c);
     }
+    flush_stats(a.giant, stats);
 }
 
 template <int K, int D>
@@ -996,13 +1004,15 @@ __global__ void __launch_bounds__(kWave) k_wave_one(Plan P, WaveArgs a, DevGraph
     Dedup<D> dd = wave_dedup<D>(W, cache, dyn, a);
     uint32_t n = *a.heavyCount;
     if (n > a.heavyCap) n = a.heavyCap;
+    unsigned long long stats[kWaveStats] = {0, 0, 0, 0, 0};
     for (;;) {
         const uint32_t h = wave_take(a.cursor);
         if (h >= n) break;
         const uint64_t c = wave_seed<K, D>(W, G, a.refs, a.heavy[h].item, dd, a.rowWords, a.sink,
-                                           (unsigned long long)h << a.sink.hoBits, a.giant, 0u, h);
+                                           (unsigned long long)h << a.sink.hoBits, a.giant, 0u, h, stats);
         if (lane == 0 && c) atomicAdd(a.counts, (unsigned long long)c);
     }
+    flush_stats(a.giant, stats);
 }
 
 // ------------------------------------------------------------ giant seeds
@@ -1110,6 +1120,7 @@ __global__ void __launch_bounds__(kWave) k_wave_sub(WaveArgs a, DevGraph G) {
     const uint32_t capSubs = ga.cap * (uint32_t)kSubsPerGiant;
     if (n > capSubs) n = capSubs;
     uint32_t loaded = 0xffffffffu;
+    unsigned long long stats[kWaveStats] = {0, 0, 0, 0, 0};
     for (;;) {
         const uint32_t si = wave_take(ga.subCursor);
         if (si >= n) break;
@@ -1141,7 +1152,7 @@ __global__ void __launch_bounds__(kWave) k_wave_sub(WaveArgs a, DevGraph G) {
             dd.shared = (K == kPairs && D != kDedupSort) ? ga.rows + (int64_t)g * a.rowWords : nullptr;
             bool aborted = false;
             c = wave_run<K, D>(W, G, a.refs + plan, dd, a.rowWords, a.sink, (unsigned long long)hv << a.sink.hoBits,
-                               ga, false, HeavyItem{plan, 0u}, hv, foundFlag, aborted);
+                               ga, false, HeavyItem{plan, 0u}, hv, foundFlag, aborted, stats);
             dd.shared = nullptr;
         }
         if (lane == 0 && c) atomicAdd(a.counts + 2 * W.plan.rule + K, (unsigned long long)c);
@@ -1152,6 +1163,7 @@ __global__ void __launch_bounds__(kWave) k_wave_sub(WaveArgs a, DevGraph G) {
             for (int64_t w = lane; w < a.rowWords; w += kWave) row[w] = 0u;   // read again only by later launches
         }
     }
+    flush_stats(ga, stats);
 }
 
 // Plain streaming copy: the yardstick that scales hardware byte counters.

@@ -23,7 +23,8 @@ namespace gr {
 constexpr int kMaxBody  = 8;    // body atoms per rule (GPU_03/gpu_common.cuh:25-27)
 constexpr int kMaxSlots = 16;   // distinct variables per rule (same source)
 constexpr int kWave     = 64;   // CDNA wavefront width
-constexpr int kChunkSeeds = 1024;   // seeds per thread-mode work unit
+constexpr int kChunkSeeds = 1024;   // seeds per thread-mode work unit, at most
+constexpr int kMinChunkSeeds = 64;  // ... and at least (small batches use smaller units)
 
 enum LevelKind : int32_t {
     kCheck = 0,   // both terms bound: binary search of one row

@@ -73,11 +73,11 @@ int64_t request_bytes(const CompiledRule& cr) {
     int64_t items = 0, chunks = 0;
     if (cr.supportLaunch) {
         items += cr.supItems;
-        chunks += (cr.supItems + kChunkSeeds - 1) / kChunkSeeds;
+        chunks += (cr.supItems + kMinChunkSeeds - 1) / kMinChunkSeeds;
     }
     if (cr.pairsLaunch) {
         items += cr.pairItems;
-        chunks += (cr.pairItems + kChunkSeeds - 1) / kChunkSeeds;
+        chunks += (cr.pairItems + kMinChunkSeeds - 1) / kMinChunkSeeds;
     }
     return items * (int64_t)sizeof(HeavyItem) + chunks * (int64_t)sizeof(Chunk) +
            4 * (int64_t)(sizeof(Plan) + sizeof(ConflictRefs)) + 16;
