@@ -83,8 +83,11 @@ bool Context::upload() {
 }
 
 bool Context::ensure_dedup() {
-    rowWords = (g.nE + 31) / 32;
-    if (rowWords < 1) rowWords = 1;
+    // wave-mode bitmaps index head objects by rank among the keys of one
+    // (direction, predicate): as many bits as the largest such key set
+    int64_t maxKeys = 1;
+    for (int32_t p = 0; p < g.P(); ++p) maxKeys = std::max<int64_t>(maxKeys, std::max(g.nkeys(0, p), g.nkeys(1, p)));
+    rowWords = (maxKeys + 31) / 32;
     const int64_t rowBytes = rowWords * 4;
     rowsFit = rowBytes * numCUs <= kRowBudget;
     const size_t blockLds = wave_static_lds() + (size_t)rowBytes + 1024;

@@ -486,13 +486,31 @@ __device__ __forceinline__ uint32_t ld_rlx(const uint32_t* p) {
 // For the bitmaps, the lane whose atomicOr finds a word still zero lists it
 // in LDS so that clearing touches only those words; insert returns 1 to the
 // one lane whose call set the bit, so the lanes' sums are the distinct count.
+// The bitmaps index a head object x by its rank among the keys of the
+// plan's head-object space (Plan::hoLo/hoHi/hoRow0/hoKeys): one key-index
+// block load and a popcount, so a bitmap needs only as many bits as the
+// largest key set of one (direction, predicate) -- not one per term id.
 template <int D>
 struct Dedup {
     uint32_t* bits;     // the wave's own bitmap: its HBM row (kDedupRows) or LDS (kDedupLds)
     uint32_t* shared;   // a giant's HBM row, shared by its sub-items (then used instead of bits), or null
     WaveLds* W;
     int* cache;
-    __device__ __forceinline__ bool seen(int x) const {
+    const KeyBlock* blocks;
+    // Rank of head object x (sort path: x itself). x is a key of the space by
+    // construction; the clamp only keeps a wrong plan from writing out of bounds.
+    __device__ __forceinline__ int rank(int x) const {
+        if (D == kDedupSort) return x;
+        const Plan& P = W->plan;
+        const int64_t base = (int64_t)(uint32_t)P.hoLo | ((int64_t)P.hoHi << 32);
+        const KeyBlock kb = blocks[base + (x >> 6)];
+        const uint64_t bit = 1ull << (x & 63);
+        const int r = (int)(kb.rank + (uint32_t)__popcll(kb.bits & (bit - 1))) - P.hoRow0;
+        return r < 0 ? 0 : (r >= P.hoKeys ? P.hoKeys - 1 : r);
+    }
+    __device__ __forceinline__ bool seen(int x) const { return seen_r(rank(x)); }
+    __device__ __forceinline__ uint32_t insert(int x) { return insert_r(rank(x)); }
+    __device__ __forceinline__ bool seen_r(int x) const {
         if (D == kDedupSort) return cache[x & (kCacheSlots - 1)] == x;
         uint32_t w;
         if (shared) w = __hip_atomic_load(shared + (x >> 5), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -500,7 +518,7 @@ struct Dedup {
         else w = bits[x >> 5];
         return (w >> (x & 31)) & 1u;
     }
-    __device__ __forceinline__ uint32_t insert(int x) {
+    __device__ __forceinline__ uint32_t insert_r(int x) {
         if (D == kDedupSort) {
             cache[x & (kCacheSlots - 1)] = x;
             return 0;
@@ -518,8 +536,8 @@ struct Dedup {
         if (D == kDedupRows) atomicAnd(bits + w, 0u);
         else bits[w] = 0u;
     }
-    // The bitmap word holding x (bitmap modes), for tests batched ahead of the marks.
-    __device__ __forceinline__ uint32_t word(int x) const {
+    // The bitmap word holding rank x (bitmap modes), for tests batched ahead of the marks.
+    __device__ __forceinline__ uint32_t word_r(int x) const {
         if (shared) return __hip_atomic_load(shared + (x >> 5), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (D == kDedupRows) return __hip_atomic_load(bits + (x >> 5), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return bits[x >> 5];
@@ -545,13 +563,16 @@ __device__ __forceinline__ uint32_t mark_batch(Dedup<D>& dd, const int (&ho)[U],
         }
         return 0;
     }
+    int r[U];
     uint32_t w[U];
 #pragma unroll
-    for (int k = 0; k < U; ++k) w[k] = ho[k] >= 0 ? dd.word(ho[k]) : 0xffffffffu;
+    for (int k = 0; k < U; ++k) r[k] = ho[k] >= 0 ? dd.rank(ho[k]) : -1;
+#pragma unroll
+    for (int k = 0; k < U; ++k) w[k] = r[k] >= 0 ? dd.word_r(r[k]) : 0xffffffffu;
 #pragma unroll
     for (int k = 0; k < U; ++k) {
         emit[k] = -1;
-        if (ho[k] >= 0 && !((w[k] >> (ho[k] & 31)) & 1u)) n += dd.insert(ho[k]);
+        if (r[k] >= 0 && !((w[k] >> (r[k] & 31)) & 1u)) n += dd.insert_r(r[k]);
     }
     return n;
 }
@@ -943,12 +964,14 @@ __global__ void __launch_bounds__(kBlock) k_thread_one(Plan P, OneArgs a, DevGra
 // wave's row), the bitmap in LDS (dynamic shared memory, cleared once here)
 // or the sort path's LDS cache.
 template <int D>
-__device__ __forceinline__ Dedup<D> wave_dedup(WaveLds& W, int* cache, uint32_t* dyn, const WaveArgs& a) {
+__device__ __forceinline__ Dedup<D> wave_dedup(WaveLds& W, int* cache, uint32_t* dyn, const WaveArgs& a,
+                                               const DevGraph& G) {
     Dedup<D> dd;
     dd.W = &W;
     dd.cache = cache;
     dd.bits = nullptr;
     dd.shared = nullptr;
+    dd.blocks = G.blocks;
     if (D == kDedupRows) dd.bits = a.rows + (int64_t)blockIdx.x * a.rowWords;
     if (D == kDedupLds) {
         dd.bits = dyn;
@@ -974,7 +997,7 @@ __global__ void __launch_bounds__(kWave) k_wave_batch(WaveArgs a, DevGraph G) {
     __shared__ int cache[D == kDedupSort ? kCacheSlots : 1];
     extern __shared__ uint32_t dyn[];
     const int lane = lane_id();
-    Dedup<D> dd = wave_dedup<D>(W, cache, dyn, a);
+    Dedup<D> dd = wave_dedup<D>(W, cache, dyn, a, G);
     uint32_t n = *a.heavyCount;
     if (n > a.heavyCap) n = a.heavyCap;
     uint32_t loaded = 0xffffffffu;
@@ -1001,7 +1024,7 @@ __global__ void __launch_bounds__(kWave) k_wave_one(Plan P, WaveArgs a, DevGraph
     extern __shared__ uint32_t dyn[];
     const int lane = lane_id();
     copy_plan(W, &P);
-    Dedup<D> dd = wave_dedup<D>(W, cache, dyn, a);
+    Dedup<D> dd = wave_dedup<D>(W, cache, dyn, a, G);
     uint32_t n = *a.heavyCount;
     if (n > a.heavyCap) n = a.heavyCap;
     unsigned long long stats[kWaveStats] = {0, 0, 0, 0, 0};
@@ -1114,7 +1137,7 @@ __global__ void __launch_bounds__(kWave) k_wave_sub(WaveArgs a, DevGraph G) {
     __shared__ WaveLds W;
     __shared__ int cache[D == kDedupSort ? kCacheSlots : 1];
     const int lane = lane_id();
-    Dedup<D> dd = wave_dedup<D>(W, cache, nullptr, a);
+    Dedup<D> dd = wave_dedup<D>(W, cache, nullptr, a, G);
     const GiantArgs& ga = a.giant;
     uint32_t n = *ga.subCount;
     const uint32_t capSubs = ga.cap * (uint32_t)kSubsPerGiant;

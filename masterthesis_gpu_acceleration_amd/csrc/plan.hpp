@@ -75,6 +75,11 @@ struct alignas(16) Plan {
     int32_t hoRef;       // SUPPORT: head object term ref; PAIRS: slot of the head object
     int32_t pairLevel;   // PAIRS: level binding the head object; -1 if bound by the seed
     int32_t budget;      // thread-mode candidates per seed before escalation
+    // PAIRS with pairLevel >= 0: the head objects are keys of one (direction,
+    // predicate) -- the values the pair level enumerates -- so wave-mode
+    // bitmaps index them by rank among those keys: key-index block run
+    // hoLo | hoHi << 32, global row number hoRow0 of its first key, hoKeys keys.
+    int32_t hoLo, hoHi, hoRow0, hoKeys;
     int32_t pad0, pad1;
     Level lev[kMaxBody];
 };

@@ -197,6 +197,33 @@ bool wave_order(const Plan& thread, std::vector<int>& ord) {
     return true;
 }
 
+// Rank space of a PAIRS plan's head objects: the values its pair level
+// enumerates are the keys of one (direction, predicate) -- objects (POS keys)
+// for an SPO row, subjects (SPO keys) for a POS row; for a SCAN, the side the
+// head object takes. Rows are numbered direction-major, then by predicate,
+// then by key (as the device key index is built), so the first key's row
+// number is the number of keys before it.
+void head_object_ranks(Plan& Q, const HostGraph& g) {
+    const Level& lv = Q.lev[Q.pairLevel];
+    const int32_t kd = level_kind(lv.flags);
+    const int64_t nB = g.nBlocks();
+    int32_t p, dir2;
+    if (kd == kScan) {
+        p = (int32_t)(std::upper_bound(g.predOff.begin(), g.predOff.end(), lv.lo) - g.predOff.begin()) - 1;
+        dir2 = lv.val == Q.hoRef ? kSPO : kPOS;   // the scan binds subject (val) and object (val2)
+    } else {
+        const int64_t base = (int64_t)(uint32_t)lv.lo | ((int64_t)lv.hi << 32);
+        const int32_t dir = (int32_t)(base / ((int64_t)g.P() * nB));
+        p = (int32_t)((base / nB) % g.P());
+        dir2 = dir == kSPO ? kPOS : kSPO;
+    }
+    const int64_t base2 = ((int64_t)dir2 * g.P() + p) * nB;
+    Q.hoLo = (int32_t)(uint32_t)(base2 & 0xffffffffLL);
+    Q.hoHi = (int32_t)(base2 >> 32);
+    Q.hoRow0 = dir2 == kSPO ? g.keysSOff[p] : (int32_t)g.keysS.size() + g.keysOOff[p];
+    Q.hoKeys = g.nkeys(dir2, p);
+}
+
 }  // namespace
 
 CompiledRule compile_rule(const RuleIR& r, const HostGraph& g, const Tuning& t) {
@@ -319,6 +346,7 @@ CompiledRule compile_rule(const RuleIR& r, const HostGraph& g, const Tuning& t) 
                         break;
                     }
                 }
+            if (Q.pairLevel >= 0) head_object_ranks(Q, g);
         };
         finish(P, B);
         c.pairItems = P.seedEnd - P.seedBegin;
