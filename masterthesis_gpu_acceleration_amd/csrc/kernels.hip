@@ -997,9 +997,10 @@ __global__ void __launch_bounds__(kWave) k_wave_batch(WaveArgs a, DevGraph G) {
     __shared__ int cache[D == kDedupSort ? kCacheSlots : 1];
     extern __shared__ uint32_t dyn[];
     const int lane = lane_id();
-    Dedup<D> dd = wave_dedup<D>(W, cache, dyn, a, G);
     uint32_t n = *a.heavyCount;
     if (n > a.heavyCap) n = a.heavyCap;
+    if (n == 0) return;   // nothing escalated: no cursor traffic
+    Dedup<D> dd = wave_dedup<D>(W, cache, dyn, a, G);
     uint32_t loaded = 0xffffffffu;
     unsigned long long stats[kWaveStats] = {0, 0, 0, 0, 0};
     for (;;) {
@@ -1023,10 +1024,11 @@ __global__ void __launch_bounds__(kWave) k_wave_one(Plan P, WaveArgs a, DevGraph
     __shared__ int cache[D == kDedupSort ? kCacheSlots : 1];
     extern __shared__ uint32_t dyn[];
     const int lane = lane_id();
-    copy_plan(W, &P);
-    Dedup<D> dd = wave_dedup<D>(W, cache, dyn, a, G);
     uint32_t n = *a.heavyCount;
     if (n > a.heavyCap) n = a.heavyCap;
+    if (n == 0) return;
+    copy_plan(W, &P);
+    Dedup<D> dd = wave_dedup<D>(W, cache, dyn, a, G);
     unsigned long long stats[kWaveStats] = {0, 0, 0, 0, 0};
     for (;;) {
         const uint32_t h = wave_take(a.cursor);
@@ -1137,11 +1139,12 @@ __global__ void __launch_bounds__(kWave) k_wave_sub(WaveArgs a, DevGraph G) {
     __shared__ WaveLds W;
     __shared__ int cache[D == kDedupSort ? kCacheSlots : 1];
     const int lane = lane_id();
-    Dedup<D> dd = wave_dedup<D>(W, cache, nullptr, a, G);
     const GiantArgs& ga = a.giant;
     uint32_t n = *ga.subCount;
     const uint32_t capSubs = ga.cap * (uint32_t)kSubsPerGiant;
     if (n > capSubs) n = capSubs;
+    if (n == 0) return;   // no giants listed
+    Dedup<D> dd = wave_dedup<D>(W, cache, nullptr, a, G);
     uint32_t loaded = 0xffffffffu;
     unsigned long long stats[kWaveStats] = {0, 0, 0, 0, 0};
     for (;;) {
