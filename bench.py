@@ -159,8 +159,9 @@ def main():
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (rules are independent; see "
-              "DESIGN.md for how the work would shard). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: this library has no multi-GPU path (rules are independent: "
+              "each GPU would hold the whole graph and take a share of the rules). Run with --gpus 1.",
+              file=sys.stderr)
         sys.exit(2)
 
     if args.copy_probe:

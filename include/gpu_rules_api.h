@@ -162,9 +162,12 @@ int gpu_autotune(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags, 
  * moved to wave mode), "thread_waves_per_cu", "wave_waves_per_cu",
  * "giant_after" (wave-mode steps before a seed is cut into sub-items that
  * all waves share; 0 never),
- * "use_graph" (0/1), "dedup" (body-pair dedup of wave-mode seeds: 0 auto --
- * bitmap rows unless one nE-bit row per CU exceeds 8 GiB, then sort-unique;
- * 1 bitmap rows; 2 sort-unique), "sort_cap" (initial key capacity of the
+ * "use_graph" (0/1), "dedup" (body-pair dedup of wave-mode seeds; the
+ * bitmaps index head objects by rank among the keys of one (direction,
+ * predicate): 0 auto -- LDS bitmaps when one fits a block and at least 8
+ * wave-mode waves per CU still fit, else HBM bitmap rows unless a row per CU
+ * exceeds 8 GiB, else sort-unique; 1 HBM bitmap rows; 2 sort-unique; 3 LDS
+ * bitmaps where they fit, else rows), "sort_cap" (initial key capacity of the
  * sort-unique buffers, >= 4096; they grow on overflow). Returns 0, or -1 for
  * an unknown key or value. gpu_get_tuning also reads "num_cus",
  * "wave_slots", "sort_active" (0/1), "sort_cap", "pairs_dedup" (the
