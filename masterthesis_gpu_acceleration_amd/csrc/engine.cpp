@@ -167,6 +167,7 @@ bool run_one(Context& C, const Plan& P, const ConflictRefs& R, const Plan& PW, c
     }
     const int mode = PW.kind == kPairs ? C.pairsDedup() : kDedupRows;
     const bool sort = mode == kDedupSort;
+    const int slots = std::max(plan_slots(P, R), plan_slots(PW, RW));
     if (sort && !C.sortb.ensure(1)) return false;
     const int hoBits = bits_for((uint64_t)C.g.nE);
     const int hBits = bits_for((uint64_t)items);
@@ -177,12 +178,12 @@ bool run_one(Context& C, const Plan& P, const ConflictRefs& R, const Plan& PW, c
         const int64_t waves = (items + kWave - 1) / kWave;
         const int blocks = (int)std::min<int64_t>((waves + 3) / 4, C.threadBlocks());
         OneArgs oa{conf ? d_refs : nullptr, (uint32_t)items, C.heavy1Cap, d_count, C.heavy1.as<HeavyItem>(), ctl + 0};
-        launch_thread_one(P.kind, P, oa, C.dg, blocks, C.st);
+        launch_thread_one(P.kind, slots, P, oa, C.dg, blocks, C.st);
         SortSink sink{};
         if (sort) sink = SortSink{C.sortb.keys[0].as<unsigned long long>(), ctl + 2, C.sortb.cap, hoBits};
         WaveArgs wa{nullptr, conf ? d_refs + 1 : nullptr, C.heavy1.as<HeavyItem>(), ctl + 0, C.heavy1Cap, 0, ctl + 1,
                     d_count, C.dedup.as<uint32_t>(), C.rowWords, sink};
-        launch_wave_one(PW.kind, mode, PW, wa, C.dg, C.waveBlocks(), C.st);
+        launch_wave_one(PW.kind, slots, mode, PW, wa, C.dg, C.waveBlocks(), C.st);
         if (sort) {
             SortJob j{{C.sortb.keys[0].as<unsigned long long>(), C.sortb.keys[1].as<unsigned long long>()},
                       ctl + 2, C.sortb.cap, sort_max_tiles(C.sortb.cap), C.sortb.hist.as<uint32_t>(),
@@ -252,6 +253,7 @@ bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const s
                  int32_t flags, hipStream_t st) {
     B.ctx = &C;
     B.n = (int32_t)rules.size();
+    B.slots[0] = B.slots[1] = 1;
     B.flags = flags;
     B.comp.assign(rules.size(), CompiledRule{});
     B.status = parseStatus;
@@ -271,6 +273,7 @@ bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const s
         refs[k] = R;
         wrefs[k] = RW;
         plans[k].rule = wplans[k].rule = (int32_t)i;
+        B.slots[ph] = std::max({B.slots[ph], plan_slots(P, R), plan_slots(W, RW)});
         jobs.push_back(Job{ph, k, n});
         items[ph] += (uint64_t)n;
     };
@@ -350,7 +353,7 @@ void batch_enqueue(Context& C, Batch& B, const ExecTarget& t) {
             ThreadArgs ta{B.plans.as<Plan>(), B.refs.as<ConflictRefs>(), B.chunks.as<Chunk>() + B.chunkOff[ph], nc,
                           B.heavyCap[ph], c + 0, counts, B.heavy.as<HeavyItem>() + B.heavyOff[ph], c + 1};
             const int blocks = std::min<int>(C.threadBlocks(), (int)((nc + 3) / 4));
-            launch_thread_batch(ph, ta, C.dg, std::max(blocks, 1), st);
+            launch_thread_batch(ph, B.slots[ph], ta, C.dg, std::max(blocks, 1), st);
         }
         if (t.ev) (void)hipEventRecord(t.ev[1 + 2 * ph], st);
         if (nc) {
@@ -362,9 +365,9 @@ void batch_enqueue(Context& C, Batch& B, const ExecTarget& t) {
                         c + 1, B.heavyCap[ph], 0, c + 2, counts, t.dedup, t.rowWords, sink};
             const int mode = ph == kPairs ? t.pairsDedup : kDedupRows;
             giant_prepare(*t.giants, t.giantAfter, wa.giant, st, ph);
-            launch_wave_batch(ph, mode, wa, C.dg, t.waveBlocks, st);
+            launch_wave_batch(ph, B.slots[ph], mode, wa, C.dg, t.waveBlocks, st);
             if (t.giantAfter > 0) {
-                launch_giants(ph, mode, wa, C.dg, t.waveBlocks, st);
+                launch_giants(ph, B.slots[ph], mode, wa, C.dg, t.waveBlocks, st);
                 (void)hipMemcpyAsync(B.hGiant + ph, wa.giant.count, 4, hipMemcpyDeviceToHost, st);
             }
             (void)hipMemcpyAsync(B.hWaveStats + ph * kWaveStats, wa.giant.stats, kWaveStats * 8,

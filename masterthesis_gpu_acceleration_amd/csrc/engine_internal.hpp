@@ -151,6 +151,7 @@ struct Batch : Handle {
     uint32_t heavyCap[2] = {};
     uint64_t heavyOff[2] = {};
     uint32_t capRules = 0;
+    int slots[2] = {1, 1};   // variable slots of the plans of each phase (picks the kernels' slot class)
     double kernelMs[4] = {};
     DevBuf plans, wplans, refs, wrefs, chunks, counts, heavy, ctl;
     unsigned long long* hCounts = nullptr;   // pinned

@@ -33,9 +33,9 @@ struct OneArgs {
 
 // Dedup strategy of the body-pair wave-mode kernels.
 enum DedupMode : int32_t {
-    kDedupRows = 0,   // per-wave bitmap row of nE bits in HBM (atomicOr)
+    kDedupRows = 0,   // per-wave bitmap row in HBM over head-object ranks (atomicOr)
     kDedupSort = 1,   // keys (seed << hoBits | head object) emitted to HBM, then radix sort + unique
-    kDedupLds = 2,    // per-wave bitmap of nE bits in LDS (dynamic shared memory)
+    kDedupLds = 2,    // the same bitmap per wave in LDS (dynamic shared memory)
 };
 
 // Emission target of the sort-unique path: a wave appends the head objects it
@@ -106,12 +106,16 @@ struct WaveArgs {
     GiantArgs giant;              // batch launches; after == 0 for per-rule launches
 };
 
-// `kind` is the plan kind (kSupport / kPairs) of every plan the launch sees.
-void launch_thread_batch(int kind, const ThreadArgs& a, const DevGraph& G, int blocks, hipStream_t s);
-void launch_thread_one(int kind, const Plan& P, const OneArgs& a, const DevGraph& G, int blocks, hipStream_t s);
+// `kind` is the plan kind (kSupport / kPairs) of every plan the launch sees;
+// `slots` is at least plan_slots() of each of them (the kernels of the
+// smallest slot class holding it run).
+void launch_thread_batch(int kind, int slots, const ThreadArgs& a, const DevGraph& G, int blocks, hipStream_t s);
+void launch_thread_one(int kind, int slots, const Plan& P, const OneArgs& a, const DevGraph& G, int blocks,
+                       hipStream_t s);
 // `dedup` (DedupMode) matters for kPairs only.
-void launch_wave_batch(int kind, int dedup, const WaveArgs& a, const DevGraph& G, int blocks, hipStream_t s);
-void launch_wave_one(int kind, int dedup, const Plan& P, const WaveArgs& a, const DevGraph& G, int blocks,
+void launch_wave_batch(int kind, int slots, int dedup, const WaveArgs& a, const DevGraph& G, int blocks,
+                       hipStream_t s);
+void launch_wave_one(int kind, int slots, int dedup, const Plan& P, const WaveArgs& a, const DevGraph& G, int blocks,
                      hipStream_t s);
 // Static LDS of a wave-mode block (bytes), and the dynamic-LDS limit to
 // request for the kDedupLds kernels.
@@ -119,7 +123,24 @@ size_t wave_static_lds();
 void prepare_wave_kernels(size_t dynLds);
 // The giant-seed pass that follows a wave-mode batch launch: expansion of
 // the listed giants, then their sub-items (same `a` as the wave launch).
-void launch_giants(int kind, int dedup, const WaveArgs& a, const DevGraph& G, int blocks, hipStream_t s);
+void launch_giants(int kind, int slots, int dedup, const WaveArgs& a, const DevGraph& G, int blocks, hipStream_t s);
+
+// Launchers of one slot class (kernels with NS variable slots per lane),
+// defined in search.hpp and instantiated once per class.
+template <int NS>
+struct SlotKernels {
+    static void thread_batch(int kind, const ThreadArgs& a, const DevGraph& G, int blocks, hipStream_t s);
+    static void thread_one(int kind, const Plan& P, const OneArgs& a, const DevGraph& G, int blocks, hipStream_t s);
+    static void wave_batch(int kind, int dedup, const WaveArgs& a, const DevGraph& G, int blocks, hipStream_t s);
+    static void wave_one(int kind, int dedup, const Plan& P, const WaveArgs& a, const DevGraph& G, int blocks,
+                         hipStream_t s);
+    static void giants(int kind, int dedup, const WaveArgs& a, const DevGraph& G, int blocks, hipStream_t s);
+    static void prepare(size_t dynLds);
+};
+extern template struct SlotKernels<4>;
+extern template struct SlotKernels<8>;
+extern template struct SlotKernels<16>;
+
 void launch_copy(const void* src, void* dst, int64_t bytes, hipStream_t s);
 void launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t s);
 void launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);

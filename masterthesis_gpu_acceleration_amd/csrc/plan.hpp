@@ -92,6 +92,30 @@ struct alignas(16) ConflictRefs {
     int32_t hs, ho, pad0, pad1;
 };
 
+// Variable slots a plan and its conflict refs use: one past the largest slot
+// any term ref names (constants are negative refs).
+inline int plan_slots(const Plan& P, const ConflictRefs& R) {
+    int n = 1;
+    auto see = [&](int32_t r) { if (r >= 0 && r + 1 > n) n = r + 1; };
+    see(P.hsRef);
+    see(P.hoRef);
+    for (int L = 0; L < P.nlev && L < kMaxBody; ++L) {
+        see(P.lev[L].key);
+        see(P.lev[L].val);
+        see(P.lev[L].val2);
+    }
+    for (int j = 0; j < kMaxBody; ++j) {
+        see(R.sref[j]);
+        see(R.oref[j]);
+    }
+    see(R.hs);
+    see(R.ho);
+    return n;
+}
+
+// The slot class whose kernels run a launch of plans with `slots` slots.
+constexpr int slot_class(int slots) { return slots <= 4 ? 4 : (slots <= 8 ? 8 : 16); }
+
 // One unit of thread-mode work: up to kChunkSeeds consecutive seeds of one plan.
 struct alignas(16) Chunk {
     uint32_t plan;       // index into the plan table

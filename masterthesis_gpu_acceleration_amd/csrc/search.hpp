@@ -1,0 +1,1283 @@
+// search.hpp -- the counting kernels' device code, templated on the number of
+// variable slots a launch's plans use (NS = 4, 8 or 16). Each slot class is
+// compiled in its own translation unit (search_s4.hip, search_s8.hip,
+// search_s16.hip); kernels.hip picks the class per launch.
+//
+// Work model
+//   * A SEED is one head triple (SUPPORT plans) or one head-subject key
+//     (PAIRS plans). Seeds are independent, so a seed can be abandoned and
+//     redone elsewhere without changing any count.
+//   * Every lane runs an iterative depth-first search over the plan's levels:
+//     one uniform loop in which each lane advances its own frame (entry
+//     cursor, end) at its own depth. Frames and the plan live in LDS, the
+//     variable slots in registers (NS of them: binding, lookup and the
+//     injectivity test are select chains over the slots, so a launch whose
+//     rules have few variables runs short chains), so the control flow stays
+//     flat and the register count does not grow with the body length.
+//   * Thread mode: one lane per seed, with a work budget; a wave works through
+//     a chunk of seeds of one plan and a lane takes the next seed as soon as
+//     its own is done. A seed that uses up its budget, or (PAIRS) whose
+//     distinct head objects outgrow the lane's register set, is appended to
+//     the heavy queue and its partial work discarded.
+//   * Wave mode: one wave64 per heavy seed, with a shared LDS pool of entries
+//     that idle lanes draw from and busy lanes donate to (wave_run). PAIRS
+//     seeds dedup their head objects in a bitmap indexed by the object's rank
+//     among the keys of one (direction, predicate) -- an HBM row per wave or
+//     dynamic LDS -- whose set words are listed so that counting and clearing
+//     touch only those words, or emit (seed, object) keys that the radix sort
+//     + unique pass of sort_unique.hip counts.
+//
+// Semantics, as the reference CPU counters define them
+// (Rewritten_CPP/SupportCounting.cpp:299-479, CPU_01_nonparallel/
+// SupportCounting.cpp:83-247): a new variable binding must differ from every
+// variable bound before it; a ground triple may not repeat an earlier body
+// triple or (SUPPORT) the head triple -- levels whose atoms could coincide
+// carry a conflict mask computed by the planner.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "kernels.hpp"
+#include "plan.hpp"
+
+namespace gr {
+
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / kWave;
+constexpr int kPlanWords = (int)(sizeof(Plan) / 4);
+
+constexpr int kTouchCap = 256;      // dedup-row words listed per wave-mode seed
+constexpr int kCacheSlots = 512;    // sort path: LDS cache of emitted head objects per wave (direct mapped)
+constexpr int kOneChunkSeeds = 64;  // seeds per wave in per-rule launches (one rule fills the GPU)
+constexpr int kLeafBatch = 8;       // last-level candidates a lane tries per step of its own search (wave mode)
+constexpr int kLeafBatchT = 4;      // the same in thread mode (keeps it within 128 VGPRs: 4 waves per SIMD)
+
+struct Frame {
+    int i;     // next entry
+    int end;   // one past the last entry
+};
+
+// Per-wave LDS of thread mode: the plan being evaluated and one frame per
+// (level, lane).
+struct ThreadLds {
+    Plan plan;
+    Frame frame[kMaxBody][kWave];
+    Frame cur[kMaxBody][kWave];   // ISECT levels: cursor into row 2 and its end
+};
+
+// Wave mode adds the seed's shared pool -- a frame at level poolLevel whose
+// entries [poolNext, poolEnd) any idle lane may take, with the bindings
+// poolV that hold above that level -- and the dedup words touched so far.
+struct WaveLds : ThreadLds {
+    int poolV[kMaxSlots];
+    int poolLevel;
+    int poolNext;
+    int poolEnd;
+    uint32_t poolGen;
+    uint32_t nTouched;
+    uint32_t pad[3];
+    uint32_t touched[kTouchCap];
+};
+
+static_assert(sizeof(Plan) % 4 == 0, "plan must be word-sized");
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+__device__ __forceinline__ int wave_in_block() { return threadIdx.x / kWave; }
+
+// Value of a term reference: slot r (r >= 0) or constant -(r+1).
+template <int NS>
+__device__ __forceinline__ int term_value(const int (&v)[NS], int r) {
+    int x = -(r + 1);
+#pragma unroll
+    for (int j = 0; j < NS; ++j) x = (r == j) ? v[j] : x;
+    return x;
+}
+
+template <int NS>
+__device__ __forceinline__ void set_slot(int (&v)[NS], int slot, int x) {
+#pragma unroll
+    for (int j = 0; j < NS; ++j) v[j] = (slot == j) ? x : v[j];
+}
+
+// Is x already bound to one of the first nbound slots?
+template <int NS>
+__device__ __forceinline__ bool taken(const int (&v)[NS], int nbound, int x) {
+    bool d = false;
+#pragma unroll
+    for (int j = 0; j < NS; ++j) d |= (j < nbound) & (v[j] == x);
+    return d;
+}
+
+// Index in col[lo,hi) holding `target` (rows are sorted), or -1.
+__device__ __forceinline__ int find_in_row(const int32_t* __restrict__ col, int lo, int hi, int target) {
+    int l = lo, h = hi;
+    while (l < h) {
+        const int m = (l + h) >> 1;
+        if (col[m] < target) l = m + 1;
+        else h = m;
+    }
+    return (l < hi && col[l] == target) ? l : -1;
+}
+
+// Would ground triple (s, o) of a level repeat an earlier one? Only rules
+// with constants reach this; it reads the conflict table from memory.
+template <int NS>
+__device__ __forceinline__ bool repeats(const ConflictRefs* __restrict__ R, const int (&v)[NS], int mask, int s,
+                                        int o) {
+    for (int j = 0; j < kMaxBody; ++j)
+        if ((mask & (1 << j)) && term_value(v, R->sref[j]) == s && term_value(v, R->oref[j]) == o) return true;
+    return (mask & 256) && term_value(v, R->hs) == s && term_value(v, R->ho) == o;
+}
+
+// Distinct head objects of one seed, in registers (thread mode).
+struct RegSet {
+    static constexpr int kCap = 8;
+    int e[kCap];
+    int n;
+    __device__ __forceinline__ bool seen(int x) const {
+        bool f = false;
+#pragma unroll
+        for (int j = 0; j < kCap; ++j) f |= (j < n) & (e[j] == x);
+        return f;
+    }
+    __device__ __forceinline__ bool insert(int x) {
+        if (n == kCap) return false;
+#pragma unroll
+        for (int j = kCap - 1; j > 0; --j) e[j] = e[j - 1];
+        e[0] = x;
+        ++n;
+        return true;
+    }
+};
+
+// Entry range of term `keyv` in the (direction, predicate) block run `base`.
+__device__ __forceinline__ Frame key_row(const DevGraph& G, int64_t base, int keyv) {
+    const KeyBlock kb = G.blocks[base + (keyv >> 6)];
+    const uint64_t bit = 1ull << (keyv & 63);
+    if (!(kb.bits & bit)) return Frame{0, 0};
+    const uint32_t r = kb.rank + (uint32_t)__popcll(kb.bits & (bit - 1));
+    return Frame{G.rowStart[r], G.rowStart[r + 1]};
+}
+
+// Entry range of a level under the current bindings (ISECT: its row 1).
+template <int NS>
+__device__ __forceinline__ Frame enter_level(const Level& lv, const DevGraph& G, const int (&v)[NS]) {
+    const int kind = level_kind(lv.flags);
+    if (kind == kScan) return Frame{lv.lo, lv.hi};
+    const int64_t base = (int64_t)(uint32_t)lv.lo | ((int64_t)lv.hi << 32);
+    const Frame row = key_row(G, base, term_value(v, lv.key));
+    if (kind != kCheck) return row;
+    const int at = find_in_row(G.col, row.i, row.end, term_value(v, lv.val));
+    return at < 0 ? Frame{0, 0} : Frame{at, at + 1};
+}
+
+// Row 2 of an ISECT level under the current bindings.
+template <int NS>
+__device__ __forceinline__ Frame isect_row(const Level& lv, const DevGraph& G, const int (&v)[NS]) {
+    const int64_t base = (int64_t)(uint32_t)lv.lo2 | ((int64_t)lv.hi2 << 32);
+    return key_row(G, base, term_value(v, lv.val2));
+}
+
+// Entering level L in a lane's own search: an ISECT level also places the
+// lane's row-2 cursor (no entry can match if row 2 is empty); the CHECK
+// after an ISECT level is the entry the intersection matched.
+template <int NS>
+__device__ __forceinline__ Frame enter(ThreadLds& W, const Plan& P, int L, const DevGraph& G,
+                                       const int (&v)[NS]) {
+    const Level& lv = P.lev[L];
+    const int kind = level_kind(lv.flags);
+    if (kind == kIsect) {
+        const Frame r2 = isect_row(lv, G, v);
+        if (r2.i >= r2.end) return Frame{0, 0};
+        W.cur[L][lane_id()] = r2;
+    } else if (kind == kCheck && L > 0 && level_kind(P.lev[L - 1].flags) == kIsect) {
+        const int j = W.cur[L - 1][lane_id()].i;
+        return Frame{j, j + 1};
+    }
+    return enter_level(lv, G, v);
+}
+
+// Is x in row 2 of ISECT level L? Row 1 is enumerated in increasing x, so
+// the lane's cursor only moves forward: gallop from it, then binary search.
+// Leaves the cursor at the first entry >= x (the match, if any).
+__device__ __forceinline__ bool isect_next(ThreadLds& W, int L, const DevGraph& G, int x) {
+    const int lane = lane_id();
+    int j = W.cur[L][lane].i;
+    const int e = W.cur[L][lane].end;
+    bool hit = false;
+    if (j < e) {
+        const int y = G.col[j];
+        if (y < x) {
+            int lo = j, hi = e, step = 1;
+            for (;;) {
+                const int t = lo + step;
+                if (t >= e) break;
+                if (G.col[t] >= x) {
+                    hi = t;
+                    break;
+                }
+                lo = t;
+                step <<= 1;
+            }
+            int l = lo + 1, h = hi;
+            while (l < h) {
+                const int m = (l + h) >> 1;
+                if (G.col[m] < x) l = m + 1;
+                else h = m;
+            }
+            j = l;
+            hit = j < e && G.col[j] == x;
+        } else {
+            hit = y == x;
+        }
+    }
+    W.cur[L][lane].i = j;
+    return hit;
+}
+
+// The same test for one entry drawn from a shared pool (no cursor to reuse):
+// binary search of row 2; on a match the lane's cursor points at it.
+template <int NS>
+__device__ __forceinline__ bool isect_find(ThreadLds& W, int L, const Level& lv, const DevGraph& G,
+                                           const int (&v)[NS], int x) {
+    const Frame r2 = isect_row(lv, G, v);
+    const int at = find_in_row(G.col, r2.i, r2.end, x);
+    if (at < 0) return false;
+    W.cur[L][lane_id()] = Frame{at, r2.end};
+    return true;
+}
+
+// Binds entry i of a level. False if the candidate violates injectivity or
+// repeats an earlier ground triple.
+// x is col[i], loaded by the caller.
+template <int NS>
+__device__ __forceinline__ bool bind_entry_x(const Level& lv, const DevGraph& G, const ConflictRefs* R,
+                                             int (&v)[NS], int i, int x) {
+    const int f = lv.flags;
+    const int kind = level_kind(f);
+    const int nbound = level_nbound(f);
+    int k = 0;
+    if (kind == kExt || kind == kIsect) {
+        if (taken(v, nbound, x)) return false;
+        set_slot(v, lv.val, x);
+    } else if (kind == kScan) {
+        k = G.coo[i];
+        if (lv.val == lv.val2) {
+            if (k != x || taken(v, nbound, k)) return false;
+            set_slot(v, lv.val, k);
+        } else {
+            if (k == x || taken(v, nbound, k) || taken(v, nbound, x)) return false;
+            set_slot(v, lv.val, k);
+            set_slot(v, lv.val2, x);
+        }
+    }
+    const int conf = level_conflicts(f);
+    if (conf) {
+        if (kind != kScan) k = term_value(v, lv.key);
+        const bool spo = level_dir(f) == kSPO;
+        if (repeats(R, v, conf, spo ? k : x, spo ? x : k)) return false;
+    }
+    return true;
+}
+
+template <int NS>
+__device__ __forceinline__ bool bind_entry(const Level& lv, const DevGraph& G, const ConflictRefs* R,
+                                           int (&v)[NS], int i) {
+    return bind_entry_x(lv, G, R, v, i, G.col[i]);
+}
+
+// Binds the head of a SUPPORT seed; false if the head triple cannot match.
+template <int NS>
+__device__ __forceinline__ bool bind_support_seed(const Plan& P, const DevGraph& G, int item, int (&v)[NS]) {
+    const int e = P.seedBegin + item;
+    const int key = G.coo[e];
+    const int val = G.col[e];
+    const int s = P.seedDir == kSPO ? key : val;
+    const int o = P.seedDir == kSPO ? val : key;
+    if (P.hsRef < 0 && -(P.hsRef + 1) != s) return false;
+    if (P.hoRef < 0 && -(P.hoRef + 1) != o) return false;
+    if (P.hsRef >= 0 && P.hoRef >= 0) {
+        if (P.hsRef == P.hoRef) {
+            if (s != o) return false;
+        } else if (s == o) {
+            return false;
+        }
+    }
+    if (P.hsRef >= 0) set_slot(v, P.hsRef, s);
+    if (P.hoRef >= 0) set_slot(v, P.hoRef, o);
+    return true;
+}
+
+// Binds a seed's head slots. False if the seed is not a match.
+template <int NS>
+__device__ __forceinline__ bool bind_seed(const Plan& P, const DevGraph& G, int item, int (&v)[NS]) {
+#pragma unroll
+    for (int j = 0; j < NS; ++j) v[j] = -1;
+    if (P.kind == kSupport) return bind_support_seed(P, G, item, v);
+    v[0] = G.keys[P.seedBegin + item];
+    return true;
+}
+
+__device__ __forceinline__ void copy_plan(ThreadLds& W, const Plan* src) {
+    const int32_t* s = reinterpret_cast<const int32_t*>(src);
+    int32_t* d = reinterpret_cast<int32_t*>(&W.plan);
+    for (int w = lane_id(); w < kPlanWords; w += kWave) d[w] = s[w];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ------------------------------------------------------------ helpers
+
+__device__ __forceinline__ uint64_t wave_sum(uint64_t x) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, kWave);
+    return x;
+}
+
+__device__ __forceinline__ uint32_t wave_take(uint32_t* cursor) {
+    uint32_t c = 0;
+    if (lane_id() == 0) c = atomicAdd(cursor, 1u);
+    c = __shfl(c, 0, kWave);
+    return __builtin_amdgcn_readfirstlane(c);
+}
+
+// Wave-aggregated append of this wave's escalated seeds.
+__device__ __forceinline__ void push_heavy(bool heavy, uint32_t plan, uint32_t item, HeavyItem* q, uint32_t* qCount,
+                                           uint32_t qCap) {
+    const uint64_t mask = __ballot(heavy);
+    if (mask == 0) return;
+    uint32_t base = 0;
+    if (lane_id() == 0) base = atomicAdd(qCount, (uint32_t)__popcll(mask));
+    base = __shfl(base, 0, kWave);
+    if (heavy) {
+        const uint32_t pos = base + (uint32_t)__popcll(mask & ((1ull << lane_id()) - 1ull));
+        if (pos < qCap) q[pos] = HeavyItem{plan, item};
+    }
+}
+
+// Thread mode over seeds [begin, begin + count) of the plan in W. Every lane
+// runs its own depth-first search; a lane whose seed is finished takes the
+// next one (ballot prefix count), so no lane waits for the slowest seed.
+// Returns this lane's count; escalated seeds go to the heavy queue. K is the
+// plan kind of every plan the launch sees (kSupport or kPairs).
+template <int K, int NS>
+__device__ __forceinline__ uint64_t thread_chunk(ThreadLds& W, const DevGraph& G, const ConflictRefs* R,
+                                                 uint32_t planIdx, uint32_t begin, uint32_t count, HeavyItem* hq,
+                                                 uint32_t* hCount, uint32_t hCap) {
+    const int lane = lane_id();
+    const Plan& P = W.plan;
+    const int nlev = P.nlev;
+    const bool pairs = K == kPairs && P.pairLevel >= 0;
+    const int pl = P.pairLevel;
+    const uint64_t below = (1ull << lane) - 1ull;
+    int v[NS];
+    RegSet set;
+    set.n = 0;
+    int d = 0, budget = 0;
+    bool active = false;
+    uint32_t item = 0, next = 0;
+    uint64_t mine = 0;
+    for (;;) {
+        const bool idle = !active;
+        const uint64_t want = __ballot(idle);
+        if (want && next < count) {
+            const uint32_t off = next + (uint32_t)__popcll(want & below);
+            if (idle && off < count) {
+                item = begin + off;
+                active = bind_seed(P, G, (int)item, v);
+                if (active) {
+                    W.frame[0][lane] = enter(W, P, 0, G, v);
+                    d = 0;
+                    budget = P.budget;
+                    set.n = 0;
+                }
+            }
+            next += (uint32_t)__popcll(want);
+        }
+        if (!__any(active)) {
+            if (next >= count) break;
+            continue;
+        }
+        bool heavyNow = false;
+        if (active) {
+            const Frame fr = W.frame[d][lane];
+            if (fr.i >= fr.end) {
+                if (d == 0) {
+                    active = false;
+                    if (pairs) mine += (uint64_t)set.n;
+                } else {
+                    --d;
+                }
+            } else if (d + 1 == nlev && fr.end - fr.i > 1) {
+                // last level: up to kLeafBatchT candidates per step, loaded together
+                const int take = fr.end - fr.i < kLeafBatchT ? fr.end - fr.i : kLeafBatchT;
+                W.frame[d][lane].i = fr.i + take;
+                budget -= take;
+                if (budget < 0) {
+                    heavyNow = true;
+                    active = false;
+                } else {
+                    int xs[kLeafBatchT];
+#pragma unroll
+                    for (int k = 0; k < kLeafBatchT; ++k) xs[k] = k < take ? G.col[fr.i + k] : 0;
+                    bool stop = false;
+#pragma unroll
+                    for (int k = 0; k < kLeafBatchT; ++k) {
+                        if (stop || k >= take || !bind_entry_x(P.lev[d], G, R, v, fr.i + k, xs[k])) continue;
+                        if (!pairs) {
+                            mine += 1;
+                            active = false;
+                            stop = true;
+                            continue;
+                        }
+                        const int ho = term_value(v, P.hoRef);
+                        if (d == pl && set.seen(ho)) continue;
+                        if (!set.insert(ho)) {
+                            heavyNow = true;
+                            active = false;
+                            stop = true;
+                        } else if (d != pl) {
+                            d = pl;   // head object settled: next candidate at the pair level
+                            stop = true;
+                        }
+                    }
+                }
+            } else {
+                W.frame[d][lane].i = fr.i + 1;
+                if (--budget < 0) {
+                    heavyNow = true;
+                    active = false;
+                } else {
+                    const int x = G.col[fr.i];
+                    bool ok = bind_entry_x(P.lev[d], G, R, v, fr.i, x);
+                    if (ok && level_kind(P.lev[d].flags) == kIsect) ok = isect_next(W, d, G, x);
+                    if (ok && pairs && d == pl) ok = !set.seen(term_value(v, P.hoRef));
+                    if (ok) {
+                        if (d + 1 == nlev) {
+                            if (pairs) {
+                                if (!set.insert(term_value(v, P.hoRef))) {
+                                    heavyNow = true;
+                                    active = false;
+                                } else {
+                                    d = pl;   // head object settled: next candidate at the pair level
+                                }
+                            } else {
+                                mine += 1;
+                                active = false;
+                            }
+                        } else {
+                            ++d;
+                            W.frame[d][lane] = enter(W, P, d, G, v);
+                        }
+                    }
+                }
+            }
+        }
+        push_heavy(heavyNow, planIdx, item, hq, hCount, hCap);
+    }
+    return mine;
+}
+
+// Wave mode: the whole wave on one seed. Level-0 entries are handed out one
+// at a time from an LDS cursor.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ uint32_t bcast(uint32_t x) { return (uint32_t)__shfl((int)x, 0, kWave); }
+__device__ __forceinline__ uint32_t ld_rlx(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Head-object dedup of one PAIRS seed in wave mode.
+//   kDedupRows: the wave's bitmap row of nE bits in HBM (atomicOr);
+//   kDedupLds:  the same bitmap in the wave's LDS;
+//   kDedupSort: nothing is marked; the caller emits keys to HBM, and a small
+//               direct-mapped LDS cache of emitted objects prunes repeats.
+// For the bitmaps, the lane whose atomicOr finds a word still zero lists it
+// in LDS so that clearing touches only those words; insert returns 1 to the
+// one lane whose call set the bit, so the lanes' sums are the distinct count.
+// The bitmaps index a head object x by its rank among the keys of the
+// plan's head-object space (Plan::hoLo/hoHi/hoRow0/hoKeys): one key-index
+// block load and a popcount, so a bitmap needs only as many bits as the
+// largest key set of one (direction, predicate) -- not one per term id.
+template <int D>
+struct Dedup {
+    uint32_t* bits;     // the wave's own bitmap: its HBM row (kDedupRows) or LDS (kDedupLds)
+    uint32_t* shared;   // a giant's HBM row, shared by its sub-items (then used instead of bits), or null
+    WaveLds* W;
+    int* cache;
+    const KeyBlock* blocks;
+    // Rank of head object x (sort path: x itself). x is a key of the space by
+    // construction; the clamp only keeps a wrong plan from writing out of bounds.
+    __device__ __forceinline__ int rank(int x) const {
+        if (D == kDedupSort) return x;
+        const Plan& P = W->plan;
+        const int64_t base = (int64_t)(uint32_t)P.hoLo | ((int64_t)P.hoHi << 32);
+        const KeyBlock kb = blocks[base + (x >> 6)];
+        const uint64_t bit = 1ull << (x & 63);
+        const int r = (int)(kb.rank + (uint32_t)__popcll(kb.bits & (bit - 1))) - P.hoRow0;
+        return r < 0 ? 0 : (r >= P.hoKeys ? P.hoKeys - 1 : r);
+    }
+    __device__ __forceinline__ bool seen(int x) const { return seen_r(rank(x)); }
+    __device__ __forceinline__ uint32_t insert(int x) { return insert_r(rank(x)); }
+    __device__ __forceinline__ bool seen_r(int x) const {
+        if (D == kDedupSort) return cache[x & (kCacheSlots - 1)] == x;
+        uint32_t w;
+        if (shared) w = __hip_atomic_load(shared + (x >> 5), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else if (D == kDedupRows) w = __hip_atomic_load(bits + (x >> 5), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else w = bits[x >> 5];
+        return (w >> (x & 31)) & 1u;
+    }
+    __device__ __forceinline__ uint32_t insert_r(int x) {
+        if (D == kDedupSort) {
+            cache[x & (kCacheSlots - 1)] = x;
+            return 0;
+        }
+        const uint32_t bit = 1u << (x & 31);
+        if (shared) return (atomicOr(shared + (x >> 5), bit) & bit) ? 0u : 1u;
+        const uint32_t old = atomicOr(bits + (x >> 5), bit);
+        if (old == 0) {
+            const uint32_t k = atomicAdd(&W->nTouched, 1u);
+            if (k < (uint32_t)kTouchCap) W->touched[k] = (uint32_t)(x >> 5);
+        }
+        return (old & bit) ? 0u : 1u;
+    }
+    __device__ __forceinline__ void clear_word(uint32_t w) {
+        if (D == kDedupRows) atomicAnd(bits + w, 0u);
+        else bits[w] = 0u;
+    }
+    // The bitmap word holding rank x (bitmap modes), for tests batched ahead of the marks.
+    __device__ __forceinline__ uint32_t word_r(int x) const {
+        if (shared) return __hip_atomic_load(shared + (x >> 5), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (D == kDedupRows) return __hip_atomic_load(bits + (x >> 5), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return bits[x >> 5];
+    }
+};
+
+// Marks the head objects of a batch of last-level groundings (ho[k] >= 0;
+// -1: none). The word tests are issued together, then the marks together,
+// so a batch costs two memory round trips instead of two per candidate.
+// Returns the objects this lane marked first; on the sort path emit[k] gets
+// the objects to emit.
+template <int D, int U>
+__device__ __forceinline__ uint32_t mark_batch(Dedup<D>& dd, const int (&ho)[U], int (&emit)[U]) {
+    uint32_t n = 0;
+    if (D == kDedupSort) {
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            emit[k] = -1;
+            if (ho[k] >= 0 && !dd.seen(ho[k])) {
+                dd.insert(ho[k]);
+                emit[k] = ho[k];
+            }
+        }
+        return 0;
+    }
+    int r[U];
+    uint32_t w[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) r[k] = ho[k] >= 0 ? dd.rank(ho[k]) : -1;
+#pragma unroll
+    for (int k = 0; k < U; ++k) w[k] = r[k] >= 0 ? dd.word_r(r[k]) : 0xffffffffu;
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+        emit[k] = -1;
+        if (r[k] >= 0 && !((w[k] >> (r[k] & 31)) & 1u)) n += dd.insert_r(r[k]);
+    }
+    return n;
+}
+
+// Sort path: wave-aggregated append of this step's emitted head objects as
+// keys keyHi | object. Called by the whole wave.
+__device__ __forceinline__ void emit_keys(const SortSink& sink, unsigned long long keyHi, int emit) {
+    const uint64_t em = __ballot(emit >= 0);
+    if (em == 0) return;
+    uint32_t base = 0;
+    if (lane_id() == 0) base = atomicAdd(sink.count, (uint32_t)__popcll(em));
+    base = __shfl(base, 0, kWave);
+    if (emit >= 0) {
+        const uint32_t pos = base + (uint32_t)__popcll(em & ((1ull << lane_id()) - 1ull));
+        if (pos < sink.cap) sink.keys[pos] = keyHi | (unsigned long long)(uint32_t)emit;
+    }
+}
+
+constexpr int kLeafUnroll = 8;   // last-level pool entries an idle lane takes per draw
+
+// Wave mode: the whole wave on one seed, with a shared pool of work.
+//   * The pool starts as the seed's level-0 row. Each step, the idle lanes
+//     draw consecutive pool entries (coalesced): from a last-level pool each
+//     takes kLeafUnroll entries and finishes them on the spot; from any other
+//     level each takes one entry, binds it on top of the pool's bindings and
+//     searches below it on its own (frames in LDS), as in thread mode.
+//   * When the pool is empty and some lanes are idle, the busy lane with the
+//     most untried entries in its shallowest open frame donates the rest of
+//     that frame (with its bindings) as the new pool. Long rows at any depth
+//     -- hub rows -- are thus spread over the wave instead of scanned by one
+//     lane.
+//   * SUPPORT seeds end at the first complete grounding (__any). PAIRS seeds
+//     count each head object once (Dedup); a lane whose head object is
+//     already counted drops the rest of that subtree.
+//   * A seed (not a sub-item) still running after ga.after steps is listed
+//     as a giant and abandoned (`aborted`; its partial count is dropped).
+//     A SUPPORT sub-item shares its giant's `found` flag: it polls it and
+//     counts only if it is the one to set it.
+// The pool (poolLevel/Next/End/V) is set up by the caller, all lanes idle.
+// Returns the count this wave adds (SUPPORT: 0/1; PAIRS: distinct head
+// objects it marked first, or 0 on the sort path, which counts later).
+template <int K, int D, int NS>
+__device__ __forceinline__ uint64_t wave_run(WaveLds& W, const DevGraph& G, const ConflictRefs* R, Dedup<D>& dd,
+                                             int64_t rowWords, const SortSink& sink, unsigned long long keyHi,
+                                             const GiantArgs& ga, bool mayGiant, HeavyItem self, uint32_t heavyIdx,
+                                             uint32_t* foundFlag, bool& aborted,
+                                             unsigned long long (&stats)[kWaveStats]) {
+    const int lane = lane_id();
+    const Plan& P = W.plan;
+    const int nlev = P.nlev;
+    const int last = nlev - 1;
+    const bool pairs = K == kPairs && P.pairLevel >= 0;
+    const int pl = P.pairLevel;
+    const uint64_t below = (1ull << lane) - 1ull;
+    int v[NS];
+#pragma unroll
+    for (int j = 0; j < NS; ++j) v[j] = -1;
+    bool busy = false, found = false, stopped = false;
+    aborted = false;
+    int d = 0, base = 0;            // own search: current level, first level it owns
+    uint32_t gen = 0xffffffffu;     // pool whose bindings v holds (idle lanes)
+    uint64_t mine = 0;
+    int steps = 0;
+    uint32_t nDonate = 0, nLeafDraw = 0, nDraw = 0;
+    for (;;) {
+        // ---- A: idle lanes draw from the pool
+        const uint64_t idleM = __ballot(!busy);
+        const int pn = uniform(W.poolNext), pe = uniform(W.poolEnd), plv = uniform(W.poolLevel);
+        const uint32_t pg = (uint32_t)uniform((int)W.poolGen);
+        if (idleM != 0 && pn < pe) {
+            const int n = __popcll(idleM);
+            const int r = __popcll(idleM & below);
+            int next = pe;
+            bool drop = false;
+            if (pairs && pl < plv) {   // every entry shares a head object bound above the pool
+                const int hr = P.hoRef;
+                const int hov = hr >= 0 ? W.poolV[hr] : -(hr + 1);
+                drop = dd.seen(uniform(hov));
+            }
+            if (!drop) {
+                if (!busy && gen != pg) {
+#pragma unroll
+                    for (int j = 0; j < NS; ++j) v[j] = W.poolV[j];
+                    gen = pg;
+                }
+                if (plv == last) {
+                    ++nLeafDraw;
+                    next = pn + n * kLeafUnroll < pe ? pn + n * kLeafUnroll : pe;
+                    int xs[kLeafUnroll];
+#pragma unroll
+                    for (int k = 0; k < kLeafUnroll; ++k) {
+                        const int i = pn + r + n * k;
+                        xs[k] = (!busy && i < pe) ? G.col[i] : 0;
+                    }
+                    bool hit = false;
+                    int hos[kLeafUnroll];
+#pragma unroll
+                    for (int k = 0; k < kLeafUnroll; ++k) {
+                        const int i = pn + r + n * k;
+                        hos[k] = -1;
+                        if (!busy && i < pe && bind_entry_x(P.lev[last], G, R, v, i, xs[k])) {
+                            if (!pairs) found = true;
+                            else hos[k] = term_value(v, P.hoRef);
+                        }
+                    }
+                    if (pairs) {
+                        if (pl < last) {   // one head object for the whole pool: mark it once
+#pragma unroll
+                            for (int k = 1; k < kLeafUnroll; ++k)
+                                if (hos[0] < 0) hos[0] = hos[k];
+#pragma unroll
+                            for (int k = 1; k < kLeafUnroll; ++k) hos[k] = -1;
+                        }
+                        int emits[kLeafUnroll];
+                        mine += mark_batch<D, kLeafUnroll>(dd, hos, emits);
+                        hit = hos[0] >= 0;
+                        if (D == kDedupSort) {
+#pragma unroll
+                            for (int k = 0; k < kLeafUnroll; ++k) emit_keys(sink, keyHi, emits[k]);
+                        }
+                    }
+                    // head object bound above the pool and now counted: the rest adds nothing
+                    if (pairs && pl < last && __any(hit)) next = pe;
+                } else {
+                    ++nDraw;
+                    next = pn + n < pe ? pn + n : pe;
+                    const int i = pn + r;
+                    if (!busy && i < pe) {
+                        const int x = G.col[i];
+                        bool ok = bind_entry_x(P.lev[plv], G, R, v, i, x);
+                        if (ok && level_kind(P.lev[plv].flags) == kIsect) ok = isect_find(W, plv, P.lev[plv], G, v, x);
+                        if (ok && pairs && plv == pl) ok = !dd.seen(term_value(v, P.hoRef));
+                        if (ok) {
+                            d = base = plv + 1;
+                            W.frame[d][lane] = enter(W, P, d, G, v);
+                            busy = true;
+                        }
+                    }
+                }
+            }
+            wave_sync();
+            if (lane == 0) W.poolNext = next;
+        }
+        if (!pairs && __any(found)) break;
+        // ---- B: busy lanes advance their own search by one step
+        int emit = -1;
+        int emits[kLeafBatch];
+#pragma unroll
+        for (int k = 0; k < kLeafBatch; ++k) emits[k] = -1;
+        if (busy) {
+            if (pairs && pl < base && dd.seen(term_value(v, P.hoRef))) {
+                busy = false;   // head object counted by another lane
+            } else {
+                const Frame fr = W.frame[d][lane];
+                if (fr.i >= fr.end) {
+                    if (d == base) busy = false;
+                    else --d;
+                } else if (d == last && fr.end - fr.i > 1) {
+                    // last level: up to kLeafBatch candidates per step, loaded together
+                    const int take = fr.end - fr.i < kLeafBatch ? fr.end - fr.i : kLeafBatch;
+                    W.frame[d][lane].i = fr.i + take;
+                    int xs[kLeafBatch];
+#pragma unroll
+                    for (int k = 0; k < kLeafBatch; ++k) xs[k] = k < take ? G.col[fr.i + k] : 0;
+                    int hos[kLeafBatch];
+                    bool any = false;
+#pragma unroll
+                    for (int k = 0; k < kLeafBatch; ++k) {
+                        hos[k] = -1;
+                        if (k < take && bind_entry_x(P.lev[d], G, R, v, fr.i + k, xs[k])) {
+                            hos[k] = pairs ? term_value(v, P.hoRef) : 0;
+                            any = true;
+                        }
+                    }
+                    if (!pairs) {
+                        if (any) {
+                            found = true;
+                            busy = false;
+                        }
+                    } else {
+                        if (d != pl) {   // head object bound above: one grounding settles it
+#pragma unroll
+                            for (int k = 1; k < kLeafBatch; ++k)
+                                if (hos[0] < 0) hos[0] = hos[k];
+#pragma unroll
+                            for (int k = 1; k < kLeafBatch; ++k) hos[k] = -1;
+                            if (any) {
+                                if (pl >= base) d = pl;   // head object settled: next candidate at the pair level
+                                else busy = false;
+                            }
+                        }
+                        mine += mark_batch<D, kLeafBatch>(dd, hos, emits);
+                    }
+                } else {
+                    W.frame[d][lane].i = fr.i + 1;
+                    const int x = G.col[fr.i];
+                    bool ok = bind_entry_x(P.lev[d], G, R, v, fr.i, x);
+                    if (ok && level_kind(P.lev[d].flags) == kIsect) ok = isect_next(W, d, G, x);
+                    if (ok && pairs && d == pl) ok = !dd.seen(term_value(v, P.hoRef));
+                    if (ok) {
+                        if (d == last) {
+                            if (pairs) {
+                                const int ho = term_value(v, P.hoRef);
+                                mine += dd.insert(ho);
+                                emit = ho;
+                                if (pl >= base) d = pl;   // head object settled: next candidate at the pair level
+                                else busy = false;
+                            } else {
+                                found = true;
+                                busy = false;
+                            }
+                        } else {
+                            ++d;
+                            W.frame[d][lane] = enter(W, P, d, G, v);
+                        }
+                    }
+                }
+            }
+        }
+        if (D == kDedupSort && pairs) {
+            emit_keys(sink, keyHi, emit);
+#pragma unroll
+            for (int k = 0; k < kLeafBatch; ++k) emit_keys(sink, keyHi, emits[k]);
+        }
+        if (!pairs && __any(found)) break;
+        // ---- C: done, hand the seed over as a giant, or donate work to the idle lanes
+        wave_sync();
+        const uint64_t busyM = __ballot(busy);
+        const bool poolEmpty = uniform(W.poolNext) >= uniform(W.poolEnd);
+        if (busyM == 0 && poolEmpty) break;
+        ++steps;
+        if (mayGiant && steps >= ga.after) {
+            mayGiant = false;   // one try per seed
+            uint32_t slot = 0xffffffffu;
+            if (lane == 0) {
+                const uint32_t gi = atomicAdd(ga.count, 1u);
+                if (gi < ga.cap) {
+                    ga.list[gi] = self;
+                    ga.heavyIdx[gi] = heavyIdx;
+                    slot = gi;
+                }
+            }
+            if (bcast(slot) != 0xffffffffu) {
+                aborted = true;
+                break;
+            }
+        }
+        if (foundFlag && (steps & 31) == 0) {   // another sub-item of this giant found a grounding
+            uint32_t f = lane == 0 ? ld_rlx(foundFlag) : 0u;
+            if (bcast(f)) {
+                stopped = true;
+                break;
+            }
+        }
+        if (busyM != ~0ull && poolEmpty) {
+            int rem = 0, lvl = 0;
+            if (busy) {
+                for (int l = base; l <= d; ++l) {
+                    const Frame f = W.frame[l][lane];
+                    if (f.end > f.i) {
+                        rem = f.end - f.i;
+                        lvl = l;
+                        break;
+                    }
+                }
+            }
+            uint64_t key = ((uint64_t)(uint32_t)rem << 8) | (uint64_t)lane;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const uint64_t o = __shfl_xor(key, off, kWave);
+                key = o > key ? o : key;
+            }
+            const int donor = (int)(key & 255u);
+            if ((key >> 8) != 0) {
+                ++nDonate;
+                if (lane == donor) {
+                    const Frame f = W.frame[lvl][lane];
+                    W.poolLevel = lvl;
+                    W.poolNext = f.i;
+                    W.poolEnd = f.end;
+                    W.poolGen = W.poolGen + 1u;
+#pragma unroll
+                    for (int j = 0; j < NS; ++j) W.poolV[j] = v[j];
+                    W.frame[lvl][lane].end = f.i;
+                }
+                wave_sync();
+            }
+        }
+    }
+    stats[0] += 1;
+    stats[1] += (unsigned long long)steps;
+    stats[2] += nDonate;
+    stats[3] += nLeafDraw;
+    stats[4] += nDraw;
+    if (!pairs) {
+        if (aborted || stopped || !__any(found)) return 0;
+        if (!foundFlag) return 1;
+        uint32_t win = 0;
+        if (lane == 0) win = atomicCAS(foundFlag, 0u, 1u) == 0u ? 1u : 0u;
+        return bcast(win);
+    }
+    if (D == kDedupSort) return 0;
+    // clear the wave's own bitmap (a sub-item marked only its giant's row: nothing listed)
+    wave_sync();
+    const uint32_t nt = W.nTouched;
+    if (nt <= (uint32_t)kTouchCap) {
+        for (uint32_t k = lane; k < nt; k += kWave) dd.clear_word(W.touched[k]);
+    } else {
+        for (int64_t w = lane; w < rowWords; w += kWave) dd.clear_word((uint32_t)w);
+    }
+    if (D == kDedupLds) wave_sync();
+    const uint64_t total = wave_sum(mine);
+    return aborted ? 0 : total;
+}
+
+// Sets up the pool of heavy seed `item` (its level-0 row) and runs it.
+template <int K, int D, int NS>
+__device__ __forceinline__ uint64_t wave_seed(WaveLds& W, const DevGraph& G, const ConflictRefs* R, uint32_t item,
+                                              Dedup<D>& dd, int64_t rowWords, const SortSink& sink,
+                                              unsigned long long keyHi, const GiantArgs& ga, uint32_t planIdx,
+                                              uint32_t heavyIdx, unsigned long long (&stats)[kWaveStats]) {
+    const int lane = lane_id();
+    const Plan& P = W.plan;
+    int v[NS];
+    if (!bind_seed(P, G, (int)item, v)) return 0;
+    const Frame f0 = enter(W, P, 0, G, v);
+    if (f0.i >= f0.end) return 0;
+    if (lane < NS) W.poolV[lane] = term_value(v, lane);
+    if (lane == 0) {
+        W.poolLevel = 0;
+        W.poolNext = f0.i;
+        W.poolEnd = f0.end;
+        W.poolGen = W.poolGen + 1u;
+        W.nTouched = 0;
+    }
+    if (D == kDedupSort && K == kPairs)
+        for (int k = lane; k < kCacheSlots; k += kWave) dd.cache[k] = -1;
+    wave_sync();
+    bool aborted = false;
+    return wave_run<K, D, NS>(W, G, R, dd, rowWords, sink, keyHi, ga, ga.after > 0, HeavyItem{planIdx, item}, heavyIdx,
+                          nullptr, aborted, stats);
+}
+
+// ------------------------------------------------------------- kernels
+
+// Persistent thread mode over a chunk list: each wave repeatedly takes <= 64
+// seeds of one plan, one per lane.
+template <int K, int NS>
+__global__ void __launch_bounds__(kBlock) k_thread_batch(ThreadArgs a, DevGraph G) {
+    __shared__ ThreadLds lds[kWavesPerBlock];
+    ThreadLds& W = lds[wave_in_block()];
+    const int lane = lane_id();
+    uint32_t loaded = 0xffffffffu;
+    for (;;) {
+        const uint32_t c = wave_take(a.cursor);
+        if (c >= a.nChunks) break;
+        const Chunk ch = a.chunks[c];
+        if (ch.gate && a.counts[2 * a.plans[ch.plan].rule + kSupport] == 0) continue;
+        if (ch.plan != loaded) {
+            copy_plan(W, a.plans + ch.plan);
+            loaded = ch.plan;
+        }
+        const uint64_t mine = thread_chunk<K, NS>(W, G, a.refs + ch.plan, ch.plan, ch.begin, ch.count, a.heavy,
+                                           a.heavyCount, a.heavyCap);
+        const uint64_t tot = wave_sum(mine);
+        if (lane == 0 && tot) atomicAdd(a.counts + 2 * W.plan.rule + K, (unsigned long long)tot);
+    }
+}
+
+// One plan passed by value in the kernel arguments (per-rule launches);
+// waves stride over its seeds.
+template <int K, int NS>
+__global__ void __launch_bounds__(kBlock) k_thread_one(Plan P, OneArgs a, DevGraph G) {
+    __shared__ ThreadLds lds[kWavesPerBlock];
+    ThreadLds& W = lds[wave_in_block()];
+    const int lane = lane_id();
+    copy_plan(W, &P);
+    const uint32_t waves = (gridDim.x * blockDim.x) / kWave;
+    for (uint32_t base = ((blockIdx.x * blockDim.x + threadIdx.x) / kWave) * kOneChunkSeeds; base < a.nItems;
+         base += waves * kOneChunkSeeds) {
+        const uint32_t cnt = a.nItems - base < (uint32_t)kOneChunkSeeds ? a.nItems - base : (uint32_t)kOneChunkSeeds;
+        const uint64_t mine = thread_chunk<K, NS>(W, G, a.refs, 0u, base, cnt, a.heavy, a.heavyCount, a.heavyCap);
+        const uint64_t tot = wave_sum(mine);
+        if (lane == 0 && tot) atomicAdd(a.count, (unsigned long long)tot);
+    }
+}
+
+// Dedup state of one wave-mode wave: D selects the bitmap in HBM (this
+// wave's row), the bitmap in LDS (dynamic shared memory, cleared once here)
+// or the sort path's LDS cache.
+template <int D>
+__device__ __forceinline__ Dedup<D> wave_dedup(WaveLds& W, int* cache, uint32_t* dyn, const WaveArgs& a,
+                                               const DevGraph& G) {
+    Dedup<D> dd;
+    dd.W = &W;
+    dd.cache = cache;
+    dd.bits = nullptr;
+    dd.shared = nullptr;
+    dd.blocks = G.blocks;
+    if (D == kDedupRows) dd.bits = a.rows + (int64_t)blockIdx.x * a.rowWords;
+    if (D == kDedupLds) {
+        dd.bits = dyn;
+        for (int64_t w = lane_id(); w < a.rowWords; w += kWave) dyn[w] = 0u;
+    }
+    if (lane_id() == 0) W.poolGen = 0;
+    wave_sync();
+    return dd;
+}
+
+// Adds a wave's counters to the launch's (once per wave).
+__device__ __forceinline__ void flush_stats(const GiantArgs& ga, const unsigned long long (&stats)[kWaveStats]) {
+    if (ga.stats && lane_id() == 0)
+        for (int q = 0; q < kWaveStats; ++q)
+            if (stats[q]) atomicAdd(ga.stats + q, stats[q]);
+}
+
+// Persistent wave mode over the heavy queue: one wave per block, one heavy
+// seed at a time.
+template <int K, int D, int NS>
+__global__ void __launch_bounds__(kWave) k_wave_batch(WaveArgs a, DevGraph G) {
+    __shared__ WaveLds W;
+    __shared__ int cache[D == kDedupSort ? kCacheSlots : 1];
+    extern __shared__ uint32_t dyn[];
+    const int lane = lane_id();
+    uint32_t n = *a.heavyCount;
+    if (n > a.heavyCap) n = a.heavyCap;
+    if (n == 0) return;   // nothing escalated: no cursor traffic
+    Dedup<D> dd = wave_dedup<D>(W, cache, dyn, a, G);
+    uint32_t loaded = 0xffffffffu;
+    unsigned long long stats[kWaveStats] = {0, 0, 0, 0, 0};
+    for (;;) {
+        const uint32_t h = wave_take(a.cursor);
+        if (h >= n) break;
+        const HeavyItem it = a.heavy[h];
+        if (it.plan != loaded) {
+            copy_plan(W, a.plans + it.plan);
+            loaded = it.plan;
+        }
+        const uint64_t c = wave_seed<K, D, NS>(W, G, a.refs + it.plan, it.item, dd, a.rowWords, a.sink,
+                                           (unsigned long long)h << a.sink.hoBits, a.giant, it.plan, h, stats);
+        if (lane == 0 && c) atomicAdd(a.counts + 2 * W.plan.rule + K, (unsigned long long)c);
+    }
+    flush_stats(a.giant, stats);
+}
+
+template <int K, int D, int NS>
+__global__ void __launch_bounds__(kWave) k_wave_one(Plan P, WaveArgs a, DevGraph G) {
+    __shared__ WaveLds W;
+    __shared__ int cache[D == kDedupSort ? kCacheSlots : 1];
+    extern __shared__ uint32_t dyn[];
+    const int lane = lane_id();
+    uint32_t n = *a.heavyCount;
+    if (n > a.heavyCap) n = a.heavyCap;
+    if (n == 0) return;
+    copy_plan(W, &P);
+    Dedup<D> dd = wave_dedup<D>(W, cache, dyn, a, G);
+    unsigned long long stats[kWaveStats] = {0, 0, 0, 0, 0};
+    for (;;) {
+        const uint32_t h = wave_take(a.cursor);
+        if (h >= n) break;
+        const uint64_t c = wave_seed<K, D, NS>(W, G, a.refs, a.heavy[h].item, dd, a.rowWords, a.sink,
+                                           (unsigned long long)h << a.sink.hoBits, a.giant, 0u, h, stats);
+        if (lane == 0 && c) atomicAdd(a.counts, (unsigned long long)c);
+    }
+    flush_stats(a.giant, stats);
+}
+
+// ------------------------------------------------------------ giant seeds
+
+// One wave per listed giant: binds the seed again and writes its sub-items
+// -- kSubsPerGiant ranges of its level-0 row, or, when that row is shorter
+// than kSubMinRow, up to 4 ranges of the level-1 row below each level-0
+// entry (one lane per entry), with that entry's bindings -- and records
+// how many it wrote. Capacity is never short: cap * kSubsPerGiant slots.
+template <int NS>
+__global__ void __launch_bounds__(kWave) k_giant_expand(GiantArgs ga, const Plan* plans, const ConflictRefs* refs,
+                                                        DevGraph G) {
+    __shared__ ThreadLds W;
+    const int lane = lane_id();
+    uint32_t n = *ga.count;
+    if (n > ga.cap) n = ga.cap;
+    for (uint32_t g = blockIdx.x; g < n; g += gridDim.x) {
+        const HeavyItem it = ga.list[g];
+        const uint32_t hv = ga.heavyIdx[g];
+        copy_plan(W, plans + it.plan);
+        const Plan& P = W.plan;
+        int v[NS];
+        uint32_t total = 0;
+        if (bind_seed(P, G, (int)it.item, v)) {
+            const Frame f0 = enter(W, P, 0, G, v);
+            const int len0 = f0.end - f0.i;
+            if (len0 >= kSubMinRow || (len0 > 0 && P.nlev == 1)) {
+                const int parts = len0 < kSubsPerGiant ? len0 : kSubsPerGiant;
+                const int chunk = (len0 + parts - 1) / parts;
+                const int items = (len0 + chunk - 1) / chunk;
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(ga.subCount, (uint32_t)items);
+                base = bcast(base);
+                for (int c = lane; c < items; c += kWave) {
+                    SubItem& S = ga.subs[base + c];
+                    S.giant = g;
+                    S.plan = it.plan;
+                    S.level = 0;
+                    S.begin = f0.i + c * chunk;
+                    S.end = f0.i + (c + 1) * chunk < f0.end ? f0.i + (c + 1) * chunk : f0.end;
+                    S.heavy = hv;
+#pragma unroll
+                    for (int j = 0; j < kMaxSlots; ++j) S.v[j] = -1;
+#pragma unroll
+                    for (int j = 0; j < NS; ++j) S.v[j] = v[j];
+                }
+                total = (uint32_t)items;
+            } else if (len0 > 0) {
+                // short level-0 row (< kSubMinRow <= 64 entries): one lane per entry, cut its level-1 row
+                int mine = 0, chunk = 1;
+                Frame f1{0, 0};
+                if (lane < len0) {
+                    const int i = f0.i + lane;
+                    const int x = G.col[i];
+                    bool ok = bind_entry_x(P.lev[0], G, refs + it.plan, v, i, x);
+                    if (ok && level_kind(P.lev[0].flags) == kIsect) ok = isect_find(W, 0, P.lev[0], G, v, x);
+                    if (ok) {
+                        f1 = enter(W, P, 1, G, v);
+                        const int len1 = f1.end - f1.i;
+                        if (len1 > 0) {
+                            const int parts = len1 < 4 ? len1 : 4;
+                            chunk = (len1 + parts - 1) / parts;
+                            mine = (len1 + chunk - 1) / chunk;
+                        }
+                    }
+                }
+                // exclusive prefix of the lanes' item counts
+                int incl = mine;
+#pragma unroll
+                for (int off = 1; off < kWave; off <<= 1) {
+                    const int y = __shfl_up(incl, off, kWave);
+                    if (lane >= off) incl += y;
+                }
+                const int all = __shfl(incl, kWave - 1, kWave);
+                uint32_t base = 0;
+                if (lane == 0 && all > 0) base = atomicAdd(ga.subCount, (uint32_t)all);
+                base = bcast(base) + (uint32_t)(incl - mine);
+                for (int c = 0; c < mine; ++c) {
+                    SubItem& S = ga.subs[base + c];
+                    S.giant = g;
+                    S.plan = it.plan;
+                    S.level = 1;
+                    S.begin = f1.i + c * chunk;
+                    S.end = f1.i + (c + 1) * chunk < f1.end ? f1.i + (c + 1) * chunk : f1.end;
+                    S.heavy = hv;
+#pragma unroll
+                    for (int j = 0; j < kMaxSlots; ++j) S.v[j] = -1;
+#pragma unroll
+                    for (int j = 0; j < NS; ++j) S.v[j] = v[j];
+                }
+                total = (uint32_t)all;
+            }
+        }
+        if (lane == 0) ga.remaining[g] = total;
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// The sub-items of all giants, over all waves: each one is a pool (its range
+// and bindings) run by wave_run against the giant's shared row or found
+// flag. The sub-item that finishes a giant's last clears its row.
+template <int K, int D, int NS>
+__global__ void __launch_bounds__(kWave) k_wave_sub(WaveArgs a, DevGraph G) {
+    __shared__ WaveLds W;
+    __shared__ int cache[D == kDedupSort ? kCacheSlots : 1];
+    const int lane = lane_id();
+    const GiantArgs& ga = a.giant;
+    uint32_t n = *ga.subCount;
+    const uint32_t capSubs = ga.cap * (uint32_t)kSubsPerGiant;
+    if (n > capSubs) n = capSubs;
+    if (n == 0) return;   // no giants listed
+    Dedup<D> dd = wave_dedup<D>(W, cache, nullptr, a, G);
+    uint32_t loaded = 0xffffffffu;
+    unsigned long long stats[kWaveStats] = {0, 0, 0, 0, 0};
+    for (;;) {
+        const uint32_t si = wave_take(ga.subCursor);
+        if (si >= n) break;
+        const SubItem& S = ga.subs[si];
+        const uint32_t g = (uint32_t)uniform((int)S.giant);
+        const uint32_t plan = (uint32_t)uniform((int)S.plan);
+        const uint32_t hv = (uint32_t)uniform((int)S.heavy);
+        if (plan != loaded) {
+            copy_plan(W, a.plans + plan);
+            loaded = plan;
+        }
+        // existence searches (SUPPORT, and PAIRS plans whose head object is bound by the seed) count a giant once
+        uint32_t* foundFlag = (K == kSupport || W.plan.pairLevel < 0) ? ga.found + g : nullptr;
+        uint32_t skip = 0;
+        if (foundFlag && lane == 0) skip = ld_rlx(foundFlag);
+        uint64_t c = 0;
+        if (!bcast(skip)) {
+            if (lane < kMaxSlots) W.poolV[lane] = S.v[lane];
+            if (lane == 0) {
+                W.poolLevel = S.level;
+                W.poolNext = S.begin;
+                W.poolEnd = S.end;
+                W.poolGen = W.poolGen + 1u;
+                W.nTouched = 0;
+            }
+            if (D == kDedupSort && K == kPairs)
+                for (int k = lane; k < kCacheSlots; k += kWave) cache[k] = -1;
+            wave_sync();
+            dd.shared = (K == kPairs && D != kDedupSort) ? ga.rows + (int64_t)g * a.rowWords : nullptr;
+            bool aborted = false;
+            c = wave_run<K, D, NS>(W, G, a.refs + plan, dd, a.rowWords, a.sink, (unsigned long long)hv << a.sink.hoBits,
+                               ga, false, HeavyItem{plan, 0u}, hv, foundFlag, aborted, stats);
+            dd.shared = nullptr;
+        }
+        if (lane == 0 && c) atomicAdd(a.counts + 2 * W.plan.rule + K, (unsigned long long)c);
+        uint32_t last = 0;
+        if (lane == 0) last = atomicSub(ga.remaining + g, 1u) == 1u ? 1u : 0u;
+        if (bcast(last) && K == kPairs && D != kDedupSort) {
+            uint32_t* row = ga.rows + (int64_t)g * a.rowWords;
+            for (int64_t w = lane; w < a.rowWords; w += kWave) row[w] = 0u;   // read again only by later launches
+        }
+    }
+    flush_stats(ga, stats);
+}
+
+// ---------------------------------------------------------------- launchers
+
+// One instantiation per plan kind, so a profile names the phase and the
+// SUPPORT kernels carry no distinct-set code. Wave-mode launches run one wave
+// per block; kDedupLds blocks carry their bitmap (rowWords words) in dynamic
+// LDS.
+template <int NS>
+void SlotKernels<NS>::thread_batch(int kind, const ThreadArgs& a, const DevGraph& G, int blocks, hipStream_t s) {
+    if (kind == kSupport) hipLaunchKernelGGL((k_thread_batch<kSupport, NS>), dim3(blocks), dim3(kBlock), 0, s, a, G);
+    else hipLaunchKernelGGL((k_thread_batch<kPairs, NS>), dim3(blocks), dim3(kBlock), 0, s, a, G);
+}
+
+template <int NS>
+void SlotKernels<NS>::thread_one(int kind, const Plan& P, const OneArgs& a, const DevGraph& G, int blocks,
+                                 hipStream_t s) {
+    if (kind == kSupport) hipLaunchKernelGGL((k_thread_one<kSupport, NS>), dim3(blocks), dim3(kBlock), 0, s, P, a, G);
+    else hipLaunchKernelGGL((k_thread_one<kPairs, NS>), dim3(blocks), dim3(kBlock), 0, s, P, a, G);
+}
+
+template <int NS>
+void SlotKernels<NS>::wave_batch(int kind, int dedup, const WaveArgs& a, const DevGraph& G, int blocks,
+                                 hipStream_t s) {
+    if (kind == kSupport)
+        hipLaunchKernelGGL((k_wave_batch<kSupport, kDedupRows, NS>), dim3(blocks), dim3(kWave), 0, s, a, G);
+    else if (dedup == kDedupSort)
+        hipLaunchKernelGGL((k_wave_batch<kPairs, kDedupSort, NS>), dim3(blocks), dim3(kWave), 0, s, a, G);
+    else if (dedup == kDedupLds)
+        hipLaunchKernelGGL((k_wave_batch<kPairs, kDedupLds, NS>), dim3(blocks), dim3(kWave), (size_t)a.rowWords * 4,
+                           s, a, G);
+    else
+        hipLaunchKernelGGL((k_wave_batch<kPairs, kDedupRows, NS>), dim3(blocks), dim3(kWave), 0, s, a, G);
+}
+
+template <int NS>
+void SlotKernels<NS>::wave_one(int kind, int dedup, const Plan& P, const WaveArgs& a, const DevGraph& G, int blocks,
+                               hipStream_t s) {
+    if (kind == kSupport)
+        hipLaunchKernelGGL((k_wave_one<kSupport, kDedupRows, NS>), dim3(blocks), dim3(kWave), 0, s, P, a, G);
+    else if (dedup == kDedupSort)
+        hipLaunchKernelGGL((k_wave_one<kPairs, kDedupSort, NS>), dim3(blocks), dim3(kWave), 0, s, P, a, G);
+    else if (dedup == kDedupLds)
+        hipLaunchKernelGGL((k_wave_one<kPairs, kDedupLds, NS>), dim3(blocks), dim3(kWave), (size_t)a.rowWords * 4, s,
+                           P, a, G);
+    else
+        hipLaunchKernelGGL((k_wave_one<kPairs, kDedupRows, NS>), dim3(blocks), dim3(kWave), 0, s, P, a, G);
+}
+
+template <int NS>
+void SlotKernels<NS>::giants(int kind, int dedup, const WaveArgs& a, const DevGraph& G, int blocks, hipStream_t s) {
+    const int eb = blocks < 1024 ? blocks : 1024;
+    hipLaunchKernelGGL(k_giant_expand<NS>, dim3(eb), dim3(kWave), 0, s, a.giant, a.plans, a.refs, G);
+    if (kind == kSupport)
+        hipLaunchKernelGGL((k_wave_sub<kSupport, kDedupRows, NS>), dim3(blocks), dim3(kWave), 0, s, a, G);
+    else if (dedup == kDedupSort)
+        hipLaunchKernelGGL((k_wave_sub<kPairs, kDedupSort, NS>), dim3(blocks), dim3(kWave), 0, s, a, G);
+    else
+        hipLaunchKernelGGL((k_wave_sub<kPairs, kDedupRows, NS>), dim3(blocks), dim3(kWave), 0, s, a, G);
+}
+
+template <int NS>
+void SlotKernels<NS>::prepare(size_t dynLds) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wave_batch<kPairs, kDedupLds, NS>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)dynLds);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wave_one<kPairs, kDedupLds, NS>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)dynLds);
+}
+
+}  // namespace gr
