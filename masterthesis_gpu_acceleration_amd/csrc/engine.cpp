@@ -72,11 +72,29 @@ bool Context::upload() {
     m.insert(m.end(), g.keysO.begin(), g.keysO.end());
     if (!put(keys, m, st)) return false;
     GR_HIP(hipStreamSynchronize(st));
+    // crank: each entry's value ranked among the keys of the other direction
+    // of its predicate (every value of (d, p) is a key of (1-d, p))
+    m.assign((size_t)2 * T, 0);
+    {
+        std::vector<int32_t> pos((size_t)std::max<int64_t>(g.nE, 1), 0);
+        for (int d = 0; d < 2; ++d) {
+            const std::vector<int32_t>& val = d == 0 ? g.spoVal : g.posVal;
+            const std::vector<int32_t>& ok = d == 0 ? g.keysO : g.keysS;
+            const std::vector<int32_t>& oo = d == 0 ? g.keysOOff : g.keysSOff;
+            for (int32_t p = 0; p < P; ++p) {
+                for (int32_t r = oo[p]; r < oo[p + 1]; ++r) pos[ok[r]] = r - oo[p];
+                for (int32_t e = g.predOff[p]; e < g.predOff[p + 1]; ++e) m[(size_t)d * T + e] = pos[val[e]];
+            }
+        }
+    }
+    if (!put(crank, m, st)) return false;
+    GR_HIP(hipStreamSynchronize(st));
     dg.blocks = blocks.as<KeyBlock>();
     dg.rowStart = rowStart.as<int32_t>();
     dg.col = col.as<int32_t>();
     dg.coo = coo.as<int32_t>();
     dg.keys = keys.as<int32_t>();
+    dg.crank = crank.as<int32_t>();
     if (!one.ensure(512)) return false;
     GR_HIP(hipMemset(one.p, 0, 512));
     return ensure_dedup();

@@ -190,7 +190,7 @@ struct Context : Handle {
     size_t ldsPerBlock = 65536, ldsPerCU = 65536;
     hipStream_t st = nullptr;
     hipEvent_t ev[8] = {};
-    DevBuf blocks, rowStart, col, coo, keys;
+    DevBuf blocks, rowStart, col, coo, keys, crank;
     DevGraph dg{};
     // per-rule launch scratch
     DevBuf one;       // [0] count (u64); +64: u32 control words; +128: ConflictRefs

@@ -80,7 +80,8 @@ struct alignas(16) Plan {
     // bitmaps index them by rank among those keys: key-index block run
     // hoLo | hoHi << 32, global row number hoRow0 of its first key, hoKeys keys.
     int32_t hoLo, hoHi, hoRow0, hoKeys;
-    int32_t pad0, pad1;
+    int32_t hoByEntry;   // 1: the pair level binds the head object as col[e], whose rank is crank[e]
+    int32_t pad1;
     Level lev[kMaxBody];
 };
 
@@ -153,6 +154,9 @@ struct DevGraph {
     const int32_t* col;       // [2T]
     const int32_t* coo;       // [2T]
     const int32_t* keys;      // [rows]
+    // [2T] for entry e of (direction d, predicate p): the rank of col[e] among
+    // the keys of (1-d, p) -- the bitmap index of a head object bound by e
+    const int32_t* crank;
 };
 
 }  // namespace gr

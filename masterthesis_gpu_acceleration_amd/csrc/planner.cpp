@@ -222,6 +222,7 @@ void head_object_ranks(Plan& Q, const HostGraph& g) {
     Q.hoHi = (int32_t)(base2 >> 32);
     Q.hoRow0 = dir2 == kSPO ? g.keysSOff[p] : (int32_t)g.keysS.size() + g.keysOOff[p];
     Q.hoKeys = g.nkeys(dir2, p);
+    Q.hoByEntry = kd == kScan ? 0 : 1;
 }
 
 }  // namespace

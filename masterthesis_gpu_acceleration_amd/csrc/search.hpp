@@ -513,6 +513,7 @@ struct Dedup {
     WaveLds* W;
     int* cache;
     const KeyBlock* blocks;
+    const int32_t* crank;
     // Rank of head object x (sort path: x itself). x is a key of the space by
     // construction; the clamp only keeps a wrong plan from writing out of bounds.
     __device__ __forceinline__ int rank(int x) const {
@@ -523,6 +524,13 @@ struct Dedup {
         const uint64_t bit = 1ull << (x & 63);
         const int r = (int)(kb.rank + (uint32_t)__popcll(kb.bits & (bit - 1))) - P.hoRow0;
         return r < 0 ? 0 : (r >= P.hoKeys ? P.hoKeys - 1 : r);
+    }
+    // Rank of head object x bound by entry i at the pair level: crank[i] when
+    // the pair level binds it as col[i] (the plan's rank space by
+    // construction), else from the key index (sort path: x itself).
+    __device__ __forceinline__ int rank_at(int i, int x) const {
+        if (D == kDedupSort) return x;
+        return W->plan.hoByEntry ? crank[i] : rank(x);
     }
     __device__ __forceinline__ bool seen(int x) const { return seen_r(rank(x)); }
     __device__ __forceinline__ uint32_t insert(int x) { return insert_r(rank(x)); }
@@ -561,12 +569,12 @@ struct Dedup {
 };
 
 // Marks the head objects of a batch of last-level groundings (ho[k] >= 0;
-// -1: none). The word tests are issued together, then the marks together,
-// so a batch costs two memory round trips instead of two per candidate.
-// Returns the objects this lane marked first; on the sort path emit[k] gets
-// the objects to emit.
+// -1: none), with their ranks hr[k] (bitmap paths). The word tests are
+// issued together, then the marks together, so a batch costs two memory
+// round trips instead of two per candidate. Returns the objects this lane
+// marked first; on the sort path emit[k] gets the objects to emit.
 template <int D, int U>
-__device__ __forceinline__ uint32_t mark_batch(Dedup<D>& dd, const int (&ho)[U], int (&emit)[U]) {
+__device__ __forceinline__ uint32_t mark_batch(Dedup<D>& dd, const int (&ho)[U], const int (&hr)[U], int (&emit)[U]) {
     uint32_t n = 0;
     if (D == kDedupSort) {
 #pragma unroll
@@ -582,7 +590,7 @@ __device__ __forceinline__ uint32_t mark_batch(Dedup<D>& dd, const int (&ho)[U],
     int r[U];
     uint32_t w[U];
 #pragma unroll
-    for (int k = 0; k < U; ++k) r[k] = ho[k] >= 0 ? dd.rank(ho[k]) : -1;
+    for (int k = 0; k < U; ++k) r[k] = ho[k] >= 0 ? hr[k] : -1;
 #pragma unroll
     for (int k = 0; k < U; ++k) w[k] = r[k] >= 0 ? dd.word_r(r[k]) : 0xffffffffu;
 #pragma unroll
@@ -649,6 +657,7 @@ __device__ __forceinline__ uint64_t wave_run(WaveLds& W, const DevGraph& G, cons
     bool busy = false, found = false, stopped = false;
     aborted = false;
     int d = 0, base = 0;            // own search: current level, first level it owns
+    int hr = -1;                    // rank of the head object once v binds it (pairs)
     uint32_t gen = 0xffffffffu;     // pool whose bindings v holds (idle lanes)
     uint64_t mine = 0;
     int steps = 0;
@@ -663,10 +672,12 @@ __device__ __forceinline__ uint64_t wave_run(WaveLds& W, const DevGraph& G, cons
             const int r = __popcll(idleM & below);
             int next = pe;
             bool drop = false;
+            int hru = -1;              // rank of a head object bound above the pool
             if (pairs && pl < plv) {   // every entry shares a head object bound above the pool
-                const int hr = P.hoRef;
-                const int hov = hr >= 0 ? W.poolV[hr] : -(hr + 1);
-                drop = dd.seen(uniform(hov));
+                const int ref = P.hoRef;
+                const int hov = ref >= 0 ? W.poolV[ref] : -(ref + 1);
+                hru = uniform(dd.rank(uniform(hov)));
+                drop = dd.seen_r(hru);
             }
             if (!drop) {
                 if (!busy && gen != pg) {
@@ -677,11 +688,14 @@ __device__ __forceinline__ uint64_t wave_run(WaveLds& W, const DevGraph& G, cons
                 if (plv == last) {
                     ++nLeafDraw;
                     next = pn + n * kLeafUnroll < pe ? pn + n * kLeafUnroll : pe;
-                    int xs[kLeafUnroll];
+                    const bool byEntry = pairs && pl == last && D != kDedupSort && P.hoByEntry;
+                    int xs[kLeafUnroll], rs[kLeafUnroll];
 #pragma unroll
                     for (int k = 0; k < kLeafUnroll; ++k) {
                         const int i = pn + r + n * k;
-                        xs[k] = (!busy && i < pe) ? G.col[i] : 0;
+                        const bool in = !busy && i < pe;
+                        xs[k] = in ? G.col[i] : 0;
+                        rs[k] = (in && byEntry) ? G.crank[i] : hru;
                     }
                     bool hit = false;
                     int hos[kLeafUnroll];
@@ -690,8 +704,12 @@ __device__ __forceinline__ uint64_t wave_run(WaveLds& W, const DevGraph& G, cons
                         const int i = pn + r + n * k;
                         hos[k] = -1;
                         if (!busy && i < pe && bind_entry_x(P.lev[last], G, R, v, i, xs[k])) {
-                            if (!pairs) found = true;
-                            else hos[k] = term_value(v, P.hoRef);
+                            if (!pairs) {
+                                found = true;
+                            } else {
+                                hos[k] = term_value(v, P.hoRef);
+                                if (pl == last && !byEntry) rs[k] = dd.rank_at(i, hos[k]);
+                            }
                         }
                     }
                     if (pairs) {
@@ -703,7 +721,7 @@ __device__ __forceinline__ uint64_t wave_run(WaveLds& W, const DevGraph& G, cons
                             for (int k = 1; k < kLeafUnroll; ++k) hos[k] = -1;
                         }
                         int emits[kLeafUnroll];
-                        mine += mark_batch<D, kLeafUnroll>(dd, hos, emits);
+                        mine += mark_batch<D, kLeafUnroll>(dd, hos, rs, emits);
                         hit = hos[0] >= 0;
                         if (D == kDedupSort) {
 #pragma unroll
@@ -720,7 +738,12 @@ __device__ __forceinline__ uint64_t wave_run(WaveLds& W, const DevGraph& G, cons
                         const int x = G.col[i];
                         bool ok = bind_entry_x(P.lev[plv], G, R, v, i, x);
                         if (ok && level_kind(P.lev[plv].flags) == kIsect) ok = isect_find(W, plv, P.lev[plv], G, v, x);
-                        if (ok && pairs && plv == pl) ok = !dd.seen(term_value(v, P.hoRef));
+                        if (ok && pairs && plv == pl) {
+                            hr = dd.rank_at(i, term_value(v, P.hoRef));
+                            ok = !dd.seen_r(hr);
+                        } else if (pairs && pl < plv) {
+                            hr = hru;
+                        }
                         if (ok) {
                             d = base = plv + 1;
                             W.frame[d][lane] = enter(W, P, d, G, v);
@@ -739,7 +762,7 @@ __device__ __forceinline__ uint64_t wave_run(WaveLds& W, const DevGraph& G, cons
 #pragma unroll
         for (int k = 0; k < kLeafBatch; ++k) emits[k] = -1;
         if (busy) {
-            if (pairs && pl < base && dd.seen(term_value(v, P.hoRef))) {
+            if (pairs && pl < base && dd.seen_r(hr)) {
                 busy = false;   // head object counted by another lane
             } else {
                 const Frame fr = W.frame[d][lane];
@@ -750,9 +773,13 @@ __device__ __forceinline__ uint64_t wave_run(WaveLds& W, const DevGraph& G, cons
                     // last level: up to kLeafBatch candidates per step, loaded together
                     const int take = fr.end - fr.i < kLeafBatch ? fr.end - fr.i : kLeafBatch;
                     W.frame[d][lane].i = fr.i + take;
-                    int xs[kLeafBatch];
+                    const bool byEntry = pairs && d == pl && D != kDedupSort && P.hoByEntry;
+                    int xs[kLeafBatch], rs[kLeafBatch];
 #pragma unroll
-                    for (int k = 0; k < kLeafBatch; ++k) xs[k] = k < take ? G.col[fr.i + k] : 0;
+                    for (int k = 0; k < kLeafBatch; ++k) {
+                        xs[k] = k < take ? G.col[fr.i + k] : 0;
+                        rs[k] = (k < take && byEntry) ? G.crank[fr.i + k] : hr;
+                    }
                     int hos[kLeafBatch];
                     bool any = false;
 #pragma unroll
@@ -760,6 +787,7 @@ __device__ __forceinline__ uint64_t wave_run(WaveLds& W, const DevGraph& G, cons
                         hos[k] = -1;
                         if (k < take && bind_entry_x(P.lev[d], G, R, v, fr.i + k, xs[k])) {
                             hos[k] = pairs ? term_value(v, P.hoRef) : 0;
+                            if (pairs && d == pl && !byEntry) rs[k] = dd.rank_at(fr.i + k, hos[k]);
                             any = true;
                         }
                     }
@@ -780,20 +808,22 @@ __device__ __forceinline__ uint64_t wave_run(WaveLds& W, const DevGraph& G, cons
                                 else busy = false;
                             }
                         }
-                        mine += mark_batch<D, kLeafBatch>(dd, hos, emits);
+                        mine += mark_batch<D, kLeafBatch>(dd, hos, rs, emits);
                     }
                 } else {
                     W.frame[d][lane].i = fr.i + 1;
                     const int x = G.col[fr.i];
                     bool ok = bind_entry_x(P.lev[d], G, R, v, fr.i, x);
                     if (ok && level_kind(P.lev[d].flags) == kIsect) ok = isect_next(W, d, G, x);
-                    if (ok && pairs && d == pl) ok = !dd.seen(term_value(v, P.hoRef));
+                    if (ok && pairs && d == pl) {
+                        hr = dd.rank_at(fr.i, term_value(v, P.hoRef));
+                        ok = !dd.seen_r(hr);
+                    }
                     if (ok) {
                         if (d == last) {
                             if (pairs) {
-                                const int ho = term_value(v, P.hoRef);
-                                mine += dd.insert(ho);
-                                emit = ho;
+                                mine += dd.insert_r(hr);
+                                emit = term_value(v, P.hoRef);
                                 if (pl >= base) d = pl;   // head object settled: next candidate at the pair level
                                 else busy = false;
                             } else {
@@ -988,6 +1018,7 @@ __device__ __forceinline__ Dedup<D> wave_dedup(WaveLds& W, int* cache, uint32_t*
     dd.bits = nullptr;
     dd.shared = nullptr;
     dd.blocks = G.blocks;
+    dd.crank = G.crank;
     if (D == kDedupRows) dd.bits = a.rows + (int64_t)blockIdx.x * a.rowWords;
     if (D == kDedupLds) {
         dd.bits = dyn;
