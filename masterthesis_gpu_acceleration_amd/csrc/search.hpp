@@ -615,6 +615,51 @@ __device__ __forceinline__ void emit_keys(const SortSink& sink, unsigned long lo
     }
 }
 
+// Bitmap paths: marks ranks hr[k] (>= 0; -1: none), tests first, as mark_batch.
+template <int D, int U>
+__device__ __forceinline__ uint32_t mark_ranks(Dedup<D>& dd, const int (&hr)[U]) {
+    uint32_t w[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) w[k] = hr[k] >= 0 ? dd.word_r(hr[k]) : 0xffffffffu;
+    uint32_t n = 0;
+#pragma unroll
+    for (int k = 0; k < U; ++k)
+        if (!((w[k] >> (hr[k] & 31)) & 1u)) n += dd.insert_r(hr[k]);
+    return n;
+}
+
+// Candidates i[k] (valid where in[k]) of a last level that is a plain EXT --
+// no triple conflicts -- whose bound variable is, for PAIRS, the head object
+// with rank crank[i]: a candidate grounds iff its value is not bound yet.
+// Returns the grounding mask; rs[k] gets the rank of each grounding (PAIRS),
+// else -1. Straight-line code: no per-candidate branches.
+template <int U, int NS>
+__device__ __forceinline__ uint32_t leaf_fast(const DevGraph& G, const int (&v)[NS], int nbound, bool pairs,
+                                              const int (&i)[U], const bool (&in)[U], int (&rs)[U]) {
+    int xs[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+        xs[k] = in[k] ? G.col[i[k]] : 0;
+        rs[k] = (in[k] && pairs) ? G.crank[i[k]] : 0;
+    }
+    uint32_t m = 0;
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+        const bool ok = in[k] && !taken(v, nbound, xs[k]);
+        m |= (uint32_t)ok << k;
+        rs[k] = ok ? rs[k] : -1;
+    }
+    return m;
+}
+
+// Whether level `last` takes the leaf_fast path in plan P under dedup D.
+template <int D>
+__device__ __forceinline__ bool leaf_fast_ok(const Plan& P, int last, bool pairs) {
+    const int f = P.lev[last].flags;
+    if (level_kind(f) != kExt || level_conflicts(f) != 0) return false;
+    return !pairs || (D != kDedupSort && P.pairLevel == last && P.hoByEntry);
+}
+
 constexpr int kLeafUnroll = 8;   // last-level pool entries an idle lane takes per draw
 
 // Wave mode: the whole wave on one seed, with a shared pool of work.
@@ -685,7 +730,20 @@ __device__ __forceinline__ uint64_t wave_run(WaveLds& W, const DevGraph& G, cons
                     for (int j = 0; j < NS; ++j) v[j] = W.poolV[j];
                     gen = pg;
                 }
-                if (plv == last) {
+                if (plv == last && leaf_fast_ok<D>(P, last, pairs)) {
+                    ++nLeafDraw;
+                    next = pn + n * kLeafUnroll < pe ? pn + n * kLeafUnroll : pe;
+                    int is[kLeafUnroll], rs[kLeafUnroll];
+                    bool in[kLeafUnroll];
+#pragma unroll
+                    for (int k = 0; k < kLeafUnroll; ++k) {
+                        is[k] = pn + r + n * k;
+                        in[k] = !busy && is[k] < pe;
+                    }
+                    const uint32_t m = leaf_fast(G, v, level_nbound(P.lev[last].flags), pairs, is, in, rs);
+                    if (!pairs) found = found || m != 0u;
+                    else mine += mark_ranks<D, kLeafUnroll>(dd, rs);
+                } else if (plv == last) {
                     ++nLeafDraw;
                     next = pn + n * kLeafUnroll < pe ? pn + n * kLeafUnroll : pe;
                     const bool byEntry = pairs && pl == last && D != kDedupSort && P.hoByEntry;
@@ -769,6 +827,26 @@ __device__ __forceinline__ uint64_t wave_run(WaveLds& W, const DevGraph& G, cons
                 if (fr.i >= fr.end) {
                     if (d == base) busy = false;
                     else --d;
+                } else if (d == last && fr.end - fr.i > 1 && leaf_fast_ok<D>(P, last, pairs)) {
+                    // last level, plain EXT: up to kLeafBatch candidates per step, straight-line
+                    const int take = fr.end - fr.i < kLeafBatch ? fr.end - fr.i : kLeafBatch;
+                    W.frame[d][lane].i = fr.i + take;
+                    int is[kLeafBatch], rs[kLeafBatch];
+                    bool in[kLeafBatch];
+#pragma unroll
+                    for (int k = 0; k < kLeafBatch; ++k) {
+                        is[k] = fr.i + k;
+                        in[k] = k < take;
+                    }
+                    const uint32_t m = leaf_fast(G, v, level_nbound(P.lev[d].flags), pairs, is, in, rs);
+                    if (!pairs) {
+                        if (m) {
+                            found = true;
+                            busy = false;
+                        }
+                    } else {
+                        mine += mark_ranks<D, kLeafBatch>(dd, rs);
+                    }
                 } else if (d == last && fr.end - fr.i > 1) {
                     // last level: up to kLeafBatch candidates per step, loaded together
                     const int take = fr.end - fr.i < kLeafBatch ? fr.end - fr.i : kLeafBatch;
