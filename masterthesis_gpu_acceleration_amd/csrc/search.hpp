@@ -63,7 +63,9 @@ struct Frame {
 struct ThreadLds {
     Plan plan;
     Frame frame[kMaxBody][kWave];
-    Frame cur[kMaxBody][kWave];   // ISECT levels: cursor into row 2 and its end
+    // ISECT level L: cursor into row 2 and its end, at cur[L >> 1] (an ISECT
+    // level is always followed by its CHECK, so no two share an index)
+    Frame cur[kMaxBody / 2][kWave];
 };
 
 // Wave mode adds the seed's shared pool -- a frame at level poolLevel whose
@@ -190,9 +192,9 @@ __device__ __forceinline__ Frame enter(ThreadLds& W, const Plan& P, int L, const
     if (kind == kIsect) {
         const Frame r2 = isect_row(lv, G, v);
         if (r2.i >= r2.end) return Frame{0, 0};
-        W.cur[L][lane_id()] = r2;
+        W.cur[L >> 1][lane_id()] = r2;
     } else if (kind == kCheck && L > 0 && level_kind(P.lev[L - 1].flags) == kIsect) {
-        const int j = W.cur[L - 1][lane_id()].i;
+        const int j = W.cur[(L - 1) >> 1][lane_id()].i;
         return Frame{j, j + 1};
     }
     return enter_level(lv, G, v);
@@ -203,8 +205,8 @@ __device__ __forceinline__ Frame enter(ThreadLds& W, const Plan& P, int L, const
 // Leaves the cursor at the first entry >= x (the match, if any).
 __device__ __forceinline__ bool isect_next(ThreadLds& W, int L, const DevGraph& G, int x) {
     const int lane = lane_id();
-    int j = W.cur[L][lane].i;
-    const int e = W.cur[L][lane].end;
+    int j = W.cur[L >> 1][lane].i;
+    const int e = W.cur[L >> 1][lane].end;
     bool hit = false;
     if (j < e) {
         const int y = G.col[j];
@@ -232,7 +234,7 @@ __device__ __forceinline__ bool isect_next(ThreadLds& W, int L, const DevGraph& 
             hit = y == x;
         }
     }
-    W.cur[L][lane].i = j;
+    W.cur[L >> 1][lane].i = j;
     return hit;
 }
 
@@ -244,7 +246,7 @@ __device__ __forceinline__ bool isect_find(ThreadLds& W, int L, const Level& lv,
     const Frame r2 = isect_row(lv, G, v);
     const int at = find_in_row(G.col, r2.i, r2.end, x);
     if (at < 0) return false;
-    W.cur[L][lane_id()] = Frame{at, r2.end};
+    W.cur[L >> 1][lane_id()] = Frame{at, r2.end};
     return true;
 }
 
@@ -357,6 +359,44 @@ __device__ __forceinline__ void push_heavy(bool heavy, uint32_t plan, uint32_t i
     }
 }
 
+// Candidates i[k] (valid where in[k]) of a last level that is a plain EXT --
+// no triple conflicts -- whose bound variable is, for PAIRS, the head object
+// with rank crank[i]: a candidate grounds iff its value is not bound yet.
+// Returns the grounding mask; xs[k] gets each candidate's value, rs[k] the
+// rank of each grounding (when `pairs`; else 0), -1 where none. Straight-line
+// code: no per-candidate branches.
+template <int U, int NS>
+__device__ __forceinline__ uint32_t leaf_fast(const DevGraph& G, const int (&v)[NS], int nbound, bool pairs,
+                                              const int (&i)[U], const bool (&in)[U], int (&xs)[U], int (&rs)[U]) {
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+        xs[k] = in[k] ? G.col[i[k]] : 0;
+        rs[k] = (in[k] && pairs) ? G.crank[i[k]] : 0;
+    }
+    uint32_t m = 0;
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+        const bool ok = in[k] && !taken(v, nbound, xs[k]);
+        m |= (uint32_t)ok << k;
+        rs[k] = ok ? rs[k] : -1;
+    }
+    return m;
+}
+
+// Whether level `last` is a plain EXT that, for PAIRS, binds the head object.
+__device__ __forceinline__ bool leaf_plain(const Plan& P, int last, bool pairs) {
+    const int f = P.lev[last].flags;
+    if (level_kind(f) != kExt || level_conflicts(f) != 0) return false;
+    return !pairs || P.pairLevel == last;
+}
+
+// Whether level `last` takes the leaf_fast path of wave mode under dedup D
+// (bitmap marks by crank rank).
+template <int D>
+__device__ __forceinline__ bool leaf_fast_ok(const Plan& P, int last, bool pairs) {
+    return leaf_plain(P, last, pairs) && (!pairs || (D != kDedupSort && P.hoByEntry));
+}
+
 // Thread mode over seeds [begin, begin + count) of the plan in W. Every lane
 // runs its own depth-first search; a lane whose seed is finished takes the
 // next one (ballot prefix count), so no lane waits for the slowest seed.
@@ -409,6 +449,40 @@ __device__ __forceinline__ uint64_t thread_chunk(ThreadLds& W, const DevGraph& G
                     if (pairs) mine += (uint64_t)set.n;
                 } else {
                     --d;
+                }
+            } else if (d + 1 == nlev && fr.end - fr.i > 1 && leaf_plain(P, d, pairs)) {
+                // last level, plain EXT: up to kLeafBatchT candidates per step, straight-line
+                const int take = fr.end - fr.i < kLeafBatchT ? fr.end - fr.i : kLeafBatchT;
+                W.frame[d][lane].i = fr.i + take;
+                budget -= take;
+                if (budget < 0) {
+                    heavyNow = true;
+                    active = false;
+                } else {
+                    int is[kLeafBatchT], xs[kLeafBatchT], rs[kLeafBatchT];
+                    bool in[kLeafBatchT];
+#pragma unroll
+                    for (int k = 0; k < kLeafBatchT; ++k) {
+                        is[k] = fr.i + k;
+                        in[k] = k < take;
+                    }
+                    const uint32_t m = leaf_fast(G, v, level_nbound(P.lev[d].flags), false, is, in, xs, rs);
+                    if (!pairs) {
+                        if (m) {
+                            mine += 1;
+                            active = false;
+                        }
+                    } else {
+                        // a row holds each value once: the new objects are distinct
+                        bool over = false;
+#pragma unroll
+                        for (int k = 0; k < kLeafBatchT; ++k)
+                            if (((m >> k) & 1u) && !set.seen(xs[k])) over = over || !set.insert(xs[k]);
+                        if (over) {
+                            heavyNow = true;
+                            active = false;
+                        }
+                    }
                 }
             } else if (d + 1 == nlev && fr.end - fr.i > 1) {
                 // last level: up to kLeafBatchT candidates per step, loaded together
@@ -628,38 +702,6 @@ __device__ __forceinline__ uint32_t mark_ranks(Dedup<D>& dd, const int (&hr)[U])
     return n;
 }
 
-// Candidates i[k] (valid where in[k]) of a last level that is a plain EXT --
-// no triple conflicts -- whose bound variable is, for PAIRS, the head object
-// with rank crank[i]: a candidate grounds iff its value is not bound yet.
-// Returns the grounding mask; rs[k] gets the rank of each grounding (PAIRS),
-// else -1. Straight-line code: no per-candidate branches.
-template <int U, int NS>
-__device__ __forceinline__ uint32_t leaf_fast(const DevGraph& G, const int (&v)[NS], int nbound, bool pairs,
-                                              const int (&i)[U], const bool (&in)[U], int (&rs)[U]) {
-    int xs[U];
-#pragma unroll
-    for (int k = 0; k < U; ++k) {
-        xs[k] = in[k] ? G.col[i[k]] : 0;
-        rs[k] = (in[k] && pairs) ? G.crank[i[k]] : 0;
-    }
-    uint32_t m = 0;
-#pragma unroll
-    for (int k = 0; k < U; ++k) {
-        const bool ok = in[k] && !taken(v, nbound, xs[k]);
-        m |= (uint32_t)ok << k;
-        rs[k] = ok ? rs[k] : -1;
-    }
-    return m;
-}
-
-// Whether level `last` takes the leaf_fast path in plan P under dedup D.
-template <int D>
-__device__ __forceinline__ bool leaf_fast_ok(const Plan& P, int last, bool pairs) {
-    const int f = P.lev[last].flags;
-    if (level_kind(f) != kExt || level_conflicts(f) != 0) return false;
-    return !pairs || (D != kDedupSort && P.pairLevel == last && P.hoByEntry);
-}
-
 constexpr int kLeafUnroll = 8;   // last-level pool entries an idle lane takes per draw
 
 // Wave mode: the whole wave on one seed, with a shared pool of work.
@@ -733,14 +775,14 @@ __device__ __forceinline__ uint64_t wave_run(WaveLds& W, const DevGraph& G, cons
                 if (plv == last && leaf_fast_ok<D>(P, last, pairs)) {
                     ++nLeafDraw;
                     next = pn + n * kLeafUnroll < pe ? pn + n * kLeafUnroll : pe;
-                    int is[kLeafUnroll], rs[kLeafUnroll];
+                    int is[kLeafUnroll], xs[kLeafUnroll], rs[kLeafUnroll];
                     bool in[kLeafUnroll];
 #pragma unroll
                     for (int k = 0; k < kLeafUnroll; ++k) {
                         is[k] = pn + r + n * k;
                         in[k] = !busy && is[k] < pe;
                     }
-                    const uint32_t m = leaf_fast(G, v, level_nbound(P.lev[last].flags), pairs, is, in, rs);
+                    const uint32_t m = leaf_fast(G, v, level_nbound(P.lev[last].flags), pairs, is, in, xs, rs);
                     if (!pairs) found = found || m != 0u;
                     else mine += mark_ranks<D, kLeafUnroll>(dd, rs);
                 } else if (plv == last) {
@@ -831,14 +873,14 @@ __device__ __forceinline__ uint64_t wave_run(WaveLds& W, const DevGraph& G, cons
                     // last level, plain EXT: up to kLeafBatch candidates per step, straight-line
                     const int take = fr.end - fr.i < kLeafBatch ? fr.end - fr.i : kLeafBatch;
                     W.frame[d][lane].i = fr.i + take;
-                    int is[kLeafBatch], rs[kLeafBatch];
+                    int is[kLeafBatch], xs[kLeafBatch], rs[kLeafBatch];
                     bool in[kLeafBatch];
 #pragma unroll
                     for (int k = 0; k < kLeafBatch; ++k) {
                         is[k] = fr.i + k;
                         in[k] = k < take;
                     }
-                    const uint32_t m = leaf_fast(G, v, level_nbound(P.lev[d].flags), pairs, is, in, rs);
+                    const uint32_t m = leaf_fast(G, v, level_nbound(P.lev[d].flags), pairs, is, in, xs, rs);
                     if (!pairs) {
                         if (m) {
                             found = true;
