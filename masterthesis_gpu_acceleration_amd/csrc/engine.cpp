@@ -108,7 +108,7 @@ bool Context::ensure_dedup() {
     rowWords = (maxKeys + 31) / 32;
     const int64_t rowBytes = rowWords * 4;
     rowsFit = rowBytes * numCUs <= kRowBudget;
-    const size_t blockLds = wave_static_lds() + (size_t)rowBytes + 1024;
+    const size_t blockLds = (wave_static_lds() + (size_t)rowBytes + 511) / 512 * 512;   // LDS granules
     ldsFits = blockLds <= ldsPerBlock;
     ldsPreferred = ldsFits && ldsPerCU / blockLds >= (size_t)kLdsMinWaves;
     waveSlots = numCUs * waveWavesPerCU;

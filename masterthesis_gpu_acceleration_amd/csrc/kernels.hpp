@@ -117,8 +117,8 @@ void launch_wave_batch(int kind, int slots, int dedup, const WaveArgs& a, const 
                        hipStream_t s);
 void launch_wave_one(int kind, int slots, int dedup, const Plan& P, const WaveArgs& a, const DevGraph& G, int blocks,
                      hipStream_t s);
-// Static LDS of a wave-mode block (bytes), and the dynamic-LDS limit to
-// request for the kDedupLds kernels.
+// Static LDS of a kDedupLds wave-mode block (bytes, from the runtime), and
+// the dynamic-LDS limit to request for the kDedupLds kernels.
 size_t wave_static_lds();
 void prepare_wave_kernels(size_t dynLds);
 // The giant-seed pass that follows a wave-mode batch launch: expansion of
@@ -136,6 +136,7 @@ struct SlotKernels {
                          hipStream_t s);
     static void giants(int kind, int dedup, const WaveArgs& a, const DevGraph& G, int blocks, hipStream_t s);
     static void prepare(size_t dynLds);
+    static size_t lds_static();   // static LDS of the kDedupLds wave kernel (bytes)
 };
 extern template struct SlotKernels<4>;
 extern template struct SlotKernels<8>;

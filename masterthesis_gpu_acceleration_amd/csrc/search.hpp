@@ -50,7 +50,7 @@ constexpr int kPlanWords = (int)(sizeof(Plan) / 4);
 constexpr int kTouchCap = 256;      // dedup-row words listed per wave-mode seed
 constexpr int kCacheSlots = 512;    // sort path: LDS cache of emitted head objects per wave (direct mapped)
 constexpr int kOneChunkSeeds = 64;  // seeds per wave in per-rule launches (one rule fills the GPU)
-constexpr int kLeafBatch = 8;       // last-level candidates a lane tries per step of its own search (wave mode)
+constexpr int kLeafBatch = 16;      // last-level candidates a lane tries per step of its own search (wave mode)
 constexpr int kLeafBatchT = 4;      // the same in thread mode (keeps it within 128 VGPRs: 4 waves per SIMD)
 
 struct Frame {
@@ -702,7 +702,7 @@ __device__ __forceinline__ uint32_t mark_ranks(Dedup<D>& dd, const int (&hr)[U])
     return n;
 }
 
-constexpr int kLeafUnroll = 8;   // last-level pool entries an idle lane takes per draw
+constexpr int kLeafUnroll = 16;  // last-level pool entries an idle lane takes per draw
 
 // Wave mode: the whole wave on one seed, with a shared pool of work.
 //   * The pool starts as the seed's level-0 row. Each step, the idle lanes
@@ -1421,6 +1421,14 @@ void SlotKernels<NS>::giants(int kind, int dedup, const WaveArgs& a, const DevGr
         hipLaunchKernelGGL((k_wave_sub<kPairs, kDedupSort, NS>), dim3(blocks), dim3(kWave), 0, s, a, G);
     else
         hipLaunchKernelGGL((k_wave_sub<kPairs, kDedupRows, NS>), dim3(blocks), dim3(kWave), 0, s, a, G);
+}
+
+template <int NS>
+size_t SlotKernels<NS>::lds_static() {
+    hipFuncAttributes fa{};
+    if (hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k_wave_batch<kPairs, kDedupLds, NS>)) != hipSuccess)
+        return sizeof(WaveLds) + kCacheSlots * sizeof(int);
+    return fa.sharedSizeBytes;
 }
 
 template <int NS>
