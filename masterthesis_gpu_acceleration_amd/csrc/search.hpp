@@ -70,7 +70,8 @@ struct ThreadLds {
 
 // Wave mode adds the seed's shared pool -- a frame at level poolLevel whose
 // entries [poolNext, poolEnd) any idle lane may take, with the bindings
-// poolV that hold above that level -- and the dedup words touched so far.
+// poolV that hold above that level -- and the number of dedup-row words
+// touched so far (HBM rows; the list itself is the kernel's `cache` array).
 struct WaveLds : ThreadLds {
     int poolV[kMaxSlots];
     int poolLevel;
@@ -79,8 +80,13 @@ struct WaveLds : ThreadLds {
     uint32_t poolGen;
     uint32_t nTouched;
     uint32_t pad[3];
-    uint32_t touched[kTouchCap];
 };
+
+// Per-wave LDS array beside WaveLds: the sort path's cache of emitted head
+// objects, the HBM-row path's list of touched row words; the LDS-bitmap
+// path needs neither (it clears its whole bitmap per seed).
+template <int D>
+constexpr int wave_cache_words() { return D == kDedupSort ? kCacheSlots : (D == kDedupRows ? kTouchCap : 1); }
 
 static_assert(sizeof(Plan) % 4 == 0, "plan must be word-sized");
 
@@ -585,7 +591,7 @@ struct Dedup {
     uint32_t* bits;     // the wave's own bitmap: its HBM row (kDedupRows) or LDS (kDedupLds)
     uint32_t* shared;   // a giant's HBM row, shared by its sub-items (then used instead of bits), or null
     WaveLds* W;
-    int* cache;
+    int* cache;         // wave_cache_words<D>() words: emitted-object cache (sort) or touched-word list (rows)
     const KeyBlock* blocks;
     const int32_t* crank;
     // Rank of head object x (sort path: x itself). x is a key of the space by
@@ -624,9 +630,9 @@ struct Dedup {
         const uint32_t bit = 1u << (x & 31);
         if (shared) return (atomicOr(shared + (x >> 5), bit) & bit) ? 0u : 1u;
         const uint32_t old = atomicOr(bits + (x >> 5), bit);
-        if (old == 0) {
+        if (D == kDedupRows && old == 0) {   // list the word for the clear
             const uint32_t k = atomicAdd(&W->nTouched, 1u);
-            if (k < (uint32_t)kTouchCap) W->touched[k] = (uint32_t)(x >> 5);
+            if (k < (uint32_t)kTouchCap) cache[k] = x >> 5;
         }
         return (old & bit) ? 0u : 1u;
     }
@@ -1043,13 +1049,17 @@ __device__ __forceinline__ uint64_t wave_run(WaveLds& W, const DevGraph& G, cons
     if (D == kDedupSort) return 0;
     // clear the wave's own bitmap (a sub-item marked only its giant's row: nothing listed)
     wave_sync();
-    const uint32_t nt = W.nTouched;
-    if (nt <= (uint32_t)kTouchCap) {
-        for (uint32_t k = lane; k < nt; k += kWave) dd.clear_word(W.touched[k]);
-    } else {
+    if (D == kDedupLds) {
         for (int64_t w = lane; w < rowWords; w += kWave) dd.clear_word((uint32_t)w);
+        wave_sync();
+    } else {
+        const uint32_t nt = W.nTouched;
+        if (nt <= (uint32_t)kTouchCap) {
+            for (uint32_t k = lane; k < nt; k += kWave) dd.clear_word((uint32_t)dd.cache[k]);
+        } else {
+            for (int64_t w = lane; w < rowWords; w += kWave) dd.clear_word((uint32_t)w);
+        }
     }
-    if (D == kDedupLds) wave_sync();
     const uint64_t total = wave_sum(mine);
     return aborted ? 0 : total;
 }
@@ -1161,7 +1171,7 @@ __device__ __forceinline__ void flush_stats(const GiantArgs& ga, const unsigned 
 template <int K, int D, int NS>
 __global__ void __launch_bounds__(kWave) k_wave_batch(WaveArgs a, DevGraph G) {
     __shared__ WaveLds W;
-    __shared__ int cache[D == kDedupSort ? kCacheSlots : 1];
+    __shared__ int cache[wave_cache_words<D>()];
     extern __shared__ uint32_t dyn[];
     const int lane = lane_id();
     uint32_t n = *a.heavyCount;
@@ -1188,7 +1198,7 @@ __global__ void __launch_bounds__(kWave) k_wave_batch(WaveArgs a, DevGraph G) {
 template <int K, int D, int NS>
 __global__ void __launch_bounds__(kWave) k_wave_one(Plan P, WaveArgs a, DevGraph G) {
     __shared__ WaveLds W;
-    __shared__ int cache[D == kDedupSort ? kCacheSlots : 1];
+    __shared__ int cache[wave_cache_words<D>()];
     extern __shared__ uint32_t dyn[];
     const int lane = lane_id();
     uint32_t n = *a.heavyCount;
@@ -1309,7 +1319,7 @@ __global__ void __launch_bounds__(kWave) k_giant_expand(GiantArgs ga, const Plan
 template <int K, int D, int NS>
 __global__ void __launch_bounds__(kWave) k_wave_sub(WaveArgs a, DevGraph G) {
     __shared__ WaveLds W;
-    __shared__ int cache[D == kDedupSort ? kCacheSlots : 1];
+    __shared__ int cache[wave_cache_words<D>()];
     const int lane = lane_id();
     const GiantArgs& ga = a.giant;
     uint32_t n = *ga.subCount;
