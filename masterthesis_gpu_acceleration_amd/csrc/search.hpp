@@ -92,6 +92,8 @@ static_assert(sizeof(Plan) % 4 == 0, "plan must be word-sized");
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 __device__ __forceinline__ int wave_in_block() { return threadIdx.x / kWave; }
+// A value every lane holds alike, as a scalar.
+__device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
 
 // Value of a term reference: slot r (r >= 0) or constant -(r+1).
 template <int NS>
@@ -414,9 +416,12 @@ __device__ __forceinline__ uint64_t thread_chunk(ThreadLds& W, const DevGraph& G
                                                  uint32_t* hCount, uint32_t hCap) {
     const int lane = lane_id();
     const Plan& P = W.plan;
-    const int nlev = P.nlev;
-    const bool pairs = K == kPairs && P.pairLevel >= 0;
-    const int pl = P.pairLevel;
+    // plan-wide fields as wave-uniform values (scalar registers)
+    const int nlev = uniform(P.nlev);
+    const int pl = uniform(P.pairLevel);
+    const int hoRef = uniform(P.hoRef);
+    const int budget0 = uniform(P.budget);
+    const bool pairs = K == kPairs && pl >= 0;
     const uint64_t below = (1ull << lane) - 1ull;
     int v[NS];
     RegSet set;
@@ -436,7 +441,7 @@ __device__ __forceinline__ uint64_t thread_chunk(ThreadLds& W, const DevGraph& G
                 if (active) {
                     W.frame[0][lane] = enter(W, P, 0, G, v);
                     d = 0;
-                    budget = P.budget;
+                    budget = budget0;
                     set.n = 0;
                 }
             }
@@ -512,7 +517,7 @@ __device__ __forceinline__ uint64_t thread_chunk(ThreadLds& W, const DevGraph& G
                             stop = true;
                             continue;
                         }
-                        const int ho = term_value(v, P.hoRef);
+                        const int ho = term_value(v, hoRef);
                         if (d == pl && set.seen(ho)) continue;
                         if (!set.insert(ho)) {
                             heavyNow = true;
@@ -533,11 +538,11 @@ __device__ __forceinline__ uint64_t thread_chunk(ThreadLds& W, const DevGraph& G
                     const int x = G.col[fr.i];
                     bool ok = bind_entry_x(P.lev[d], G, R, v, fr.i, x);
                     if (ok && level_kind(P.lev[d].flags) == kIsect) ok = isect_next(W, d, G, x);
-                    if (ok && pairs && d == pl) ok = !set.seen(term_value(v, P.hoRef));
+                    if (ok && pairs && d == pl) ok = !set.seen(term_value(v, hoRef));
                     if (ok) {
                         if (d + 1 == nlev) {
                             if (pairs) {
-                                if (!set.insert(term_value(v, P.hoRef))) {
+                                if (!set.insert(term_value(v, hoRef))) {
                                     heavyNow = true;
                                     active = false;
                                 } else {
@@ -568,7 +573,6 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
 __device__ __forceinline__ uint32_t bcast(uint32_t x) { return (uint32_t)__shfl((int)x, 0, kWave); }
 __device__ __forceinline__ uint32_t ld_rlx(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -739,10 +743,12 @@ __device__ __forceinline__ uint64_t wave_run(WaveLds& W, const DevGraph& G, cons
                                              unsigned long long (&stats)[kWaveStats]) {
     const int lane = lane_id();
     const Plan& P = W.plan;
-    const int nlev = P.nlev;
+    const int nlev = uniform(P.nlev);
     const int last = nlev - 1;
-    const bool pairs = K == kPairs && P.pairLevel >= 0;
-    const int pl = P.pairLevel;
+    const int pl = uniform(P.pairLevel);
+    const int hoRef = uniform(P.hoRef);
+    const bool hoByEntry = uniform(P.hoByEntry) != 0;
+    const bool pairs = K == kPairs && pl >= 0;
     const uint64_t below = (1ull << lane) - 1ull;
     int v[NS];
 #pragma unroll
@@ -767,7 +773,7 @@ __device__ __forceinline__ uint64_t wave_run(WaveLds& W, const DevGraph& G, cons
             bool drop = false;
             int hru = -1;              // rank of a head object bound above the pool
             if (pairs && pl < plv) {   // every entry shares a head object bound above the pool
-                const int ref = P.hoRef;
+                const int ref = hoRef;
                 const int hov = ref >= 0 ? W.poolV[ref] : -(ref + 1);
                 hru = uniform(dd.rank(uniform(hov)));
                 drop = dd.seen_r(hru);
@@ -794,7 +800,7 @@ __device__ __forceinline__ uint64_t wave_run(WaveLds& W, const DevGraph& G, cons
                 } else if (plv == last) {
                     ++nLeafDraw;
                     next = pn + n * kLeafUnroll < pe ? pn + n * kLeafUnroll : pe;
-                    const bool byEntry = pairs && pl == last && D != kDedupSort && P.hoByEntry;
+                    const bool byEntry = pairs && pl == last && D != kDedupSort && hoByEntry;
                     int xs[kLeafUnroll], rs[kLeafUnroll];
 #pragma unroll
                     for (int k = 0; k < kLeafUnroll; ++k) {
@@ -813,7 +819,7 @@ __device__ __forceinline__ uint64_t wave_run(WaveLds& W, const DevGraph& G, cons
                             if (!pairs) {
                                 found = true;
                             } else {
-                                hos[k] = term_value(v, P.hoRef);
+                                hos[k] = term_value(v, hoRef);
                                 if (pl == last && !byEntry) rs[k] = dd.rank_at(i, hos[k]);
                             }
                         }
@@ -845,7 +851,7 @@ __device__ __forceinline__ uint64_t wave_run(WaveLds& W, const DevGraph& G, cons
                         bool ok = bind_entry_x(P.lev[plv], G, R, v, i, x);
                         if (ok && level_kind(P.lev[plv].flags) == kIsect) ok = isect_find(W, plv, P.lev[plv], G, v, x);
                         if (ok && pairs && plv == pl) {
-                            hr = dd.rank_at(i, term_value(v, P.hoRef));
+                            hr = dd.rank_at(i, term_value(v, hoRef));
                             ok = !dd.seen_r(hr);
                         } else if (pairs && pl < plv) {
                             hr = hru;
@@ -899,7 +905,7 @@ __device__ __forceinline__ uint64_t wave_run(WaveLds& W, const DevGraph& G, cons
                     // last level: up to kLeafBatch candidates per step, loaded together
                     const int take = fr.end - fr.i < kLeafBatch ? fr.end - fr.i : kLeafBatch;
                     W.frame[d][lane].i = fr.i + take;
-                    const bool byEntry = pairs && d == pl && D != kDedupSort && P.hoByEntry;
+                    const bool byEntry = pairs && d == pl && D != kDedupSort && hoByEntry;
                     int xs[kLeafBatch], rs[kLeafBatch];
 #pragma unroll
                     for (int k = 0; k < kLeafBatch; ++k) {
@@ -912,7 +918,7 @@ __device__ __forceinline__ uint64_t wave_run(WaveLds& W, const DevGraph& G, cons
                     for (int k = 0; k < kLeafBatch; ++k) {
                         hos[k] = -1;
                         if (k < take && bind_entry_x(P.lev[d], G, R, v, fr.i + k, xs[k])) {
-                            hos[k] = pairs ? term_value(v, P.hoRef) : 0;
+                            hos[k] = pairs ? term_value(v, hoRef) : 0;
                             if (pairs && d == pl && !byEntry) rs[k] = dd.rank_at(fr.i + k, hos[k]);
                             any = true;
                         }
@@ -942,14 +948,14 @@ __device__ __forceinline__ uint64_t wave_run(WaveLds& W, const DevGraph& G, cons
                     bool ok = bind_entry_x(P.lev[d], G, R, v, fr.i, x);
                     if (ok && level_kind(P.lev[d].flags) == kIsect) ok = isect_next(W, d, G, x);
                     if (ok && pairs && d == pl) {
-                        hr = dd.rank_at(fr.i, term_value(v, P.hoRef));
+                        hr = dd.rank_at(fr.i, term_value(v, hoRef));
                         ok = !dd.seen_r(hr);
                     }
                     if (ok) {
                         if (d == last) {
                             if (pairs) {
                                 mine += dd.insert_r(hr);
-                                emit = term_value(v, P.hoRef);
+                                emit = term_value(v, hoRef);
                                 if (pl >= base) d = pl;   // head object settled: next candidate at the pair level
                                 else busy = false;
                             } else {
