@@ -88,6 +88,30 @@ def pmc_traffic(workload, kernel):
         return None, None, None
 
 
+def snapshot_leg(eng, words, flags, res, load_triples_s):
+    """Graph load times: from term-id triples (host sort + index build +
+    upload) against a binary snapshot of the same graph (map + checks +
+    upload); the snapshot's context must give the same counts."""
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "g.snap")
+        t = time.time()
+        eng.save_snapshot(p)
+        save_s = time.time() - t
+        size = os.path.getsize(p)
+        t = time.time()
+        e2 = L.Engine.from_snapshot(p)
+        snap_s = time.time() - t
+        try:
+            r = e2.evaluate(words, flags)
+            same = bool(np.array_equal(r["support"], res["support"]) and np.array_equal(r["bodyPairs"], res["bodyPairs"]))
+        finally:
+            e2.close()
+    log("load: from triples %.2fs, snapshot save %.2fs (%.1f MB), from snapshot %.2fs, same counts %s" % (
+        load_triples_s, save_s, size / 1e6, snap_s, same))
+    return {"from_triples": round(load_triples_s, 3), "snapshot_save": round(save_s, 3),
+            "from_snapshot": round(snap_s, 3), "snapshot_mb": round(size / 1e6, 1), "same_counts": same}
+
+
 def describe(wl):
     g = wl.graph
     nb = [len(r[1]) for r in wl.rules]
@@ -150,6 +174,7 @@ def main():
     ap.add_argument("--compare-dedup", action="store_true",
                     help="also time the batch under each body-pair dedup strategy (default for dense_conf)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-snapshot", action="store_true", help="skip the graph-load leg (triples vs snapshot)")
     ap.add_argument("--no-per-rule", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tune", default="",
@@ -185,7 +210,8 @@ def main():
     words = L.rule_words_for_graph(rules, g)
     t = time.time()
     eng = L.Engine.from_graph(g)
-    log("graph resident in HBM (%.2fs)" % (time.time() - t))
+    load_triples_s = time.time() - t
+    log("graph resident in HBM (%.2fs)" % load_triples_s)
 
     flags = 0 if args.gated else L.UNGATED
     if args.tune:
@@ -277,6 +303,9 @@ def main():
                      "traffic_source": src, "l2_hit": l2hit, "kernel": names[dom], "kernel_ms": round(float(kms[dom]), 4),
                      "algorithmic_bytes": bytes_phase, "kernel_ms_all": [round(float(x), 4) for x in kms]},
     }
+
+    if not args.no_snapshot:
+        out["load_s"] = snapshot_leg(eng, words, flags, res, load_triples_s)
 
     if args.compare_dedup or args.workload == "dense_conf":
         # body-pair dedup strategies on the same batch (eager passes, best of 3)

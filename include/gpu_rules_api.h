@@ -99,6 +99,19 @@ typedef struct {
 void* gpu_load_graph_triples(const int32_t* s, const int32_t* p, const int32_t* o, int64_t n,
                              int64_t nTerms);
 
+/* Binary graph snapshots. No counterpart in the reference, whose loader
+ * parses the Turtle file and sorts the triples on every load
+ * (GPU_03/gpu_rules_lib.cu:246-325, GPU_03/RdfIndexes.cpp:146-415).
+ * gpu_save_snapshot writes the built graph of a context or host handle --
+ * the predicate-grouped sorted triples in both directions, the term
+ * dictionary in id order and the Turtle prefix lines -- to `path`; 0 or -1.
+ * gpu_load_snapshot maps such a file, checks it (sizes, term ids, sort
+ * order, both directions holding the same triples) and returns a context
+ * exactly like gpu_load_graph on the original graph (same term ids, same
+ * rule resolution, same results), or NULL. */
+int gpu_save_snapshot(void* handle, const char* path);
+void* gpu_load_snapshot(const char* path);
+
 /* Evaluates n structured rules in one batched pass (persistent thread-mode
  * kernel over all rules' seeds, wave-mode kernel over escalated seeds,
  * captured in a HIP graph and replayed while the batch shape is unchanged).
@@ -221,6 +234,8 @@ int64_t gpu_monotonic_ns(void);
 void* gpu_host_load_turtle(const char* ttlPath);
 /* Host-only graph handle from term-id triples (as gpu_load_graph_triples). */
 void* gpu_host_load_triples(const int32_t* s, const int32_t* p, const int32_t* o, int64_t n, int64_t nTerms);
+/* Host-only graph handle from a snapshot (as gpu_load_snapshot). */
+void* gpu_host_load_snapshot(const char* path);
 void gpu_host_free(void* host);
 /* Term count, predicate count and distinct triple count of a graph handle
  * (host handle or full context). */

@@ -27,7 +27,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-pthread", "-Wall", "-Wno-unused-result
 HIPFLAGS = ["-x", "hip", "--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-fno-gpu-rdc"]
 
-SOURCES = ["graph_host.cpp", "rules_host.cpp", "planner.cpp", "traffic.cpp", "engine.cpp", "serve.cpp",
+SOURCES = ["graph_host.cpp", "rules_host.cpp", "planner.cpp", "traffic.cpp", "snapshot.cpp", "engine.cpp", "serve.cpp",
            "kernels.hip", "search_s4.hip", "search_s8.hip", "search_s16.hip", "sort_unique.hip"]
 
 

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "engine_internal.hpp"
+#include "snapshot.hpp"
 #include "traffic.hpp"
 
 namespace gr {
@@ -625,6 +626,38 @@ void* gpu_load_graph_triples(const int32_t* s, const int32_t* p, const int32_t* 
     }
 }
 
+void* gpu_load_snapshot(const char* path) {
+    if (!path) return nullptr;
+    try {
+        HostGraph g;
+        std::string err;
+        if (!load_snapshot(path, g, err)) {
+            std::fprintf(stderr, "[gpu_rules] failed to load snapshot: %s (%s)\n", path, err.c_str());
+            return nullptr;
+        }
+        return new_context_from(std::move(g));
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "[gpu_rules] gpu_load_snapshot error: %s\n", e.what());
+        return nullptr;
+    }
+}
+
+int gpu_save_snapshot(void* handle, const char* path) {
+    const HostGraph* g = as_graph(handle);
+    if (!g || !path) return -1;
+    try {
+        std::string err;
+        if (!save_snapshot(*g, path, err)) {
+            std::fprintf(stderr, "[gpu_rules] failed to save snapshot: %s\n", err.c_str());
+            return -1;
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "[gpu_rules] gpu_save_snapshot error: %s\n", e.what());
+        return -1;
+    }
+}
+
 void gpu_free_graph(void* ctx) {
     Context* C = as_ctx(ctx);
     if (!C) return;
@@ -1047,6 +1080,25 @@ void* gpu_host_load_triples(const int32_t* s, const int32_t* p, const int32_t* o
         return H;
     } catch (const std::exception& e) {
         std::fprintf(stderr, "[gpu_rules] gpu_host_load_triples error: %s\n", e.what());
+        return nullptr;
+    }
+}
+
+void* gpu_host_load_snapshot(const char* path) {
+    if (!path) return nullptr;
+    try {
+        auto* H = new HostHandle();
+        H->magic = kMagicHost;
+        std::string err;
+        if (!load_snapshot(path, H->g, err)) {
+            std::fprintf(stderr, "[gpu_rules] failed to load snapshot: %s (%s)\n", path, err.c_str());
+            delete H;
+            return nullptr;
+        }
+        H->tr.reset(new TermResolver(H->g));
+        return H;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "[gpu_rules] gpu_host_load_snapshot error: %s\n", e.what());
         return nullptr;
     }
 }

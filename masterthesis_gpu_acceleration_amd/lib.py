@@ -74,6 +74,9 @@ _SIGS = {
     "gpu_host_load_turtle": (ctypes.c_void_p, [ctypes.c_char_p]),
     "gpu_host_load_triples": (ctypes.c_void_p, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                                 ctypes.c_int64]),
+    "gpu_host_load_snapshot": (ctypes.c_void_p, [ctypes.c_char_p]),
+    "gpu_save_snapshot": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
+    "gpu_load_snapshot": (ctypes.c_void_p, [ctypes.c_char_p]),
     "gpu_host_free": (None, [ctypes.c_void_p]),
     "gpu_host_graph_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64),
                                            ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64)]),
@@ -157,6 +160,34 @@ class HostGraph:
         if not self.h:
             raise RuntimeError("gpu_host_load_triples failed")
 
+    @classmethod
+    def _wrap(cls, h, what: str):
+        if not h:
+            raise RuntimeError(what + " failed")
+        obj = cls.__new__(cls)
+        obj.lib = load_library()
+        obj.h = h
+        return obj
+
+    @classmethod
+    def from_turtle(cls, path: str):
+        return cls._wrap(load_library().gpu_host_load_turtle(path.encode()), "gpu_host_load_turtle")
+
+    @classmethod
+    def from_snapshot(cls, path: str):
+        return cls._wrap(load_library().gpu_host_load_snapshot(path.encode()), "gpu_host_load_snapshot")
+
+    def save_snapshot(self, path: str):
+        if self.lib.gpu_save_snapshot(self.h, path.encode()) != 0:
+            raise RuntimeError("gpu_save_snapshot failed for " + path)
+
+    def info(self):
+        """(terms, predicates, triples)."""
+        nt, np_, ntr = ctypes.c_int64(), ctypes.c_int32(), ctypes.c_int64()
+        if self.lib.gpu_host_graph_info(self.h, ctypes.byref(nt), ctypes.byref(np_), ctypes.byref(ntr)) != 0:
+            raise RuntimeError("gpu_host_graph_info failed")
+        return nt.value, np_.value, ntr.value
+
     def traffic(self, rule_words: np.ndarray) -> np.ndarray:
         """Per rule: [support bytes, pair bytes, support nodes, pair nodes]."""
         w = np.ascontiguousarray(rule_words, np.int32)
@@ -192,6 +223,18 @@ class Engine:
         if not ctx:
             raise RuntimeError("gpu_load_graph failed for " + path)
         return cls(ctx, lib)
+
+    @classmethod
+    def from_snapshot(cls, path: str):
+        lib = load_library()
+        ctx = lib.gpu_load_snapshot(path.encode())
+        if not ctx:
+            raise RuntimeError("gpu_load_snapshot failed for " + path)
+        return cls(ctx, lib)
+
+    def save_snapshot(self, path: str):
+        if self.lib.gpu_save_snapshot(self.ctx, path.encode()) != 0:
+            raise RuntimeError("gpu_save_snapshot failed for " + path)
 
     @classmethod
     def from_triples(cls, s, p, o, n_terms: int):

@@ -270,3 +270,54 @@ def test_giant_seeds_random_rules(seed):
                 assert res[i]["bodySize"] == exp[i][4], tag
     finally:
         eng.close()
+
+
+@pytest.mark.parametrize("name", ["tiny_4", "dense_conf_300", "dbpedia_shaped_5k"])
+def test_snapshot_context_matches(name):
+    """A context loaded from a snapshot of a term-id graph gives the golden
+    counts, identical to the context it was saved from."""
+    say("snapshot", name)
+    fx, g, rules = goldens.load(name)
+    words = L.rule_words_for_graph(rules, g)
+    eng = L.Engine.from_graph(g)
+    with tempfile.TemporaryDirectory() as d:
+        snap = os.path.join(d, "g.snap")
+        try:
+            ref = eng.evaluate(words, L.UNGATED).copy()
+            eng.save_snapshot(snap)
+        finally:
+            eng.close()
+        eng2 = L.Engine.from_snapshot(snap)
+        try:
+            res = eng2.evaluate(words, L.UNGATED)
+            check(fx, rules, res, False, name + "/snapshot")
+            assert np.array_equal(res, ref)
+        finally:
+            eng2.close()
+
+
+def test_snapshot_text_abi():
+    """Turtle graph -> context -> snapshot -> context: the reference entry
+    points resolve rule text to the same terms and give the same results."""
+    say("snapshot-text-abi")
+    fx, g, rules = goldens.load("small_paths_2k")
+    api = L.GpuRulesLib()
+    with tempfile.TemporaryDirectory() as d:
+        ttl = os.path.join(d, "g.ttl")
+        snap = os.path.join(d, "g.snap")
+        W.write_ttl(g, ttl)
+        ctx = api.load_graph(ttl)
+        try:
+            ref = [api.evaluate_rule(ctx, t) for t in fx["rules_text"]]
+            assert api.lib.gpu_save_snapshot(ctx, snap.encode()) == 0
+        finally:
+            api.free_graph(ctx)
+        ctx2 = api.lib.gpu_load_snapshot(snap.encode())
+        assert ctx2
+        try:
+            for i, t in enumerate(fx["rules_text"]):
+                out = api.evaluate_rule(ctx2, t)
+                assert (out.support, out.headSize, out.headSupport, out.bodySize) == (
+                    ref[i].support, ref[i].headSize, ref[i].headSupport, ref[i].bodySize), i
+        finally:
+            api.free_graph(ctx2)
