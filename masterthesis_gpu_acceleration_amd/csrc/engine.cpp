@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "engine_internal.hpp"
+#include "graph_device.hpp"
 #include "snapshot.hpp"
 #include "traffic.hpp"
 
@@ -543,6 +544,37 @@ TermResolver* as_resolver(void* h) {
     return nullptr;
 }
 
+// The adjacency arrays of g from its triples: with the GPU's sort when
+// device_build_wanted (falling back to the host build if it cannot run),
+// else on the host. Both give the same arrays.
+void build_arrays(HostGraph& g, std::vector<int32_t>& S, std::vector<int32_t>& P, std::vector<int32_t>& O,
+                  int64_t nTerms) {
+    if (device_build_wanted((int64_t)S.size())) {
+        HostGraph d;
+        std::string err;
+        if (build_graph_device(d, S.data(), P.data(), O.data(), (int64_t)S.size(), nTerms, err)) {
+            g.nE = d.nE;
+            g.predTerm = std::move(d.predTerm);
+            g.predOf = std::move(d.predOf);
+            g.predOff = std::move(d.predOff);
+            g.spoKey = std::move(d.spoKey);
+            g.spoVal = std::move(d.spoVal);
+            g.posKey = std::move(d.posKey);
+            g.posVal = std::move(d.posVal);
+            g.keysS = std::move(d.keysS);
+            g.keysSOff = std::move(d.keysSOff);
+            g.keysO = std::move(d.keysO);
+            g.keysOOff = std::move(d.keysOOff);
+            std::vector<int32_t>().swap(S);
+            std::vector<int32_t>().swap(P);
+            std::vector<int32_t>().swap(O);
+            return;
+        }
+        std::fprintf(stderr, "[gpu_rules] device build not used (%s); building on the host\n", err.c_str());
+    }
+    build_graph(g, S, P, O, nTerms);
+}
+
 Context* new_context_from(HostGraph&& g) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
@@ -595,10 +627,12 @@ void* gpu_load_graph(const char* ttlPath) {
     try {
         HostGraph g;
         std::string err;
-        if (!load_turtle(ttlPath, g, err)) {
+        std::vector<int32_t> S, P, O;
+        if (!parse_turtle(ttlPath, g, S, P, O, err)) {
             std::fprintf(stderr, "[gpu_rules] failed to parse TTL: %s (%s)\n", ttlPath, err.c_str());
             return nullptr;
         }
+        build_arrays(g, S, P, O, (int64_t)g.terms.size());
         return new_context_from(std::move(g));
     } catch (const std::exception& e) {
         std::fprintf(stderr, "[gpu_rules] gpu_load_graph error: %s\n", e.what());
@@ -618,7 +652,7 @@ void* gpu_load_graph_triples(const int32_t* s, const int32_t* p, const int32_t* 
         }
         HostGraph g;
         std::vector<int32_t> S(s, s + n), P(p, p + n), O(o, o + n);
-        build_graph(g, S, P, O, nTerms);
+        build_arrays(g, S, P, O, nTerms);
         return new_context_from(std::move(g));
     } catch (const std::exception& e) {
         std::fprintf(stderr, "[gpu_rules] gpu_load_graph_triples error: %s\n", e.what());

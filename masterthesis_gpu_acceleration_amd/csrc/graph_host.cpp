@@ -378,16 +378,22 @@ std::unordered_map<std::string, std::string> scan_prefix_lines(const std::string
     return out;
 }
 
-bool load_turtle(const std::string& path, HostGraph& g, std::string& err) {
+bool parse_turtle(const std::string& path, HostGraph& g, std::vector<int32_t>& S, std::vector<int32_t>& P,
+                  std::vector<int32_t>& O, std::string& err) {
     std::ifstream f(path, std::ios::binary);
     if (!f) {
         err = "cannot open " + path;
         return false;
     }
     std::string text((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
-    std::vector<int32_t> S, P, O;
     TurtleReader(text, g, S, P, O).run();
     g.prefixes = scan_prefix_lines(text);
+    return true;
+}
+
+bool load_turtle(const std::string& path, HostGraph& g, std::string& err) {
+    std::vector<int32_t> S, P, O;
+    if (!parse_turtle(path, g, S, P, O, err)) return false;
     build_graph(g, S, P, O, (int64_t)g.terms.size());
     return true;
 }

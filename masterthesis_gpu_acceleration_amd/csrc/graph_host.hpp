@@ -63,6 +63,11 @@ void build_graph(HostGraph& g, std::vector<int32_t>& s, std::vector<int32_t>& p,
 // sets err if the file cannot be read.
 bool load_turtle(const std::string& path, HostGraph& g, std::string& err);
 
+// The parsing half of load_turtle: terms and prefixes into g, the triples'
+// term ids into S, P, O (the arrays are left to build).
+bool parse_turtle(const std::string& path, HostGraph& g, std::vector<int32_t>& S, std::vector<int32_t>& P,
+                  std::vector<int32_t>& O, std::string& err);
+
 // Prefix declarations as the rule parser sees them: line-based scan for
 // "@prefix name: <iri>" / "PREFIX name: <iri>" (GPU_03/RuleParser.cpp:51-96).
 std::unordered_map<std::string, std::string> scan_prefix_lines(const std::string& text);

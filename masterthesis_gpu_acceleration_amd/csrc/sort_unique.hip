@@ -206,19 +206,25 @@ __global__ void __launch_bounds__(kSortBlock) k_unique_count(UniqueArgs a) {
 
 }  // namespace
 
-void launch_sort_unique(const SortJob& j, hipStream_t s) {
-    const int passes = sort_passes(j.bits);
-    const int blocks = j.blocks > 0 ? j.blocks : 1;
+int launch_radix_sort(unsigned long long* const keys[2], const uint32_t* count, uint32_t cap, uint32_t* hist,
+                      int bits, int blocks, hipStream_t s) {
+    const int passes = sort_passes(bits);
+    if (blocks < 1) blocks = 1;
     int cur = 0;
     for (int p = 0; p < passes; ++p) {
-        RadixPass a{j.keys[cur], j.keys[cur ^ 1], j.count, j.cap, j.maxTiles, j.hist, p * kRadixBits};
+        RadixPass a{keys[cur], keys[cur ^ 1], count, cap, sort_max_tiles(cap), hist, p * kRadixBits};
         hipLaunchKernelGGL(k_radix_hist, dim3(blocks), dim3(kSortBlock), 0, s, a);
         hipLaunchKernelGGL(k_radix_scan, dim3(kRadix), dim3(kSortBlock), 0, s, a);
         hipLaunchKernelGGL(k_radix_scatter, dim3(blocks), dim3(kSortBlock), 0, s, a);
         cur ^= 1;
     }
+    return cur;
+}
+
+void launch_sort_unique(const SortJob& j, hipStream_t s) {
+    const int cur = launch_radix_sort(j.keys, j.count, j.cap, j.hist, j.bits, j.blocks, s);
     UniqueArgs u{j.keys[cur], j.count, j.cap, j.hoBits, j.heavy, j.plans, j.counts};
-    hipLaunchKernelGGL(k_unique_count, dim3(blocks), dim3(kSortBlock), 0, s, u);
+    hipLaunchKernelGGL(k_unique_count, dim3(j.blocks > 0 ? j.blocks : 1), dim3(kSortBlock), 0, s, u);
 }
 
 }  // namespace gr

@@ -58,4 +58,11 @@ inline int bits_for(uint64_t n) {
 // in a HIP graph before the count is known.
 void launch_sort_unique(const SortJob& j, hipStream_t s);
 
+// Just the LSD passes: sorts the first min(*count, cap) keys of keys[0] by
+// their low `bits` bits (stable), ping-ponging through keys[1]; hist holds
+// sort_hist_words(cap) words. Returns the index of the buffer holding the
+// sorted keys (passes % 2).
+int launch_radix_sort(unsigned long long* const keys[2], const uint32_t* count, uint32_t cap, uint32_t* hist,
+                      int bits, int blocks, hipStream_t s);
+
 }  // namespace gr
