@@ -102,6 +102,13 @@ bool Context::upload() {
     return ensure_dedup();
 }
 
+int Context::waveBlocksFor(int slots) const {
+    if (pairsDedup() != kDedupLds) return waveBlocks();
+    const size_t blockLds = (wave_static_lds(slots) + (size_t)rowWords * 4 + 511) / 512 * 512;
+    const int perCU = std::max<int>(1, (int)(ldsPerCU / blockLds));
+    return std::max(1, numCUs * std::min(waveWavesPerCU, perCU));
+}
+
 bool Context::ensure_dedup() {
     // wave-mode bitmaps index head objects by rank among the keys of one
     // (direction, predicate): as many bits as the largest such key set
@@ -203,7 +210,7 @@ bool run_one(Context& C, const Plan& P, const ConflictRefs& R, const Plan& PW, c
         if (sort) sink = SortSink{C.sortb.keys[0].as<unsigned long long>(), ctl + 2, C.sortb.cap, hoBits};
         WaveArgs wa{nullptr, conf ? d_refs + 1 : nullptr, C.heavy1.as<HeavyItem>(), ctl + 0, C.heavy1Cap, 0, ctl + 1,
                     d_count, C.dedup.as<uint32_t>(), C.rowWords, sink};
-        launch_wave_one(PW.kind, slots, mode, PW, wa, C.dg, C.waveBlocks(), C.st);
+        launch_wave_one(PW.kind, slots, mode, PW, wa, C.dg, C.waveBlocksFor(slots), C.st);
         if (sort) {
             SortJob j{{C.sortb.keys[0].as<unsigned long long>(), C.sortb.keys[1].as<unsigned long long>()},
                       ctl + 2, C.sortb.cap, sort_max_tiles(C.sortb.cap), C.sortb.hist.as<uint32_t>(),
@@ -385,9 +392,10 @@ void batch_enqueue(Context& C, Batch& B, const ExecTarget& t) {
                         c + 1, B.heavyCap[ph], 0, c + 2, counts, t.dedup, t.rowWords, sink};
             const int mode = ph == kPairs ? t.pairsDedup : kDedupRows;
             giant_prepare(*t.giants, t.giantAfter, wa.giant, st, ph);
-            launch_wave_batch(ph, B.slots[ph], mode, wa, C.dg, t.waveBlocks, st);
+            const int wb = t.pairsDedup == kDedupLds ? C.waveBlocksFor(B.slots[ph]) : t.waveBlocks;
+            launch_wave_batch(ph, B.slots[ph], mode, wa, C.dg, wb, st);
             if (t.giantAfter > 0) {
-                launch_giants(ph, B.slots[ph], mode, wa, C.dg, t.waveBlocks, st);
+                launch_giants(ph, B.slots[ph], mode, wa, C.dg, wb, st);
                 (void)hipMemcpyAsync(B.hGiant + ph, wa.giant.count, 4, hipMemcpyDeviceToHost, st);
             }
             (void)hipMemcpyAsync(B.hWaveStats + ph * kWaveStats, wa.giant.stats, kWaveStats * 8,

@@ -213,6 +213,10 @@ struct Context : Handle {
     int64_t lastEmitted = 0;
 
     int waveBlocks() const { return std::max(1, waveSlots); }   // one wave per wave-mode block
+    // Wave-mode blocks of a launch whose plans need `slots` (plan_slots):
+    // with LDS bitmaps, as many as fit a CU's LDS for that slot class
+    // (smaller classes keep fewer frames), else waveBlocks().
+    int waveBlocksFor(int slots) const;
     int threadBlocks() const { return std::max(1, numCUs * threadWavesPerCU / 4); }
     int sortBlocks() const { return std::max(1, numCUs * 4); }
     bool sortActive() const { return dedupMode == 2 || (dedupMode == 0 && !rowsFit && !ldsFits); }

@@ -69,11 +69,12 @@ void launch_giants(int kind, int slots, int dedup, const WaveArgs& a, const DevG
     }
 }
 
-// The largest static LDS of the three classes' kDedupLds wave kernels (they
-// share one layout, so they agree).
-size_t wave_static_lds() {
-    const size_t a = SlotKernels<4>::lds_static(), b = SlotKernels<8>::lds_static(), c = SlotKernels<16>::lds_static();
-    return a > b ? (a > c ? a : c) : (b > c ? b : c);
+size_t wave_static_lds(int slots) {
+    switch (slot_class(slots)) {
+        case 4: return SlotKernels<4>::lds_static();
+        case 8: return SlotKernels<8>::lds_static();
+        default: return SlotKernels<16>::lds_static();
+    }
 }
 
 void prepare_wave_kernels(size_t dynLds) {

@@ -117,9 +117,10 @@ void launch_wave_batch(int kind, int slots, int dedup, const WaveArgs& a, const 
                        hipStream_t s);
 void launch_wave_one(int kind, int slots, int dedup, const Plan& P, const WaveArgs& a, const DevGraph& G, int blocks,
                      hipStream_t s);
-// Static LDS of a kDedupLds wave-mode block (bytes, from the runtime), and
-// the dynamic-LDS limit to request for the kDedupLds kernels.
-size_t wave_static_lds();
+// Static LDS of a kDedupLds wave-mode block of the slot class for `slots`
+// (bytes, from the runtime; 16: the largest class), and the dynamic-LDS
+// limit to request for the kDedupLds kernels.
+size_t wave_static_lds(int slots = 16);
 void prepare_wave_kernels(size_t dynLds);
 // The giant-seed pass that follows a wave-mode batch launch: expansion of
 // the listed giants, then their sub-items (same `a` as the wave launch).

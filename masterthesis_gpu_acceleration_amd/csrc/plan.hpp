@@ -93,10 +93,12 @@ struct alignas(16) ConflictRefs {
     int32_t hs, ho, pad0, pad1;
 };
 
-// Variable slots a plan and its conflict refs use: one past the largest slot
-// any term ref names (constants are negative refs).
+// The slot class a plan needs: its variable slots (one past the largest
+// slot any term ref of the plan or its conflict refs names; constants are
+// negative refs) and its levels, since a class keeps frames for as many
+// levels as it has slots (at most kMaxBody).
 inline int plan_slots(const Plan& P, const ConflictRefs& R) {
-    int n = 1;
+    int n = P.nlev > 1 ? P.nlev : 1;
     auto see = [&](int32_t r) { if (r >= 0 && r + 1 > n) n = r + 1; };
     see(P.hsRef);
     see(P.hoRef);
@@ -114,7 +116,7 @@ inline int plan_slots(const Plan& P, const ConflictRefs& R) {
     return n;
 }
 
-// The slot class whose kernels run a launch of plans with `slots` slots.
+// The slot class whose kernels run a launch of plans needing `slots` (plan_slots).
 constexpr int slot_class(int slots) { return slots <= 4 ? 4 : (slots <= 8 ? 8 : 16); }
 
 // One unit of thread-mode work: up to kChunkSeeds consecutive seeds of one plan.

@@ -60,19 +60,26 @@ struct Frame {
 
 // Per-wave LDS of thread mode: the plan being evaluated and one frame per
 // (level, lane).
-struct ThreadLds {
+// Levels the kernels of slot class NS keep frames for: as many as slots, at
+// most kMaxBody (the host sends a plan to a class that covers both its
+// variables and its levels).
+constexpr int levels_of(int ns) { return ns < kMaxBody ? ns : kMaxBody; }
+
+template <int NL>
+struct ThreadLdsT {
     Plan plan;
-    Frame frame[kMaxBody][kWave];
+    Frame frame[NL][kWave];
     // ISECT level L: cursor into row 2 and its end, at cur[L >> 1] (an ISECT
     // level is always followed by its CHECK, so no two share an index)
-    Frame cur[kMaxBody / 2][kWave];
+    Frame cur[NL / 2][kWave];
 };
 
 // Wave mode adds the seed's shared pool -- a frame at level poolLevel whose
 // entries [poolNext, poolEnd) any idle lane may take, with the bindings
 // poolV that hold above that level -- and the number of dedup-row words
 // touched so far (HBM rows; the list itself is the kernel's `cache` array).
-struct WaveLds : ThreadLds {
+template <int NL>
+struct WaveLdsT : ThreadLdsT<NL> {
     int poolV[kMaxSlots];
     int poolLevel;
     int poolNext;
@@ -82,7 +89,7 @@ struct WaveLds : ThreadLds {
     uint32_t pad[3];
 };
 
-// Per-wave LDS array beside WaveLds: the sort path's cache of emitted head
+// Per-wave LDS array beside WaveLdsT: the sort path's cache of emitted head
 // objects, the HBM-row path's list of touched row words; the LDS-bitmap
 // path needs neither (it clears its whole bitmap per seed).
 template <int D>
@@ -193,7 +200,7 @@ __device__ __forceinline__ Frame isect_row(const Level& lv, const DevGraph& G, c
 // lane's row-2 cursor (no entry can match if row 2 is empty); the CHECK
 // after an ISECT level is the entry the intersection matched.
 template <int NS>
-__device__ __forceinline__ Frame enter(ThreadLds& W, const Plan& P, int L, const DevGraph& G,
+__device__ __forceinline__ Frame enter(ThreadLdsT<levels_of(NS)>& W, const Plan& P, int L, const DevGraph& G,
                                        const int (&v)[NS]) {
     const Level& lv = P.lev[L];
     const int kind = level_kind(lv.flags);
@@ -211,7 +218,8 @@ __device__ __forceinline__ Frame enter(ThreadLds& W, const Plan& P, int L, const
 // Is x in row 2 of ISECT level L? Row 1 is enumerated in increasing x, so
 // the lane's cursor only moves forward: gallop from it, then binary search.
 // Leaves the cursor at the first entry >= x (the match, if any).
-__device__ __forceinline__ bool isect_next(ThreadLds& W, int L, const DevGraph& G, int x) {
+template <int NL>
+__device__ __forceinline__ bool isect_next(ThreadLdsT<NL>& W, int L, const DevGraph& G, int x) {
     const int lane = lane_id();
     int j = W.cur[L >> 1][lane].i;
     const int e = W.cur[L >> 1][lane].end;
@@ -249,7 +257,7 @@ __device__ __forceinline__ bool isect_next(ThreadLds& W, int L, const DevGraph& 
 // The same test for one entry drawn from a shared pool (no cursor to reuse):
 // binary search of row 2; on a match the lane's cursor points at it.
 template <int NS>
-__device__ __forceinline__ bool isect_find(ThreadLds& W, int L, const Level& lv, const DevGraph& G,
+__device__ __forceinline__ bool isect_find(ThreadLdsT<levels_of(NS)>& W, int L, const Level& lv, const DevGraph& G,
                                            const int (&v)[NS], int x) {
     const Frame r2 = isect_row(lv, G, v);
     const int at = find_in_row(G.col, r2.i, r2.end, x);
@@ -329,7 +337,8 @@ __device__ __forceinline__ bool bind_seed(const Plan& P, const DevGraph& G, int 
     return true;
 }
 
-__device__ __forceinline__ void copy_plan(ThreadLds& W, const Plan* src) {
+template <int NL>
+__device__ __forceinline__ void copy_plan(ThreadLdsT<NL>& W, const Plan* src) {
     const int32_t* s = reinterpret_cast<const int32_t*>(src);
     int32_t* d = reinterpret_cast<int32_t*>(&W.plan);
     for (int w = lane_id(); w < kPlanWords; w += kWave) d[w] = s[w];
@@ -411,7 +420,7 @@ __device__ __forceinline__ bool leaf_fast_ok(const Plan& P, int last, bool pairs
 // Returns this lane's count; escalated seeds go to the heavy queue. K is the
 // plan kind of every plan the launch sees (kSupport or kPairs).
 template <int K, int NS>
-__device__ __forceinline__ uint64_t thread_chunk(ThreadLds& W, const DevGraph& G, const ConflictRefs* R,
+__device__ __forceinline__ uint64_t thread_chunk(ThreadLdsT<levels_of(NS)>& W, const DevGraph& G, const ConflictRefs* R,
                                                  uint32_t planIdx, uint32_t begin, uint32_t count, HeavyItem* hq,
                                                  uint32_t* hCount, uint32_t hCap) {
     const int lane = lane_id();
@@ -590,11 +599,11 @@ __device__ __forceinline__ uint32_t ld_rlx(const uint32_t* p) {
 // plan's head-object space (Plan::hoLo/hoHi/hoRow0/hoKeys): one key-index
 // block load and a popcount, so a bitmap needs only as many bits as the
 // largest key set of one (direction, predicate) -- not one per term id.
-template <int D>
+template <int D, int NS>
 struct Dedup {
     uint32_t* bits;     // the wave's own bitmap: its HBM row (kDedupRows) or LDS (kDedupLds)
     uint32_t* shared;   // a giant's HBM row, shared by its sub-items (then used instead of bits), or null
-    WaveLds* W;
+    WaveLdsT<levels_of(NS)>* W;
     int* cache;         // wave_cache_words<D>() words: emitted-object cache (sort) or touched-word list (rows)
     const KeyBlock* blocks;
     const int32_t* crank;
@@ -657,8 +666,8 @@ struct Dedup {
 // issued together, then the marks together, so a batch costs two memory
 // round trips instead of two per candidate. Returns the objects this lane
 // marked first; on the sort path emit[k] gets the objects to emit.
-template <int D, int U>
-__device__ __forceinline__ uint32_t mark_batch(Dedup<D>& dd, const int (&ho)[U], const int (&hr)[U], int (&emit)[U]) {
+template <int D, int U, int NS>
+__device__ __forceinline__ uint32_t mark_batch(Dedup<D, NS>& dd, const int (&ho)[U], const int (&hr)[U], int (&emit)[U]) {
     uint32_t n = 0;
     if (D == kDedupSort) {
 #pragma unroll
@@ -700,8 +709,8 @@ __device__ __forceinline__ void emit_keys(const SortSink& sink, unsigned long lo
 }
 
 // Bitmap paths: marks ranks hr[k] (>= 0; -1: none), tests first, as mark_batch.
-template <int D, int U>
-__device__ __forceinline__ uint32_t mark_ranks(Dedup<D>& dd, const int (&hr)[U]) {
+template <int D, int U, int NS>
+__device__ __forceinline__ uint32_t mark_ranks(Dedup<D, NS>& dd, const int (&hr)[U]) {
     uint32_t w[U];
 #pragma unroll
     for (int k = 0; k < U; ++k) w[k] = hr[k] >= 0 ? dd.word_r(hr[k]) : 0xffffffffu;
@@ -736,7 +745,8 @@ constexpr int kLeafUnroll = 16;  // last-level pool entries an idle lane takes p
 // Returns the count this wave adds (SUPPORT: 0/1; PAIRS: distinct head
 // objects it marked first, or 0 on the sort path, which counts later).
 template <int K, int D, int NS>
-__device__ __forceinline__ uint64_t wave_run(WaveLds& W, const DevGraph& G, const ConflictRefs* R, Dedup<D>& dd,
+__device__ __forceinline__ uint64_t wave_run(WaveLdsT<levels_of(NS)>& W, const DevGraph& G, const ConflictRefs* R,
+                                             Dedup<D, NS>& dd,
                                              int64_t rowWords, const SortSink& sink, unsigned long long keyHi,
                                              const GiantArgs& ga, bool mayGiant, HeavyItem self, uint32_t heavyIdx,
                                              uint32_t* foundFlag, bool& aborted,
@@ -1072,8 +1082,8 @@ __device__ __forceinline__ uint64_t wave_run(WaveLds& W, const DevGraph& G, cons
 
 // Sets up the pool of heavy seed `item` (its level-0 row) and runs it.
 template <int K, int D, int NS>
-__device__ __forceinline__ uint64_t wave_seed(WaveLds& W, const DevGraph& G, const ConflictRefs* R, uint32_t item,
-                                              Dedup<D>& dd, int64_t rowWords, const SortSink& sink,
+__device__ __forceinline__ uint64_t wave_seed(WaveLdsT<levels_of(NS)>& W, const DevGraph& G, const ConflictRefs* R,
+                                              uint32_t item, Dedup<D, NS>& dd, int64_t rowWords, const SortSink& sink,
                                               unsigned long long keyHi, const GiantArgs& ga, uint32_t planIdx,
                                               uint32_t heavyIdx, unsigned long long (&stats)[kWaveStats]) {
     const int lane = lane_id();
@@ -1104,8 +1114,8 @@ __device__ __forceinline__ uint64_t wave_seed(WaveLds& W, const DevGraph& G, con
 // seeds of one plan, one per lane.
 template <int K, int NS>
 __global__ void __launch_bounds__(kBlock) k_thread_batch(ThreadArgs a, DevGraph G) {
-    __shared__ ThreadLds lds[kWavesPerBlock];
-    ThreadLds& W = lds[wave_in_block()];
+    __shared__ ThreadLdsT<levels_of(NS)> lds[kWavesPerBlock];
+    ThreadLdsT<levels_of(NS)>& W = lds[wave_in_block()];
     const int lane = lane_id();
     uint32_t loaded = 0xffffffffu;
     for (;;) {
@@ -1128,8 +1138,8 @@ __global__ void __launch_bounds__(kBlock) k_thread_batch(ThreadArgs a, DevGraph 
 // waves stride over its seeds.
 template <int K, int NS>
 __global__ void __launch_bounds__(kBlock) k_thread_one(Plan P, OneArgs a, DevGraph G) {
-    __shared__ ThreadLds lds[kWavesPerBlock];
-    ThreadLds& W = lds[wave_in_block()];
+    __shared__ ThreadLdsT<levels_of(NS)> lds[kWavesPerBlock];
+    ThreadLdsT<levels_of(NS)>& W = lds[wave_in_block()];
     const int lane = lane_id();
     copy_plan(W, &P);
     const uint32_t waves = (gridDim.x * blockDim.x) / kWave;
@@ -1145,10 +1155,10 @@ __global__ void __launch_bounds__(kBlock) k_thread_one(Plan P, OneArgs a, DevGra
 // Dedup state of one wave-mode wave: D selects the bitmap in HBM (this
 // wave's row), the bitmap in LDS (dynamic shared memory, cleared once here)
 // or the sort path's LDS cache.
-template <int D>
-__device__ __forceinline__ Dedup<D> wave_dedup(WaveLds& W, int* cache, uint32_t* dyn, const WaveArgs& a,
-                                               const DevGraph& G) {
-    Dedup<D> dd;
+template <int D, int NS>
+__device__ __forceinline__ Dedup<D, NS> wave_dedup(WaveLdsT<levels_of(NS)>& W, int* cache, uint32_t* dyn,
+                                                   const WaveArgs& a, const DevGraph& G) {
+    Dedup<D, NS> dd;
     dd.W = &W;
     dd.cache = cache;
     dd.bits = nullptr;
@@ -1176,14 +1186,14 @@ __device__ __forceinline__ void flush_stats(const GiantArgs& ga, const unsigned 
 // seed at a time.
 template <int K, int D, int NS>
 __global__ void __launch_bounds__(kWave) k_wave_batch(WaveArgs a, DevGraph G) {
-    __shared__ WaveLds W;
+    __shared__ WaveLdsT<levels_of(NS)> W;
     __shared__ int cache[wave_cache_words<D>()];
     extern __shared__ uint32_t dyn[];
     const int lane = lane_id();
     uint32_t n = *a.heavyCount;
     if (n > a.heavyCap) n = a.heavyCap;
     if (n == 0) return;   // nothing escalated: no cursor traffic
-    Dedup<D> dd = wave_dedup<D>(W, cache, dyn, a, G);
+    Dedup<D, NS> dd = wave_dedup<D, NS>(W, cache, dyn, a, G);
     uint32_t loaded = 0xffffffffu;
     unsigned long long stats[kWaveStats] = {0, 0, 0, 0, 0};
     for (;;) {
@@ -1203,7 +1213,7 @@ __global__ void __launch_bounds__(kWave) k_wave_batch(WaveArgs a, DevGraph G) {
 
 template <int K, int D, int NS>
 __global__ void __launch_bounds__(kWave) k_wave_one(Plan P, WaveArgs a, DevGraph G) {
-    __shared__ WaveLds W;
+    __shared__ WaveLdsT<levels_of(NS)> W;
     __shared__ int cache[wave_cache_words<D>()];
     extern __shared__ uint32_t dyn[];
     const int lane = lane_id();
@@ -1211,7 +1221,7 @@ __global__ void __launch_bounds__(kWave) k_wave_one(Plan P, WaveArgs a, DevGraph
     if (n > a.heavyCap) n = a.heavyCap;
     if (n == 0) return;
     copy_plan(W, &P);
-    Dedup<D> dd = wave_dedup<D>(W, cache, dyn, a, G);
+    Dedup<D, NS> dd = wave_dedup<D, NS>(W, cache, dyn, a, G);
     unsigned long long stats[kWaveStats] = {0, 0, 0, 0, 0};
     for (;;) {
         const uint32_t h = wave_take(a.cursor);
@@ -1233,7 +1243,7 @@ __global__ void __launch_bounds__(kWave) k_wave_one(Plan P, WaveArgs a, DevGraph
 template <int NS>
 __global__ void __launch_bounds__(kWave) k_giant_expand(GiantArgs ga, const Plan* plans, const ConflictRefs* refs,
                                                         DevGraph G) {
-    __shared__ ThreadLds W;
+    __shared__ ThreadLdsT<levels_of(NS)> W;
     const int lane = lane_id();
     uint32_t n = *ga.count;
     if (n > ga.cap) n = ga.cap;
@@ -1324,7 +1334,7 @@ __global__ void __launch_bounds__(kWave) k_giant_expand(GiantArgs ga, const Plan
 // flag. The sub-item that finishes a giant's last clears its row.
 template <int K, int D, int NS>
 __global__ void __launch_bounds__(kWave) k_wave_sub(WaveArgs a, DevGraph G) {
-    __shared__ WaveLds W;
+    __shared__ WaveLdsT<levels_of(NS)> W;
     __shared__ int cache[wave_cache_words<D>()];
     const int lane = lane_id();
     const GiantArgs& ga = a.giant;
@@ -1332,7 +1342,7 @@ __global__ void __launch_bounds__(kWave) k_wave_sub(WaveArgs a, DevGraph G) {
     const uint32_t capSubs = ga.cap * (uint32_t)kSubsPerGiant;
     if (n > capSubs) n = capSubs;
     if (n == 0) return;   // no giants listed
-    Dedup<D> dd = wave_dedup<D>(W, cache, nullptr, a, G);
+    Dedup<D, NS> dd = wave_dedup<D, NS>(W, cache, nullptr, a, G);
     uint32_t loaded = 0xffffffffu;
     unsigned long long stats[kWaveStats] = {0, 0, 0, 0, 0};
     for (;;) {
@@ -1443,7 +1453,7 @@ template <int NS>
 size_t SlotKernels<NS>::lds_static() {
     hipFuncAttributes fa{};
     if (hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k_wave_batch<kPairs, kDedupLds, NS>)) != hipSuccess)
-        return sizeof(WaveLds) + kCacheSlots * sizeof(int);
+        return sizeof(WaveLdsT<levels_of(NS)>) + kCacheSlots * sizeof(int);
     return fa.sharedSizeBytes;
 }
 
