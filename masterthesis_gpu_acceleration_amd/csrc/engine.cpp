@@ -147,10 +147,10 @@ void giant_prepare(GiantBufs& gb, int after, GiantArgs& ga, hipStream_t s, int p
     ga.stats = gb.stats.as<unsigned long long>() + phase * kWaveStats;
     launch_fill_u64(ga.stats, kWaveStats, 0, s);
     launch_fill_u32(ctl, 4, 0, s);
-    launch_fill_u32(gb.remaining.as<uint32_t>(), kGiantCap, 0, s);
-    launch_fill_u32(gb.found.as<uint32_t>(), kGiantCap, 0, s);
+    launch_fill_u32(gb.remaining.as<uint32_t>(), gb.cap, 0, s);
+    launch_fill_u32(gb.found.as<uint32_t>(), gb.cap, 0, s);
     ga.after = after;
-    ga.cap = kGiantCap;
+    ga.cap = gb.cap;
     ga.list = gb.list.as<HeavyItem>();
     ga.heavyIdx = gb.heavyIdx.as<uint32_t>();
     ga.count = ctl + 0;
@@ -961,7 +961,7 @@ int gpu_autotune(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags, 
         }
         // steps before a wave-mode seed is cut into sub-items
         int bestGa = C->giantAfter;
-        for (int gaf : {512, 2048, 8192}) {
+        for (int gaf : {1024, 2048, 3072, 4096, 8192}) {
             C->giantAfter = gaf;
             if (!trial(ms)) return -1;
             if (ms < bestMs) { bestMs = ms; bestGa = gaf; }

@@ -108,20 +108,30 @@ struct SortBufs {
 // flags, and per-giant HBM bitmap rows (all zero between uses: the last
 // sub-item of a giant clears its row). Counters, remaining counts and flags
 // are zeroed by fill kernels before each wave-mode phase.
-constexpr uint32_t kGiantCap = 512;
+// Giants listed per wave-mode phase: up to kGiantCapMax, fewer when their
+// bitmap rows would pass kGiantRowBudget, never fewer than kGiantCapMin.
+// A seed past the step cap when the list is full runs to its end as usual.
+constexpr uint32_t kGiantCapMin = 512;
+constexpr uint32_t kGiantCapMax = 2048;
+constexpr size_t kGiantRowBudget = (size_t)256 << 20;
 struct GiantBufs {
     DevBuf list, heavyIdx, subs, ctl, remaining, found, rows, stats;
     int64_t rowWords = 0;
+    uint32_t cap = 0;
     bool ensure(int64_t words) {
         words = std::max<int64_t>(words, 1);
         if (rows.p && words == rowWords) return true;
         release();
-        const size_t rowBytes = (size_t)kGiantCap * (size_t)words * 4;
-        if (!list.ensure(kGiantCap * sizeof(HeavyItem)) || !heavyIdx.ensure(kGiantCap * 4) ||
-            !subs.ensure((size_t)kGiantCap * kSubsPerGiant * sizeof(SubItem)) || !ctl.ensure(16) ||
-            !remaining.ensure(kGiantCap * 4) || !found.ensure(kGiantCap * 4) || !rows.ensure(rowBytes) ||
+        const size_t perRow = (size_t)words * 4;
+        const size_t fit = kGiantRowBudget / perRow;
+        const uint32_t c = (uint32_t)std::min<size_t>(kGiantCapMax, std::max<size_t>(kGiantCapMin, fit));
+        const size_t rowBytes = (size_t)c * perRow;
+        if (!list.ensure(c * sizeof(HeavyItem)) || !heavyIdx.ensure(c * 4) ||
+            !subs.ensure((size_t)c * kSubsPerGiant * sizeof(SubItem)) || !ctl.ensure(16) ||
+            !remaining.ensure(c * 4) || !found.ensure(c * 4) || !rows.ensure(rowBytes) ||
             !stats.ensure(2 * kWaveStats * 8))
             return false;
+        cap = c;
         if (hipMemset(rows.p, 0, rowBytes) != hipSuccess) return false;
         rowWords = words;
         return true;
@@ -130,6 +140,7 @@ struct GiantBufs {
         list.release(); heavyIdx.release(); subs.release(); ctl.release(); remaining.release(); found.release();
         rows.release(); stats.release();
         rowWords = 0;
+        cap = 0;
     }
     size_t bytes() const {
         return list.bytes + heavyIdx.bytes + subs.bytes + ctl.bytes + remaining.bytes + found.bytes + rows.bytes;
