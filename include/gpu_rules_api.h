@@ -175,7 +175,9 @@ int gpu_autotune(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags, 
  * moved to wave mode), "thread_waves_per_cu", "wave_waves_per_cu",
  * "giant_after" (wave-mode steps before a seed is cut into sub-items that
  * all waves share; 0 never),
- * "use_graph" (0/1), "dedup" (body-pair dedup of wave-mode seeds; the
+ * "use_graph" (0/1), "overlap" (0/1, default 1: graph replays of ungated
+ * batches run the support and body-pair phases side by side on two
+ * streams), "dedup" (body-pair dedup of wave-mode seeds; the
  * bitmaps index head objects by rank among the keys of one (direction,
  * predicate): 0 auto -- LDS bitmaps when one fits a block and at least 8
  * wave-mode waves per CU still fit, else HBM bitmap rows unless a row per CU

@@ -26,6 +26,9 @@ bool Context::upload() {
     if (prop.sharedMemPerBlock > 0) ldsPerBlock = prop.sharedMemPerBlock;
     ldsPerCU = prop.maxSharedMemoryPerMultiProcessor > 0 ? prop.maxSharedMemoryPerMultiProcessor : ldsPerBlock;
     GR_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    GR_HIP(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));
+    GR_HIP(hipEventCreateWithFlags(&forkEv, hipEventDisableTiming));
+    GR_HIP(hipEventCreateWithFlags(&joinEv, hipEventDisableTiming));
     for (auto& e : ev) GR_HIP(hipEventCreate(&e));
     GR_HIP(hipHostMalloc((void**)&hOne, 512, hipHostMallocDefault));
 
@@ -139,7 +142,7 @@ bool Context::ensure_dedup() {
         if (!dedup.ensure(bytes)) return false;
         GR_HIP(hipMemset(dedup.p, 0, dedup.bytes));
     }
-    return giants.ensure(rowWords);
+    return giants.ensure(rowWords) && giants2.ensure(rowWords);
 }
 
 void giant_prepare(GiantBufs& gb, int after, GiantArgs& ga, hipStream_t s, int phase) {
@@ -373,7 +376,16 @@ void batch_enqueue(Context& C, Batch& B, const ExecTarget& t) {
     launch_fill_u64(counts, 2 * (int64_t)B.capRules, 0, st);
     launch_fill_u32(ctl, kCtlWords, 0, st);
     if (t.ev) (void)hipEventRecord(t.ev[0], st);
+    // ungated: no pairs chunk reads a support count, so the phases may overlap
+    const bool fork = t.st2 && t.giants2 && !t.ev && (B.flags & GPU_RULES_UNGATED) && B.nChunks[0] && B.nChunks[1];
+    if (fork) {
+        (void)hipEventRecord(t.forkEv, st);
+        (void)hipStreamWaitEvent(t.st2, t.forkEv, 0);
+    }
+    const hipStream_t st0 = st;
     for (int ph = 0; ph < 2; ++ph) {
+        const hipStream_t st = (fork && ph == kPairs) ? t.st2 : st0;
+        GiantBufs* gbp = (fork && ph == kPairs) ? t.giants2 : t.giants;
         const uint32_t nc = B.nChunks[ph];
         uint32_t* c = ctl + ph * 4;
         if (nc) {
@@ -391,7 +403,7 @@ void batch_enqueue(Context& C, Batch& B, const ExecTarget& t) {
             WaveArgs wa{B.wplans.as<Plan>(), B.wrefs.as<ConflictRefs>(), B.heavy.as<HeavyItem>() + B.heavyOff[ph],
                         c + 1, B.heavyCap[ph], 0, c + 2, counts, t.dedup, t.rowWords, sink};
             const int mode = ph == kPairs ? t.pairsDedup : kDedupRows;
-            giant_prepare(*t.giants, t.giantAfter, wa.giant, st, ph);
+            giant_prepare(*gbp, t.giantAfter, wa.giant, st, ph);
             const int wb = t.pairsDedup == kDedupLds ? C.waveBlocksFor(B.slots[ph]) : t.waveBlocks;
             launch_wave_batch(ph, B.slots[ph], mode, wa, C.dg, wb, st);
             if (t.giantAfter > 0) {
@@ -411,8 +423,12 @@ void batch_enqueue(Context& C, Batch& B, const ExecTarget& t) {
         }
         if (t.ev) (void)hipEventRecord(t.ev[2 + 2 * ph], st);
     }
-    (void)hipMemcpyAsync(B.hCounts, counts, (size_t)2 * B.capRules * 8, hipMemcpyDeviceToHost, st);
-    (void)hipMemcpyAsync(B.hCtl, ctl, kCtlWords * 4, hipMemcpyDeviceToHost, st);
+    if (fork) {
+        (void)hipEventRecord(t.joinEv, t.st2);
+        (void)hipStreamWaitEvent(st0, t.joinEv, 0);
+    }
+    (void)hipMemcpyAsync(B.hCounts, counts, (size_t)2 * B.capRules * 8, hipMemcpyDeviceToHost, st0);
+    (void)hipMemcpyAsync(B.hCtl, ctl, kCtlWords * 4, hipMemcpyDeviceToHost, st0);
 }
 
 int64_t batch_emitted(const Batch& B) { return B.hCtl ? (int64_t)B.hCtl[7] : 0; }
@@ -454,7 +470,7 @@ bool batch_run(Context& C, Batch& B, GpuRuleResult64* out) {
             if (B.exec && (B.capturedDedup != C.dedup.p || B.capturedBlocks[0] != C.threadBlocks() ||
                            B.capturedBlocks[1] != C.waveBlocks() || B.capturedSort != sp || B.capturedSortCap != sc ||
                            B.capturedMode != mode || B.capturedGiants != C.giants.rows.p ||
-                           B.capturedGiantAfter != C.giantAfter)) {
+                           B.capturedGiantAfter != C.giantAfter || B.capturedOverlap != (int)C.overlap)) {
                 (void)hipGraphExecDestroy(B.exec);
                 (void)hipGraphDestroy(B.graph);
                 B.exec = nullptr;
@@ -469,9 +485,17 @@ bool batch_run(Context& C, Batch& B, GpuRuleResult64* out) {
                 B.capturedMode = mode;
                 B.capturedGiants = C.giants.rows.p;
                 B.capturedGiantAfter = C.giantAfter;
+                B.capturedOverlap = (int)C.overlap;
                 GR_HIP(hipStreamBeginCapture(C.st, hipStreamCaptureModeThreadLocal));
-                batch_enqueue(C, B, ExecTarget{C.st, C.dedup.as<uint32_t>(), C.rowWords, C.waveBlocks(), nullptr, sb,
-                                               C.sortBlocks(), mode, &C.giants, C.giantAfter});
+                ExecTarget t{C.st, C.dedup.as<uint32_t>(), C.rowWords, C.waveBlocks(), nullptr, sb,
+                             C.sortBlocks(), mode, &C.giants, C.giantAfter};
+                if (C.overlap) {
+                    t.st2 = C.st2;
+                    t.forkEv = C.forkEv;
+                    t.joinEv = C.joinEv;
+                    t.giants2 = &C.giants2;
+                }
+                batch_enqueue(C, B, t);
                 GR_HIP(hipStreamEndCapture(C.st, &B.graph));
                 GR_HIP(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
             }
@@ -1040,6 +1064,7 @@ int gpu_set_tuning(void* ctx, const char* key, int64_t value) {
         C->lastBatch = nullptr;
         if (!C->ensure_dedup()) return -1;
     } else if (k == "use_graph") C->useGraph = value != 0;
+    else if (k == "overlap") C->overlap = value != 0;
     else if (k == "dedup") {
         if (value < 0 || value > 3) return -1;
         C->dedupMode = (int)value;
@@ -1066,6 +1091,7 @@ int64_t gpu_get_tuning(void* ctx, const char* key) {
     if (k == "thread_waves_per_cu") return C->threadWavesPerCU;
     if (k == "wave_waves_per_cu") return C->waveWavesPerCU;
     if (k == "use_graph") return C->useGraph ? 1 : 0;
+    if (k == "overlap") return C->overlap ? 1 : 0;
     if (k == "num_cus") return C->numCUs;
     if (k == "wave_slots") return C->waveSlots;
     if (k == "dedup") return C->dedupMode;

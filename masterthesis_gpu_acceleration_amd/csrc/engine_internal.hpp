@@ -176,6 +176,7 @@ struct Batch : Handle {
     int capturedMode = -1;
     const void* capturedGiants = nullptr;
     int capturedGiantAfter = -1;
+    int capturedOverlap = -1;
     int capturedBlocks[2] = {0, 0};
     double lastMs = 0, lastSupMs = 0, lastPairMs = 0, lastSortMs = 0;
     int64_t lastHeavy = 0;
@@ -214,6 +215,13 @@ struct Context : Handle {
     int waveSlots = 0;
     SortBufs sortb;   // sort-unique dedup buffers (allocated when that path is used)
     GiantBufs giants; // giant-seed pass
+    // Ungated batches replayed as graphs run the support and body-pair phases
+    // side by side: the pairs phase on st2 (forked from st by forkEv, joined
+    // back by joinEv) with its own giant buffers.
+    bool overlap = true;
+    hipStream_t st2 = nullptr;
+    hipEvent_t forkEv = nullptr, joinEv = nullptr;
+    GiantBufs giants2;
     int giantAfter = 2048;   // wave-mode steps on one seed before it is listed as a giant (0: never)
     int64_t lastGiants = 0;
     unsigned long long lastWaveStats[2 * kWaveStats] = {};
@@ -257,10 +265,14 @@ inline Batch::~Batch() {
 
 inline Context::~Context() {
     if (st) (void)hipStreamSynchronize(st);
-    blocks.release(); rowStart.release(); col.release(); coo.release(); keys.release();
-    one.release(); heavy1.release(); dedup.release(); sortb.release(); giants.release();
+    if (st2) (void)hipStreamSynchronize(st2);
+    blocks.release(); rowStart.release(); col.release(); coo.release(); keys.release(); crank.release();
+    one.release(); heavy1.release(); dedup.release(); sortb.release(); giants.release(); giants2.release();
     if (hOne) (void)hipHostFree(hOne);
     for (auto& e : ev) if (e) (void)hipEventDestroy(e);
+    if (forkEv) (void)hipEventDestroy(forkEv);
+    if (joinEv) (void)hipEventDestroy(joinEv);
+    if (st2) (void)hipStreamDestroy(st2);
     if (st) (void)hipStreamDestroy(st);
 }
 
@@ -285,6 +297,11 @@ struct ExecTarget {
     int pairsDedup;   // DedupMode of the body-pair wave-mode kernel
     GiantBufs* giants;
     int giantAfter;
+    // optional: run the pairs phase of an ungated batch on st2 beside the
+    // support phase (untimed passes only), with giants2 as its giant buffers
+    hipStream_t st2 = nullptr;
+    hipEvent_t forkEv = nullptr, joinEv = nullptr;
+    GiantBufs* giants2 = nullptr;
 };
 
 // Zeroes the giant pass's counters and flags for one wave-mode phase and

@@ -379,3 +379,29 @@ def test_device_build_turtle(monkeypatch):
                 api.free_graph(ctx)
         assert snaps["0"] == snaps["1"]
         assert outs["0"] == outs["1"]
+
+
+@pytest.mark.parametrize("name", ["dbpedia_shaped_5k", "hub_fanout_20k", "dense_conf_300"])
+def test_overlapped_phases_replay(name):
+    """Graph replays of an ungated batch with the support and body-pair
+    phases on two streams: every replay, interleaved with eager passes and
+    replays of the sequential graph, equals the eager result."""
+    say("overlap", name)
+    fx, g, rules = goldens.load(name)
+    eng = L.Engine.from_graph(g)
+    try:
+        words = L.rule_words_for_graph(rules, g)
+        eng.set_tuning("use_graph", 0)
+        eager = eng.evaluate(words, L.UNGATED).copy()
+        check(fx, rules, eager, False, name + "/eager")
+        eng.set_tuning("use_graph", 1)
+        b = eng.batch(words, L.UNGATED)
+        for k in range(6):
+            eng.set_tuning("overlap", k % 2)
+            assert np.array_equal(b.run(), eager), (name, k)
+            other = eng.evaluate(words[: 10 + 5 * k], 0)
+            check(fx, rules[: 10 + 5 * k], other, True, name + "/interleaved")
+        b.close()
+    finally:
+        eng.set_tuning("overlap", 1)
+        eng.close()
