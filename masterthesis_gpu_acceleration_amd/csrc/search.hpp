@@ -603,6 +603,7 @@ template <int D, int NS>
 struct Dedup {
     uint32_t* bits;     // the wave's own bitmap: its HBM row (kDedupRows) or LDS (kDedupLds)
     uint32_t* shared;   // a giant's HBM row, shared by its sub-items (then used instead of bits), or null
+    uint32_t* merge;    // LDS bitmaps: a giant's HBM row the wave's bitmap is ORed into when the sub-item ends
     WaveLdsT<levels_of(NS)>* W;
     int* cache;         // wave_cache_words<D>() words: emitted-object cache (sort) or touched-word list (rows)
     const KeyBlock* blocks;
@@ -1066,8 +1067,19 @@ __device__ __forceinline__ uint64_t wave_run(WaveLdsT<levels_of(NS)>& W, const D
     // clear the wave's own bitmap (a sub-item marked only its giant's row: nothing listed)
     wave_sync();
     if (D == kDedupLds) {
-        for (int64_t w = lane; w < rowWords; w += kWave) dd.clear_word((uint32_t)w);
+        // a giant's sub-item ORs its bitmap into the giant's row: the bits
+        // that OR sets first are the objects it adds (each bit is set first
+        // exactly once over all sub-items)
+        uint64_t fresh = 0;
+        for (int64_t w = lane; w < rowWords; w += kWave) {
+            if (dd.merge) {
+                const uint32_t x = dd.bits[w];
+                if (x) fresh += (uint64_t)__popc(x & ~atomicOr(dd.merge + w, x));
+            }
+            dd.clear_word((uint32_t)w);
+        }
         wave_sync();
+        if (dd.merge) mine = fresh;
     } else {
         const uint32_t nt = W.nTouched;
         if (nt <= (uint32_t)kTouchCap) {
@@ -1163,6 +1175,7 @@ __device__ __forceinline__ Dedup<D, NS> wave_dedup(WaveLdsT<levels_of(NS)>& W, i
     dd.cache = cache;
     dd.bits = nullptr;
     dd.shared = nullptr;
+    dd.merge = nullptr;
     dd.blocks = G.blocks;
     dd.crank = G.crank;
     if (D == kDedupRows) dd.bits = a.rows + (int64_t)blockIdx.x * a.rowWords;
@@ -1336,13 +1349,14 @@ template <int K, int D, int NS>
 __global__ void __launch_bounds__(kWave) k_wave_sub(WaveArgs a, DevGraph G) {
     __shared__ WaveLdsT<levels_of(NS)> W;
     __shared__ int cache[wave_cache_words<D>()];
+    extern __shared__ uint32_t dyn[];
     const int lane = lane_id();
     const GiantArgs& ga = a.giant;
     uint32_t n = *ga.subCount;
     const uint32_t capSubs = ga.cap * (uint32_t)kSubsPerGiant;
     if (n > capSubs) n = capSubs;
     if (n == 0) return;   // no giants listed
-    Dedup<D, NS> dd = wave_dedup<D, NS>(W, cache, nullptr, a, G);
+    Dedup<D, NS> dd = wave_dedup<D, NS>(W, cache, dyn, a, G);
     uint32_t loaded = 0xffffffffu;
     unsigned long long stats[kWaveStats] = {0, 0, 0, 0, 0};
     for (;;) {
@@ -1373,11 +1387,14 @@ __global__ void __launch_bounds__(kWave) k_wave_sub(WaveArgs a, DevGraph G) {
             if (D == kDedupSort && K == kPairs)
                 for (int k = lane; k < kCacheSlots; k += kWave) cache[k] = -1;
             wave_sync();
-            dd.shared = (K == kPairs && D != kDedupSort) ? ga.rows + (int64_t)g * a.rowWords : nullptr;
+            uint32_t* const row = ga.rows + (int64_t)g * a.rowWords;
+            dd.shared = (K == kPairs && D == kDedupRows) ? row : nullptr;   // marks go straight to the row
+            dd.merge = (K == kPairs && D == kDedupLds) ? row : nullptr;     // marks go to LDS, merged at the end
             bool aborted = false;
             c = wave_run<K, D, NS>(W, G, a.refs + plan, dd, a.rowWords, a.sink, (unsigned long long)hv << a.sink.hoBits,
                                ga, false, HeavyItem{plan, 0u}, hv, foundFlag, aborted, stats);
             dd.shared = nullptr;
+            dd.merge = nullptr;
         }
         if (lane == 0 && c) atomicAdd(a.counts + 2 * W.plan.rule + K, (unsigned long long)c);
         uint32_t last = 0;
@@ -1445,6 +1462,9 @@ void SlotKernels<NS>::giants(int kind, int dedup, const WaveArgs& a, const DevGr
         hipLaunchKernelGGL((k_wave_sub<kSupport, kDedupRows, NS>), dim3(blocks), dim3(kWave), 0, s, a, G);
     else if (dedup == kDedupSort)
         hipLaunchKernelGGL((k_wave_sub<kPairs, kDedupSort, NS>), dim3(blocks), dim3(kWave), 0, s, a, G);
+    else if (dedup == kDedupLds)
+        hipLaunchKernelGGL((k_wave_sub<kPairs, kDedupLds, NS>), dim3(blocks), dim3(kWave), (size_t)a.rowWords * 4, s,
+                           a, G);
     else
         hipLaunchKernelGGL((k_wave_sub<kPairs, kDedupRows, NS>), dim3(blocks), dim3(kWave), 0, s, a, G);
 }
@@ -1462,6 +1482,8 @@ void SlotKernels<NS>::prepare(size_t dynLds) {
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wave_batch<kPairs, kDedupLds, NS>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)dynLds);
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wave_one<kPairs, kDedupLds, NS>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)dynLds);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wave_sub<kPairs, kDedupLds, NS>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)dynLds);
 }
 
