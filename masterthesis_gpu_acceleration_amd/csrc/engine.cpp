@@ -985,7 +985,7 @@ int gpu_autotune(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags, 
         }
         // steps before a wave-mode seed is cut into sub-items
         int bestGa = C->giantAfter;
-        for (int gaf : {1024, 2048, 3072, 4096, 8192}) {
+        for (int gaf : {512, 1024, 2048, 4096, 8192}) {
             C->giantAfter = gaf;
             if (!trial(ms)) return -1;
             if (ms < bestMs) { bestMs = ms; bestGa = gaf; }

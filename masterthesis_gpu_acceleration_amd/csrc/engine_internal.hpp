@@ -108,11 +108,12 @@ struct SortBufs {
 // flags, and per-giant HBM bitmap rows (all zero between uses: the last
 // sub-item of a giant clears its row). Counters, remaining counts and flags
 // are zeroed by fill kernels before each wave-mode phase.
-// Giants listed per wave-mode phase: up to kGiantCapMax, fewer when their
-// bitmap rows would pass kGiantRowBudget, never fewer than kGiantCapMin.
+// Giants listed per wave-mode phase: up to kGiantCapMax (sub-items 96 B x
+// 256 per giant, 100 MB at the maximum), fewer when their bitmap rows would
+// pass kGiantRowBudget, never fewer than kGiantCapMin.
 // A seed past the step cap when the list is full runs to its end as usual.
 constexpr uint32_t kGiantCapMin = 512;
-constexpr uint32_t kGiantCapMax = 2048;
+constexpr uint32_t kGiantCapMax = 4096;
 constexpr size_t kGiantRowBudget = (size_t)256 << 20;
 struct GiantBufs {
     DevBuf list, heavyIdx, subs, ctl, remaining, found, rows, stats;
