@@ -198,7 +198,7 @@ struct Context : Handle {
     bool useGraph = true;
     int dedupMode = 0;        // body-pair dedup: 0 auto, 1 HBM bitmap rows, 2 sort-unique, 3 LDS bitmap
     bool rowsFit = true;      // HBM bitmap rows fit kRowBudget at one wave per CU
-    bool ldsFits = false;     // one nE-bit bitmap fits a wave-mode block's LDS
+    bool ldsFits = false;     // one head-object bitmap (rowWords words) fits a wave-mode block's LDS
     bool ldsPreferred = false;  // ... and still leaves room for kLdsMinWaves wave-mode waves per CU
     size_t ldsPerBlock = 65536, ldsPerCU = 65536;
     hipStream_t st = nullptr;

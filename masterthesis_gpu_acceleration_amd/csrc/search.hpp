@@ -588,17 +588,18 @@ __device__ __forceinline__ uint32_t ld_rlx(const uint32_t* p) {
 }
 
 // Head-object dedup of one PAIRS seed in wave mode.
-//   kDedupRows: the wave's bitmap row of nE bits in HBM (atomicOr);
-//   kDedupLds:  the same bitmap in the wave's LDS;
+//   kDedupRows: the wave's bitmap row in HBM (atomicOr);
+//   kDedupLds:  the same bitmap in the wave's LDS (cleared whole per seed);
 //   kDedupSort: nothing is marked; the caller emits keys to HBM, and a small
 //               direct-mapped LDS cache of emitted objects prunes repeats.
-// For the bitmaps, the lane whose atomicOr finds a word still zero lists it
-// in LDS so that clearing touches only those words; insert returns 1 to the
-// one lane whose call set the bit, so the lanes' sums are the distinct count.
+// For HBM rows, the lane whose atomicOr finds a word still zero lists it in
+// LDS so that clearing touches only those words; insert returns 1 to the one
+// lane whose call set the bit, so the lanes' sums are the distinct count.
 // The bitmaps index a head object x by its rank among the keys of the
-// plan's head-object space (Plan::hoLo/hoHi/hoRow0/hoKeys): one key-index
-// block load and a popcount, so a bitmap needs only as many bits as the
-// largest key set of one (direction, predicate) -- not one per term id.
+// plan's head-object space (Plan::hoLo/hoHi/hoRow0/hoKeys) -- crank[e] when
+// the pair level binds x as col[e], else one key-index block load and a
+// popcount -- so a bitmap needs only as many bits as the largest key set of
+// one (direction, predicate), not one per term id.
 template <int D, int NS>
 struct Dedup {
     uint32_t* bits;     // the wave's own bitmap: its HBM row (kDedupRows) or LDS (kDedupLds)
@@ -1122,8 +1123,8 @@ __device__ __forceinline__ uint64_t wave_seed(WaveLdsT<levels_of(NS)>& W, const 
 
 // ------------------------------------------------------------- kernels
 
-// Persistent thread mode over a chunk list: each wave repeatedly takes <= 64
-// seeds of one plan, one per lane.
+// Persistent thread mode over a chunk list: each wave repeatedly takes a
+// chunk of seeds of one plan; its lanes take the chunk's seeds one at a time.
 template <int K, int NS>
 __global__ void __launch_bounds__(kBlock) k_thread_batch(ThreadArgs a, DevGraph G) {
     __shared__ ThreadLdsT<levels_of(NS)> lds[kWavesPerBlock];
