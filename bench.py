@@ -169,6 +169,9 @@ def main():
                     help="serving mode: Poisson request stream over the high-fan-out graph")
     ap.add_argument("--requests", type=int, default=6000, help="requests in the serving stream")
     ap.add_argument("--rate", type=float, default=150.0, help="requests per second (serving mode)")
+    ap.add_argument("--rates", default="75,150,300,600,1200",
+                    help="serving mode: arrival rates of the rate sweep (comma list; empty: no sweep)")
+    ap.add_argument("--sweep-requests", type=int, default=1500, help="requests per rate of the serving sweep")
     ap.add_argument("--max-work", type=float, default=1e8,
                     help="serving: drop requests whose estimated search work exceeds this (0: keep all)")
     ap.add_argument("--compare-dedup", action="store_true",
@@ -402,6 +405,28 @@ def serve_mode(args):
     }
     log("serving: %.0f rules/s, p50 %.3f ms, p99 %.3f ms, %d batches" % (
         r["rules_per_s"], out["latency_ms"]["p50"], out["latency_ms"]["p99"], st["batches"]))
+    # rate sweep: the first requests of the same stream, arrival times rescaled
+    # to each rate (a Poisson stream stays Poisson under rescaling)
+    sweep = []
+    n = min(len(reqs), args.sweep_requests)
+    for rate in [float(x) for x in args.rates.split(",") if x.strip()]:
+        arr = np.asarray(arrivals[:n], np.float64) * (args.rate / rate)
+        srv = S.Server(eng, max_batch=2048, max_in_flight=3, flags=flags)
+        try:
+            rr = S.run_stream(srv, words[:n], arr)
+            ss = srv.stats()
+        finally:
+            srv.stop()
+        lt = rr["latency_ms"]
+        row = {"rate_per_s": rate, "requests": n, "rules_per_s": round(rr["rules_per_s"], 1),
+               "p50_ms": round(float(np.percentile(lt, 50)), 3), "p99_ms": round(float(np.percentile(lt, 99)), 3),
+               "max_ms": round(float(lt.max()), 3), "batches": ss["batches"],
+               "mean_batch": round(float(rr["completions"]["batchSize"].mean()), 1)}
+        sweep.append(row)
+        log("serving sweep: %(rate_per_s).0f/s offered -> %(rules_per_s).0f rules/s, p50 %(p50_ms).1f ms, "
+            "p99 %(p99_ms).1f ms, mean batch %(mean_batch).1f" % row)
+    if sweep:
+        out["rate_sweep"] = sweep
     eng.close()
     print(json.dumps(out), flush=True)
 
