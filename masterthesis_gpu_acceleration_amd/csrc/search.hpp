@@ -724,6 +724,10 @@ __device__ __forceinline__ uint32_t mark_ranks(Dedup<D, NS>& dd, const int (&hr)
 }
 
 constexpr int kLeafUnroll = 16;  // last-level pool entries an idle lane takes per draw
+// The same for last levels off the straight-line path (conflict checks,
+// SCAN levels, a head object bound above): each candidate inlines the full
+// binder, so a short batch keeps the kernel's code small.
+constexpr int kLeafGeneric = 4;
 
 // Wave mode: the whole wave on one seed, with a shared pool of work.
 //   * The pool starts as the seed's level-0 row. Each step, the idle lanes
@@ -811,20 +815,20 @@ __device__ __forceinline__ uint64_t wave_run(WaveLdsT<levels_of(NS)>& W, const D
                     else mine += mark_ranks<D, kLeafUnroll>(dd, rs);
                 } else if (plv == last) {
                     ++nLeafDraw;
-                    next = pn + n * kLeafUnroll < pe ? pn + n * kLeafUnroll : pe;
+                    next = pn + n * kLeafGeneric < pe ? pn + n * kLeafGeneric : pe;
                     const bool byEntry = pairs && pl == last && D != kDedupSort && hoByEntry;
-                    int xs[kLeafUnroll], rs[kLeafUnroll];
+                    int xs[kLeafGeneric], rs[kLeafGeneric];
 #pragma unroll
-                    for (int k = 0; k < kLeafUnroll; ++k) {
+                    for (int k = 0; k < kLeafGeneric; ++k) {
                         const int i = pn + r + n * k;
                         const bool in = !busy && i < pe;
                         xs[k] = in ? G.col[i] : 0;
                         rs[k] = (in && byEntry) ? G.crank[i] : hru;
                     }
                     bool hit = false;
-                    int hos[kLeafUnroll];
+                    int hos[kLeafGeneric];
 #pragma unroll
-                    for (int k = 0; k < kLeafUnroll; ++k) {
+                    for (int k = 0; k < kLeafGeneric; ++k) {
                         const int i = pn + r + n * k;
                         hos[k] = -1;
                         if (!busy && i < pe && bind_entry_x(P.lev[last], G, R, v, i, xs[k])) {
@@ -839,17 +843,17 @@ __device__ __forceinline__ uint64_t wave_run(WaveLdsT<levels_of(NS)>& W, const D
                     if (pairs) {
                         if (pl < last) {   // one head object for the whole pool: mark it once
 #pragma unroll
-                            for (int k = 1; k < kLeafUnroll; ++k)
+                            for (int k = 1; k < kLeafGeneric; ++k)
                                 if (hos[0] < 0) hos[0] = hos[k];
 #pragma unroll
-                            for (int k = 1; k < kLeafUnroll; ++k) hos[k] = -1;
+                            for (int k = 1; k < kLeafGeneric; ++k) hos[k] = -1;
                         }
-                        int emits[kLeafUnroll];
-                        mine += mark_batch<D, kLeafUnroll>(dd, hos, rs, emits);
+                        int emits[kLeafGeneric];
+                        mine += mark_batch<D, kLeafGeneric>(dd, hos, rs, emits);
                         hit = hos[0] >= 0;
                         if (D == kDedupSort) {
 #pragma unroll
-                            for (int k = 0; k < kLeafUnroll; ++k) emit_keys(sink, keyHi, emits[k]);
+                            for (int k = 0; k < kLeafGeneric; ++k) emit_keys(sink, keyHi, emits[k]);
                         }
                     }
                     // head object bound above the pool and now counted: the rest adds nothing
@@ -882,9 +886,9 @@ __device__ __forceinline__ uint64_t wave_run(WaveLdsT<levels_of(NS)>& W, const D
         if (!pairs && __any(found)) break;
         // ---- B: busy lanes advance their own search by one step
         int emit = -1;
-        int emits[kLeafBatch];
+        int emits[kLeafGeneric];
 #pragma unroll
-        for (int k = 0; k < kLeafBatch; ++k) emits[k] = -1;
+        for (int k = 0; k < kLeafGeneric; ++k) emits[k] = -1;
         if (busy) {
             if (pairs && pl < base && dd.seen_r(hr)) {
                 busy = false;   // head object counted by another lane
@@ -914,20 +918,20 @@ __device__ __forceinline__ uint64_t wave_run(WaveLdsT<levels_of(NS)>& W, const D
                         mine += mark_ranks<D, kLeafBatch>(dd, rs);
                     }
                 } else if (d == last && fr.end - fr.i > 1) {
-                    // last level: up to kLeafBatch candidates per step, loaded together
-                    const int take = fr.end - fr.i < kLeafBatch ? fr.end - fr.i : kLeafBatch;
+                    // last level: up to kLeafGeneric candidates per step, loaded together
+                    const int take = fr.end - fr.i < kLeafGeneric ? fr.end - fr.i : kLeafGeneric;
                     W.frame[d][lane].i = fr.i + take;
                     const bool byEntry = pairs && d == pl && D != kDedupSort && hoByEntry;
-                    int xs[kLeafBatch], rs[kLeafBatch];
+                    int xs[kLeafGeneric], rs[kLeafGeneric];
 #pragma unroll
-                    for (int k = 0; k < kLeafBatch; ++k) {
+                    for (int k = 0; k < kLeafGeneric; ++k) {
                         xs[k] = k < take ? G.col[fr.i + k] : 0;
                         rs[k] = (k < take && byEntry) ? G.crank[fr.i + k] : hr;
                     }
-                    int hos[kLeafBatch];
+                    int hos[kLeafGeneric];
                     bool any = false;
 #pragma unroll
-                    for (int k = 0; k < kLeafBatch; ++k) {
+                    for (int k = 0; k < kLeafGeneric; ++k) {
                         hos[k] = -1;
                         if (k < take && bind_entry_x(P.lev[d], G, R, v, fr.i + k, xs[k])) {
                             hos[k] = pairs ? term_value(v, hoRef) : 0;
@@ -943,16 +947,16 @@ __device__ __forceinline__ uint64_t wave_run(WaveLdsT<levels_of(NS)>& W, const D
                     } else {
                         if (d != pl) {   // head object bound above: one grounding settles it
 #pragma unroll
-                            for (int k = 1; k < kLeafBatch; ++k)
+                            for (int k = 1; k < kLeafGeneric; ++k)
                                 if (hos[0] < 0) hos[0] = hos[k];
 #pragma unroll
-                            for (int k = 1; k < kLeafBatch; ++k) hos[k] = -1;
+                            for (int k = 1; k < kLeafGeneric; ++k) hos[k] = -1;
                             if (any) {
                                 if (pl >= base) d = pl;   // head object settled: next candidate at the pair level
                                 else busy = false;
                             }
                         }
-                        mine += mark_batch<D, kLeafBatch>(dd, hos, rs, emits);
+                        mine += mark_batch<D, kLeafGeneric>(dd, hos, rs, emits);
                     }
                 } else {
                     W.frame[d][lane].i = fr.i + 1;
@@ -985,7 +989,7 @@ __device__ __forceinline__ uint64_t wave_run(WaveLdsT<levels_of(NS)>& W, const D
         if (D == kDedupSort && pairs) {
             emit_keys(sink, keyHi, emit);
 #pragma unroll
-            for (int k = 0; k < kLeafBatch; ++k) emit_keys(sink, keyHi, emits[k]);
+            for (int k = 0; k < kLeafGeneric; ++k) emit_keys(sink, keyHi, emits[k]);
         }
         if (!pairs && __any(found)) break;
         // ---- C: done, hand the seed over as a giant, or donate work to the idle lanes
