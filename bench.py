@@ -178,6 +178,8 @@ def main():
                     help="also time the batch under each body-pair dedup strategy (default for dense_conf)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-snapshot", action="store_true", help="skip the graph-load leg (triples vs snapshot)")
+    ap.add_argument("--no-overlap", action="store_true",
+                    help="graph replays run the support and body-pair phases one after the other (per-kernel profiles)")
     ap.add_argument("--no-per-rule", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tune", default="",
@@ -217,6 +219,8 @@ def main():
     log("graph resident in HBM (%.2fs)" % load_triples_s)
 
     flags = 0 if args.gated else L.UNGATED
+    if args.no_overlap:
+        eng.set_tuning("overlap", 0)
     if args.tune:
         knobs = [int(x) for x in args.tune.split(",")]
         for key, val in zip(("budget", "thread_waves_per_cu", "wave_waves_per_cu", "dedup", "giant_after"), knobs):

@@ -955,11 +955,21 @@ int gpu_autotune(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags, 
             if (ms < bestMs) { bestMs = ms; bestBudget = b; }
         }
         C->tune.budget = bestBudget;
+        // thread-mode waves: the most waves within 2% of the fastest, so that
+        // near-ties (the phases share the GPU in a replay) resolve the same way
+        // every time and each kernel keeps its full share when run alone
         int bestTw = C->threadWavesPerCU;
-        for (int t : tw) {
-            C->threadWavesPerCU = t;
+        double twMs[4] = {};
+        double twMin = bestMs;
+        for (int q = 0; q < 4; ++q) {
+            C->threadWavesPerCU = tw[q];
             if (!trial(ms)) return -1;
-            if (ms < bestMs) { bestMs = ms; bestTw = t; }
+            twMs[q] = ms;
+            twMin = std::min(twMin, ms);
+        }
+        if (twMin < bestMs) {
+            for (int q = 0; q < 4; ++q)
+                if (twMs[q] <= twMin * 1.02) { bestTw = tw[q]; bestMs = twMs[q]; }
         }
         C->threadWavesPerCU = bestTw;
         int bestWw = C->waveWavesPerCU;
