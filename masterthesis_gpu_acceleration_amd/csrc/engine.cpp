@@ -585,18 +585,10 @@ void build_arrays(HostGraph& g, std::vector<int32_t>& S, std::vector<int32_t>& P
         HostGraph d;
         std::string err;
         if (build_graph_device(d, S.data(), P.data(), O.data(), (int64_t)S.size(), nTerms, err)) {
-            g.nE = d.nE;
-            g.predTerm = std::move(d.predTerm);
-            g.predOf = std::move(d.predOf);
-            g.predOff = std::move(d.predOff);
-            g.spoKey = std::move(d.spoKey);
-            g.spoVal = std::move(d.spoVal);
-            g.posKey = std::move(d.posKey);
-            g.posVal = std::move(d.posVal);
-            g.keysS = std::move(d.keysS);
-            g.keysSOff = std::move(d.keysSOff);
-            g.keysO = std::move(d.keysO);
-            g.keysOOff = std::move(d.keysOOff);
+            // the device build fills everything but what parsing produced
+            d.terms = std::move(g.terms);
+            d.prefixes = std::move(g.prefixes);
+            g = std::move(d);
             std::vector<int32_t>().swap(S);
             std::vector<int32_t>().swap(P);
             std::vector<int32_t>().swap(O);
