@@ -180,6 +180,8 @@ def main():
     ap.add_argument("--no-snapshot", action="store_true", help="skip the graph-load leg (triples vs snapshot)")
     ap.add_argument("--no-overlap", action="store_true",
                     help="graph replays run the support and body-pair phases one after the other (per-kernel profiles)")
+    ap.add_argument("--no-fuse", action="store_true",
+                    help="every rule's support gets its own pass (no support counting in the body-pair pass)")
     ap.add_argument("--no-per-rule", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tune", default="",
@@ -221,6 +223,8 @@ def main():
     flags = 0 if args.gated else L.UNGATED
     if args.no_overlap:
         eng.set_tuning("overlap", 0)
+    if args.no_fuse:
+        eng.set_tuning("fuse_support", 0)
     if args.tune:
         knobs = [int(x) for x in args.tune.split(",")]
         for key, val in zip(("budget", "thread_waves_per_cu", "wave_waves_per_cu", "dedup", "giant_after"), knobs):
@@ -300,7 +304,8 @@ def main():
         "rules_with_support": int((res["support"] > 0).sum()),
         "escalated_seeds": int(heavy),
         "tuning": {k: eng.get_tuning(k) for k in ("budget", "thread_waves_per_cu", "wave_waves_per_cu", "pairs_dedup",
-                                                  "giant_after", "wave_slots")},
+                                                  "giant_after", "wave_slots", "fuse_active")},
+        "support_fused_rules": eng.get_tuning("last_fused"),
         "e2e_rules_per_s": round(n / e2e_s, 1),
         "batch_build_s": round(build_s, 4),
         "latency_ms": {"batched_p50": round(pct([x for x in dev_ms], 0.5), 4),

@@ -150,7 +150,9 @@ int gpu_last_dedup_stats(void* ctx, double* sortMs, int64_t* keysEmitted, int64_
 /* Host model of the HBM bytes each rule's searches read (the same plans,
  * walked serially): out[4*i] support bytes, out[4*i+1] body-pair bytes,
  * out[4*i+2], out[4*i+3] search levels entered. Works on a context or a
- * host handle; does no device work. */
+ * host handle; does no device work. On a context with fuse_support active,
+ * a rule whose support the body-pair pass counts has no support bytes and
+ * its body-pair bytes include the support lookups. */
 int gpu_plan_traffic(void* handle, const GpuRuleDesc* rules, int32_t n, int64_t* out);
 
 /* Measurement yardstick: copies `bytes` (multiple of 16) between two fresh
@@ -177,7 +179,11 @@ int gpu_autotune(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags, 
  * all waves share; 0 never),
  * "use_graph" (0/1), "overlap" (0/1, default 1: graph replays of ungated
  * batches run the support and body-pair phases side by side on two
- * streams), "dedup" (body-pair dedup of wave-mode seeds; the
+ * streams), "fuse_support" (0/1, default 1: with bitmap dedup, batches
+ * count the support of rules with distinct head variables and no body atom
+ * that can ground to the head triple in the body-pair pass -- a head triple
+ * (s, o) is supported iff (s, o) is a distinct body pair -- instead of a
+ * support pass of their own), "dedup" (body-pair dedup of wave-mode seeds; the
  * bitmaps index head objects by rank among the keys of one (direction,
  * predicate): 0 auto -- LDS bitmaps when one fits a block and at least 8
  * wave-mode waves per CU still fit, else HBM bitmap rows unless a row per CU
@@ -186,8 +192,10 @@ int gpu_autotune(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags, 
  * sort-unique buffers, >= 4096; they grow on overflow). Returns 0, or -1 for
  * an unknown key or value. gpu_get_tuning also reads "num_cus",
  * "wave_slots", "sort_active" (0/1), "sort_cap", "pairs_dedup" (the
- * DedupMode in use: 0 HBM rows, 1 sort, 2 LDS) and "last_giants" (seeds cut
- * into sub-items by the last batch). */
+ * DedupMode in use: 0 HBM rows, 1 sort, 2 LDS), "fuse_active" (0/1: the
+ * dedup in use lets fuse_support act), "last_giants" (seeds cut into
+ * sub-items by the last batch) and "last_fused" (rules whose support the last
+ * batch counted in its body-pair pass). */
 int gpu_set_tuning(void* ctx, const char* key, int64_t value);
 int64_t gpu_get_tuning(void* ctx, const char* key);
 

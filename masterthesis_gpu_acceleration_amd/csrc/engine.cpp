@@ -293,18 +293,19 @@ bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const s
     std::vector<ConflictRefs> refs(np), wrefs(np);
     std::vector<Chunk> lists[2];
     uint64_t items[2] = {0, 0};
-    struct Job { int ph; size_t k; int64_t n; };
+    struct Job { int ph; size_t k; int64_t n; bool fus; };
     std::vector<Job> jobs;
     auto add = [&](int ph, size_t i, const Plan& P, const ConflictRefs& R, const Plan& W, const ConflictRefs& RW,
-                   int64_t n) {
+                   int64_t n, bool fus) {
         const size_t k = 2 * i + ph;
         plans[k] = P;
         wplans[k] = W;
         refs[k] = R;
         wrefs[k] = RW;
         plans[k].rule = wplans[k].rule = (int32_t)i;
+        plans[k].supFused = wplans[k].supFused = (ph == kPairs && fus) ? 1 : 0;
         B.slots[ph] = std::max({B.slots[ph], plan_slots(P, R), plan_slots(W, RW)});
-        jobs.push_back(Job{ph, k, n});
+        jobs.push_back(Job{ph, k, n, fus});
         items[ph] += (uint64_t)n;
     };
     for (size_t i = 0; i < rules.size(); ++i) {
@@ -313,24 +314,35 @@ bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const s
         cr = compile_rule(rules[i], C.g, C.tune);
         B.status[i] = cr.status;
         if (cr.status != kRuleOk) continue;
+        // the pairs pass may count this rule's support (a fused launch then skips its support chunks)
+        const bool fus = cr.supFromPairs && cr.supItems > 0 && cr.pairItems > 0;
         if (cr.supportLaunch && cr.supItems > 0)
-            add(0, i, cr.sup, cr.supRefs, cr.supWave, cr.supWaveRefs, cr.supItems);
+            add(0, i, cr.sup, cr.supRefs, cr.supWave, cr.supWaveRefs, cr.supItems, fus);
         if (cr.pairsLaunch && cr.pairItems > 0)
-            add(1, i, cr.pairs, cr.pairsRefs, cr.pairsWave, cr.pairsWaveRefs, cr.pairItems);
+            add(1, i, cr.pairs, cr.pairsRefs, cr.pairsWave, cr.pairsWaveRefs, cr.pairItems, fus);
     }
     // Seeds per thread-mode work unit: 1024 (fewer plan loads and cursor
     // atomics) unless that leaves fewer than two units per thread-mode wave,
     // down to 64 -- a batch of few rules with few, costly seeds each still
     // spreads over the whole GPU.
     const uint64_t waves = (uint64_t)C.numCUs * (uint64_t)C.threadWavesPerCU;
+    B.nFusable = 0;
+    B.nFused = 0;
+    for (const Job& j : jobs) B.nFused += (j.ph == kPairs && j.fus) ? 1 : 0;
     for (int ph = 0; ph < 2; ++ph) {
         int64_t chunk = kChunkSeeds;
         while (chunk > kMinChunkSeeds && items[ph] / (uint64_t)chunk < 2 * waves) chunk >>= 1;
-        for (const Job& j : jobs)
-            if (j.ph == ph)
-                for (int64_t b = 0; b < j.n; b += chunk)
-                    lists[ph].push_back(Chunk{(uint32_t)j.k, (uint32_t)b, (uint32_t)std::min<int64_t>(chunk, j.n - b),
-                                              (ph == 1 && gated) ? 1u : 0u});
+        // support chunks of fusable rules go last (pass 1)
+        for (int pass = 0; pass < (ph == kSupport ? 2 : 1); ++pass)
+            for (const Job& j : jobs) {
+                if (j.ph != ph || (ph == kSupport && (int)j.fus != pass)) continue;
+                const uint32_t gate = (ph == kPairs && gated) ? (j.fus ? 2u : 1u) : 0u;
+                for (int64_t b = 0; b < j.n; b += chunk) {
+                    lists[ph].push_back(
+                        Chunk{(uint32_t)j.k, (uint32_t)b, (uint32_t)std::min<int64_t>(chunk, j.n - b), gate});
+                    if (ph == kSupport && j.fus) ++B.nFusable;
+                }
+            }
     }
     std::vector<Chunk> all;
     uint64_t heavyTotal = 0;
@@ -368,7 +380,9 @@ bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const s
 // The device pass of a batch (what the HIP graph captures): per phase, one
 // persistent thread-mode kernel over all chunks, then one wave-mode kernel
 // over the seeds it escalated. PAIRS chunks of rules whose support is 0 are
-// skipped on the device when the batch is gated.
+// skipped on the device when the batch is gated. With bitmap dedup the pairs
+// phase also counts the support of fusable rules, whose support chunks are
+// then left out (and their pairs chunks are not gated).
 void batch_enqueue(Context& C, Batch& B, const ExecTarget& t) {
     hipStream_t st = t.st;
     uint32_t* ctl = B.ctl.as<uint32_t>();
@@ -376,8 +390,10 @@ void batch_enqueue(Context& C, Batch& B, const ExecTarget& t) {
     launch_fill_u64(counts, 2 * (int64_t)B.capRules, 0, st);
     launch_fill_u32(ctl, kCtlWords, 0, st);
     if (t.ev) (void)hipEventRecord(t.ev[0], st);
+    const bool fuse = C.fuseActive(t.pairsDedup);
+    const uint32_t ncPh[2] = {B.nChunks[0] - (fuse ? B.nFusable : 0u), B.nChunks[1]};
     // ungated: no pairs chunk reads a support count, so the phases may overlap
-    const bool fork = t.st2 && t.giants2 && !t.ev && (B.flags & GPU_RULES_UNGATED) && B.nChunks[0] && B.nChunks[1];
+    const bool fork = t.st2 && t.giants2 && !t.ev && (B.flags & GPU_RULES_UNGATED) && ncPh[0] && ncPh[1];
     if (fork) {
         (void)hipEventRecord(t.forkEv, st);
         (void)hipStreamWaitEvent(t.st2, t.forkEv, 0);
@@ -386,11 +402,13 @@ void batch_enqueue(Context& C, Batch& B, const ExecTarget& t) {
     for (int ph = 0; ph < 2; ++ph) {
         const hipStream_t st = (fork && ph == kPairs) ? t.st2 : st0;
         GiantBufs* gbp = (fork && ph == kPairs) ? t.giants2 : t.giants;
-        const uint32_t nc = B.nChunks[ph];
+        const uint32_t nc = ncPh[ph];
+        const bool fusePh = fuse && ph == kPairs;
         uint32_t* c = ctl + ph * 4;
         if (nc) {
             ThreadArgs ta{B.plans.as<Plan>(), B.refs.as<ConflictRefs>(), B.chunks.as<Chunk>() + B.chunkOff[ph], nc,
-                          B.heavyCap[ph], c + 0, counts, B.heavy.as<HeavyItem>() + B.heavyOff[ph], c + 1};
+                          B.heavyCap[ph], c + 0, counts, B.heavy.as<HeavyItem>() + B.heavyOff[ph], c + 1,
+                          fusePh ? 1 : 0};
             const int blocks = std::min<int>(C.threadBlocks(), (int)((nc + 3) / 4));
             launch_thread_batch(ph, B.slots[ph], ta, C.dg, std::max(blocks, 1), st);
         }
@@ -401,7 +419,7 @@ void batch_enqueue(Context& C, Batch& B, const ExecTarget& t) {
             SortSink sink{};
             if (sort) sink = SortSink{t.sort->keys[0].as<unsigned long long>(), c + 3, t.sort->cap, hoBits};
             WaveArgs wa{B.wplans.as<Plan>(), B.wrefs.as<ConflictRefs>(), B.heavy.as<HeavyItem>() + B.heavyOff[ph],
-                        c + 1, B.heavyCap[ph], 0, c + 2, counts, t.dedup, t.rowWords, sink};
+                        c + 1, B.heavyCap[ph], fusePh ? 1u : 0u, c + 2, counts, t.dedup, t.rowWords, sink};
             const int mode = ph == kPairs ? t.pairsDedup : kDedupRows;
             giant_prepare(*gbp, t.giantAfter, wa.giant, st, ph);
             const int wb = t.pairsDedup == kDedupLds ? C.waveBlocksFor(B.slots[ph]) : t.waveBlocks;
@@ -470,7 +488,8 @@ bool batch_run(Context& C, Batch& B, GpuRuleResult64* out) {
             if (B.exec && (B.capturedDedup != C.dedup.p || B.capturedBlocks[0] != C.threadBlocks() ||
                            B.capturedBlocks[1] != C.waveBlocks() || B.capturedSort != sp || B.capturedSortCap != sc ||
                            B.capturedMode != mode || B.capturedGiants != C.giants.rows.p ||
-                           B.capturedGiantAfter != C.giantAfter || B.capturedOverlap != (int)C.overlap)) {
+                           B.capturedGiantAfter != C.giantAfter || B.capturedOverlap != (int)C.overlap ||
+                           B.capturedFuse != (int)C.fuseActive(mode))) {
                 (void)hipGraphExecDestroy(B.exec);
                 (void)hipGraphDestroy(B.graph);
                 B.exec = nullptr;
@@ -486,6 +505,7 @@ bool batch_run(Context& C, Batch& B, GpuRuleResult64* out) {
                 B.capturedGiants = C.giants.rows.p;
                 B.capturedGiantAfter = C.giantAfter;
                 B.capturedOverlap = (int)C.overlap;
+                B.capturedFuse = (int)C.fuseActive(mode);
                 GR_HIP(hipStreamBeginCapture(C.st, hipStreamCaptureModeThreadLocal));
                 ExecTarget t{C.st, C.dedup.as<uint32_t>(), C.rowWords, C.waveBlocks(), nullptr, sb,
                              C.sortBlocks(), mode, &C.giants, C.giantAfter};
@@ -548,6 +568,7 @@ bool batch_run(Context& C, Batch& B, GpuRuleResult64* out) {
     C.lastHeavy = B.lastHeavy;
     C.lastEmitted = sb ? B.lastEmitted : 0;
     C.lastGiants = (int64_t)B.hGiant[0] + B.hGiant[1];
+    C.lastFused = C.fuseActive(mode) ? B.nFused : 0;
     for (int k = 0; k < 2 * kWaveStats; ++k) C.lastWaveStats[k] = B.hWaveStats[k];
     for (int k = 0; k < 4; ++k) C.lastKernelMs[k] = B.kernelMs[k];
     return true;
@@ -1023,15 +1044,20 @@ int gpu_plan_traffic(void* handle, const GpuRuleDesc* rules, int32_t n, int64_t*
     if (!g || (!rules && n > 0) || (!out && n > 0)) return -1;
     try {
         Tuning t;
+        // a context whose batches count fusable rules' support in the pairs pass is modelled that way
+        const Context* C = as_ctx(handle);
+        const bool fuse = C && C->fuseActive(C->pairsDedup());
         std::vector<CompiledRule> crs(std::max<int32_t>(n, 0));
         std::vector<TrafficJob> jobs;
         std::vector<int64_t> slot;   // out index of each job
         for (int32_t i = 0; i < n; ++i) {
             crs[i] = compile_rule(rule_from_words(rules[i].words), *g, t);
-            const CompiledRule& cr = crs[i];
+            CompiledRule& cr = crs[i];
             for (int k = 0; k < 4; ++k) out[4 * i + k] = 0;
             if (cr.status != kRuleOk) continue;
-            if (cr.supportLaunch) {
+            const bool fus = fuse && cr.supFromPairs && cr.supItems > 0 && cr.pairItems > 0;
+            cr.pairs.supFused = fus ? 1 : 0;
+            if (cr.supportLaunch && !fus) {
                 jobs.push_back(TrafficJob{&cr.sup, &cr.supRefs, cr.supItems});
                 slot.push_back(4 * (int64_t)i);
             }
@@ -1067,6 +1093,7 @@ int gpu_set_tuning(void* ctx, const char* key, int64_t value) {
         if (!C->ensure_dedup()) return -1;
     } else if (k == "use_graph") C->useGraph = value != 0;
     else if (k == "overlap") C->overlap = value != 0;
+    else if (k == "fuse_support") C->fuseSupport = value != 0;
     else if (k == "dedup") {
         if (value < 0 || value > 3) return -1;
         C->dedupMode = (int)value;
@@ -1094,6 +1121,8 @@ int64_t gpu_get_tuning(void* ctx, const char* key) {
     if (k == "wave_waves_per_cu") return C->waveWavesPerCU;
     if (k == "use_graph") return C->useGraph ? 1 : 0;
     if (k == "overlap") return C->overlap ? 1 : 0;
+    if (k == "fuse_support") return C->fuseSupport ? 1 : 0;
+    if (k == "fuse_active") return C->fuseActive(C->pairsDedup()) ? 1 : 0;
     if (k == "num_cus") return C->numCUs;
     if (k == "wave_slots") return C->waveSlots;
     if (k == "dedup") return C->dedupMode;
@@ -1101,6 +1130,7 @@ int64_t gpu_get_tuning(void* ctx, const char* key) {
     if (k == "pairs_dedup") return C->pairsDedup();
     if (k == "giant_after") return C->giantAfter;
     if (k == "last_giants") return C->lastGiants;
+    if (k == "last_fused") return C->lastFused;
     // wave-mode counters of the last batch: wave_<phase>_<seeds|steps|donations|leaf_draws|draws>
     static const char* kStatNames[kWaveStats] = {"seeds", "steps", "donations", "leaf_draws", "draws"};
     for (int ph = 0; ph < 2; ++ph)

@@ -162,6 +162,11 @@ struct Batch : Handle {
     uint32_t chunkOff[2] = {};
     uint32_t heavyCap[2] = {};
     uint64_t heavyOff[2] = {};
+    // support chunks of rules whose support a fused pairs launch counts
+    // (CompiledRule::supFromPairs); they end the support chunk list, so such
+    // a launch runs only the first nChunks[0] - nFusable
+    uint32_t nFusable = 0;
+    int64_t nFused = 0;   // rules with a fusable support (pairs plans with supFused)
     uint32_t capRules = 0;
     int slots[2] = {1, 1};   // variable slots of the plans of each phase (picks the kernels' slot class)
     double kernelMs[4] = {};
@@ -178,6 +183,7 @@ struct Batch : Handle {
     const void* capturedGiants = nullptr;
     int capturedGiantAfter = -1;
     int capturedOverlap = -1;
+    int capturedFuse = -1;
     int capturedBlocks[2] = {0, 0};
     double lastMs = 0, lastSupMs = 0, lastPairMs = 0, lastSortMs = 0;
     int64_t lastHeavy = 0;
@@ -224,7 +230,12 @@ struct Context : Handle {
     hipEvent_t forkEv = nullptr, joinEv = nullptr;
     GiantBufs giants2;
     int giantAfter = 2048;   // wave-mode steps on one seed before it is listed as a giant (0: never)
+    // Count the support of qualifying rules in the body-pair pass instead of
+    // a support pass of their own; needs bitmap dedup (rows or LDS).
+    bool fuseSupport = true;
+    bool fuseActive(int mode) const { return fuseSupport && (mode == kDedupRows || mode == kDedupLds); }
     int64_t lastGiants = 0;
+    int64_t lastFused = 0;   // rules whose support the last batch counted in its pairs pass
     unsigned long long lastWaveStats[2 * kWaveStats] = {};
     Batch* lastBatch = nullptr;
     double lastMs = 0, lastSupMs = 0, lastPairMs = 0, lastSortMs = 0;

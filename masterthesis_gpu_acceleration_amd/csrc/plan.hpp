@@ -81,7 +81,15 @@ struct alignas(16) Plan {
     // hoLo | hoHi << 32, global row number hoRow0 of its first key, hoKeys keys.
     int32_t hoLo, hoHi, hoRow0, hoKeys;
     int32_t hoByEntry;   // 1: the pair level binds the head object as col[e], whose rank is crank[e]
-    int32_t pad1;
+    // PAIRS, supFused = 1: the plan can count the rule's support as well (when
+    // the launch's `fuse` is set): a head triple (s, p_head, o) is supported
+    // iff (s, o) is a distinct body pair, so each seed s adds the head objects
+    // it found that are objects of s in the head predicate's SPO rows, whose
+    // key-index block run is supLo | supHi << 32. Only rules where no body
+    // atom can ground to the head triple itself qualify (the support search
+    // excludes that triple, the pair search does not).
+    int32_t supFused;
+    int32_t supLo, supHi, pad2, pad3;
     Level lev[kMaxBody];
 };
 
@@ -124,7 +132,7 @@ struct alignas(16) Chunk {
     uint32_t plan;       // index into the plan table
     uint32_t begin;      // first seed (relative to the plan's seed range)
     uint32_t count;      // seeds in this chunk (<= kChunkSeeds)
-    uint32_t gate;       // 1: skip if the rule's support is 0 (PAIRS chunks only)
+    uint32_t gate;       // PAIRS chunks: 1 skip if the rule's support is 0; 2 the same unless the launch fuses support
 };
 
 // A seed escalated from thread mode to wave mode.

@@ -359,6 +359,29 @@ CompiledRule compile_rule(const RuleIR& r, const HostGraph& g, const Tuning& t) 
             c.pairsWaveOk = true;
         }
     }
+
+    // -------------------------------------- support from the body pairs
+    // A head triple (s, o) is supported iff the body has an injective
+    // grounding with those values that does not use the triple itself; a
+    // distinct body pair (s, o) has one, and it cannot use that triple when
+    // no body atom can ground to it (atom_eq == kNever against the head, the
+    // test that sets the support plan's head-conflict bit). So the support
+    // is the number of distinct body pairs (s, o) with (s, p_head, o) a
+    // triple, which the pair plans can count at the end of each seed.
+    if (c.supportLaunch && c.pairsLaunch && r.head.s.v != r.head.o.v && c.pairs.pairLevel >= 0 &&
+        c.pairsWave.pairLevel >= 0) {
+        bool headFree = true;
+        for (const AtomIR& a : r.body)
+            if (a.pterm == r.head.pterm && atom_eq(a, r.head) != kNever) headFree = false;
+        if (headFree) {
+            const int64_t base = ((int64_t)kSPO * g.P() + hp) * g.nBlocks();
+            for (Plan* Q : {&c.pairs, &c.pairsWave}) {
+                Q->supLo = (int32_t)(uint32_t)(base & 0xffffffffLL);
+                Q->supHi = (int32_t)(base >> 32);
+            }
+            c.supFromPairs = true;
+        }
+    }
     return c;
 }
 

@@ -28,6 +28,10 @@ struct CompiledRule {
     Plan pairs{}, pairsWave{};
     ConflictRefs supRefs{}, supWaveRefs{}, pairsRefs{}, pairsWaveRefs{};
     bool supWaveOk = false, pairsWaveOk = false;
+    // The pair plans can count the support too (Plan::supFused): distinct head
+    // variables, the head object bound at a pair level in both plans, and no
+    // body atom that could ground to the head triple itself.
+    bool supFromPairs = false;
     int64_t supItems = 0, pairItems = 0;
 };
 

@@ -177,6 +177,25 @@ __device__ __forceinline__ Frame key_row(const DevGraph& G, int64_t base, int ke
     return Frame{G.rowStart[r], G.rowStart[r + 1]};
 }
 
+// Fused support: the key-index block run of a plan's head-predicate SPO rows.
+__device__ __forceinline__ int64_t sup_base(const Plan& P) {
+    return (int64_t)(uint32_t)P.supLo | ((int64_t)P.supHi << 32);
+}
+
+// Thread mode, fused support: how many of seed s's distinct head objects
+// (set) are objects of s in the head predicate -- the supported head triples
+// of s. One binary search in s's head row per object.
+__device__ __forceinline__ uint32_t head_hits(const DevGraph& G, int64_t base, int s, const RegSet& set) {
+    const Frame row = key_row(G, base, s);
+    uint32_t n = 0;
+    if (row.i < row.end) {
+#pragma unroll
+        for (int j = 0; j < RegSet::kCap; ++j)
+            if (j < set.n && find_in_row(G.col, row.i, row.end, set.e[j]) >= 0) ++n;
+    }
+    return n;
+}
+
 // Entry range of a level under the current bindings (ISECT: its row 1).
 template <int NS>
 __device__ __forceinline__ Frame enter_level(const Level& lv, const DevGraph& G, const int (&v)[NS]) {
@@ -418,11 +437,13 @@ __device__ __forceinline__ bool leaf_fast_ok(const Plan& P, int last, bool pairs
 // runs its own depth-first search; a lane whose seed is finished takes the
 // next one (ballot prefix count), so no lane waits for the slowest seed.
 // Returns this lane's count; escalated seeds go to the heavy queue. K is the
-// plan kind of every plan the launch sees (kSupport or kPairs).
+// plan kind of every plan the launch sees (kSupport or kPairs). With `fuse`,
+// a PAIRS plan with supFused adds each finished seed's supported head triples
+// to `sup` (seeds moved to the heavy queue count there instead).
 template <int K, int NS>
 __device__ __forceinline__ uint64_t thread_chunk(ThreadLdsT<levels_of(NS)>& W, const DevGraph& G, const ConflictRefs* R,
                                                  uint32_t planIdx, uint32_t begin, uint32_t count, HeavyItem* hq,
-                                                 uint32_t* hCount, uint32_t hCap) {
+                                                 uint32_t* hCount, uint32_t hCap, bool fuse, uint64_t& sup) {
     const int lane = lane_id();
     const Plan& P = W.plan;
     // plan-wide fields as wave-uniform values (scalar registers)
@@ -431,6 +452,8 @@ __device__ __forceinline__ uint64_t thread_chunk(ThreadLdsT<levels_of(NS)>& W, c
     const int hoRef = uniform(P.hoRef);
     const int budget0 = uniform(P.budget);
     const bool pairs = K == kPairs && pl >= 0;
+    const bool fs = pairs && fuse && uniform(P.supFused) != 0;   // also count the seeds' supported head triples
+    const int64_t supBase = (int64_t)(uint32_t)uniform(P.supLo) | ((int64_t)uniform(P.supHi) << 32);
     const uint64_t below = (1ull << lane) - 1ull;
     int v[NS];
     RegSet set;
@@ -467,6 +490,7 @@ __device__ __forceinline__ uint64_t thread_chunk(ThreadLdsT<levels_of(NS)>& W, c
                 if (d == 0) {
                     active = false;
                     if (pairs) mine += (uint64_t)set.n;
+                    if (fs) sup += head_hits(G, supBase, v[0], set);
                 } else {
                     --d;
                 }
@@ -627,6 +651,15 @@ struct Dedup {
         if (D == kDedupSort) return x;
         return W->plan.hoByEntry ? crank[i] : rank(x);
     }
+    // Rank of x if it is a key of the head-object space, else -1.
+    __device__ __forceinline__ int rank_key(int x) const {
+        const Plan& P = W->plan;
+        const int64_t base = (int64_t)(uint32_t)P.hoLo | ((int64_t)P.hoHi << 32);
+        const KeyBlock kb = blocks[base + (x >> 6)];
+        const uint64_t bit = 1ull << (x & 63);
+        if (!(kb.bits & bit)) return -1;
+        return (int)(kb.rank + (uint32_t)__popcll(kb.bits & (bit - 1))) - P.hoRow0;
+    }
     __device__ __forceinline__ bool seen(int x) const { return seen_r(rank(x)); }
     __device__ __forceinline__ uint32_t insert(int x) { return insert_r(rank(x)); }
     __device__ __forceinline__ bool seen_r(int x) const {
@@ -662,6 +695,25 @@ struct Dedup {
         return bits[x >> 5];
     }
 };
+
+// Wave mode, fused support: how many head objects marked in a PAIRS seed's
+// bitmap are objects of seed s in the head predicate (supported head
+// triples). `row`: a giant's HBM row; null: the wave's own bitmap. Called by
+// the whole wave after its marks; returns the wave's total.
+template <int D, int NS>
+__device__ __forceinline__ uint64_t head_hits_wave(const Dedup<D, NS>& dd, const DevGraph& G, int s,
+                                                   const uint32_t* row) {
+    const Frame hr = key_row(G, sup_base(dd.W->plan), s);
+    uint64_t n = 0;
+    for (int i = hr.i + lane_id(); i < hr.end; i += kWave) {
+        const int r = dd.rank_key(G.col[i]);
+        if (r < 0) continue;   // not a value the pair level can bind: no body pair
+        const uint32_t w = row ? __hip_atomic_load(row + (r >> 5), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                               : dd.word_r(r);
+        n += (w >> (r & 31)) & 1u;
+    }
+    return wave_sum(n);
+}
 
 // Marks the head objects of a batch of last-level groundings (ho[k] >= 0;
 // -1: none), with their ranks hr[k] (bitmap paths). The word tests are
@@ -747,6 +799,9 @@ constexpr int kLeafGeneric = 4;
 //     as a giant and abandoned (`aborted`; its partial count is dropped).
 //     A SUPPORT sub-item shares its giant's `found` flag: it polls it and
 //     counts only if it is the one to set it.
+//   * supSeed >= 0 (fused support, bitmap dedup): before the bitmap is
+//     cleared, the supported head triples of seed supSeed are added to
+//     supOut (nothing if the seed was abandoned).
 // The pool (poolLevel/Next/End/V) is set up by the caller, all lanes idle.
 // Returns the count this wave adds (SUPPORT: 0/1; PAIRS: distinct head
 // objects it marked first, or 0 on the sort path, which counts later).
@@ -755,7 +810,7 @@ __device__ __forceinline__ uint64_t wave_run(WaveLdsT<levels_of(NS)>& W, const D
                                              Dedup<D, NS>& dd,
                                              int64_t rowWords, const SortSink& sink, unsigned long long keyHi,
                                              const GiantArgs& ga, bool mayGiant, HeavyItem self, uint32_t heavyIdx,
-                                             uint32_t* foundFlag, bool& aborted,
+                                             uint32_t* foundFlag, bool& aborted, int supSeed, uint64_t& supOut,
                                              unsigned long long (&stats)[kWaveStats]) {
     const int lane = lane_id();
     const Plan& P = W.plan;
@@ -1071,6 +1126,7 @@ __device__ __forceinline__ uint64_t wave_run(WaveLdsT<levels_of(NS)>& W, const D
     if (D == kDedupSort) return 0;
     // clear the wave's own bitmap (a sub-item marked only its giant's row: nothing listed)
     wave_sync();
+    if (supSeed >= 0 && !aborted) supOut += head_hits_wave(dd, G, supSeed, nullptr);
     if (D == kDedupLds) {
         // a giant's sub-item ORs its bitmap into the giant's row: the bits
         // that OR sets first are the objects it adds (each bit is set first
@@ -1102,7 +1158,8 @@ template <int K, int D, int NS>
 __device__ __forceinline__ uint64_t wave_seed(WaveLdsT<levels_of(NS)>& W, const DevGraph& G, const ConflictRefs* R,
                                               uint32_t item, Dedup<D, NS>& dd, int64_t rowWords, const SortSink& sink,
                                               unsigned long long keyHi, const GiantArgs& ga, uint32_t planIdx,
-                                              uint32_t heavyIdx, unsigned long long (&stats)[kWaveStats]) {
+                                              uint32_t heavyIdx, bool fuse, uint64_t& sup,
+                                              unsigned long long (&stats)[kWaveStats]) {
     const int lane = lane_id();
     const Plan& P = W.plan;
     int v[NS];
@@ -1121,8 +1178,9 @@ __device__ __forceinline__ uint64_t wave_seed(WaveLdsT<levels_of(NS)>& W, const 
         for (int k = lane; k < kCacheSlots; k += kWave) dd.cache[k] = -1;
     wave_sync();
     bool aborted = false;
+    const int supSeed = (K == kPairs && D != kDedupSort && fuse && P.supFused) ? uniform(v[0]) : -1;
     return wave_run<K, D, NS>(W, G, R, dd, rowWords, sink, keyHi, ga, ga.after > 0, HeavyItem{planIdx, item}, heavyIdx,
-                          nullptr, aborted, stats);
+                          nullptr, aborted, supSeed, sup, stats);
 }
 
 // ------------------------------------------------------------- kernels
@@ -1139,15 +1197,21 @@ __global__ void __launch_bounds__(kBlock) k_thread_batch(ThreadArgs a, DevGraph 
         const uint32_t c = wave_take(a.cursor);
         if (c >= a.nChunks) break;
         const Chunk ch = a.chunks[c];
-        if (ch.gate && a.counts[2 * a.plans[ch.plan].rule + kSupport] == 0) continue;
+        if ((ch.gate == 1u || (ch.gate == 2u && !a.fuse)) && a.counts[2 * a.plans[ch.plan].rule + kSupport] == 0)
+            continue;
         if (ch.plan != loaded) {
             copy_plan(W, a.plans + ch.plan);
             loaded = ch.plan;
         }
+        uint64_t sup = 0;
         const uint64_t mine = thread_chunk<K, NS>(W, G, a.refs + ch.plan, ch.plan, ch.begin, ch.count, a.heavy,
-                                           a.heavyCount, a.heavyCap);
+                                           a.heavyCount, a.heavyCap, a.fuse != 0, sup);
         const uint64_t tot = wave_sum(mine);
         if (lane == 0 && tot) atomicAdd(a.counts + 2 * W.plan.rule + K, (unsigned long long)tot);
+        if (K == kPairs && a.fuse && W.plan.supFused) {
+            const uint64_t ts = wave_sum(sup);
+            if (lane == 0 && ts) atomicAdd(a.counts + 2 * W.plan.rule + kSupport, (unsigned long long)ts);
+        }
     }
 }
 
@@ -1163,7 +1227,9 @@ __global__ void __launch_bounds__(kBlock) k_thread_one(Plan P, OneArgs a, DevGra
     for (uint32_t base = ((blockIdx.x * blockDim.x + threadIdx.x) / kWave) * kOneChunkSeeds; base < a.nItems;
          base += waves * kOneChunkSeeds) {
         const uint32_t cnt = a.nItems - base < (uint32_t)kOneChunkSeeds ? a.nItems - base : (uint32_t)kOneChunkSeeds;
-        const uint64_t mine = thread_chunk<K, NS>(W, G, a.refs, 0u, base, cnt, a.heavy, a.heavyCount, a.heavyCap);
+        uint64_t sup = 0;
+        const uint64_t mine =
+            thread_chunk<K, NS>(W, G, a.refs, 0u, base, cnt, a.heavy, a.heavyCount, a.heavyCap, false, sup);
         const uint64_t tot = wave_sum(mine);
         if (lane == 0 && tot) atomicAdd(a.count, (unsigned long long)tot);
     }
@@ -1222,9 +1288,12 @@ __global__ void __launch_bounds__(kWave) k_wave_batch(WaveArgs a, DevGraph G) {
             copy_plan(W, a.plans + it.plan);
             loaded = it.plan;
         }
+        uint64_t sup = 0;
         const uint64_t c = wave_seed<K, D, NS>(W, G, a.refs + it.plan, it.item, dd, a.rowWords, a.sink,
-                                           (unsigned long long)h << a.sink.hoBits, a.giant, it.plan, h, stats);
+                                           (unsigned long long)h << a.sink.hoBits, a.giant, it.plan, h, a.fuse != 0,
+                                           sup, stats);
         if (lane == 0 && c) atomicAdd(a.counts + 2 * W.plan.rule + K, (unsigned long long)c);
+        if (lane == 0 && sup) atomicAdd(a.counts + 2 * W.plan.rule + kSupport, (unsigned long long)sup);
     }
     flush_stats(a.giant, stats);
 }
@@ -1244,8 +1313,9 @@ __global__ void __launch_bounds__(kWave) k_wave_one(Plan P, WaveArgs a, DevGraph
     for (;;) {
         const uint32_t h = wave_take(a.cursor);
         if (h >= n) break;
+        uint64_t sup = 0;
         const uint64_t c = wave_seed<K, D, NS>(W, G, a.refs, a.heavy[h].item, dd, a.rowWords, a.sink,
-                                           (unsigned long long)h << a.sink.hoBits, a.giant, 0u, h, stats);
+                                           (unsigned long long)h << a.sink.hoBits, a.giant, 0u, h, false, sup, stats);
         if (lane == 0 && c) atomicAdd(a.counts, (unsigned long long)c);
     }
     flush_stats(a.giant, stats);
@@ -1349,7 +1419,8 @@ __global__ void __launch_bounds__(kWave) k_giant_expand(GiantArgs ga, const Plan
 
 // The sub-items of all giants, over all waves: each one is a pool (its range
 // and bindings) run by wave_run against the giant's shared row or found
-// flag. The sub-item that finishes a giant's last clears its row.
+// flag. The sub-item that finishes a giant's last clears its row, after
+// counting the seed's supported head triples from it (fused support).
 template <int K, int D, int NS>
 __global__ void __launch_bounds__(kWave) k_wave_sub(WaveArgs a, DevGraph G) {
     __shared__ WaveLdsT<levels_of(NS)> W;
@@ -1396,16 +1467,26 @@ __global__ void __launch_bounds__(kWave) k_wave_sub(WaveArgs a, DevGraph G) {
             dd.shared = (K == kPairs && D == kDedupRows) ? row : nullptr;   // marks go straight to the row
             dd.merge = (K == kPairs && D == kDedupLds) ? row : nullptr;     // marks go to LDS, merged at the end
             bool aborted = false;
+            uint64_t none = 0;
             c = wave_run<K, D, NS>(W, G, a.refs + plan, dd, a.rowWords, a.sink, (unsigned long long)hv << a.sink.hoBits,
-                               ga, false, HeavyItem{plan, 0u}, hv, foundFlag, aborted, stats);
+                               ga, false, HeavyItem{plan, 0u}, hv, foundFlag, aborted, -1, none, stats);
             dd.shared = nullptr;
             dd.merge = nullptr;
         }
         if (lane == 0 && c) atomicAdd(a.counts + 2 * W.plan.rule + K, (unsigned long long)c);
+        // fused support: the giant's row is complete once its last sub-item
+        // is done; that sub-item counts the seed's supported head triples
+        const bool fs = K == kPairs && D != kDedupSort && a.fuse && W.plan.supFused;
+        if (fs) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         uint32_t last = 0;
         if (lane == 0) last = atomicSub(ga.remaining + g, 1u) == 1u ? 1u : 0u;
         if (bcast(last) && K == kPairs && D != kDedupSort) {
             uint32_t* row = ga.rows + (int64_t)g * a.rowWords;
+            if (fs) {
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                const uint64_t sup = head_hits_wave(dd, G, uniform(S.v[0]), row);
+                if (lane == 0 && sup) atomicAdd(a.counts + 2 * W.plan.rule + kSupport, (unsigned long long)sup);
+            }
             for (int64_t w = lane; w < a.rowWords; w += kWave) row[w] = 0u;   // read again only by later launches
         }
     }

@@ -12,6 +12,9 @@
 //   ISECT             an EXT candidate, plus a second row lookup per level
 //                     entered and 4 B per row-2 probe of the forward cursor
 //                     (gallop + binary search); the CHECK after it costs nothing
+//   fused support     PAIRS plans with supFused, per seed: the head row lookup
+//                     of the seed (as a row lookup), then a binary search of
+//                     it (as a CHECK probe) per distinct head object
 // It returns no counts to callers; the counting path is the device's.
 #include <algorithm>
 #include <atomic>
@@ -214,6 +217,27 @@ struct Walker {
         const int64_t ks = P.seedBegin + item;
         v[0] = ks < (int64_t)g.keysS.size() ? g.keysS[ks] : g.keysO[ks - g.keysS.size()];
         dfs(0, P.pairLevel >= 0);
+        if (P.supFused && P.pairLevel >= 0) head_hits();
+    }
+
+    // Fused support: the seed's head row, then one binary search of it per
+    // distinct head object found.
+    void head_hits() {
+        const int64_t base = (int64_t)(uint32_t)P.supLo | ((int64_t)P.supHi << 32);
+        int64_t lo, hi;
+        key_row(base, v[0], lo, hi);
+        if (lo >= hi) return;
+        for (int64_t w : touched)
+            for (uint64_t m = bits[w]; m; m &= m - 1) {
+                const int x = (int)(w * 64 + __builtin_ctzll(m));
+                int64_t l = lo, h = hi;
+                while (l < h) {
+                    const int64_t mid = (l + h) >> 1;
+                    bytes += 4;
+                    if (col(mid) < x) l = mid + 1; else h = mid;
+                }
+                if (l < hi) bytes += 4;
+            }
     }
 };
 

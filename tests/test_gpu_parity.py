@@ -272,6 +272,49 @@ def test_giant_seeds_random_rules(seed):
         eng.close()
 
 
+@pytest.mark.parametrize("seed", range(3))
+def test_fused_support_vs_oracle(seed):
+    """Support counted in the body-pair pass (fuse_support): rules over two
+    predicates, so body atoms on the head predicate are common -- most cannot
+    ground to the head triple (the rule is fused), some can (it keeps its
+    support pass). Bitmap rows, sort-unique (never fused) and LDS bitmaps;
+    thread mode, wave mode and giant sub-items; gated and ungated; fusion on
+    and off: counts equal the oracle's."""
+    say("fused-support", seed)
+    g = W.tiny_random(9600 + seed, n_entities=14 + seed, n_predicates=2, n_triples=90 + 10 * seed)
+    rng = np.random.default_rng(600 + seed)
+    rules = [W.random_rule(rng, g.n_entities, g.n_predicates, 1 + k % 5, n_vars=3 + k % 3, p_const=0.1)
+             for k in range(150)]
+    exp = Oracle(g.s, g.p, g.o).eval(W.encode_rules(rules))
+    eng = L.Engine.from_graph(g)
+    try:
+        words = L.rule_words_for_graph(rules, g)
+        for dedup in (1, 2, 3):
+            eng.set_tuning("dedup", dedup)
+            for budget, after in ((4096, 2048), (1, 2048), (1, 1)):
+                eng.set_tuning("budget", budget)
+                eng.set_tuning("giant_after", after)
+                for fuse in (1, 0):
+                    eng.set_tuning("fuse_support", fuse)
+                    assert eng.get_tuning("fuse_active") == (1 if fuse and dedup != 2 else 0)
+                    for flags in (L.UNGATED, 0):
+                        res = eng.evaluate(words, flags)
+                        fused = eng.get_tuning("last_fused")
+                        if fuse and dedup != 2:
+                            assert fused > 0, (seed, dedup, budget, after, flags)
+                        else:
+                            assert fused == 0, (seed, dedup, budget, after, flags)
+                        for i in range(len(rules)):
+                            tag = (seed, dedup, budget, after, fuse, flags, i, W.rule_text(rules[i]))
+                            assert res[i]["support"] == exp[i][0], tag
+                            assert res[i]["bodySize"] == exp[i][4], tag
+                            if flags == L.UNGATED:
+                                assert res[i]["bodyPairs"] == exp[i][3], tag
+    finally:
+        eng.set_tuning("fuse_support", 1)
+        eng.close()
+
+
 @pytest.mark.parametrize("name", ["tiny_4", "dense_conf_300", "dbpedia_shaped_5k"])
 def test_snapshot_context_matches(name):
     """A context loaded from a snapshot of a term-id graph gives the golden
@@ -385,7 +428,10 @@ def test_device_build_turtle(monkeypatch):
 def test_overlapped_phases_replay(name):
     """Graph replays of an ungated batch with the support and body-pair
     phases on two streams: every replay, interleaved with eager passes and
-    replays of the sequential graph, equals the eager result."""
+    replays of the sequential graph, equals the eager result. Support fusion
+    is switched off on alternate pairs of replays (with it, these batches
+    have few or no support chunks left to overlap), which also recaptures
+    the graph."""
     say("overlap", name)
     fx, g, rules = goldens.load(name)
     eng = L.Engine.from_graph(g)
@@ -396,12 +442,14 @@ def test_overlapped_phases_replay(name):
         check(fx, rules, eager, False, name + "/eager")
         eng.set_tuning("use_graph", 1)
         b = eng.batch(words, L.UNGATED)
-        for k in range(6):
+        for k in range(8):
             eng.set_tuning("overlap", k % 2)
+            eng.set_tuning("fuse_support", (k // 2) % 2)
             assert np.array_equal(b.run(), eager), (name, k)
             other = eng.evaluate(words[: 10 + 5 * k], 0)
             check(fx, rules[: 10 + 5 * k], other, True, name + "/interleaved")
         b.close()
     finally:
         eng.set_tuning("overlap", 1)
+        eng.set_tuning("fuse_support", 1)
         eng.close()

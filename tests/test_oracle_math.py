@@ -11,7 +11,8 @@ from oracle import Oracle
 
 
 def brute(g, rule):
-    """Direct restatement of the definitions by exhaustive assignment."""
+    """Direct restatement of the definitions by exhaustive assignment:
+    (support, set of distinct body pairs)."""
     T = set(zip(g.s.tolist(), g.p.tolist(), g.o.tolist()))
     head, body = rule
     vars_ = W.rule_vars(rule)
@@ -60,7 +61,7 @@ def brute(g, rule):
             b = dict(zip(bv, combo))
             if body_ok(b, []):
                 pairs.add((b[hv[0][1]], b[hv[1][1]]))
-    return support, len(pairs)
+    return support, pairs
 
 
 @pytest.mark.parametrize("seed", range(8))
@@ -75,8 +76,62 @@ def test_oracle_equals_brute_force(seed):
     for i, r in enumerate(rules):
         sup, pairs = brute(g, r)
         assert res[i][0] == sup, (i, r, res[i], sup)
-        assert res[i][3] == pairs, (i, r, res[i], pairs)
-        assert res[i][4] == (pairs if sup > 0 else 0)
+        assert res[i][3] == len(pairs), (i, r, res[i], pairs)
+        assert res[i][4] == (len(pairs) if sup > 0 else 0)
+
+
+def _term_eq(a, b):
+    """'never' / 'maybe' / 'always': can terms a and b take the same value
+    under an injective binding (distinct variables never do; a variable may
+    equal a constant)?"""
+    if a[0] == "v" and b[0] == "v":
+        return "always" if a[1] == b[1] else "never"
+    if a[0] == "c" and b[0] == "c":
+        return "always" if a[1] == b[1] else "never"
+    return "maybe"
+
+
+def _fusable(rule):
+    """The library's condition for counting support in the body-pair pass:
+    distinct head variables, both in the body, and no body atom on the head
+    predicate that can ground to the head triple."""
+    head, body = rule
+    if head[0][0] != "v" or head[2][0] != "v" or head[0][1] == head[2][1]:
+        return False
+    inbody = {t[1] for x in body for t in (x[0], x[2]) if t[0] == "v"}
+    if head[0][1] not in inbody or head[2][1] not in inbody:
+        return False
+    for x in body:
+        if x[1] == head[1] and _term_eq(x[0], head[0]) != "never" and _term_eq(x[2], head[2]) != "never":
+            return False
+    return True
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_support_from_pairs_identity(seed):
+    """For every fusable rule, support = #{distinct body pairs (s, o) with
+    (s, p_head, o) a triple}, by exhaustive assignment; the identity fails
+    for some rules the condition excludes (a body atom that can reuse the
+    head triple), so the exclusion is needed."""
+    g = W.tiny_random(700 + seed, n_entities=5 + seed % 2, n_predicates=2, n_triples=18 + seed)
+    T = set(zip(g.s.tolist(), g.p.tolist(), g.o.tolist()))
+    rng = np.random.default_rng(70 + seed)
+    a, b, c = W.V("a"), W.V("b"), W.V("c")
+    rules = [((a, 0, b), [(a, 0, b)]), ((a, 0, b), [(a, 0, c), (c, 1, b)]), ((a, 0, b), [(b, 0, a)])]
+    rules += [W.random_rule(rng, g.n_entities, g.n_predicates, 1 + k % 4, n_vars=3 + k % 2, p_const=0.15)
+              for k in range(40)]
+    fused = broken = 0
+    for r in rules:
+        sup, pairs = brute(g, r)
+        hp = r[0][1]
+        hits = sum((s, hp, o) in T for (s, o) in pairs)
+        if _fusable(r):
+            fused += 1
+            assert sup == hits, (r, sup, hits)
+        elif r[0][0][0] == "v" and r[0][2][0] == "v" and r[0][0][1] != r[0][2][1] and sup != hits:
+            broken += 1
+    assert fused >= 10
+    assert broken >= 1   # (?a p ?b) <= (?a p ?b): support 0, every p pair a hit
 
 
 def _rand_graph(seed, n=40, P=3, m=500):
