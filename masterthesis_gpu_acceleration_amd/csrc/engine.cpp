@@ -100,6 +100,7 @@ bool Context::upload() {
     dg.coo = coo.as<int32_t>();
     dg.keys = keys.as<int32_t>();
     dg.crank = crank.as<int32_t>();
+    dg.nBlocks = g.nBlocks();
     if (!one.ensure(512)) return false;
     GR_HIP(hipMemset(one.p, 0, 512));
     return ensure_dedup();
@@ -296,16 +297,16 @@ bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const s
     struct Job { int ph; size_t k; int64_t n; bool fus; };
     std::vector<Job> jobs;
     auto add = [&](int ph, size_t i, const Plan& P, const ConflictRefs& R, const Plan& W, const ConflictRefs& RW,
-                   int64_t n, bool fus) {
+                   int64_t n, int32_t supPred) {
         const size_t k = 2 * i + ph;
         plans[k] = P;
         wplans[k] = W;
         refs[k] = R;
         wrefs[k] = RW;
         plans[k].rule = wplans[k].rule = (int32_t)i;
-        plans[k].supFused = wplans[k].supFused = (ph == kPairs && fus) ? 1 : 0;
+        plans[k].supPred = wplans[k].supPred = ph == kPairs ? supPred : -1;
         B.slots[ph] = std::max({B.slots[ph], plan_slots(P, R), plan_slots(W, RW)});
-        jobs.push_back(Job{ph, k, n, fus});
+        jobs.push_back(Job{ph, k, n, supPred >= 0});
         items[ph] += (uint64_t)n;
     };
     for (size_t i = 0; i < rules.size(); ++i) {
@@ -315,11 +316,11 @@ bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const s
         B.status[i] = cr.status;
         if (cr.status != kRuleOk) continue;
         // the pairs pass may count this rule's support (a fused launch then skips its support chunks)
-        const bool fus = cr.supFromPairs && cr.supItems > 0 && cr.pairItems > 0;
+        const int32_t supPred = (cr.supPred >= 0 && cr.supItems > 0 && cr.pairItems > 0) ? cr.supPred : -1;
         if (cr.supportLaunch && cr.supItems > 0)
-            add(0, i, cr.sup, cr.supRefs, cr.supWave, cr.supWaveRefs, cr.supItems, fus);
+            add(0, i, cr.sup, cr.supRefs, cr.supWave, cr.supWaveRefs, cr.supItems, supPred);
         if (cr.pairsLaunch && cr.pairItems > 0)
-            add(1, i, cr.pairs, cr.pairsRefs, cr.pairsWave, cr.pairsWaveRefs, cr.pairItems, fus);
+            add(1, i, cr.pairs, cr.pairsRefs, cr.pairsWave, cr.pairsWaveRefs, cr.pairItems, supPred);
     }
     // Seeds per thread-mode work unit: 1024 (fewer plan loads and cursor
     // atomics) unless that leaves fewer than two units per thread-mode wave,
@@ -1055,8 +1056,8 @@ int gpu_plan_traffic(void* handle, const GpuRuleDesc* rules, int32_t n, int64_t*
             CompiledRule& cr = crs[i];
             for (int k = 0; k < 4; ++k) out[4 * i + k] = 0;
             if (cr.status != kRuleOk) continue;
-            const bool fus = fuse && cr.supFromPairs && cr.supItems > 0 && cr.pairItems > 0;
-            cr.pairs.supFused = fus ? 1 : 0;
+            const bool fus = fuse && cr.supPred >= 0 && cr.supItems > 0 && cr.pairItems > 0;
+            cr.pairs.supPred = fus ? cr.supPred : -1;
             if (cr.supportLaunch && !fus) {
                 jobs.push_back(TrafficJob{&cr.sup, &cr.supRefs, cr.supItems});
                 slot.push_back(4 * (int64_t)i);

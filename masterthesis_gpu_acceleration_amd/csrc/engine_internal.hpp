@@ -163,10 +163,10 @@ struct Batch : Handle {
     uint32_t heavyCap[2] = {};
     uint64_t heavyOff[2] = {};
     // support chunks of rules whose support a fused pairs launch counts
-    // (CompiledRule::supFromPairs); they end the support chunk list, so such
+    // (CompiledRule::supPred); they end the support chunk list, so such
     // a launch runs only the first nChunks[0] - nFusable
     uint32_t nFusable = 0;
-    int64_t nFused = 0;   // rules with a fusable support (pairs plans with supFused)
+    int64_t nFused = 0;   // rules with a fusable support (pairs plans with supPred >= 0)
     uint32_t capRules = 0;
     int slots[2] = {1, 1};   // variable slots of the plans of each phase (picks the kernels' slot class)
     double kernelMs[4] = {};

@@ -19,7 +19,7 @@ struct ThreadArgs {
     unsigned long long* counts;   // [2*rule + kind]
     HeavyItem* heavy;
     uint32_t* heavyCount;
-    int32_t fuse;                 // PAIRS: plans with supFused also count support (into [2*rule + kSupport])
+    int32_t fuse;                 // PAIRS: plans with supPred >= 0 also count support (into [2*rule + kSupport])
 };
 
 // Thread-mode pass over all seeds of one plan given by value.
@@ -100,7 +100,7 @@ struct WaveArgs {
     const HeavyItem* heavy;
     const uint32_t* heavyCount;
     uint32_t heavyCap;
-    uint32_t fuse;                // batch PAIRS launches: plans with supFused also count support (bitmap dedup only)
+    uint32_t fuse;                // batch PAIRS launches: plans with supPred >= 0 also count support (bitmap dedup only)
     uint32_t* cursor;
     unsigned long long* counts;   // batch: [2*rule + kind]; by-value kernel: counts[0]
     uint32_t* rows;               // per-wave dedup rows, all zero between seeds (kDedupRows)

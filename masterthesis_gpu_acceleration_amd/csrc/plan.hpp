@@ -81,15 +81,15 @@ struct alignas(16) Plan {
     // hoLo | hoHi << 32, global row number hoRow0 of its first key, hoKeys keys.
     int32_t hoLo, hoHi, hoRow0, hoKeys;
     int32_t hoByEntry;   // 1: the pair level binds the head object as col[e], whose rank is crank[e]
-    // PAIRS, supFused = 1: the plan can count the rule's support as well (when
-    // the launch's `fuse` is set): a head triple (s, p_head, o) is supported
-    // iff (s, o) is a distinct body pair, so each seed s adds the head objects
-    // it found that are objects of s in the head predicate's SPO rows, whose
-    // key-index block run is supLo | supHi << 32. Only rules where no body
-    // atom can ground to the head triple itself qualify (the support search
-    // excludes that triple, the pair search does not).
-    int32_t supFused;
-    int32_t supLo, supHi, pad2, pad3;
+    // PAIRS, supPred >= 0: the plan also counts the rule's support (when the
+    // launch's `fuse` is set): a head triple (s, p_head, o) is supported iff
+    // (s, o) is a distinct body pair, so each seed s adds the head objects it
+    // found that are objects of s in the SPO rows of predicate supPred (the
+    // head predicate; its key-index block run starts at supPred * nBlocks).
+    // Only rules where no body atom can ground to the head triple itself
+    // qualify (the support search excludes that triple, the pair search does
+    // not). -1: no support counting here.
+    int32_t supPred;
     Level lev[kMaxBody];
 };
 
@@ -167,6 +167,7 @@ struct DevGraph {
     // [2T] for entry e of (direction d, predicate p): the rank of col[e] among
     // the keys of (1-d, p) -- the bitmap index of a head object bound by e
     const int32_t* crank;
+    int64_t nBlocks;          // key-index blocks per (direction, predicate): ceil(nE/64)
 };
 
 }  // namespace gr

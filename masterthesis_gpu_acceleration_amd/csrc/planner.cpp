@@ -229,6 +229,7 @@ void head_object_ranks(Plan& Q, const HostGraph& g) {
 
 CompiledRule compile_rule(const RuleIR& r, const HostGraph& g, const Tuning& t) {
     CompiledRule c;
+    for (Plan* Q : {&c.sup, &c.supWave, &c.pairs, &c.pairsWave}) Q->supPred = -1;
     const int nb = (int)r.body.size();
     const int32_t hp = g.dense_pred(r.head.pterm);
 
@@ -373,14 +374,7 @@ CompiledRule compile_rule(const RuleIR& r, const HostGraph& g, const Tuning& t) 
         bool headFree = true;
         for (const AtomIR& a : r.body)
             if (a.pterm == r.head.pterm && atom_eq(a, r.head) != kNever) headFree = false;
-        if (headFree) {
-            const int64_t base = ((int64_t)kSPO * g.P() + hp) * g.nBlocks();
-            for (Plan* Q : {&c.pairs, &c.pairsWave}) {
-                Q->supLo = (int32_t)(uint32_t)(base & 0xffffffffLL);
-                Q->supHi = (int32_t)(base >> 32);
-            }
-            c.supFromPairs = true;
-        }
+        if (headFree) c.supPred = hp;   // the batch sets Plan::supPred when it fuses
     }
     return c;
 }

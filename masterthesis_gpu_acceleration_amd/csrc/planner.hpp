@@ -28,10 +28,11 @@ struct CompiledRule {
     Plan pairs{}, pairsWave{};
     ConflictRefs supRefs{}, supWaveRefs{}, pairsRefs{}, pairsWaveRefs{};
     bool supWaveOk = false, pairsWaveOk = false;
-    // The pair plans can count the support too (Plan::supFused): distinct head
+    // The pair plans can count the support too (Plan::supPred): distinct head
     // variables, the head object bound at a pair level in both plans, and no
-    // body atom that could ground to the head triple itself.
-    bool supFromPairs = false;
+    // body atom that could ground to the head triple itself. The head
+    // predicate, or -1.
+    int32_t supPred = -1;
     int64_t supItems = 0, pairItems = 0;
 };
 

@@ -177,9 +177,10 @@ __device__ __forceinline__ Frame key_row(const DevGraph& G, int64_t base, int ke
     return Frame{G.rowStart[r], G.rowStart[r + 1]};
 }
 
-// Fused support: the key-index block run of a plan's head-predicate SPO rows.
-__device__ __forceinline__ int64_t sup_base(const Plan& P) {
-    return (int64_t)(uint32_t)P.supLo | ((int64_t)P.supHi << 32);
+// Fused support: the key-index block run of a plan's head-predicate SPO rows
+// (SPO runs come first, one per predicate).
+__device__ __forceinline__ int64_t sup_base(const Plan& P, const DevGraph& G) {
+    return (int64_t)P.supPred * G.nBlocks;
 }
 
 // Thread mode, fused support: how many of seed s's distinct head objects
@@ -438,7 +439,7 @@ __device__ __forceinline__ bool leaf_fast_ok(const Plan& P, int last, bool pairs
 // next one (ballot prefix count), so no lane waits for the slowest seed.
 // Returns this lane's count; escalated seeds go to the heavy queue. K is the
 // plan kind of every plan the launch sees (kSupport or kPairs). With `fuse`,
-// a PAIRS plan with supFused adds each finished seed's supported head triples
+// a PAIRS plan with supPred >= 0 adds each finished seed's supported head triples
 // to `sup` (seeds moved to the heavy queue count there instead).
 template <int K, int NS>
 __device__ __forceinline__ uint64_t thread_chunk(ThreadLdsT<levels_of(NS)>& W, const DevGraph& G, const ConflictRefs* R,
@@ -452,8 +453,8 @@ __device__ __forceinline__ uint64_t thread_chunk(ThreadLdsT<levels_of(NS)>& W, c
     const int hoRef = uniform(P.hoRef);
     const int budget0 = uniform(P.budget);
     const bool pairs = K == kPairs && pl >= 0;
-    const bool fs = pairs && fuse && uniform(P.supFused) != 0;   // also count the seeds' supported head triples
-    const int64_t supBase = (int64_t)(uint32_t)uniform(P.supLo) | ((int64_t)uniform(P.supHi) << 32);
+    const bool fs = pairs && fuse && uniform(P.supPred) >= 0;   // also count the seeds' supported head triples
+    const int64_t supBase = (int64_t)uniform(P.supPred) * G.nBlocks;
     const uint64_t below = (1ull << lane) - 1ull;
     int v[NS];
     RegSet set;
@@ -703,7 +704,7 @@ struct Dedup {
 template <int D, int NS>
 __device__ __forceinline__ uint64_t head_hits_wave(const Dedup<D, NS>& dd, const DevGraph& G, int s,
                                                    const uint32_t* row) {
-    const Frame hr = key_row(G, sup_base(dd.W->plan), s);
+    const Frame hr = key_row(G, sup_base(dd.W->plan, G), s);
     uint64_t n = 0;
     for (int i = hr.i + lane_id(); i < hr.end; i += kWave) {
         const int r = dd.rank_key(G.col[i]);
@@ -1178,7 +1179,7 @@ __device__ __forceinline__ uint64_t wave_seed(WaveLdsT<levels_of(NS)>& W, const 
         for (int k = lane; k < kCacheSlots; k += kWave) dd.cache[k] = -1;
     wave_sync();
     bool aborted = false;
-    const int supSeed = (K == kPairs && D != kDedupSort && fuse && P.supFused) ? uniform(v[0]) : -1;
+    const int supSeed = (K == kPairs && D != kDedupSort && fuse && P.supPred >= 0) ? uniform(v[0]) : -1;
     return wave_run<K, D, NS>(W, G, R, dd, rowWords, sink, keyHi, ga, ga.after > 0, HeavyItem{planIdx, item}, heavyIdx,
                           nullptr, aborted, supSeed, sup, stats);
 }
@@ -1208,7 +1209,7 @@ __global__ void __launch_bounds__(kBlock) k_thread_batch(ThreadArgs a, DevGraph 
                                            a.heavyCount, a.heavyCap, a.fuse != 0, sup);
         const uint64_t tot = wave_sum(mine);
         if (lane == 0 && tot) atomicAdd(a.counts + 2 * W.plan.rule + K, (unsigned long long)tot);
-        if (K == kPairs && a.fuse && W.plan.supFused) {
+        if (K == kPairs && a.fuse && W.plan.supPred >= 0) {
             const uint64_t ts = wave_sum(sup);
             if (lane == 0 && ts) atomicAdd(a.counts + 2 * W.plan.rule + kSupport, (unsigned long long)ts);
         }
@@ -1476,7 +1477,7 @@ __global__ void __launch_bounds__(kWave) k_wave_sub(WaveArgs a, DevGraph G) {
         if (lane == 0 && c) atomicAdd(a.counts + 2 * W.plan.rule + K, (unsigned long long)c);
         // fused support: the giant's row is complete once its last sub-item
         // is done; that sub-item counts the seed's supported head triples
-        const bool fs = K == kPairs && D != kDedupSort && a.fuse && W.plan.supFused;
+        const bool fs = K == kPairs && D != kDedupSort && a.fuse && W.plan.supPred >= 0;
         if (fs) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         uint32_t last = 0;
         if (lane == 0) last = atomicSub(ga.remaining + g, 1u) == 1u ? 1u : 0u;

@@ -12,9 +12,12 @@
 //   ISECT             an EXT candidate, plus a second row lookup per level
 //                     entered and 4 B per row-2 probe of the forward cursor
 //                     (gallop + binary search); the CHECK after it costs nothing
-//   fused support     PAIRS plans with supFused, per seed: the head row lookup
-//                     of the seed (as a row lookup), then a binary search of
-//                     it (as a CHECK probe) per distinct head object
+//   fused support     PAIRS plans with supPred >= 0, per seed: the head row lookup
+//                     of the seed (as a row lookup), then, as the device
+//                     checks it: up to 8 distinct head objects (a thread-mode
+//                     seed) a binary search of the row (as a CHECK probe) per
+//                     object; more (only wave mode holds them) 20 B per row
+//                     entry (col + the key-index block of its rank)
 // It returns no counts to callers; the counting path is the device's.
 #include <algorithm>
 #include <atomic>
@@ -217,16 +220,24 @@ struct Walker {
         const int64_t ks = P.seedBegin + item;
         v[0] = ks < (int64_t)g.keysS.size() ? g.keysS[ks] : g.keysO[ks - g.keysS.size()];
         dfs(0, P.pairLevel >= 0);
-        if (P.supFused && P.pairLevel >= 0) head_hits();
+        if (P.supPred >= 0 && P.pairLevel >= 0) head_hits();
     }
 
     // Fused support: the seed's head row, then one binary search of it per
-    // distinct head object found.
+    // distinct head object found (thread mode, at most kThreadSet objects),
+    // or one pass over it (wave mode).
+    static constexpr int64_t kThreadSet = 8;
     void head_hits() {
-        const int64_t base = (int64_t)(uint32_t)P.supLo | ((int64_t)P.supHi << 32);
+        const int64_t base = ((int64_t)kSPO * g.P() + P.supPred) * g.nBlocks();
         int64_t lo, hi;
         key_row(base, v[0], lo, hi);
         if (lo >= hi) return;
+        int64_t n = 0;
+        for (int64_t w : touched) n += __builtin_popcountll(bits[w]);
+        if (n > kThreadSet) {
+            bytes += 20 * (hi - lo);
+            return;
+        }
         for (int64_t w : touched)
             for (uint64_t m = bits[w]; m; m &= m - 1) {
                 const int x = (int)(w * 64 + __builtin_ctzll(m));
