@@ -491,7 +491,7 @@ __device__ __forceinline__ uint64_t thread_chunk(ThreadLdsT<levels_of(NS)>& W, c
                 if (d == 0) {
                     active = false;
                     if (pairs) mine += (uint64_t)set.n;
-                    if (fs) sup += head_hits(G, supBase, v[0], set);
+                    if (fs && set.n > 0) sup += head_hits(G, supBase, v[0], set);
                 } else {
                     --d;
                 }
@@ -698,13 +698,13 @@ struct Dedup {
 };
 
 // Wave mode, fused support: how many head objects marked in a PAIRS seed's
-// bitmap are objects of seed s in the head predicate (supported head
-// triples). `row`: a giant's HBM row; null: the wave's own bitmap. Called by
-// the whole wave after its marks; returns the wave's total.
+// bitmap are objects of the seed in the head predicate (supported head
+// triples); hr is the seed's row there. `row`: a giant's HBM row; null: the
+// wave's own bitmap. Called by the whole wave after its marks; returns the
+// wave's total.
 template <int D, int NS>
-__device__ __forceinline__ uint64_t head_hits_wave(const Dedup<D, NS>& dd, const DevGraph& G, int s,
+__device__ __forceinline__ uint64_t head_hits_wave(const Dedup<D, NS>& dd, const DevGraph& G, Frame hr,
                                                    const uint32_t* row) {
-    const Frame hr = key_row(G, sup_base(dd.W->plan, G), s);
     uint64_t n = 0;
     for (int i = hr.i + lane_id(); i < hr.end; i += kWave) {
         const int r = dd.rank_key(G.col[i]);
@@ -800,9 +800,9 @@ constexpr int kLeafGeneric = 4;
 //     as a giant and abandoned (`aborted`; its partial count is dropped).
 //     A SUPPORT sub-item shares its giant's `found` flag: it polls it and
 //     counts only if it is the one to set it.
-//   * supSeed >= 0 (fused support, bitmap dedup): before the bitmap is
-//     cleared, the supported head triples of seed supSeed are added to
-//     supOut (nothing if the seed was abandoned).
+//   * supOn (fused support, bitmap dedup): before the bitmap is cleared, the
+//     supported head triples of the seed, whose head row is supRow, are
+//     added to supOut (nothing if the seed was abandoned or found no pair).
 // The pool (poolLevel/Next/End/V) is set up by the caller, all lanes idle.
 // Returns the count this wave adds (SUPPORT: 0/1; PAIRS: distinct head
 // objects it marked first, or 0 on the sort path, which counts later).
@@ -811,7 +811,8 @@ __device__ __forceinline__ uint64_t wave_run(WaveLdsT<levels_of(NS)>& W, const D
                                              Dedup<D, NS>& dd,
                                              int64_t rowWords, const SortSink& sink, unsigned long long keyHi,
                                              const GiantArgs& ga, bool mayGiant, HeavyItem self, uint32_t heavyIdx,
-                                             uint32_t* foundFlag, bool& aborted, int supSeed, uint64_t& supOut,
+                                             uint32_t* foundFlag, bool& aborted, bool supOn, Frame supRow,
+                                             uint64_t& supOut,
                                              unsigned long long (&stats)[kWaveStats]) {
     const int lane = lane_id();
     const Plan& P = W.plan;
@@ -1127,7 +1128,7 @@ __device__ __forceinline__ uint64_t wave_run(WaveLdsT<levels_of(NS)>& W, const D
     if (D == kDedupSort) return 0;
     // clear the wave's own bitmap (a sub-item marked only its giant's row: nothing listed)
     wave_sync();
-    if (supSeed >= 0 && !aborted) supOut += head_hits_wave(dd, G, supSeed, nullptr);
+    if (supOn && !aborted && wave_sum(mine) != 0) supOut += head_hits_wave(dd, G, supRow, nullptr);
     if (D == kDedupLds) {
         // a giant's sub-item ORs its bitmap into the giant's row: the bits
         // that OR sets first are the objects it adds (each bit is set first
@@ -1179,9 +1180,11 @@ __device__ __forceinline__ uint64_t wave_seed(WaveLdsT<levels_of(NS)>& W, const 
         for (int k = lane; k < kCacheSlots; k += kWave) dd.cache[k] = -1;
     wave_sync();
     bool aborted = false;
-    const int supSeed = (K == kPairs && D != kDedupSort && fuse && P.supPred >= 0) ? uniform(v[0]) : -1;
+    // fused support: the seed's head row, looked up now so its loads overlap the search
+    const bool supOn = K == kPairs && D != kDedupSort && fuse && P.supPred >= 0;
+    const Frame supRow = supOn ? key_row(G, sup_base(P, G), v[0]) : Frame{0, 0};
     return wave_run<K, D, NS>(W, G, R, dd, rowWords, sink, keyHi, ga, ga.after > 0, HeavyItem{planIdx, item}, heavyIdx,
-                          nullptr, aborted, supSeed, sup, stats);
+                          nullptr, aborted, supOn, supRow, sup, stats);
 }
 
 // ------------------------------------------------------------- kernels
@@ -1470,7 +1473,8 @@ __global__ void __launch_bounds__(kWave) k_wave_sub(WaveArgs a, DevGraph G) {
             bool aborted = false;
             uint64_t none = 0;
             c = wave_run<K, D, NS>(W, G, a.refs + plan, dd, a.rowWords, a.sink, (unsigned long long)hv << a.sink.hoBits,
-                               ga, false, HeavyItem{plan, 0u}, hv, foundFlag, aborted, -1, none, stats);
+                               ga, false, HeavyItem{plan, 0u}, hv, foundFlag, aborted, false, Frame{0, 0}, none,
+                               stats);
             dd.shared = nullptr;
             dd.merge = nullptr;
         }
@@ -1485,7 +1489,8 @@ __global__ void __launch_bounds__(kWave) k_wave_sub(WaveArgs a, DevGraph G) {
             uint32_t* row = ga.rows + (int64_t)g * a.rowWords;
             if (fs) {
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-                const uint64_t sup = head_hits_wave(dd, G, uniform(S.v[0]), row);
+                const Frame hr = key_row(G, sup_base(W.plan, G), uniform(S.v[0]));
+                const uint64_t sup = head_hits_wave(dd, G, hr, row);
                 if (lane == 0 && sup) atomicAdd(a.counts + 2 * W.plan.rule + kSupport, (unsigned long long)sup);
             }
             for (int64_t w = lane; w < a.rowWords; w += kWave) row[w] = 0u;   // read again only by later launches

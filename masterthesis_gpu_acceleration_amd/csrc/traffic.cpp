@@ -12,8 +12,8 @@
 //   ISECT             an EXT candidate, plus a second row lookup per level
 //                     entered and 4 B per row-2 probe of the forward cursor
 //                     (gallop + binary search); the CHECK after it costs nothing
-//   fused support     PAIRS plans with supPred >= 0, per seed: the head row lookup
-//                     of the seed (as a row lookup), then, as the device
+//   fused support     PAIRS plans with supPred >= 0, per seed with a body pair:
+//                     the seed's head row lookup (as a row lookup), then, as the device
 //                     checks it: up to 8 distinct head objects (a thread-mode
 //                     seed) a binary search of the row (as a CHECK probe) per
 //                     object; more (only wave mode holds them) 20 B per row
@@ -228,12 +228,13 @@ struct Walker {
     // or one pass over it (wave mode).
     static constexpr int64_t kThreadSet = 8;
     void head_hits() {
+        int64_t n = 0;
+        for (int64_t w : touched) n += __builtin_popcountll(bits[w]);
+        if (n == 0) return;   // no body pair: nothing to look up
         const int64_t base = ((int64_t)kSPO * g.P() + P.supPred) * g.nBlocks();
         int64_t lo, hi;
         key_row(base, v[0], lo, hi);
         if (lo >= hi) return;
-        int64_t n = 0;
-        for (int64_t w : touched) n += __builtin_popcountll(bits[w]);
         if (n > kThreadSet) {
             bytes += 20 * (hi - lo);
             return;
