@@ -958,7 +958,7 @@ int gpu_autotune(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags, 
             return true;
         };
         // coordinate descent: budget, then thread-mode waves, then wave-mode waves
-        const int32_t budgets[] = {256, 1024, 4096, 16384, 65536};
+        const int32_t budgets[] = {64, 128, 256, 1024, 4096, 16384, 65536};
         const int tw[] = {8, 16, 24, 32};
         const int ww[] = {4, 8, 16};
         double bestMs = 1e30, ms = 0;
