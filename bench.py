@@ -180,12 +180,15 @@ def main():
     ap.add_argument("--no-snapshot", action="store_true", help="skip the graph-load leg (triples vs snapshot)")
     ap.add_argument("--no-overlap", action="store_true",
                     help="graph replays run the support and body-pair phases one after the other (per-kernel profiles)")
+    ap.add_argument("--set", action="append", default=[], metavar="KEY=VALUE",
+                    help="extra tuning key for the engine (gpu_set_tuning), applied before autotuning")
     ap.add_argument("--no-fuse", action="store_true",
                     help="every rule's support gets its own pass (no support counting in the body-pair pass)")
     ap.add_argument("--no-per-rule", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tune", default="",
-                    help="BUDGET,THREAD_WAVES_PER_CU,WAVE_WAVES_PER_CU[,DEDUP[,GIANT_AFTER]]: fixed tuning, no autotune")
+                    help="BUDGET,THREAD_WAVES_PER_CU,WAVE_WAVES_PER_CU[,DEDUP[,GIANT_AFTER[,CHUNKS_PER_WAVE]]]: fixed "
+                         "tuning, no autotune")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run the streaming copy yardstick over this many bytes (counter scaling) and exit")
     args = ap.parse_args()
@@ -225,17 +228,22 @@ def main():
         eng.set_tuning("overlap", 0)
     if args.no_fuse:
         eng.set_tuning("fuse_support", 0)
+    for kv in args.set:
+        key, val = kv.split("=", 1)
+        eng.set_tuning(key, int(val))
     if args.tune:
         knobs = [int(x) for x in args.tune.split(",")]
-        for key, val in zip(("budget", "thread_waves_per_cu", "wave_waves_per_cu", "dedup", "giant_after"), knobs):
+        for key, val in zip(("budget", "thread_waves_per_cu", "wave_waves_per_cu", "dedup", "giant_after",
+                             "chunks_per_wave"), knobs):
             eng.set_tuning(key, val)
         log("fixed tuning: %s" % args.tune)
     elif not args.no_autotune:
         t = time.time()
         tl = eng.autotune(words, flags)
         best = tl[np.argmin(tl[:, 3])]
-        log("autotune (%.1fs, %d trials): budget %d, thread waves/CU %d, wave waves/CU %d -> %.3f ms" % (
-            time.time() - t, len(tl), best[0], best[1], best[2], best[3]))
+        log("autotune (%.1fs, %d trials): budget %d, thread waves/CU %d, wave waves/CU %d -> %.3f ms; "
+            "chunks/wave %d, giant_after %d" % (time.time() - t, len(tl), best[0], best[1], best[2], best[3],
+                                                eng.get_tuning("chunks_per_wave"), eng.get_tuning("giant_after")))
         for r in tl:
             log("   trial budget %6d tw %2d ww %2d  %.3f ms  escalated %d  pairs dedup %d  giant_after %d" % tuple(r))
     t = time.time()
@@ -304,7 +312,7 @@ def main():
         "rules_with_support": int((res["support"] > 0).sum()),
         "escalated_seeds": int(heavy),
         "tuning": {k: eng.get_tuning(k) for k in ("budget", "thread_waves_per_cu", "wave_waves_per_cu", "pairs_dedup",
-                                                  "giant_after", "wave_slots", "fuse_active")},
+                                                  "giant_after", "chunks_per_wave", "wave_slots", "fuse_active")},
         "support_fused_rules": eng.get_tuning("last_fused"),
         "e2e_rules_per_s": round(n / e2e_s, 1),
         "batch_build_s": round(build_s, 4),

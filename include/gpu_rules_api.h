@@ -164,7 +164,8 @@ double gpu_copy_probe(void* ctx, int64_t bytes);
 /* Autotuner: times the given rules (a sample of the expected workload)
  * under candidate settings -- thread-mode work budget per seed (which
  * decides how much work stays one-lane-per-seed and how much moves to
- * wave64 mode), thread-mode and wave-mode waves per CU, head-object bitmaps
+ * wave64 mode), thread-mode waves per CU and work units per wave, wave-mode
+ * waves per CU, head-object bitmaps
  * in HBM rows or LDS (auto dedup only), and the steps before a wave-mode
  * seed is cut into sub-items -- by coordinate descent, and keeps the fastest
  * in the context. If `log` is given it receives up to logCap rows of
@@ -177,6 +178,8 @@ int gpu_autotune(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags, 
  * moved to wave mode), "thread_waves_per_cu", "wave_waves_per_cu",
  * "giant_after" (wave-mode steps before a seed is cut into sub-items that
  * all waves share; 0 never),
+ * "chunks_per_wave" (thread-mode work units per thread-mode wave a batch
+ * aims for, default 2; units hold 64 to 1,024 seeds),
  * "use_graph" (0/1), "overlap" (0/1, default 1: graph replays of ungated
  * batches run the support and body-pair phases side by side on two
  * streams), "fuse_support" (0/1, default 1: with bitmap dedup, batches

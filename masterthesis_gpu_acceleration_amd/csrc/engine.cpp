@@ -323,16 +323,16 @@ bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const s
             add(1, i, cr.pairs, cr.pairsRefs, cr.pairsWave, cr.pairsWaveRefs, cr.pairItems, supPred);
     }
     // Seeds per thread-mode work unit: 1024 (fewer plan loads and cursor
-    // atomics) unless that leaves fewer than two units per thread-mode wave,
-    // down to 64 -- a batch of few rules with few, costly seeds each still
-    // spreads over the whole GPU.
+    // atomics) unless that leaves fewer than chunksPerWave units per
+    // thread-mode wave, down to 64 -- a batch of few rules with few, costly
+    // seeds each still spreads over the whole GPU.
     const uint64_t waves = (uint64_t)C.numCUs * (uint64_t)C.threadWavesPerCU;
     B.nFusable = 0;
     B.nFused = 0;
     for (const Job& j : jobs) B.nFused += (j.ph == kPairs && j.fus) ? 1 : 0;
     for (int ph = 0; ph < 2; ++ph) {
         int64_t chunk = kChunkSeeds;
-        while (chunk > kMinChunkSeeds && items[ph] / (uint64_t)chunk < 2 * waves) chunk >>= 1;
+        while (chunk > kMinChunkSeeds && items[ph] / (uint64_t)chunk < (uint64_t)C.chunksPerWave * waves) chunk >>= 1;
         // support chunks of fusable rules go last (pass 1)
         for (int pass = 0; pass < (ph == kSupport ? 2 : 1); ++pass)
             for (const Job& j : jobs) {
@@ -986,6 +986,14 @@ int gpu_autotune(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags, 
                 if (twMs[q] <= twMin * 1.02) { bestTw = tw[q]; bestMs = twMs[q]; }
         }
         C->threadWavesPerCU = bestTw;
+        // thread-mode work units per wave (unit size)
+        int bestCpw = C->chunksPerWave;
+        for (int cpw : {2, 4, 8, 16}) {
+            C->chunksPerWave = cpw;
+            if (!trial(ms)) return -1;
+            if (ms < bestMs) { bestMs = ms; bestCpw = cpw; }
+        }
+        C->chunksPerWave = bestCpw;
         int bestWw = C->waveWavesPerCU;
         for (int w : ww) {
             C->waveWavesPerCU = w;
@@ -1095,6 +1103,7 @@ int gpu_set_tuning(void* ctx, const char* key, int64_t value) {
     } else if (k == "use_graph") C->useGraph = value != 0;
     else if (k == "overlap") C->overlap = value != 0;
     else if (k == "fuse_support") C->fuseSupport = value != 0;
+    else if (k == "chunks_per_wave") C->chunksPerWave = (int)std::min<int64_t>(1024, std::max<int64_t>(1, value));
     else if (k == "dedup") {
         if (value < 0 || value > 3) return -1;
         C->dedupMode = (int)value;
@@ -1123,6 +1132,7 @@ int64_t gpu_get_tuning(void* ctx, const char* key) {
     if (k == "use_graph") return C->useGraph ? 1 : 0;
     if (k == "overlap") return C->overlap ? 1 : 0;
     if (k == "fuse_support") return C->fuseSupport ? 1 : 0;
+    if (k == "chunks_per_wave") return C->chunksPerWave;
     if (k == "fuse_active") return C->fuseActive(C->pairsDedup()) ? 1 : 0;
     if (k == "num_cus") return C->numCUs;
     if (k == "wave_slots") return C->waveSlots;

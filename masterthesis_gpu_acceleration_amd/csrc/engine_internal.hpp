@@ -233,6 +233,9 @@ struct Context : Handle {
     // Count the support of qualifying rules in the body-pair pass instead of
     // a support pass of their own; needs bitmap dedup (rows or LDS).
     bool fuseSupport = true;
+    // Thread-mode work units per thread-mode wave a batch aims for (smaller
+    // units when the seeds are few): more units even out the waves' ends.
+    int chunksPerWave = 2;
     bool fuseActive(int mode) const { return fuseSupport && (mode == kDedupRows || mode == kDedupLds); }
     int64_t lastGiants = 0;
     int64_t lastFused = 0;   // rules whose support the last batch counted in its pairs pass

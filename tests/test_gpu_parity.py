@@ -272,6 +272,30 @@ def test_giant_seeds_random_rules(seed):
         eng.close()
 
 
+@pytest.mark.parametrize("name", ["dbpedia_shaped_5k", "dense_conf_300"])
+def test_autotune_keeps_counts(name):
+    """The autotuner times its candidate settings on a batch and keeps the
+    fastest; every setting it keeps is one of its candidates, and counts
+    under the chosen settings equal the goldens."""
+    say("autotune", name)
+    fx, g, rules = goldens.load(name)
+    eng = L.Engine.from_graph(g)
+    try:
+        words = L.rule_words_for_graph(rules, g)
+        log = eng.autotune(words, L.UNGATED)
+        assert len(log) >= 20, len(log)
+        assert eng.get_tuning("budget") in (64, 128, 256, 1024, 4096, 16384, 65536)
+        assert eng.get_tuning("thread_waves_per_cu") in (8, 16, 24, 32)
+        assert eng.get_tuning("chunks_per_wave") in (2, 4, 8, 16)
+        assert eng.get_tuning("wave_waves_per_cu") in (4, 8, 16)
+        assert eng.get_tuning("giant_after") in (512, 1024, 2048, 4096, 8192)
+        assert eng.get_tuning("use_graph") == 1
+        check(fx, rules, eng.evaluate(words, L.UNGATED), False, name + "/autotuned")
+        check(fx, rules, eng.evaluate(words, 0), True, name + "/autotuned-gated")
+    finally:
+        eng.close()
+
+
 @pytest.mark.parametrize("seed", range(3))
 def test_fused_support_vs_oracle(seed):
     """Support counted in the body-pair pass (fuse_support): rules over two
