@@ -186,7 +186,9 @@ int gpu_autotune(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags, 
  * count the support of rules with distinct head variables and no body atom
  * that can ground to the head triple in the body-pair pass -- a head triple
  * (s, o) is supported iff (s, o) is a distinct body pair -- instead of a
- * support pass of their own), "dedup" (body-pair dedup of wave-mode seeds; the
+ * support pass of their own; in gated batches such rules' body pairs are
+ * then searched even where the support turns out 0), "dedup" (body-pair
+ * dedup of wave-mode seeds; the
  * bitmaps index head objects by rank among the keys of one (direction,
  * predicate): 0 auto -- LDS bitmaps when one fits a block and at least 8
  * wave-mode waves per CU still fit, else HBM bitmap rows unless a row per CU

@@ -296,7 +296,7 @@ def test_autotune_keeps_counts(name):
         eng.close()
 
 
-@pytest.mark.parametrize("seed", range(3))
+@pytest.mark.parametrize("seed", range(6))
 def test_fused_support_vs_oracle(seed):
     """Support counted in the body-pair pass (fuse_support): rules over two
     predicates, so body atoms on the head predicate are common -- most cannot
@@ -307,8 +307,8 @@ def test_fused_support_vs_oracle(seed):
     say("fused-support", seed)
     g = W.tiny_random(9600 + seed, n_entities=14 + seed, n_predicates=2, n_triples=90 + 10 * seed)
     rng = np.random.default_rng(600 + seed)
-    rules = [W.random_rule(rng, g.n_entities, g.n_predicates, 1 + k % 5, n_vars=3 + k % 3, p_const=0.1)
-             for k in range(150)]
+    rules = [W.random_rule(rng, g.n_entities, g.n_predicates, 1 + k % (5 if seed < 3 else 8), n_vars=3 + k % 4,
+                           p_const=0.1) for k in range(150)]
     exp = Oracle(g.s, g.p, g.o).eval(W.encode_rules(rules))
     eng = L.Engine.from_graph(g)
     try:
