@@ -2,10 +2,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <climits>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <string>
 #include <thread>
@@ -280,6 +282,50 @@ bool eval_rule_plain(Context& C, const RuleIR& r, bool gated, GpuRuleResult64& r
 
 constexpr int kCtlWords = 8;   // per phase: thread cursor, heavy count, wave cursor, spare
 
+// Body pairs depend only on a rule's body and on which of its variables are
+// the head subject and object, not on the head predicate. Rules are keyed by
+// their body with the head subject named 0, the head object 1 and the other
+// variables numbered by first appearance, atoms sorted (equal keys: the
+// bodies are equal up to a renaming that fixes the head variables, so their
+// injective groundings -- and distinct pairs -- correspond one to one). The
+// first rule of a key runs the search; from[i] is that rule for the others,
+// -1 for rules with their own search (or none).
+std::vector<int32_t> shared_pairs(const std::vector<RuleIR>& rules, const std::vector<CompiledRule>& comp,
+                                  const std::vector<int32_t>& status) {
+    std::vector<int32_t> from(rules.size(), -1);
+    std::map<std::vector<int64_t>, int32_t> first;
+    std::vector<int32_t> name;
+    std::vector<int64_t> key;
+    for (size_t i = 0; i < rules.size(); ++i) {
+        if (status[i] != kRuleOk || !comp[i].pairsLaunch || comp[i].pairItems <= 0) continue;
+        const RuleIR& r = rules[i];
+        if (!r.head.s.var || !r.head.o.var || r.head.s.v == r.head.o.v) continue;
+        int32_t maxVar = std::max(r.head.s.v, r.head.o.v);
+        for (const AtomIR& a : r.body) {
+            if (a.s.var) maxVar = std::max(maxVar, a.s.v);
+            if (a.o.var) maxVar = std::max(maxVar, a.o.v);
+        }
+        name.assign((size_t)maxVar + 1, -1);
+        name[r.head.s.v] = 0;
+        name[r.head.o.v] = 1;
+        int32_t next = 2;
+        auto enc = [&](const TermIR& t) -> int64_t {
+            if (!t.var) return 2 * (int64_t)t.v + 1;
+            if (name[t.v] < 0) name[t.v] = next++;
+            return 2 * (int64_t)name[t.v];
+        };
+        std::vector<std::array<int64_t, 3>> atoms;
+        for (const AtomIR& a : r.body) atoms.push_back({(int64_t)a.pterm, enc(a.s), enc(a.o)});
+        std::sort(atoms.begin(), atoms.end());
+        key.clear();
+        for (const auto& a : atoms) key.insert(key.end(), a.begin(), a.end());
+        auto it = first.find(key);
+        if (it == first.end()) first.emplace(key, (int32_t)i);
+        else from[i] = it->second;
+    }
+    return from;
+}
+
 bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const std::vector<int32_t>& parseStatus,
                  int32_t flags, hipStream_t st) {
     B.ctx = &C;
@@ -294,10 +340,10 @@ bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const s
     std::vector<ConflictRefs> refs(np), wrefs(np);
     std::vector<Chunk> lists[2];
     uint64_t items[2] = {0, 0};
-    struct Job { int ph; size_t k; int64_t n; bool fus; };
+    struct Job { int ph; size_t k; int64_t n; bool fus; bool shared; };
     std::vector<Job> jobs;
     auto add = [&](int ph, size_t i, const Plan& P, const ConflictRefs& R, const Plan& W, const ConflictRefs& RW,
-                   int64_t n, int32_t supPred) {
+                   int64_t n, int32_t supPred, bool fus) {
         const size_t k = 2 * i + ph;
         plans[k] = P;
         wplans[k] = W;
@@ -306,7 +352,7 @@ bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const s
         plans[k].rule = wplans[k].rule = (int32_t)i;
         plans[k].supPred = wplans[k].supPred = ph == kPairs ? supPred : -1;
         B.slots[ph] = std::max({B.slots[ph], plan_slots(P, R), plan_slots(W, RW)});
-        jobs.push_back(Job{ph, k, n, supPred >= 0});
+        jobs.push_back(Job{ph, k, n, fus, false});
         items[ph] += (uint64_t)n;
     };
     for (size_t i = 0; i < rules.size(); ++i) {
@@ -314,13 +360,58 @@ bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const s
         CompiledRule& cr = B.comp[i];
         cr = compile_rule(rules[i], C.g, C.tune);
         B.status[i] = cr.status;
-        if (cr.status != kRuleOk) continue;
-        // the pairs pass may count this rule's support (a fused launch then skips its support chunks)
-        const int32_t supPred = (cr.supPred >= 0 && cr.supItems > 0 && cr.pairItems > 0) ? cr.supPred : -1;
+    }
+    // rules with the body of an earlier rule take its body-pair count
+    B.pairsFrom = C.shareBodies ? shared_pairs(rules, B.comp, B.status) : std::vector<int32_t>(rules.size(), -1);
+    std::vector<std::vector<int32_t>> members(rules.size());
+    B.nShared = 0;
+    for (size_t i = 0; i < rules.size(); ++i)
+        if (B.pairsFrom[i] >= 0) {
+            members[B.pairsFrom[i]].push_back((int32_t)i);
+            ++B.nShared;
+        }
+    // Rules whose support the pairs pass may count (a fused launch then skips
+    // their support chunks): a search's own rule alone (Plan::supPred = its
+    // head predicate), or, when rules share the search, every qualifying one
+    // through a member list in supTab (supPred <= -2).
+    auto fusable = [&](size_t i) {
+        const CompiledRule& cr = B.comp[i];
+        return B.status[i] == kRuleOk && cr.supPred >= 0 && cr.supItems > 0;
+    };
+    std::vector<uint8_t> fused(rules.size(), 0);
+    std::vector<int32_t> supField(rules.size(), -1);
+    std::vector<int32_t> supTab(1, 0);   // word 0 unused: the buffer is never empty
+    for (size_t i = 0; i < rules.size(); ++i) {
+        if (B.status[i] != kRuleOk || B.pairsFrom[i] >= 0) continue;
+        const CompiledRule& cr = B.comp[i];
+        if (!cr.pairsLaunch || cr.pairItems <= 0) continue;
+        std::vector<int32_t> list;
+        if (fusable(i)) list.push_back((int32_t)i);
+        for (int32_t m : members[i])
+            if (fusable((size_t)m)) list.push_back(m);
+        if (list.empty()) continue;
+        for (int32_t r : list) fused[r] = 1;
+        if (list.size() == 1 && list[0] == (int32_t)i) {
+            supField[i] = cr.supPred;
+        } else {
+            supField[i] = -(int32_t)supTab.size() - 2;
+            supTab.push_back((int32_t)list.size());
+            for (int32_t r : list) {
+                supTab.push_back(r);
+                supTab.push_back(B.comp[r].supPred);
+            }
+        }
+    }
+    for (size_t i = 0; i < rules.size(); ++i) {
+        if (B.status[i] != kRuleOk) continue;
+        const CompiledRule& cr = B.comp[i];
         if (cr.supportLaunch && cr.supItems > 0)
-            add(0, i, cr.sup, cr.supRefs, cr.supWave, cr.supWaveRefs, cr.supItems, supPred);
-        if (cr.pairsLaunch && cr.pairItems > 0)
-            add(1, i, cr.pairs, cr.pairsRefs, cr.pairsWave, cr.pairsWaveRefs, cr.pairItems, supPred);
+            add(0, i, cr.sup, cr.supRefs, cr.supWave, cr.supWaveRefs, cr.supItems, -1, fused[i] != 0);
+        if (B.pairsFrom[i] < 0 && cr.pairsLaunch && cr.pairItems > 0) {
+            add(1, i, cr.pairs, cr.pairsRefs, cr.pairsWave, cr.pairsWaveRefs, cr.pairItems, supField[i],
+                supField[i] != -1);
+            if (!members[i].empty()) jobs.back().shared = true;
+        }
     }
     // Seeds per thread-mode work unit: 1024 (fewer plan loads and cursor
     // atomics) unless that leaves fewer than chunksPerWave units per
@@ -329,7 +420,7 @@ bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const s
     const uint64_t waves = (uint64_t)C.numCUs * (uint64_t)C.threadWavesPerCU;
     B.nFusable = 0;
     B.nFused = 0;
-    for (const Job& j : jobs) B.nFused += (j.ph == kPairs && j.fus) ? 1 : 0;
+    for (size_t i = 0; i < rules.size(); ++i) B.nFused += fused[i];
     for (int ph = 0; ph < 2; ++ph) {
         int64_t chunk = kChunkSeeds;
         while (chunk > kMinChunkSeeds && items[ph] / (uint64_t)chunk < (uint64_t)C.chunksPerWave * waves) chunk >>= 1;
@@ -337,7 +428,8 @@ bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const s
         for (int pass = 0; pass < (ph == kSupport ? 2 : 1); ++pass)
             for (const Job& j : jobs) {
                 if (j.ph != ph || (ph == kSupport && (int)j.fus != pass)) continue;
-                const uint32_t gate = (ph == kPairs && gated) ? (j.fus ? 2u : 1u) : 0u;
+                // a search whose count other rules share runs whatever its own rule's support
+                const uint32_t gate = (ph == kPairs && gated && !j.shared) ? (j.fus ? 2u : 1u) : 0u;
                 for (int64_t b = 0; b < j.n; b += chunk) {
                     lists[ph].push_back(
                         Chunk{(uint32_t)j.k, (uint32_t)b, (uint32_t)std::min<int64_t>(chunk, j.n - b), gate});
@@ -358,7 +450,7 @@ bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const s
     if (all.empty()) all.push_back(Chunk{0, 0, 0, 0});
     B.capRules = std::max<uint32_t>((uint32_t)rules.size(), 1);
     if (!put(B.plans, plans, st) || !put(B.wplans, wplans, st) || !put(B.refs, refs, st) ||
-        !put(B.wrefs, wrefs, st) || !put(B.chunks, all, st))
+        !put(B.wrefs, wrefs, st) || !put(B.chunks, all, st) || !put(B.supTab, supTab, st))
         return false;
     if (!B.counts.ensure((size_t)2 * B.capRules * 8) ||
         !B.heavy.ensure((size_t)std::max<uint64_t>(heavyTotal, 1) * sizeof(HeavyItem)) || !B.ctl.ensure(kCtlWords * 4))
@@ -409,7 +501,7 @@ void batch_enqueue(Context& C, Batch& B, const ExecTarget& t) {
         if (nc) {
             ThreadArgs ta{B.plans.as<Plan>(), B.refs.as<ConflictRefs>(), B.chunks.as<Chunk>() + B.chunkOff[ph], nc,
                           B.heavyCap[ph], c + 0, counts, B.heavy.as<HeavyItem>() + B.heavyOff[ph], c + 1,
-                          fusePh ? 1 : 0};
+                          fusePh ? 1 : 0, B.supTab.as<int32_t>()};
             const int blocks = std::min<int>(C.threadBlocks(), (int)((nc + 3) / 4));
             launch_thread_batch(ph, B.slots[ph], ta, C.dg, std::max(blocks, 1), st);
         }
@@ -421,6 +513,7 @@ void batch_enqueue(Context& C, Batch& B, const ExecTarget& t) {
             if (sort) sink = SortSink{t.sort->keys[0].as<unsigned long long>(), c + 3, t.sort->cap, hoBits};
             WaveArgs wa{B.wplans.as<Plan>(), B.wrefs.as<ConflictRefs>(), B.heavy.as<HeavyItem>() + B.heavyOff[ph],
                         c + 1, B.heavyCap[ph], fusePh ? 1u : 0u, c + 2, counts, t.dedup, t.rowWords, sink};
+            wa.supTab = B.supTab.as<int32_t>();
             const int mode = ph == kPairs ? t.pairsDedup : kDedupRows;
             giant_prepare(*gbp, t.giantAfter, wa.giant, st, ph);
             const int wb = t.pairsDedup == kDedupLds ? C.waveBlocksFor(B.slots[ph]) : t.waveBlocks;
@@ -466,7 +559,8 @@ void batch_collect(Batch& B, GpuRuleResult64* out) {
         r.bodyPairs = -1;
         if (r.status != kRuleOk) continue;
         r.support = cr.supportLaunch ? (int64_t)B.hCounts[2 * i] : 0;
-        if (!gated || r.support > 0) r.bodyPairs = cr.pairsLaunch ? (int64_t)B.hCounts[2 * i + 1] : 0;
+        const size_t src = B.pairsFrom.empty() || B.pairsFrom[i] < 0 ? (size_t)i : (size_t)B.pairsFrom[i];
+        if (!gated || r.support > 0) r.bodyPairs = cr.pairsLaunch ? (int64_t)B.hCounts[2 * src + 1] : 0;
         r.bodySize = r.support > 0 && r.bodyPairs > 0 ? r.bodyPairs : 0;
         finish_measures(r);
     }
@@ -570,6 +664,7 @@ bool batch_run(Context& C, Batch& B, GpuRuleResult64* out) {
     C.lastEmitted = sb ? B.lastEmitted : 0;
     C.lastGiants = (int64_t)B.hGiant[0] + B.hGiant[1];
     C.lastFused = C.fuseActive(mode) ? B.nFused : 0;
+    C.lastShared = B.nShared;
     for (int k = 0; k < 2 * kWaveStats; ++k) C.lastWaveStats[k] = B.hWaveStats[k];
     for (int k = 0; k < 4; ++k) C.lastKernelMs[k] = B.kernelMs[k];
     return true;
@@ -1054,24 +1149,47 @@ int gpu_plan_traffic(void* handle, const GpuRuleDesc* rules, int32_t n, int64_t*
     try {
         Tuning t;
         // a context whose batches count fusable rules' support in the pairs pass is modelled that way
+        // (and a context that shares body-pair searches between equal bodies, that way)
         const Context* C = as_ctx(handle);
         const bool fuse = C && C->fuseActive(C->pairsDedup());
         std::vector<CompiledRule> crs(std::max<int32_t>(n, 0));
+        std::vector<RuleIR> irs(std::max<int32_t>(n, 0));
+        std::vector<int32_t> status(std::max<int32_t>(n, 0));
+        for (int32_t i = 0; i < n; ++i) {
+            irs[i] = rule_from_words(rules[i].words);
+            crs[i] = compile_rule(irs[i], *g, t);
+            status[i] = crs[i].status;
+        }
+        const std::vector<int32_t> from =
+            C && C->shareBodies ? shared_pairs(irs, crs, status) : std::vector<int32_t>(crs.size(), -1);
+        // fused support, as a batch does it: a search's own rule and the
+        // qualifying rules that share it
+        auto fusable = [&](int32_t i) { return crs[i].status == kRuleOk && crs[i].supPred >= 0 && crs[i].supItems > 0; };
+        std::vector<uint8_t> fusedRule(crs.size(), 0);
+        std::vector<std::vector<int32_t>> heads(crs.size());
+        if (fuse) {
+            for (int32_t i = 0; i < n; ++i) {
+                const int32_t src = from[i] < 0 ? i : from[i];
+                if (crs[src].status != kRuleOk || !crs[src].pairsLaunch || crs[src].pairItems <= 0 || !fusable(i))
+                    continue;
+                fusedRule[i] = 1;
+                heads[src].push_back(crs[i].supPred);
+            }
+        }
         std::vector<TrafficJob> jobs;
         std::vector<int64_t> slot;   // out index of each job
         for (int32_t i = 0; i < n; ++i) {
-            crs[i] = compile_rule(rule_from_words(rules[i].words), *g, t);
             CompiledRule& cr = crs[i];
             for (int k = 0; k < 4; ++k) out[4 * i + k] = 0;
             if (cr.status != kRuleOk) continue;
-            const bool fus = fuse && cr.supPred >= 0 && cr.supItems > 0 && cr.pairItems > 0;
-            cr.pairs.supPred = fus ? cr.supPred : -1;
+            const bool fus = fusedRule[i] != 0;
+            cr.pairs.supPred = -1;
             if (cr.supportLaunch && !fus) {
                 jobs.push_back(TrafficJob{&cr.sup, &cr.supRefs, cr.supItems});
                 slot.push_back(4 * (int64_t)i);
             }
-            if (cr.pairsLaunch) {
-                jobs.push_back(TrafficJob{&cr.pairs, &cr.pairsRefs, cr.pairItems});
+            if (cr.pairsLaunch && from[i] < 0) {
+                jobs.push_back(TrafficJob{&cr.pairs, &cr.pairsRefs, cr.pairItems, heads[i].empty() ? nullptr : &heads[i]});
                 slot.push_back(4 * (int64_t)i + 1);
             }
         }
@@ -1103,6 +1221,7 @@ int gpu_set_tuning(void* ctx, const char* key, int64_t value) {
     } else if (k == "use_graph") C->useGraph = value != 0;
     else if (k == "overlap") C->overlap = value != 0;
     else if (k == "fuse_support") C->fuseSupport = value != 0;
+    else if (k == "share_bodies") C->shareBodies = value != 0;
     else if (k == "chunks_per_wave") C->chunksPerWave = (int)std::min<int64_t>(1024, std::max<int64_t>(1, value));
     else if (k == "dedup") {
         if (value < 0 || value > 3) return -1;
@@ -1132,6 +1251,8 @@ int64_t gpu_get_tuning(void* ctx, const char* key) {
     if (k == "use_graph") return C->useGraph ? 1 : 0;
     if (k == "overlap") return C->overlap ? 1 : 0;
     if (k == "fuse_support") return C->fuseSupport ? 1 : 0;
+    if (k == "share_bodies") return C->shareBodies ? 1 : 0;
+    if (k == "last_shared") return C->lastShared;
     if (k == "chunks_per_wave") return C->chunksPerWave;
     if (k == "fuse_active") return C->fuseActive(C->pairsDedup()) ? 1 : 0;
     if (k == "num_cus") return C->numCUs;

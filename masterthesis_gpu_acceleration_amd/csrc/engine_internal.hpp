@@ -167,10 +167,15 @@ struct Batch : Handle {
     // a launch runs only the first nChunks[0] - nFusable
     uint32_t nFusable = 0;
     int64_t nFused = 0;   // rules with a fusable support (pairs plans with supPred >= 0)
+    // per rule: the rule whose body-pair search it shares (same body up to
+    // renaming), or -1; nShared rules have one
+    std::vector<int32_t> pairsFrom;
+    int64_t nShared = 0;
     uint32_t capRules = 0;
     int slots[2] = {1, 1};   // variable slots of the plans of each phase (picks the kernels' slot class)
     double kernelMs[4] = {};
     DevBuf plans, wplans, refs, wrefs, chunks, counts, heavy, ctl;
+    DevBuf supTab;   // member lists of shared searches that count several rules' support (Plan::supPred)
     unsigned long long* hCounts = nullptr;   // pinned
     size_t hCountsCap = 0;                   // words
     uint32_t* hCtl = nullptr;                // pinned
@@ -236,9 +241,13 @@ struct Context : Handle {
     // Thread-mode work units per thread-mode wave a batch aims for (smaller
     // units when the seeds are few): more units even out the waves' ends.
     int chunksPerWave = 2;
+    // Rules with the same body (up to renaming, head variables fixed) share
+    // one body-pair search.
+    bool shareBodies = true;
     bool fuseActive(int mode) const { return fuseSupport && (mode == kDedupRows || mode == kDedupLds); }
     int64_t lastGiants = 0;
     int64_t lastFused = 0;   // rules whose support the last batch counted in its pairs pass
+    int64_t lastShared = 0;  // rules of the last batch that took another rule's body-pair count
     unsigned long long lastWaveStats[2 * kWaveStats] = {};
     Batch* lastBatch = nullptr;
     double lastMs = 0, lastSupMs = 0, lastPairMs = 0, lastSortMs = 0;
@@ -338,6 +347,10 @@ int64_t batch_emitted(const Batch& B);
 // Compiles rules into B and uploads plans and chunk lists on stream st.
 bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const std::vector<int32_t>& status,
                  int32_t flags, hipStream_t st);
+// For each rule, the earlier rule with the same body (up to a renaming that
+// fixes the head variables) whose body-pair count it can take, or -1.
+std::vector<int32_t> shared_pairs(const std::vector<RuleIR>& rules, const std::vector<CompiledRule>& comp,
+                                  const std::vector<int32_t>& status);
 // Enqueues B's device pass (counter fills, kernels, result copies) on t.st.
 void batch_enqueue(Context& C, Batch& B, const ExecTarget& t);
 // Fills out[B.n] from B's host-side counters after its pass completed.

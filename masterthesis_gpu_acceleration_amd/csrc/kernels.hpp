@@ -19,7 +19,8 @@ struct ThreadArgs {
     unsigned long long* counts;   // [2*rule + kind]
     HeavyItem* heavy;
     uint32_t* heavyCount;
-    int32_t fuse;                 // PAIRS: plans with supPred >= 0 also count support (into [2*rule + kSupport])
+    int32_t fuse;                 // PAIRS: plans with supPred != -1 also count support (into [2*rule + kSupport])
+    const int32_t* supTab;        // member lists of plans with supPred <= -2 (Plan::supPred)
 };
 
 // Thread-mode pass over all seeds of one plan given by value.
@@ -100,13 +101,14 @@ struct WaveArgs {
     const HeavyItem* heavy;
     const uint32_t* heavyCount;
     uint32_t heavyCap;
-    uint32_t fuse;                // batch PAIRS launches: plans with supPred >= 0 also count support (bitmap dedup only)
+    uint32_t fuse;                // batch PAIRS launches: plans with supPred != -1 also count support (bitmap dedup only)
     uint32_t* cursor;
     unsigned long long* counts;   // batch: [2*rule + kind]; by-value kernel: counts[0]
     uint32_t* rows;               // per-wave dedup rows, all zero between seeds (kDedupRows)
     int64_t rowWords;
     SortSink sink;                // kDedupSort
     GiantArgs giant;              // batch launches; after == 0 for per-rule launches
+    const int32_t* supTab;        // member lists of plans with supPred <= -2 (batch launches)
 };
 
 // `kind` is the plan kind (kSupport / kPairs) of every plan the launch sees;

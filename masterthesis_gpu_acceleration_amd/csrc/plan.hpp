@@ -88,7 +88,9 @@ struct alignas(16) Plan {
     // head predicate; its key-index block run starts at supPred * nBlocks).
     // Only rules where no body atom can ground to the head triple itself
     // qualify (the support search excludes that triple, the pair search does
-    // not). -1: no support counting here.
+    // not). -1: no support counting here. <= -2: the plan's search is shared
+    // by rules with equal bodies, and it counts the support of several of
+    // them: supTab[-supPred - 2] = n, then n (rule, head predicate) pairs.
     int32_t supPred;
     Level lev[kMaxBody];
 };

@@ -440,11 +440,13 @@ __device__ __forceinline__ bool leaf_fast_ok(const Plan& P, int last, bool pairs
 // Returns this lane's count; escalated seeds go to the heavy queue. K is the
 // plan kind of every plan the launch sees (kSupport or kPairs). With `fuse`,
 // a PAIRS plan with supPred >= 0 adds each finished seed's supported head triples
-// to `sup` (seeds moved to the heavy queue count there instead).
+// to `sup`, one with a member list (supPred <= -2) adds each member's to its
+// counter in `counts` (seeds moved to the heavy queue count there instead).
 template <int K, int NS>
 __device__ __forceinline__ uint64_t thread_chunk(ThreadLdsT<levels_of(NS)>& W, const DevGraph& G, const ConflictRefs* R,
                                                  uint32_t planIdx, uint32_t begin, uint32_t count, HeavyItem* hq,
-                                                 uint32_t* hCount, uint32_t hCap, bool fuse, uint64_t& sup) {
+                                                 uint32_t* hCount, uint32_t hCap, bool fuse, uint64_t& sup,
+                                                 unsigned long long* counts, const int32_t* supTab) {
     const int lane = lane_id();
     const Plan& P = W.plan;
     // plan-wide fields as wave-uniform values (scalar registers)
@@ -453,8 +455,11 @@ __device__ __forceinline__ uint64_t thread_chunk(ThreadLdsT<levels_of(NS)>& W, c
     const int hoRef = uniform(P.hoRef);
     const int budget0 = uniform(P.budget);
     const bool pairs = K == kPairs && pl >= 0;
-    const bool fs = pairs && fuse && uniform(P.supPred) >= 0;   // also count the seeds' supported head triples
-    const int64_t supBase = (int64_t)uniform(P.supPred) * G.nBlocks;
+    // also count the seeds' supported head triples: of this plan's rule, or of
+    // each rule in its member list (added to their counters directly)
+    const int supPred = uniform(P.supPred);
+    const bool fs = pairs && fuse && supPred != -1;
+    const int32_t* supList = (fs && supPred <= -2) ? supTab + (-supPred - 2) : nullptr;
     const uint64_t below = (1ull << lane) - 1ull;
     int v[NS];
     RegSet set;
@@ -491,7 +496,15 @@ __device__ __forceinline__ uint64_t thread_chunk(ThreadLdsT<levels_of(NS)>& W, c
                 if (d == 0) {
                     active = false;
                     if (pairs) mine += (uint64_t)set.n;
-                    if (fs && set.n > 0) sup += head_hits(G, supBase, v[0], set);
+                    if (fs && set.n > 0) {
+                        const int nm = supList ? supList[0] : 1;
+                        for (int m = 0; m < nm; ++m) {
+                            const int pred = supList ? supList[2 + 2 * m] : supPred;
+                            const uint32_t h = head_hits(G, (int64_t)pred * G.nBlocks, v[0], set);
+                            if (!supList) sup += h;
+                            else if (h) atomicAdd(counts + 2 * supList[1 + 2 * m] + kSupport, (unsigned long long)h);
+                        }
+                    }
                 } else {
                     --d;
                 }
@@ -802,7 +815,9 @@ constexpr int kLeafGeneric = 4;
 //     counts only if it is the one to set it.
 //   * supOn (fused support, bitmap dedup): before the bitmap is cleared, the
 //     supported head triples of the seed, whose head row is supRow, are
-//     added to supOut (nothing if the seed was abandoned or found no pair).
+//     added to supOut -- or, with a member list, each member's (head rows of
+//     seed supSubject) to its counter in `counts` -- nothing if the seed was
+//     abandoned or found no pair.
 // The pool (poolLevel/Next/End/V) is set up by the caller, all lanes idle.
 // Returns the count this wave adds (SUPPORT: 0/1; PAIRS: distinct head
 // objects it marked first, or 0 on the sort path, which counts later).
@@ -812,6 +827,7 @@ __device__ __forceinline__ uint64_t wave_run(WaveLdsT<levels_of(NS)>& W, const D
                                              int64_t rowWords, const SortSink& sink, unsigned long long keyHi,
                                              const GiantArgs& ga, bool mayGiant, HeavyItem self, uint32_t heavyIdx,
                                              uint32_t* foundFlag, bool& aborted, bool supOn, Frame supRow,
+                                             const int32_t* supList, int supSubject, unsigned long long* counts,
                                              uint64_t& supOut,
                                              unsigned long long (&stats)[kWaveStats]) {
     const int lane = lane_id();
@@ -1128,7 +1144,15 @@ __device__ __forceinline__ uint64_t wave_run(WaveLdsT<levels_of(NS)>& W, const D
     if (D == kDedupSort) return 0;
     // clear the wave's own bitmap (a sub-item marked only its giant's row: nothing listed)
     wave_sync();
-    if (supOn && !aborted && wave_sum(mine) != 0) supOut += head_hits_wave(dd, G, supRow, nullptr);
+    if (supOn && !aborted && wave_sum(mine) != 0) {
+        const int nm = supList ? supList[0] : 1;
+        for (int m = 0; m < nm; ++m) {
+            const Frame hr = supList ? key_row(G, (int64_t)supList[2 + 2 * m] * G.nBlocks, supSubject) : supRow;
+            const uint64_t h = head_hits_wave(dd, G, hr, nullptr);
+            if (!supList) supOut += h;
+            else if (lane == 0 && h) atomicAdd(counts + 2 * supList[1 + 2 * m] + kSupport, (unsigned long long)h);
+        }
+    }
     if (D == kDedupLds) {
         // a giant's sub-item ORs its bitmap into the giant's row: the bits
         // that OR sets first are the objects it adds (each bit is set first
@@ -1161,6 +1185,7 @@ __device__ __forceinline__ uint64_t wave_seed(WaveLdsT<levels_of(NS)>& W, const 
                                               uint32_t item, Dedup<D, NS>& dd, int64_t rowWords, const SortSink& sink,
                                               unsigned long long keyHi, const GiantArgs& ga, uint32_t planIdx,
                                               uint32_t heavyIdx, bool fuse, uint64_t& sup,
+                                              unsigned long long* counts, const int32_t* supTab,
                                               unsigned long long (&stats)[kWaveStats]) {
     const int lane = lane_id();
     const Plan& P = W.plan;
@@ -1181,10 +1206,11 @@ __device__ __forceinline__ uint64_t wave_seed(WaveLdsT<levels_of(NS)>& W, const 
     wave_sync();
     bool aborted = false;
     // fused support: the seed's head row, looked up now so its loads overlap the search
-    const bool supOn = K == kPairs && D != kDedupSort && fuse && P.supPred >= 0;
-    const Frame supRow = supOn ? key_row(G, sup_base(P, G), v[0]) : Frame{0, 0};
+    const bool supOn = K == kPairs && D != kDedupSort && fuse && P.supPred != -1;
+    const int32_t* supList = (supOn && P.supPred <= -2) ? supTab + (-P.supPred - 2) : nullptr;
+    const Frame supRow = (supOn && !supList) ? key_row(G, sup_base(P, G), v[0]) : Frame{0, 0};
     return wave_run<K, D, NS>(W, G, R, dd, rowWords, sink, keyHi, ga, ga.after > 0, HeavyItem{planIdx, item}, heavyIdx,
-                          nullptr, aborted, supOn, supRow, sup, stats);
+                          nullptr, aborted, supOn, supRow, supList, uniform(v[0]), counts, sup, stats);
 }
 
 // ------------------------------------------------------------- kernels
@@ -1209,7 +1235,7 @@ __global__ void __launch_bounds__(kBlock) k_thread_batch(ThreadArgs a, DevGraph 
         }
         uint64_t sup = 0;
         const uint64_t mine = thread_chunk<K, NS>(W, G, a.refs + ch.plan, ch.plan, ch.begin, ch.count, a.heavy,
-                                           a.heavyCount, a.heavyCap, a.fuse != 0, sup);
+                                           a.heavyCount, a.heavyCap, a.fuse != 0, sup, a.counts, a.supTab);
         const uint64_t tot = wave_sum(mine);
         if (lane == 0 && tot) atomicAdd(a.counts + 2 * W.plan.rule + K, (unsigned long long)tot);
         if (K == kPairs && a.fuse && W.plan.supPred >= 0) {
@@ -1233,7 +1259,8 @@ __global__ void __launch_bounds__(kBlock) k_thread_one(Plan P, OneArgs a, DevGra
         const uint32_t cnt = a.nItems - base < (uint32_t)kOneChunkSeeds ? a.nItems - base : (uint32_t)kOneChunkSeeds;
         uint64_t sup = 0;
         const uint64_t mine =
-            thread_chunk<K, NS>(W, G, a.refs, 0u, base, cnt, a.heavy, a.heavyCount, a.heavyCap, false, sup);
+            thread_chunk<K, NS>(W, G, a.refs, 0u, base, cnt, a.heavy, a.heavyCount, a.heavyCap, false, sup, nullptr,
+                                nullptr);
         const uint64_t tot = wave_sum(mine);
         if (lane == 0 && tot) atomicAdd(a.count, (unsigned long long)tot);
     }
@@ -1295,7 +1322,7 @@ __global__ void __launch_bounds__(kWave) k_wave_batch(WaveArgs a, DevGraph G) {
         uint64_t sup = 0;
         const uint64_t c = wave_seed<K, D, NS>(W, G, a.refs + it.plan, it.item, dd, a.rowWords, a.sink,
                                            (unsigned long long)h << a.sink.hoBits, a.giant, it.plan, h, a.fuse != 0,
-                                           sup, stats);
+                                           sup, a.counts, a.supTab, stats);
         if (lane == 0 && c) atomicAdd(a.counts + 2 * W.plan.rule + K, (unsigned long long)c);
         if (lane == 0 && sup) atomicAdd(a.counts + 2 * W.plan.rule + kSupport, (unsigned long long)sup);
     }
@@ -1319,7 +1346,8 @@ __global__ void __launch_bounds__(kWave) k_wave_one(Plan P, WaveArgs a, DevGraph
         if (h >= n) break;
         uint64_t sup = 0;
         const uint64_t c = wave_seed<K, D, NS>(W, G, a.refs, a.heavy[h].item, dd, a.rowWords, a.sink,
-                                           (unsigned long long)h << a.sink.hoBits, a.giant, 0u, h, false, sup, stats);
+                                           (unsigned long long)h << a.sink.hoBits, a.giant, 0u, h, false, sup, nullptr,
+                                           nullptr, stats);
         if (lane == 0 && c) atomicAdd(a.counts, (unsigned long long)c);
     }
     flush_stats(a.giant, stats);
@@ -1473,15 +1501,15 @@ __global__ void __launch_bounds__(kWave) k_wave_sub(WaveArgs a, DevGraph G) {
             bool aborted = false;
             uint64_t none = 0;
             c = wave_run<K, D, NS>(W, G, a.refs + plan, dd, a.rowWords, a.sink, (unsigned long long)hv << a.sink.hoBits,
-                               ga, false, HeavyItem{plan, 0u}, hv, foundFlag, aborted, false, Frame{0, 0}, none,
-                               stats);
+                               ga, false, HeavyItem{plan, 0u}, hv, foundFlag, aborted, false, Frame{0, 0}, nullptr,
+                               0, nullptr, none, stats);
             dd.shared = nullptr;
             dd.merge = nullptr;
         }
         if (lane == 0 && c) atomicAdd(a.counts + 2 * W.plan.rule + K, (unsigned long long)c);
         // fused support: the giant's row is complete once its last sub-item
         // is done; that sub-item counts the seed's supported head triples
-        const bool fs = K == kPairs && D != kDedupSort && a.fuse && W.plan.supPred >= 0;
+        const bool fs = K == kPairs && D != kDedupSort && a.fuse && W.plan.supPred != -1;
         if (fs) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         uint32_t last = 0;
         if (lane == 0) last = atomicSub(ga.remaining + g, 1u) == 1u ? 1u : 0u;
@@ -1489,9 +1517,17 @@ __global__ void __launch_bounds__(kWave) k_wave_sub(WaveArgs a, DevGraph G) {
             uint32_t* row = ga.rows + (int64_t)g * a.rowWords;
             if (fs) {
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-                const Frame hr = key_row(G, sup_base(W.plan, G), uniform(S.v[0]));
-                const uint64_t sup = head_hits_wave(dd, G, hr, row);
-                if (lane == 0 && sup) atomicAdd(a.counts + 2 * W.plan.rule + kSupport, (unsigned long long)sup);
+                const int supPred = W.plan.supPred;
+                const int32_t* supList = supPred <= -2 ? a.supTab + (-supPred - 2) : nullptr;
+                const int s0 = uniform(S.v[0]);
+                const int nm = supList ? supList[0] : 1;
+                for (int m = 0; m < nm; ++m) {
+                    const int pred = supList ? supList[2 + 2 * m] : supPred;
+                    const int rule = supList ? supList[1 + 2 * m] : W.plan.rule;
+                    const Frame hr = key_row(G, (int64_t)pred * G.nBlocks, s0);
+                    const uint64_t sup = head_hits_wave(dd, G, hr, row);
+                    if (lane == 0 && sup) atomicAdd(a.counts + 2 * rule + kSupport, (unsigned long long)sup);
+                }
             }
             for (int64_t w = lane; w < a.rowWords; w += kWave) row[w] = 0u;   // read again only by later launches
         }

@@ -45,6 +45,7 @@ struct Walker {
     // with the words set listed so a seed clears only those
     std::vector<uint64_t>& bits;
     std::vector<int64_t>& touched;
+    const std::vector<int32_t>* heads = nullptr;   // fused support of several head predicates
 
     bool seen(int x) const { return (bits[x >> 6] >> (x & 63)) & 1u; }
     void insert(int x) {
@@ -220,18 +221,23 @@ struct Walker {
         const int64_t ks = P.seedBegin + item;
         v[0] = ks < (int64_t)g.keysS.size() ? g.keysS[ks] : g.keysO[ks - g.keysS.size()];
         dfs(0, P.pairLevel >= 0);
-        if (P.supPred >= 0 && P.pairLevel >= 0) head_hits();
+        if (P.pairLevel >= 0) {
+            if (heads)
+                for (int32_t q : *heads) head_hits(q);
+            else if (P.supPred >= 0)
+                head_hits(P.supPred);
+        }
     }
 
     // Fused support: the seed's head row, then one binary search of it per
     // distinct head object found (thread mode, at most kThreadSet objects),
     // or one pass over it (wave mode).
     static constexpr int64_t kThreadSet = 8;
-    void head_hits() {
+    void head_hits(int32_t pred) {
         int64_t n = 0;
         for (int64_t w : touched) n += __builtin_popcountll(bits[w]);
         if (n == 0) return;   // no body pair: nothing to look up
-        const int64_t base = ((int64_t)kSPO * g.P() + P.supPred) * g.nBlocks();
+        const int64_t base = ((int64_t)kSPO * g.P() + pred) * g.nBlocks();
         int64_t lo, hi;
         key_row(base, v[0], lo, hi);
         if (lo >= hi) return;
@@ -288,6 +294,7 @@ void plan_traffic_many(const std::vector<TrafficJob>& jobs, const HostGraph& g, 
             if (t >= tasks.size()) break;
             const TrafficJob& J = jobs[tasks[t].job];
             Walker w{*J.plan, *J.refs, g, g.T(), 0, 0, {}, bits, touched};
+            w.heads = J.heads;
             for (int64_t i = tasks[t].begin; i < tasks[t].end; ++i) w.seed(i);
             w.clear_set();
             bytes[tasks[t].job] += w.bytes;

@@ -20,6 +20,9 @@ struct TrafficJob {
     const Plan* plan;
     const ConflictRefs* refs;
     int64_t items;
+    // PAIRS: head predicates whose support the search also counts (a shared
+    // search's member list); null: the plan's own supPred, if any
+    const std::vector<int32_t>* heads = nullptr;
 };
 void plan_traffic_many(const std::vector<TrafficJob>& jobs, const HostGraph& g, std::vector<TrafficEstimate>& out,
                        int threads);

@@ -272,6 +272,68 @@ def test_giant_seeds_random_rules(seed):
         eng.close()
 
 
+def _variants(rng, rule, n_predicates, k):
+    """k rules with rule's body up to renaming: another head predicate, the
+    non-head variables renamed, the body atoms shuffled."""
+    head, body = rule
+    hv = {t[1] for t in (head[0], head[2]) if t[0] == "v"}
+    out = []
+    for j in range(k):
+        ren = lambda t: t if t[0] == "c" or t[1] in hv else ("v", t[1] + "_%d" % j)
+        b = [(ren(a[0]), a[1], ren(a[2])) for a in body]
+        b = [b[q] for q in rng.permutation(len(b))]
+        out.append(((head[0], int(rng.integers(n_predicates)), head[2]), b))
+    return out
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_shared_bodies_vs_oracle(seed):
+    """Rules with equal bodies up to renaming share one body-pair search
+    (share_bodies): base rules plus variants with other head predicates,
+    renamed variables and shuffled atoms -- some variants fused, some not.
+    Every dedup mode, gated and ungated, sharing and fusion on and off:
+    counts equal the oracle's."""
+    say("shared-bodies", seed)
+    g = W.tiny_random(9700 + seed, n_entities=16 + seed, n_predicates=3, n_triples=120 + 10 * seed)
+    rng = np.random.default_rng(700 + seed)
+    base = [W.random_rule(rng, g.n_entities, g.n_predicates, 1 + k % 4, n_vars=3 + k % 3, p_const=0.1)
+            for k in range(50)]
+    rules = []
+    for r in base:
+        rules.append(r)
+        rules += _variants(rng, r, g.n_predicates, 2)
+    exp = Oracle(g.s, g.p, g.o).eval(W.encode_rules(rules))
+    eng = L.Engine.from_graph(g)
+    try:
+        words = L.rule_words_for_graph(rules, g)
+        for dedup in (1, 2, 3):
+            eng.set_tuning("dedup", dedup)
+            for share in (1, 0):
+                eng.set_tuning("share_bodies", share)
+                for fuse in (1, 0):
+                    eng.set_tuning("fuse_support", fuse)
+                    for flags in (L.UNGATED, 0):
+                        res = eng.evaluate(words, flags)
+                        shared = eng.get_tuning("last_shared")
+                        assert (shared > 0) if share else (shared == 0), (seed, dedup, share, fuse, flags, shared)
+                        for i in range(len(rules)):
+                            tag = (seed, dedup, share, fuse, flags, i, W.rule_text(rules[i]))
+                            assert res[i]["support"] == exp[i][0], tag
+                            assert res[i]["bodySize"] == exp[i][4], tag
+                            if flags == L.UNGATED:
+                                assert res[i]["bodyPairs"] == exp[i][3], tag
+        # the traffic model counts a shared search once
+        eng.set_tuning("share_bodies", 1)
+        t_on = eng.traffic(words)
+        eng.set_tuning("share_bodies", 0)
+        t_off = eng.traffic(words)
+        assert t_on[:, 1].sum() < t_off[:, 1].sum()
+    finally:
+        eng.set_tuning("share_bodies", 1)
+        eng.set_tuning("fuse_support", 1)
+        eng.close()
+
+
 @pytest.mark.parametrize("name", ["dbpedia_shaped_5k", "dense_conf_300"])
 def test_autotune_keeps_counts(name):
     """The autotuner times its candidate settings on a batch and keeps the

@@ -134,6 +134,31 @@ def test_support_from_pairs_identity(seed):
     assert broken >= 1   # (?a p ?b) <= (?a p ?b): support 0, every p pair a hit
 
 
+@pytest.mark.parametrize("seed", range(4))
+def test_pairs_do_not_depend_on_head_predicate(seed):
+    """The distinct body pairs of a rule are the same set for any head
+    predicate, any renaming of the non-head variables and any atom order
+    (what lets rules with equal bodies share one search), by exhaustive
+    assignment; swapping the head variables' roles transposes the set."""
+    g = W.tiny_random(800 + seed, n_entities=5 + seed % 2, n_predicates=3, n_triples=20 + seed)
+    rng = np.random.default_rng(80 + seed)
+    checked = 0
+    for k in range(30):
+        r = W.random_rule(rng, g.n_entities, g.n_predicates, 1 + k % 3, n_vars=3 + k % 2, p_const=0.15)
+        head, body = r
+        if head[0][0] != "v" or head[2][0] != "v" or head[0][1] == head[2][1]:
+            continue
+        _, pairs = brute(g, r)
+        hv = {head[0][1], head[2][1]}
+        ren = lambda t: t if t[0] == "c" or t[1] in hv else ("v", t[1] + "_r")
+        b2 = [(ren(a[0]), a[1], ren(a[2])) for a in body][::-1]
+        for q in range(g.n_predicates):
+            assert brute(g, ((head[0], q, head[2]), b2))[1] == pairs, (r, q)
+        assert brute(g, ((head[2], head[1], head[0]), body))[1] == {(o, s) for (s, o) in pairs}
+        checked += 1
+    assert checked >= 10
+
+
 def _rand_graph(seed, n=40, P=3, m=500):
     return W.tiny_random(seed, n_entities=n, n_predicates=P, n_triples=m)
 
