@@ -152,7 +152,8 @@ int gpu_last_dedup_stats(void* ctx, double* sortMs, int64_t* keysEmitted, int64_
  * out[4*i+2], out[4*i+3] search levels entered. Works on a context or a
  * host handle; does no device work. On a context with fuse_support active,
  * a rule whose support the body-pair pass counts has no support bytes and
- * its body-pair bytes include the support lookups. */
+ * the body-pair bytes include the support lookups; with share_bodies, a rule
+ * that takes another rule's body-pair count has no body-pair bytes. */
 int gpu_plan_traffic(void* handle, const GpuRuleDesc* rules, int32_t n, int64_t* out);
 
 /* Measurement yardstick: copies `bytes` (multiple of 16) between two fresh
@@ -178,6 +179,9 @@ int gpu_autotune(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags, 
  * moved to wave mode), "thread_waves_per_cu", "wave_waves_per_cu",
  * "giant_after" (wave-mode steps before a seed is cut into sub-items that
  * all waves share; 0 never),
+ * "share_bodies" (0/1, default 1: rules of a batch whose bodies are equal
+ * up to a renaming that fixes the head variables share one body-pair
+ * search, which also counts each one's fused support),
  * "chunks_per_wave" (thread-mode work units per thread-mode wave a batch
  * aims for, default 2; units hold 64 to 1,024 seeds),
  * "use_graph" (0/1), "overlap" (0/1, default 1: graph replays of ungated
@@ -199,8 +203,9 @@ int gpu_autotune(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags, 
  * "wave_slots", "sort_active" (0/1), "sort_cap", "pairs_dedup" (the
  * DedupMode in use: 0 HBM rows, 1 sort, 2 LDS), "fuse_active" (0/1: the
  * dedup in use lets fuse_support act), "last_giants" (seeds cut into
- * sub-items by the last batch) and "last_fused" (rules whose support the last
- * batch counted in its body-pair pass). */
+ * sub-items by the last batch), "last_fused" (rules whose support the last
+ * batch counted in its body-pair pass) and "last_shared" (rules of the last
+ * batch that took another rule's body-pair count). */
 int gpu_set_tuning(void* ctx, const char* key, int64_t value);
 int64_t gpu_get_tuning(void* ctx, const char* key);
 
