@@ -7,10 +7,11 @@
 #include <climits>
 #include <cstdio>
 #include <cstring>
-#include <map>
+#include <exception>
 #include <memory>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "engine_internal.hpp"
@@ -292,8 +293,15 @@ constexpr int kCtlWords = 8;   // per phase: thread cursor, heavy count, wave cu
 // -1 for rules with their own search (or none).
 std::vector<int32_t> shared_pairs(const std::vector<RuleIR>& rules, const std::vector<CompiledRule>& comp,
                                   const std::vector<int32_t>& status) {
+    struct KeyHash {
+        size_t operator()(const std::vector<int64_t>& k) const {
+            uint64_t h = 1469598103934665603ull;
+            for (int64_t x : k) h = (h ^ (uint64_t)x) * 1099511628211ull;
+            return (size_t)h;
+        }
+    };
     std::vector<int32_t> from(rules.size(), -1);
-    std::map<std::vector<int64_t>, int32_t> first;
+    std::unordered_map<std::vector<int64_t>, int32_t, KeyHash> first;
     std::vector<int32_t> name;
     std::vector<int64_t> key;
     for (size_t i = 0; i < rules.size(); ++i) {
@@ -355,11 +363,29 @@ bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const s
         jobs.push_back(Job{ph, k, n, fus, false});
         items[ph] += (uint64_t)n;
     };
-    for (size_t i = 0; i < rules.size(); ++i) {
-        if (B.status[i] != kRuleOk) continue;
-        CompiledRule& cr = B.comp[i];
-        cr = compile_rule(rules[i], C.g, C.tune);
-        B.status[i] = cr.status;
+    // compile on host threads: rules are independent and the graph is only read
+    {
+        const size_t nr = rules.size();
+        const unsigned hw = std::thread::hardware_concurrency();
+        const size_t nt = std::min<size_t>(std::min<unsigned>(hw ? hw : 1u, 16u), std::max<size_t>(1, nr / 256));
+        std::vector<std::exception_ptr> errs(nt);
+        auto compile_range = [&](size_t t) {
+            try {
+                for (size_t i = t; i < nr; i += nt) {
+                    if (B.status[i] != kRuleOk) continue;
+                    B.comp[i] = compile_rule(rules[i], C.g, C.tune);
+                    B.status[i] = B.comp[i].status;
+                }
+            } catch (...) {
+                errs[t] = std::current_exception();
+            }
+        };
+        std::vector<std::thread> pool;
+        for (size_t t = 1; t < nt; ++t) pool.emplace_back(compile_range, t);
+        compile_range(0);
+        for (auto& th : pool) th.join();
+        for (auto& e : errs)
+            if (e) std::rethrow_exception(e);
     }
     // rules with the body of an earlier rule take its body-pair count
     B.pairsFrom = C.shareBodies ? shared_pairs(rules, B.comp, B.status) : std::vector<int32_t>(rules.size(), -1);
