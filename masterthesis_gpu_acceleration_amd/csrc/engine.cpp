@@ -501,7 +501,9 @@ bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const s
 // over the seeds it escalated. PAIRS chunks of rules whose support is 0 are
 // skipped on the device when the batch is gated. With bitmap dedup the pairs
 // phase also counts the support of fusable rules, whose support chunks are
-// then left out (and their pairs chunks are not gated).
+// then left out (and their pairs chunks are not gated). A search shared by
+// rules with equal bodies is never gated; each of its fusable rules' support
+// goes to that rule's counter, so the phases never write the same counter.
 void batch_enqueue(Context& C, Batch& B, const ExecTarget& t) {
     hipStream_t st = t.st;
     uint32_t* ctl = B.ctl.as<uint32_t>();
