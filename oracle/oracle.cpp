@@ -1,6 +1,6 @@
 // oracle.cpp -- CPU restatement of the rule support / body-size counting
 // semantics. TEST INFRASTRUCTURE ONLY: tests/, __graft_entry__.smoke() and
-// bench.py's cpu_baseline leg use it as the checker. The product library
+// bench.py's cpu_baseline leg use it to check results. The product library
 // (libgpu_rules.so) never links, loads or calls it.
 //
 // Semantics restated from the reference's two CPU counters (the algorithm is

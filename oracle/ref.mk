@@ -1,6 +1,6 @@
 # Builds the reference's two single-threaded CPU counters straight from their
-# sources under /root/reference into oracle/_ref/ (git-ignored). They are
-# checkers only: tests/golden/make_golden.py runs them to write fixtures, and
+# sources under /root/reference into oracle/_ref/ (git-ignored). They only
+# check results: tests/golden/make_golden.py runs them to write fixtures, and
 # bench.py may time them as the "reference" CPU baseline. Flags follow the
 # reference notebooks (RunBenchmark.ipynb cells 5-6: g++ -O3 -std=c++17).
 #
