@@ -168,7 +168,8 @@ double gpu_copy_probe(void* ctx, int64_t bytes);
  * wave64 mode), thread-mode waves per CU and work units per wave, wave-mode
  * waves per CU, head-object bitmaps
  * in HBM rows or LDS (auto dedup only), and the steps before a wave-mode
- * seed is cut into sub-items -- by coordinate descent, and keeps the fastest
+ * seed is cut into sub-items -- by coordinate descent, then tries the budget
+ * and thread-mode waves again with the rest chosen, and keeps the fastest
  * in the context. If `log` is given it receives up to logCap rows of
  * (budget, thread waves/CU, wave waves/CU, ms, escalated seeds, pairs dedup
  * mode, giant_after); *logLen gets the row count. 0 or -1. */

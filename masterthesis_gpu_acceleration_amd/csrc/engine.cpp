@@ -1146,6 +1146,25 @@ int gpu_autotune(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags, 
             if (ms < bestMs) { bestMs = ms; bestGa = gaf; }
         }
         C->giantAfter = bestGa;
+        // second pass over the budget and the thread-mode waves with every other
+        // setting chosen: the first pass tried them under the defaults, where a
+        // budget can win that loses once the waves and work units are tuned
+        for (int32_t b : budgets) {
+            if (b == C->tune.budget) continue;
+            const int32_t keep = C->tune.budget;
+            C->tune.budget = b;
+            if (!trial(ms)) return -1;
+            if (ms < bestMs) bestMs = ms;
+            else C->tune.budget = keep;
+        }
+        for (int q = 0; q < 4; ++q) {
+            if (tw[q] == C->threadWavesPerCU) continue;
+            const int keep = C->threadWavesPerCU;
+            C->threadWavesPerCU = tw[q];
+            if (!trial(ms)) return -1;
+            if (ms < bestMs) bestMs = ms;
+            else C->threadWavesPerCU = keep;
+        }
         C->useGraph = graphWas;
         if (logLen) *logLen = std::min(nlog, logCap);
         return 0;
