@@ -168,9 +168,9 @@ double gpu_copy_probe(void* ctx, int64_t bytes);
  * wave64 mode), thread-mode waves per CU and work units per wave, wave-mode
  * waves per CU, head-object bitmaps
  * in HBM rows or LDS (auto dedup only), and the steps before a wave-mode
- * seed is cut into sub-items -- by coordinate descent, then tries the budget
- * and thread-mode waves again with the rest chosen, and keeps the fastest
- * in the context. If `log` is given it receives up to logCap rows of
+ * seed is cut into sub-items -- budget and thread-mode waves as a grid,
+ * the others one at a time, then budget and waves again -- and keeps the
+ * fastest in the context. If `log` is given it receives up to logCap rows of
  * (budget, thread waves/CU, wave waves/CU, ms, escalated seeds, pairs dedup
  * mode, giant_after); *logLen gets the row count. 0 or -1. */
 int gpu_autotune(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags, double* log, int32_t logCap,

@@ -150,12 +150,18 @@ bool Context::ensure_dedup() {
 }
 
 void giant_prepare(GiantBufs& gb, int after, GiantArgs& ga, hipStream_t s, int phase) {
+    ZeroRanges z{};
+    giant_args(gb, after, ga, phase, z);
+    launch_zero(z, s);
+}
+
+void giant_args(GiantBufs& gb, int after, GiantArgs& ga, int phase, ZeroRanges& z) {
     uint32_t* ctl = gb.ctl.as<uint32_t>();
     ga.stats = gb.stats.as<unsigned long long>() + phase * kWaveStats;
-    launch_fill_u64(ga.stats, kWaveStats, 0, s);
-    launch_fill_u32(ctl, 4, 0, s);
-    launch_fill_u32(gb.remaining.as<uint32_t>(), gb.cap, 0, s);
-    launch_fill_u32(gb.found.as<uint32_t>(), gb.cap, 0, s);
+    zero_add(z, ga.stats, 2 * kWaveStats);
+    zero_add(z, ctl, 4);
+    zero_add(z, gb.remaining.as<uint32_t>(), gb.cap);
+    zero_add(z, gb.found.as<uint32_t>(), gb.cap);
     ga.after = after;
     ga.cap = gb.cap;
     ga.list = gb.list.as<HeavyItem>();
@@ -508,13 +514,25 @@ void batch_enqueue(Context& C, Batch& B, const ExecTarget& t) {
     hipStream_t st = t.st;
     uint32_t* ctl = B.ctl.as<uint32_t>();
     unsigned long long* counts = B.counts.as<unsigned long long>();
-    launch_fill_u64(counts, 2 * (int64_t)B.capRules, 0, st);
-    launch_fill_u32(ctl, kCtlWords, 0, st);
-    if (t.ev) (void)hipEventRecord(t.ev[0], st);
     const bool fuse = C.fuseActive(t.pairsDedup);
     const uint32_t ncPh[2] = {B.nChunks[0] - (fuse ? B.nFusable : 0u), B.nChunks[1]};
     // ungated: no pairs chunk reads a support count, so the phases may overlap
     const bool fork = t.st2 && t.giants2 && !t.ev && (B.flags & GPU_RULES_UNGATED) && ncPh[0] && ncPh[1];
+    // one launch zeroes the counters and each phase's giant counters and
+    // flags -- except a second phase's that share the first phase's giant
+    // buffers, which are zeroed between the phases
+    GiantBufs* const gbOf[2] = {t.giants, fork ? t.giants2 : t.giants};
+    GiantArgs gas[2] = {};
+    bool early[2] = {false, false};
+    ZeroRanges z{};
+    zero_add(z, counts, 4 * (int64_t)B.capRules);
+    zero_add(z, ctl, kCtlWords);
+    for (int ph = 0; ph < 2; ++ph) {
+        early[ph] = ncPh[ph] && !(ph == kPairs && ncPh[0] && gbOf[1] == gbOf[0]);
+        if (early[ph]) giant_args(*gbOf[ph], t.giantAfter, gas[ph], ph, z);
+    }
+    launch_zero(z, st);
+    if (t.ev) (void)hipEventRecord(t.ev[0], st);
     if (fork) {
         (void)hipEventRecord(t.forkEv, st);
         (void)hipStreamWaitEvent(t.st2, t.forkEv, 0);
@@ -522,7 +540,7 @@ void batch_enqueue(Context& C, Batch& B, const ExecTarget& t) {
     const hipStream_t st0 = st;
     for (int ph = 0; ph < 2; ++ph) {
         const hipStream_t st = (fork && ph == kPairs) ? t.st2 : st0;
-        GiantBufs* gbp = (fork && ph == kPairs) ? t.giants2 : t.giants;
+        GiantBufs* gbp = gbOf[ph];
         const uint32_t nc = ncPh[ph];
         const bool fusePh = fuse && ph == kPairs;
         uint32_t* c = ctl + ph * 4;
@@ -543,7 +561,8 @@ void batch_enqueue(Context& C, Batch& B, const ExecTarget& t) {
                         c + 1, B.heavyCap[ph], fusePh ? 1u : 0u, c + 2, counts, t.dedup, t.rowWords, sink};
             wa.supTab = B.supTab.as<int32_t>();
             const int mode = ph == kPairs ? t.pairsDedup : kDedupRows;
-            giant_prepare(*gbp, t.giantAfter, wa.giant, st, ph);
+            if (!early[ph]) giant_prepare(*gbp, t.giantAfter, gas[ph], st, ph);
+            wa.giant = gas[ph];
             const int wb = t.pairsDedup == kDedupLds ? C.waveBlocksFor(B.slots[ph]) : t.waveBlocks;
             launch_wave_batch(ph, B.slots[ph], mode, wa, C.dg, wb, st);
             if (t.giantAfter > 0) {
@@ -1080,34 +1099,36 @@ int gpu_autotune(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags, 
             ++nlog;
             return true;
         };
-        // coordinate descent: budget, then thread-mode waves, then wave-mode waves
+        // budget and thread-mode waves together (they interact: a small budget
+        // pays only with many waves, and at few waves every budget ties), then
+        // the other settings one at a time
         const int32_t budgets[] = {64, 128, 256, 1024, 4096, 16384, 65536};
         const int tw[] = {8, 16, 24, 32};
         const int ww[] = {4, 8, 16};
+        constexpr int nB = 7, nT = 4;
+        double grid[nB][nT];
         double bestMs = 1e30, ms = 0;
-        int32_t bestBudget = C->tune.budget;
-        for (int32_t b : budgets) {
-            C->tune.budget = b;
-            if (!trial(ms)) return -1;
-            if (ms < bestMs) { bestMs = ms; bestBudget = b; }
-        }
-        C->tune.budget = bestBudget;
-        // thread-mode waves: the most waves within 2% of the fastest, so that
-        // near-ties (the phases share the GPU in a replay) resolve the same way
-        // every time and each kernel keeps its full share when run alone
-        int bestTw = C->threadWavesPerCU;
-        double twMs[4] = {};
-        double twMin = bestMs;
-        for (int q = 0; q < 4; ++q) {
-            C->threadWavesPerCU = tw[q];
-            if (!trial(ms)) return -1;
-            twMs[q] = ms;
-            twMin = std::min(twMin, ms);
-        }
-        if (twMin < bestMs) {
-            for (int q = 0; q < 4; ++q)
-                if (twMs[q] <= twMin * 1.02) { bestTw = tw[q]; bestMs = twMs[q]; }
-        }
+        for (int b = 0; b < nB; ++b)
+            for (int q = 0; q < nT; ++q) {
+                C->tune.budget = budgets[b];
+                C->threadWavesPerCU = tw[q];
+                if (!trial(ms)) return -1;
+                grid[b][q] = ms;
+                bestMs = std::min(bestMs, ms);
+            }
+        // the most thread-mode waves within 2% of the fastest (in a replay the
+        // phases share the GPU, so neighbouring settings tie and the choice
+        // would otherwise follow noise), then the fastest budget with them
+        int bestB = -1, bestQ = -1;
+        for (int q = nT - 1; q >= 0 && bestB < 0; --q)
+            for (int b = 0; b < nB; ++b)
+                if (grid[b][q] <= bestMs * 1.02 && (bestB < 0 || grid[b][q] < grid[bestB][bestQ])) {
+                    bestB = b;
+                    bestQ = q;
+                }
+        C->tune.budget = budgets[bestB];
+        const int bestTw = tw[bestQ];
+        bestMs = grid[bestB][bestQ];
         C->threadWavesPerCU = bestTw;
         // thread-mode work units per wave (unit size)
         int bestCpw = C->chunksPerWave;
@@ -1147,8 +1168,8 @@ int gpu_autotune(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags, 
         }
         C->giantAfter = bestGa;
         // second pass over the budget and the thread-mode waves with every other
-        // setting chosen: the first pass tried them under the defaults, where a
-        // budget can win that loses once the waves and work units are tuned
+        // setting chosen (the grid tried them under the default work units,
+        // wave-mode waves and giant threshold)
         for (int32_t b : budgets) {
             if (b == C->tune.budget) continue;
             const int32_t keep = C->tune.budget;

@@ -331,6 +331,8 @@ struct ExecTarget {
 // Zeroes the giant pass's counters and flags for one wave-mode phase and
 // fills a GiantArgs for it (after == 0: no seed is listed).
 void giant_prepare(GiantBufs& gb, int after, GiantArgs& ga, hipStream_t s, int phase);
+// The same without the launch: fills ga and appends the words to zero to z.
+void giant_args(GiantBufs& gb, int after, GiantArgs& ga, int phase, ZeroRanges& z);
 
 // Bytes of bitmap rows allowed before body-pair dedup switches to the
 // sort-unique path (auto mode).

@@ -25,6 +25,15 @@ __global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         p[i] = v;
 }
+// One range per grid row.
+__global__ void k_zero_ranges(ZeroRanges z) {
+    const int r = blockIdx.y;
+    if (r >= z.count) return;
+    uint32_t* p = z.p[r];
+    const int64_t n = z.n[r];
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
 
 void launch_thread_batch(int kind, int slots, const ThreadArgs& a, const DevGraph& G, int blocks, hipStream_t s) {
     switch (slot_class(slots)) {
@@ -98,6 +107,12 @@ void launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hip
 }
 void launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s) {
     hipLaunchKernelGGL(k_fill_u32, dim3(fill_blocks(n)), dim3(256), 0, s, p, n, v);
+}
+void launch_zero(const ZeroRanges& z, hipStream_t s) {
+    if (z.count <= 0) return;
+    int64_t most = 1;
+    for (int r = 0; r < z.count; ++r) most = z.n[r] > most ? z.n[r] : most;
+    hipLaunchKernelGGL(k_zero_ranges, dim3(fill_blocks(most), z.count), dim3(256), 0, s, z);
 }
 
 }  // namespace gr

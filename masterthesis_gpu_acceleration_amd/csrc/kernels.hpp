@@ -148,6 +148,24 @@ extern template struct SlotKernels<4>;
 extern template struct SlotKernels<8>;
 extern template struct SlotKernels<16>;
 
+// Word ranges zeroed by one launch (a pass's counters and flags in one
+// kernel instead of one fill each).
+constexpr int kZeroRanges = 12;
+struct ZeroRanges {
+    uint32_t* p[kZeroRanges];
+    int64_t n[kZeroRanges];   // 32-bit words
+    int32_t count;
+};
+// Appends a range; false when the list is full.
+inline bool zero_add(ZeroRanges& z, void* p, int64_t words) {
+    if (z.count >= kZeroRanges) return false;
+    z.p[z.count] = static_cast<uint32_t*>(p);
+    z.n[z.count] = words;
+    ++z.count;
+    return true;
+}
+void launch_zero(const ZeroRanges& z, hipStream_t s);
+
 void launch_copy(const void* src, void* dst, int64_t bytes, hipStream_t s);
 void launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t s);
 void launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);
