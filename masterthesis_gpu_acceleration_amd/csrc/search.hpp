@@ -477,9 +477,13 @@ __device__ __forceinline__ uint64_t thread_chunk(ThreadLdsT<levels_of(NS)>& W, c
                 item = begin + off;
                 active = bind_seed(P, G, (int)item, v);
                 if (active) {
-                    W.frame[0][lane] = enter(W, P, 0, G, v);
+                    const Frame f0 = enter(W, P, 0, G, v);
+                    W.frame[0][lane] = f0;
                     d = 0;
-                    budget = budget0;
+                    // a distinct-pair search visits every level-0 entry at one
+                    // budget unit or more each: a longer level-0 row than the
+                    // budget is sure to escalate, so it does at its first step
+                    budget = (pairs && f0.end - f0.i > budget0) ? 0 : budget0;
                     set.n = 0;
                 }
             }
