@@ -84,7 +84,9 @@ typedef struct {
     int64_t bodyPairs;     /* distinct body pairs regardless of support; -1 if not computed   */
     double  headCoverage;
     double  confidence;
-    int32_t status;        /* 0 ok, 1 unsupported shape, 2 parse error */
+    int32_t status;        /* 0 ok, 1 unsupported shape, 2 parse error or failed evaluation
+                              (serving: device error, counts beyond the library's buffers),
+                              3 refused by the serving memory budget */
     int32_t latencyUs;     /* GPU_RULES_PER_RULE: host wall time of this rule's launches (us) */
 } GpuRuleResult64;
 
@@ -199,7 +201,10 @@ int gpu_autotune(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags, 
  * wave-mode waves per CU still fit, else HBM bitmap rows unless a row per CU
  * exceeds 8 GiB, else sort-unique; 1 HBM bitmap rows; 2 sort-unique; 3 LDS
  * bitmaps where they fit, else rows), "sort_cap" (initial key capacity of the
- * sort-unique buffers, >= 4096; they grow on overflow). Returns 0, or -1 for
+ * sort-unique buffers, >= 4096; they grow on overflow), "sort_cap_max" (the
+ * largest capacity they may grow to, 4096 .. 2^31, default 2^31: a pass that
+ * emits more keys fails -- gpu_evaluate_rules* and gpu_batch_run return -1,
+ * served requests complete with status 2). Returns 0, or -1 for
  * an unknown key or value. gpu_get_tuning also reads "num_cus",
  * "wave_slots", "sort_active" (0/1), "sort_cap", "pairs_dedup" (the
  * DedupMode in use: 0 HBM rows, 1 sort, 2 LDS), "fuse_active" (0/1: the
@@ -267,6 +272,15 @@ int64_t gpu_host_term_id(void* handle, const char* term);
 /* Parses a rule line (isJson = 0) or JSON object into GPU_RULE_WORDS words.
  * Unknown constants get ids >= the graph's term count. 0 or -1. */
 int gpu_host_parse_rule(void* handle, const char* rule, int32_t isJson, int32_t* words);
+/* The host's decision on a pass's device counters, as the library takes it
+ * after every pass: which = 0, keys the sort-unique path emitted (count)
+ * against the buffers' capacity and the largest capacity allowed -- 0
+ * counted, 1 grow the buffers and run the pass again, 2 fail; which = 1,
+ * seeds escalated to wave mode against the heavy queue's capacity -- 0 or 2.
+ * -1 on a negative argument or unknown `which`. Both counters are 64-bit
+ * or bounded on the device, so they cannot wrap. */
+int32_t gpu_host_count_action(int32_t which, int64_t count, int64_t cap, int64_t capMax);
+
 /* Head figures and plan facts of a structured rule: info[0] headSize,
  * info[1] headSupport, info[2] status, info[3] support launched (0/1),
  * info[4] body pairs launched (0/1), info[5] support seeds, info[6] pair seeds. */

@@ -193,12 +193,20 @@ bool run_one(Context& C, const Plan& P, const ConflictRefs& R, const Plan& PW, c
     count = 0;
     ms = 0;
     if (items <= 0) return true;
+    if ((uint64_t)items > kMaxPhaseSeeds) {
+        std::fprintf(stderr, "[gpu_rules] a rule with %lld seeds exceeds the heavy queue's 32-bit range\n",
+                     (long long)items);
+        return false;
+    }
     if ((uint64_t)items > C.heavy1Cap) {
         if (!C.heavy1.ensure((size_t)items * sizeof(HeavyItem))) return false;
         C.heavy1Cap = (uint32_t)items;
     }
+    // C.one: [0] count (u64); +64: u32 heavy count, wave cursor; +96: u64 keys
+    // emitted (sort path); +128: ConflictRefs
     unsigned long long* d_count = C.one.as<unsigned long long>();
-    uint32_t* ctl = reinterpret_cast<uint32_t*>(C.one.as<char>() + 64);   // heavy count, wave cursor, keys emitted
+    uint32_t* ctl = reinterpret_cast<uint32_t*>(C.one.as<char>() + 64);
+    unsigned long long* d_emitted = reinterpret_cast<unsigned long long*>(C.one.as<char>() + 96);
     ConflictRefs* d_refs = reinterpret_cast<ConflictRefs*>(C.one.as<char>() + 128);
     const bool conf = has_conflicts(P) || has_conflicts(PW);
     if (conf) {
@@ -208,41 +216,51 @@ bool run_one(Context& C, const Plan& P, const ConflictRefs& R, const Plan& PW, c
     const int mode = PW.kind == kPairs ? C.pairsDedup() : kDedupRows;
     const bool sort = mode == kDedupSort;
     const int slots = std::max(plan_slots(P, R), plan_slots(PW, RW));
-    if (sort && !C.sortb.ensure(1)) return false;
+    if (sort && !C.sortb.ensure(1, kSortCapMin, C.sortCapMax)) return false;
     const int hoBits = bits_for((uint64_t)C.g.nE);
     const int hBits = bits_for((uint64_t)items);
     for (;;) {
         launch_fill_u64(d_count, 1, 0, C.st);
-        launch_fill_u32(ctl, 4, 0, C.st);
+        launch_fill_u32(ctl, 16, 0, C.st);   // control words and the emitted-key counter
         GR_HIP(hipEventRecord(C.ev[0], C.st));
         const int64_t waves = (items + kWave - 1) / kWave;
         const int blocks = (int)std::min<int64_t>((waves + 3) / 4, C.threadBlocks());
         OneArgs oa{conf ? d_refs : nullptr, (uint32_t)items, C.heavy1Cap, d_count, C.heavy1.as<HeavyItem>(), ctl + 0};
         launch_thread_one(P.kind, slots, P, oa, C.dg, blocks, C.st);
         SortSink sink{};
-        if (sort) sink = SortSink{C.sortb.keys[0].as<unsigned long long>(), ctl + 2, C.sortb.cap, hoBits};
+        if (sort) sink = SortSink{C.sortb.keys[0].as<unsigned long long>(), d_emitted, C.sortb.cap, hoBits};
         WaveArgs wa{nullptr, conf ? d_refs + 1 : nullptr, C.heavy1.as<HeavyItem>(), ctl + 0, C.heavy1Cap, 0, ctl + 1,
                     d_count, C.dedup.as<uint32_t>(), C.rowWords, sink};
         launch_wave_one(PW.kind, slots, mode, PW, wa, C.dg, C.waveBlocksFor(slots), C.st);
         if (sort) {
             SortJob j{{C.sortb.keys[0].as<unsigned long long>(), C.sortb.keys[1].as<unsigned long long>()},
-                      ctl + 2, C.sortb.cap, sort_max_tiles(C.sortb.cap), C.sortb.hist.as<uint32_t>(),
+                      d_emitted, C.sortb.cap, sort_max_tiles(C.sortb.cap), C.sortb.hist.as<uint32_t>(),
                       hoBits + hBits, hoBits, nullptr, nullptr, d_count, C.sortBlocks()};
             launch_sort_unique(j, C.st);
         }
         GR_HIP(hipGetLastError());
         GR_HIP(hipEventRecord(C.ev[1], C.st));
-        GR_HIP(hipMemcpyAsync(C.hOne, C.one.p, 80, hipMemcpyDeviceToHost, C.st));
+        GR_HIP(hipMemcpyAsync(C.hOne, C.one.p, 128, hipMemcpyDeviceToHost, C.st));
         GR_HIP(hipStreamSynchronize(C.st));
         const uint32_t* hctl = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(C.hOne) + 64);
-        if (sort) C.lastEmitted = hctl[2];
-        if (sort && hctl[2] > C.sortb.cap) {
-            if (C.sortb.cap >= kSortCapMax) {
-                std::fprintf(stderr, "[gpu_rules] body-pair keys of one rule exceed %u\n", kSortCapMax);
+        const uint64_t emitted = *reinterpret_cast<const unsigned long long*>(reinterpret_cast<char*>(C.hOne) + 96);
+        if (heavy_count_action(hctl[0], C.heavy1Cap) != kCountsOk) {
+            std::fprintf(stderr, "[gpu_rules] %u seeds escalated, beyond the heavy queue's %u\n", hctl[0],
+                         C.heavy1Cap);
+            return false;
+        }
+        if (sort) {
+            C.lastEmitted = (int64_t)emitted;
+            const int32_t act = sort_count_action(emitted, C.sortb.cap, C.sortCapMax);
+            if (act == kCountsFail) {
+                std::fprintf(stderr, "[gpu_rules] body-pair keys of one rule (%llu) exceed the sort capacity %u\n",
+                             (unsigned long long)emitted, C.sortb.cap);
                 return false;
             }
-            if (!C.sortb.ensure(hctl[2])) return false;
-            continue;
+            if (act == kCountsRegrow) {
+                if (!C.sortb.ensure(emitted, kSortCapMin, C.sortCapMax)) return false;
+                continue;
+            }
         }
         float f = 0;
         GR_HIP(hipEventElapsedTime(&f, C.ev[0], C.ev[1]));
@@ -286,8 +304,6 @@ bool eval_rule_plain(Context& C, const RuleIR& r, bool gated, GpuRuleResult64& r
 }
 
 // ------------------------------------------------------------ batch path
-
-constexpr int kCtlWords = 8;   // per phase: thread cursor, heavy count, wave cursor, spare
 
 // Body pairs depend only on a rule's body and on which of its variables are
 // the head subject and object, not on the head predicate. Rules are keyed by
@@ -469,6 +485,12 @@ bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const s
                 }
             }
     }
+    for (int ph = 0; ph < 2; ++ph)
+        if (items[ph] > kMaxPhaseSeeds) {
+            std::fprintf(stderr, "[gpu_rules] a batch phase with %llu seeds exceeds the heavy queue's 32-bit range; "
+                                 "split the batch\n", (unsigned long long)items[ph]);
+            return false;
+        }
     std::vector<Chunk> all;
     uint64_t heavyTotal = 0;
     for (int ph = 0; ph < 2; ++ph) {
@@ -510,7 +532,8 @@ bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const s
 // then left out (and their pairs chunks are not gated). A search shared by
 // rules with equal bodies is never gated; each of its fusable rules' support
 // goes to that rule's counter, so the phases never write the same counter.
-void batch_enqueue(Context& C, Batch& B, const ExecTarget& t) {
+hipError_t batch_enqueue(Context& C, Batch& B, const ExecTarget& t) {
+    HipFirst err;
     hipStream_t st = t.st;
     uint32_t* ctl = B.ctl.as<uint32_t>();
     unsigned long long* counts = B.counts.as<unsigned long long>();
@@ -532,10 +555,10 @@ void batch_enqueue(Context& C, Batch& B, const ExecTarget& t) {
         if (early[ph]) giant_args(*gbOf[ph], t.giantAfter, gas[ph], ph, z);
     }
     launch_zero(z, st);
-    if (t.ev) (void)hipEventRecord(t.ev[0], st);
+    if (t.ev) err(hipEventRecord(t.ev[0], st), "hipEventRecord");
     if (fork) {
-        (void)hipEventRecord(t.forkEv, st);
-        (void)hipStreamWaitEvent(t.st2, t.forkEv, 0);
+        err(hipEventRecord(t.forkEv, st), "hipEventRecord(fork)");
+        err(hipStreamWaitEvent(t.st2, t.forkEv, 0), "hipStreamWaitEvent(fork)");
     }
     const hipStream_t st0 = st;
     for (int ph = 0; ph < 2; ++ph) {
@@ -551,12 +574,12 @@ void batch_enqueue(Context& C, Batch& B, const ExecTarget& t) {
             const int blocks = std::min<int>(C.threadBlocks(), (int)((nc + 3) / 4));
             launch_thread_batch(ph, B.slots[ph], ta, C.dg, std::max(blocks, 1), st);
         }
-        if (t.ev) (void)hipEventRecord(t.ev[1 + 2 * ph], st);
+        if (t.ev) err(hipEventRecord(t.ev[1 + 2 * ph], st), "hipEventRecord");
         if (nc) {
             const bool sort = ph == kPairs && t.pairsDedup == kDedupSort && t.sort;
             const int hoBits = bits_for((uint64_t)C.g.nE);
             SortSink sink{};
-            if (sort) sink = SortSink{t.sort->keys[0].as<unsigned long long>(), c + 3, t.sort->cap, hoBits};
+            if (sort) sink = SortSink{t.sort->keys[0].as<unsigned long long>(), ctl_emitted(ctl, ph), t.sort->cap, hoBits};
             WaveArgs wa{B.wplans.as<Plan>(), B.wrefs.as<ConflictRefs>(), B.heavy.as<HeavyItem>() + B.heavyOff[ph],
                         c + 1, B.heavyCap[ph], fusePh ? 1u : 0u, c + 2, counts, t.dedup, t.rowWords, sink};
             wa.supTab = B.supTab.as<int32_t>();
@@ -567,30 +590,46 @@ void batch_enqueue(Context& C, Batch& B, const ExecTarget& t) {
             launch_wave_batch(ph, B.slots[ph], mode, wa, C.dg, wb, st);
             if (t.giantAfter > 0) {
                 launch_giants(ph, B.slots[ph], mode, wa, C.dg, wb, st);
-                (void)hipMemcpyAsync(B.hGiant + ph, wa.giant.count, 4, hipMemcpyDeviceToHost, st);
+                err(hipMemcpyAsync(B.hGiant + ph, wa.giant.count, 4, hipMemcpyDeviceToHost, st), "giant count copy");
             }
-            (void)hipMemcpyAsync(B.hWaveStats + ph * kWaveStats, wa.giant.stats, kWaveStats * 8,
-                                 hipMemcpyDeviceToHost, st);
+            err(hipMemcpyAsync(B.hWaveStats + ph * kWaveStats, wa.giant.stats, kWaveStats * 8, hipMemcpyDeviceToHost,
+                               st), "wave stats copy");
             if (sort) {
-                if (t.ev) (void)hipEventRecord(t.ev[6], st);
+                if (t.ev) err(hipEventRecord(t.ev[6], st), "hipEventRecord");
                 SortJob j{{t.sort->keys[0].as<unsigned long long>(), t.sort->keys[1].as<unsigned long long>()},
-                          c + 3, t.sort->cap, sort_max_tiles(t.sort->cap), t.sort->hist.as<uint32_t>(),
+                          ctl_emitted(ctl, ph), t.sort->cap, sort_max_tiles(t.sort->cap), t.sort->hist.as<uint32_t>(),
                           hoBits + bits_for(B.heavyCap[ph]), hoBits, B.heavy.as<HeavyItem>() + B.heavyOff[ph],
                           B.wplans.as<Plan>(), counts, t.sortBlocks};
                 launch_sort_unique(j, st);
             }
         }
-        if (t.ev) (void)hipEventRecord(t.ev[2 + 2 * ph], st);
+        if (t.ev) err(hipEventRecord(t.ev[2 + 2 * ph], st), "hipEventRecord");
     }
     if (fork) {
-        (void)hipEventRecord(t.joinEv, t.st2);
-        (void)hipStreamWaitEvent(st0, t.joinEv, 0);
+        err(hipEventRecord(t.joinEv, t.st2), "hipEventRecord(join)");
+        err(hipStreamWaitEvent(st0, t.joinEv, 0), "hipStreamWaitEvent(join)");
     }
-    (void)hipMemcpyAsync(B.hCounts, counts, (size_t)2 * B.capRules * 8, hipMemcpyDeviceToHost, st0);
-    (void)hipMemcpyAsync(B.hCtl, ctl, kCtlWords * 4, hipMemcpyDeviceToHost, st0);
+    err(hipMemcpyAsync(B.hCounts, counts, (size_t)2 * B.capRules * 8, hipMemcpyDeviceToHost, st0), "counts copy");
+    err(hipMemcpyAsync(B.hCtl, ctl, kCtlWords * 4, hipMemcpyDeviceToHost, st0), "control copy");
+    err(hipGetLastError(), "kernel launch");
+    if (!err.ok()) log_err(err.what, err.e);
+    return err.e;
 }
 
-int64_t batch_emitted(const Batch& B) { return B.hCtl ? (int64_t)B.hCtl[7] : 0; }
+int64_t batch_emitted(const Batch& B) {
+    return B.hCtl ? (int64_t)*ctl_emitted(B.hCtl, kPairs) : 0;
+}
+
+bool batch_heavy_ok(const Batch& B) {
+    if (!B.hCtl) return true;
+    for (int ph = 0; ph < 2; ++ph)
+        if (heavy_count_action(B.hCtl[4 * ph + 1], B.heavyCap[ph]) != kCountsOk) {
+            std::fprintf(stderr, "[gpu_rules] phase %d: %u seeds escalated, beyond the heavy queue's %u\n", ph,
+                         B.hCtl[4 * ph + 1], B.heavyCap[ph]);
+            return false;
+        }
+    return true;
+}
 
 void batch_collect(Batch& B, GpuRuleResult64* out) {
     B.lastHeavy = (int64_t)B.hCtl[1] + B.hCtl[5];
@@ -618,7 +657,7 @@ bool batch_run(Context& C, Batch& B, GpuRuleResult64* out) {
     SortBufs* sb = nullptr;
     const int mode = C.pairsDedup();
     if (mode == kDedupSort && B.nChunks[kPairs]) {
-        if (!C.sortb.ensure(1)) return false;
+        if (!C.sortb.ensure(1, kSortCapMin, C.sortCapMax)) return false;
         sb = &C.sortb;
     }
     for (;;) {
@@ -657,8 +696,13 @@ bool batch_run(Context& C, Batch& B, GpuRuleResult64* out) {
                     t.joinEv = C.joinEv;
                     t.giants2 = &C.giants2;
                 }
-                batch_enqueue(C, B, t);
+                const hipError_t ee = batch_enqueue(C, B, t);
                 GR_HIP(hipStreamEndCapture(C.st, &B.graph));
+                if (ee != hipSuccess) {
+                    (void)hipGraphDestroy(B.graph);
+                    B.graph = nullptr;
+                    return false;
+                }
                 GR_HIP(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
             }
             GR_HIP(hipEventRecord(C.ev[0], C.st));
@@ -671,8 +715,8 @@ bool batch_run(Context& C, Batch& B, GpuRuleResult64* out) {
             B.lastSupMs = B.lastPairMs = B.lastSortMs = 0;
             for (double& k : B.kernelMs) k = 0;
         } else {
-            batch_enqueue(C, B, ExecTarget{C.st, C.dedup.as<uint32_t>(), C.rowWords, C.waveBlocks(), C.ev, sb,
-                                           C.sortBlocks(), mode, &C.giants, C.giantAfter});
+            GR_HIP(batch_enqueue(C, B, ExecTarget{C.st, C.dedup.as<uint32_t>(), C.rowWords, C.waveBlocks(), C.ev, sb,
+                                                  C.sortBlocks(), mode, &C.giants, C.giantAfter}));
             GR_HIP(hipEventRecord(C.ev[5], C.st));
             GR_HIP(hipStreamSynchronize(C.st));
             float t[5] = {0, 0, 0, 0, 0};
@@ -692,13 +736,19 @@ bool batch_run(Context& C, Batch& B, GpuRuleResult64* out) {
             B.lastMs = t[4];
         }
         GR_HIP(hipGetLastError());
-        if (sb && batch_emitted(B) > (int64_t)sb->cap) {   // keys overflowed: larger buffer, run again
-            if (sb->cap >= kSortCapMax) {
-                std::fprintf(stderr, "[gpu_rules] body-pair keys of one batch exceed %u\n", kSortCapMax);
+        if (!batch_heavy_ok(B)) return false;
+        if (sb) {   // keys overflowed: larger buffer and run again, or fail past the largest one
+            const uint64_t emitted = (uint64_t)batch_emitted(B);
+            const int32_t act = sort_count_action(emitted, sb->cap, C.sortCapMax);
+            if (act == kCountsFail) {
+                std::fprintf(stderr, "[gpu_rules] body-pair keys of one batch (%llu) exceed the sort capacity %u\n",
+                             (unsigned long long)emitted, sb->cap);
                 return false;
             }
-            if (!sb->ensure((uint64_t)batch_emitted(B))) return false;
-            continue;
+            if (act == kCountsRegrow) {
+                if (!sb->ensure(emitted, kSortCapMin, C.sortCapMax)) return false;
+                continue;
+            }
         }
         break;
     }
@@ -1301,9 +1351,15 @@ int gpu_set_tuning(void* ctx, const char* key, int64_t value) {
         C->giantAfter = (int)std::max<int64_t>(0, value);
     } else if (k == "sort_cap") {
         // start the sort-unique buffers at this many keys (they still grow on overflow)
-        if (value < kSortTile || value > (int64_t)kSortCapMax) return -1;
+        if (value < kSortTile || value > (int64_t)C->sortCapMax) return -1;
         C->sortb.release();
-        if (!C->sortb.ensure((uint64_t)value, (uint64_t)value)) return -1;
+        if (!C->sortb.ensure((uint64_t)value, (uint64_t)value, C->sortCapMax)) return -1;
+    } else if (k == "sort_cap_max") {
+        // largest key capacity the sort-unique buffers may grow to; a pass
+        // that emits more keys fails (default and upper limit 2^31)
+        if (value < kSortTile || value > (int64_t)kSortCapMax) return -1;
+        C->sortCapMax = (uint32_t)value;
+        if (C->sortb.cap > C->sortCapMax) C->sortb.release();
     } else return -1;
     if (C->lastBatch) { delete C->lastBatch; C->lastBatch = nullptr; }
     return 0;
@@ -1338,6 +1394,7 @@ int64_t gpu_get_tuning(void* ctx, const char* key) {
             if (k == std::string(ph ? "wave_pairs_" : "wave_support_") + kStatNames[q])
                 return (int64_t)C->lastWaveStats[ph * kWaveStats + q];
     if (k == "sort_cap") return C->sortb.cap;
+    if (k == "sort_cap_max") return C->sortCapMax;
     return -1;
 }
 
@@ -1433,6 +1490,13 @@ int gpu_host_parse_rule(void* handle, const char* rule, int32_t isJson, int32_t*
     } catch (const std::exception&) {
         return -1;
     }
+}
+
+int32_t gpu_host_count_action(int32_t which, int64_t count, int64_t cap, int64_t capMax) {
+    if (count < 0 || cap < 0 || capMax < 0) return -1;
+    if (which == 0) return sort_count_action((uint64_t)count, (uint64_t)cap, (uint64_t)capMax);
+    if (which == 1) return heavy_count_action((uint64_t)count, (uint64_t)cap);
+    return -1;
 }
 
 int gpu_host_plan_info(void* handle, const int32_t* words, int64_t* info) {

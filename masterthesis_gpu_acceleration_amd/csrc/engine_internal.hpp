@@ -75,17 +75,18 @@ struct DevBuf {
 // Buffers of the sort-unique dedup path: two key arrays (LSD ping-pong) of
 // `cap` keys each and the radix histogram. The capacity is a power of two;
 // it doubles (at least) whenever a pass emitted more keys than it holds, up
-// to kSortCapMax keys (32 GiB of key arrays -- sized for a 288 GB device).
+// to kSortCapMax keys (32 GiB of key arrays -- sized for a 288 GB device;
+// key positions stay below 2^31, so the sort kernels index in 32 bits).
 constexpr uint32_t kSortCapMin = 1u << 22;
 constexpr uint32_t kSortCapMax = 1u << 31;
 struct SortBufs {
     DevBuf keys[2], hist;
     uint32_t cap = 0;
-    // Capacity for at least `want` keys (clamped to kSortCapMax); the first
+    // Capacity for at least `want` keys (clamped to capMax); the first
     // allocation holds at least `first` keys.
-    bool ensure(uint64_t want, uint64_t first = kSortCapMin) {
+    bool ensure(uint64_t want, uint64_t first = kSortCapMin, uint64_t capMax = kSortCapMax) {
         uint64_t c = cap ? cap : kSortTile;
-        while ((c < want || (!cap && c < first)) && c < kSortCapMax) c <<= 1;
+        while ((c < want || (!cap && c < first)) && c < capMax) c <<= 1;
         if (c == cap) return true;
         if (!keys[0].ensure(c * 8) || !keys[1].ensure(c * 8) || !hist.ensure(sort_hist_words((uint32_t)c) * 4)) {
             release();
@@ -150,6 +151,38 @@ struct GiantBufs {
 
 struct Context;
 
+// What the host does with a pass's device counters (decided apart from the
+// device code, so the decisions can be checked with forced values):
+//   sort_count_action: keys emitted vs the sort buffers' capacity and the
+//     largest capacity allowed -- counted, regrow and run again, or fail;
+//   heavy_count_action: seeds escalated vs the heavy queue's capacity (the
+//     queue holds every seed of the phase, so a larger count means a broken
+//     pass) -- counted or fail.
+enum CountAction : int32_t { kCountsOk = 0, kCountsRegrow = 1, kCountsFail = 2 };
+inline int32_t sort_count_action(uint64_t emitted, uint64_t cap, uint64_t capMax) {
+    if (emitted <= cap) return kCountsOk;
+    return cap < capMax ? kCountsRegrow : kCountsFail;
+}
+inline int32_t heavy_count_action(uint64_t escalated, uint64_t cap) {
+    return escalated <= cap ? kCountsOk : kCountsFail;
+}
+// Seeds one phase of a batch may hold: the heavy queue's count, cursor and
+// capacity are 32-bit (a larger batch is refused by batch_build).
+constexpr uint64_t kMaxPhaseSeeds = 0xFFFFFFF0ull;
+
+// First error of a sequence of asynchronous HIP calls.
+struct HipFirst {
+    hipError_t e = hipSuccess;
+    const char* what = nullptr;
+    void operator()(hipError_t x, const char* w) {
+        if (e == hipSuccess && x != hipSuccess) {
+            e = x;
+            what = w;
+        }
+    }
+    bool ok() const { return e == hipSuccess; }
+};
+
 // A compiled, device-resident batch of rules.
 struct Batch : Handle {
     Context* ctx = nullptr;
@@ -178,7 +211,7 @@ struct Batch : Handle {
     DevBuf supTab;   // member lists of shared searches that count several rules' support (Plan::supPred)
     unsigned long long* hCounts = nullptr;   // pinned
     size_t hCountsCap = 0;                   // words
-    uint32_t* hCtl = nullptr;                // pinned
+    uint32_t* hCtl = nullptr;                // pinned copy of ctl (kCtlWords words)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     const void* capturedDedup = nullptr;   // context scratch the graph was captured with
@@ -226,6 +259,7 @@ struct Context : Handle {
     int64_t rowWords = 0;
     int waveSlots = 0;
     SortBufs sortb;   // sort-unique dedup buffers (allocated when that path is used)
+    uint32_t sortCapMax = kSortCapMax;   // largest key capacity they may grow to ("sort_cap_max")
     GiantBufs giants; // giant-seed pass
     // Ungated batches replayed as graphs run the support and body-pair phases
     // side by side: the pairs phase on st2 (forked from st by forkEv, joined
@@ -343,8 +377,17 @@ constexpr int64_t kRowBudget = int64_t(8) << 30;
 // rows (16 waves per CU).
 constexpr int kLdsMinWaves = 8;
 
+// Control words of a batch pass (device `ctl`, host copy `hCtl`): per phase
+// ph, words 4*ph + {0 thread cursor, 1 heavy count, 2 wave cursor, 3 spare},
+// then the phases' 64-bit emitted-key counters (sort path) at words 8..11.
+constexpr int kCtlWords = 12;
+inline unsigned long long* ctl_emitted(uint32_t* ctl, int ph) {
+    return reinterpret_cast<unsigned long long*>(ctl + 8) + ph;
+}
 // Keys emitted by B's last pass (sort path), from its host control copy.
 int64_t batch_emitted(const Batch& B);
+// Escalated seeds of B's last pass within each phase's heavy queue?
+bool batch_heavy_ok(const Batch& B);
 
 // Compiles rules into B and uploads plans and chunk lists on stream st.
 bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const std::vector<int32_t>& status,
@@ -354,7 +397,8 @@ bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const s
 std::vector<int32_t> shared_pairs(const std::vector<RuleIR>& rules, const std::vector<CompiledRule>& comp,
                                   const std::vector<int32_t>& status);
 // Enqueues B's device pass (counter fills, kernels, result copies) on t.st.
-void batch_enqueue(Context& C, Batch& B, const ExecTarget& t);
+// Returns the first error of the enqueued calls (logged), or hipSuccess.
+hipError_t batch_enqueue(Context& C, Batch& B, const ExecTarget& t);
 // Fills out[B.n] from B's host-side counters after its pass completed.
 void batch_collect(Batch& B, GpuRuleResult64* out);
 // Builds-if-needed, runs (graph replay or eager) and collects on the context's stream.

@@ -124,18 +124,18 @@ bool build_graph_device(HostGraph& g, const int32_t* s, const int32_t* p, const 
     DevMem ds, dd, dO, k0, k1, k2, k3, hist, cnt;
     if (!ds.alloc((size_t)n * 4) || !dd.alloc((size_t)n * 4) || !dO.alloc((size_t)n * 4) || !k0.alloc((size_t)cap * 8) ||
         !k1.alloc((size_t)cap * 8) || !k2.alloc((size_t)cap * 8) || !k3.alloc((size_t)cap * 8) ||
-        !hist.alloc(sort_hist_words(cap) * 4) || !cnt.alloc(4)) {
+        !hist.alloc(sort_hist_words(cap) * 4) || !cnt.alloc(8)) {
         err = "not enough device memory for the device build";
         return false;
     }
     // uploads complete before the kernels start (synchronous copies)
-    const uint32_t n32 = (uint32_t)n;
+    const unsigned long long n64 = (unsigned long long)n;
     if (n > 0) {
         GD_CHECK(hipMemcpy(ds.p, s, (size_t)n * 4, hipMemcpyHostToDevice));
         GD_CHECK(hipMemcpy(dd.p, dp.data(), (size_t)n * 4, hipMemcpyHostToDevice));
         GD_CHECK(hipMemcpy(dO.p, o, (size_t)n * 4, hipMemcpyHostToDevice));
     }
-    GD_CHECK(hipMemcpy(cnt.p, &n32, 4, hipMemcpyHostToDevice));
+    GD_CHECK(hipMemcpy(cnt.p, &n64, 8, hipMemcpyHostToDevice));
     hipStream_t st = nullptr;
     GD_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     struct StreamGuard {
@@ -157,8 +157,8 @@ bool build_graph_device(HostGraph& g, const int32_t* s, const int32_t* p, const 
     unsigned long long* const posKeys[2] = {k2.as<unsigned long long>(), k3.as<unsigned long long>()};
     int a = 0, b = 0;
     if (n > 0) {
-        a = launch_radix_sort(spoKeys, cnt.as<uint32_t>(), cap, hist.as<uint32_t>(), keyBits, cus * 4, st);
-        b = launch_radix_sort(posKeys, cnt.as<uint32_t>(), cap, hist.as<uint32_t>(), keyBits, cus * 4, st);
+        a = launch_radix_sort(spoKeys, cnt.as<unsigned long long>(), cap, hist.as<uint32_t>(), keyBits, cus * 4, st);
+        b = launch_radix_sort(posKeys, cnt.as<unsigned long long>(), cap, hist.as<uint32_t>(), keyBits, cus * 4, st);
         GD_CHECK(hipGetLastError());
     }
     GD_CHECK(hipStreamSynchronize(st));

@@ -41,12 +41,12 @@ enum DedupMode : int32_t {
 };
 
 // Emission target of the sort-unique path: a wave appends the head objects it
-// finds for heavy seed h as keys (h << hoBits) | object. `count` keeps
-// counting past `cap` (nothing is written there), so the host can tell an
-// overflow and rerun with a larger buffer.
+// finds for heavy seed h as keys (h << hoBits) | object. `count` (64-bit, so
+// it cannot wrap) keeps counting past `cap` (nothing is written there), so
+// the host can tell an overflow and rerun with a larger buffer or fail.
 struct SortSink {
     unsigned long long* keys;
-    uint32_t* count;
+    unsigned long long* count;
     uint32_t cap;
     int32_t hoBits;
 };

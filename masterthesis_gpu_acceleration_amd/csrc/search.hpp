@@ -771,12 +771,12 @@ __device__ __forceinline__ uint32_t mark_batch(Dedup<D, NS>& dd, const int (&ho)
 __device__ __forceinline__ void emit_keys(const SortSink& sink, unsigned long long keyHi, int emit) {
     const uint64_t em = __ballot(emit >= 0);
     if (em == 0) return;
-    uint32_t base = 0;
-    if (lane_id() == 0) base = atomicAdd(sink.count, (uint32_t)__popcll(em));
+    unsigned long long base = 0;
+    if (lane_id() == 0) base = atomicAdd(sink.count, (unsigned long long)__popcll(em));
     base = __shfl(base, 0, kWave);
     if (emit >= 0) {
-        const uint32_t pos = base + (uint32_t)__popcll(em & ((1ull << lane_id()) - 1ull));
-        if (pos < sink.cap) sink.keys[pos] = keyHi | (unsigned long long)(uint32_t)emit;
+        const unsigned long long pos = base + (unsigned long long)__popcll(em & ((1ull << lane_id()) - 1ull));
+        if (pos < (unsigned long long)sink.cap) sink.keys[pos] = keyHi | (unsigned long long)(uint32_t)emit;
     }
 }
 

@@ -56,6 +56,7 @@ struct Slot {
     int64_t reserved = 0;
     int64_t admitNs = 0;
     bool busy = false;
+    bool retired = false;    // its buffers could not be regrown: no further batches
     ~Slot() {
         if (st) (void)hipStreamSynchronize(st);
         dedup.release();
@@ -124,7 +125,7 @@ struct Server : Handle {
             if (!S->dedup.ensure(bytes)) return false;
             GR_HIP(hipMemset(S->dedup.p, 0, bytes));
             if (S->sort) {
-                if (!S->sortb.ensure(1)) return false;
+                if (!S->sortb.ensure(1, kSortCapMin, C->sortCapMax)) return false;
                 bytes += S->sortb.bytes();
             }
             if (!S->giants.ensure(S->rowWords)) return false;
@@ -157,32 +158,50 @@ struct Server : Handle {
         nCompleted++;
     }
 
-    // Finished slots -> completion queue.
+    // Finished slots -> completion queue. A batch whose pass failed, whose
+    // escalated seeds passed the heavy queue, or whose body-pair keys passed
+    // the largest sort capacity completes with status kRuleError for every
+    // request; a slot whose sort buffers could not be regrown is retired.
     int reap() {
         int busy = 0;
         for (auto& S : slots) {
             if (!S->busy) continue;
             hipError_t q = hipEventQuery(S->done);
             if (q == hipErrorNotReady) { ++busy; continue; }
+            bool bad = false;
             if (q != hipSuccess) {
                 log_err("serve: batch failed", q);
-                failed = true;
-            } else if (S->sort && batch_emitted(S->batch) > (int64_t)S->sortb.cap && S->sortb.cap < kSortCapMax) {
-                // body-pair keys overflowed the slot's sort buffers: grow them and run the batch again
-                const size_t was = S->sortb.bytes();
-                if (S->sortb.ensure((uint64_t)batch_emitted(S->batch))) {
-                    budget -= (int64_t)(S->sortb.bytes() - was);
-                    launch(*S);
-                    ++busy;
-                    continue;
+                bad = true;
+            } else if (!batch_heavy_ok(S->batch)) {
+                bad = true;
+            } else if (S->sort) {
+                const uint64_t emitted = (uint64_t)batch_emitted(S->batch);
+                const int32_t act = sort_count_action(emitted, S->sortb.cap, C->sortCapMax);
+                if (act == kCountsRegrow) {
+                    // body-pair keys overflowed the slot's sort buffers: grow them and run the batch again
+                    const size_t was = S->sortb.bytes();
+                    if (S->sortb.ensure(emitted, kSortCapMin, C->sortCapMax)) {
+                        budget -= (int64_t)(S->sortb.bytes() - was);
+                        if (launch(*S)) {
+                            ++busy;
+                            continue;
+                        }
+                    } else {
+                        std::fprintf(stderr, "[gpu_rules] serve: sort buffers of a slot could not grow; slot retired\n");
+                        S->retired = true;
+                    }
+                    bad = true;
+                } else if (act == kCountsFail) {
+                    std::fprintf(stderr, "[gpu_rules] serve: body-pair keys of a batch (%llu) exceed the sort "
+                                         "capacity %u\n", (unsigned long long)emitted, S->sortb.cap);
+                    bad = true;
                 }
-                failed = true;
-                q = hipErrorOutOfMemory;
             }
+            if (bad) failed = true;
             S->out.resize(S->reqs.size());
             batch_collect(S->batch, S->out.data());
             for (size_t i = 0; i < S->reqs.size(); ++i) {
-                if (q != hipSuccess) S->out[i].status = kRuleError;
+                if (bad) S->out[i].status = kRuleError;
                 complete(S->reqs[i], S->out[i], S->admitNs, (int32_t)S->reqs.size());
             }
             S->reqs.clear();
@@ -194,18 +213,48 @@ struct Server : Handle {
         return busy;
     }
 
-    // The slot's batch -> its stream, with the slot's dedup rows or sort buffers.
-    void launch(Slot& S) {
-        batch_enqueue(*C, S.batch, ExecTarget{S.st, S.dedup.as<uint32_t>(), S.rowWords, std::max(1, S.waveSlots),
-                                              nullptr, S.sort ? &S.sortb : nullptr, C->sortBlocks(), S.mode,
-                                              &S.giants, C->giantAfter});
-        (void)hipEventRecord(S.done, S.st);
+    // The slot's batch -> its stream, with the slot's dedup rows or sort
+    // buffers. False if an enqueued call failed (logged).
+    bool launch(Slot& S) {
+        const hipError_t e = batch_enqueue(*C, S.batch, ExecTarget{S.st, S.dedup.as<uint32_t>(), S.rowWords,
+                                                                   std::max(1, S.waveSlots), nullptr,
+                                                                   S.sort ? &S.sortb : nullptr, C->sortBlocks(),
+                                                                   S.mode, &S.giants, C->giantAfter});
+        const hipError_t r = hipEventRecord(S.done, S.st);
+        if (r != hipSuccess) log_err("serve: hipEventRecord", r);
+        return e == hipSuccess && r == hipSuccess;
+    }
+
+    // Completes every pending request with `status` (no usable slot is left).
+    void fail_pending(int32_t status) {
+        std::deque<Request> all;
+        {
+            std::lock_guard<std::mutex> g(mu);
+            all.swap(pending);
+        }
+        for (auto& r : all) {
+            GpuRuleResult64 res;
+            std::memset(&res, 0, sizeof(res));
+            res.status = status;
+            res.bodyPairs = -1;
+            complete(r, res, mono_ns(), 0);
+        }
     }
 
     // Pending requests -> a batch on a free slot. Returns true if launched.
     bool admit() {
         Slot* S = nullptr;
-        for (auto& s : slots) if (!s->busy) { S = s.get(); break; }
+        bool usable = false;
+        for (auto& s : slots) {
+            if (s->retired) continue;
+            usable = true;
+            if (!s->busy) { S = s.get(); break; }
+        }
+        if (!usable) {   // every slot retired: nothing can run any more
+            failed = true;
+            fail_pending(kRuleError);
+            return false;
+        }
         if (!S) return false;
         std::vector<Request> take;
         {
@@ -260,7 +309,18 @@ struct Server : Handle {
             }
             return false;
         }
-        launch(*S);
+        if (!launch(*S)) {   // nothing to wait for: the batch's requests fail now
+            failed = true;
+            (void)hipStreamSynchronize(S->st);
+            for (auto& r : admitted) {
+                GpuRuleResult64 res;
+                std::memset(&res, 0, sizeof(res));
+                res.status = kRuleError;
+                res.bodyPairs = -1;
+                complete(r, res, mono_ns(), 0);
+            }
+            return false;
+        }
         S->reqs = std::move(admitted);
         S->reserved = need;
         S->admitNs = mono_ns();

@@ -35,7 +35,7 @@ constexpr int kSortWaves = kSortBlock / kWave;
 struct RadixPass {
     const unsigned long long* in;
     unsigned long long* out;
-    const uint32_t* count;
+    const unsigned long long* count;
     uint32_t cap;
     uint32_t maxTiles;
     uint32_t* hist;
@@ -44,7 +44,7 @@ struct RadixPass {
 
 struct UniqueArgs {
     const unsigned long long* keys;
-    const uint32_t* count;
+    const unsigned long long* count;
     uint32_t cap;
     int32_t hoBits;
     const HeavyItem* heavy;
@@ -52,9 +52,9 @@ struct UniqueArgs {
     unsigned long long* counts;
 };
 
-__device__ __forceinline__ uint32_t key_count(const uint32_t* count, uint32_t cap) {
-    const uint32_t n = *count;
-    return n < cap ? n : cap;
+__device__ __forceinline__ uint32_t key_count(const unsigned long long* count, uint32_t cap) {
+    const unsigned long long n = *count;
+    return n < (unsigned long long)cap ? (uint32_t)n : cap;
 }
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
@@ -206,7 +206,7 @@ __global__ void __launch_bounds__(kSortBlock) k_unique_count(UniqueArgs a) {
 
 }  // namespace
 
-int launch_radix_sort(unsigned long long* const keys[2], const uint32_t* count, uint32_t cap, uint32_t* hist,
+int launch_radix_sort(unsigned long long* const keys[2], const unsigned long long* count, uint32_t cap, uint32_t* hist,
                       int bits, int blocks, hipStream_t s) {
     const int passes = sort_passes(bits);
     if (blocks < 1) blocks = 1;

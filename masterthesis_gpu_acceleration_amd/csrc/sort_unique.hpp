@@ -24,7 +24,8 @@ inline uint64_t sort_hist_words(uint32_t cap) {
     return (uint64_t)kRadix * (sort_max_tiles(cap) + 1);
 }
 
-// One sort-unique job: the keys are in keys[0][0, min(*count, cap)), each
+// One sort-unique job: the keys are in keys[0][0, min(*count, cap)) (the
+// count is 64-bit and may pass cap; cap <= 2^31, so positions fit 32 bits), each
 // key (h << hoBits) | object for heavy seed h. `bits` significant key bits
 // decide the number of 8-bit LSD passes; the sorted keys end in
 // keys[passes % 2]. The unique pass adds the number of distinct keys of each
@@ -32,7 +33,7 @@ inline uint64_t sort_hist_words(uint32_t cap) {
 // when heavy is null, all distinct keys to counts[0].
 struct SortJob {
     unsigned long long* keys[2];
-    const uint32_t* count;
+    const unsigned long long* count;
     uint32_t cap;
     uint32_t maxTiles;
     uint32_t* hist;
@@ -62,7 +63,7 @@ void launch_sort_unique(const SortJob& j, hipStream_t s);
 // their low `bits` bits (stable), ping-ponging through keys[1]; hist holds
 // sort_hist_words(cap) words. Returns the index of the buffer holding the
 // sorted keys (passes % 2).
-int launch_radix_sort(unsigned long long* const keys[2], const uint32_t* count, uint32_t cap, uint32_t* hist,
+int launch_radix_sort(unsigned long long* const keys[2], const unsigned long long* count, uint32_t cap, uint32_t* hist,
                       int bits, int blocks, hipStream_t s);
 
 }  // namespace gr

@@ -42,6 +42,33 @@ def test_library_exports_every_declared_symbol():
     assert set(L.exported_names()) <= exported
 
 
+def test_count_decisions_with_forced_counters():
+    """The host's decisions on a pass's device counters (taken by the batch,
+    per-rule and serving paths after every pass), fed counter values no test
+    graph reaches: key counts past 2^32 must never be taken as counted, and
+    past the largest capacity they must fail rather than regrow."""
+    act = L.load_library().gpu_host_count_action
+    cap_max = 1 << 31
+    SORT, HEAVY = 0, 1
+    OK, REGROW, FAIL = 0, 1, 2
+    assert act(SORT, 0, 4096, cap_max) == OK
+    assert act(SORT, 4096, 4096, cap_max) == OK
+    assert act(SORT, 4097, 4096, cap_max) == REGROW
+    assert act(SORT, cap_max, cap_max, cap_max) == OK
+    assert act(SORT, cap_max + 1, cap_max, cap_max) == FAIL
+    # a count that a 32-bit counter would have wrapped to a small value
+    for emitted in ((1 << 32) + 5, 5_370_000_000, (1 << 40)):
+        assert act(SORT, emitted, 1 << 22, cap_max) == REGROW
+        assert act(SORT, emitted, cap_max, cap_max) == FAIL
+    # a lowered maximum (the "sort_cap_max" tuning key) fails at that capacity
+    assert act(SORT, 5000, 4096, 4096) == FAIL
+    assert act(HEAVY, 10, 10, 0) == OK
+    assert act(HEAVY, 11, 10, 0) == FAIL
+    assert act(HEAVY, (1 << 32) + 1, 0xFFFFFFF0, 0) == FAIL
+    assert act(2, 0, 0, 0) == -1
+    assert act(SORT, -1, 0, 0) == -1
+
+
 def test_result_struct_layout_ctypes():
     assert ctypes.sizeof(L.GpuRuleResult) == 56
     offs = [getattr(L.GpuRuleResult, f).offset for f, _ in L.GpuRuleResult._fields_]
