@@ -1,37 +1,58 @@
 #!/usr/bin/env python3
 """Benchmark of the rule support / body-pair counting path on one MI355X.
 
-Workload (default): the DBpedia-shaped configuration -- 206,000 triples over
-50,000 entities and 300 predicates (predicate frequency Zipf s=1.0,
-per-predicate out-degree lognormal(0,1)), 10,000 closed rules of 1-3 body
-atoms -- generated from a fixed seed (the reference's datasets are not
-available, so the data is synthetic).
+Default run (`python bench.py --gpus 1 --steps K --warmup W`), one JSON line:
 
-A step is one evaluation of all rules: exact support and distinct body-pair
-count for every rule (the body pairs of every rule are counted, also when
-support is 0, i.e. the work CPU_01_nonparallel does), from a batch compiled
-once and kept resident in HBM, replayed as a HIP graph. The JSON line also
-reports:
-  * per_rule: the same rules through the per-rule launch sequence of the
-    reference library (one support launch and one body-size launch per rule,
-    each followed by a sync and a device-to-host copy), with p50/p99 latency;
-  * roofline: algorithmic bytes of the dominant kernel (host traffic model of
-    the same plans) over its event-timed duration, against 8 TB/s;
+* headline -- DBpedia-shaped configuration: 206,000 triples over 50,000
+  entities and 300 predicates (predicate frequency Zipf s=1.0, per-predicate
+  out-degree lognormal(0,1)), 10,000 closed rules of 1-3 body atoms. A step is
+  one evaluation of all rules (exact support and distinct body-pair count of
+  every rule, also where support is 0 -- the work CPU_01_nonparallel does)
+  from a batch compiled once and kept resident in HBM, replayed as a HIP
+  graph. `value` = rules/s over K timed steps.
+* "hub_fanout" -- the same measurement on the high-fan-out configuration
+  (6.8M triples, 500,000 entities, 60 predicates, ~50 hubs of degree ~1e5,
+  50 rules of 2-4 atoms), with its own ms_per_step, roofline and CPU baseline.
+* "serving" -- the continuous-batching server over the resident high-fan-out
+  graph: the serving stream (1-8 atoms, predicates Zipf s=1.0, Poisson
+  arrivals; no work filter) and, beside it, the same stream with paths of
+  estimated search work above 1e8 redrawn; p50/p99/max latency from arrival
+  to result, served rate, mean batch.
+
+Every block also reports:
+  * per_rule: the same rules through the reference library's per-rule launch
+    sequence (support launch + sync + copy, body-size launch + sync + copy),
+    with the reference bench's statistics of the per-rule device times
+    (avg/median/std/min/max/P95, overall and by body size) and p99;
+  * parity_vs_golden: the timed counts against the reference programs'
+    outputs for the same seeded graph and rules (tests/golden/*.json);
+  * roofline: algorithmic bytes of the dominant phase (host traffic model of
+    the same plans) over the event-timed duration of the kernels that run
+    that phase (thread-mode kernel + wave-mode kernel with its sub-item
+    pass), against 8 TB/s; counters (traffic, L2 hit) from the committed
+    rocprofv3 summary, with the tuning it was taken at;
   * cpu_baseline: the reference's CPU_01_nonparallel program (built from the
-    reference sources into oracle/_ref/, single-threaded) on a bounded sample
-    of the same rules, or the project's CPU oracle if that build is absent.
+    reference sources into oracle/_ref/, single-threaded), sharded over the
+    host cores -- the rate on all cores and on one core.
+
+The data is synthetic (the reference's datasets are not available), generated
+from fixed seeds.
 
 Usage: python bench.py [--steps K] [--warmup W] [--gpus 1]
+       python bench.py --workload hub_fanout --only      (one configuration)
+       python bench.py --serve                           (serving block only)
 """
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import os
 import subprocess
 import sys
 import tempfile
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -43,6 +64,8 @@ from masterthesis_gpu_acceleration_amd import workloads as W  # noqa: E402
 
 METRIC = "rules/s & body-pairs/s (batched vs per-rule launch); p50/p99 rule latency"
 HBM_PEAK_GBPS = 8000.0
+GOLDEN_DIR = os.path.join(ROOT, "tests", "golden")
+CPU01 = os.path.join(ROOT, "oracle", "_ref", "cpu01")
 
 
 def log(*a):
@@ -50,11 +73,23 @@ def log(*a):
 
 
 def pct(x, q):
+    """The reference bench's percentile: sorted[ceil(q n) - 1]
+    (GPU_03/bench_gpu.cu:725-729)."""
     x = np.sort(np.asarray(x, np.float64))
     if x.size == 0:
         return 0.0
     k = max(0, min(x.size - 1, int(np.ceil(q * x.size)) - 1))
     return float(x[k])
+
+
+def stats(x):
+    """avg/median/std/min/max/P95 as GPU_03/bench_gpu.cu:714-760 prints them, plus p99."""
+    x = np.asarray(x, np.float64)
+    if x.size == 0:
+        return {}
+    return {"n": int(x.size), "avg": round(float(x.mean()), 5), "median": round(float(np.median(x)), 5),
+            "std": round(float(x.std()), 5), "min": round(float(x.min()), 5), "max": round(float(x.max()), 5),
+            "p95": round(pct(x, 0.95), 5), "p99": round(pct(x, 0.99), 5)}
 
 
 def cpu_model():
@@ -67,31 +102,324 @@ def cpu_model():
     return "unknown"
 
 
+def host_cores():
+    """Cores this process may use (the box's CPU share)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        return os.cpu_count() or 1
+
+
+# name: (generator, golden fixture, CPU-baseline sample in rules (0: all))
 WORKLOADS = {
-    # name: (generator, CPU-baseline sample size in rules)
-    "dbpedia_shaped": (lambda: W.dbpedia_shaped(), 2000),
-    "small_paths": (lambda: W.small_paths(), 10),
-    "hub_fanout": (lambda: W.hub_fanout(), 3),
-    "dense_conf": (lambda: W.dense_conf(), 8),
+    "dbpedia_shaped": (lambda: W.dbpedia_shaped(), "dbpedia_shaped_full", 0),
+    "hub_fanout": (lambda: W.hub_fanout(), "hub_fanout_full", 16),
+    "small_paths": (lambda: W.small_paths(), "small_paths_full", 0),
+    "dense_conf": (lambda: W.dense_conf(), "dense_conf_full", 0),
 }
 
 
-def pmc_traffic(workload, kernel):
-    """HBM bytes per dispatch of `kernel` from the committed counter summary
-    (rocprofv3 FETCH_SIZE and WRITE_SIZE runs, scaled by the copy yardstick),
-    or None when that workload has not been profiled."""
+def describe(wl):
+    g = wl.graph
+    nb = [len(r[1]) for r in wl.rules]
+    return "%s: %d triples, %d entities, %d predicates, %d rules of %d-%d atoms" % (
+        wl.name, g.n_triples, g.n_entities, g.n_predicates, len(wl.rules), min(nb), max(nb))
+
+
+# ------------------------------------------------------------ parity
+
+def graph_sha(g):
+    h = hashlib.sha256()
+    for a in (g.s, g.p, g.o):
+        h.update(np.ascontiguousarray(a, np.int32).tobytes())
+    return h.hexdigest()
+
+
+def head_pair_in_body(rule):
+    head, body = rule
+    bv = {t[1] for a in body for t in (a[0], a[2]) if t[0] == "v"}
+    return all(t[0] == "v" and t[1] in bv for t in (head[0], head[2]))
+
+
+def parity_vs_golden(name, wl, res, gated):
+    """The counts of the timed batch against the reference programs' outputs
+    for the same graph and rules (CPU_01_nonparallel's support, head figures
+    and body size; body pairs are 0 where a head variable is absent from the
+    body, as Rewritten_CPP counts them)."""
+    path = os.path.join(GOLDEN_DIR, name + ".json")
+    try:
+        fx = json.load(open(path))
+    except OSError:
+        return {"fixture": name, "ok": None, "note": "fixture absent"}
+    if fx["graph_sha256"] != graph_sha(wl.graph) or fx["rules_text"] != [W.rule_text(r) for r in wl.rules]:
+        return {"fixture": name, "ok": False, "note": "generated workload differs from the fixture's"}
+    bad = 0
+    for i, r in enumerate(wl.rules):
+        sup, hsize, hsup, body = fx["cpu01"][i]
+        pairs = body if head_pair_in_body(r) else 0
+        got = res[i]
+        want = [sup, hsize, hsup, pairs if sup > 0 else 0]
+        have = [int(got["support"]), int(got["headSize"]), int(got["headSupport"]), int(got["bodySize"])]
+        if not gated:
+            want.append(pairs)
+            have.append(int(got["bodyPairs"]))
+        bad += int(want != have or int(got["status"]) != 0)
+    return {"fixture": name, "rules": len(wl.rules), "mismatches": bad, "ok": bad == 0,
+            "compared": "support, headSize, headSupport, bodySize" + ("" if gated else ", bodyPairs")
+                        + " vs CPU_01_nonparallel"}
+
+
+# ------------------------------------------------------------ CPU baseline
+
+def cpu_baseline(wl, sample):
+    """The reference CPU_01_nonparallel program (single-threaded), one
+    process per host core, each on a share of the rules (round robin) of the
+    first `sample` rules (0: all). Rule compute = sum of its per-rule support
+    + confidence times, as the reference notebooks count it (thesis p.43):
+    the one-core rate is rules / total rule compute, the all-core rate rules
+    / the largest process's rule compute. Falls back to the project's CPU
+    oracle on one core when the reference build is absent."""
+    rules = wl.rules if sample <= 0 else wl.rules[:sample]
+    cores = max(1, min(host_cores(), 16, len(rules)))
+    if not os.path.exists(CPU01):
+        from oracle import Oracle  # the port, when the reference build is absent
+        g = wl.graph
+        o = Oracle(g.s, g.p, g.o)
+        t = time.time()
+        o.eval(W.encode_rules(rules))
+        dt = time.time() - t
+        return {"value": round(len(rules) / dt, 2), "unit": "rules/s", "cores": 1, "kind": "port",
+                "sample": "%d of the %d rules, full graph; project CPU oracle on one core; host %s" % (
+                    len(rules), len(wl.rules), cpu_model())}
+    with tempfile.TemporaryDirectory() as d:
+        ttl = os.path.join(d, "g.ttl")
+        W.write_ttl(wl.graph, ttl)
+        shards = [rules[k::cores] for k in range(cores)]
+
+        def run(k):
+            rf = os.path.join(d, "r%02d.txt" % k)
+            W.write_rules(shards[k], rf)
+            t = time.time()
+            out = subprocess.run([CPU01, ttl, rf, "--json"], capture_output=True, text=True, timeout=400, check=True)
+            return json.loads(out.stdout), time.time() - t
+
+        t = time.time()
+        with ThreadPoolExecutor(cores) as ex:
+            runs = list(ex.map(run, range(cores)))
+        wall = time.time() - t
+    per_proc = [sum(r["supportMs"] + r["confidenceMs"] for r in rows) for rows, _ in runs]
+    total_ms = sum(per_proc)
+    n = sum(len(rows) for rows, _ in runs)
+    return {"value": round(n / (max(per_proc) / 1000.0), 2), "unit": "rules/s", "cores": cores, "kind": "reference",
+            "single_core": round(n / (total_ms / 1000.0), 2),
+            "sample": "%s of the %d rules, full graph; CPU_01_nonparallel, %d single-threaded processes (one per core, "
+                      "rules round robin); rule compute %.1f ms in all, %.1f ms in the largest share; wall %.1f s "
+                      "incl. each process's Turtle load; host %s, %d cores available" % (
+                          "all" if n == len(wl.rules) else "first %d" % n, len(wl.rules), cores, total_ms,
+                          max(per_proc), wall, cpu_model(), host_cores())}
+
+
+# ------------------------------------------------------------ one configuration
+
+def pmc_traffic(workload, tuning):
+    """HBM bytes per dispatch of the dominant phase's kernels from the
+    committed counter summary (rocprofv3 FETCH_SIZE and WRITE_SIZE runs,
+    scaled by the copy yardstick), and whether it was taken at this tuning."""
     p = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     try:
         d = json.load(open(p))[workload]
-        return int(d["kernels"][kernel]["hbm_bytes"]), d["kernels"][kernel]["l2_hit"], d["source"]
+        d["phase_hbm_bytes"]
     except (OSError, KeyError, ValueError):
-        return None, None, None
+        return None
+    prof_t = d.get("tuning", {})
+    same = all(prof_t.get(k) == tuning.get(k) for k in prof_t) if prof_t else False
+    return {"traffic": int(d["phase_hbm_bytes"]), "l2_hit": d["l2_hit"], "traffic_source": d["source"],
+            "traffic_ms": d.get("phase_ms"), "traffic_tuning": prof_t, "traffic_tuning_matches": same}
+
+
+TUNE_KEYS = ("budget", "thread_waves_per_cu", "wave_waves_per_cu", "pairs_dedup", "giant_after", "chunks_per_wave")
+
+
+def tune_engine(eng, words, flags, args):
+    if args.tune:
+        knobs = [int(x) for x in args.tune.split(",")]
+        for key, val in zip(("budget", "thread_waves_per_cu", "wave_waves_per_cu", "dedup", "giant_after",
+                             "chunks_per_wave"), knobs):
+            eng.set_tuning(key, val)
+        log("fixed tuning: %s" % args.tune)
+        return
+    if args.no_autotune:
+        return
+    t = time.time()
+    tl = eng.autotune(words, flags)
+    best = tl[np.argmin(tl[:, 3])]
+    log("autotune (%.1fs, %d trials): budget %d, thread waves/CU %d, wave waves/CU %d -> %.3f ms; chunks/wave %d, "
+        "giant_after %d" % (time.time() - t, len(tl), best[0], best[1], best[2], best[3],
+                            eng.get_tuning("chunks_per_wave"), eng.get_tuning("giant_after")))
+
+
+def time_batch(eng, words, flags, steps, warmup):
+    """ms per step of K graph replays of a resident batch (each step waits for
+    its results), device ms per replay, and the last results."""
+    batch = eng.batch(words, flags)
+    try:
+        for _ in range(warmup):
+            res = batch.run()
+        dev = []
+        t = time.perf_counter()
+        for _ in range(steps):
+            res = batch.run()
+            dev.append(eng.last_stats()["total_ms"])
+        wall = time.perf_counter() - t
+    finally:
+        batch.close()
+    return wall * 1000.0 / steps, dev, res.copy()
+
+
+def measure(args, name, wl, decompose):
+    """One configuration's block of the JSON line."""
+    gen, golden, sample = WORKLOADS[name]
+    g = wl.graph
+    rules = wl.rules
+    n = len(rules)
+    words = L.rule_words_for_graph(rules, g)
+    t = time.time()
+    eng = L.Engine.from_graph(g)
+    load_s = time.time() - t
+    log("%s: graph resident in HBM (%.2fs)" % (name, load_s))
+    flags = 0 if args.gated else L.UNGATED
+    try:
+        if args.no_overlap:
+            eng.set_tuning("overlap", 0)
+        if args.no_fuse:
+            eng.set_tuning("fuse_support", 0)
+        for kv in args.set:
+            key, val = kv.split("=", 1)
+            eng.set_tuning(key, int(val))
+        tune_engine(eng, words, flags, args)
+        tuning = {k: eng.get_tuning(k) for k in TUNE_KEYS}
+        t = time.time()
+        ms_step, dev_ms, res = time_batch(eng, words, flags, args.steps, args.warmup)
+        log("%s: timed %d steps: %.3f ms/step (device %.3f ms)" % (name, args.steps, ms_step, float(np.mean(dev_ms))))
+        rules_per_s = n / (ms_step / 1000.0)
+        pairs = int(res["bodyPairs"][res["bodyPairs"] > 0].sum())
+        searches = {"support": eng.get_tuning("last_support_searches"), "pairs": eng.get_tuning("last_pair_searches"),
+                    "support_fused_rules": eng.get_tuning("last_fused"),
+                    "rules_sharing_a_search": eng.get_tuning("last_shared")}
+        block = {
+            "value": round(rules_per_s, 1), "unit": "rules/s", "ms_per_step": round(ms_step, 4),
+            "config": {"workload": describe(wl) + (", exact support + distinct body pairs for %s, resident batch, "
+                                                    "HIP graph replay" % ("rules with support > 0" if args.gated
+                                                                          else "every rule")),
+                       "rules": n, "triples": g.n_triples},
+            "body_pairs_per_s": round(pairs / (ms_step / 1000.0), 1), "body_pairs_total": pairs,
+            "rules_with_support": int((res["support"] > 0).sum()),
+            "escalated_seeds": int(eng.last_stats()["heavy_seeds"]),
+            "searches": searches,
+            "tuning": dict(tuning, wave_slots=eng.get_tuning("wave_slots"), fuse_active=eng.get_tuning("fuse_active")),
+            "latency_ms": {"batched_p50": round(pct(dev_ms, 0.5), 4), "batched_p99": round(pct(dev_ms, 0.99), 4)},
+            "parity_vs_golden": parity_vs_golden(golden, wl, res, args.gated),
+        }
+        log("%s: parity vs golden: %s" % (name, block["parity_vs_golden"]))
+
+        # end to end: compile + upload + run + readback of a fresh batch (PCIe-inclusive; never `value`)
+        t = time.perf_counter()
+        eng.evaluate(words, flags)
+        block["e2e_rules_per_s"] = round(n / (time.perf_counter() - t), 1)
+
+        # roofline of the dominant phase: eager passes with HIP events on the library's stream
+        eng.set_tuning("use_graph", 0)
+        kms = np.zeros(4)
+        reps = 3
+        for _ in range(reps):
+            eng.evaluate(words, flags)
+            kms += eng.kernel_ms()
+        kms /= reps
+        eng.set_tuning("use_graph", 1)
+        traffic = eng.traffic(words)
+        phase_ms = [kms[0] + kms[1], kms[2] + kms[3]]
+        phase = int(np.argmax(phase_ms))
+        bytes_phase = int(traffic[:, phase].sum())
+        achieved = bytes_phase / (phase_ms[phase] * 1e-3) / 1e9 if phase_ms[phase] > 0 else 0.0
+        kernels = ["k_thread_batch<%d>" % phase, "k_wave_batch<%d>" % phase, "k_giant_expand", "k_wave_sub<%d>" % phase]
+        roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBPS, 5), "traffic": None,
+                "phase": ("support", "body pairs")[phase], "kernels": kernels,
+                "phase_ms": round(float(phase_ms[phase]), 4), "algorithmic_bytes": bytes_phase,
+                "kernel_ms": {"thread": round(float(kms[2 * phase]), 4),
+                              "wave_and_sub_items": round(float(kms[2 * phase + 1]), 4)},
+                "kernel_ms_all": [round(float(x), 4) for x in kms]}
+        pm = pmc_traffic(name + ("_gated" if args.gated else ""), tuning)
+        if pm:
+            roof.update(pm)
+            if pm.get("traffic_ms"):
+                roof["counter_GBps"] = round(pm["traffic"] / (pm["traffic_ms"] * 1e-3) / 1e9, 2)
+        block["roofline"] = roof
+        log("%s: kernels ms %s; phase %s %.3f ms, %.1f GB/s model bytes" % (
+            name, [round(float(x), 3) for x in kms], roof["phase"], phase_ms[phase], achieved))
+
+        if decompose:
+            # what batching gains without sharing equal bodies' searches and without counting support in the
+            # body-pair pass (same rules, same tuning)
+            dec = {}
+            for label, knobs in (("no_shared_bodies", {"share_bodies": 0}), ("no_fused_support", {"fuse_support": 0}),
+                                 ("neither", {"share_bodies": 0, "fuse_support": 0})):
+                for k, v in knobs.items():
+                    eng.set_tuning(k, v)
+                ms, _, r2 = time_batch(eng, words, flags, args.steps, 2)
+                dec[label] = {"rules_per_s": round(n / (ms / 1000.0), 1), "ms_per_step": round(ms, 4),
+                              "pair_searches": eng.get_tuning("last_pair_searches"),
+                              "support_searches": eng.get_tuning("last_support_searches"),
+                              "same_counts": bool(np.array_equal(r2["support"], res["support"]) and
+                                                  np.array_equal(r2["bodyPairs"], res["bodyPairs"]))}
+                for k in knobs:
+                    eng.set_tuning(k, 1)
+                log("%s: %s: %s" % (name, label, dec[label]))
+            block["decomposition"] = dec
+
+        if not args.no_snapshot:
+            block["load_s"] = snapshot_leg(eng, words, flags, res, load_s)
+
+        if args.compare_dedup or name == "dense_conf":
+            block["dedup_compare"] = dedup_compare(eng, words, flags, res)
+
+        if not args.no_per_rule:
+            t = time.perf_counter()
+            pr = eng.evaluate(words, flags | L.PER_RULE)
+            pr_wall = time.perf_counter() - t
+            sup_ms, conf_ms = eng.rule_times(n)
+            lat = pr["latencyUs"].astype(np.float64) / 1000.0
+            nb = np.array([len(r[1]) for r in rules])
+            same = bool(np.array_equal(pr["support"], res["support"]) and
+                        np.array_equal(pr["bodyPairs"], res["bodyPairs"]))
+            block["per_rule"] = {
+                "rules_per_s": round(n / pr_wall, 1), "wall_s": round(pr_wall, 3),
+                "p50_ms": round(pct(lat, 0.5), 4), "p99_ms": round(pct(lat, 0.99), 4),
+                "same_counts_as_batched": same,
+                "host_wall_ms": stats(lat),
+                "support_ms": stats(sup_ms), "confidence_ms": stats(conf_ms),
+                "by_body_size": {str(k): {"support_ms": stats(sup_ms[nb == k]), "confidence_ms": stats(conf_ms[nb == k])}
+                                 for k in sorted(set(nb.tolist()))}}
+            block["speedup_vs_per_rule"] = round(rules_per_s / (n / pr_wall), 2)
+            block["latency_ms"]["per_rule_p50"] = block["per_rule"]["p50_ms"]
+            block["latency_ms"]["per_rule_p99"] = block["per_rule"]["p99_ms"]
+            log("%s: per-rule launch: %.1f rules/s, p50 %.3f ms p99 %.3f ms, same counts: %s" % (
+                name, n / pr_wall, block["per_rule"]["p50_ms"], block["per_rule"]["p99_ms"], same))
+    finally:
+        eng.close()
+
+    if not args.no_cpu:
+        t = time.time()
+        block["cpu_baseline"] = cpu_baseline(wl, args.cpu_sample or sample)
+        log("%s: cpu baseline %.1fs: %s rules/s on %d cores" % (name, time.time() - t, block["cpu_baseline"]["value"],
+                                                               block["cpu_baseline"]["cores"]))
+    return block
 
 
 def snapshot_leg(eng, words, flags, res, load_triples_s):
-    """Graph load times: from term-id triples (host sort + index build +
-    upload) against a binary snapshot of the same graph (map + checks +
-    upload); the snapshot's context must give the same counts."""
+    """Graph load times: from term-id triples (host or device sort + index
+    build + upload) against a binary snapshot of the same graph (map + checks
+    + upload); the snapshot's context must give the same counts."""
     with tempfile.TemporaryDirectory() as d:
         p = os.path.join(d, "g.snap")
         t = time.time()
@@ -106,56 +434,93 @@ def snapshot_leg(eng, words, flags, res, load_triples_s):
             same = bool(np.array_equal(r["support"], res["support"]) and np.array_equal(r["bodyPairs"], res["bodyPairs"]))
         finally:
             e2.close()
-    log("load: from triples %.2fs, snapshot save %.2fs (%.1f MB), from snapshot %.2fs, same counts %s" % (
-        load_triples_s, save_s, size / 1e6, snap_s, same))
     return {"from_triples": round(load_triples_s, 3), "snapshot_save": round(save_s, 3),
             "from_snapshot": round(snap_s, 3), "snapshot_mb": round(size / 1e6, 1), "same_counts": same}
 
 
-def describe(wl):
-    g = wl.graph
-    nb = [len(r[1]) for r in wl.rules]
-    return "%s: %d triples, %d entities, %d predicates, %d rules of %d-%d atoms" % (
-        wl.name, g.n_triples, g.n_entities, g.n_predicates, len(wl.rules), min(nb), max(nb))
+def dedup_compare(eng, words, flags, res):
+    """Body-pair dedup strategies on the same batch (eager passes, best of 3)."""
+    cmp = {}
+    eng.set_tuning("use_graph", 0)
+    for mode, label in ((1, "bitmap_hbm_rows"), (3, "bitmap_lds"), (2, "sort_unique")):
+        eng.set_tuning("dedup", mode)
+        if mode == 3 and eng.get_tuning("pairs_dedup") != 2:
+            continue   # the bitmap does not fit a block's LDS
+        eng.evaluate(words, flags)
+        times = []
+        for _ in range(3):
+            t = time.perf_counter()
+            r = eng.evaluate(words, flags)
+            times.append(time.perf_counter() - t)
+        k = eng.kernel_ms()
+        ds = eng.dedup_stats()
+        same = bool(np.array_equal(r["bodyPairs"], res["bodyPairs"]) and np.array_equal(r["support"], res["support"]))
+        cmp[label] = {"ms": round(min(times) * 1e3, 3), "pairs_wave_ms": round(float(k[3]), 3),
+                      "sort_ms": round(ds["sort_ms"], 3), "keys_emitted": int(ds["keys_emitted"]), "same_counts": same}
+        log("dedup %s: %s" % (label, cmp[label]))
+    eng.set_tuning("dedup", 0)
+    eng.set_tuning("use_graph", 1)
+    return cmp
 
 
-def cpu_baseline(wl, rules, sample):
-    """Reference CPU_01_nonparallel (single thread) on the first `sample`
-    rules; rule compute = sum of its per-rule support + confidence times, as
-    the reference notebooks count it (RunBenchmark.ipynb; thesis p.43)."""
-    exe = os.path.join(ROOT, "oracle", "_ref", "cpu01")
-    sub = rules[:sample]
-    if os.path.exists(exe):
-        with tempfile.TemporaryDirectory() as d:
-            ttl = os.path.join(d, "g.ttl")
-            rf = os.path.join(d, "r.txt")
-            W.write_ttl(wl.graph, ttl)
-            W.write_rules(sub, rf)
-            t = time.time()
-            try:
-                out = subprocess.run([exe, ttl, rf, "--json"], capture_output=True, text=True, timeout=180,
-                                     check=True)
-            except subprocess.TimeoutExpired:
-                return {"value": None, "unit": "rules/s", "cores": 1, "kind": "reference",
-                        "sample": "first %d rules did not finish within 180 s on one core" % len(sub)}
-            wall = time.time() - t
-        rows = json.loads(out.stdout)
-        ms = sum(r["supportMs"] + r["confidenceMs"] for r in rows)
-        return {"value": round(len(rows) / (ms / 1000.0), 2), "unit": "rules/s", "cores": 1, "kind": "reference",
-                "sample": "first %d of the %d rules, full graph; CPU_01_nonparallel rule compute %.1f ms "
-                          "(wall incl. Turtle load %.1f s); host %s, nproc %d" % (
-                              len(rows), len(rules), ms, wall, cpu_model(), os.cpu_count() or 0)}
-    from oracle import Oracle  # CPU oracle (the port) when the reference build is absent
-    g = wl.graph
-    o = Oracle(g.s, g.p, g.o)
-    words = W.encode_rules(sub)
+# ------------------------------------------------------------ serving
+
+def serve_leg(eng, g, n, rate, max_work, label):
+    """The first n requests of the 100,000-request serving stream (1-8 atoms,
+    predicates Zipf s=1.0), arrivals Poisson at `rate`, through the server;
+    max_work: paths of larger estimated search work are redrawn (None: kept)."""
+    from masterthesis_gpu_acceleration_amd import serving as S
     t = time.time()
-    o.eval(words)
-    dt = time.time() - t
-    return {"value": round(len(sub) / dt, 2), "unit": "rules/s", "cores": 1, "kind": "port",
-            "sample": "first %d of the %d rules, full graph; project CPU oracle; host %s, nproc %d" % (
-                len(sub), len(rules), cpu_model(), os.cpu_count() or 0)}
+    arrivals, reqs = W.serving_requests(graph=g, n_requests=100000, rate_per_s=rate, max_work=max_work)
+    kept = getattr(W.serving_requests, "last_kept", {}) if max_work else {}
+    arrivals, reqs = arrivals[:n], reqs[:n]
+    words = L.rule_words_for_graph(reqs, g)
+    log("serving %s: %d requests at %.0f/s (%.1fs to generate)" % (label, n, rate, time.time() - t))
+    srv = S.Server(eng, max_batch=2048, max_in_flight=3, flags=0)
+    try:
+        r = S.run_stream(srv, words, arrivals, progress=lambda sub, done: log(
+            "serving %s: %d submitted, %d completed" % (label, sub, done)))
+        st = srv.stats()
+    finally:
+        srv.stop()
+    lat = r["latency_ms"]
+    comp = r["completions"]
+    nb = np.array([len(q[1]) for q in reqs])
+    out = {"requests": n, "offered_per_s": rate, "served_rules_per_s": round(r["rules_per_s"], 1),
+           "span_s": round(r["span_s"], 3),
+           "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3),
+                          "max": round(float(lat.max()), 3)},
+           "latency_p99_ms_by_atoms": {str(k): round(float(np.percentile(lat[nb == k], 99)), 3)
+                                       for k in sorted(set(nb.tolist()))},
+           "mean_batch": round(float(comp["batchSize"].mean()), 1), "batches": st["batches"],
+           "refused": st["refused"], "device_failure": st["device_failure"],
+           "errors": int((comp["result"]["status"] == 2).sum()),
+           "rules_with_support": int((comp["result"]["support"] > 0).sum()),
+           "work_filter": ("paths of estimated search work above %.0e redrawn; kept fraction per path length %s" % (
+               max_work, {str(k): round(v, 4) for k, v in kept.items()})) if max_work else "none"}
+    log("serving %s: %.0f rules/s served, p50 %.1f ms, p99 %.1f ms, max %.1f ms, mean batch %.1f" % (
+        label, r["rules_per_s"], out["latency_ms"]["p50"], out["latency_ms"]["p99"], out["latency_ms"]["max"],
+        out["mean_batch"]))
+    return out
 
+
+def serving_block(args, g):
+    eng = L.Engine.from_graph(g)
+    try:
+        eng.evaluate(L.rule_words_for_graph(W.serving_stream(256, graph=g).rules, g), 0)   # warm the kernels
+        blk = {"config": "continuous-batching server (3 slots, batches up to 2048 requests) over the resident "
+                         "high-fan-out graph (6.8M triples); support for every request, body pairs when support > 0 "
+                         "(reference semantics); latency from scheduled arrival to result on the host",
+               "stream": serve_leg(eng, g, args.serve_requests, args.serve_rate, None, "stream")}
+        if args.serve_filtered_requests > 0:
+            blk["stream_work_filtered"] = serve_leg(eng, g, args.serve_filtered_requests, args.serve_filtered_rate, 1e8,
+                                                    "filtered")
+    finally:
+        eng.close()
+    return blk
+
+
+# ------------------------------------------------------------ main
 
 def main():
     ap = argparse.ArgumentParser()
@@ -163,17 +528,15 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="dbpedia_shaped", choices=sorted(WORKLOADS))
+    ap.add_argument("--only", action="store_true", help="measure --workload alone (no high-fan-out or serving block)")
     ap.add_argument("--cpu-sample", type=int, default=0, help="rules in the CPU-baseline sample (0: per workload)")
     ap.add_argument("--gated", action="store_true", help="body pairs only for rules with support > 0")
-    ap.add_argument("--serve", action="store_true",
-                    help="serving mode: Poisson request stream over the high-fan-out graph")
-    ap.add_argument("--requests", type=int, default=6000, help="requests in the serving stream")
-    ap.add_argument("--rate", type=float, default=150.0, help="requests per second (serving mode)")
-    ap.add_argument("--rates", default="75,150,300,600,1200",
-                    help="serving mode: arrival rates of the rate sweep (comma list; empty: no sweep)")
-    ap.add_argument("--sweep-requests", type=int, default=1500, help="requests per rate of the serving sweep")
-    ap.add_argument("--max-work", type=float, default=1e8,
-                    help="serving: drop requests whose estimated search work exceeds this (0: keep all)")
+    ap.add_argument("--serve", action="store_true", help="only the serving block")
+    ap.add_argument("--no-serve", action="store_true", help="leave out the serving block")
+    ap.add_argument("--serve-requests", type=int, default=9000)
+    ap.add_argument("--serve-rate", type=float, default=150.0)
+    ap.add_argument("--serve-filtered-requests", type=int, default=30000)
+    ap.add_argument("--serve-filtered-rate", type=float, default=1000.0)
     ap.add_argument("--compare-dedup", action="store_true",
                     help="also time the batch under each body-pair dedup strategy (default for dense_conf)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -185,6 +548,7 @@ def main():
     ap.add_argument("--no-fuse", action="store_true",
                     help="every rule's support gets its own pass (no support counting in the body-pair pass)")
     ap.add_argument("--no-per-rule", action="store_true")
+    ap.add_argument("--no-decompose", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tune", default="",
                     help="BUDGET,THREAD_WAVES_PER_CU,WAVE_WAVES_PER_CU[,DEDUP[,GIANT_AFTER[,CHUNKS_PER_WAVE]]]: fixed "
@@ -208,243 +572,29 @@ def main():
         eng.close()
         return
 
+    t0 = time.time()
     if args.serve:
-        return serve_mode(args)
+        g = W.hub_fanout().graph
+        print(json.dumps({"metric": METRIC, "serving": serving_block(args, g)}), flush=True)
+        return
 
-    t0 = time.time()
-    gen, sample = WORKLOADS[args.workload]
-    wl = gen()
-    g = wl.graph
-    rules = wl.rules
+    wl = WORKLOADS[args.workload][0]()
     log("workload %s (%.1fs)" % (describe(wl), time.time() - t0))
-    words = L.rule_words_for_graph(rules, g)
-    t = time.time()
-    eng = L.Engine.from_graph(g)
-    load_triples_s = time.time() - t
-    log("graph resident in HBM (%.2fs)" % load_triples_s)
-
-    flags = 0 if args.gated else L.UNGATED
-    if args.no_overlap:
-        eng.set_tuning("overlap", 0)
-    if args.no_fuse:
-        eng.set_tuning("fuse_support", 0)
-    for kv in args.set:
-        key, val = kv.split("=", 1)
-        eng.set_tuning(key, int(val))
-    if args.tune:
-        knobs = [int(x) for x in args.tune.split(",")]
-        for key, val in zip(("budget", "thread_waves_per_cu", "wave_waves_per_cu", "dedup", "giant_after",
-                             "chunks_per_wave"), knobs):
-            eng.set_tuning(key, val)
-        log("fixed tuning: %s" % args.tune)
-    elif not args.no_autotune:
-        t = time.time()
-        tl = eng.autotune(words, flags)
-        best = tl[np.argmin(tl[:, 3])]
-        log("autotune (%.1fs, %d trials): budget %d, thread waves/CU %d, wave waves/CU %d -> %.3f ms; "
-            "chunks/wave %d, giant_after %d" % (time.time() - t, len(tl), best[0], best[1], best[2], best[3],
-                                                eng.get_tuning("chunks_per_wave"), eng.get_tuning("giant_after")))
-        for r in tl:
-            log("   trial budget %6d tw %2d ww %2d  %.3f ms  escalated %d  pairs dedup %d  giant_after %d" % tuple(r))
-    t = time.time()
-    batch = eng.batch(words, flags)
-    build_s = time.time() - t
-    log("batch compiled and uploaded (%.3fs)" % build_s)
-    for _ in range(args.warmup):
-        res = batch.run()
-    log("warmup done")
-    dev_ms = []
-    t = time.perf_counter()
-    for _ in range(args.steps):
-        res = batch.run()
-        dev_ms.append(eng.last_stats()["total_ms"])
-    wall = time.perf_counter() - t
-    ms_step = wall * 1000.0 / args.steps
-    res = res.copy()
-    log("timed %d steps: %.3f ms/step (device %.3f ms)" % (args.steps, ms_step, float(np.mean(dev_ms))))
-    n = len(rules)
-    pairs = int(res["bodyPairs"][res["bodyPairs"] > 0].sum())
-    rules_per_s = n / (ms_step / 1000.0)
-
-    # end to end: compile + upload + run of a fresh batch
-    t = time.perf_counter()
-    eng.evaluate(words, flags)
-    e2e_s = time.perf_counter() - t
-
-    # roofline of the dominant kernel: eager pass with per-kernel events
-    eng.set_tuning("use_graph", 0)
-    kms = np.zeros(4)
-    reps = 3
-    for _ in range(reps):
-        eng.evaluate(words, flags)
-        kms += eng.kernel_ms()
-    kms /= reps
-    eng.set_tuning("use_graph", 1)
-    traffic = eng.traffic(words)
-    names = ["k_thread_batch<0>", "k_wave_batch<0>", "k_thread_batch<1>", "k_wave_batch<1>"]   # <plan kind>
-    dom = int(np.argmax(kms))
-    phase = dom // 2
-    bytes_phase = int(traffic[:, phase].sum())
-    achieved = bytes_phase / (kms[dom] * 1e-3) / 1e9 if kms[dom] > 0 else 0.0
-    heavy = eng.last_stats()["heavy_seeds"]
-    hbm, l2hit, src = pmc_traffic(args.workload + ("_gated" if args.gated else ""), names[dom])
-    log("kernels ms: %s; escalated seeds %d" % (", ".join("%s=%.3f" % (a, b) for a, b in zip(names, kms)), heavy))
-
+    head = measure(args, args.workload, wl, decompose=not args.no_decompose)
     out = {
-        "metric": METRIC,
-        "value": round(rules_per_s, 1),
-        "unit": "rules/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "int32",
-        "data": "synthetic",
-        "config": {"workload": describe(wl) + (", exact support + distinct body pairs for %s, resident batch, "
-                                                "HIP graph replay" % ("rules with support > 0" if args.gated
-                                                                      else "every rule")),
-                   "rules": n, "triples": g.n_triples},
-        "body_pairs_per_s": round(pairs / (ms_step / 1000.0), 1),
-        "body_pairs_total": pairs,
-        "rules_with_support": int((res["support"] > 0).sum()),
-        "escalated_seeds": int(heavy),
-        "tuning": {k: eng.get_tuning(k) for k in ("budget", "thread_waves_per_cu", "wave_waves_per_cu", "pairs_dedup",
-                                                  "giant_after", "chunks_per_wave", "wave_slots", "fuse_active")},
-        "support_fused_rules": eng.get_tuning("last_fused"),
-        "e2e_rules_per_s": round(n / e2e_s, 1),
-        "batch_build_s": round(build_s, 4),
-        "latency_ms": {"batched_p50": round(pct([x for x in dev_ms], 0.5), 4),
-                       "batched_p99": round(pct([x for x in dev_ms], 0.99), 4)},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBPS, 5), "traffic": hbm,
-                     "traffic_source": src, "l2_hit": l2hit, "kernel": names[dom], "kernel_ms": round(float(kms[dom]), 4),
-                     "algorithmic_bytes": bytes_phase, "kernel_ms_all": [round(float(x), 4) for x in kms]},
+        "metric": METRIC, "value": head.pop("value"), "unit": "rules/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": head.pop("ms_per_step"), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int32", "data": "synthetic",
     }
-
-    if not args.no_snapshot:
-        out["load_s"] = snapshot_leg(eng, words, flags, res, load_triples_s)
-
-    if args.compare_dedup or args.workload == "dense_conf":
-        # body-pair dedup strategies on the same batch (eager passes, best of 3)
-        cmp = {}
-        eng.set_tuning("use_graph", 0)
-        for mode, label in ((1, "bitmap_hbm_rows"), (3, "bitmap_lds"), (2, "sort_unique")):
-            eng.set_tuning("dedup", mode)
-            if mode == 3 and eng.get_tuning("pairs_dedup") != 2:
-                continue   # the bitmap does not fit a block's LDS
-            eng.evaluate(words, flags)
-            times = []
-            for _ in range(3):
-                t = time.perf_counter()
-                r = eng.evaluate(words, flags)
-                times.append(time.perf_counter() - t)
-            k = eng.kernel_ms()
-            ds = eng.dedup_stats()
-            same = bool(np.array_equal(r["bodyPairs"], res["bodyPairs"]) and np.array_equal(r["support"], res["support"]))
-            cmp[label] = {"ms": round(min(times) * 1e3, 3), "pairs_wave_ms": round(float(k[3]), 3),
-                          "sort_ms": round(ds["sort_ms"], 3), "keys_emitted": int(ds["keys_emitted"]),
-                          "same_counts": same}
-            log("dedup %s: %s" % (label, cmp[label]))
-        eng.set_tuning("dedup", 0)
-        eng.set_tuning("use_graph", 1)
-        out["dedup_compare"] = cmp
-
-    if not args.no_per_rule:
-        t = time.perf_counter()
-        pr = eng.evaluate(words, flags | L.PER_RULE)
-        pr_wall = time.perf_counter() - t
-        lat = pr["latencyUs"].astype(np.float64) / 1000.0
-        same = bool(np.array_equal(pr["support"], res["support"]) and np.array_equal(pr["bodyPairs"], res["bodyPairs"]))
-        out["per_rule"] = {"rules_per_s": round(n / pr_wall, 1), "wall_s": round(pr_wall, 3),
-                           "p50_ms": round(pct(lat, 0.5), 4), "p99_ms": round(pct(lat, 0.99), 4),
-                           "same_counts_as_batched": same}
-        out["speedup_vs_per_rule"] = round(rules_per_s / (n / pr_wall), 2)
-        out["latency_ms"]["per_rule_p50"] = out["per_rule"]["p50_ms"]
-        out["latency_ms"]["per_rule_p99"] = out["per_rule"]["p99_ms"]
-        log("per-rule launch: %.1f rules/s, p50 %.3f ms p99 %.3f ms, same counts: %s" % (
-            n / pr_wall, out["per_rule"]["p50_ms"], out["per_rule"]["p99_ms"], same))
-
-    if not args.no_cpu:
+    out.update(head)
+    if not args.only and args.workload != "hub_fanout":
         t = time.time()
-        out["cpu_baseline"] = cpu_baseline(wl, rules, args.cpu_sample or sample)
-        log("cpu baseline %.1fs: %s" % (time.time() - t, out["cpu_baseline"]["value"]))
-
-    batch.close()
-    eng.close()
-    print(json.dumps(out), flush=True)
-
-
-def serve_mode(args):
-    """Serving: a Poisson stream of rule requests (1-8 atoms, predicates
-    Zipf s=1.0) against the resident high-fan-out graph, through the
-    library's continuous-batching server; reports rules/s and p50/p99 latency
-    from arrival to result."""
-    from masterthesis_gpu_acceleration_amd import serving as S
-    t0 = time.time()
-    wl = W.hub_fanout()
-    g = wl.graph
-    arrivals, reqs = W.serving_requests(graph=g, n_requests=args.requests, rate_per_s=args.rate,
-                                        max_work=args.max_work or None)
-    kept = getattr(W.serving_requests, "last_kept", {})
-    log("serving workload: %s; %d requests at %.0f/s, kept fraction per path length %s (%.1fs)" % (
-        describe(wl), len(reqs), args.rate, kept, time.time() - t0))
-    words = L.rule_words_for_graph(reqs, g)
-    eng = L.Engine.from_graph(g)
-    flags = 0     # reference semantics: body size only for rules with support > 0
-    eng.evaluate(words[:256], flags)     # warm the kernels and the dedup rows
-    srv = S.Server(eng, max_batch=2048, max_in_flight=3, flags=flags)
-    try:
-        r = S.run_stream(srv, words, arrivals,
-                         progress=lambda sub, done: log("serving: %d submitted, %d completed" % (sub, done)))
-        st = srv.stats()
-    finally:
-        srv.stop()
-    lat = r["latency_ms"]
-    comp = r["completions"]
-    out = {
-        "metric": METRIC, "value": round(r["rules_per_s"], 1), "unit": "rules/s", "n_gpus": 1,
-        "steps": 1, "warmup": 0, "ms_per_step": round(r["span_s"] * 1000.0, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic",
-        "config": {"workload": "serving: Poisson stream of %d rule requests (1-8 atoms, predicates Zipf s=1.0, "
-                               "paths with estimated search work above %.0e redrawn) at %.0f/s over %s; support for "
-                               "every request, body pairs when support > 0"
-                               % (len(reqs), args.max_work, args.rate, describe(wl)),
-                   "requests": len(reqs), "rate_per_s": args.rate, "max_work": args.max_work,
-                   "kept_fraction_by_path_length": {str(k): round(v, 4) for k, v in kept.items()}},
-        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4),
-                       "max": round(float(lat.max()), 4)},
-        "serving": st,
-        "mean_batch": round(float(comp["batchSize"].mean()), 1),
-        "rules_with_support": int((comp["result"]["support"] > 0).sum()),
-    }
-    log("serving: %.0f rules/s, p50 %.3f ms, p99 %.3f ms, %d batches" % (
-        r["rules_per_s"], out["latency_ms"]["p50"], out["latency_ms"]["p99"], st["batches"]))
-    # rate sweep: the first requests of the same stream, arrival times rescaled
-    # to each rate (a Poisson stream stays Poisson under rescaling)
-    sweep = []
-    n = min(len(reqs), args.sweep_requests)
-    for rate in [float(x) for x in args.rates.split(",") if x.strip()]:
-        arr = np.asarray(arrivals[:n], np.float64) * (args.rate / rate)
-        srv = S.Server(eng, max_batch=2048, max_in_flight=3, flags=flags)
-        try:
-            rr = S.run_stream(srv, words[:n], arr)
-            ss = srv.stats()
-        finally:
-            srv.stop()
-        lt = rr["latency_ms"]
-        row = {"rate_per_s": rate, "requests": n, "rules_per_s": round(rr["rules_per_s"], 1),
-               "p50_ms": round(float(np.percentile(lt, 50)), 3), "p99_ms": round(float(np.percentile(lt, 99)), 3),
-               "max_ms": round(float(lt.max()), 3), "batches": ss["batches"],
-               "mean_batch": round(float(rr["completions"]["batchSize"].mean()), 1)}
-        sweep.append(row)
-        log("serving sweep: %(rate_per_s).0f/s offered -> %(rules_per_s).0f rules/s, p50 %(p50_ms).1f ms, "
-            "p99 %(p99_ms).1f ms, mean batch %(mean_batch).1f" % row)
-    if sweep:
-        out["rate_sweep"] = sweep
-    eng.close()
+        hub = W.hub_fanout()
+        log("workload %s (%.1fs)" % (describe(hub), time.time() - t))
+        out["hub_fanout"] = measure(args, "hub_fanout", hub, decompose=False)
+        if not args.no_serve:
+            out["serving"] = serving_block(args, hub.graph)
+    log("bench done in %.1fs" % (time.time() - t0))
     print(json.dumps(out), flush=True)
 
 

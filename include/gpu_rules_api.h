@@ -137,6 +137,13 @@ void gpu_batch_free(void* batch);
 /* Device time of the last gpu_evaluate_rules* call (ms) and its phases. */
 int gpu_last_batch_stats(void* ctx, double* totalMs, double* supportMs, double* pairsMs, int64_t* heavySeeds);
 
+/* Per rule of the last GPU_RULES_PER_RULE evaluation, the device time of its
+ * support launches and of its body-size launches (ms, HIP events around
+ * each rule's launches, as GPU_03/bench_gpu.cu:540-679 times each rule;
+ * 0 where a launch did not run). Fills up to n entries of each array and
+ * returns how many, or -1 on a bad argument. */
+int gpu_last_rule_times(void* ctx, double* supportMs, double* confidenceMs, int32_t n);
+
 /* Device time of the four batched kernels of the last eager (non-graph)
  * batch: support thread mode, support wave mode, pairs thread mode, pairs
  * wave mode (ms). Zero after a graph replay. */
@@ -210,8 +217,11 @@ int gpu_autotune(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags, 
  * DedupMode in use: 0 HBM rows, 1 sort, 2 LDS), "fuse_active" (0/1: the
  * dedup in use lets fuse_support act), "last_giants" (seeds cut into
  * sub-items by the last batch), "last_fused" (rules whose support the last
- * batch counted in its body-pair pass) and "last_shared" (rules of the last
- * batch that took another rule's body-pair count). */
+ * batch counted in its body-pair pass), "last_shared" (rules of the last
+ * batch that took another rule's body-pair count), "last_support_searches"
+ * and "last_pair_searches" (support and body-pair plans the last batch ran:
+ * rules whose support the pairs pass counts, and rules that share another
+ * rule's body-pair search, need none of their own). */
 int gpu_set_tuning(void* ctx, const char* key, int64_t value);
 int64_t gpu_get_tuning(void* ctx, const char* key);
 

@@ -469,6 +469,8 @@ bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const s
     B.nFusable = 0;
     B.nFused = 0;
     for (size_t i = 0; i < rules.size(); ++i) B.nFused += fused[i];
+    B.nSearches[0] = B.nSearches[1] = 0;
+    for (const Job& j : jobs) ++B.nSearches[j.ph];
     for (int ph = 0; ph < 2; ++ph) {
         int64_t chunk = kChunkSeeds;
         while (chunk > kMinChunkSeeds && items[ph] / (uint64_t)chunk < (uint64_t)C.chunksPerWave * waves) chunk >>= 1;
@@ -762,6 +764,9 @@ bool batch_run(Context& C, Batch& B, GpuRuleResult64* out) {
     C.lastGiants = (int64_t)B.hGiant[0] + B.hGiant[1];
     C.lastFused = C.fuseActive(mode) ? B.nFused : 0;
     C.lastShared = B.nShared;
+    // fused support chunks do not run: those rules' support is counted by the pairs searches
+    C.lastSearches[0] = B.nSearches[0] - (C.fuseActive(mode) ? B.nFused : 0);
+    C.lastSearches[1] = B.nSearches[1];
     for (int k = 0; k < 2 * kWaveStats; ++k) C.lastWaveStats[k] = B.hWaveStats[k];
     for (int k = 0; k < 4; ++k) C.lastKernelMs[k] = B.kernelMs[k];
     return true;
@@ -987,6 +992,8 @@ static int eval_batch(Context* C, std::vector<RuleIR>& rules, std::vector<int32_
         const bool gated = !(flags & GPU_RULES_UNGATED);
         double tot = 0, ts = 0, tp = 0;
         int64_t heavy = 0;
+        C->lastRuleSupMs.assign(rules.size(), 0.0);
+        C->lastRuleConfMs.assign(rules.size(), 0.0);
         for (size_t i = 0; i < rules.size(); ++i) {
             if (st[i] != kRuleOk) {
                 std::memset(&out[i], 0, sizeof(out[i]));
@@ -1001,6 +1008,8 @@ static int eval_batch(Context* C, std::vector<RuleIR>& rules, std::vector<int32_
             out[i].latencyUs = (int32_t)std::min<int64_t>(
                 std::chrono::duration_cast<std::chrono::microseconds>(t1 - t0).count(), INT32_MAX);
             ts += a; tp += b; tot += a + b;
+            C->lastRuleSupMs[i] = a;
+            C->lastRuleConfMs[i] = b;
             heavy += C->lastHeavy;
         }
         C->lastMs = tot; C->lastSupMs = ts; C->lastPairMs = tp; C->lastHeavy = heavy;
@@ -1254,6 +1263,17 @@ int gpu_last_dedup_stats(void* ctx, double* sortMs, int64_t* keysEmitted, int64_
     return 0;
 }
 
+int gpu_last_rule_times(void* ctx, double* supportMs, double* confidenceMs, int32_t n) {
+    Context* C = as_ctx(ctx);
+    if (!C || n < 0 || (n > 0 && (!supportMs || !confidenceMs))) return -1;
+    const int32_t k = (int32_t)std::min<size_t>((size_t)n, C->lastRuleSupMs.size());
+    for (int32_t i = 0; i < k; ++i) {
+        supportMs[i] = C->lastRuleSupMs[i];
+        confidenceMs[i] = C->lastRuleConfMs[i];
+    }
+    return k;
+}
+
 int gpu_last_kernel_ms(void* ctx, double* ms4) {
     Context* C = as_ctx(ctx);
     if (!C || !ms4) return -1;
@@ -1377,6 +1397,8 @@ int64_t gpu_get_tuning(void* ctx, const char* key) {
     if (k == "fuse_support") return C->fuseSupport ? 1 : 0;
     if (k == "share_bodies") return C->shareBodies ? 1 : 0;
     if (k == "last_shared") return C->lastShared;
+    if (k == "last_support_searches") return C->lastSearches[0];
+    if (k == "last_pair_searches") return C->lastSearches[1];
     if (k == "chunks_per_wave") return C->chunksPerWave;
     if (k == "fuse_active") return C->fuseActive(C->pairsDedup()) ? 1 : 0;
     if (k == "num_cus") return C->numCUs;

@@ -204,6 +204,7 @@ struct Batch : Handle {
     // renaming), or -1; nShared rules have one
     std::vector<int32_t> pairsFrom;
     int64_t nShared = 0;
+    int64_t nSearches[2] = {0, 0};   // plans launched per phase (support searches, body-pair searches)
     uint32_t capRules = 0;
     int slots[2] = {1, 1};   // variable slots of the plans of each phase (picks the kernels' slot class)
     double kernelMs[4] = {};
@@ -282,6 +283,10 @@ struct Context : Handle {
     int64_t lastGiants = 0;
     int64_t lastFused = 0;   // rules whose support the last batch counted in its pairs pass
     int64_t lastShared = 0;  // rules of the last batch that took another rule's body-pair count
+    int64_t lastSearches[2] = {0, 0};   // support / body-pair plans the last batch launched
+    // per rule of the last GPU_RULES_PER_RULE evaluation: device time of its
+    // support launches and of its body-size launches (ms; 0 where none ran)
+    std::vector<double> lastRuleSupMs, lastRuleConfMs;
     unsigned long long lastWaveStats[2 * kWaveStats] = {};
     Batch* lastBatch = nullptr;
     double lastMs = 0, lastSupMs = 0, lastPairMs = 0, lastSortMs = 0;

@@ -66,6 +66,7 @@ _SIGS = {
     "gpu_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p,
                                     ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
     "gpu_last_kernel_ms": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "gpu_last_rule_times": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32]),
     "gpu_last_dedup_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
                                             ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
     "gpu_plan_traffic": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]),
@@ -298,6 +299,15 @@ class Engine:
                                  ctypes.byref(n)) != 0:
             raise RuntimeError("gpu_autotune failed")
         return log[: n.value]
+
+    def rule_times(self, n: int):
+        """(support ms, confidence ms) per rule of the last per-rule evaluation."""
+        a = np.zeros(n, np.float64)
+        b = np.zeros(n, np.float64)
+        k = self.lib.gpu_last_rule_times(self.ctx, a.ctypes.data, b.ctypes.data, int(n))
+        if k < 0:
+            raise RuntimeError("gpu_last_rule_times failed")
+        return a[:k], b[:k]
 
     def kernel_ms(self):
         """[support thread, support wave, pairs thread, pairs wave] of the last

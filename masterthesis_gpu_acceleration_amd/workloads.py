@@ -857,6 +857,18 @@ def serving_requests(seed=SEED_BASE + 5, graph: Graph | None = None, n_requests=
     return t, rules
 
 
+def serving_stream(n_first=400, n_requests=100000, graph: Graph | None = None, seed=SEED_BASE + 5) -> Workload:
+    """The first `n_first` requests of the serving stream -- `n_requests`
+    Poisson arrivals, 1-8 atoms, predicates Zipf s=1.0, no work filter --
+    over the high-fan-out graph (generated unless given), as a Workload. The
+    requests a stream holds depend on its length (the paths of each length
+    are drawn in pools sized for the whole stream), so the prefix is taken
+    from the full-length stream; arrival rates only rescale the arrival times."""
+    g = graph if graph is not None else hub_fanout().graph
+    _, rules = serving_requests(seed=seed, graph=g, n_requests=n_requests, rate_per_s=1000.0, max_work=None)
+    return Workload(g, rules[:n_first], "serving_stream")
+
+
 # ------------------------------------------------------------ rule encoding
 
 RULE_WORDS = 28   # [nBody, head(s,p,o), body[8](s,p,o)]

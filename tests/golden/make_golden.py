@@ -7,7 +7,7 @@ graph and rules from a seed (plus a sha256 of the generated triples, or the
 triples themselves for tiny graphs), the rule lines fed to the reference, and
 the integers the two reference programs printed for every rule.
 
-    python tests/golden/make_golden.py            # all fixture sets
+    python tests/golden/make_golden.py            # all fixture sets (c3 and c5 take about an hour)
     python tests/golden/make_golden.py tiny c1    # selected sets
 
 Run in the build container only (it needs /root/reference); the GPU box and
@@ -19,10 +19,12 @@ import hashlib
 import json
 import os
 import re
+import resource
 import subprocess
 import sys
 import tempfile
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -73,25 +75,95 @@ def run_rewritten(ttl, rules_txt, timeout):
     return [[int(a), int(b)] for a, b in _RW.findall(out)], dt
 
 
-def make_fixture(name, graph, rules, generator=None, inline_triples=False, timeout=1800):
+def _limit_memory(gb):
+    def f():
+        resource.setrlimit(resource.RLIMIT_AS, (gb << 30, gb << 30))
+    return f
+
+
+def run_parallel(d, ttl, rules, workers, chunk, cpu01_timeout, rw_timeout, rw_mem_gb, rewritten=True):
+    """Both reference programs on large graphs: CPU_01 over chunks of
+    `chunk` rules, Rewritten rule by rule (its pair set is global, so its
+    memory follows the largest rule) under an address-space cap of
+    `rw_mem_gb` GiB and a time limit; `workers` processes at a time, each
+    parsing the Turtle file once. Returns (cpu01 rows, rewritten rows with
+    None where the program was withheld, ran out of memory or time, the
+    reasons, cpu01 seconds, rewritten seconds). rewritten=False runs CPU_01
+    only (the canonical expectation; Rewritten is then withheld for every rule)."""
+    starts = list(range(0, len(rules), chunk))
+
+    def one_cpu01(k):
+        part = rules[k:k + chunk]
+        rf = os.path.join(d, "c%06d.txt" % k)
+        W.write_rules(part, rf)
+        rows, dt = run_cpu01(ttl, rf, cpu01_timeout)
+        assert len(rows) == len(part), (k, len(rows), len(part))
+        return k, rows, dt
+
+    def one_rw(i):
+        if not rewritten:
+            return i, None, 0.0, "not run on this fixture set (CPU_01 only)"
+        if len(rules[i][1]) > 7:
+            return i, None, 0.0, "8 body atoms: Rewritten_CPP's used-triple buffer holds 8 (SupportCounting.hpp:20)"
+        rf = os.path.join(d, "w%06d.txt" % i)
+        W.write_rules([rules[i]], rf)
+        t = time.time()
+        try:
+            p = subprocess.run([os.path.join(REF_DIR, "rewritten"), ttl, rf], capture_output=True, text=True,
+                               timeout=rw_timeout, preexec_fn=_limit_memory(rw_mem_gb))
+        except subprocess.TimeoutExpired:
+            return i, None, time.time() - t, "did not finish within %d s" % rw_timeout
+        dt = time.time() - t
+        rows = [[int(a), int(b)] for a, b in _RW.findall(p.stdout)]
+        if p.returncode != 0 or len(rows) != 1:
+            return i, None, dt, "exit %d within a %d GiB address-space cap" % (p.returncode, rw_mem_gb)
+        return i, rows[0], dt, None
+
+    cpu01 = [None] * len(rules)
+    t_cpu01 = 0.0
+    with ThreadPoolExecutor(workers) as ex:
+        for k, rows, dt in ex.map(one_cpu01, starts):
+            cpu01[k:k + len(rows)] = rows
+            t_cpu01 += dt
+            print("  cpu01 rules %d..%d %.1fs" % (k, k + len(rows) - 1, dt), flush=True)
+    rewritten = [None] * len(rules)
+    skipped = {}
+    t_rw = 0.0
+    with ThreadPoolExecutor(workers) as ex:
+        for i, row, dt, why in ex.map(one_rw, range(len(rules))):
+            rewritten[i] = row
+            t_rw += dt
+            if why:
+                skipped[str(i)] = why
+            print("  rewritten rule %d %.1fs %s" % (i, dt, why or row), flush=True)
+    return cpu01, rewritten, skipped, t_cpu01, t_rw
+
+
+def make_fixture(name, graph, rules, generator=None, inline_triples=False, timeout=1800, parallel=None):
     """Runs both reference programs on (graph, rules). Rules with 8 body atoms
     are withheld from Rewritten_CPP: its used-triple buffer holds 8 entries
-    (Rewritten_CPP/SupportCounting.hpp:20) and support needs 9."""
+    (Rewritten_CPP/SupportCounting.hpp:20) and support needs 9. `parallel`
+    (a dict of run_parallel's keyword arguments) runs them in parallel
+    processes, for the full-size graphs."""
+    skipped = {}
     with tempfile.TemporaryDirectory() as d:
         ttl = os.path.join(d, "g.ttl")
         W.write_ttl(graph, ttl)
-        allr = os.path.join(d, "all.txt")
-        W.write_rules(rules, allr)
-        cpu01, t_cpu01 = run_cpu01(ttl, allr, timeout)
-        rw_idx = [i for i, r in enumerate(rules) if len(r[1]) <= 7]
-        rwf = os.path.join(d, "rw.txt")
-        W.write_rules([rules[i] for i in rw_idx], rwf)
-        rw_rows, t_rw = run_rewritten(ttl, rwf, timeout)
+        if parallel:
+            cpu01, rewritten, skipped, t_cpu01, t_rw = run_parallel(d, ttl, rules, **parallel)
+        else:
+            allr = os.path.join(d, "all.txt")
+            W.write_rules(rules, allr)
+            cpu01, t_cpu01 = run_cpu01(ttl, allr, timeout)
+            rw_idx = [i for i, r in enumerate(rules) if len(r[1]) <= 7]
+            rwf = os.path.join(d, "rw.txt")
+            W.write_rules([rules[i] for i in rw_idx], rwf)
+            rw_rows, t_rw = run_rewritten(ttl, rwf, timeout)
+            assert len(rw_rows) == len(rw_idx)
+            rewritten = [None] * len(rules)
+            for i, row in zip(rw_idx, rw_rows):
+                rewritten[i] = row
     assert len(cpu01) == len(rules), (len(cpu01), len(rules))
-    assert len(rw_rows) == len(rw_idx)
-    rewritten = [None] * len(rules)
-    for i, row in zip(rw_idx, rw_rows):
-        rewritten[i] = row
     fx = {
         "name": name,
         "generator": generator,
@@ -105,6 +177,8 @@ def make_fixture(name, graph, rules, generator=None, inline_triples=False, timeo
         "rewritten": rewritten,    # [support, bodySize] per rule, null if withheld
         "ref_seconds": {"cpu01": round(t_cpu01, 3), "rewritten": round(t_rw, 3)},
     }
+    if skipped:
+        fx["rewritten_skipped"] = skipped   # rule index -> why Rewritten_CPP has no row
     if inline_triples:
         fx["triples"] = [graph.s.tolist(), graph.p.tolist(), graph.o.tolist()]
     path = os.path.join(HERE, name + ".json")
@@ -165,6 +239,36 @@ def fx_hub_small():
               n_rules=16)
     w = W.hub_fanout(**kw)
     make_fixture("hub_fanout_20k", w.graph, w.rules, {"fn": "hub_fanout", "kwargs": kw})
+
+
+# Full-size configurations: both reference programs in parallel processes.
+# A C3 rule takes CPU_01 up to about a minute and Rewritten up to about six
+# (its global pair set holds up to ~2e8 pairs, within the 12 GiB cap).
+BIG = dict(workers=4, chunk=5, cpu01_timeout=3600, rw_timeout=2400, rw_mem_gb=12)
+
+
+def fx_c3():
+    w = W.hub_fanout()
+    make_fixture("hub_fanout_full", w.graph, w.rules, {"fn": "hub_fanout", "kwargs": {}}, parallel=BIG)
+
+
+def fx_c4():
+    w = W.dense_conf()
+    make_fixture("dense_conf_full", w.graph, w.rules, {"fn": "dense_conf", "kwargs": {}},
+                 parallel=dict(BIG, chunk=4, rw_mem_gb=8))
+
+
+C5_FIRST = 400
+
+
+def fx_c5():
+    """The first requests of the serving stream (1-8 atoms, no work filter)
+    over the high-fan-out graph; CPU_01 only (Rewritten's Turtle load alone
+    takes ~50 s per process on this graph, and it runs rule by rule here)."""
+    w = W.serving_stream(n_first=C5_FIRST)
+    make_fixture("serving_stream_400", w.graph, w.rules,
+                 {"fn": "serving_stream", "kwargs": {"n_first": C5_FIRST}},
+                 parallel=dict(BIG, chunk=20, workers=6, rewritten=False))
 
 
 FRONTEND_CASES = [
@@ -238,6 +342,9 @@ SETS = {
     "c2": fx_c2,
     "dense300": fx_dense_small,
     "hub20k": fx_hub_small,
+    "c3": fx_c3,
+    "c4": fx_c4,
+    "c5": fx_c5,
 }
 
 if __name__ == "__main__":

@@ -25,7 +25,7 @@ def body_has_constant(rule):
     return any(t[0] == "c" for a in rule[1] for t in (a[0], a[2]))
 
 
-@pytest.mark.parametrize("name", goldens.names())
+@pytest.mark.parametrize("name", goldens.names(cpu=True))
 def test_oracle_matches_reference_programs(name):
     fx, g, rules = goldens.load(name)
     res = Oracle(g.s, g.p, g.o).eval(W.encode_rules(rules))
@@ -64,3 +64,32 @@ def test_fixtures_exercise_the_divergences():
             if rw is not None and body_has_constant(r) and rw[0] > 0 and rw[1] != fx["cpu01"][i][3]:
                 n_const_diff += 1
     assert n_const_diff >= 1
+
+
+@pytest.mark.parametrize("name", goldens.HIGH_FANOUT + ("dense_conf_full",))
+def test_reference_programs_agree_at_full_size(name):
+    """On the full-size fixtures the GPU is checked against CPU_01's values
+    directly; where Rewritten_CPP also finished, the two programs must agree
+    (up to Rewritten's documented 0-for-1 support defect), so the expected
+    values rest on two independent implementations. Reads the JSON only."""
+    import json
+    import os
+    with open(os.path.join(goldens.GOLDEN, name + ".json")) as f:
+        fx = json.load(f)
+    rules = [goldens.rule_from_json(r) for r in fx["rules"]]
+    compared = 0
+    for i, r in enumerate(rules):
+        rw = fx["rewritten"][i]
+        if rw is None or body_has_constant(r):
+            continue
+        c = fx["cpu01"][i]
+        pairs = c[3] if goldens.head_pair_in_body(r) else 0
+        if rw[0] != c[0]:
+            assert rw[0] == 0 and c[0] == 1, (name, i, rw, c)
+            continue
+        assert rw[1] == (pairs if c[0] > 0 else 0), (name, i, rw, c)
+        compared += 1
+    skipped = fx.get("rewritten_skipped", {})
+    assert compared + len(skipped) >= len(rules) - 2, (name, compared, len(skipped))
+    if name != "serving_stream_400":
+        assert compared >= len(rules) // 2
