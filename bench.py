@@ -303,6 +303,10 @@ def measure(args, name, wl, decompose):
         log("%s: timed %d steps: %.3f ms/step (device %.3f ms)" % (name, args.steps, ms_step, float(np.mean(dev_ms))))
         rules_per_s = n / (ms_step / 1000.0)
         pairs = int(res["bodyPairs"][res["bodyPairs"] > 0].sum())
+        wave = {ph: {q: eng.get_tuning("wave_%s_%s" % (ph, q)) for q in ("seeds", "steps", "donations", "leaf_draws",
+                                                                          "draws")}
+                for ph in ("support", "pairs")}
+        wave["giants"] = eng.get_tuning("last_giants")
         searches = {"support": eng.get_tuning("last_support_searches"), "pairs": eng.get_tuning("last_pair_searches"),
                     "support_fused_rules": eng.get_tuning("last_fused"),
                     "rules_sharing_a_search": eng.get_tuning("last_shared")}
@@ -316,6 +320,7 @@ def measure(args, name, wl, decompose):
             "rules_with_support": int((res["support"] > 0).sum()),
             "escalated_seeds": int(eng.last_stats()["heavy_seeds"]),
             "searches": searches,
+            "wave_mode": wave,
             "tuning": dict(tuning, wave_slots=eng.get_tuning("wave_slots"), fuse_active=eng.get_tuning("fuse_active")),
             "latency_ms": {"batched_p50": round(pct(dev_ms, 0.5), 4), "batched_p99": round(pct(dev_ms, 0.99), 4)},
             "parity_vs_golden": parity_vs_golden(golden, wl, res, args.gated),
@@ -529,6 +534,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="dbpedia_shaped", choices=sorted(WORKLOADS))
     ap.add_argument("--only", action="store_true", help="measure --workload alone (no high-fan-out or serving block)")
+    ap.add_argument("--atoms", default="", help="diagnostics: keep only rules with these body lengths (comma list)")
     ap.add_argument("--cpu-sample", type=int, default=0, help="rules in the CPU-baseline sample (0: per workload)")
     ap.add_argument("--gated", action="store_true", help="body pairs only for rules with support > 0")
     ap.add_argument("--serve", action="store_true", help="only the serving block")
@@ -579,6 +585,9 @@ def main():
         return
 
     wl = WORKLOADS[args.workload][0]()
+    if args.atoms:
+        keep = {int(x) for x in args.atoms.split(",")}
+        wl = W.Workload(wl.graph, [r for r in wl.rules if len(r[1]) in keep], wl.name)
     log("workload %s (%.1fs)" % (describe(wl), time.time() - t0))
     head = measure(args, args.workload, wl, decompose=not args.no_decompose)
     out = {

@@ -268,7 +268,7 @@ def fx_c5():
     w = W.serving_stream(n_first=C5_FIRST)
     make_fixture("serving_stream_400", w.graph, w.rules,
                  {"fn": "serving_stream", "kwargs": {"n_first": C5_FIRST}},
-                 parallel=dict(BIG, chunk=20, workers=6, rewritten=False))
+                 parallel=dict(BIG, chunk=20, workers=4, rewritten=False))
 
 
 FRONTEND_CASES = [
