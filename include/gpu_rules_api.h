@@ -67,6 +67,21 @@ int gpu_query_vram(size_t* freeBytesOut, size_t* totalBytesOut);
  * Additive API (no counterpart in the reference).
  * ------------------------------------------------------------------------ */
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving or mining loops,
+ * harnesses that forbid hipMalloc: gpu_evaluate_rule with the same
+ * arguments, results and return conventions, but its kernels' scratch
+ * (counters, control words, conflict tables and the heavy-seed queue) is a
+ * fixed 16 MiB device array that every GPU holds from library load to
+ * process end, so a call allocates nothing (gpu_evaluate_rule grows its
+ * queue on demand). Returns -1 also when a launch of the rule has more seeds
+ * than the array holds (2,097,088: the rule's head triples, or its body
+ * atom's distinct head subjects) or than the context's "noalloc_max_seeds",
+ * or when sort-unique dedup is in use and its key buffers are missing or
+ * too small (they are never grown here). Calls on one GPU share the array:
+ * calls that overlap on one GPU, from any context, race. */
+int gpu_evaluate_rule_noalloc(void* ctx, const char* ruleText, GpuRuleResult* out);
+
 /* Structured rule: words[0] = number of body atoms, words[1..3] = head
  * (s, p, o), words[4+3j .. 6+3j] = body atom j (s, p, o), j < 8. A term
  * >= 0 is a term id of the graph; a term < 0 is variable -(t+1). */
@@ -208,7 +223,9 @@ int gpu_autotune(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags, 
  * wave-mode waves per CU still fit, else HBM bitmap rows unless a row per CU
  * exceeds 8 GiB, else sort-unique; 1 HBM bitmap rows; 2 sort-unique; 3 LDS
  * bitmaps where they fit, else rows), "sort_cap" (initial key capacity of the
- * sort-unique buffers, >= 4096; they grow on overflow), "sort_cap_max" (the
+ * sort-unique buffers, >= 4096; they grow on overflow), "noalloc_max_seeds"
+ * (seeds per launch gpu_evaluate_rule_noalloc accepts, 1 .. 2,097,088,
+ * default all its scratch holds), "sort_cap_max" (the
  * largest capacity they may grow to, 4096 .. 2^31, default 2^31: a pass that
  * emits more keys fails -- gpu_evaluate_rules* and gpu_batch_run return -1,
  * served requests complete with status 2). Returns 0, or -1 for

@@ -28,7 +28,8 @@ HIPFLAGS = ["-x", "hip", "--offload-arch=" + ARCH, "-Xclang", "-target-feature",
             "-fno-gpu-rdc"]
 
 SOURCES = ["graph_host.cpp", "rules_host.cpp", "planner.cpp", "traffic.cpp", "snapshot.cpp", "engine.cpp", "serve.cpp",
-           "kernels.hip", "search_s4.hip", "search_s8.hip", "search_s16.hip", "sort_unique.hip", "graph_device.hip"]
+           "kernels.hip", "search_s4.hip", "search_s8.hip", "search_s16.hip", "sort_unique.hip", "graph_device.hip",
+           "noalloc.hip"]
 
 
 def _compile(src: str) -> str:

@@ -16,6 +16,7 @@
 
 #include "engine_internal.hpp"
 #include "graph_device.hpp"
+#include "noalloc.hpp"
 #include "snapshot.hpp"
 #include "traffic.hpp"
 
@@ -34,6 +35,8 @@ bool Context::upload() {
     GR_HIP(hipEventCreateWithFlags(&joinEv, hipEventDisableTiming));
     for (auto& e : ev) GR_HIP(hipEventCreate(&e));
     GR_HIP(hipHostMalloc((void**)&hOne, 512, hipHostMallocDefault));
+    // the allocation-free entry point's scratch on this device (reported there if unavailable)
+    noallocReady = noalloc_init(device);
 
     const int32_t P = g.P();
     const int64_t T = g.T();
@@ -183,13 +186,29 @@ bool has_conflicts(const Plan& P) {
     return false;
 }
 
+// Device scratch of the per-rule launches: a 512-byte header -- [0] count
+// (u64); +64: u32 heavy count, wave cursor; +96: u64 keys emitted (sort
+// path); +128: ConflictRefs of both plans -- and the heavy-seed queue.
+// The context's own (C.one, C.heavy1, grown as rules need) or, for the
+// allocation-free entry point, the fixed device scratch array, where
+// nothing may grow (mayAlloc false).
+struct OneScratch {
+    char* hdr;
+    HeavyItem* heavy;
+    uint64_t heavyCap;
+    bool mayAlloc;
+};
+
+OneScratch context_scratch(Context& C) { return OneScratch{C.one.as<char>(), nullptr, 0, true}; }
+
 // One plan: thread-mode launch over all seeds, wave-mode launch over the
 // escalated ones, sync, copy the count back.
 // With the sort-unique dedup path, a body-pair plan's wave-mode launch
 // emits keys and the sort-unique pass counts them; a pass that emitted more
-// keys than the buffer holds is redone with a larger one.
-bool run_one(Context& C, const Plan& P, const ConflictRefs& R, const Plan& PW, const ConflictRefs& RW, int64_t items,
-             uint64_t& count, double& ms, int64_t& heavySeeds) {
+// keys than the buffer holds is redone with a larger one (when S may grow
+// buffers; else it fails).
+bool run_one(Context& C, OneScratch S, const Plan& P, const ConflictRefs& R, const Plan& PW, const ConflictRefs& RW,
+             int64_t items, uint64_t& count, double& ms, int64_t& heavySeeds) {
     count = 0;
     ms = 0;
     if (items <= 0) return true;
@@ -198,16 +217,23 @@ bool run_one(Context& C, const Plan& P, const ConflictRefs& R, const Plan& PW, c
                      (long long)items);
         return false;
     }
-    if ((uint64_t)items > C.heavy1Cap) {
-        if (!C.heavy1.ensure((size_t)items * sizeof(HeavyItem))) return false;
-        C.heavy1Cap = (uint32_t)items;
+    if (S.mayAlloc) {
+        if ((uint64_t)items > C.heavy1Cap) {
+            if (!C.heavy1.ensure((size_t)items * sizeof(HeavyItem))) return false;
+            C.heavy1Cap = (uint32_t)items;
+        }
+        S.heavy = C.heavy1.as<HeavyItem>();
+        S.heavyCap = C.heavy1Cap;
+    } else if ((uint64_t)items > S.heavyCap) {
+        std::fprintf(stderr, "[gpu_rules] allocation-free evaluation: a plan with %lld seeds exceeds the scratch "
+                             "queue's %llu\n", (long long)items, (unsigned long long)S.heavyCap);
+        return false;
     }
-    // C.one: [0] count (u64); +64: u32 heavy count, wave cursor; +96: u64 keys
-    // emitted (sort path); +128: ConflictRefs
-    unsigned long long* d_count = C.one.as<unsigned long long>();
-    uint32_t* ctl = reinterpret_cast<uint32_t*>(C.one.as<char>() + 64);
-    unsigned long long* d_emitted = reinterpret_cast<unsigned long long*>(C.one.as<char>() + 96);
-    ConflictRefs* d_refs = reinterpret_cast<ConflictRefs*>(C.one.as<char>() + 128);
+    const uint32_t heavyCap = (uint32_t)S.heavyCap;
+    unsigned long long* d_count = reinterpret_cast<unsigned long long*>(S.hdr);
+    uint32_t* ctl = reinterpret_cast<uint32_t*>(S.hdr + 64);
+    unsigned long long* d_emitted = reinterpret_cast<unsigned long long*>(S.hdr + 96);
+    ConflictRefs* d_refs = reinterpret_cast<ConflictRefs*>(S.hdr + 128);
     const bool conf = has_conflicts(P) || has_conflicts(PW);
     if (conf) {
         ConflictRefs both[2] = {R, RW};
@@ -216,7 +242,12 @@ bool run_one(Context& C, const Plan& P, const ConflictRefs& R, const Plan& PW, c
     const int mode = PW.kind == kPairs ? C.pairsDedup() : kDedupRows;
     const bool sort = mode == kDedupSort;
     const int slots = std::max(plan_slots(P, R), plan_slots(PW, RW));
-    if (sort && !C.sortb.ensure(1, kSortCapMin, C.sortCapMax)) return false;
+    if (sort && !S.mayAlloc && C.sortb.cap == 0) {
+        std::fprintf(stderr, "[gpu_rules] allocation-free evaluation: sort-unique dedup needs key buffers, which "
+                             "this context has not allocated (set \"sort_cap\" first)\n");
+        return false;
+    }
+    if (sort && S.mayAlloc && !C.sortb.ensure(1, kSortCapMin, C.sortCapMax)) return false;
     const int hoBits = bits_for((uint64_t)C.g.nE);
     const int hBits = bits_for((uint64_t)items);
     for (;;) {
@@ -225,11 +256,11 @@ bool run_one(Context& C, const Plan& P, const ConflictRefs& R, const Plan& PW, c
         GR_HIP(hipEventRecord(C.ev[0], C.st));
         const int64_t waves = (items + kWave - 1) / kWave;
         const int blocks = (int)std::min<int64_t>((waves + 3) / 4, C.threadBlocks());
-        OneArgs oa{conf ? d_refs : nullptr, (uint32_t)items, C.heavy1Cap, d_count, C.heavy1.as<HeavyItem>(), ctl + 0};
+        OneArgs oa{conf ? d_refs : nullptr, (uint32_t)items, heavyCap, d_count, S.heavy, ctl + 0};
         launch_thread_one(P.kind, slots, P, oa, C.dg, blocks, C.st);
         SortSink sink{};
         if (sort) sink = SortSink{C.sortb.keys[0].as<unsigned long long>(), d_emitted, C.sortb.cap, hoBits};
-        WaveArgs wa{nullptr, conf ? d_refs + 1 : nullptr, C.heavy1.as<HeavyItem>(), ctl + 0, C.heavy1Cap, 0, ctl + 1,
+        WaveArgs wa{nullptr, conf ? d_refs + 1 : nullptr, S.heavy, ctl + 0, heavyCap, 0, ctl + 1,
                     d_count, C.dedup.as<uint32_t>(), C.rowWords, sink};
         launch_wave_one(PW.kind, slots, mode, PW, wa, C.dg, C.waveBlocksFor(slots), C.st);
         if (sort) {
@@ -240,13 +271,12 @@ bool run_one(Context& C, const Plan& P, const ConflictRefs& R, const Plan& PW, c
         }
         GR_HIP(hipGetLastError());
         GR_HIP(hipEventRecord(C.ev[1], C.st));
-        GR_HIP(hipMemcpyAsync(C.hOne, C.one.p, 128, hipMemcpyDeviceToHost, C.st));
+        GR_HIP(hipMemcpyAsync(C.hOne, S.hdr, 128, hipMemcpyDeviceToHost, C.st));
         GR_HIP(hipStreamSynchronize(C.st));
         const uint32_t* hctl = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(C.hOne) + 64);
         const uint64_t emitted = *reinterpret_cast<const unsigned long long*>(reinterpret_cast<char*>(C.hOne) + 96);
-        if (heavy_count_action(hctl[0], C.heavy1Cap) != kCountsOk) {
-            std::fprintf(stderr, "[gpu_rules] %u seeds escalated, beyond the heavy queue's %u\n", hctl[0],
-                         C.heavy1Cap);
+        if (heavy_count_action(hctl[0], heavyCap) != kCountsOk) {
+            std::fprintf(stderr, "[gpu_rules] %u seeds escalated, beyond the heavy queue's %u\n", hctl[0], heavyCap);
             return false;
         }
         if (sort) {
@@ -258,6 +288,12 @@ bool run_one(Context& C, const Plan& P, const ConflictRefs& R, const Plan& PW, c
                 return false;
             }
             if (act == kCountsRegrow) {
+                if (!S.mayAlloc) {
+                    std::fprintf(stderr, "[gpu_rules] allocation-free evaluation: body-pair keys of one rule "
+                                         "(%llu) exceed the sort buffers' %u\n", (unsigned long long)emitted,
+                                 C.sortb.cap);
+                    return false;
+                }
                 if (!C.sortb.ensure(emitted, kSortCapMin, C.sortCapMax)) return false;
                 continue;
             }
@@ -276,7 +312,8 @@ void finish_measures(GpuRuleResult64& r) {
     r.confidence = r.bodySize > 0 ? (double)r.support / (double)r.bodySize : 0.0;
 }
 
-bool eval_rule_plain(Context& C, const RuleIR& r, bool gated, GpuRuleResult64& res, double& supMs, double& confMs) {
+bool eval_rule_plain(Context& C, OneScratch S, const RuleIR& r, bool gated, GpuRuleResult64& res, double& supMs,
+                     double& confMs) {
     std::memset(&res, 0, sizeof(res));
     res.bodyPairs = -1;
     supMs = confMs = 0;
@@ -287,13 +324,13 @@ bool eval_rule_plain(Context& C, const RuleIR& r, bool gated, GpuRuleResult64& r
     if (cr.status != kRuleOk) return true;
     int64_t heavy = 0;
     uint64_t sup = 0;
-    if (cr.supportLaunch && !run_one(C, cr.sup, cr.supRefs, cr.supWave, cr.supWaveRefs, cr.supItems, sup, supMs, heavy))
+    if (cr.supportLaunch && !run_one(C, S, cr.sup, cr.supRefs, cr.supWave, cr.supWaveRefs, cr.supItems, sup, supMs, heavy))
         return false;
     res.support = (int64_t)sup;
     if (!gated || sup > 0) {
         uint64_t pairs = 0;
         if (cr.pairsLaunch &&
-            !run_one(C, cr.pairs, cr.pairsRefs, cr.pairsWave, cr.pairsWaveRefs, cr.pairItems, pairs, confMs, heavy))
+            !run_one(C, S, cr.pairs, cr.pairsRefs, cr.pairsWave, cr.pairsWaveRefs, cr.pairItems, pairs, confMs, heavy))
             return false;
         res.bodyPairs = (int64_t)pairs;
     }
@@ -943,10 +980,10 @@ void gpu_free_graph(void* ctx) {
     delete C;
 }
 
-static int eval_one_abi(Context* C, const RuleIR& r, GpuRuleResult* out) {
+static int eval_one_abi(Context* C, const RuleIR& r, GpuRuleResult* out, OneScratch S) {
     GpuRuleResult64 res;
     double supMs = 0, confMs = 0;
-    if (!eval_rule_plain(*C, r, true, res, supMs, confMs)) return -1;
+    if (!eval_rule_plain(*C, S, r, true, res, supMs, confMs)) return -1;
     out->support = (int)res.support;
     out->headSize = (int)res.headSize;
     out->headSupport = (int)res.headSupport;
@@ -966,9 +1003,29 @@ int gpu_evaluate_rule(void* ctx, const char* ruleText, GpuRuleResult* out) {
     std::memset(out, 0, sizeof(*out));
     try {
         RuleIR r = parse_rule_text(ruleText, *C->tr);
-        return eval_one_abi(C, r, out);
+        return eval_one_abi(C, r, out, context_scratch(*C));
     } catch (const std::exception& e) {
         std::fprintf(stderr, "[gpu_rules] gpu_evaluate_rule error: %s\n", e.what());
+        return -1;
+    }
+}
+
+int gpu_evaluate_rule_noalloc(void* ctx, const char* ruleText, GpuRuleResult* out) {
+    Context* C = as_ctx(ctx);
+    if (!C || !ruleText || !out) return -1;
+    std::memset(out, 0, sizeof(*out));
+    char* s = C->noallocReady ? static_cast<char*>(noalloc_scratch(C->device)) : nullptr;
+    if (!s) {
+        std::fprintf(stderr, "[gpu_rules] gpu_evaluate_rule_noalloc: no device scratch on device %d\n", C->device);
+        return -1;
+    }
+    try {
+        RuleIR r = parse_rule_text(ruleText, *C->tr);
+        OneScratch S{s, reinterpret_cast<HeavyItem*>(s + kNoAllocHeader),
+                     std::min<uint64_t>(kNoAllocMaxSeeds, C->noallocMaxSeeds), false};
+        return eval_one_abi(C, r, out, S);
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "[gpu_rules] gpu_evaluate_rule_noalloc error: %s\n", e.what());
         return -1;
     }
 }
@@ -979,7 +1036,7 @@ int gpu_evaluate_rule_json(void* ctx, const char* ruleJson, GpuRuleResult* out) 
     std::memset(out, 0, sizeof(*out));
     try {
         RuleIR r = parse_rule_json(ruleJson, *C->tr);
-        return eval_one_abi(C, r, out);
+        return eval_one_abi(C, r, out, context_scratch(*C));
     } catch (const std::exception& e) {
         std::fprintf(stderr, "[gpu_rules] gpu_evaluate_rule_json error: %s\n", e.what());
         return -1;
@@ -1003,7 +1060,7 @@ static int eval_batch(Context* C, std::vector<RuleIR>& rules, std::vector<int32_
             }
             double a = 0, b = 0;
             const auto t0 = std::chrono::steady_clock::now();
-            if (!eval_rule_plain(*C, rules[i], gated, out[i], a, b)) return -1;
+            if (!eval_rule_plain(*C, context_scratch(*C), rules[i], gated, out[i], a, b)) return -1;
             const auto t1 = std::chrono::steady_clock::now();
             out[i].latencyUs = (int32_t)std::min<int64_t>(
                 std::chrono::duration_cast<std::chrono::microseconds>(t1 - t0).count(), INT32_MAX);
@@ -1374,6 +1431,10 @@ int gpu_set_tuning(void* ctx, const char* key, int64_t value) {
         if (value < kSortTile || value > (int64_t)C->sortCapMax) return -1;
         C->sortb.release();
         if (!C->sortb.ensure((uint64_t)value, (uint64_t)value, C->sortCapMax)) return -1;
+    } else if (k == "noalloc_max_seeds") {
+        // seeds per launch the allocation-free entry point accepts (at most what its scratch holds)
+        if (value < 1 || (uint64_t)value > kNoAllocMaxSeeds) return -1;
+        C->noallocMaxSeeds = (uint64_t)value;
     } else if (k == "sort_cap_max") {
         // largest key capacity the sort-unique buffers may grow to; a pass
         // that emits more keys fails (default and upper limit 2^31)
@@ -1417,6 +1478,7 @@ int64_t gpu_get_tuning(void* ctx, const char* key) {
                 return (int64_t)C->lastWaveStats[ph * kWaveStats + q];
     if (k == "sort_cap") return C->sortb.cap;
     if (k == "sort_cap_max") return C->sortCapMax;
+    if (k == "noalloc_max_seeds") return (int64_t)C->noallocMaxSeeds;
     return -1;
 }
 

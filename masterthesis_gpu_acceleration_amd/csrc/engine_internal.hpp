@@ -261,6 +261,8 @@ struct Context : Handle {
     int waveSlots = 0;
     SortBufs sortb;   // sort-unique dedup buffers (allocated when that path is used)
     uint32_t sortCapMax = kSortCapMax;   // largest key capacity they may grow to ("sort_cap_max")
+    bool noallocReady = false;           // the allocation-free entry point's device scratch is mapped
+    uint64_t noallocMaxSeeds = ~0ull;    // its seeds-per-launch limit ("noalloc_max_seeds"; capped by the scratch)
     GiantBufs giants; // giant-seed pass
     // Ungated batches replayed as graphs run the support and body-pair phases
     // side by side: the pairs phase on st2 (forked from st by forkEv, joined

@@ -48,6 +48,7 @@ _SIGS = {
     "gpu_load_graph": (ctypes.c_void_p, [ctypes.c_char_p]),
     "gpu_evaluate_rule": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(GpuRuleResult)]),
     "gpu_evaluate_rule_json": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(GpuRuleResult)]),
+    "gpu_evaluate_rule_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(GpuRuleResult)]),
     "gpu_free_graph": (None, [ctypes.c_void_p]),
     "gpu_query_vram": (ctypes.c_int, [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]),
     "gpu_load_graph_triples": (ctypes.c_void_p, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
