@@ -304,8 +304,12 @@ def measure(args, name, wl, decompose):
         rules_per_s = n / (ms_step / 1000.0)
         pairs = int(res["bodyPairs"][res["bodyPairs"] > 0].sum())
         wave = {ph: {q: eng.get_tuning("wave_%s_%s" % (ph, q)) for q in ("seeds", "steps", "donations", "leaf_draws",
-                                                                          "draws")}
+                                                                          "draws", "busy_lane_steps", "draw_lanes")}
                 for ph in ("support", "pairs")}
+        cyc = {q: eng.get_tuning("wave_pairs_cycles_" + q) for q in ("take", "setup", "leaf_draw", "draw", "own",
+                                                                     "step_end", "seed_end", "sub_items")}
+        if any(cyc.values()):   # the profiling build (GPU_RULES_LIB=.../libgpu_rules_prof.so)
+            wave["pairs_cycles"] = cyc
         wave["giants"] = eng.get_tuning("last_giants")
         searches = {"support": eng.get_tuning("last_support_searches"), "pairs": eng.get_tuning("last_pair_searches"),
                     "support_fused_rules": eng.get_tuning("last_fused"),

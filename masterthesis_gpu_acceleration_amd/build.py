@@ -1,6 +1,11 @@
 """Builds libgpu_rules.so (host C++17 + HIP kernels for gfx950) in-tree.
 
     python -m masterthesis_gpu_acceleration_amd.build
+    python -m masterthesis_gpu_acceleration_amd.build --profile
+
+--profile builds libgpu_rules_prof.so instead (objects in _build_prof/): the
+same library with the wave-mode kernels summing cycles per search section
+into their counters (GR_STEP_PROFILE); load it with GPU_RULES_LIB=<path>.
 
 Objects go to masterthesis_gpu_acceleration_amd/_build/, the library to
 masterthesis_gpu_acceleration_amd/libgpu_rules.so (both git-ignored; the
@@ -32,10 +37,10 @@ SOURCES = ["graph_host.cpp", "rules_host.cpp", "planner.cpp", "traffic.cpp", "sn
            "noalloc.hip"]
 
 
-def _compile(src: str) -> str:
+def _compile(src: str, obj_dir: str = OBJ, extra=()) -> str:
     path = os.path.join(CSRC, src)
-    obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
-    flags = COMMON + (HIPFLAGS if src.endswith(".hip") else ["-x", "c++", "-D__HIP_PLATFORM_AMD__"])
+    obj = os.path.join(obj_dir, os.path.splitext(src)[0] + ".o")
+    flags = COMMON + (HIPFLAGS + list(extra) if src.endswith(".hip") else ["-x", "c++", "-D__HIP_PLATFORM_AMD__"])
     cmd = [HIPCC] + flags + ["-c", path, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -48,21 +53,24 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(verbose: bool = True) -> str:
-    os.makedirs(OBJ, exist_ok=True)
+def build(verbose: bool = True, profile: bool = False) -> str:
+    obj_dir = OBJ + "_prof" if profile else OBJ
+    lib = os.path.join(PKG, "libgpu_rules_prof.so") if profile else LIB
+    extra = ["-DGR_STEP_PROFILE"] if profile else []
+    os.makedirs(obj_dir, exist_ok=True)
     with ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
-        objs = list(ex.map(_compile, SOURCES))
-    tmp = LIB + ".tmp"
+        objs = list(ex.map(lambda s: _compile(s, obj_dir, extra), SOURCES))
+    tmp = lib + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", "-pthread", "--offload-arch=" + ARCH, "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("link failed")
-    os.replace(tmp, LIB)
+    os.replace(tmp, lib)
     if verbose:
-        print("built", os.path.relpath(LIB, ROOT))
-    return LIB
+        print("built", os.path.relpath(lib, ROOT))
+    return lib
 
 
 if __name__ == "__main__":
-    build()
+    build(profile="--profile" in sys.argv[1:])

@@ -1463,6 +1463,10 @@ int64_t gpu_get_tuning(void* ctx, const char* key) {
     if (k == "chunks_per_wave") return C->chunksPerWave;
     if (k == "fuse_active") return C->fuseActive(C->pairsDedup()) ? 1 : 0;
     if (k == "num_cus") return C->numCUs;
+    if (k == "lds_per_cu") return (int64_t)C->ldsPerCU;
+    if (k == "lds_per_block") return (int64_t)C->ldsPerBlock;
+    if (k == "wave_lds_static") return (int64_t)wave_static_lds(4);
+    if (k == "wave_lds_blocks_per_cu") return wave_lds_blocks_per_cu(4, (size_t)C->rowWords * 4);
     if (k == "wave_slots") return C->waveSlots;
     if (k == "dedup") return C->dedupMode;
     if (k == "sort_active") return C->sortActive() ? 1 : 0;
@@ -1470,8 +1474,11 @@ int64_t gpu_get_tuning(void* ctx, const char* key) {
     if (k == "giant_after") return C->giantAfter;
     if (k == "last_giants") return C->lastGiants;
     if (k == "last_fused") return C->lastFused;
-    // wave-mode counters of the last batch: wave_<phase>_<seeds|steps|donations|leaf_draws|draws>
-    static const char* kStatNames[kWaveStats] = {"seeds", "steps", "donations", "leaf_draws", "draws"};
+    // wave-mode counters of the last batch: wave_<phase>_<seeds|steps|donations|leaf_draws|draws|busy_lane_steps|draw_lanes>
+    static const char* kStatNames[kWaveStats] = {"seeds", "steps", "donations", "leaf_draws", "draws",
+                                                 "busy_lane_steps", "draw_lanes", "cycles_take", "cycles_setup",
+                                                 "cycles_leaf_draw", "cycles_draw", "cycles_own", "cycles_step_end",
+                                                 "cycles_seed_end", "cycles_sub_items"};
     for (int ph = 0; ph < 2; ++ph)
         for (int q = 0; q < kWaveStats; ++q)
             if (k == std::string(ph ? "wave_pairs_" : "wave_support_") + kStatNames[q])

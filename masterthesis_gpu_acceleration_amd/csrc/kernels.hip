@@ -86,6 +86,14 @@ size_t wave_static_lds(int slots) {
     }
 }
 
+int wave_lds_blocks_per_cu(int slots, size_t dynLds) {
+    switch (slot_class(slots)) {
+        case 4: return SlotKernels<4>::lds_blocks_per_cu(dynLds);
+        case 8: return SlotKernels<8>::lds_blocks_per_cu(dynLds);
+        default: return SlotKernels<16>::lds_blocks_per_cu(dynLds);
+    }
+}
+
 void prepare_wave_kernels(size_t dynLds) {
     SlotKernels<4>::prepare(dynLds);
     SlotKernels<8>::prepare(dynLds);

@@ -91,8 +91,16 @@ struct GiantArgs {
 };
 
 // Wave-mode counters of one launch: seeds and sub-items run, loop steps,
-// donations, pool draws at the last level, pool draws above it.
-constexpr int kWaveStats = 5;
+// donations, pool draws at the last level, pool draws above it, lanes busy
+// with their own search at the end of a step and lanes that took pool
+// entries in a step (both summed over steps); then kProfSlots cycle sums of
+// the profiling build (kernels compiled with GR_STEP_PROFILE; zero
+// otherwise): taking a seed and its plan, setting the seed up, pool draws at
+// the last level, other pool draws, own-search steps, the end of a step
+// (donation), the end of a seed (fused support, bitmap clear), sub-items.
+constexpr int kProfBase = 7;
+constexpr int kProfSlots = 8;
+constexpr int kWaveStats = kProfBase + kProfSlots;
 
 // Wave-mode pass over the heavy queue.
 struct WaveArgs {
@@ -126,6 +134,7 @@ void launch_wave_one(int kind, int slots, int dedup, const Plan& P, const WaveAr
 // (bytes, from the runtime; 16: the largest class), and the dynamic-LDS
 // limit to request for the kDedupLds kernels.
 size_t wave_static_lds(int slots = 16);
+int wave_lds_blocks_per_cu(int slots, size_t dynLds);
 void prepare_wave_kernels(size_t dynLds);
 // The giant-seed pass that follows a wave-mode batch launch: expansion of
 // the listed giants, then their sub-items (same `a` as the wave launch).
@@ -143,6 +152,9 @@ struct SlotKernels {
     static void giants(int kind, int dedup, const WaveArgs& a, const DevGraph& G, int blocks, hipStream_t s);
     static void prepare(size_t dynLds);
     static size_t lds_static();   // static LDS of the kDedupLds wave kernel (bytes)
+    // resident one-wave blocks per CU of the kDedupLds pairs wave kernel with
+    // `dynLds` bytes of dynamic LDS (the runtime's occupancy query), 0 on error
+    static int lds_blocks_per_cu(size_t dynLds);
 };
 extern template struct SlotKernels<4>;
 extern template struct SlotKernels<8>;

@@ -43,6 +43,16 @@
 
 namespace gr {
 
+// Cycle attribution (profiling build only): s_memtime sums per section of
+// the wave-mode search, into the wave counters from kProfBase on.
+#ifdef GR_STEP_PROFILE
+#define GR_PROF_NOW() ((unsigned long long)__builtin_readcyclecounter())
+#define GR_PROF_ADD(slot, t0) (stats[kProfBase + (slot)] += GR_PROF_NOW() - (t0))
+#else
+#define GR_PROF_NOW() 0ull
+#define GR_PROF_ADD(slot, t0) ((void)(t0))
+#endif
+
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr int kPlanWords = (int)(sizeof(Plan) / 4);
@@ -427,6 +437,54 @@ __device__ __forceinline__ bool leaf_plain(const Plan& P, int last, bool pairs) 
     return !pairs || P.pairLevel == last;
 }
 
+// Single-frame count (thread mode). A PAIRS seed whose search stands at a
+// plain-EXT pair level that is the last level, with no head object counted
+// yet and no untried entry left in any frame above, has one row left to
+// give head objects: each entry of it grounds unless a bound variable holds
+// its value (rows hold each value once), so its distinct head objects number
+// the entries minus the bound values found in the row -- one binary search
+// per bound variable instead of enumerating the row. The fused support of
+// such a seed is the head-row entries (at most kShortHead) in the row that no
+// bound variable holds. Frames shorter than kShortMin are enumerated.
+constexpr int kShortMin = 16;
+constexpr int kShortHead = 16;
+
+template <int NL>
+__device__ __forceinline__ bool frames_above_done(const ThreadLdsT<NL>& W, int d) {
+    const int lane = lane_id();
+    bool done = true;
+#pragma unroll
+    for (int l = 0; l < NL; ++l)
+        if (l < d) done = done && W.frame[l][lane].i >= W.frame[l][lane].end;
+    return done;
+}
+
+// Entries of row fr (at the last level) that no bound slot j < nb holds.
+template <int NS>
+__device__ __forceinline__ int frame_count(const DevGraph& G, Frame fr, const int (&v)[NS], int nb) {
+    int n = fr.end - fr.i;
+#pragma unroll
+    for (int j = 0; j < NS; ++j)
+        if (j < nb && find_in_row(G.col, fr.i, fr.end, v[j]) >= 0) --n;
+    return n;
+}
+
+// Head-row entries of seed s (head predicate run `base`) that row fr holds
+// and no bound slot takes: the seed's supported head triples when fr is its
+// only frame. Returns -1 if the head row is longer than kShortHead.
+template <int NS>
+__device__ __forceinline__ int frame_head_hits(const DevGraph& G, int64_t base, int s, Frame fr, const int (&v)[NS],
+                                               int nb) {
+    const Frame hr = key_row(G, base, s);
+    if (hr.end - hr.i > kShortHead) return -1;
+    int n = 0;
+    for (int x = hr.i; x < hr.end; ++x) {
+        const int h = G.col[x];
+        if (!taken(v, nb, h) && find_in_row(G.col, fr.i, fr.end, h) >= 0) ++n;
+    }
+    return n;
+}
+
 // Whether level `last` takes the leaf_fast path of wave mode under dedup D
 // (bitmap marks by crank rank).
 template <int D>
@@ -460,12 +518,15 @@ __device__ __forceinline__ uint64_t thread_chunk(ThreadLdsT<levels_of(NS)>& W, c
     const int supPred = uniform(P.supPred);
     const bool fs = pairs && fuse && supPred != -1;
     const int32_t* supList = (fs && supPred <= -2) ? supTab + (-supPred - 2) : nullptr;
+    // single-frame counts apply to this plan (a plain-EXT pair level that is the last level)
+    const bool shortOk = pairs && pl == nlev - 1 && leaf_plain(P, nlev - 1, pairs);
     const uint64_t below = (1ull << lane) - 1ull;
     int v[NS];
     RegSet set;
     set.n = 0;
     int d = 0, budget = 0;
     bool active = false;
+    bool noCut = false;   // this seed's head rows are too long for a single-frame count
     uint32_t item = 0, next = 0;
     uint64_t mine = 0;
     for (;;) {
@@ -485,6 +546,7 @@ __device__ __forceinline__ uint64_t thread_chunk(ThreadLdsT<levels_of(NS)>& W, c
                     // budget is sure to escalate, so it does at its first step
                     budget = (pairs && f0.end - f0.i > budget0) ? 0 : budget0;
                     set.n = 0;
+                    noCut = false;
                 }
             }
             next += (uint32_t)__popcll(want);
@@ -494,6 +556,37 @@ __device__ __forceinline__ uint64_t thread_chunk(ThreadLdsT<levels_of(NS)>& W, c
             continue;
         }
         bool heavyNow = false;
+        if (active && shortOk && !noCut && d + 1 == nlev && set.n == 0) {
+            const Frame fr = W.frame[d][lane];
+            if (fr.end - fr.i >= kShortMin && frames_above_done(W, d)) {
+                // the seed's only row left, nothing counted: count it without enumerating
+                const int nb = level_nbound(P.lev[d].flags);
+                bool cut = true;
+                if (fs) {   // the head rows must be short enough to search them
+                    const int nm = supList ? supList[0] : 1;
+                    for (int m = 0; m < nm && cut; ++m) {
+                        const int pred = supList ? supList[2 + 2 * m] : supPred;
+                        const Frame hr = key_row(G, (int64_t)pred * G.nBlocks, v[0]);
+                        cut = hr.end - hr.i <= kShortHead;
+                    }
+                    noCut = !cut;
+                }
+                if (cut) {
+                    mine += (uint64_t)frame_count(G, fr, v, nb);
+                    if (fs) {
+                        const int nm = supList ? supList[0] : 1;
+                        for (int m = 0; m < nm; ++m) {
+                            const int pred = supList ? supList[2 + 2 * m] : supPred;
+                            const int h = frame_head_hits(G, (int64_t)pred * G.nBlocks, v[0], fr, v, nb);
+                            if (h <= 0) continue;
+                            if (!supList) sup += (uint64_t)h;
+                            else atomicAdd(counts + 2 * supList[1 + 2 * m] + kSupport, (unsigned long long)h);
+                        }
+                    }
+                    active = false;
+                }
+            }
+        }
         if (active) {
             const Frame fr = W.frame[d][lane];
             if (fr.i >= fr.end) {
@@ -733,6 +826,35 @@ __device__ __forceinline__ uint64_t head_hits_wave(const Dedup<D, NS>& dd, const
     return wave_sum(n);
 }
 
+// LDS bitmap (the wave's own): marks ranks r[k] (>= 0; -1: none). The bit
+// tests are read together, unconditionally (a slot without a rank reads
+// word 0: lanes reading one address share a broadcast, and its bit is 0);
+// then an atomic goes out for each bit found clear, all before any returned
+// word is used, so a batch costs two LDS round trips however many marks it
+// makes (no atomic for slots without a rank or for bits already set). A
+// mark whose bit the returned word still lacked was set first by this lane.
+template <int D, int U, int NS>
+__device__ __forceinline__ uint32_t mark_lds(Dedup<D, NS>& dd, const int (&r)[U]) {
+    uint32_t bit[U], w[U], old[U];
+    int idx[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+        const bool in = r[k] >= 0;
+        bit[k] = in ? 1u << (r[k] & 31) : 0u;
+        idx[k] = in ? r[k] >> 5 : 0;
+        w[k] = dd.bits[idx[k]];
+    }
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+        old[k] = ~0u;
+        if (bit[k] & ~w[k]) old[k] = atomicOr(dd.bits + idx[k], bit[k]);
+    }
+    uint32_t n = 0;
+#pragma unroll
+    for (int k = 0; k < U; ++k) n += (bit[k] & ~old[k]) ? 1u : 0u;
+    return n;
+}
+
 // Marks the head objects of a batch of last-level groundings (ho[k] >= 0;
 // -1: none), with their ranks hr[k] (bitmap paths). The word tests are
 // issued together, then the marks together, so a batch costs two memory
@@ -756,6 +878,11 @@ __device__ __forceinline__ uint32_t mark_batch(Dedup<D, NS>& dd, const int (&ho)
     uint32_t w[U];
 #pragma unroll
     for (int k = 0; k < U; ++k) r[k] = ho[k] >= 0 ? hr[k] : -1;
+    if (D == kDedupLds) {
+#pragma unroll
+        for (int k = 0; k < U; ++k) emit[k] = -1;
+        return mark_lds<D, U, NS>(dd, r);
+    }
 #pragma unroll
     for (int k = 0; k < U; ++k) w[k] = r[k] >= 0 ? dd.word_r(r[k]) : 0xffffffffu;
 #pragma unroll
@@ -780,9 +907,10 @@ __device__ __forceinline__ void emit_keys(const SortSink& sink, unsigned long lo
     }
 }
 
-// Bitmap paths: marks ranks hr[k] (>= 0; -1: none), tests first, as mark_batch.
+// Bitmap paths: marks ranks hr[k] (>= 0; -1: none); HBM rows test first, as mark_batch.
 template <int D, int U, int NS>
 __device__ __forceinline__ uint32_t mark_ranks(Dedup<D, NS>& dd, const int (&hr)[U]) {
+    if (D == kDedupLds) return mark_lds<D, U, NS>(dd, hr);
     uint32_t w[U];
 #pragma unroll
     for (int k = 0; k < U; ++k) w[k] = hr[k] >= 0 ? dd.word_r(hr[k]) : 0xffffffffu;
@@ -854,14 +982,18 @@ __device__ __forceinline__ uint64_t wave_run(WaveLdsT<levels_of(NS)>& W, const D
     uint64_t mine = 0;
     int steps = 0;
     uint32_t nDonate = 0, nLeafDraw = 0, nDraw = 0;
+    uint64_t busyLanes = 0, drawLanes = 0;
     for (;;) {
         // ---- A: idle lanes draw from the pool
+        unsigned long long tp = GR_PROF_NOW();
+        int profA = 3;
         const uint64_t idleM = __ballot(!busy);
         const int pn = uniform(W.poolNext), pe = uniform(W.poolEnd), plv = uniform(W.poolLevel);
         const uint32_t pg = (uint32_t)uniform((int)W.poolGen);
         if (idleM != 0 && pn < pe) {
             const int n = __popcll(idleM);
             const int r = __popcll(idleM & below);
+            drawLanes += (uint64_t)(pe - pn < n ? pe - pn : n);
             int next = pe;
             bool drop = false;
             int hru = -1;              // rank of a head object bound above the pool
@@ -879,6 +1011,7 @@ __device__ __forceinline__ uint64_t wave_run(WaveLdsT<levels_of(NS)>& W, const D
                 }
                 if (plv == last && leaf_fast_ok<D>(P, last, pairs)) {
                     ++nLeafDraw;
+                    profA = 2;
                     next = pn + n * kLeafUnroll < pe ? pn + n * kLeafUnroll : pe;
                     int is[kLeafUnroll], xs[kLeafUnroll], rs[kLeafUnroll];
                     bool in[kLeafUnroll];
@@ -892,6 +1025,7 @@ __device__ __forceinline__ uint64_t wave_run(WaveLdsT<levels_of(NS)>& W, const D
                     else mine += mark_ranks<D, kLeafUnroll>(dd, rs);
                 } else if (plv == last) {
                     ++nLeafDraw;
+                    profA = 2;
                     next = pn + n * kLeafGeneric < pe ? pn + n * kLeafGeneric : pe;
                     const bool byEntry = pairs && pl == last && D != kDedupSort && hoByEntry;
                     int xs[kLeafGeneric], rs[kLeafGeneric];
@@ -960,6 +1094,8 @@ __device__ __forceinline__ uint64_t wave_run(WaveLdsT<levels_of(NS)>& W, const D
             wave_sync();
             if (lane == 0) W.poolNext = next;
         }
+        GR_PROF_ADD(profA, tp);
+        tp = GR_PROF_NOW();
         if (!pairs && __any(found)) break;
         // ---- B: busy lanes advance their own search by one step
         int emit = -1;
@@ -1068,6 +1204,8 @@ __device__ __forceinline__ uint64_t wave_run(WaveLdsT<levels_of(NS)>& W, const D
 #pragma unroll
             for (int k = 0; k < kLeafGeneric; ++k) emit_keys(sink, keyHi, emits[k]);
         }
+        GR_PROF_ADD(4, tp);
+        tp = GR_PROF_NOW();
         if (!pairs && __any(found)) break;
         // ---- C: done, hand the seed over as a giant, or donate work to the idle lanes
         wave_sync();
@@ -1075,6 +1213,7 @@ __device__ __forceinline__ uint64_t wave_run(WaveLdsT<levels_of(NS)>& W, const D
         const bool poolEmpty = uniform(W.poolNext) >= uniform(W.poolEnd);
         if (busyM == 0 && poolEmpty) break;
         ++steps;
+        busyLanes += (uint64_t)__popcll(busyM);
         if (mayGiant && steps >= ga.after) {
             mayGiant = false;   // one try per seed
             uint32_t slot = 0xffffffffu;
@@ -1132,12 +1271,16 @@ __device__ __forceinline__ uint64_t wave_run(WaveLdsT<levels_of(NS)>& W, const D
                 wave_sync();
             }
         }
+        GR_PROF_ADD(5, tp);
     }
+    const unsigned long long tEnd = GR_PROF_NOW();
     stats[0] += 1;
     stats[1] += (unsigned long long)steps;
     stats[2] += nDonate;
     stats[3] += nLeafDraw;
     stats[4] += nDraw;
+    stats[5] += busyLanes;
+    stats[6] += drawLanes;
     if (!pairs) {
         if (aborted || stopped || !__any(found)) return 0;
         if (!foundFlag) return 1;
@@ -1180,6 +1323,7 @@ __device__ __forceinline__ uint64_t wave_run(WaveLdsT<levels_of(NS)>& W, const D
         }
     }
     const uint64_t total = wave_sum(mine);
+    GR_PROF_ADD(6, tEnd);
     return aborted ? 0 : total;
 }
 
@@ -1193,6 +1337,7 @@ __device__ __forceinline__ uint64_t wave_seed(WaveLdsT<levels_of(NS)>& W, const 
                                               unsigned long long (&stats)[kWaveStats]) {
     const int lane = lane_id();
     const Plan& P = W.plan;
+    const unsigned long long tp = GR_PROF_NOW();
     int v[NS];
     if (!bind_seed(P, G, (int)item, v)) return 0;
     const Frame f0 = enter(W, P, 0, G, v);
@@ -1213,6 +1358,7 @@ __device__ __forceinline__ uint64_t wave_seed(WaveLdsT<levels_of(NS)>& W, const 
     const bool supOn = K == kPairs && D != kDedupSort && fuse && P.supPred != -1;
     const int32_t* supList = (supOn && P.supPred <= -2) ? supTab + (-P.supPred - 2) : nullptr;
     const Frame supRow = (supOn && !supList) ? key_row(G, sup_base(P, G), v[0]) : Frame{0, 0};
+    GR_PROF_ADD(1, tp);
     return wave_run<K, D, NS>(W, G, R, dd, rowWords, sink, keyHi, ga, ga.after > 0, HeavyItem{planIdx, item}, heavyIdx,
                           nullptr, aborted, supOn, supRow, supList, uniform(v[0]), counts, sup, stats);
 }
@@ -1314,8 +1460,9 @@ __global__ void __launch_bounds__(kWave) k_wave_batch(WaveArgs a, DevGraph G) {
     if (n == 0) return;   // nothing escalated: no cursor traffic
     Dedup<D, NS> dd = wave_dedup<D, NS>(W, cache, dyn, a, G);
     uint32_t loaded = 0xffffffffu;
-    unsigned long long stats[kWaveStats] = {0, 0, 0, 0, 0};
+    unsigned long long stats[kWaveStats] = {};
     for (;;) {
+        const unsigned long long tp = GR_PROF_NOW();
         const uint32_t h = wave_take(a.cursor);
         if (h >= n) break;
         const HeavyItem it = a.heavy[h];
@@ -1323,6 +1470,7 @@ __global__ void __launch_bounds__(kWave) k_wave_batch(WaveArgs a, DevGraph G) {
             copy_plan(W, a.plans + it.plan);
             loaded = it.plan;
         }
+        GR_PROF_ADD(0, tp);
         uint64_t sup = 0;
         const uint64_t c = wave_seed<K, D, NS>(W, G, a.refs + it.plan, it.item, dd, a.rowWords, a.sink,
                                            (unsigned long long)h << a.sink.hoBits, a.giant, it.plan, h, a.fuse != 0,
@@ -1344,7 +1492,7 @@ __global__ void __launch_bounds__(kWave) k_wave_one(Plan P, WaveArgs a, DevGraph
     if (n == 0) return;
     copy_plan(W, &P);
     Dedup<D, NS> dd = wave_dedup<D, NS>(W, cache, dyn, a, G);
-    unsigned long long stats[kWaveStats] = {0, 0, 0, 0, 0};
+    unsigned long long stats[kWaveStats] = {};
     for (;;) {
         const uint32_t h = wave_take(a.cursor);
         if (h >= n) break;
@@ -1470,8 +1618,9 @@ __global__ void __launch_bounds__(kWave) k_wave_sub(WaveArgs a, DevGraph G) {
     if (n == 0) return;   // no giants listed
     Dedup<D, NS> dd = wave_dedup<D, NS>(W, cache, dyn, a, G);
     uint32_t loaded = 0xffffffffu;
-    unsigned long long stats[kWaveStats] = {0, 0, 0, 0, 0};
+    unsigned long long stats[kWaveStats] = {};
     for (;;) {
+        const unsigned long long tsub = GR_PROF_NOW();
         const uint32_t si = wave_take(ga.subCursor);
         if (si >= n) break;
         const SubItem& S = ga.subs[si];
@@ -1535,6 +1684,7 @@ __global__ void __launch_bounds__(kWave) k_wave_sub(WaveArgs a, DevGraph G) {
             }
             for (int64_t w = lane; w < a.rowWords; w += kWave) row[w] = 0u;   // read again only by later launches
         }
+        GR_PROF_ADD(7, tsub);
     }
     flush_stats(ga, stats);
 }
@@ -1607,6 +1757,15 @@ size_t SlotKernels<NS>::lds_static() {
     if (hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k_wave_batch<kPairs, kDedupLds, NS>)) != hipSuccess)
         return sizeof(WaveLdsT<levels_of(NS)>) + kCacheSlots * sizeof(int);
     return fa.sharedSizeBytes;
+}
+
+template <int NS>
+int SlotKernels<NS>::lds_blocks_per_cu(size_t dynLds) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &n, reinterpret_cast<const void*>(&k_wave_batch<kPairs, kDedupLds, NS>), kWave, dynLds) != hipSuccess)
+        return 0;
+    return n;
 }
 
 template <int NS>

@@ -12,6 +12,13 @@
 //   ISECT             an EXT candidate, plus a second row lookup per level
 //                     entered and 4 B per row-2 probe of the forward cursor
 //                     (gallop + binary search); the CHECK after it costs nothing
+//   single frame      a PAIRS seed whose search reaches a plain-EXT last pair
+//                     level with nothing counted and no untried entry above,
+//                     on a row of at least 16 entries, counts that row
+//                     without enumerating it (as the thread-mode kernel
+//                     does): one binary search per bound variable; with
+//                     fused support the head row lookups first, then the
+//                     head-row entries (4 B each) and a binary search each
 //   fused support     PAIRS plans with supPred >= 0, per seed with a body pair:
 //                     the seed's head row lookup (as a row lookup), then, as the device
 //                     checks it: up to 8 distinct head objects (a thread-mode
@@ -46,6 +53,9 @@ struct Walker {
     std::vector<uint64_t>& bits;
     std::vector<int64_t>& touched;
     const std::vector<int32_t>* heads = nullptr;   // fused support of several head predicates
+    bool shortOk = false;    // the plan's last level is a plain-EXT pair level (single-frame counts apply)
+    bool counted = false;    // the current seed was counted by a single-frame count (its support too)
+    static constexpr int64_t kShortMin = 16, kShortHead = 16;   // as the thread-mode kernel
 
     bool seen(int x) const { return (bits[x >> 6] >> (x & 63)) & 1u; }
     void insert(int x) {
@@ -177,11 +187,55 @@ struct Walker {
         return true;
     }
 
+    // Binary search of x in [lo, hi) as the device's find_in_row reads it.
+    bool search(int64_t lo, int64_t hi, int x) {
+        int64_t l = lo, h = hi;
+        while (l < h) {
+            const int64_t m = (l + h) >> 1;
+            bytes += 4;
+            if (col(m) < x) l = m + 1; else h = m;
+        }
+        if (l < hi) bytes += 4;
+        return l < hi && col(l) == x;
+    }
+
+    // Single-frame count of the seed's last row [lo, hi) (see the header);
+    // false if a head row is too long, when the row is enumerated instead.
+    bool single_frame(int L, int64_t lo, int64_t hi) {
+        const int nb = level_nbound(P.lev[L].flags);
+        std::vector<int32_t> preds;
+        if (heads) preds = *heads;
+        else if (P.supPred >= 0) preds.push_back(P.supPred);
+        for (int32_t q : preds) {
+            int64_t a, b;
+            key_row(((int64_t)kSPO * g.P() + q) * g.nBlocks(), v[0], a, b);
+            if (b - a > kShortHead) return false;
+        }
+        for (int j = 0; j < nb; ++j) search(lo, hi, v[j]);
+        for (int32_t q : preds) {
+            int64_t a, b;
+            key_row(((int64_t)kSPO * g.P() + q) * g.nBlocks(), v[0], a, b);
+            for (int64_t x = a; x < b; ++x) {
+                bytes += 4;
+                const int h = col(x);
+                bool t = false;
+                for (int j = 0; j < nb; ++j) t = t || v[j] == h;
+                if (!t) search(lo, hi, h);
+            }
+        }
+        counted = true;
+        return true;
+    }
+
     // Returns true when a complete grounding was found (existence mode).
-    bool dfs(int L, bool pairs) {
+    // `only`: no untried entry is left in the frames above this level.
+    bool dfs(int L, bool pairs, bool only) {
         ++nodes;
         int64_t lo, hi;
         range(L, lo, hi);
+        if (pairs && shortOk && L + 1 == P.nlev && only && touched.empty() && hi - lo >= kShortMin &&
+            single_frame(L, lo, hi))
+            return false;
         const bool isect = level_kind(P.lev[L].flags) == kIsect;
         for (int64_t i = lo; i < hi; ++i) {
             if (!bind(L, i)) continue;
@@ -189,14 +243,15 @@ struct Walker {
             if (pairs && L == P.pairLevel) {
                 const int ho = tv(P.hoRef);
                 if (seen(ho)) continue;
-                if (L + 1 == P.nlev || dfs(L + 1, false)) insert(ho);
+                if (L + 1 == P.nlev || dfs(L + 1, false, false)) insert(ho);
                 continue;
             }
             if (L + 1 == P.nlev) {
                 if (!pairs) return true;
                 continue;
             }
-            if (dfs(L + 1, pairs) && !pairs) return true;
+            if (dfs(L + 1, pairs, only && i + 1 == hi) && !pairs) return true;
+            if (counted) return false;
         }
         return false;
     }
@@ -214,14 +269,18 @@ struct Walker {
             if (P.hsRef >= 0 && P.hoRef >= 0 && ((P.hsRef == P.hoRef) != (s == o))) return;
             if (P.hsRef >= 0) v[P.hsRef] = s;
             if (P.hoRef >= 0) v[P.hoRef] = o;
-            dfs(0, false);
+            dfs(0, false, false);
             return;
         }
         bytes += 4;
         const int64_t ks = P.seedBegin + item;
         v[0] = ks < (int64_t)g.keysS.size() ? g.keysS[ks] : g.keysO[ks - g.keysS.size()];
-        dfs(0, P.pairLevel >= 0);
-        if (P.pairLevel >= 0) {
+        const int last = P.nlev - 1;
+        shortOk = P.pairLevel >= 0 && P.pairLevel == last && level_kind(P.lev[last].flags) == kExt &&
+                  level_conflicts(P.lev[last].flags) == 0;
+        counted = false;
+        dfs(0, P.pairLevel >= 0, true);
+        if (P.pairLevel >= 0 && !counted) {
             if (heads)
                 for (int32_t q : *heads) head_hits(q);
             else if (P.supPred >= 0)

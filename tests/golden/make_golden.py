@@ -126,17 +126,17 @@ def run_parallel(d, ttl, rules, workers, chunk, cpu01_timeout, rw_timeout, rw_me
             cpu01[k:k + len(rows)] = rows
             t_cpu01 += dt
             print("  cpu01 rules %d..%d %.1fs" % (k, k + len(rows) - 1, dt), flush=True)
-    rewritten = [None] * len(rules)
+    rw_rows = [None] * len(rules)
     skipped = {}
     t_rw = 0.0
     with ThreadPoolExecutor(workers) as ex:
         for i, row, dt, why in ex.map(one_rw, range(len(rules))):
-            rewritten[i] = row
+            rw_rows[i] = row
             t_rw += dt
             if why:
                 skipped[str(i)] = why
             print("  rewritten rule %d %.1fs %s" % (i, dt, why or row), flush=True)
-    return cpu01, rewritten, skipped, t_cpu01, t_rw
+    return cpu01, rw_rows, skipped, t_cpu01, t_rw
 
 
 def make_fixture(name, graph, rules, generator=None, inline_triples=False, timeout=1800, parallel=None):
@@ -263,12 +263,12 @@ C5_FIRST = 400
 
 def fx_c5():
     """The first requests of the serving stream (1-8 atoms, no work filter)
-    over the high-fan-out graph; CPU_01 only (Rewritten's Turtle load alone
-    takes ~50 s per process on this graph, and it runs rule by rule here)."""
+    over the high-fan-out graph (about an hour: Rewritten loads the Turtle
+    file once per rule, ~30 s each on this graph)."""
     w = W.serving_stream(n_first=C5_FIRST)
     make_fixture("serving_stream_400", w.graph, w.rules,
                  {"fn": "serving_stream", "kwargs": {"n_first": C5_FIRST}},
-                 parallel=dict(BIG, chunk=20, workers=4, rewritten=False))
+                 parallel=dict(BIG, chunk=20, workers=4))
 
 
 FRONTEND_CASES = [

@@ -127,6 +127,39 @@ def test_random_rules_vs_oracle(seed):
         eng.close()
 
 
+@pytest.mark.parametrize("seed", range(4))
+def test_single_frame_counts_vs_oracle(seed):
+    """Rows of 16 entries and more on a dense graph with self-loops, so
+    seeds whose last row is their only frame are counted without
+    enumeration, and those rows hold the bound values the count must leave
+    out (the seed itself, the intermediate variables). Rules of 1-3 atoms,
+    some with constants, fused support on and off, every bitmap dedup mode,
+    the batched and the per-rule path, against the oracle."""
+    say("single-frame", seed)
+    g = W.tiny_random(7100 + seed, n_entities=30 + seed, n_predicates=2, n_triples=1100 + 40 * seed)
+    rng = np.random.default_rng(70 + seed)
+    rules = [W.random_rule(rng, g.n_entities, g.n_predicates, 1 + k % 3, n_vars=2 + k % 3,
+                           p_const=0.1 if k % 4 == 3 else 0.0, closed=True, reflexive_head=(k % 17 == 5))
+             for k in range(90)]
+    exp = Oracle(g.s, g.p, g.o).eval(W.encode_rules(rules))
+    eng = L.Engine.from_graph(g)
+    try:
+        words = L.rule_words_for_graph(rules, g)
+        for dedup, fuse, flags in ((0, 1, L.UNGATED), (1, 1, L.UNGATED), (3, 0, L.UNGATED), (0, 1, 0),
+                                   (0, 1, L.UNGATED | L.PER_RULE)):
+            eng.set_tuning("dedup", dedup)
+            eng.set_tuning("fuse_support", fuse)
+            res = eng.evaluate(words, flags)
+            for i in range(len(rules)):
+                tag = (seed, dedup, fuse, flags, i, W.rule_text(rules[i]))
+                assert res[i]["support"] == exp[i][0], tag
+                assert res[i]["bodySize"] == exp[i][4], tag
+                if flags & L.UNGATED:
+                    assert res[i]["bodyPairs"] == exp[i][3], tag
+    finally:
+        eng.close()
+
+
 @pytest.mark.parametrize("name", ["tiny_0", "tiny_3", "small_paths_2k"])
 def test_text_abi(name):
     """The reference entry points: TTL file in, rule lines and JSON objects

@@ -66,7 +66,7 @@ def test_fixtures_exercise_the_divergences():
     assert n_const_diff >= 1
 
 
-@pytest.mark.parametrize("name", goldens.HIGH_FANOUT + ("dense_conf_full",))
+@pytest.mark.parametrize("name", [n for n in goldens.HIGH_FANOUT + ("dense_conf_full",) if n in goldens.names()])
 def test_reference_programs_agree_at_full_size(name):
     """On the full-size fixtures the GPU is checked against CPU_01's values
     directly; where Rewritten_CPP also finished, the two programs must agree
