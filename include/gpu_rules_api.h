@@ -296,6 +296,11 @@ int gpu_host_graph_info(void* handle, int64_t* nTerms, int32_t* nPredicates, int
 /* Term id of a term string as the Turtle reader stores it (IRIs without
  * brackets), or -1. */
 int64_t gpu_host_term_id(void* handle, const char* term);
+/* The string of term `id` as the Turtle reader stores it (IRIs without
+ * brackets; constants a parsed rule introduced included), written
+ * NUL-terminated into buf (at most cap bytes, cap >= 1). Returns its full
+ * length, or -1 for an unknown id or a bad argument. */
+int64_t gpu_host_term_string(void* handle, int64_t id, char* buf, int64_t cap);
 /* Parses a rule line (isJson = 0) or JSON object into GPU_RULE_WORDS words.
  * Unknown constants get ids >= the graph's term count. 0 or -1. */
 int gpu_host_parse_rule(void* handle, const char* rule, int32_t isJson, int32_t* words);

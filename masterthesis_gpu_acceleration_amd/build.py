@@ -23,6 +23,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 OBJ = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libgpu_rules.so")
+CLI = os.path.join(PKG, "gpu_rules_counter")   # command-line counter over the library (CPU_01's output)
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("GPU_RULES_ARCH", "gfx950")
 
@@ -69,7 +70,25 @@ def build(verbose: bool = True, profile: bool = False) -> str:
     os.replace(tmp, lib)
     if verbose:
         print("built", os.path.relpath(lib, ROOT))
+    if not profile:
+        build_cli(verbose)
     return lib
+
+
+def build_cli(verbose: bool = True) -> str:
+    """gpu_rules_counter: host C++ over the C ABI, linked to the library next to it."""
+    src = os.path.join(CSRC, "cli", "gpu_rules_counter.cpp")
+    tmp = CLI + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(ROOT, "include"), src, "-o", tmp, "-L" + PKG,
+           "-lgpu_rules", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("gpu_rules_counter build failed")
+    os.replace(tmp, CLI)
+    if verbose:
+        print("built", os.path.relpath(CLI, ROOT))
+    return CLI
 
 
 if __name__ == "__main__":

@@ -1570,6 +1570,17 @@ int64_t gpu_host_term_id(void* handle, const char* term) {
     return g->terms.find(term);
 }
 
+int64_t gpu_host_term_string(void* handle, int64_t id, char* buf, int64_t cap) {
+    TermResolver* tr = as_resolver(handle);
+    if (!tr || !buf || cap < 1 || id < 0 || id > INT32_MAX) return -1;
+    const std::string* s = tr->name((int32_t)id);
+    if (!s) return -1;
+    const size_t k = std::min<size_t>(s->size(), (size_t)cap - 1);
+    std::memcpy(buf, s->data(), k);
+    buf[k] = '\0';
+    return (int64_t)s->size();
+}
+
 int gpu_host_parse_rule(void* handle, const char* rule, int32_t isJson, int32_t* words) {
     TermResolver* tr = as_resolver(handle);
     if (!tr || !rule || !words) return -1;

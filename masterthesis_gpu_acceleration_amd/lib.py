@@ -86,6 +86,7 @@ _SIGS = {
     "gpu_host_parse_rule": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int32, ctypes.c_void_p]),
     "gpu_host_plan_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "gpu_host_count_action": (ctypes.c_int32, [ctypes.c_int32, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64]),
+    "gpu_host_term_string": (ctypes.c_int64, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_char_p, ctypes.c_int64]),
     "gpu_rules_version": (ctypes.c_char_p, []),
 }
 

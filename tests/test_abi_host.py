@@ -225,3 +225,18 @@ def test_no_gpu_context_fails_loudly():
     g = W.tiny_random(1)
     with pytest.raises(RuntimeError):
         L.Engine.from_graph(g)
+
+
+def test_counter_cli_usage_and_missing_graph():
+    """gpu_rules_counter (the CPU_01 wire-format counter over the library)
+    exits 1 with a message on missing arguments or an unreadable graph, as
+    the reference program ends on an exception."""
+    cli = os.path.join(ROOT, "masterthesis_gpu_acceleration_amd", "gpu_rules_counter")
+    assert os.path.exists(cli)
+    r = subprocess.run([cli], capture_output=True, text=True)
+    assert r.returncode == 1 and "usage" in r.stderr
+    with tempfile.TemporaryDirectory() as d:
+        rf = os.path.join(d, "r.txt")
+        open(rf, "w").write("( ?a <https://ex.org/p/0> ?b ) => ( ?a <https://ex.org/p/1> ?b )\n")
+        r = subprocess.run([cli, os.path.join(d, "missing.ttl"), rf, "--json"], capture_output=True, text=True)
+        assert r.returncode == 1 and r.stdout == ""
