@@ -240,3 +240,55 @@ def test_counter_cli_usage_and_missing_graph():
         open(rf, "w").write("( ?a <https://ex.org/p/0> ?b ) => ( ?a <https://ex.org/p/1> ?b )\n")
         r = subprocess.run([cli, os.path.join(d, "missing.ttl"), rf, "--json"], capture_output=True, text=True)
         assert r.returncode == 1 and r.stdout == ""
+
+
+SCALA = os.path.join(ROOT, "masterthesis_gpu_acceleration_amd", "bindings", "scala", "GpuRulesBatchLib.scala")
+
+
+def c_prototypes():
+    """name -> list of C parameter declarations, from include/gpu_rules_api.h."""
+    src = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    out = {}
+    for m in re.finditer(r"\b(gpu_\w+)\s*\(([^)]*)\)\s*;", src):
+        params = [p.strip() for p in m.group(2).split(",") if p.strip() and p.strip() != "void"]
+        out[m.group(1)] = params
+    return out
+
+
+def test_scala_binding_matches_header():
+    """The Scala JNA binding of the additive entry points (not compiled here:
+    no JVM) declares functions the header declares, with as many arguments,
+    pointer arguments where the C parameter is a pointer and Int/Long where
+    it is a 32/64-bit integer; its structures list the C fields in C order,
+    as the ctypes binding does (whose sizes and offsets are pinned above)."""
+    text = open(SCALA).read()
+    protos = c_prototypes()
+    decls = re.findall(r"def (gpu_\w+)\(([^)]*)\):\s*(\w+)", text, flags=re.S)
+    assert len(decls) >= 15
+    for name, args, ret in decls:
+        assert name in protos, name
+        params = protos[name]
+        sargs = [a.strip() for a in args.replace("\n", " ").split(",") if a.strip()]
+        assert len(sargs) == len(params), (name, sargs, params)
+        for sa, cp in zip(sargs, params):
+            stype = sa.split(":", 1)[1].strip()
+            if "*" in cp:
+                assert stype in ("Pointer", "String", "GpuRuleResultJ", "GpuRuleResult64", "GpuServeCompletion",
+                                 "DoubleByReference", "LongByReference", "IntByReference", "Array[String]",
+                                 "Array[Int]", "Array[Long]"), (name, sa, cp)
+            elif cp.startswith("int64_t"):
+                assert stype == "Long", (name, sa, cp)
+            elif cp.startswith("int32_t") or cp.startswith("int "):
+                assert stype == "Int", (name, sa, cp)
+    def fields(cls):
+        m = re.search(r"class %s extends Structure \{(.*?)override def getFieldOrder.*?asList\((.*?)\)" % cls, text,
+                      flags=re.S)
+        assert m, cls
+        declared = re.findall(r"var (\w+):", m.group(1))
+        order = re.findall(r'"(\w+)"', m.group(2))
+        assert declared == order, cls
+        return order
+    assert fields("GpuRuleResultJ") == [f for f, _ in L.GpuRuleResult._fields_]
+    assert fields("GpuRuleResult64") == [f for f, _ in L.GpuRuleResult64._fields_]
+    from masterthesis_gpu_acceleration_amd import serving as S
+    assert fields("GpuServeCompletion") == list(S.COMPLETION_DTYPE.names)
