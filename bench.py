@@ -29,8 +29,11 @@ Every block also reports:
   * roofline: algorithmic bytes of the dominant phase (host traffic model of
     the same plans) over the event-timed duration of the kernels that run
     that phase (thread-mode kernel + wave-mode kernel with its sub-item
-    pass), against 8 TB/s; counters (traffic, L2 hit) from the committed
-    rocprofv3 summary, with the tuning it was taken at;
+    pass), against 8 TB/s; counters (traffic, L2 hit) and the traced phase
+    time from the committed rocprofv3 summary (profiles/pmc_traffic.json);
+  * tuning: each configuration runs at the tuning its committed profile
+    was taken at (the autotuner's choice on MI355X), so the live figures and
+    the profile describe the same launches; --autotune re-runs the tuner;
   * cpu_baseline: the reference's CPU_01_nonparallel program (built from the
     reference sources into oracle/_ref/, single-threaded), sharded over the
     host cores -- the rate on all cores and on one core.
@@ -221,41 +224,65 @@ def cpu_baseline(wl, sample):
 
 # ------------------------------------------------------------ one configuration
 
-def pmc_traffic(workload, tuning):
-    """HBM bytes per dispatch of the dominant phase's kernels from the
-    committed counter summary (rocprofv3 FETCH_SIZE and WRITE_SIZE runs,
-    scaled by the copy yardstick), and whether it was taken at this tuning."""
-    p = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+PROFILE_TABLE = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+
+
+def profile_entry(workload):
+    """The committed profile of a configuration (rocprofv3 kernel trace and
+    counter runs, summarised into profiles/pmc_traffic.json), or None."""
     try:
-        d = json.load(open(p))[workload]
-        d["phase_hbm_bytes"]
-    except (OSError, KeyError, ValueError):
+        d = json.load(open(PROFILE_TABLE))[workload]
+        d["phase_hbm_bytes"], d["tuning"]
+    except (OSError, KeyError, ValueError, TypeError):
         return None
-    prof_t = d.get("tuning", {})
-    same = all(prof_t.get(k) == tuning.get(k) for k in prof_t) if prof_t else False
+    return d
+
+
+def pmc_traffic(workload, tuning):
+    """HBM bytes per pass of the dominant phase's kernels from the committed
+    counter summary (FETCH_SIZE and WRITE_SIZE runs, scaled by the copy
+    yardstick), the L2 hit rate, the phase's kernel time in the profiled
+    run, and whether the profile was taken at this tuning."""
+    d = profile_entry(workload)
+    if d is None:
+        return None
+    prof_t = d["tuning"]
+    same = all(prof_t.get(k) == tuning.get(k) for k in TUNE_SET)
     return {"traffic": int(d["phase_hbm_bytes"]), "l2_hit": d["l2_hit"], "traffic_source": d["source"],
-            "traffic_ms": d.get("phase_ms"), "traffic_tuning": prof_t, "traffic_tuning_matches": same}
+            "profile_phase_ms": d.get("phase_ms"), "profile_frac": d.get("frac"),
+            "profile_kernels": d.get("phase_kernels"), "traffic_tuning": prof_t, "traffic_tuning_matches": same}
 
 
-TUNE_KEYS = ("budget", "thread_waves_per_cu", "wave_waves_per_cu", "pairs_dedup", "giant_after", "chunks_per_wave")
+# settable tuning keys, in the order --tune takes them
+TUNE_SET = ("budget", "thread_waves_per_cu", "wave_waves_per_cu", "dedup", "giant_after", "chunks_per_wave")
 
 
-def tune_engine(eng, words, flags, args):
+def tune_engine(eng, name, words, flags, args):
+    """Fixed tuning (--tune), the autotuner (--autotune, or when no profile of
+    this configuration is committed), else the tuning the committed profile
+    was taken at (the autotuner's choice on MI355X). Returns the source."""
+    prof = profile_entry(name)
     if args.tune:
         knobs = [int(x) for x in args.tune.split(",")]
-        for key, val in zip(("budget", "thread_waves_per_cu", "wave_waves_per_cu", "dedup", "giant_after",
-                             "chunks_per_wave"), knobs):
+        for key, val in zip(TUNE_SET, knobs):
             eng.set_tuning(key, val)
         log("fixed tuning: %s" % args.tune)
-        return
+        return "--tune " + args.tune
     if args.no_autotune:
-        return
+        return "library defaults"
+    if prof is not None and not args.autotune:
+        for key in TUNE_SET:
+            eng.set_tuning(key, int(prof["tuning"][key]))
+        log("tuning of the committed profile: %s" % prof["tuning"])
+        return "profiles/pmc_traffic.json (the autotuner's choice when the profile was taken)"
     t = time.time()
     tl = eng.autotune(words, flags)
     best = tl[np.argmin(tl[:, 3])]
     log("autotune (%.1fs, %d trials): budget %d, thread waves/CU %d, wave waves/CU %d -> %.3f ms; chunks/wave %d, "
-        "giant_after %d" % (time.time() - t, len(tl), best[0], best[1], best[2], best[3],
-                            eng.get_tuning("chunks_per_wave"), eng.get_tuning("giant_after")))
+        "giant_after %d, dedup %d" % (time.time() - t, len(tl), best[0], best[1], best[2], best[3],
+                                      eng.get_tuning("chunks_per_wave"), eng.get_tuning("giant_after"),
+                                      eng.get_tuning("dedup")))
+    return "autotuner (%d trials, %.1f s)" % (len(tl), time.time() - t)
 
 
 def time_batch(eng, words, flags, steps, warmup):
@@ -296,8 +323,8 @@ def measure(args, name, wl, decompose):
         for kv in args.set:
             key, val = kv.split("=", 1)
             eng.set_tuning(key, int(val))
-        tune_engine(eng, words, flags, args)
-        tuning = {k: eng.get_tuning(k) for k in TUNE_KEYS}
+        tuning_source = tune_engine(eng, name + ("_gated" if args.gated else ""), words, flags, args)
+        tuning = {k: eng.get_tuning(k) for k in TUNE_SET}
         t = time.time()
         ms_step, dev_ms, res = time_batch(eng, words, flags, args.steps, args.warmup)
         log("%s: timed %d steps: %.3f ms/step (device %.3f ms)" % (name, args.steps, ms_step, float(np.mean(dev_ms))))
@@ -325,7 +352,8 @@ def measure(args, name, wl, decompose):
             "escalated_seeds": int(eng.last_stats()["heavy_seeds"]),
             "searches": searches,
             "wave_mode": wave,
-            "tuning": dict(tuning, wave_slots=eng.get_tuning("wave_slots"), fuse_active=eng.get_tuning("fuse_active")),
+            "tuning": dict(tuning, pairs_dedup=eng.get_tuning("pairs_dedup"), wave_slots=eng.get_tuning("wave_slots"),
+                          fuse_active=eng.get_tuning("fuse_active"), source=tuning_source),
             "latency_ms": {"batched_p50": round(pct(dev_ms, 0.5), 4), "batched_p99": round(pct(dev_ms, 0.99), 4)},
             "parity_vs_golden": parity_vs_golden(golden, wl, res, args.gated),
         }
@@ -361,8 +389,10 @@ def measure(args, name, wl, decompose):
         pm = pmc_traffic(name + ("_gated" if args.gated else ""), tuning)
         if pm:
             roof.update(pm)
-            if pm.get("traffic_ms"):
-                roof["counter_GBps"] = round(pm["traffic"] / (pm["traffic_ms"] * 1e-3) / 1e9, 2)
+            if pm.get("profile_phase_ms"):
+                # HBM bytes the counters saw over the same kernels' traced time
+                roof["counter_GBps"] = round(pm["traffic"] / (pm["profile_phase_ms"] * 1e-3) / 1e9, 2)
+                roof["phase_ms_vs_profile"] = round(float(phase_ms[phase]) / pm["profile_phase_ms"], 4)
         block["roofline"] = roof
         log("%s: kernels ms %s; phase %s %.3f ms, %.1f GB/s model bytes" % (
             name, [round(float(x), 3) for x in kms], roof["phase"], phase_ms[phase], achieved))
@@ -559,7 +589,9 @@ def main():
                     help="every rule's support gets its own pass (no support counting in the body-pair pass)")
     ap.add_argument("--no-per-rule", action="store_true")
     ap.add_argument("--no-decompose", action="store_true")
-    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="library default tuning")
+    ap.add_argument("--autotune", action="store_true",
+                    help="run the autotuner even where a committed profile names the tuning it was taken at")
     ap.add_argument("--tune", default="",
                     help="BUDGET,THREAD_WAVES_PER_CU,WAVE_WAVES_PER_CU[,DEDUP[,GIANT_AFTER[,CHUNKS_PER_WAVE]]]: fixed "
                          "tuning, no autotune")

@@ -107,8 +107,8 @@ bool Context::upload() {
     dg.keys = keys.as<int32_t>();
     dg.crank = crank.as<int32_t>();
     dg.nBlocks = g.nBlocks();
-    if (!one.ensure(512)) return false;
-    GR_HIP(hipMemset(one.p, 0, 512));
+    if (!one.ensure(kOneBytes)) return false;
+    GR_HIP(hipMemset(one.p, 0, kOneBytes));
     return ensure_dedup();
 }
 
@@ -186,12 +186,14 @@ bool has_conflicts(const Plan& P) {
     return false;
 }
 
-// Device scratch of the per-rule launches: a 512-byte header -- [0] count
-// (u64); +64: u32 heavy count, wave cursor; +96: u64 keys emitted (sort
-// path); +128: ConflictRefs of both plans -- and the heavy-seed queue.
-// The context's own (C.one, C.heavy1, grown as rules need) or, for the
-// allocation-free entry point, the fixed device scratch array, where
-// nothing may grow (mayAlloc false).
+// Device scratch of the per-rule launches: a header -- [0] count (u64);
+// [8] the giant sub-items' count (u64); +64: u32 heavy count, wave cursor;
+// +96: u64 keys emitted (sort path); +128: ConflictRefs of both plans (to
+// +288); +kOnePlanOff: the wave-mode plan for the giant kernels -- and the
+// heavy-seed queue. The context's own (C.one, kOneBytes, and C.heavy1,
+// grown as rules need) or, for the allocation-free entry point, the fixed
+// device scratch array (a kNoAllocHeader-byte header), where nothing may
+// grow (mayAlloc false) and giant seeds are not cut.
 struct OneScratch {
     char* hdr;
     HeavyItem* heavy;
@@ -250,9 +252,26 @@ bool run_one(Context& C, OneScratch S, const Plan& P, const ConflictRefs& R, con
     if (sort && S.mayAlloc && !C.sortb.ensure(1, kSortCapMin, C.sortCapMax)) return false;
     const int hoBits = bits_for((uint64_t)C.g.nE);
     const int hBits = bits_for((uint64_t)items);
+    // Seeds still running after giantAfter wave-mode steps are cut into
+    // sub-items that all waves share, as in batch passes (the wave kernel
+    // stores its plan, as rule 0, where the giant kernels read it). The
+    // allocation-free path runs every seed to its end on one wave.
+    const bool giants = S.mayAlloc && C.giantAfter > 0;
+    Plan* d_plan = reinterpret_cast<Plan*>(S.hdr + kOnePlanOff);
+    Plan pw = PW;
+    pw.rule = 0;
     for (;;) {
-        launch_fill_u64(d_count, 1, 0, C.st);
-        launch_fill_u32(ctl, 16, 0, C.st);   // control words and the emitted-key counter
+        GiantArgs ga{};
+        if (giants) {
+            ZeroRanges z{};
+            zero_add(z, d_count, 4);   // the count and the sub-items' count
+            zero_add(z, ctl, 16);      // control words and the emitted-key counter
+            giant_args(C.giants, C.giantAfter, ga, 0, z);
+            launch_zero(z, C.st);
+        } else {
+            launch_fill_u64(d_count, 1, 0, C.st);
+            launch_fill_u32(ctl, 16, 0, C.st);   // control words and the emitted-key counter
+        }
         GR_HIP(hipEventRecord(C.ev[0], C.st));
         const int64_t waves = (items + kWave - 1) / kWave;
         const int blocks = (int)std::min<int64_t>((waves + 3) / 4, C.threadBlocks());
@@ -260,9 +279,12 @@ bool run_one(Context& C, OneScratch S, const Plan& P, const ConflictRefs& R, con
         launch_thread_one(P.kind, slots, P, oa, C.dg, blocks, C.st);
         SortSink sink{};
         if (sort) sink = SortSink{C.sortb.keys[0].as<unsigned long long>(), d_emitted, C.sortb.cap, hoBits};
-        WaveArgs wa{nullptr, conf ? d_refs + 1 : nullptr, S.heavy, ctl + 0, heavyCap, 0, ctl + 1,
+        WaveArgs wa{giants ? d_plan : nullptr, conf ? d_refs + 1 : nullptr, S.heavy, ctl + 0, heavyCap, 0, ctl + 1,
                     d_count, C.dedup.as<uint32_t>(), C.rowWords, sink};
-        launch_wave_one(PW.kind, slots, mode, PW, wa, C.dg, C.waveBlocksFor(slots), C.st);
+        wa.giant = ga;
+        wa.planOut = giants ? d_plan : nullptr;
+        launch_wave_one(PW.kind, slots, mode, pw, wa, C.dg, C.waveBlocksFor(slots), C.st);
+        if (giants) launch_giants(PW.kind, slots, mode, wa, C.dg, C.waveBlocksFor(slots), C.st);
         if (sort) {
             SortJob j{{C.sortb.keys[0].as<unsigned long long>(), C.sortb.keys[1].as<unsigned long long>()},
                       d_emitted, C.sortb.cap, sort_max_tiles(C.sortb.cap), C.sortb.hist.as<uint32_t>(),
@@ -301,7 +323,7 @@ bool run_one(Context& C, OneScratch S, const Plan& P, const ConflictRefs& R, con
         float f = 0;
         GR_HIP(hipEventElapsedTime(&f, C.ev[0], C.ev[1]));
         ms = f;
-        count = C.hOne[0];
+        count = C.hOne[0] + C.hOne[1];
         heavySeeds += hctl[0];
         return true;
     }

@@ -23,6 +23,12 @@ constexpr uint32_t kMagicHost = 0x47524831;  // "GRH1"
 constexpr uint32_t kMagicCtx = 0x47524331;   // "GRC1"
 constexpr uint32_t kMagicBatch = 0x47524231; // "GRB1"
 
+// Per-rule launch header (the context's own; the allocation-free entry
+// point's is kNoAllocHeader bytes and stops before the plan).
+constexpr size_t kOnePlanOff = 512;
+constexpr size_t kOneBytes = 1024;
+static_assert(kOnePlanOff + sizeof(Plan) <= kOneBytes, "plan beyond the per-rule header");
+
 inline void log_err(const char* what, hipError_t e) {
     std::fprintf(stderr, "[gpu_rules] %s: %s\n", what, hipGetErrorString(e));
 }
@@ -251,7 +257,7 @@ struct Context : Handle {
     DevBuf blocks, rowStart, col, coo, keys, crank;
     DevGraph dg{};
     // per-rule launch scratch
-    DevBuf one;       // [0] count (u64); +64: u32 control words; +128: ConflictRefs
+    DevBuf one;       // kOneBytes: the per-rule launch header (counts, control words, ConflictRefs, plan)
     DevBuf heavy1;
     uint32_t heavy1Cap = 0;
     unsigned long long* hOne = nullptr;   // pinned

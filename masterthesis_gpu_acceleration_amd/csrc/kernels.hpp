@@ -117,6 +117,8 @@ struct WaveArgs {
     SortSink sink;                // kDedupSort
     GiantArgs giant;              // batch launches; after == 0 for per-rule launches
     const int32_t* supTab;        // member lists of plans with supPred <= -2 (batch launches)
+    Plan* planOut;                // per-rule launches with giants: the by-value kernel stores its plan here
+                                  // (plans[0] of the giant kernels)
 };
 
 // `kind` is the plan kind (kSupport / kPairs) of every plan the launch sees;

@@ -411,19 +411,37 @@ __device__ __forceinline__ void push_heavy(bool heavy, uint32_t plan, uint32_t i
 // with rank crank[i]: a candidate grounds iff its value is not bound yet.
 // Returns the grounding mask; xs[k] gets each candidate's value, rs[k] the
 // rank of each grounding (when `pairs`; else 0), -1 where none. Straight-line
-// code: no per-candidate branches.
+// code: every load is issued (lanes without a candidate read entry `safe`, a
+// valid index, and drop the value), and the injectivity test is a minimum of
+// XORs against the bound slots (unbound slots hold -1, which no entry does),
+// so no candidate gets a branch of its own or a wait between its loads.
 template <int U, int NS>
 __device__ __forceinline__ uint32_t leaf_fast(const DevGraph& G, const int (&v)[NS], int nbound, bool pairs,
-                                              const int (&i)[U], const bool (&in)[U], int (&xs)[U], int (&rs)[U]) {
+                                              const int (&i)[U], const bool (&in)[U], int safe, int (&xs)[U],
+                                              int (&rs)[U]) {
+    int at[U];
 #pragma unroll
     for (int k = 0; k < U; ++k) {
-        xs[k] = in[k] ? G.col[i[k]] : 0;
-        rs[k] = (in[k] && pairs) ? G.crank[i[k]] : 0;
+        at[k] = in[k] ? i[k] : safe;
+        xs[k] = G.col[at[k]];
     }
+    if (pairs) {
+#pragma unroll
+        for (int k = 0; k < U; ++k) rs[k] = G.crank[at[k]];
+    } else {
+#pragma unroll
+        for (int k = 0; k < U; ++k) rs[k] = 0;
+    }
+    int vb[NS];
+#pragma unroll
+    for (int j = 0; j < NS; ++j) vb[j] = j < nbound ? v[j] : -1;
     uint32_t m = 0;
 #pragma unroll
     for (int k = 0; k < U; ++k) {
-        const bool ok = in[k] && !taken(v, nbound, xs[k]);
+        uint32_t d = 0xFFFFFFFFu;
+#pragma unroll
+        for (int j = 0; j < NS; ++j) d = min(d, (uint32_t)(vb[j] ^ xs[k]));
+        const bool ok = in[k] & (d != 0u);
         m |= (uint32_t)ok << k;
         rs[k] = ok ? rs[k] : -1;
     }
@@ -621,7 +639,7 @@ __device__ __forceinline__ uint64_t thread_chunk(ThreadLdsT<levels_of(NS)>& W, c
                         is[k] = fr.i + k;
                         in[k] = k < take;
                     }
-                    const uint32_t m = leaf_fast(G, v, level_nbound(P.lev[d].flags), false, is, in, xs, rs);
+                    const uint32_t m = leaf_fast(G, v, level_nbound(P.lev[d].flags), false, is, in, fr.i, xs, rs);
                     if (!pairs) {
                         if (m) {
                             mine += 1;
@@ -986,7 +1004,7 @@ __device__ __forceinline__ uint64_t wave_run(WaveLdsT<levels_of(NS)>& W, const D
     for (;;) {
         // ---- A: idle lanes draw from the pool
         unsigned long long tp = GR_PROF_NOW();
-        int profA = 3;
+        [[maybe_unused]] int profA = 3;
         const uint64_t idleM = __ballot(!busy);
         const int pn = uniform(W.poolNext), pe = uniform(W.poolEnd), plv = uniform(W.poolLevel);
         const uint32_t pg = (uint32_t)uniform((int)W.poolGen);
@@ -1020,7 +1038,7 @@ __device__ __forceinline__ uint64_t wave_run(WaveLdsT<levels_of(NS)>& W, const D
                         is[k] = pn + r + n * k;
                         in[k] = !busy && is[k] < pe;
                     }
-                    const uint32_t m = leaf_fast(G, v, level_nbound(P.lev[last].flags), pairs, is, in, xs, rs);
+                    const uint32_t m = leaf_fast(G, v, level_nbound(P.lev[last].flags), pairs, is, in, pn, xs, rs);
                     if (!pairs) found = found || m != 0u;
                     else mine += mark_ranks<D, kLeafUnroll>(dd, rs);
                 } else if (plv == last) {
@@ -1030,11 +1048,11 @@ __device__ __forceinline__ uint64_t wave_run(WaveLdsT<levels_of(NS)>& W, const D
                     const bool byEntry = pairs && pl == last && D != kDedupSort && hoByEntry;
                     int xs[kLeafGeneric], rs[kLeafGeneric];
 #pragma unroll
-                    for (int k = 0; k < kLeafGeneric; ++k) {
+                    for (int k = 0; k < kLeafGeneric; ++k) {   // loads issued together (entry pn is valid)
                         const int i = pn + r + n * k;
-                        const bool in = !busy && i < pe;
-                        xs[k] = in ? G.col[i] : 0;
-                        rs[k] = (in && byEntry) ? G.crank[i] : hru;
+                        const int at = (!busy && i < pe) ? i : pn;
+                        xs[k] = G.col[at];
+                        rs[k] = byEntry ? G.crank[at] : hru;
                     }
                     bool hit = false;
                     int hos[kLeafGeneric];
@@ -1121,7 +1139,7 @@ __device__ __forceinline__ uint64_t wave_run(WaveLdsT<levels_of(NS)>& W, const D
                         is[k] = fr.i + k;
                         in[k] = k < take;
                     }
-                    const uint32_t m = leaf_fast(G, v, level_nbound(P.lev[d].flags), pairs, is, in, xs, rs);
+                    const uint32_t m = leaf_fast(G, v, level_nbound(P.lev[d].flags), pairs, is, in, fr.i, xs, rs);
                     if (!pairs) {
                         if (m) {
                             found = true;
@@ -1490,6 +1508,11 @@ __global__ void __launch_bounds__(kWave) k_wave_one(Plan P, WaveArgs a, DevGraph
     uint32_t n = *a.heavyCount;
     if (n > a.heavyCap) n = a.heavyCap;
     if (n == 0) return;
+    if (a.planOut && blockIdx.x == 0) {   // the giant kernels read the plan from memory
+        const int32_t* src = reinterpret_cast<const int32_t*>(&P);
+        int32_t* dst = reinterpret_cast<int32_t*>(a.planOut);
+        for (int k = lane; k < kPlanWords; k += kWave) dst[k] = src[k];
+    }
     copy_plan(W, &P);
     Dedup<D, NS> dd = wave_dedup<D, NS>(W, cache, dyn, a, G);
     unsigned long long stats[kWaveStats] = {};
