@@ -14,10 +14,11 @@ Default run (`python bench.py --gpus 1 --steps K --warmup W`), one JSON line:
   (6.8M triples, 500,000 entities, 60 predicates, ~50 hubs of degree ~1e5,
   50 rules of 2-4 atoms), with its own ms_per_step, roofline and CPU baseline.
 * "serving" -- the continuous-batching server over the resident high-fan-out
-  graph: the serving stream (1-8 atoms, predicates Zipf s=1.0, Poisson
-  arrivals; no work filter) and, beside it, the same stream with paths of
-  estimated search work above 1e8 redrawn; p50/p99/max latency from arrival
-  to result, served rate, mean batch.
+  graph: its capacity (the stream offered far above what it serves), the
+  serving stream (1-8 atoms, predicates Zipf s=1.0, Poisson arrivals; no
+  work filter) and, beside it, the same stream with paths of estimated search
+  work above 1e8 redrawn; p50/p99/max latency from arrival to result, served
+  rate, mean batch, requests routed to heavy batches.
 
 Every block also reports:
   * per_rule: the same rules through the reference library's per-rule launch
@@ -61,6 +62,12 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# Hardware queues per process (read when the HIP runtime starts): the
+# library's two streams plus the server's three slot streams need five; with
+# HIP's default of four, two slots share a queue and each waits for the
+# other's batches.
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < 8:
+    os.environ["GPU_MAX_HW_QUEUES"] = "8"
 
 from masterthesis_gpu_acceleration_amd import lib as L  # noqa: E402
 from masterthesis_gpu_acceleration_amd import workloads as W  # noqa: E402
@@ -526,6 +533,7 @@ def serve_leg(eng, g, n, rate, max_work, label):
     comp = r["completions"]
     nb = np.array([len(q[1]) for q in reqs])
     out = {"requests": n, "offered_per_s": rate, "served_rules_per_s": round(r["rules_per_s"], 1),
+           "heavy_requests": st.get("heavy_requests"), "heavy_batches": st.get("heavy_batches"),
            "span_s": round(r["span_s"], 3),
            "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3),
                           "max": round(float(lat.max()), 3)},
@@ -546,11 +554,21 @@ def serve_leg(eng, g, n, rate, max_work, label):
 def serving_block(args, g):
     eng = L.Engine.from_graph(g)
     try:
+        if args.serve_heavy_work >= 0:
+            eng.set_tuning("serve_heavy_work", int(args.serve_heavy_work))
         eng.evaluate(L.rule_words_for_graph(W.serving_stream(256, graph=g).rules, g), 0)   # warm the kernels
         blk = {"config": "continuous-batching server (3 slots, batches up to 2048 requests) over the resident "
                          "high-fan-out graph (6.8M triples); support for every request, body pairs when support > 0 "
-                         "(reference semantics); latency from scheduled arrival to result on the host",
-               "stream": serve_leg(eng, g, args.serve_requests, args.serve_rate, None, "stream")}
+                         "(reference semantics); latency from scheduled arrival to result on the host; requests of "
+                         "estimated search work above serve_heavy_work in heavy batches of their own (one in flight, "
+                         "serve_heavy_percent of the grid)",
+               "serve_heavy_work": eng.get_tuning("serve_heavy_work"),
+               "serve_heavy_percent": eng.get_tuning("serve_heavy_percent")}
+        if args.serve_capacity_requests > 0:
+            # offered far above what the device serves: the served rate is the capacity
+            blk["capacity"] = serve_leg(eng, g, args.serve_capacity_requests, args.serve_capacity_rate, None,
+                                        "capacity")
+        blk["stream"] = serve_leg(eng, g, args.serve_requests, args.serve_rate, None, "stream")
         if args.serve_filtered_requests > 0:
             blk["stream_work_filtered"] = serve_leg(eng, g, args.serve_filtered_requests, args.serve_filtered_rate, 1e8,
                                                     "filtered")
@@ -575,6 +593,11 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="leave out the serving block")
     ap.add_argument("--serve-requests", type=int, default=9000)
     ap.add_argument("--serve-rate", type=float, default=150.0)
+    ap.add_argument("--serve-capacity-requests", type=int, default=3000,
+                    help="requests of the capacity leg (offered at --serve-capacity-rate; 0: no capacity leg)")
+    ap.add_argument("--serve-capacity-rate", type=float, default=2000.0)
+    ap.add_argument("--serve-heavy-work", type=float, default=-1,
+                    help="serve_heavy_work tuning for the server (-1: the library default; 0: no heavy batches)")
     ap.add_argument("--serve-filtered-requests", type=int, default=30000)
     ap.add_argument("--serve-filtered-rate", type=float, default=1000.0)
     ap.add_argument("--compare-dedup", action="store_true",

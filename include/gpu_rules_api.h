@@ -250,9 +250,13 @@ int64_t gpu_get_tuning(void* ctx, const char* key);
  * by the batches in flight (memBudgetBytes; <= 0 means half the free HBM),
  * and launches them as one batch; batches in different slots overlap on the
  * device. A request that cannot fit even on an idle device completes with
- * status 3. Results are exact per rule, independent of batching. While a
- * server runs, its context must not be used by other calls; stop it before
- * gpu_free_graph. */
+ * status 3. Requests whose estimated search work (gpu_host_rules_work)
+ * passes the context's "serve_heavy_work" tuning (taken at start; 0: off)
+ * run in heavy batches of their own -- one in flight at a time, on
+ * "serve_heavy_percent" of the grid -- so light requests are neither
+ * batched with them nor kept off the device by them. Results are exact per
+ * rule, independent of batching. While a server runs, its context must not
+ * be used by other calls; stop it before gpu_free_graph. */
 typedef struct {
     uint64_t tag;              /* the caller's request id */
     GpuRuleResult64 result;
@@ -273,6 +277,9 @@ int gpu_serve_poll(void* srv, GpuServeCompletion* out, int32_t maxOut, int32_t w
  * memory, largest batch, peak batches in flight. Returns 0, 1 if a batch
  * failed on the device, -1 on a bad handle. */
 int gpu_serve_stats(void* srv, int64_t* stats);
+/* Requests routed to heavy batches and heavy batches launched so far;
+ * 0, or -1 on a bad handle. No counterpart in the reference. */
+int gpu_serve_heavy_stats(void* srv, int64_t* heavyRequests, int64_t* heavyBatches);
 /* HBM one execution slot holds for the server's life (bitmap rows, sort and
  * giant-seed buffers), outside the per-request budget; -1 on a bad handle. */
 int64_t gpu_serve_slot_bytes(void* srv);
@@ -317,6 +324,13 @@ int32_t gpu_host_count_action(int32_t which, int64_t count, int64_t cap, int64_t
  * info[1] headSupport, info[2] status, info[3] support launched (0/1),
  * info[4] body pairs launched (0/1), info[5] support seeds, info[6] pair seeds. */
 int gpu_host_plan_info(void* handle, const int32_t* words, int64_t* info);
+/* Estimated search work of n structured rules (GPU_RULE_WORDS words each):
+ * candidates their support and body-pair searches enumerate, from per-
+ * (direction, predicate) row statistics -- the figure the server schedules
+ * by (order of magnitude, not a bound); 0 for rules not counted on the
+ * device. 0 or -1. No counterpart in the reference: it is for callers that
+ * schedule rule batches. */
+int gpu_host_rules_work(void* handle, const int32_t* words, int32_t n, double* work);
 
 /* Library version string. */
 const char* gpu_rules_version(void);

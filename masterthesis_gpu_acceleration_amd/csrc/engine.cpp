@@ -35,6 +35,13 @@ bool Context::upload() {
     GR_HIP(hipEventCreateWithFlags(&joinEv, hipEventDisableTiming));
     for (auto& e : ev) GR_HIP(hipEventCreate(&e));
     GR_HIP(hipHostMalloc((void**)&hOne, 512, hipHostMallocDefault));
+    // batch buffers grow stream-ordered from the device's default pool: let the
+    // pool keep what they free, so regrowing reuses it instead of mapping anew
+    hipMemPool_t pool = nullptr;
+    if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess && pool) {
+        uint64_t keep = UINT64_MAX;
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    }
     // the allocation-free entry point's scratch on this device (reported there if unavailable)
     noallocReady = noalloc_init(device);
 
@@ -564,17 +571,21 @@ bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const s
     }
     if (all.empty()) all.push_back(Chunk{0, 0, 0, 0});
     B.capRules = std::max<uint32_t>((uint32_t)rules.size(), 1);
-    if (!put(B.plans, plans, st) || !put(B.wplans, wplans, st) || !put(B.refs, refs, st) ||
-        !put(B.wrefs, wrefs, st) || !put(B.chunks, all, st) || !put(B.supTab, supTab, st))
+    // stream-ordered buffers: a batch rebuilt on a serving slot grows them
+    // without waiting for the batches of other slots
+    if (!put_on(B.plans, plans, st) || !put_on(B.wplans, wplans, st) || !put_on(B.refs, refs, st) ||
+        !put_on(B.wrefs, wrefs, st) || !put_on(B.chunks, all, st) || !put_on(B.supTab, supTab, st))
         return false;
-    if (!B.counts.ensure((size_t)2 * B.capRules * 8) ||
-        !B.heavy.ensure((size_t)std::max<uint64_t>(heavyTotal, 1) * sizeof(HeavyItem)) || !B.ctl.ensure(kCtlWords * 4))
+    if (!B.counts.ensure_on((size_t)2 * B.capRules * 8, st) ||
+        !B.heavy.ensure_on((size_t)std::max<uint64_t>(heavyTotal, 1) * sizeof(HeavyItem), st) ||
+        !B.ctl.ensure_on(kCtlWords * 4, st))
         return false;
-    if (B.hCountsCap < (size_t)2 * B.capRules) {
+    if (B.hCountsCap < (size_t)2 * B.capRules) {   // pinned: grows seldom (hipHostFree waits for the device)
         if (B.hCounts) (void)hipHostFree(B.hCounts);
         B.hCounts = nullptr;
-        GR_HIP(hipHostMalloc((void**)&B.hCounts, (size_t)2 * B.capRules * 8, hipHostMallocDefault));
-        B.hCountsCap = (size_t)2 * B.capRules;
+        const size_t cap = std::max<size_t>({(size_t)2 * B.capRules, 2 * B.hCountsCap, 8192});
+        GR_HIP(hipHostMalloc((void**)&B.hCounts, cap * 8, hipHostMallocDefault));
+        B.hCountsCap = cap;
     }
     if (!B.hCtl) GR_HIP(hipHostMalloc((void**)&B.hCtl, kCtlWords * 4, hipHostMallocDefault));
     if (!B.hGiant) GR_HIP(hipHostMalloc((void**)&B.hGiant, 8, hipHostMallocDefault));
@@ -632,7 +643,7 @@ hipError_t batch_enqueue(Context& C, Batch& B, const ExecTarget& t) {
             ThreadArgs ta{B.plans.as<Plan>(), B.refs.as<ConflictRefs>(), B.chunks.as<Chunk>() + B.chunkOff[ph], nc,
                           B.heavyCap[ph], c + 0, counts, B.heavy.as<HeavyItem>() + B.heavyOff[ph], c + 1,
                           fusePh ? 1 : 0, B.supTab.as<int32_t>()};
-            const int blocks = std::min<int>(C.threadBlocks(), (int)((nc + 3) / 4));
+            const int blocks = std::min<int>(scale_grid(C.threadBlocks(), t.gridPercent), (int)((nc + 3) / 4));
             launch_thread_batch(ph, B.slots[ph], ta, C.dg, std::max(blocks, 1), st);
         }
         if (t.ev) err(hipEventRecord(t.ev[1 + 2 * ph], st), "hipEventRecord");
@@ -647,7 +658,8 @@ hipError_t batch_enqueue(Context& C, Batch& B, const ExecTarget& t) {
             const int mode = ph == kPairs ? t.pairsDedup : kDedupRows;
             if (!early[ph]) giant_prepare(*gbp, t.giantAfter, gas[ph], st, ph);
             wa.giant = gas[ph];
-            const int wb = t.pairsDedup == kDedupLds ? C.waveBlocksFor(B.slots[ph]) : t.waveBlocks;
+            const int wb = scale_grid(t.pairsDedup == kDedupLds ? C.waveBlocksFor(B.slots[ph]) : t.waveBlocks,
+                                      t.gridPercent);
             launch_wave_batch(ph, B.slots[ph], mode, wa, C.dg, wb, st);
             if (t.giantAfter > 0) {
                 launch_giants(ph, B.slots[ph], mode, wa, C.dg, wb, st);
@@ -887,6 +899,7 @@ Context* new_context_from(HostGraph&& g) {
     C->magic = kMagicCtx;
     C->g = std::move(g);
     C->tr.reset(new TermResolver(C->g));
+    C->rowStats = row_stats(C->g);
     // HBM estimate, as the reference checks before uploading (GPU_03/gpu_rules_lib.cu:222-279)
     size_t freeB = 0, totalB = 0;
     if (hipMemGetInfo(&freeB, &totalB) == hipSuccess) {
@@ -1448,6 +1461,12 @@ int gpu_set_tuning(void* ctx, const char* key, int64_t value) {
         if (!C->ensure_dedup()) return -1;
     } else if (k == "giant_after") {
         C->giantAfter = (int)std::max<int64_t>(0, value);
+    } else if (k == "serve_heavy_work") {
+        // servers started afterwards: estimated work above which a request runs in a heavy batch (0: off)
+        C->serveHeavyWork = (double)std::max<int64_t>(0, value);
+    } else if (k == "serve_heavy_percent") {
+        if (value < 1 || value > 100) return -1;
+        C->serveHeavyPercent = (int)value;
     } else if (k == "sort_cap") {
         // start the sort-unique buffers at this many keys (they still grow on overflow)
         if (value < kSortTile || value > (int64_t)C->sortCapMax) return -1;
@@ -1494,6 +1513,8 @@ int64_t gpu_get_tuning(void* ctx, const char* key) {
     if (k == "sort_active") return C->sortActive() ? 1 : 0;
     if (k == "pairs_dedup") return C->pairsDedup();
     if (k == "giant_after") return C->giantAfter;
+    if (k == "serve_heavy_work") return (int64_t)C->serveHeavyWork;
+    if (k == "serve_heavy_percent") return C->serveHeavyPercent;
     if (k == "last_giants") return C->lastGiants;
     if (k == "last_fused") return C->lastFused;
     // wave-mode counters of the last batch: wave_<phase>_<seeds|steps|donations|leaf_draws|draws|busy_lane_steps|draw_lanes>
@@ -1621,6 +1642,16 @@ int32_t gpu_host_count_action(int32_t which, int64_t count, int64_t cap, int64_t
     if (which == 0) return sort_count_action((uint64_t)count, (uint64_t)cap, (uint64_t)capMax);
     if (which == 1) return heavy_count_action((uint64_t)count, (uint64_t)cap);
     return -1;
+}
+
+int gpu_host_rules_work(void* handle, const int32_t* words, int32_t n, double* work) {
+    const HostGraph* g = as_graph(handle);
+    if (!g || n < 0 || (n > 0 && (!words || !work))) return -1;
+    const RowStats st = row_stats(*g);
+    Tuning t;
+    for (int32_t i = 0; i < n; ++i)
+        work[i] = estimate_work(compile_rule(rule_from_words(words + (size_t)i * GPU_RULE_WORDS), *g, t), *g, st);
+    return 0;
 }
 
 int gpu_host_plan_info(void* handle, const int32_t* words, int64_t* info) {

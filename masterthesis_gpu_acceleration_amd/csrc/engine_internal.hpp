@@ -52,14 +52,18 @@ struct HostHandle : Handle {
 };
 
 // Device buffer that only grows.
+//   ensure: hipMalloc / hipFree (hipFree waits for the whole device);
+//   ensure_on: stream-ordered (hipMallocAsync / hipFreeAsync on stream st),
+//     at least doubling, so a buffer of one stream -- a serving slot's batch --
+//     grows without waiting for work on other streams. Such a buffer is
+//     freed on the same stream, which must outlive it (release first).
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    hipStream_t ordered = nullptr;   // stream of a stream-ordered allocation
     bool ensure(size_t need) {
         if (need <= bytes) return true;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
+        release();
         size_t b = std::max<size_t>(need, 256);
         hipError_t e = hipMalloc(&p, b);
         if (e != hipSuccess) {
@@ -70,10 +74,28 @@ struct DevBuf {
         bytes = b;
         return true;
     }
+    bool ensure_on(size_t need, hipStream_t st) {
+        if (need <= bytes && (ordered == st || !p)) return true;
+        const size_t b = std::max<size_t>({need, bytes * 2, 256});
+        release();
+        hipError_t e = hipMallocAsync(&p, b, st);
+        if (e != hipSuccess) {
+            log_err("hipMallocAsync", e);
+            p = nullptr;
+            return false;
+        }
+        bytes = b;
+        ordered = st;
+        return true;
+    }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            if (ordered) (void)hipFreeAsync(p, ordered);
+            else (void)hipFree(p);
+        }
         p = nullptr;
         bytes = 0;
+        ordered = nullptr;
     }
     template <class T> T* as() const { return static_cast<T*>(p); }
 };
@@ -236,6 +258,9 @@ struct Batch : Handle {
     uint32_t* hGiant = nullptr;   // pinned: giants listed in each phase of the last pass
     unsigned long long* hWaveStats = nullptr;   // pinned: kWaveStats counters per phase of the last pass
     ~Batch();
+    // Frees the device buffers (stream-ordered ones on the stream they were
+    // built on, which must still exist).
+    void free_device();
 };
 
 struct Context : Handle {
@@ -278,6 +303,13 @@ struct Context : Handle {
     hipEvent_t forkEv = nullptr, joinEv = nullptr;
     GiantBufs giants2;
     int giantAfter = 2048;   // wave-mode steps on one seed before it is listed as a giant (0: never)
+    // Serving: requests whose estimated search work (estimate_work) passes
+    // serveHeavyWork run in batches of their own, one such batch in flight at
+    // a time, on serveHeavyPercent of the grid, so that light requests are
+    // neither batched with them nor kept off the device by them (0: off).
+    double serveHeavyWork = 1e11;
+    int serveHeavyPercent = 50;
+    RowStats rowStats;       // of g, for the estimates
     // Count the support of qualifying rules in the body-pair pass instead of
     // a support pass of their own; needs bitmap dedup (rows or LDS).
     bool fuseSupport = true;
@@ -322,11 +354,15 @@ struct Context : Handle {
     ~Context();
 };
 
+inline void Batch::free_device() {
+    plans.release(); wplans.release(); refs.release(); wrefs.release(); chunks.release(); counts.release();
+    heavy.release(); ctl.release(); supTab.release();
+}
+
 inline Batch::~Batch() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
-    plans.release(); wplans.release(); refs.release(); wrefs.release(); chunks.release(); counts.release();
-    heavy.release(); ctl.release();
+    free_device();
     if (hCounts) (void)hipHostFree(hCounts);
     if (hCtl) (void)hipHostFree(hCtl);
     if (hGiant) (void)hipHostFree(hGiant);
@@ -354,6 +390,14 @@ inline bool put(DevBuf& b, const std::vector<T>& v, hipStream_t st) {
     return true;
 }
 
+// The same into a stream-ordered buffer of st.
+template <class T>
+inline bool put_on(DevBuf& b, const std::vector<T>& v, hipStream_t st) {
+    if (!b.ensure_on(std::max<size_t>(v.size(), 1) * sizeof(T), st)) return false;
+    if (!v.empty()) GR_HIP(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+    return true;
+}
+
 // Where a batch's device pass runs: stream, wave-mode dedup rows or (when
 // `sort` is set) the sort-unique buffers, and (optionally) timing events
 // ev[0..4] around the four kernels, ev[6] before the sort-unique pass.
@@ -368,12 +412,18 @@ struct ExecTarget {
     int pairsDedup;   // DedupMode of the body-pair wave-mode kernel
     GiantBufs* giants;
     int giantAfter;
+    int gridPercent = 100;   // share of the thread-mode and wave-mode grids the pass launches
     // optional: run the pairs phase of an ungated batch on st2 beside the
     // support phase (untimed passes only), with giants2 as its giant buffers
     hipStream_t st2 = nullptr;
     hipEvent_t forkEv = nullptr, joinEv = nullptr;
     GiantBufs* giants2 = nullptr;
 };
+
+// `percent` of a grid of `blocks` blocks, at least one.
+inline int scale_grid(int blocks, int percent) {
+    return percent >= 100 ? blocks : std::max(1, (int)((int64_t)blocks * percent / 100));
+}
 
 // Zeroes the giant pass's counters and flags for one wave-mode phase and
 // fills a GiantArgs for it (after == 0: no seed is listed).

@@ -379,4 +379,86 @@ CompiledRule compile_rule(const RuleIR& r, const HostGraph& g, const Tuning& t) 
     return c;
 }
 
+RowStats row_stats(const HostGraph& g) {
+    RowStats st;
+    st.P = g.P();
+    st.keys.assign(2 * (size_t)st.P, 0.0);
+    st.nnz.assign(2 * (size_t)st.P, 0.0);
+    st.sq.assign(2 * (size_t)st.P, 0.0);
+    for (int dir = 0; dir < 2; ++dir) {
+        const std::vector<int32_t>& key = dir == 0 ? g.spoKey : g.posKey;
+        for (int32_t p = 0; p < st.P; ++p) {
+            const size_t at = (size_t)dir * st.P + p;
+            int32_t i = g.predOff[p];
+            const int32_t end = g.predOff[p + 1];
+            while (i < end) {
+                int32_t j = i + 1;
+                while (j < end && key[j] == key[i]) ++j;
+                const double len = j - i;
+                st.keys[at] += 1;
+                st.nnz[at] += len;
+                st.sq[at] += len * len;
+                i = j;
+            }
+        }
+    }
+    return st;
+}
+
+namespace {
+
+// (direction * P + predicate) of a level's key-index block run.
+int64_t level_row_set(int32_t lo, int32_t hi, const HostGraph& g) {
+    const int64_t base = (int64_t)(uint32_t)lo | ((int64_t)hi << 32);
+    return base / std::max<int64_t>(1, g.nBlocks());
+}
+
+double plan_work(const Plan& P, int64_t items, const HostGraph& g, const RowStats& st) {
+    if (items <= 0) return 0;
+    const int64_t nsets = 2 * (int64_t)st.P;
+    auto mean = [&](int64_t set, bool reached) {
+        if (set < 0 || set >= nsets || st.keys[set] <= 0) return 0.0;
+        const double m = st.nnz[set] / st.keys[set];
+        return reached ? std::max(m, st.sq[set] / st.nnz[set]) : m;
+    };
+    double nodes = (double)items, work = 0;
+    for (int L = 0; L < P.nlev; ++L) {
+        const Level& lv = P.lev[L];
+        const int kind = level_kind(lv.flags);
+        double f = 1;
+        if (kind == kScan) {
+            f = (double)(lv.hi - lv.lo);
+        } else if (kind == kExt || kind == kIsect) {
+            const int64_t set = level_row_set(lv.lo, lv.hi, g);
+            if (lv.key < 0) {   // constant key: its row
+                const int32_t p = (int32_t)(set % std::max(1, st.P));
+                const auto rr = g.row((int)(set / std::max(1, st.P)), p, -(lv.key + 1));
+                f = rr.second - rr.first;
+            } else {
+                f = mean(set, L > 0);
+            }
+            if (kind == kIsect) f = std::min(f, mean(level_row_set(lv.lo2, lv.hi2, g), true));
+            if (L == 0 && set >= 0 && set < nsets) {   // the seeds' rows hold at most the predicate's entries
+                nodes = std::min(nodes * f, st.nnz[set]);
+                work += nodes;
+                continue;
+            }
+        }
+        nodes *= f;
+        work += nodes;
+        if (work > 1e30) return 1e30;
+    }
+    return work;
+}
+
+}  // namespace
+
+double estimate_work(const CompiledRule& c, const HostGraph& g, const RowStats& st) {
+    if (c.status != kRuleOk) return 0;
+    double w = 0;
+    if (c.supportLaunch) w += plan_work(c.sup, c.supItems, g, st);
+    if (c.pairsLaunch) w += plan_work(c.pairs, c.pairItems, g, st);
+    return w;
+}
+
 }  // namespace gr

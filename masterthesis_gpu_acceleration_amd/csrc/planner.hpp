@@ -1,6 +1,7 @@
 // planner.hpp -- compiles a rule into device plans.
 #pragma once
 #include <cstdint>
+#include <vector>
 
 #include "graph_host.hpp"
 #include "plan.hpp"
@@ -37,5 +38,23 @@ struct CompiledRule {
 };
 
 CompiledRule compile_rule(const RuleIR& r, const HostGraph& g, const Tuning& t);
+
+// Row statistics per (direction, predicate), index dir * P + p: keys, entries
+// and the sum of squared row lengths (for size-biased fan-outs).
+struct RowStats {
+    int32_t P = 0;
+    std::vector<double> keys, nnz, sq;
+};
+RowStats row_stats(const HostGraph& g);
+
+// Estimated candidates the searches of a compiled rule enumerate (support
+// plan + body-pair plan), from row statistics alone: the seeds' level-0 rows
+// at their mean length (at most the predicate's entries), each later level
+// that extends from a variable at the larger of the mean and the size-biased
+// mean row length (sum of squares / entries: the expected row of a key
+// reached over an edge, which hubs dominate), ISECT at the smaller of its
+// two rows, SCAN at the predicate's entries, CHECK at 1. An order-of-magnitude
+// figure for scheduling, not a bound.
+double estimate_work(const CompiledRule& c, const HostGraph& g, const RowStats& st);
 
 }  // namespace gr

@@ -10,11 +10,19 @@
 // new requests keep arriving. Finished slots are found by event query; their
 // results go to a completion queue read by gpu_serve_poll.
 //
+// Requests whose estimated search work (estimate_work) passes the context's
+// serveHeavyWork are taken out of the light batches into batches of their
+// own: at most one such heavy batch is in flight, launched on
+// serveHeavyPercent of the grid, so a request of seconds delays neither the
+// light requests it would have shared a batch with nor the light batches
+// that run beside it.
+//
 // Counts are exact integers computed per rule, so a rule's result does not
 // depend on which batch or slot it ran in.
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <mutex>
@@ -27,6 +35,7 @@ namespace gr {
 namespace {
 
 constexpr uint32_t kMagicServer = 0x47525331;   // "GRS1"
+constexpr int64_t kHeavyOverdueNs = 20'000'000;  // a heavy request waits at most this long for a spare slot
 
 int64_t mono_ns() {
     return std::chrono::duration_cast<std::chrono::nanoseconds>(
@@ -38,6 +47,14 @@ struct Request {
     uint64_t tag;
     GpuRuleDesc rule;
     int64_t submitNs;
+};
+
+// A request with its compiled rule and estimated work (the scheduler's own).
+struct Compiled {
+    Request r;
+    RuleIR ir;
+    CompiledRule cr;
+    double work = 0;
 };
 
 struct Slot {
@@ -56,8 +73,11 @@ struct Slot {
     int64_t reserved = 0;
     int64_t admitNs = 0;
     bool busy = false;
+    bool heavy = false;      // the batch in flight is a heavy one (reduced grid)
     bool retired = false;    // its buffers could not be regrown: no further batches
     ~Slot() {
+        if (st) (void)hipStreamSynchronize(st);
+        batch.free_device();   // stream-ordered on st: before the stream goes
         if (st) (void)hipStreamSynchronize(st);
         dedup.release();
         sortb.release();
@@ -96,21 +116,35 @@ struct Server : Handle {
     std::condition_variable cv;      // new requests / stop
     std::condition_variable doneCv;  // new completions
     std::deque<Request> pending;
+    std::deque<Compiled> heavyPending;   // scheduler thread only
+    double heavyWork = 0;                // 0: no heavy batches
+    int heavyPercent = 100;
     std::deque<GpuServeCompletion> done;
     std::thread th;
     std::atomic<bool> stopping{false};
     std::atomic<bool> failed{false};
     // statistics
     std::atomic<int64_t> nSubmitted{0}, nAdmitted{0}, nCompleted{0}, nBatches{0}, nRefused{0}, nDeferred{0};
-    std::atomic<int64_t> maxBatchSeen{0}, inFlightPeak{0};
+    std::atomic<int64_t> maxBatchSeen{0}, inFlightPeak{0}, nHeavy{0}, nHeavyBatches{0};
 
     bool init(int32_t maxInFlight, int64_t memBudget) {
+        // Each slot needs a hardware queue of its own: streams beyond
+        // GPU_MAX_HW_QUEUES (HIP's default 4) share queues, and a slot whose
+        // queue another slot's batch holds waits for it.
+        const char* q = std::getenv("GPU_MAX_HW_QUEUES");
+        const int queues = q && std::atoi(q) > 0 ? std::atoi(q) : 4;
+        if (maxInFlight + 2 > queues)
+            std::fprintf(stderr, "[gpu_rules] serve: %d slots and the context's 2 streams exceed GPU_MAX_HW_QUEUES=%d; "
+                                 "slots sharing a hardware queue wait for each other (set GPU_MAX_HW_QUEUES >= %d "
+                                 "before the process starts)\n", maxInFlight, queues, maxInFlight + 2);
         if (memBudget <= 0) {
             size_t f = 0, t = 0;
             if (hipMemGetInfo(&f, &t) != hipSuccess) return false;
             memBudget = (int64_t)(f / 2);
         }
         budget = memBudget;
+        heavyWork = C->serveHeavyWork;
+        heavyPercent = C->serveHeavyPercent;
         slotBytes = 0;
         for (int i = 0; i < maxInFlight; ++i) {
             auto S = std::make_unique<Slot>();
@@ -208,6 +242,7 @@ struct Server : Handle {
             reservedNow -= S->reserved;
             S->reserved = 0;
             S->busy = false;
+            S->heavy = false;
             doneCv.notify_all();
         }
         return busy;
@@ -219,7 +254,8 @@ struct Server : Handle {
         const hipError_t e = batch_enqueue(*C, S.batch, ExecTarget{S.st, S.dedup.as<uint32_t>(), S.rowWords,
                                                                    std::max(1, S.waveSlots), nullptr,
                                                                    S.sort ? &S.sortb : nullptr, C->sortBlocks(),
-                                                                   S.mode, &S.giants, C->giantAfter});
+                                                                   S.mode, &S.giants, C->giantAfter,
+                                                                   S.heavy ? heavyPercent : 100});
         const hipError_t r = hipEventRecord(S.done, S.st);
         if (r != hipSuccess) log_err("serve: hipEventRecord", r);
         return e == hipSuccess && r == hipSuccess;
@@ -232,6 +268,8 @@ struct Server : Handle {
             std::lock_guard<std::mutex> g(mu);
             all.swap(pending);
         }
+        for (auto& c : heavyPending) all.push_back(c.r);
+        heavyPending.clear();
         for (auto& r : all) {
             GpuRuleResult64 res;
             std::memset(&res, 0, sizeof(res));
@@ -241,21 +279,120 @@ struct Server : Handle {
         }
     }
 
+    // Fails a request that can never fit the HBM budget (status 3).
+    void refuse(Request& r) {
+        GpuRuleResult64 res;
+        std::memset(&res, 0, sizeof(res));
+        res.status = 3;
+        res.bodyPairs = -1;
+        complete(r, res, mono_ns(), 0);
+        nRefused++;
+    }
+
+    void fail_all(std::vector<Compiled>& xs) {
+        for (auto& x : xs) {
+            GpuRuleResult64 res;
+            std::memset(&res, 0, sizeof(res));
+            res.status = kRuleError;
+            res.bodyPairs = -1;
+            complete(x.r, res, mono_ns(), 0);
+        }
+    }
+
+    // Compiled requests -> a batch on slot S. False if nothing was launched.
+    bool launch_batch(Slot* S, std::vector<Compiled>& admitted, int64_t need, bool heavy) {
+        std::vector<RuleIR> rules;
+        std::vector<int32_t> status;
+        rules.reserve(admitted.size());
+        for (auto& x : admitted) {
+            rules.push_back(x.ir);
+            status.push_back(kRuleOk);
+        }
+        S->heavy = heavy;
+        if (!batch_build(*C, S->batch, rules, status, flags, S->st)) {
+            failed = true;
+            S->heavy = false;
+            fail_all(admitted);
+            return false;
+        }
+        if (!launch(*S)) {   // nothing to wait for: the batch's requests fail now
+            failed = true;
+            S->heavy = false;
+            (void)hipStreamSynchronize(S->st);
+            fail_all(admitted);
+            return false;
+        }
+        S->reqs.clear();
+        for (auto& x : admitted) S->reqs.push_back(x.r);
+        S->reserved = need;
+        S->admitNs = mono_ns();
+        S->busy = true;
+        reservedNow += need;
+        nAdmitted += (int64_t)S->reqs.size();
+        nBatches++;
+        if (heavy) nHeavyBatches++;
+        if ((int64_t)S->reqs.size() > maxBatchSeen) maxBatchSeen = (int64_t)S->reqs.size();
+        int64_t inflight = 0;
+        for (auto& s : slots) inflight += s->busy ? 1 : 0;
+        if (inflight > inFlightPeak) inFlightPeak = inflight;
+        return true;
+    }
+
     // Pending requests -> a batch on a free slot. Returns true if launched.
+    // A heavy batch (from heavyPending) goes first when none is in flight and
+    // another slot stays free for light batches (or nothing light waits,
+    // there is one slot only, or the oldest heavy request has waited
+    // kHeavyOverdueNs); light batches divert heavy requests into heavyPending
+    // as they compile them.
     bool admit() {
         Slot* S = nullptr;
-        bool usable = false;
+        int usable = 0, freeSlots = 0;
+        bool heavyInFlight = false;
         for (auto& s : slots) {
             if (s->retired) continue;
-            usable = true;
-            if (!s->busy) { S = s.get(); break; }
+            ++usable;
+            if (s->busy) heavyInFlight |= s->heavy;
+            else {
+                ++freeSlots;
+                if (!S) S = s.get();
+            }
         }
-        if (!usable) {   // every slot retired: nothing can run any more
+        if (usable == 0) {   // every slot retired: nothing can run any more
             failed = true;
             fail_pending(kRuleError);
             return false;
         }
         if (!S) return false;
+        bool lightWaiting;
+        {
+            std::lock_guard<std::mutex> g(mu);
+            lightWaiting = !pending.empty();
+        }
+        const bool idle = reservedNow == 0;
+        // a heavy batch from heavyPending, when the policy allows one now
+        auto try_heavy = [&](bool light) -> bool {
+            if (heavyInFlight || heavyPending.empty()) return false;
+            const bool overdue = mono_ns() - heavyPending.front().r.submitNs > kHeavyOverdueNs;
+            if (!(freeSlots >= 2 || !light || usable == 1 || overdue)) return false;
+            std::vector<Compiled> admitted;
+            int64_t need = 0;
+            while (!heavyPending.empty() && (int32_t)admitted.size() < maxBatch) {
+                const int64_t b = request_bytes(heavyPending.front().cr);
+                if (reservedNow + need + b > budget) {
+                    if (idle && admitted.empty()) {   // can never fit: refuse it
+                        refuse(heavyPending.front().r);
+                        heavyPending.pop_front();
+                        continue;
+                    }
+                    break;   // wait for memory to free up
+                }
+                need += b;
+                admitted.push_back(std::move(heavyPending.front()));
+                heavyPending.pop_front();
+            }
+            return !admitted.empty() && launch_batch(S, admitted, need, true);
+        };
+        if (try_heavy(lightWaiting)) return true;
         std::vector<Request> take;
         {
             std::lock_guard<std::mutex> g(mu);
@@ -265,74 +402,49 @@ struct Server : Handle {
             }
         }
         if (take.empty()) return false;
-        std::vector<RuleIR> rules;
-        std::vector<int32_t> status;
-        std::vector<Request> admitted, back;
+        std::vector<Compiled> admitted;
+        std::vector<Request> back;
         int64_t need = 0;
-        const bool idle = reservedNow == 0;
         for (auto& r : take) {
             if (!back.empty()) { back.push_back(r); continue; }
-            RuleIR ir = rule_from_words(r.rule.words);
-            const CompiledRule cr = compile_rule(ir, C->g, C->tune);
-            const int64_t b = request_bytes(cr);
+            Compiled x;
+            x.r = r;
+            x.ir = rule_from_words(r.rule.words);
+            x.cr = compile_rule(x.ir, C->g, C->tune);
+            if (heavyWork > 0) {
+                x.work = estimate_work(x.cr, C->g, C->rowStats);
+                if (x.work > heavyWork) {
+                    nHeavy++;
+                    heavyPending.push_back(std::move(x));
+                    continue;
+                }
+            }
+            const int64_t b = request_bytes(x.cr);
             if (reservedNow + need + b > budget) {
                 if (idle && admitted.empty()) {   // can never fit: refuse it
-                    GpuRuleResult64 res;
-                    std::memset(&res, 0, sizeof(res));
-                    res.status = 3;
-                    res.bodyPairs = -1;
-                    complete(r, res, mono_ns(), 0);
-                    nRefused++;
+                    refuse(r);
                     continue;
                 }
                 back.push_back(r);   // wait for memory to free up
                 continue;
             }
             need += b;
-            rules.push_back(std::move(ir));
-            status.push_back(kRuleOk);
-            admitted.push_back(r);
+            admitted.push_back(std::move(x));
         }
         if (!back.empty()) {
             nDeferred += (int64_t)back.size();
             std::lock_guard<std::mutex> g(mu);
             for (auto it = back.rbegin(); it != back.rend(); ++it) pending.push_front(*it);
         }
-        if (admitted.empty()) return false;
-        if (!batch_build(*C, S->batch, rules, status, flags, S->st)) {
-            failed = true;
-            for (auto& r : admitted) {
-                GpuRuleResult64 res;
-                std::memset(&res, 0, sizeof(res));
-                res.status = kRuleError;
-                complete(r, res, mono_ns(), 0);
+        if (admitted.empty()) {   // every request taken was heavy (or must wait): the heavy queue's turn
+            bool lightLeft;
+            {
+                std::lock_guard<std::mutex> g(mu);
+                lightLeft = !pending.empty();
             }
-            return false;
+            return try_heavy(lightLeft);
         }
-        if (!launch(*S)) {   // nothing to wait for: the batch's requests fail now
-            failed = true;
-            (void)hipStreamSynchronize(S->st);
-            for (auto& r : admitted) {
-                GpuRuleResult64 res;
-                std::memset(&res, 0, sizeof(res));
-                res.status = kRuleError;
-                res.bodyPairs = -1;
-                complete(r, res, mono_ns(), 0);
-            }
-            return false;
-        }
-        S->reqs = std::move(admitted);
-        S->reserved = need;
-        S->admitNs = mono_ns();
-        S->busy = true;
-        reservedNow += need;
-        nAdmitted += (int64_t)S->reqs.size();
-        nBatches++;
-        if ((int64_t)S->reqs.size() > maxBatchSeen) maxBatchSeen = (int64_t)S->reqs.size();
-        int64_t inflight = 0;
-        for (auto& s : slots) inflight += s->busy ? 1 : 0;
-        if (inflight > inFlightPeak) inFlightPeak = inflight;
-        return true;
+        return launch_batch(S, admitted, need, false);
     }
 
     void loop() {
@@ -343,7 +455,7 @@ struct Server : Handle {
             bool havePending;
             {
                 std::lock_guard<std::mutex> g(mu);
-                havePending = !pending.empty();
+                havePending = !pending.empty() || !heavyPending.empty();
             }
             if (stopping && !havePending && busy == 0) {
                 bool anyBusy = false;
@@ -429,6 +541,14 @@ int gpu_serve_stats(void* srv, int64_t* stats) {
     stats[6] = S->maxBatchSeen;
     stats[7] = S->inFlightPeak;
     return S->failed ? 1 : 0;
+}
+
+int gpu_serve_heavy_stats(void* srv, int64_t* heavyRequests, int64_t* heavyBatches) {
+    Server* S = as_server(srv);
+    if (!S) return -1;
+    if (heavyRequests) *heavyRequests = S->nHeavy;
+    if (heavyBatches) *heavyBatches = S->nHeavyBatches;
+    return 0;
 }
 
 int64_t gpu_serve_slot_bytes(void* srv) {

@@ -85,6 +85,7 @@ _SIGS = {
     "gpu_host_term_id": (ctypes.c_int64, [ctypes.c_void_p, ctypes.c_char_p]),
     "gpu_host_parse_rule": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int32, ctypes.c_void_p]),
     "gpu_host_plan_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "gpu_host_rules_work": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]),
     "gpu_host_count_action": (ctypes.c_int32, [ctypes.c_int32, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64]),
     "gpu_host_term_string": (ctypes.c_int64, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_char_p, ctypes.c_int64]),
     "gpu_rules_version": (ctypes.c_char_p, []),
@@ -198,6 +199,15 @@ class HostGraph:
         out = np.zeros((w.shape[0], 4), np.int64)
         if self.lib.gpu_plan_traffic(self.h, w.ctypes.data, int(w.shape[0]), out.ctypes.data) != 0:
             raise RuntimeError("gpu_plan_traffic failed")
+        return out
+
+    def work(self, rule_words: np.ndarray) -> np.ndarray:
+        """Per rule: estimated candidates its searches enumerate (the
+        figure the server schedules by)."""
+        w = np.ascontiguousarray(rule_words, np.int32)
+        out = np.zeros(w.shape[0], np.float64)
+        if self.lib.gpu_host_rules_work(self.h, w.ctypes.data, int(w.shape[0]), out.ctypes.data) != 0:
+            raise RuntimeError("gpu_host_rules_work failed")
         return out
 
     def close(self):

@@ -26,6 +26,7 @@ _SIGS = {
     "gpu_serve_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "gpu_serve_stop": (None, [ctypes.c_void_p]),
     "gpu_serve_slot_bytes": (ctypes.c_int64, [ctypes.c_void_p]),
+    "gpu_serve_heavy_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "gpu_monotonic_ns": (ctypes.c_int64, []),
 }
 L._SIGS.update(_SIGS)
@@ -75,6 +76,9 @@ class Server:
         rc = self.lib.gpu_serve_stats(self.h, s.ctypes.data)
         d = dict(zip(STAT_NAMES, (int(x) for x in s)))
         d["device_failure"] = rc == 1
+        h = np.zeros(2, np.int64)
+        if self.lib.gpu_serve_heavy_stats(self.h, h[0:].ctypes.data, h[1:].ctypes.data) == 0:
+            d["heavy_requests"], d["heavy_batches"] = int(h[0]), int(h[1])
         return d
 
     def slot_bytes(self) -> int:

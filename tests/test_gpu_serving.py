@@ -114,3 +114,33 @@ def test_poisson_stream(setup):
     assert (r["latency_ms"] > 0).all()
     say("poisson: %.0f rules/s, p50 %.3f ms, p99 %.3f ms, batches %d" % (
         r["rules_per_s"], np.percentile(r["latency_ms"], 50), np.percentile(r["latency_ms"], 99), st["batches"]))
+
+
+def test_heavy_lane(setup):
+    """Requests whose estimated work passes serve_heavy_work run in heavy
+    batches of their own on a reduced grid; every result is unchanged."""
+    say("serve heavy lane")
+    eng, words, ref = setup
+    fx, g, rules = goldens.load("dbpedia_shaped_5k")
+    est = L.HostGraph(g).work(words)
+    assert est.shape[0] == words.shape[0] and (est >= 0).all()
+    thr = int(np.percentile(est[est > 0], 80))
+    n = words.shape[0]
+    try:
+        eng.set_tuning("serve_heavy_work", thr)
+        eng.set_tuning("serve_heavy_percent", 10)
+        srv = S.Server(eng, max_batch=256, max_in_flight=3)
+        try:
+            srv.submit(words, np.arange(n, dtype=np.uint64))
+            out = collect(srv, n)
+            st = srv.stats()
+        finally:
+            srv.stop()
+    finally:
+        eng.set_tuning("serve_heavy_work", 10 ** 11)
+        eng.set_tuning("serve_heavy_percent", 50)
+    say("  threshold %d: %d heavy requests in %d heavy batches of %d" % (
+        thr, st["heavy_requests"], st["heavy_batches"], st["batches"]))
+    assert st["completed"] == n and not st["device_failure"]
+    assert st["heavy_requests"] == int((est > thr).sum()) and st["heavy_batches"] >= 1
+    assert same(out["result"], ref)
