@@ -217,7 +217,7 @@ OneScratch context_scratch(Context& C) { return OneScratch{C.one.as<char>(), nul
 // keys than the buffer holds is redone with a larger one (when S may grow
 // buffers; else it fails).
 bool run_one(Context& C, OneScratch S, const Plan& P, const ConflictRefs& R, const Plan& PW, const ConflictRefs& RW,
-             int64_t items, uint64_t& count, double& ms, int64_t& heavySeeds) {
+             int64_t items, bool cutGiants, uint64_t& count, double& ms, int64_t& heavySeeds) {
     count = 0;
     ms = 0;
     if (items <= 0) return true;
@@ -261,9 +261,10 @@ bool run_one(Context& C, OneScratch S, const Plan& P, const ConflictRefs& R, con
     const int hBits = bits_for((uint64_t)items);
     // Seeds still running after giantAfter wave-mode steps are cut into
     // sub-items that all waves share, as in batch passes (the wave kernel
-    // stores its plan, as rule 0, where the giant kernels read it). The
-    // allocation-free path runs every seed to its end on one wave.
-    const bool giants = S.mayAlloc && C.giantAfter > 0;
+    // stores its plan, as rule 0, where the giant kernels read it), for
+    // rules the caller marks (cutGiants). The allocation-free path runs every
+    // seed to its end on one wave.
+    const bool giants = cutGiants && S.mayAlloc && C.giantAfter > 0;
     Plan* d_plan = reinterpret_cast<Plan*>(S.hdr + kOnePlanOff);
     Plan pw = PW;
     pw.rule = 0;
@@ -353,13 +354,18 @@ bool eval_rule_plain(Context& C, OneScratch S, const RuleIR& r, bool gated, GpuR
     if (cr.status != kRuleOk) return true;
     int64_t heavy = 0;
     uint64_t sup = 0;
-    if (cr.supportLaunch && !run_one(C, S, cr.sup, cr.supRefs, cr.supWave, cr.supWaveRefs, cr.supItems, sup, supMs, heavy))
+    // The giant pass costs two launches per phase; a rule whose estimated
+    // work is below perRuleGiantWork has no seed long enough to pay for them.
+    const bool cut = C.perRuleGiantWork <= 0 || estimate_work(cr, C.g, C.rowStats) >= C.perRuleGiantWork;
+    if (cr.supportLaunch &&
+        !run_one(C, S, cr.sup, cr.supRefs, cr.supWave, cr.supWaveRefs, cr.supItems, cut, sup, supMs, heavy))
         return false;
     res.support = (int64_t)sup;
     if (!gated || sup > 0) {
         uint64_t pairs = 0;
         if (cr.pairsLaunch &&
-            !run_one(C, S, cr.pairs, cr.pairsRefs, cr.pairsWave, cr.pairsWaveRefs, cr.pairItems, pairs, confMs, heavy))
+            !run_one(C, S, cr.pairs, cr.pairsRefs, cr.pairsWave, cr.pairsWaveRefs, cr.pairItems, cut, pairs, confMs,
+                     heavy))
             return false;
         res.bodyPairs = (int64_t)pairs;
     }
@@ -1461,6 +1467,9 @@ int gpu_set_tuning(void* ctx, const char* key, int64_t value) {
         if (!C->ensure_dedup()) return -1;
     } else if (k == "giant_after") {
         C->giantAfter = (int)std::max<int64_t>(0, value);
+    } else if (k == "per_rule_giant_work") {
+        // per-rule launches cut giant seeds for rules of at least this estimated work (0: every rule)
+        C->perRuleGiantWork = (double)std::max<int64_t>(0, value);
     } else if (k == "serve_heavy_work") {
         // servers started afterwards: estimated work above which a request runs in a heavy batch (0: off)
         C->serveHeavyWork = (double)std::max<int64_t>(0, value);
@@ -1514,6 +1523,7 @@ int64_t gpu_get_tuning(void* ctx, const char* key) {
     if (k == "pairs_dedup") return C->pairsDedup();
     if (k == "giant_after") return C->giantAfter;
     if (k == "serve_heavy_work") return (int64_t)C->serveHeavyWork;
+    if (k == "per_rule_giant_work") return (int64_t)C->perRuleGiantWork;
     if (k == "serve_heavy_percent") return C->serveHeavyPercent;
     if (k == "last_giants") return C->lastGiants;
     if (k == "last_fused") return C->lastFused;

@@ -310,6 +310,9 @@ struct Context : Handle {
     double serveHeavyWork = 1e11;
     int serveHeavyPercent = 50;
     RowStats rowStats;       // of g, for the estimates
+    // Per-rule launches cut giant seeds into sub-items only for rules of at
+    // least this estimated work (0: every rule).
+    double perRuleGiantWork = 1e7;
     // Count the support of qualifying rules in the body-pair pass instead of
     // a support pass of their own; needs bitmap dedup (rows or LDS).
     bool fuseSupport = true;

@@ -253,16 +253,19 @@ def test_forced_giant_seeds(engines, name, dedup):
     """giant_after 1 and budget 1: nearly every wave-mode seed is abandoned
     after one step and listed as a giant, cut into sub-items (level-0 or
     level-1 ranges) that all waves run against the giant's shared bitmap
-    row, sort keys or found flag; every count stays exact."""
+    row, sort keys or found flag; every count stays exact. The per-rule
+    launches cut giants too (per_rule_giant_work 0: for every rule)."""
     say("giants", name, dedup)
     eng, fx, g, rules = engine_for(engines, name)
     words = L.rule_words_for_graph(rules, g)
     eng.set_tuning("budget", 1)
     eng.set_tuning("dedup", dedup)
     eng.set_tuning("giant_after", 1)
+    eng.set_tuning("per_rule_giant_work", 0)
     try:
         giants = 0
-        for flags, graph in ((L.UNGATED, 1), (L.UNGATED, 0), (0, 1), (L.UNGATED, 1)):
+        for flags, graph in ((L.UNGATED, 1), (L.UNGATED, 0), (0, 1), (L.UNGATED, 1), (L.UNGATED | L.PER_RULE, 1),
+                             (L.PER_RULE, 1)):
             eng.set_tuning("use_graph", graph)
             res = eng.evaluate(words, flags)
             check(fx, rules, res, not (flags & L.UNGATED), "%s/giant%d/%d" % (name, dedup, flags))
@@ -275,6 +278,7 @@ def test_forced_giant_seeds(engines, name, dedup):
         eng.set_tuning("dedup", 0)
         eng.set_tuning("giant_after", 2048)
         eng.set_tuning("use_graph", 1)
+        eng.set_tuning("per_rule_giant_work", 10 ** 7)
 
 
 @pytest.mark.parametrize("seed", range(3))
@@ -293,11 +297,12 @@ def test_giant_seeds_random_rules(seed):
         words = L.rule_words_for_graph(rules, g)
         eng.set_tuning("budget", 1)
         eng.set_tuning("giant_after", 1)
-        for dedup in (1, 3):
+        eng.set_tuning("per_rule_giant_work", 0)
+        for dedup, flags in ((1, L.UNGATED), (3, L.UNGATED), (3, L.UNGATED | L.PER_RULE), (2, L.UNGATED | L.PER_RULE)):
             eng.set_tuning("dedup", dedup)
-            res = eng.evaluate(words, L.UNGATED)
+            res = eng.evaluate(words, flags)
             for i in range(len(rules)):
-                tag = (seed, dedup, i, W.rule_text(rules[i]))
+                tag = (seed, dedup, flags, i, W.rule_text(rules[i]))
                 assert res[i]["support"] == exp[i][0], tag
                 assert res[i]["bodyPairs"] == exp[i][3], tag
                 assert res[i]["bodySize"] == exp[i][4], tag
