@@ -6,6 +6,7 @@
 #include <chrono>
 #include <climits>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <exception>
 #include <memory>
@@ -121,7 +122,7 @@ bool Context::upload() {
 
 int Context::waveBlocksFor(int slots) const {
     if (pairsDedup() != kDedupLds) return waveBlocks();
-    const size_t blockLds = (wave_static_lds(slots) + (size_t)rowWords * 4 + 511) / 512 * 512;
+    const size_t blockLds = (wave_static_lds(slots, waveEngine) + (size_t)rowWords * 4 + 511) / 512 * 512;
     const int perCU = std::max<int>(1, (int)(ldsPerCU / blockLds));
     return std::max(1, numCUs * std::min(waveWavesPerCU, perCU));
 }
@@ -134,7 +135,7 @@ bool Context::ensure_dedup() {
     rowWords = (maxKeys + 31) / 32;
     const int64_t rowBytes = rowWords * 4;
     rowsFit = rowBytes * numCUs <= kRowBudget;
-    const size_t blockLds = (wave_static_lds() + (size_t)rowBytes + 511) / 512 * 512;   // LDS granules
+    const size_t blockLds = (wave_static_lds(16, waveEngine) + (size_t)rowBytes + 511) / 512 * 512;   // LDS granules
     ldsFits = blockLds <= ldsPerBlock;
     ldsPreferred = ldsFits && ldsPerCU / blockLds >= (size_t)kLdsMinWaves;
     waveSlots = numCUs * waveWavesPerCU;
@@ -277,7 +278,9 @@ bool run_one(Context& C, OneScratch S, const Plan& P, const ConflictRefs& R, con
             giant_args(C.giants, C.giantAfter, ga, 0, z);
             launch_zero(z, C.st);
         } else {
-            launch_fill_u64(d_count, 1, 0, C.st);
+            // both count words: the count is read as their sum, and a giant pass of an
+            // earlier rule leaves its sub-items' count in the second
+            launch_fill_u64(d_count, 2, 0, C.st);
             launch_fill_u32(ctl, 16, 0, C.st);   // control words and the emitted-key counter
         }
         GR_HIP(hipEventRecord(C.ev[0], C.st));
@@ -291,6 +294,7 @@ bool run_one(Context& C, OneScratch S, const Plan& P, const ConflictRefs& R, con
                     d_count, C.dedup.as<uint32_t>(), C.rowWords, sink};
         wa.giant = ga;
         wa.planOut = giants ? d_plan : nullptr;
+        wa.engine = C.waveEngine;
         launch_wave_one(PW.kind, slots, mode, pw, wa, C.dg, C.waveBlocksFor(slots), C.st);
         if (giants) launch_giants(PW.kind, slots, mode, wa, C.dg, C.waveBlocksFor(slots), C.st);
         if (sort) {
@@ -661,6 +665,7 @@ hipError_t batch_enqueue(Context& C, Batch& B, const ExecTarget& t) {
             WaveArgs wa{B.wplans.as<Plan>(), B.wrefs.as<ConflictRefs>(), B.heavy.as<HeavyItem>() + B.heavyOff[ph],
                         c + 1, B.heavyCap[ph], fusePh ? 1u : 0u, c + 2, counts, t.dedup, t.rowWords, sink};
             wa.supTab = B.supTab.as<int32_t>();
+            wa.engine = C.waveEngine;
             const int mode = ph == kPairs ? t.pairsDedup : kDedupRows;
             if (!early[ph]) giant_prepare(*gbp, t.giantAfter, gas[ph], st, ph);
             wa.giant = gas[ph];
@@ -749,6 +754,7 @@ bool batch_run(Context& C, Batch& B, GpuRuleResult64* out) {
                            B.capturedBlocks[1] != C.waveBlocks() || B.capturedSort != sp || B.capturedSortCap != sc ||
                            B.capturedMode != mode || B.capturedGiants != C.giants.rows.p ||
                            B.capturedGiantAfter != C.giantAfter || B.capturedOverlap != (int)C.overlap ||
+                           B.capturedEngine != C.waveEngine ||
                            B.capturedFuse != (int)C.fuseActive(mode))) {
                 (void)hipGraphExecDestroy(B.exec);
                 (void)hipGraphDestroy(B.graph);
@@ -764,6 +770,7 @@ bool batch_run(Context& C, Batch& B, GpuRuleResult64* out) {
                 B.capturedMode = mode;
                 B.capturedGiants = C.giants.rows.p;
                 B.capturedGiantAfter = C.giantAfter;
+                B.capturedEngine = C.waveEngine;
                 B.capturedOverlap = (int)C.overlap;
                 B.capturedFuse = (int)C.fuseActive(mode);
                 GR_HIP(hipStreamBeginCapture(C.st, hipStreamCaptureModeThreadLocal));
@@ -906,6 +913,9 @@ Context* new_context_from(HostGraph&& g) {
     C->g = std::move(g);
     C->tr.reset(new TermResolver(C->g));
     C->rowStats = row_stats(C->g);
+    // GPU_RULES_WAVE_ENGINE (0 or 1) picks the wave-mode engine of new contexts
+    if (const char* we = std::getenv("GPU_RULES_WAVE_ENGINE"))
+        if ((we[0] == '0' || we[0] == '1') && we[1] == 0) C->waveEngine = we[0] - '0';
     // HBM estimate, as the reference checks before uploading (GPU_03/gpu_rules_lib.cu:222-279)
     size_t freeB = 0, totalB = 0;
     if (hipMemGetInfo(&freeB, &totalB) == hipSuccess) {
@@ -1465,6 +1475,13 @@ int gpu_set_tuning(void* ctx, const char* key, int64_t value) {
         delete C->lastBatch;
         C->lastBatch = nullptr;
         if (!C->ensure_dedup()) return -1;
+    } else if (k == "wave_engine") {
+        // wave-mode search: 0 pool of entries, 1 level-synchronous
+        if (value < 0 || value > 1) return -1;
+        C->waveEngine = (int)value;
+        delete C->lastBatch;
+        C->lastBatch = nullptr;
+        if (!C->ensure_dedup()) return -1;
     } else if (k == "giant_after") {
         C->giantAfter = (int)std::max<int64_t>(0, value);
     } else if (k == "per_rule_giant_work") {
@@ -1515,13 +1532,14 @@ int64_t gpu_get_tuning(void* ctx, const char* key) {
     if (k == "num_cus") return C->numCUs;
     if (k == "lds_per_cu") return (int64_t)C->ldsPerCU;
     if (k == "lds_per_block") return (int64_t)C->ldsPerBlock;
-    if (k == "wave_lds_static") return (int64_t)wave_static_lds(4);
-    if (k == "wave_lds_blocks_per_cu") return wave_lds_blocks_per_cu(4, (size_t)C->rowWords * 4);
+    if (k == "wave_lds_static") return (int64_t)wave_static_lds(4, C->waveEngine);
+    if (k == "wave_lds_blocks_per_cu") return wave_lds_blocks_per_cu(4, (size_t)C->rowWords * 4, C->waveEngine);
     if (k == "wave_slots") return C->waveSlots;
     if (k == "dedup") return C->dedupMode;
     if (k == "sort_active") return C->sortActive() ? 1 : 0;
     if (k == "pairs_dedup") return C->pairsDedup();
     if (k == "giant_after") return C->giantAfter;
+    if (k == "wave_engine") return C->waveEngine;
     if (k == "serve_heavy_work") return (int64_t)C->serveHeavyWork;
     if (k == "per_rule_giant_work") return (int64_t)C->perRuleGiantWork;
     if (k == "serve_heavy_percent") return C->serveHeavyPercent;

@@ -78,19 +78,19 @@ void launch_giants(int kind, int slots, int dedup, const WaveArgs& a, const DevG
     }
 }
 
-size_t wave_static_lds(int slots) {
+size_t wave_static_lds(int slots, int engine) {
     switch (slot_class(slots)) {
-        case 4: return SlotKernels<4>::lds_static();
-        case 8: return SlotKernels<8>::lds_static();
-        default: return SlotKernels<16>::lds_static();
+        case 4: return SlotKernels<4>::lds_static(engine);
+        case 8: return SlotKernels<8>::lds_static(engine);
+        default: return SlotKernels<16>::lds_static(engine);
     }
 }
 
-int wave_lds_blocks_per_cu(int slots, size_t dynLds) {
+int wave_lds_blocks_per_cu(int slots, size_t dynLds, int engine) {
     switch (slot_class(slots)) {
-        case 4: return SlotKernels<4>::lds_blocks_per_cu(dynLds);
-        case 8: return SlotKernels<8>::lds_blocks_per_cu(dynLds);
-        default: return SlotKernels<16>::lds_blocks_per_cu(dynLds);
+        case 4: return SlotKernels<4>::lds_blocks_per_cu(dynLds, engine);
+        case 8: return SlotKernels<8>::lds_blocks_per_cu(dynLds, engine);
+        default: return SlotKernels<16>::lds_blocks_per_cu(dynLds, engine);
     }
 }
 

@@ -119,6 +119,7 @@ struct WaveArgs {
     const int32_t* supTab;        // member lists of plans with supPred <= -2 (batch launches)
     Plan* planOut;                // per-rule launches with giants: the by-value kernel stores its plan here
                                   // (plans[0] of the giant kernels)
+    int32_t engine;               // wave-mode search: 0 pool of entries, 1 level-synchronous (search.hpp)
 };
 
 // `kind` is the plan kind (kSupport / kPairs) of every plan the launch sees;
@@ -135,8 +136,8 @@ void launch_wave_one(int kind, int slots, int dedup, const Plan& P, const WaveAr
 // Static LDS of a kDedupLds wave-mode block of the slot class for `slots`
 // (bytes, from the runtime; 16: the largest class), and the dynamic-LDS
 // limit to request for the kDedupLds kernels.
-size_t wave_static_lds(int slots = 16);
-int wave_lds_blocks_per_cu(int slots, size_t dynLds);
+size_t wave_static_lds(int slots = 16, int engine = 0);
+int wave_lds_blocks_per_cu(int slots, size_t dynLds, int engine = 0);
 void prepare_wave_kernels(size_t dynLds);
 // The giant-seed pass that follows a wave-mode batch launch: expansion of
 // the listed giants, then their sub-items (same `a` as the wave launch).
@@ -153,10 +154,10 @@ struct SlotKernels {
                          hipStream_t s);
     static void giants(int kind, int dedup, const WaveArgs& a, const DevGraph& G, int blocks, hipStream_t s);
     static void prepare(size_t dynLds);
-    static size_t lds_static();   // static LDS of the kDedupLds wave kernel (bytes)
+    static size_t lds_static(int engine);   // static LDS of the kDedupLds wave kernel (bytes)
     // resident one-wave blocks per CU of the kDedupLds pairs wave kernel with
     // `dynLds` bytes of dynamic LDS (the runtime's occupancy query), 0 on error
-    static int lds_blocks_per_cu(size_t dynLds);
+    static int lds_blocks_per_cu(size_t dynLds, int engine);
 };
 extern template struct SlotKernels<4>;
 extern template struct SlotKernels<8>;

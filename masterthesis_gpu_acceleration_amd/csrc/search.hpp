@@ -90,6 +90,7 @@ struct ThreadLdsT {
 // touched so far (HBM rows; the list itself is the kernel's `cache` array).
 template <int NL>
 struct WaveLdsT : ThreadLdsT<NL> {
+    static constexpr bool kLevelSync = false;
     int poolV[kMaxSlots];
     int poolLevel;
     int poolNext;
@@ -367,8 +368,8 @@ __device__ __forceinline__ bool bind_seed(const Plan& P, const DevGraph& G, int 
     return true;
 }
 
-template <int NL>
-__device__ __forceinline__ void copy_plan(ThreadLdsT<NL>& W, const Plan* src) {
+template <class Lds>
+__device__ __forceinline__ void copy_plan(Lds& W, const Plan* src) {
     const int32_t* s = reinterpret_cast<const int32_t*>(src);
     int32_t* d = reinterpret_cast<int32_t*>(&W.plan);
     for (int w = lane_id(); w < kPlanWords; w += kWave) d[w] = s[w];
@@ -758,7 +759,8 @@ struct Dedup {
     uint32_t* bits;     // the wave's own bitmap: its HBM row (kDedupRows) or LDS (kDedupLds)
     uint32_t* shared;   // a giant's HBM row, shared by its sub-items (then used instead of bits), or null
     uint32_t* merge;    // LDS bitmaps: a giant's HBM row the wave's bitmap is ORed into when the sub-item ends
-    WaveLdsT<levels_of(NS)>* W;
+    const Plan* plan;   // the wave's plan (LDS)
+    uint32_t* nTouched; // HBM rows: words listed in `cache` so far (LDS)
     int* cache;         // wave_cache_words<D>() words: emitted-object cache (sort) or touched-word list (rows)
     const KeyBlock* blocks;
     const int32_t* crank;
@@ -766,7 +768,7 @@ struct Dedup {
     // construction; the clamp only keeps a wrong plan from writing out of bounds.
     __device__ __forceinline__ int rank(int x) const {
         if (D == kDedupSort) return x;
-        const Plan& P = W->plan;
+        const Plan& P = *plan;
         const int64_t base = (int64_t)(uint32_t)P.hoLo | ((int64_t)P.hoHi << 32);
         const KeyBlock kb = blocks[base + (x >> 6)];
         const uint64_t bit = 1ull << (x & 63);
@@ -778,11 +780,11 @@ struct Dedup {
     // construction), else from the key index (sort path: x itself).
     __device__ __forceinline__ int rank_at(int i, int x) const {
         if (D == kDedupSort) return x;
-        return W->plan.hoByEntry ? crank[i] : rank(x);
+        return plan->hoByEntry ? crank[i] : rank(x);
     }
     // Rank of x if it is a key of the head-object space, else -1.
     __device__ __forceinline__ int rank_key(int x) const {
-        const Plan& P = W->plan;
+        const Plan& P = *plan;
         const int64_t base = (int64_t)(uint32_t)P.hoLo | ((int64_t)P.hoHi << 32);
         const KeyBlock kb = blocks[base + (x >> 6)];
         const uint64_t bit = 1ull << (x & 63);
@@ -808,7 +810,7 @@ struct Dedup {
         if (shared) return (atomicOr(shared + (x >> 5), bit) & bit) ? 0u : 1u;
         const uint32_t old = atomicOr(bits + (x >> 5), bit);
         if (D == kDedupRows && old == 0) {   // list the word for the clear
-            const uint32_t k = atomicAdd(&W->nTouched, 1u);
+            const uint32_t k = atomicAdd(nTouched, 1u);
             if (k < (uint32_t)kTouchCap) cache[k] = x >> 5;
         }
         return (old & bit) ? 0u : 1u;
@@ -937,6 +939,67 @@ __device__ __forceinline__ uint32_t mark_ranks(Dedup<D, NS>& dd, const int (&hr)
     for (int k = 0; k < U; ++k)
         if (!((w[k] >> (hr[k] & 31)) & 1u)) n += dd.insert_r(hr[k]);
     return n;
+}
+
+// The end of a wave-mode seed or sub-item, shared by both wave engines.
+// SUPPORT (and existence-only PAIRS): 0/1, a giant's sub-item counting only
+// if it is the one to set the giant's found flag. PAIRS with bitmap dedup:
+// the fused support of the seed from its bitmap (supOn), then the bitmap is
+// cleared (a giant's LDS sub-item ORs its bitmap into the giant's row first:
+// the bits that OR sets first are the objects it adds); returns the distinct
+// head objects this wave marked first (0 if aborted, and on the sort path,
+// which counts later).
+template <int K, int D, int NS>
+__device__ __forceinline__ uint64_t seed_finish(Dedup<D, NS>& dd, const DevGraph& G, int64_t rowWords, bool pairs,
+                                                bool aborted, bool stopped, bool found, uint32_t* foundFlag,
+                                                uint64_t mine, bool supOn, Frame supRow, const int32_t* supList,
+                                                int supSubject, unsigned long long* counts, uint64_t& supOut,
+                                                unsigned long long (&stats)[kWaveStats], unsigned long long tEnd) {
+    const int lane = lane_id();
+    if (!pairs) {
+        if (aborted || stopped || !__any(found)) return 0;
+        if (!foundFlag) return 1;
+        uint32_t win = 0;
+        if (lane == 0) win = atomicCAS(foundFlag, 0u, 1u) == 0u ? 1u : 0u;
+        return bcast(win);
+    }
+    if (D == kDedupSort) return 0;
+    // clear the wave's own bitmap (a sub-item marked only its giant's row: nothing listed)
+    wave_sync();
+    if (supOn && !aborted && wave_sum(mine) != 0) {
+        const int nm = supList ? supList[0] : 1;
+        for (int m = 0; m < nm; ++m) {
+            const Frame hr = supList ? key_row(G, (int64_t)supList[2 + 2 * m] * G.nBlocks, supSubject) : supRow;
+            const uint64_t h = head_hits_wave(dd, G, hr, nullptr);
+            if (!supList) supOut += h;
+            else if (lane == 0 && h) atomicAdd(counts + 2 * supList[1 + 2 * m] + kSupport, (unsigned long long)h);
+        }
+    }
+    if (D == kDedupLds) {
+        // a giant's sub-item ORs its bitmap into the giant's row: the bits
+        // that OR sets first are the objects it adds (each bit is set first
+        // exactly once over all sub-items)
+        uint64_t fresh = 0;
+        for (int64_t w = lane; w < rowWords; w += kWave) {
+            if (dd.merge) {
+                const uint32_t x = dd.bits[w];
+                if (x) fresh += (uint64_t)__popc(x & ~atomicOr(dd.merge + w, x));
+            }
+            dd.clear_word((uint32_t)w);
+        }
+        wave_sync();
+        if (dd.merge) mine = fresh;
+    } else {
+        const uint32_t nt = *dd.nTouched;
+        if (nt <= (uint32_t)kTouchCap) {
+            for (uint32_t k = lane; k < nt; k += kWave) dd.clear_word((uint32_t)dd.cache[k]);
+        } else {
+            for (int64_t w = lane; w < rowWords; w += kWave) dd.clear_word((uint32_t)w);
+        }
+    }
+    const uint64_t total = wave_sum(mine);
+    GR_PROF_ADD(6, tEnd);
+    return aborted ? 0 : total;
 }
 
 constexpr int kLeafUnroll = 16;  // last-level pool entries an idle lane takes per draw
@@ -1299,50 +1362,8 @@ __device__ __forceinline__ uint64_t wave_run(WaveLdsT<levels_of(NS)>& W, const D
     stats[4] += nDraw;
     stats[5] += busyLanes;
     stats[6] += drawLanes;
-    if (!pairs) {
-        if (aborted || stopped || !__any(found)) return 0;
-        if (!foundFlag) return 1;
-        uint32_t win = 0;
-        if (lane == 0) win = atomicCAS(foundFlag, 0u, 1u) == 0u ? 1u : 0u;
-        return bcast(win);
-    }
-    if (D == kDedupSort) return 0;
-    // clear the wave's own bitmap (a sub-item marked only its giant's row: nothing listed)
-    wave_sync();
-    if (supOn && !aborted && wave_sum(mine) != 0) {
-        const int nm = supList ? supList[0] : 1;
-        for (int m = 0; m < nm; ++m) {
-            const Frame hr = supList ? key_row(G, (int64_t)supList[2 + 2 * m] * G.nBlocks, supSubject) : supRow;
-            const uint64_t h = head_hits_wave(dd, G, hr, nullptr);
-            if (!supList) supOut += h;
-            else if (lane == 0 && h) atomicAdd(counts + 2 * supList[1 + 2 * m] + kSupport, (unsigned long long)h);
-        }
-    }
-    if (D == kDedupLds) {
-        // a giant's sub-item ORs its bitmap into the giant's row: the bits
-        // that OR sets first are the objects it adds (each bit is set first
-        // exactly once over all sub-items)
-        uint64_t fresh = 0;
-        for (int64_t w = lane; w < rowWords; w += kWave) {
-            if (dd.merge) {
-                const uint32_t x = dd.bits[w];
-                if (x) fresh += (uint64_t)__popc(x & ~atomicOr(dd.merge + w, x));
-            }
-            dd.clear_word((uint32_t)w);
-        }
-        wave_sync();
-        if (dd.merge) mine = fresh;
-    } else {
-        const uint32_t nt = W.nTouched;
-        if (nt <= (uint32_t)kTouchCap) {
-            for (uint32_t k = lane; k < nt; k += kWave) dd.clear_word((uint32_t)dd.cache[k]);
-        } else {
-            for (int64_t w = lane; w < rowWords; w += kWave) dd.clear_word((uint32_t)w);
-        }
-    }
-    const uint64_t total = wave_sum(mine);
-    GR_PROF_ADD(6, tEnd);
-    return aborted ? 0 : total;
+    return seed_finish<K, D, NS>(dd, G, rowWords, pairs, aborted, stopped, found, foundFlag, mine, supOn, supRow,
+                                 supList, supSubject, counts, supOut, stats, tEnd);
 }
 
 // Sets up the pool of heavy seed `item` (its level-0 row) and runs it.
@@ -1379,6 +1400,505 @@ __device__ __forceinline__ uint64_t wave_seed(WaveLdsT<levels_of(NS)>& W, const 
     GR_PROF_ADD(1, tp);
     return wave_run<K, D, NS>(W, G, R, dd, rowWords, sink, keyHi, ga, ga.after > 0, HeavyItem{planIdx, item}, heavyIdx,
                           nullptr, aborted, supOn, supRow, supList, uniform(v[0]), counts, sup, stats);
+}
+
+// ------------------------------------------------- level-synchronous wave mode
+//
+// The second wave-mode engine (WaveArgs::engine = 1). The pool engine above
+// gives every lane a subtree of its own, so the lanes of a wave stand at
+// different levels: a step runs the code of every level some lane is at, and
+// each lane's step is a chain of dependent loads. Here the whole wave expands
+// one level per step:
+//   * every level keeps a ring of open ITEMS: an entry range of the level
+//     (an EXT/ISECT/SCAN row, a CHECK's match), the bindings above it and,
+//     below the pair level, the rank of the head object already bound;
+//   * a step takes the deepest level with items -- depth first, so a ring
+//     never holds more than the wave's children of one step -- or the level
+//     above it while the deepest has less work than kLsFill candidates and
+//     room for more, and deals that level's next entries to the lanes, one
+//     per lane, balanced over its items (an inclusive scan of the items'
+//     remaining lengths and a binary search of it per lane): every lane runs
+//     the same level's code on a candidate of its own;
+//   * candidates that bind become items of the next level, appended in lane
+//     order, or, at the last level, groundings: marks in the seed's dedup set
+//     (PAIRS) or the end of the search (existence);
+//   * a last-level item of a plain EXT binding the head object that is at
+//     least kLsStream entries long is streamed by the whole wave with the
+//     straight-line leaf test, up to kLeafUnroll entries per lane per step;
+//   * below the pair level, an item whose head object is counted already is
+//     dropped, as the pool engine drops such subtrees.
+// The dedup set, giants, found flags and the end of a seed are shared with
+// the pool engine.
+
+// Open items per level: at most one per lane, and a power of two.
+constexpr int ls_cap(int ns) { return ns <= 4 ? 64 : (ns <= 8 ? 32 : 16); }
+constexpr int kLsStream = 512;
+constexpr int kLsFill = 32;
+constexpr int kLsLeafChunk = 8;   // last-level entries a lane takes from one item per step (chunked leaf steps)
+
+template <int NS>
+struct LsLdsT {
+    static constexpr bool kLevelSync = true;
+    static constexpr int NL = levels_of(NS);
+    static constexpr int C = ls_cap(NS);
+    Plan plan;
+    int i0, e0, h0;           // level 0's one item (a seed's level-0 row, or a sub-item's range there)
+    uint32_t nTouched;        // HBM rows: dedup-row words listed so far
+    int v0[NS];
+    int itI[NL - 1][C];       // levels 1..NL-1, per ring slot: next entry,
+    int itEnd[NL - 1][C];     // end of the range,
+    int itHr[NL - 1][C];      // rank of the head object bound above the level (PAIRS), else -1,
+    int itV[NL - 1][NS][C];   // bindings above the level
+};
+
+template <int NS>
+__device__ __forceinline__ int* ls_i(LsLdsT<NS>& W, int L, int s) { return L == 0 ? &W.i0 : &W.itI[L - 1][s]; }
+template <int NS>
+__device__ __forceinline__ int* ls_e(LsLdsT<NS>& W, int L, int s) { return L == 0 ? &W.e0 : &W.itEnd[L - 1][s]; }
+template <int NS>
+__device__ __forceinline__ int* ls_hr(LsLdsT<NS>& W, int L, int s) { return L == 0 ? &W.h0 : &W.itHr[L - 1][s]; }
+template <int NS>
+__device__ __forceinline__ int* ls_v(LsLdsT<NS>& W, int L, int k, int s) {
+    return L == 0 ? &W.v0[k] : &W.itV[L - 1][k][s];
+}
+
+// Wave-uniform ring state, one value per level (select chains, no scratch).
+template <int NL>
+__device__ __forceinline__ int rget(const int (&a)[NL], int L) {
+    int x = 0;
+#pragma unroll
+    for (int l = 0; l < NL; ++l) x = l == L ? a[l] : x;
+    return x;
+}
+template <int NL>
+__device__ __forceinline__ void radd(int (&a)[NL], int L, int d) {
+#pragma unroll
+    for (int l = 0; l < NL; ++l) a[l] += l == L ? d : 0;
+}
+
+__device__ __forceinline__ int wave_incl_scan(int x) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const int y = __shfl_up(x, off, kWave);
+        if (lane >= off) x += y;
+    }
+    return x;
+}
+
+// The items of level L (ring head h, n of them): lane j < n loads item j
+// and returns its remaining length. With dropBound, an item whose head
+// object is counted already has nothing left to add: it is marked consumed
+// and reported empty.
+template <int D, int NS>
+__device__ __forceinline__ int ls_items(LsLdsT<NS>& W, const Dedup<D, NS>& dd, int L, int h, int n, bool dropBound,
+                                        int& ii, int& ie, int& ihr, int& sj) {
+    constexpr int C = LsLdsT<NS>::C;
+    const int lane = lane_id();
+    sj = (h + lane) & (C - 1);
+    ii = 0;
+    ie = 0;
+    ihr = -1;
+    if (lane < n) {
+        ii = *ls_i(W, L, sj);
+        ie = *ls_e(W, L, sj);
+        ihr = *ls_hr(W, L, sj);
+    }
+    int rem = ie - ii;
+    if (dropBound && rem > 0 && dd.seen_r(ihr)) {
+        rem = 0;
+        *ls_i(W, L, sj) = ie;
+    }
+    return rem;
+}
+
+// Entry range of an item of level lv under bindings v. An ISECT level's two
+// rows both list candidates of its new variable, sorted: the item enumerates
+// the shorter one and searches the other (ls_run tells them apart by which
+// row the item's range lies in).
+template <int NS>
+__device__ __forceinline__ Frame ls_enter(const Level& lv, const DevGraph& G, const int (&v)[NS]) {
+    const Frame r1 = enter_level(lv, G, v);
+    if (level_kind(lv.flags) != kIsect || r1.i >= r1.end) return r1;
+    const Frame r2 = isect_row(lv, G, v);
+    if (r2.i >= r2.end) return Frame{0, 0};
+    return r2.end - r2.i < r1.end - r1.i ? r2 : r1;
+}
+
+// Runs the search from the one item the caller placed at level `start`
+// (ring slot 0), with the same contract as wave_run.
+template <int K, int D, int NS>
+__device__ __forceinline__ uint64_t ls_run(LsLdsT<NS>& W, const DevGraph& G, const ConflictRefs* R, Dedup<D, NS>& dd,
+                                           int64_t rowWords, const SortSink& sink, unsigned long long keyHi,
+                                           const GiantArgs& ga, bool mayGiant, HeavyItem self, uint32_t heavyIdx,
+                                           uint32_t* foundFlag, bool& aborted, bool supOn, Frame supRow,
+                                           const int32_t* supList, int supSubject, unsigned long long* counts,
+                                           uint64_t& supOut, unsigned long long (&stats)[kWaveStats], int start) {
+    constexpr int NL = LsLdsT<NS>::NL;
+    constexpr int C = LsLdsT<NS>::C;
+    const int lane = lane_id();
+    const Plan& P = W.plan;
+    const int nlev = uniform(P.nlev);
+    const int last = nlev - 1;
+    const int pl = uniform(P.pairLevel);
+    const int hoRef = uniform(P.hoRef);
+    const bool pairs = K == kPairs && pl >= 0;
+    const bool fastLeaf = leaf_fast_ok<D>(P, last, pairs);
+    const uint64_t below = (1ull << lane) - 1ull;
+    int hd[NL], ct[NL];
+#pragma unroll
+    for (int l = 0; l < NL; ++l) {
+        hd[l] = 0;
+        ct[l] = l == start ? 1 : 0;
+    }
+    int deep = start;
+    bool found = false, stopped = false;
+    aborted = false;
+    uint64_t mine = 0;
+    int steps = 0;
+    uint32_t nFill = 0, nStream = 0, nBal = 0;
+    uint64_t nCand = 0, nPass = 0;
+    for (;;) {
+        unsigned long long tp = GR_PROF_NOW();
+        while (deep >= start && rget(ct, deep) == 0) --deep;
+        if (deep < start) break;
+        int X = deep;
+        bool streamed = false;
+        if (X == last && fastLeaf) {
+            // a long last-level item: the whole wave streams it
+            const int s = rget(hd, X) & (C - 1);
+            const int i0 = uniform(*ls_i(W, X, s));
+            const int e0 = uniform(*ls_e(W, X, s));
+            if (e0 - i0 >= kLsStream) {
+                streamed = true;
+                int v[NS];
+#pragma unroll
+                for (int k = 0; k < NS; ++k) v[k] = uniform(*ls_v(W, X, k, s));
+                const int lim = e0 - i0 > kWave * kLeafUnroll ? i0 + kWave * kLeafUnroll : e0;
+                int is[kLeafUnroll], xs[kLeafUnroll], rs[kLeafUnroll];
+                bool in[kLeafUnroll];
+#pragma unroll
+                for (int k = 0; k < kLeafUnroll; ++k) {
+                    is[k] = i0 + lane + kWave * k;
+                    in[k] = is[k] < lim;
+                }
+                const uint32_t m = leaf_fast(G, v, level_nbound(P.lev[last].flags), pairs, is, in, i0, xs, rs);
+                if (!pairs) found = found || m != 0u;
+                else mine += mark_ranks<D, kLeafUnroll>(dd, rs);
+                nCand += (uint64_t)(lim - i0);
+                if (lane == 0) *ls_i(W, X, s) = lim;
+                if (lim == e0) {
+                    radd(hd, X, 1);
+                    radd(ct, X, -1);
+                }
+                ++nStream;
+                GR_PROF_ADD(2, tp);
+            }
+        }
+        if (!streamed && X == last && fastLeaf) {
+            // Last level, plain EXT: the items are cut into chunks of kLsLeafChunk
+            // entries and each lane takes one chunk (the wave takes the first 64
+            // chunks in item order), so a lane's candidates share one item's
+            // bindings and up to 64 * kLsLeafChunk candidates are tested per step.
+            streamed = true;
+            constexpr int U = kLsLeafChunk;
+            int ii, ie, ihr, sj;
+            const int n = rget(ct, X);
+            const int rem = ls_items(W, dd, X, rget(hd, X), n, false, ii, ie, ihr, sj);
+            const int cnk = (rem + U - 1) / U;
+            const int incl = wave_incl_scan(cnk);
+            const int totalC = uniform(__shfl(incl, kWave - 1, kWave));
+            const int TC = totalC < kWave ? totalC : kWave;
+            const int excl = incl - cnk;
+            int tookC = TC - excl;
+            tookC = tookC < 0 ? 0 : (tookC > cnk ? cnk : tookC);
+            const int took = tookC * U < rem ? tookC * U : rem;
+            if (lane < n && took > 0) *ls_i(W, X, sj) = ii + took;
+            const uint64_t doneM = __ballot(lane < n && rem == took);
+            const int pre = doneM == ~0ull ? kWave : (int)__builtin_ctzll(~doneM);
+            int lo = 0, hi = n - 1;
+#pragma unroll
+            for (int it = 0; it < 6; ++it) {
+                const int mid = (lo + hi) >> 1;
+                const int im = __shfl(incl, mid, kWave);
+                if (im > lane) hi = mid;
+                else lo = mid + 1;
+            }
+            const int j = lo < n ? lo : n - 1;
+            const int jx = __shfl(excl, j, kWave);
+            const int ji = __shfl(ii, j, kWave);
+            const int je = __shfl(ie, j, kWave);
+            const int js = __shfl(sj, j, kWave);
+            const bool act = lane < TC;
+            const int start = act ? ji + (lane - jx) * U : 0;
+            const int end = act ? (start + U < je ? start + U : je) : 0;
+            int vb[NS];
+#pragma unroll
+            for (int k = 0; k < NS; ++k) vb[k] = act ? *ls_v(W, X, k, js) : -1;
+            int is[U], xs[U], rs[U];
+            bool in[U];
+#pragma unroll
+            for (int k = 0; k < U; ++k) {
+                is[k] = start + k;
+                in[k] = is[k] < end;
+            }
+            const uint32_t m = leaf_fast(G, vb, level_nbound(P.lev[last].flags), pairs, is, in, start, xs, rs);
+            if (!pairs) found = found || m != 0u;
+            else mine += mark_ranks<D, U>(dd, rs);
+            nCand += (uint64_t)(TC * U);   // candidate slots dealt (the last chunk of an item may be short)
+            radd(hd, X, pre);
+            radd(ct, X, -pre);
+            ++nBal;
+            GR_PROF_ADD(3, tp);
+        }
+        if (!streamed) {
+            int ii, ie, ihr, sj;
+            int rem = ls_items(W, dd, X, rget(hd, X), rget(ct, X), pairs && pl < X, ii, ie, ihr, sj);
+            int incl = wave_incl_scan(rem);
+            int total = uniform(__shfl(incl, kWave - 1, kWave));
+            if (total < kLsFill && X > start && rget(ct, X - 1) > 0 && C - rget(ct, X) >= kLsFill) {
+                // little work at the deepest level and room for more: expand the level above first
+                --X;
+                rem = ls_items(W, dd, X, rget(hd, X), rget(ct, X), pairs && pl < X, ii, ie, ihr, sj);
+                incl = wave_incl_scan(rem);
+                total = uniform(__shfl(incl, kWave - 1, kWave));
+                ++nFill;
+            }
+            const int n = rget(ct, X);
+            const int room = X < last ? (C - rget(ct, X + 1) < kWave ? C - rget(ct, X + 1) : kWave) : kWave;
+            const int T = total < room ? total : room;
+            const int excl = incl - rem;
+            int took = T - excl;
+            took = took < 0 ? 0 : (took > rem ? rem : took);
+            if (lane < n && took > 0) *ls_i(W, X, sj) = ii + took;
+            const uint64_t doneM = __ballot(lane < n && rem == took);
+            const int pre = doneM == ~0ull ? kWave : (int)__builtin_ctzll(~doneM);
+            // the item of this lane's candidate: the first j with incl_j > lane
+            int lo = 0, hi = n - 1;
+#pragma unroll
+            for (int it = 0; it < 6; ++it) {
+                const int mid = (lo + hi) >> 1;
+                const int im = __shfl(incl, mid, kWave);
+                if (im > lane) hi = mid;
+                else lo = mid + 1;
+            }
+            const int j = lo < n ? lo : n - 1;
+            const int jx = __shfl(excl, j, kWave);
+            const int ji = __shfl(ii, j, kWave);
+            const int jhr = __shfl(ihr, j, kWave);
+            const int js = __shfl(sj, j, kWave);
+            const bool act = lane < T;
+            const int e = act ? ji + (lane - jx) : 0;
+            const Level& lv = P.lev[X];
+            const int kind = uniform(level_kind(lv.flags));
+            // ISECT: the row each item searches (the one it does not enumerate), looked
+            // up by the lane holding the item
+            int r2i = 0, r2e = 0, swp = 0;
+            if (kind == kIsect) {
+                Frame other{0, 0};
+                int sw = 0;
+                if (lane < n && rem > 0) {
+                    int vj[NS];
+#pragma unroll
+                    for (int k = 0; k < NS; ++k) vj[k] = *ls_v(W, X, k, sj);
+                    const Frame r1 = enter_level(lv, G, vj);
+                    sw = (ii >= r1.i && ie <= r1.end) ? 0 : 1;   // enumerating row 2
+                    other = sw ? r1 : isect_row(lv, G, vj);
+                }
+                r2i = __shfl(other.i, j, kWave);
+                r2e = __shfl(other.end, j, kWave);
+                swp = __shfl(sw, j, kWave);
+            }
+            int v[NS];
+#pragma unroll
+            for (int k = 0; k < NS; ++k) v[k] = act ? *ls_v(W, X, k, js) : -1;
+            const int x = G.col[e];
+            bool ok = act && bind_entry_x(lv, G, R, v, e, x);
+            int at = -1;    // ISECT: the match's entry in row 2 (the CHECK level's row)
+            int e1 = e;     // the candidate's entry in the level's own row (row 1)
+            if (kind == kIsect && ok) {
+                const int m = find_in_row(G.col, r2i, r2e, x);
+                ok = m >= 0;
+                at = swp ? e : m;
+                e1 = swp ? m : e;
+            }
+            int hr = jhr;
+            if (pairs && X == pl && ok) {
+                hr = dd.rank_at(e1, term_value(v, hoRef));
+                ok = !dd.seen_r(hr);
+            }
+            nCand += (uint64_t)T;
+            nPass += (uint64_t)__popcll(__ballot(ok));
+            if (X == last) {
+                if (!pairs) {
+                    found = found || ok;
+                } else if (D == kDedupSort) {
+                    int emit = -1;
+                    if (ok) {
+                        const int ho = term_value(v, hoRef);
+                        if (!dd.seen(ho)) {
+                            dd.insert(ho);
+                            emit = ho;
+                        }
+                    }
+                    emit_keys(sink, keyHi, emit);
+                } else if (ok) {
+                    mine += dd.insert_r(hr);
+                }
+            } else {
+                int c = X + 1;
+                Frame f{0, 0};
+                if (ok) {
+                    const Level& lc = P.lev[c];
+                    f = (level_kind(lc.flags) == kCheck && kind == kIsect) ? Frame{at, at + 1} : ls_enter(lc, G, v);
+                }
+                bool has = ok && f.i < f.end;
+                // A CHECK level without conflict checks binds nothing, and its one entry
+                // (the match enter_level found) always passes: go on to the level below
+                // it at once, or, below the last level, count the grounding.
+                bool ground = false;
+                while (has && level_kind(P.lev[c].flags) == kCheck && level_conflicts(P.lev[c].flags) == 0) {
+                    if (c == last) {
+                        ground = true;
+                        has = false;
+                        break;
+                    }
+                    ++c;
+                    f = ls_enter(P.lev[c], G, v);
+                    has = f.i < f.end;
+                }
+                // the head object of such a grounding is bound at or above X (no CHECK binds it)
+                if (!pairs) {
+                    found = found || ground;
+                } else if (D == kDedupSort) {
+                    int emit = -1;
+                    if (ground) {
+                        const int ho = term_value(v, hoRef);
+                        if (!dd.seen(ho)) {
+                            dd.insert(ho);
+                            emit = ho;
+                        }
+                    }
+                    emit_keys(sink, keyHi, emit);
+                } else if (ground) {
+                    mine += dd.insert_r(hr);
+                }
+                for (int cc = X + 1; cc <= last; ++cc) {   // children, appended level by level in lane order
+                    const uint64_t cm = __ballot(has && c == cc);
+                    if (cm == 0) continue;
+                    if (has && c == cc) {
+                        const int s = (rget(hd, cc) + rget(ct, cc) + (int)__popcll(cm & below)) & (C - 1);
+                        *ls_i(W, cc, s) = f.i;
+                        *ls_e(W, cc, s) = f.end;
+                        *ls_hr(W, cc, s) = hr;
+#pragma unroll
+                        for (int k = 0; k < NS; ++k) *ls_v(W, cc, k, s) = v[k];
+                    }
+                    radd(ct, cc, (int)__popcll(cm));
+                    if (cc > deep) deep = cc;
+                }
+            }
+            radd(hd, X, pre);
+            radd(ct, X, -pre);
+            ++nBal;
+            GR_PROF_ADD(X == last ? 3 : 4, tp);
+        }
+        tp = GR_PROF_NOW();
+        wave_sync();
+        ++steps;
+        if (!pairs && __any(found)) break;
+        if (mayGiant && steps >= ga.after) {
+            mayGiant = false;   // one try per seed
+            uint32_t slot = 0xffffffffu;
+            if (lane == 0) {
+                const uint32_t gi = atomicAdd(ga.count, 1u);
+                if (gi < ga.cap) {
+                    ga.list[gi] = self;
+                    ga.heavyIdx[gi] = heavyIdx;
+                    slot = gi;
+                }
+            }
+            if (bcast(slot) != 0xffffffffu) {
+                aborted = true;
+                break;
+            }
+        }
+        if (foundFlag && (steps & 31) == 0) {   // another sub-item of this giant found a grounding
+            uint32_t f = lane == 0 ? ld_rlx(foundFlag) : 0u;
+            if (bcast(f)) {
+                stopped = true;
+                break;
+            }
+        }
+        GR_PROF_ADD(5, tp);
+    }
+    const unsigned long long tEnd = GR_PROF_NOW();
+    stats[0] += 1;
+    stats[1] += (unsigned long long)steps;
+    stats[2] += nFill;
+    stats[3] += nStream;
+    stats[4] += nBal;
+    stats[5] += nCand;
+    stats[6] += nPass;
+    return seed_finish<K, D, NS>(dd, G, rowWords, pairs, aborted, stopped, found, foundFlag, mine, supOn, supRow,
+                                 supList, supSubject, counts, supOut, stats, tEnd);
+}
+
+// Places heavy seed `item` (its level-0 row) as level 0's item and runs it.
+template <int K, int D, int NS>
+__device__ __forceinline__ uint64_t ls_seed(LsLdsT<NS>& W, const DevGraph& G, const ConflictRefs* R, uint32_t item,
+                                            Dedup<D, NS>& dd, int64_t rowWords, const SortSink& sink,
+                                            unsigned long long keyHi, const GiantArgs& ga, uint32_t planIdx,
+                                            uint32_t heavyIdx, bool fuse, uint64_t& sup, unsigned long long* counts,
+                                            const int32_t* supTab, unsigned long long (&stats)[kWaveStats]) {
+    const int lane = lane_id();
+    const Plan& P = W.plan;
+    const unsigned long long tp = GR_PROF_NOW();
+    int v[NS];
+    if (!bind_seed(P, G, (int)item, v)) return 0;
+    const Frame f0 = ls_enter(P.lev[0], G, v);
+    if (f0.i >= f0.end) return 0;
+    if (lane < NS) W.v0[lane] = term_value(v, lane);
+    if (lane == 0) {
+        W.i0 = f0.i;
+        W.e0 = f0.end;
+        W.h0 = -1;
+        W.nTouched = 0;
+    }
+    if (D == kDedupSort && K == kPairs)
+        for (int k = lane; k < kCacheSlots; k += kWave) dd.cache[k] = -1;
+    wave_sync();
+    bool aborted = false;
+    const bool supOn = K == kPairs && D != kDedupSort && fuse && P.supPred != -1;
+    const int32_t* supList = (supOn && P.supPred <= -2) ? supTab + (-P.supPred - 2) : nullptr;
+    const Frame supRow = (supOn && !supList) ? key_row(G, sup_base(P, G), v[0]) : Frame{0, 0};
+    GR_PROF_ADD(1, tp);
+    return ls_run<K, D, NS>(W, G, R, dd, rowWords, sink, keyHi, ga, ga.after > 0, HeavyItem{planIdx, item}, heavyIdx,
+                            nullptr, aborted, supOn, supRow, supList, uniform(v[0]), counts, sup, stats, 0);
+}
+
+// A giant's sub-item as the one item of its level (ring slot 0): entries
+// [begin, end) of `level` with the bindings above it; returns the level.
+template <int K, int D, int NS>
+__device__ __forceinline__ int ls_place_sub(LsLdsT<NS>& W, const Dedup<D, NS>& dd, const SubItem& S) {
+    const int lane = lane_id();
+    const Plan& P = W.plan;
+    const int lvl = uniform(S.level);
+    const int pl = uniform(P.pairLevel);
+    int hr = -1;
+    if (K == kPairs && pl >= 0 && pl < lvl) {   // the head object is bound above the sub-item's level
+        const int ref = uniform(P.hoRef);
+        const int hov = ref >= 0 ? uniform(S.v[ref]) : -(ref + 1);
+        hr = uniform(dd.rank(hov));
+    }
+    if (lane < NS) *ls_v(W, lvl, lane, 0) = S.v[lane];
+    if (lane == 0) {
+        *ls_i(W, lvl, 0) = S.begin;
+        *ls_e(W, lvl, 0) = S.end;
+        *ls_hr(W, lvl, 0) = hr;
+        W.nTouched = 0;
+    }
+    return lvl;
 }
 
 // ------------------------------------------------------------- kernels
@@ -1437,11 +1957,12 @@ __global__ void __launch_bounds__(kBlock) k_thread_one(Plan P, OneArgs a, DevGra
 // Dedup state of one wave-mode wave: D selects the bitmap in HBM (this
 // wave's row), the bitmap in LDS (dynamic shared memory, cleared once here)
 // or the sort path's LDS cache.
-template <int D, int NS>
-__device__ __forceinline__ Dedup<D, NS> wave_dedup(WaveLdsT<levels_of(NS)>& W, int* cache, uint32_t* dyn,
-                                                   const WaveArgs& a, const DevGraph& G) {
+template <int D, int NS, class Lds>
+__device__ __forceinline__ Dedup<D, NS> wave_dedup(Lds& W, int* cache, uint32_t* dyn, const WaveArgs& a,
+                                                   const DevGraph& G) {
     Dedup<D, NS> dd;
-    dd.W = &W;
+    dd.plan = &W.plan;
+    dd.nTouched = &W.nTouched;
     dd.cache = cache;
     dd.bits = nullptr;
     dd.shared = nullptr;
@@ -1453,7 +1974,8 @@ __device__ __forceinline__ Dedup<D, NS> wave_dedup(WaveLdsT<levels_of(NS)>& W, i
         dd.bits = dyn;
         for (int64_t w = lane_id(); w < a.rowWords; w += kWave) dyn[w] = 0u;
     }
-    if (lane_id() == 0) W.poolGen = 0;
+    if constexpr (!Lds::kLevelSync)
+        if (lane_id() == 0) W.poolGen = 0;
     wave_sync();
     return dd;
 }
@@ -1465,11 +1987,36 @@ __device__ __forceinline__ void flush_stats(const GiantArgs& ga, const unsigned 
             if (stats[q]) atomicAdd(ga.stats + q, stats[q]);
 }
 
+// LDS of a wave-mode block under engine E (0: pool, 1: level-synchronous).
+template <int E, int NS>
+struct WaveLdsOf {
+    using T = WaveLdsT<levels_of(NS)>;
+};
+template <int NS>
+struct WaveLdsOf<1, NS> {
+    using T = LsLdsT<NS>;
+};
+
+// One heavy seed under engine E.
+template <int K, int D, int NS, class Lds>
+__device__ __forceinline__ uint64_t run_seed(Lds& W, const DevGraph& G, const ConflictRefs* R, uint32_t item,
+                                             Dedup<D, NS>& dd, int64_t rowWords, const SortSink& sink,
+                                             unsigned long long keyHi, const GiantArgs& ga, uint32_t planIdx,
+                                             uint32_t heavyIdx, bool fuse, uint64_t& sup, unsigned long long* counts,
+                                             const int32_t* supTab, unsigned long long (&stats)[kWaveStats]) {
+    if constexpr (Lds::kLevelSync)
+        return ls_seed<K, D, NS>(W, G, R, item, dd, rowWords, sink, keyHi, ga, planIdx, heavyIdx, fuse, sup, counts,
+                                 supTab, stats);
+    else
+        return wave_seed<K, D, NS>(W, G, R, item, dd, rowWords, sink, keyHi, ga, planIdx, heavyIdx, fuse, sup, counts,
+                                   supTab, stats);
+}
+
 // Persistent wave mode over the heavy queue: one wave per block, one heavy
 // seed at a time.
-template <int K, int D, int NS>
+template <int K, int D, int NS, int E>
 __global__ void __launch_bounds__(kWave) k_wave_batch(WaveArgs a, DevGraph G) {
-    __shared__ WaveLdsT<levels_of(NS)> W;
+    __shared__ typename WaveLdsOf<E, NS>::T W;
     __shared__ int cache[wave_cache_words<D>()];
     extern __shared__ uint32_t dyn[];
     const int lane = lane_id();
@@ -1490,18 +2037,18 @@ __global__ void __launch_bounds__(kWave) k_wave_batch(WaveArgs a, DevGraph G) {
         }
         GR_PROF_ADD(0, tp);
         uint64_t sup = 0;
-        const uint64_t c = wave_seed<K, D, NS>(W, G, a.refs + it.plan, it.item, dd, a.rowWords, a.sink,
-                                           (unsigned long long)h << a.sink.hoBits, a.giant, it.plan, h, a.fuse != 0,
-                                           sup, a.counts, a.supTab, stats);
+        const uint64_t c = run_seed<K, D, NS>(W, G, a.refs + it.plan, it.item, dd, a.rowWords, a.sink,
+                                              (unsigned long long)h << a.sink.hoBits, a.giant, it.plan, h,
+                                              a.fuse != 0, sup, a.counts, a.supTab, stats);
         if (lane == 0 && c) atomicAdd(a.counts + 2 * W.plan.rule + K, (unsigned long long)c);
         if (lane == 0 && sup) atomicAdd(a.counts + 2 * W.plan.rule + kSupport, (unsigned long long)sup);
     }
     flush_stats(a.giant, stats);
 }
 
-template <int K, int D, int NS>
+template <int K, int D, int NS, int E>
 __global__ void __launch_bounds__(kWave) k_wave_one(Plan P, WaveArgs a, DevGraph G) {
-    __shared__ WaveLdsT<levels_of(NS)> W;
+    __shared__ typename WaveLdsOf<E, NS>::T W;
     __shared__ int cache[wave_cache_words<D>()];
     extern __shared__ uint32_t dyn[];
     const int lane = lane_id();
@@ -1520,9 +2067,9 @@ __global__ void __launch_bounds__(kWave) k_wave_one(Plan P, WaveArgs a, DevGraph
         const uint32_t h = wave_take(a.cursor);
         if (h >= n) break;
         uint64_t sup = 0;
-        const uint64_t c = wave_seed<K, D, NS>(W, G, a.refs, a.heavy[h].item, dd, a.rowWords, a.sink,
-                                           (unsigned long long)h << a.sink.hoBits, a.giant, 0u, h, false, sup, nullptr,
-                                           nullptr, stats);
+        const uint64_t c = run_seed<K, D, NS>(W, G, a.refs, a.heavy[h].item, dd, a.rowWords, a.sink,
+                                              (unsigned long long)h << a.sink.hoBits, a.giant, 0u, h, false, sup,
+                                              nullptr, nullptr, stats);
         if (lane == 0 && c) atomicAdd(a.counts, (unsigned long long)c);
     }
     flush_stats(a.giant, stats);
@@ -1628,9 +2175,9 @@ __global__ void __launch_bounds__(kWave) k_giant_expand(GiantArgs ga, const Plan
 // and bindings) run by wave_run against the giant's shared row or found
 // flag. The sub-item that finishes a giant's last clears its row, after
 // counting the seed's supported head triples from it (fused support).
-template <int K, int D, int NS>
+template <int K, int D, int NS, int E>
 __global__ void __launch_bounds__(kWave) k_wave_sub(WaveArgs a, DevGraph G) {
-    __shared__ WaveLdsT<levels_of(NS)> W;
+    __shared__ typename WaveLdsOf<E, NS>::T W;
     __shared__ int cache[wave_cache_words<D>()];
     extern __shared__ uint32_t dyn[];
     const int lane = lane_id();
@@ -1660,13 +2207,18 @@ __global__ void __launch_bounds__(kWave) k_wave_sub(WaveArgs a, DevGraph G) {
         if (foundFlag && lane == 0) skip = ld_rlx(foundFlag);
         uint64_t c = 0;
         if (!bcast(skip)) {
-            if (lane < kMaxSlots) W.poolV[lane] = S.v[lane];
-            if (lane == 0) {
-                W.poolLevel = S.level;
-                W.poolNext = S.begin;
-                W.poolEnd = S.end;
-                W.poolGen = W.poolGen + 1u;
-                W.nTouched = 0;
+            int start = 0;
+            if constexpr (E == 1) {
+                start = ls_place_sub<K, D, NS>(W, dd, S);
+            } else {
+                if (lane < kMaxSlots) W.poolV[lane] = S.v[lane];
+                if (lane == 0) {
+                    W.poolLevel = S.level;
+                    W.poolNext = S.begin;
+                    W.poolEnd = S.end;
+                    W.poolGen = W.poolGen + 1u;
+                    W.nTouched = 0;
+                }
             }
             if (D == kDedupSort && K == kPairs)
                 for (int k = lane; k < kCacheSlots; k += kWave) cache[k] = -1;
@@ -1676,9 +2228,14 @@ __global__ void __launch_bounds__(kWave) k_wave_sub(WaveArgs a, DevGraph G) {
             dd.merge = (K == kPairs && D == kDedupLds) ? row : nullptr;     // marks go to LDS, merged at the end
             bool aborted = false;
             uint64_t none = 0;
-            c = wave_run<K, D, NS>(W, G, a.refs + plan, dd, a.rowWords, a.sink, (unsigned long long)hv << a.sink.hoBits,
-                               ga, false, HeavyItem{plan, 0u}, hv, foundFlag, aborted, false, Frame{0, 0}, nullptr,
-                               0, nullptr, none, stats);
+            if constexpr (E == 1)
+                c = ls_run<K, D, NS>(W, G, a.refs + plan, dd, a.rowWords, a.sink,
+                                     (unsigned long long)hv << a.sink.hoBits, ga, false, HeavyItem{plan, 0u}, hv,
+                                     foundFlag, aborted, false, Frame{0, 0}, nullptr, 0, nullptr, none, stats, start);
+            else
+                c = wave_run<K, D, NS>(W, G, a.refs + plan, dd, a.rowWords, a.sink,
+                                       (unsigned long long)hv << a.sink.hoBits, ga, false, HeavyItem{plan, 0u}, hv,
+                                       foundFlag, aborted, false, Frame{0, 0}, nullptr, 0, nullptr, none, stats);
             dd.shared = nullptr;
             dd.merge = nullptr;
         }
@@ -1731,74 +2288,101 @@ void SlotKernels<NS>::thread_one(int kind, const Plan& P, const OneArgs& a, cons
     else hipLaunchKernelGGL((k_thread_one<kPairs, NS>), dim3(blocks), dim3(kBlock), 0, s, P, a, G);
 }
 
+template <int NS, int E>
+static void wave_batch_e(int kind, int dedup, const WaveArgs& a, const DevGraph& G, int blocks, hipStream_t s) {
+    if (kind == kSupport)
+        hipLaunchKernelGGL((k_wave_batch<kSupport, kDedupRows, NS, E>), dim3(blocks), dim3(kWave), 0, s, a, G);
+    else if (dedup == kDedupSort)
+        hipLaunchKernelGGL((k_wave_batch<kPairs, kDedupSort, NS, E>), dim3(blocks), dim3(kWave), 0, s, a, G);
+    else if (dedup == kDedupLds)
+        hipLaunchKernelGGL((k_wave_batch<kPairs, kDedupLds, NS, E>), dim3(blocks), dim3(kWave), (size_t)a.rowWords * 4,
+                           s, a, G);
+    else
+        hipLaunchKernelGGL((k_wave_batch<kPairs, kDedupRows, NS, E>), dim3(blocks), dim3(kWave), 0, s, a, G);
+}
+
 template <int NS>
 void SlotKernels<NS>::wave_batch(int kind, int dedup, const WaveArgs& a, const DevGraph& G, int blocks,
                                  hipStream_t s) {
+    if (a.engine == 1) wave_batch_e<NS, 1>(kind, dedup, a, G, blocks, s);
+    else wave_batch_e<NS, 0>(kind, dedup, a, G, blocks, s);
+}
+
+template <int NS, int E>
+static void wave_one_e(int kind, int dedup, const Plan& P, const WaveArgs& a, const DevGraph& G, int blocks,
+                       hipStream_t s) {
     if (kind == kSupport)
-        hipLaunchKernelGGL((k_wave_batch<kSupport, kDedupRows, NS>), dim3(blocks), dim3(kWave), 0, s, a, G);
+        hipLaunchKernelGGL((k_wave_one<kSupport, kDedupRows, NS, E>), dim3(blocks), dim3(kWave), 0, s, P, a, G);
     else if (dedup == kDedupSort)
-        hipLaunchKernelGGL((k_wave_batch<kPairs, kDedupSort, NS>), dim3(blocks), dim3(kWave), 0, s, a, G);
+        hipLaunchKernelGGL((k_wave_one<kPairs, kDedupSort, NS, E>), dim3(blocks), dim3(kWave), 0, s, P, a, G);
     else if (dedup == kDedupLds)
-        hipLaunchKernelGGL((k_wave_batch<kPairs, kDedupLds, NS>), dim3(blocks), dim3(kWave), (size_t)a.rowWords * 4,
-                           s, a, G);
+        hipLaunchKernelGGL((k_wave_one<kPairs, kDedupLds, NS, E>), dim3(blocks), dim3(kWave), (size_t)a.rowWords * 4, s,
+                           P, a, G);
     else
-        hipLaunchKernelGGL((k_wave_batch<kPairs, kDedupRows, NS>), dim3(blocks), dim3(kWave), 0, s, a, G);
+        hipLaunchKernelGGL((k_wave_one<kPairs, kDedupRows, NS, E>), dim3(blocks), dim3(kWave), 0, s, P, a, G);
 }
 
 template <int NS>
 void SlotKernels<NS>::wave_one(int kind, int dedup, const Plan& P, const WaveArgs& a, const DevGraph& G, int blocks,
                                hipStream_t s) {
+    if (a.engine == 1) wave_one_e<NS, 1>(kind, dedup, P, a, G, blocks, s);
+    else wave_one_e<NS, 0>(kind, dedup, P, a, G, blocks, s);
+}
+
+template <int NS, int E>
+static void wave_sub_e(int kind, int dedup, const WaveArgs& a, const DevGraph& G, int blocks, hipStream_t s) {
     if (kind == kSupport)
-        hipLaunchKernelGGL((k_wave_one<kSupport, kDedupRows, NS>), dim3(blocks), dim3(kWave), 0, s, P, a, G);
+        hipLaunchKernelGGL((k_wave_sub<kSupport, kDedupRows, NS, E>), dim3(blocks), dim3(kWave), 0, s, a, G);
     else if (dedup == kDedupSort)
-        hipLaunchKernelGGL((k_wave_one<kPairs, kDedupSort, NS>), dim3(blocks), dim3(kWave), 0, s, P, a, G);
+        hipLaunchKernelGGL((k_wave_sub<kPairs, kDedupSort, NS, E>), dim3(blocks), dim3(kWave), 0, s, a, G);
     else if (dedup == kDedupLds)
-        hipLaunchKernelGGL((k_wave_one<kPairs, kDedupLds, NS>), dim3(blocks), dim3(kWave), (size_t)a.rowWords * 4, s,
-                           P, a, G);
+        hipLaunchKernelGGL((k_wave_sub<kPairs, kDedupLds, NS, E>), dim3(blocks), dim3(kWave), (size_t)a.rowWords * 4, s,
+                           a, G);
     else
-        hipLaunchKernelGGL((k_wave_one<kPairs, kDedupRows, NS>), dim3(blocks), dim3(kWave), 0, s, P, a, G);
+        hipLaunchKernelGGL((k_wave_sub<kPairs, kDedupRows, NS, E>), dim3(blocks), dim3(kWave), 0, s, a, G);
 }
 
 template <int NS>
 void SlotKernels<NS>::giants(int kind, int dedup, const WaveArgs& a, const DevGraph& G, int blocks, hipStream_t s) {
     const int eb = blocks < 1024 ? blocks : 1024;
     hipLaunchKernelGGL(k_giant_expand<NS>, dim3(eb), dim3(kWave), 0, s, a.giant, a.plans, a.refs, G);
-    if (kind == kSupport)
-        hipLaunchKernelGGL((k_wave_sub<kSupport, kDedupRows, NS>), dim3(blocks), dim3(kWave), 0, s, a, G);
-    else if (dedup == kDedupSort)
-        hipLaunchKernelGGL((k_wave_sub<kPairs, kDedupSort, NS>), dim3(blocks), dim3(kWave), 0, s, a, G);
-    else if (dedup == kDedupLds)
-        hipLaunchKernelGGL((k_wave_sub<kPairs, kDedupLds, NS>), dim3(blocks), dim3(kWave), (size_t)a.rowWords * 4, s,
-                           a, G);
-    else
-        hipLaunchKernelGGL((k_wave_sub<kPairs, kDedupRows, NS>), dim3(blocks), dim3(kWave), 0, s, a, G);
+    if (a.engine == 1) wave_sub_e<NS, 1>(kind, dedup, a, G, blocks, s);
+    else wave_sub_e<NS, 0>(kind, dedup, a, G, blocks, s);
 }
 
 template <int NS>
-size_t SlotKernels<NS>::lds_static() {
+size_t SlotKernels<NS>::lds_static(int engine) {
     hipFuncAttributes fa{};
-    if (hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k_wave_batch<kPairs, kDedupLds, NS>)) != hipSuccess)
-        return sizeof(WaveLdsT<levels_of(NS)>) + kCacheSlots * sizeof(int);
+    const void* k = engine == 1 ? reinterpret_cast<const void*>(&k_wave_batch<kPairs, kDedupLds, NS, 1>)
+                                : reinterpret_cast<const void*>(&k_wave_batch<kPairs, kDedupLds, NS, 0>);
+    if (hipFuncGetAttributes(&fa, k) != hipSuccess)
+        return (engine == 1 ? sizeof(LsLdsT<NS>) : sizeof(WaveLdsT<levels_of(NS)>)) + kCacheSlots * sizeof(int);
     return fa.sharedSizeBytes;
 }
 
 template <int NS>
-int SlotKernels<NS>::lds_blocks_per_cu(size_t dynLds) {
+int SlotKernels<NS>::lds_blocks_per_cu(size_t dynLds, int engine) {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &n, reinterpret_cast<const void*>(&k_wave_batch<kPairs, kDedupLds, NS>), kWave, dynLds) != hipSuccess)
-        return 0;
+    const void* k = engine == 1 ? reinterpret_cast<const void*>(&k_wave_batch<kPairs, kDedupLds, NS, 1>)
+                                : reinterpret_cast<const void*>(&k_wave_batch<kPairs, kDedupLds, NS, 0>);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, kWave, dynLds) != hipSuccess) return 0;
     return n;
+}
+
+template <int NS, int E>
+static void prepare_e(size_t dynLds) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wave_batch<kPairs, kDedupLds, NS, E>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)dynLds);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wave_one<kPairs, kDedupLds, NS, E>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)dynLds);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wave_sub<kPairs, kDedupLds, NS, E>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)dynLds);
 }
 
 template <int NS>
 void SlotKernels<NS>::prepare(size_t dynLds) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wave_batch<kPairs, kDedupLds, NS>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)dynLds);
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wave_one<kPairs, kDedupLds, NS>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)dynLds);
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wave_sub<kPairs, kDedupLds, NS>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)dynLds);
+    prepare_e<NS, 0>(dynLds);
+    prepare_e<NS, 1>(dynLds);
 }
 
 }  // namespace gr

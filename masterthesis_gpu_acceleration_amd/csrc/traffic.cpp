@@ -68,7 +68,9 @@ struct Walker {
         touched.clear();
     }
 
-    int64_t cur[kMaxBody] = {}, curEnd[kMaxBody] = {};   // ISECT row-2 cursors
+    int64_t cur[kMaxBody] = {};                          // ISECT: entry of the match in row 2
+    int64_t oLo[kMaxBody] = {}, oHi[kMaxBody] = {};      // ISECT: the row searched
+    bool swp[kMaxBody] = {};                             // ISECT: enumerating row 2
 
     int col(int64_t e) const { return e < T ? g.spoVal[e] : g.posVal[e - T]; }
     int coo(int64_t e) const { return e < T ? g.spoKey[e] : g.posKey[e - T]; }
@@ -93,37 +95,21 @@ struct Walker {
         hi = dir * T + r.second;
     }
 
-    // Forward-cursor membership test of ISECT level L (as the device does).
-    bool isect_next(int L, int x) {
-        int64_t j = cur[L];
-        const int64_t e = curEnd[L];
-        bool hit = false;
-        if (j < e) {
+    // Membership test of ISECT candidate entry i in the row the level
+    // searches (a binary search, as the device's find_in_row reads it); on a
+    // match cur[L] is the match's entry in row 2.
+    bool isect_find(int L, int64_t i) {
+        const int x = col(i);
+        int64_t l = oLo[L], h = oHi[L];
+        while (l < h) {
+            const int64_t m = (l + h) >> 1;
             bytes += 4;
-            if (col(j) < x) {
-                int64_t lo = j, hi = e, step = 1;
-                for (;;) {
-                    const int64_t t = lo + step;
-                    if (t >= e) break;
-                    bytes += 4;
-                    if (col(t) >= x) { hi = t; break; }
-                    lo = t;
-                    step <<= 1;
-                }
-                int64_t l = lo + 1, h = hi;
-                while (l < h) {
-                    const int64_t m = (l + h) >> 1;
-                    bytes += 4;
-                    if (col(m) < x) l = m + 1; else h = m;
-                }
-                j = l;
-                hit = j < e && col(j) == x;
-            } else {
-                hit = col(j) == x;
-            }
+            if (col(m) < x) l = m + 1; else h = m;
         }
-        cur[L] = j;
-        return hit;
+        if (l < oHi[L]) bytes += 4;
+        if (!(l < oHi[L] && col(l) == x)) return false;
+        cur[L] = swp[L] ? i : l;
+        return true;
     }
 
     void range(int L, int64_t& lo, int64_t& hi) {
@@ -137,9 +123,22 @@ struct Walker {
         }
         const int64_t base = (int64_t)(uint32_t)lv.lo | ((int64_t)lv.hi << 32);
         if (kind == kIsect) {
+            // both rows list candidates of the new variable: enumerate the
+            // shorter, search the other (as the level-synchronous kernels do)
             const int64_t base2 = (int64_t)(uint32_t)lv.lo2 | ((int64_t)lv.hi2 << 32);
-            key_row(base2, tv(lv.val2), cur[L], curEnd[L]);
-            if (cur[L] >= curEnd[L]) { lo = hi = 0; return; }
+            int64_t a2, b2;
+            key_row(base2, tv(lv.val2), a2, b2);
+            if (a2 >= b2) { lo = hi = 0; return; }
+            key_row(base, tv(lv.key), lo, hi);
+            if (lo >= hi) return;
+            swp[L] = b2 - a2 < hi - lo;
+            if (swp[L]) {
+                oLo[L] = lo; oHi[L] = hi;
+                lo = a2; hi = b2;
+            } else {
+                oLo[L] = a2; oHi[L] = b2;
+            }
+            return;
         }
         key_row(base, tv(lv.key), lo, hi);
         if (kind == kCheck) {
@@ -239,7 +238,7 @@ struct Walker {
         const bool isect = level_kind(P.lev[L].flags) == kIsect;
         for (int64_t i = lo; i < hi; ++i) {
             if (!bind(L, i)) continue;
-            if (isect && !isect_next(L, col(i))) continue;
+            if (isect && !isect_find(L, i)) continue;
             if (pairs && L == P.pairLevel) {
                 const int ho = tv(P.hoRef);
                 if (seen(ho)) continue;
