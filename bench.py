@@ -261,7 +261,7 @@ def pmc_traffic(workload, tuning):
 
 
 # settable tuning keys, in the order --tune takes them
-TUNE_SET = ("budget", "thread_waves_per_cu", "wave_waves_per_cu", "dedup", "giant_after", "chunks_per_wave")
+TUNE_SET = ("budget", "thread_waves_per_cu", "wave_waves_per_cu", "dedup", "giant_after", "chunks_per_wave", "wave_engine")
 
 
 def tune_engine(eng, name, words, flags, args):
@@ -279,7 +279,8 @@ def tune_engine(eng, name, words, flags, args):
         return "library defaults"
     if prof is not None and not args.autotune:
         for key in TUNE_SET:
-            eng.set_tuning(key, int(prof["tuning"][key]))
+            if key in prof["tuning"]:
+                eng.set_tuning(key, int(prof["tuning"][key]))
         log("tuning of the committed profile: %s" % prof["tuning"])
         return "profiles/pmc_traffic.json (the autotuner's choice when the profile was taken)"
     t = time.time()

@@ -304,7 +304,7 @@ struct Context : Handle {
     hipEvent_t forkEv = nullptr, joinEv = nullptr;
     GiantBufs giants2;
     int giantAfter = 2048;   // wave-mode steps on one seed before it is listed as a giant (0: never)
-    int waveEngine = 0;      // wave-mode search: 0 pool of entries, 1 level-synchronous (WaveArgs::engine)
+    int waveEngine = 1;      // wave-mode search: 0 pool of entries, 1 level-synchronous (WaveArgs::engine)
     // Serving: requests whose estimated search work (estimate_work) passes
     // serveHeavyWork run in batches of their own, one such batch in flight at
     // a time, on serveHeavyPercent of the grid, so that light requests are

@@ -1421,20 +1421,36 @@ __device__ __forceinline__ uint64_t wave_seed(WaveLdsT<levels_of(NS)>& W, const 
 //     the same level's code on a candidate of its own;
 //   * candidates that bind become items of the next level, appended in lane
 //     order, or, at the last level, groundings: marks in the seed's dedup set
-//     (PAIRS) or the end of the search (existence);
-//   * a last-level item of a plain EXT binding the head object that is at
-//     least kLsStream entries long is streamed by the whole wave with the
-//     straight-line leaf test, up to kLeafUnroll entries per lane per step;
+//     (PAIRS) or the end of the search (existence). A CHECK level without
+//     conflict checks binds nothing and its one entry always passes, so a
+//     child at such a level goes on to the level below it at once;
+//   * a last level that is a plain EXT binding the head object (or, for
+//     existence, any plain EXT) is tested with the straight-line leaf test:
+//     an item at least kLsStream entries long is streamed by the whole wave,
+//     up to kLsStreamU entries per lane per step; shorter items are cut into
+//     chunks of kLsLeafChunk entries, one chunk per lane;
+//   * an ISECT level's two rows both list candidates of its new variable:
+//     an item enumerates the shorter and searches the other;
 //   * below the pair level, an item whose head object is counted already is
 //     dropped, as the pool engine drops such subtrees.
 // The dedup set, giants, found flags and the end of a seed are shared with
 // the pool engine.
 
 // Open items per level: at most one per lane, and a power of two.
-constexpr int ls_cap(int ns) { return ns <= 4 ? 64 : (ns <= 8 ? 32 : 16); }
+#ifndef GR_LS_CAP4
+#define GR_LS_CAP4 64
+#endif
+#ifndef GR_LS_STREAM_U
+#define GR_LS_STREAM_U 16
+#endif
+constexpr int ls_cap(int ns) { return ns <= 4 ? GR_LS_CAP4 : (ns <= 8 ? 32 : 16); }
 constexpr int kLsStream = 512;
+constexpr int kLsStreamU = GR_LS_STREAM_U;   // entries per lane per step of a streamed item
 constexpr int kLsFill = 32;
-constexpr int kLsLeafChunk = 8;   // last-level entries a lane takes from one item per step (chunked leaf steps)
+#ifndef GR_LS_CHUNK
+#define GR_LS_CHUNK 8
+#endif
+constexpr int kLsLeafChunk = GR_LS_CHUNK;   // last-level entries a lane takes from one item per step (chunked leaf steps)
 
 template <int NS>
 struct LsLdsT {
@@ -1574,17 +1590,17 @@ __device__ __forceinline__ uint64_t ls_run(LsLdsT<NS>& W, const DevGraph& G, con
                 int v[NS];
 #pragma unroll
                 for (int k = 0; k < NS; ++k) v[k] = uniform(*ls_v(W, X, k, s));
-                const int lim = e0 - i0 > kWave * kLeafUnroll ? i0 + kWave * kLeafUnroll : e0;
-                int is[kLeafUnroll], xs[kLeafUnroll], rs[kLeafUnroll];
-                bool in[kLeafUnroll];
+                const int lim = e0 - i0 > kWave * kLsStreamU ? i0 + kWave * kLsStreamU : e0;
+                int is[kLsStreamU], xs[kLsStreamU], rs[kLsStreamU];
+                bool in[kLsStreamU];
 #pragma unroll
-                for (int k = 0; k < kLeafUnroll; ++k) {
+                for (int k = 0; k < kLsStreamU; ++k) {
                     is[k] = i0 + lane + kWave * k;
                     in[k] = is[k] < lim;
                 }
                 const uint32_t m = leaf_fast(G, v, level_nbound(P.lev[last].flags), pairs, is, in, i0, xs, rs);
                 if (!pairs) found = found || m != 0u;
-                else mine += mark_ranks<D, kLeafUnroll>(dd, rs);
+                else mine += mark_ranks<D, kLsStreamU>(dd, rs);
                 nCand += (uint64_t)(lim - i0);
                 if (lane == 0) *ls_i(W, X, s) = lim;
                 if (lim == e0) {

@@ -45,7 +45,7 @@ def engine_for(cache, name):
 
 def defaults(eng):
     for k, v in (("budget", 4096), ("dedup", 0), ("giant_after", 2048), ("use_graph", 1),
-                 ("per_rule_giant_work", 10 ** 7), ("wave_engine", 0)):
+                 ("per_rule_giant_work", 10 ** 7), ("wave_engine", 1)):
         eng.set_tuning(k, v)
 
 
