@@ -148,6 +148,37 @@ __device__ __forceinline__ int find_in_row(const int32_t* __restrict__ col, int 
     return (l < hi && col[l] == target) ? l : -1;
 }
 
+// Index in col[lo,hi) holding `target`, or -1, for a row whose first and
+// last values vlo = col[lo], vhi = col[hi-1] the caller holds: a target
+// outside [vlo, vhi] costs no load; inside, up to three interpolation probes
+// narrow the bracket (term ids of a row are spread fairly evenly), then a
+// binary search finishes it. The probes only pick where to look, so the
+// answer is find_in_row's.
+__device__ __forceinline__ int find_in_row_interp(const int32_t* __restrict__ col, int lo, int hi, int target, int vlo,
+                                                  int vhi) {
+    if (lo >= hi || target < vlo || target > vhi) return -1;
+    if (target == vlo) return lo;
+    if (target == vhi) return hi - 1;
+    int l = lo, h = hi - 1, vl = vlo, vh = vhi;   // vl < target < vh, vl = col[l], vh = col[h]
+#pragma unroll
+    for (int it = 0; it < 3; ++it) {
+        if (h - l <= 8) break;
+        const float f = (float)(target - vl) / (float)(vh - vl);
+        int m = l + (int)(f * (float)(h - l));
+        m = m <= l ? l + 1 : (m >= h ? h - 1 : m);
+        const int y = col[m];
+        if (y == target) return m;
+        if (y < target) {
+            l = m;
+            vl = y;
+        } else {
+            h = m;
+            vh = y;
+        }
+    }
+    return find_in_row(col, l + 1, h, target);
+}
+
 // Would ground triple (s, o) of a level repeat an earlier one? Only rules
 // with constants reach this; it reads the conflict table from memory.
 template <int NS>
@@ -1492,13 +1523,18 @@ __device__ __forceinline__ void radd(int (&a)[NL], int L, int d) {
     for (int l = 0; l < NL; ++l) a[l] += l == L ? d : 0;
 }
 
+// Inclusive sum over lanes 0..lane, with DPP moves (no LDS round trip):
+// shifts by 1, 2, 4, 8 within each 16-lane row, then the last lane of row 0
+// (of row 2) added to row 1 (row 3), then the last lane of row 1 added to
+// rows 2 and 3. A lane whose DPP source lies outside its row, and the rows a
+// step masks off, add 0.
 __device__ __forceinline__ int wave_incl_scan(int x) {
-    const int lane = lane_id();
-#pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-        const int y = __shfl_up(x, off, kWave);
-        if (lane >= off) x += y;
-    }
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);   // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);   // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);   // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);   // row_shr:8
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);   // row_bcast:15 -> rows 1, 3
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);   // row_bcast:31 -> rows 2, 3
     return x;
 }
 
@@ -1709,10 +1745,10 @@ __device__ __forceinline__ uint64_t ls_run(LsLdsT<NS>& W, const DevGraph& G, con
             const int kind = uniform(level_kind(lv.flags));
             // ISECT: the row each item searches (the one it does not enumerate), looked
             // up by the lane holding the item
-            int r2i = 0, r2e = 0, swp = 0;
+            int r2i = 0, r2e = 0, swp = 0, r2lo = 0, r2hi = -1;
             if (kind == kIsect) {
                 Frame other{0, 0};
-                int sw = 0;
+                int sw = 0, vlo = 0, vhi = -1;
                 if (lane < n && rem > 0) {
                     int vj[NS];
 #pragma unroll
@@ -1720,10 +1756,16 @@ __device__ __forceinline__ uint64_t ls_run(LsLdsT<NS>& W, const DevGraph& G, con
                     const Frame r1 = enter_level(lv, G, vj);
                     sw = (ii >= r1.i && ie <= r1.end) ? 0 : 1;   // enumerating row 2
                     other = sw ? r1 : isect_row(lv, G, vj);
+                    if (other.i < other.end) {   // its first and last values, for the searches
+                        vlo = G.col[other.i];
+                        vhi = G.col[other.end - 1];
+                    }
                 }
                 r2i = __shfl(other.i, j, kWave);
                 r2e = __shfl(other.end, j, kWave);
                 swp = __shfl(sw, j, kWave);
+                r2lo = __shfl(vlo, j, kWave);
+                r2hi = __shfl(vhi, j, kWave);
             }
             int v[NS];
 #pragma unroll
@@ -1733,7 +1775,7 @@ __device__ __forceinline__ uint64_t ls_run(LsLdsT<NS>& W, const DevGraph& G, con
             int at = -1;    // ISECT: the match's entry in row 2 (the CHECK level's row)
             int e1 = e;     // the candidate's entry in the level's own row (row 1)
             if (kind == kIsect && ok) {
-                const int m = find_in_row(G.col, r2i, r2e, x);
+                const int m = find_in_row_interp(G.col, r2i, r2e, x, r2lo, r2hi);
                 ok = m >= 0;
                 at = swp ? e : m;
                 e1 = swp ? m : e;

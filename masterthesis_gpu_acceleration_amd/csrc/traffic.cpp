@@ -100,16 +100,41 @@ struct Walker {
     // match cur[L] is the match's entry in row 2.
     bool isect_find(int L, int64_t i) {
         const int x = col(i);
-        int64_t l = oLo[L], h = oHi[L];
-        while (l < h) {
-            const int64_t m = (l + h) >> 1;
-            bytes += 4;
-            if (col(m) < x) l = m + 1; else h = m;
-        }
-        if (l < oHi[L]) bytes += 4;
-        if (!(l < oHi[L] && col(l) == x)) return false;
-        cur[L] = swp[L] ? i : l;
+        const int64_t m = find_interp(oLo[L], oHi[L], x);
+        if (m < 0) return false;
+        cur[L] = swp[L] ? i : m;
         return true;
+    }
+
+    // The device's find_in_row_interp: the row's first and last values are
+    // read once per level entered (counted there); a target outside them
+    // reads nothing; up to three interpolation probes, then a binary search.
+    int64_t find_interp(int64_t lo, int64_t hi, int x) {
+        if (lo >= hi) return -1;
+        const int vlo = col(lo), vhi = col(hi - 1);
+        if (x < vlo || x > vhi) return -1;
+        if (x == vlo) return lo;
+        if (x == vhi) return hi - 1;
+        int64_t l = lo, h = hi - 1;
+        int vl = vlo, vh = vhi;
+        for (int it = 0; it < 3; ++it) {
+            if (h - l <= 8) break;
+            const float f = (float)(x - vl) / (float)(vh - vl);
+            int64_t m = l + (int64_t)(int)(f * (float)(int)(h - l));
+            m = m <= l ? l + 1 : (m >= h ? h - 1 : m);
+            bytes += 4;
+            const int y = col(m);
+            if (y == x) return m;
+            if (y < x) { l = m; vl = y; } else { h = m; vh = y; }
+        }
+        int64_t a = l + 1, b = h;
+        while (a < b) {
+            const int64_t m = (a + b) >> 1;
+            bytes += 4;
+            if (col(m) < x) a = m + 1; else b = m;
+        }
+        if (a < h) bytes += 4;
+        return (a < h && col(a) == x) ? a : -1;
     }
 
     void range(int L, int64_t& lo, int64_t& hi) {
@@ -138,6 +163,7 @@ struct Walker {
             } else {
                 oLo[L] = a2; oHi[L] = b2;
             }
+            bytes += 8;   // the searched row's first and last values
             return;
         }
         key_row(base, tv(lv.key), lo, hi);
