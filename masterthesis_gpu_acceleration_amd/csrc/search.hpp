@@ -1491,10 +1491,13 @@ struct LsLdsT {
     Plan plan;
     int i0, e0, h0;           // level 0's one item (a seed's level-0 row, or a sub-item's range there)
     uint32_t nTouched;        // HBM rows: dedup-row words listed so far
+    int oi0, oe0;
     int v0[NS];
     int itI[NL - 1][C];       // levels 1..NL-1, per ring slot: next entry,
     int itEnd[NL - 1][C];     // end of the range,
     int itHr[NL - 1][C];      // rank of the head object bound above the level (PAIRS), else -1,
+    int itOI[NL - 1][C];      // ISECT: the row the item searches (the one it does not enumerate):
+    int itOE[NL - 1][C];      //   begin, and end (~end when the item enumerates row 2),
     int itV[NL - 1][NS][C];   // bindings above the level
 };
 
@@ -1504,6 +1507,10 @@ template <int NS>
 __device__ __forceinline__ int* ls_e(LsLdsT<NS>& W, int L, int s) { return L == 0 ? &W.e0 : &W.itEnd[L - 1][s]; }
 template <int NS>
 __device__ __forceinline__ int* ls_hr(LsLdsT<NS>& W, int L, int s) { return L == 0 ? &W.h0 : &W.itHr[L - 1][s]; }
+template <int NS>
+__device__ __forceinline__ int* ls_oi(LsLdsT<NS>& W, int L, int s) { return L == 0 ? &W.oi0 : &W.itOI[L - 1][s]; }
+template <int NS>
+__device__ __forceinline__ int* ls_oe(LsLdsT<NS>& W, int L, int s) { return L == 0 ? &W.oe0 : &W.itOE[L - 1][s]; }
 template <int NS>
 __device__ __forceinline__ int* ls_v(LsLdsT<NS>& W, int L, int k, int s) {
     return L == 0 ? &W.v0[k] : &W.itV[L - 1][k][s];
@@ -1566,15 +1573,21 @@ __device__ __forceinline__ int ls_items(LsLdsT<NS>& W, const Dedup<D, NS>& dd, i
 
 // Entry range of an item of level lv under bindings v. An ISECT level's two
 // rows both list candidates of its new variable, sorted: the item enumerates
-// the shorter one and searches the other (ls_run tells them apart by which
-// row the item's range lies in).
+// the shorter one and searches the other, returned in `other` with its end
+// as ~end when the item enumerates row 2 (other levels: {0, 0}).
 template <int NS>
-__device__ __forceinline__ Frame ls_enter(const Level& lv, const DevGraph& G, const int (&v)[NS]) {
+__device__ __forceinline__ Frame ls_enter(const Level& lv, const DevGraph& G, const int (&v)[NS], Frame& other) {
+    other = Frame{0, 0};
     const Frame r1 = enter_level(lv, G, v);
     if (level_kind(lv.flags) != kIsect || r1.i >= r1.end) return r1;
     const Frame r2 = isect_row(lv, G, v);
     if (r2.i >= r2.end) return Frame{0, 0};
-    return r2.end - r2.i < r1.end - r1.i ? r2 : r1;
+    if (r2.end - r2.i < r1.end - r1.i) {
+        other = Frame{r1.i, ~r1.end};
+        return r2;
+    }
+    other = r2;
+    return r1;
 }
 
 // Runs the search from the one item the caller placed at level `start`
@@ -1743,34 +1756,21 @@ __device__ __forceinline__ uint64_t ls_run(LsLdsT<NS>& W, const DevGraph& G, con
             const int e = act ? ji + (lane - jx) : 0;
             const Level& lv = P.lev[X];
             const int kind = uniform(level_kind(lv.flags));
-            // ISECT: the row each item searches (the one it does not enumerate), looked
-            // up by the lane holding the item
+            const int x = G.col[e];
+            // ISECT: the row the candidate's item searches, with its first and last
+            // values (loads shared by the item's lanes)
             int r2i = 0, r2e = 0, swp = 0, r2lo = 0, r2hi = -1;
-            if (kind == kIsect) {
-                Frame other{0, 0};
-                int sw = 0, vlo = 0, vhi = -1;
-                if (lane < n && rem > 0) {
-                    int vj[NS];
-#pragma unroll
-                    for (int k = 0; k < NS; ++k) vj[k] = *ls_v(W, X, k, sj);
-                    const Frame r1 = enter_level(lv, G, vj);
-                    sw = (ii >= r1.i && ie <= r1.end) ? 0 : 1;   // enumerating row 2
-                    other = sw ? r1 : isect_row(lv, G, vj);
-                    if (other.i < other.end) {   // its first and last values, for the searches
-                        vlo = G.col[other.i];
-                        vhi = G.col[other.end - 1];
-                    }
-                }
-                r2i = __shfl(other.i, j, kWave);
-                r2e = __shfl(other.end, j, kWave);
-                swp = __shfl(sw, j, kWave);
-                r2lo = __shfl(vlo, j, kWave);
-                r2hi = __shfl(vhi, j, kWave);
+            if (kind == kIsect && act) {
+                r2i = *ls_oi(W, X, js);
+                r2e = *ls_oe(W, X, js);
+                swp = r2e < 0 ? 1 : 0;
+                r2e = swp ? ~r2e : r2e;
+                r2lo = G.col[r2i];
+                r2hi = G.col[r2e - 1];
             }
             int v[NS];
 #pragma unroll
             for (int k = 0; k < NS; ++k) v[k] = act ? *ls_v(W, X, k, js) : -1;
-            const int x = G.col[e];
             bool ok = act && bind_entry_x(lv, G, R, v, e, x);
             int at = -1;    // ISECT: the match's entry in row 2 (the CHECK level's row)
             int e1 = e;     // the candidate's entry in the level's own row (row 1)
@@ -1805,10 +1805,10 @@ __device__ __forceinline__ uint64_t ls_run(LsLdsT<NS>& W, const DevGraph& G, con
                 }
             } else {
                 int c = X + 1;
-                Frame f{0, 0};
+                Frame f{0, 0}, fo{0, 0};
                 if (ok) {
                     const Level& lc = P.lev[c];
-                    f = (level_kind(lc.flags) == kCheck && kind == kIsect) ? Frame{at, at + 1} : ls_enter(lc, G, v);
+                    f = (level_kind(lc.flags) == kCheck && kind == kIsect) ? Frame{at, at + 1} : ls_enter(lc, G, v, fo);
                 }
                 bool has = ok && f.i < f.end;
                 // A CHECK level without conflict checks binds nothing, and its one entry
@@ -1822,7 +1822,7 @@ __device__ __forceinline__ uint64_t ls_run(LsLdsT<NS>& W, const DevGraph& G, con
                         break;
                     }
                     ++c;
-                    f = ls_enter(P.lev[c], G, v);
+                    f = ls_enter(P.lev[c], G, v, fo);
                     has = f.i < f.end;
                 }
                 // the head object of such a grounding is bound at or above X (no CHECK binds it)
@@ -1849,6 +1849,8 @@ __device__ __forceinline__ uint64_t ls_run(LsLdsT<NS>& W, const DevGraph& G, con
                         *ls_i(W, cc, s) = f.i;
                         *ls_e(W, cc, s) = f.end;
                         *ls_hr(W, cc, s) = hr;
+                        *ls_oi(W, cc, s) = fo.i;
+                        *ls_oe(W, cc, s) = fo.end;
 #pragma unroll
                         for (int k = 0; k < NS; ++k) *ls_v(W, cc, k, s) = v[k];
                     }
@@ -1914,13 +1916,16 @@ __device__ __forceinline__ uint64_t ls_seed(LsLdsT<NS>& W, const DevGraph& G, co
     const unsigned long long tp = GR_PROF_NOW();
     int v[NS];
     if (!bind_seed(P, G, (int)item, v)) return 0;
-    const Frame f0 = ls_enter(P.lev[0], G, v);
+    Frame o0{0, 0};
+    const Frame f0 = ls_enter(P.lev[0], G, v, o0);
     if (f0.i >= f0.end) return 0;
     if (lane < NS) W.v0[lane] = term_value(v, lane);
     if (lane == 0) {
         W.i0 = f0.i;
         W.e0 = f0.end;
         W.h0 = -1;
+        W.oi0 = o0.i;
+        W.oe0 = o0.end;
         W.nTouched = 0;
     }
     if (D == kDedupSort && K == kPairs)
@@ -1938,10 +1943,20 @@ __device__ __forceinline__ uint64_t ls_seed(LsLdsT<NS>& W, const DevGraph& G, co
 // A giant's sub-item as the one item of its level (ring slot 0): entries
 // [begin, end) of `level` with the bindings above it; returns the level.
 template <int K, int D, int NS>
-__device__ __forceinline__ int ls_place_sub(LsLdsT<NS>& W, const Dedup<D, NS>& dd, const SubItem& S) {
+__device__ __forceinline__ int ls_place_sub(LsLdsT<NS>& W, const Dedup<D, NS>& dd, const SubItem& S,
+                                            const DevGraph& G) {
     const int lane = lane_id();
     const Plan& P = W.plan;
     const int lvl = uniform(S.level);
+    // a range of an ISECT level's row 1 (as the giant expansion cuts it) searches row 2
+    Frame other{0, 0};
+    const bool isect = level_kind(P.lev[lvl].flags) == kIsect;
+    if (isect) {
+        int v[NS];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) v[k] = uniform(S.v[k]);
+        other = isect_row(P.lev[lvl], G, v);
+    }
     const int pl = uniform(P.pairLevel);
     int hr = -1;
     if (K == kPairs && pl >= 0 && pl < lvl) {   // the head object is bound above the sub-item's level
@@ -1952,8 +1967,10 @@ __device__ __forceinline__ int ls_place_sub(LsLdsT<NS>& W, const Dedup<D, NS>& d
     if (lane < NS) *ls_v(W, lvl, lane, 0) = S.v[lane];
     if (lane == 0) {
         *ls_i(W, lvl, 0) = S.begin;
-        *ls_e(W, lvl, 0) = S.end;
+        *ls_e(W, lvl, 0) = (isect && other.i >= other.end) ? S.begin : S.end;   // row 2 empty: nothing matches
         *ls_hr(W, lvl, 0) = hr;
+        *ls_oi(W, lvl, 0) = other.i;
+        *ls_oe(W, lvl, 0) = other.end;
         W.nTouched = 0;
     }
     return lvl;
@@ -2267,7 +2284,7 @@ __global__ void __launch_bounds__(kWave) k_wave_sub(WaveArgs a, DevGraph G) {
         if (!bcast(skip)) {
             int start = 0;
             if constexpr (E == 1) {
-                start = ls_place_sub<K, D, NS>(W, dd, S);
+                start = ls_place_sub<K, D, NS>(W, dd, S, G);
             } else {
                 if (lane < kMaxSlots) W.poolV[lane] = S.v[lane];
                 if (lane == 0) {
