@@ -338,8 +338,11 @@ def measure(args, name, wl, decompose):
         log("%s: timed %d steps: %.3f ms/step (device %.3f ms)" % (name, args.steps, ms_step, float(np.mean(dev_ms))))
         rules_per_s = n / (ms_step / 1000.0)
         pairs = int(res["bodyPairs"][res["bodyPairs"] > 0].sum())
-        wave = {ph: {q: eng.get_tuning("wave_%s_%s" % (ph, q)) for q in ("seeds", "steps", "donations", "leaf_draws",
-                                                                          "draws", "busy_lane_steps", "draw_lanes")}
+        # the library's counter keys, and what they count under each wave engine
+        keys = ("seeds", "steps", "donations", "leaf_draws", "draws", "busy_lane_steps", "draw_lanes")
+        labels = keys if eng.get_tuning("wave_engine") == 0 else (
+            "seeds", "steps", "fill_steps", "streamed_steps", "balanced_steps", "candidate_slots", "candidates_passed")
+        wave = {ph: {lab: eng.get_tuning("wave_%s_%s" % (ph, q)) for q, lab in zip(keys, labels)}
                 for ph in ("support", "pairs")}
         cyc = {q: eng.get_tuning("wave_pairs_cycles_" + q) for q in ("take", "setup", "leaf_draw", "draw", "own",
                                                                      "step_end", "seed_end", "sub_items")}
