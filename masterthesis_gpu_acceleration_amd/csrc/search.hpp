@@ -1499,7 +1499,28 @@ struct LsLdsT {
     int itOI[NL - 1][C];      // ISECT: the row the item searches (the one it does not enumerate):
     int itOE[NL - 1][C];      //   begin, and end (~end when the item enumerates row 2),
     int itV[NL - 1][NS][C];   // bindings above the level
+    int own[kWave];           // a step's deal: the item whose candidates start at each position
 };
+
+// One level's item arrays (pointers chosen once per wave-uniform level):
+// entry k of slot s's bindings is v[k * vs + s]. Level 0 holds one item, at
+// slot 0.
+template <int NS>
+struct LsLevel {
+    int* i;
+    int* e;
+    int* hr;
+    int* oi;
+    int* oe;
+    int* v;
+    int vs;
+};
+template <int NS>
+__device__ __forceinline__ LsLevel<NS> ls_level(LsLdsT<NS>& W, int L) {
+    if (L == 0) return LsLevel<NS>{&W.i0, &W.e0, &W.h0, &W.oi0, &W.oe0, W.v0, 1};
+    return LsLevel<NS>{W.itI[L - 1], W.itEnd[L - 1], W.itHr[L - 1], W.itOI[L - 1], W.itOE[L - 1], &W.itV[L - 1][0][0],
+                       LsLdsT<NS>::C};
+}
 
 template <int NS>
 __device__ __forceinline__ int* ls_i(LsLdsT<NS>& W, int L, int s) { return L == 0 ? &W.i0 : &W.itI[L - 1][s]; }
@@ -1545,6 +1566,18 @@ __device__ __forceinline__ int wave_incl_scan(int x) {
     return x;
 }
 
+// Inclusive maximum over lanes 0..lane, with the same DPP moves.
+__device__ __forceinline__ int wave_incl_max(int x) {
+    constexpr int lo = -2147483647 - 1;
+    x = max(x, __builtin_amdgcn_update_dpp(lo, x, 0x111, 0xf, 0xf, false));
+    x = max(x, __builtin_amdgcn_update_dpp(lo, x, 0x112, 0xf, 0xf, false));
+    x = max(x, __builtin_amdgcn_update_dpp(lo, x, 0x114, 0xf, 0xf, false));
+    x = max(x, __builtin_amdgcn_update_dpp(lo, x, 0x118, 0xf, 0xf, false));
+    x = max(x, __builtin_amdgcn_update_dpp(lo, x, 0x142, 0xa, 0xf, false));
+    x = max(x, __builtin_amdgcn_update_dpp(lo, x, 0x143, 0xc, 0xf, false));
+    return x;
+}
+
 // The items of level L (ring head h, n of them): lane j < n loads item j
 // and returns its remaining length. With dropBound, an item whose head
 // object is counted already has nothing left to add: it is marked consumed
@@ -1554,21 +1587,38 @@ __device__ __forceinline__ int ls_items(LsLdsT<NS>& W, const Dedup<D, NS>& dd, i
                                         int& ii, int& ie, int& ihr, int& sj) {
     constexpr int C = LsLdsT<NS>::C;
     const int lane = lane_id();
+    const LsLevel<NS> lv = ls_level(W, L);
     sj = (h + lane) & (C - 1);
     ii = 0;
     ie = 0;
     ihr = -1;
     if (lane < n) {
-        ii = *ls_i(W, L, sj);
-        ie = *ls_e(W, L, sj);
-        ihr = *ls_hr(W, L, sj);
+        ii = lv.i[sj];
+        ie = lv.e[sj];
+        ihr = lv.hr[sj];
     }
     int rem = ie - ii;
     if (dropBound && rem > 0 && dd.seen_r(ihr)) {
         rem = 0;
-        *ls_i(W, L, sj) = ie;
+        lv.i[sj] = ie;
     }
     return rem;
+}
+
+// The deal of a step: positions 0..T-1 go to the items in order, item j
+// (j < n, the lane holding it) taking cnt_j positions from excl_j on. Returns
+// the item of position `lane` (for lane < T): each item with positions
+// writes its index at its first position, and an inclusive maximum over the
+// positions carries it forward.
+template <int NS>
+__device__ __forceinline__ int ls_deal(LsLdsT<NS>& W, int n, int cnt, int excl, int T) {
+    const int lane = lane_id();
+    W.own[lane] = -1;
+    wave_sync();
+    if (lane < n && cnt > 0 && excl < T) W.own[excl] = lane;
+    wave_sync();
+    const int j = wave_incl_max(W.own[lane]);
+    return j < 0 ? 0 : j;
 }
 
 // Entry range of an item of level lv under bindings v. An ISECT level's two
@@ -1678,18 +1728,11 @@ __device__ __forceinline__ uint64_t ls_run(LsLdsT<NS>& W, const DevGraph& G, con
             int tookC = TC - excl;
             tookC = tookC < 0 ? 0 : (tookC > cnk ? cnk : tookC);
             const int took = tookC * U < rem ? tookC * U : rem;
-            if (lane < n && took > 0) *ls_i(W, X, sj) = ii + took;
+            const LsLevel<NS> lx = ls_level(W, X);
+            if (lane < n && took > 0) lx.i[sj] = ii + took;
             const uint64_t doneM = __ballot(lane < n && rem == took);
             const int pre = doneM == ~0ull ? kWave : (int)__builtin_ctzll(~doneM);
-            int lo = 0, hi = n - 1;
-#pragma unroll
-            for (int it = 0; it < 6; ++it) {
-                const int mid = (lo + hi) >> 1;
-                const int im = __shfl(incl, mid, kWave);
-                if (im > lane) hi = mid;
-                else lo = mid + 1;
-            }
-            const int j = lo < n ? lo : n - 1;
+            const int j = ls_deal(W, n, cnk, excl, TC);
             const int jx = __shfl(excl, j, kWave);
             const int ji = __shfl(ii, j, kWave);
             const int je = __shfl(ie, j, kWave);
@@ -1699,7 +1742,7 @@ __device__ __forceinline__ uint64_t ls_run(LsLdsT<NS>& W, const DevGraph& G, con
             const int end = act ? (start + U < je ? start + U : je) : 0;
             int vb[NS];
 #pragma unroll
-            for (int k = 0; k < NS; ++k) vb[k] = act ? *ls_v(W, X, k, js) : -1;
+            for (int k = 0; k < NS; ++k) vb[k] = act ? lx.v[k * lx.vs + js] : -1;
             int is[U], xs[U], rs[U];
             bool in[U];
 #pragma unroll
@@ -1735,19 +1778,12 @@ __device__ __forceinline__ uint64_t ls_run(LsLdsT<NS>& W, const DevGraph& G, con
             const int excl = incl - rem;
             int took = T - excl;
             took = took < 0 ? 0 : (took > rem ? rem : took);
-            if (lane < n && took > 0) *ls_i(W, X, sj) = ii + took;
+            const LsLevel<NS> lx = ls_level(W, X);
+            if (lane < n && took > 0) lx.i[sj] = ii + took;
             const uint64_t doneM = __ballot(lane < n && rem == took);
             const int pre = doneM == ~0ull ? kWave : (int)__builtin_ctzll(~doneM);
-            // the item of this lane's candidate: the first j with incl_j > lane
-            int lo = 0, hi = n - 1;
-#pragma unroll
-            for (int it = 0; it < 6; ++it) {
-                const int mid = (lo + hi) >> 1;
-                const int im = __shfl(incl, mid, kWave);
-                if (im > lane) hi = mid;
-                else lo = mid + 1;
-            }
-            const int j = lo < n ? lo : n - 1;
+            // the item of this lane's candidate
+            const int j = ls_deal(W, n, rem, excl, T);
             const int jx = __shfl(excl, j, kWave);
             const int ji = __shfl(ii, j, kWave);
             const int jhr = __shfl(ihr, j, kWave);
@@ -1761,8 +1797,8 @@ __device__ __forceinline__ uint64_t ls_run(LsLdsT<NS>& W, const DevGraph& G, con
             // values (loads shared by the item's lanes)
             int r2i = 0, r2e = 0, swp = 0, r2lo = 0, r2hi = -1;
             if (kind == kIsect && act) {
-                r2i = *ls_oi(W, X, js);
-                r2e = *ls_oe(W, X, js);
+                r2i = lx.oi[js];
+                r2e = lx.oe[js];
                 swp = r2e < 0 ? 1 : 0;
                 r2e = swp ? ~r2e : r2e;
                 r2lo = G.col[r2i];
@@ -1770,7 +1806,7 @@ __device__ __forceinline__ uint64_t ls_run(LsLdsT<NS>& W, const DevGraph& G, con
             }
             int v[NS];
 #pragma unroll
-            for (int k = 0; k < NS; ++k) v[k] = act ? *ls_v(W, X, k, js) : -1;
+            for (int k = 0; k < NS; ++k) v[k] = act ? lx.v[k * lx.vs + js] : -1;
             bool ok = act && bind_entry_x(lv, G, R, v, e, x);
             int at = -1;    // ISECT: the match's entry in row 2 (the CHECK level's row)
             int e1 = e;     // the candidate's entry in the level's own row (row 1)
@@ -1845,14 +1881,15 @@ __device__ __forceinline__ uint64_t ls_run(LsLdsT<NS>& W, const DevGraph& G, con
                     const uint64_t cm = __ballot(has && c == cc);
                     if (cm == 0) continue;
                     if (has && c == cc) {
+                        const LsLevel<NS> lc = ls_level(W, cc);
                         const int s = (rget(hd, cc) + rget(ct, cc) + (int)__popcll(cm & below)) & (C - 1);
-                        *ls_i(W, cc, s) = f.i;
-                        *ls_e(W, cc, s) = f.end;
-                        *ls_hr(W, cc, s) = hr;
-                        *ls_oi(W, cc, s) = fo.i;
-                        *ls_oe(W, cc, s) = fo.end;
+                        lc.i[s] = f.i;
+                        lc.e[s] = f.end;
+                        lc.hr[s] = hr;
+                        lc.oi[s] = fo.i;
+                        lc.oe[s] = fo.end;
 #pragma unroll
-                        for (int k = 0; k < NS; ++k) *ls_v(W, cc, k, s) = v[k];
+                        for (int k = 0; k < NS; ++k) lc.v[k * lc.vs + s] = v[k];
                     }
                     radd(ct, cc, (int)__popcll(cm));
                     if (cc > deep) deep = cc;
