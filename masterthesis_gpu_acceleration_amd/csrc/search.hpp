@@ -1659,6 +1659,7 @@ __device__ __forceinline__ uint64_t ls_run(LsLdsT<NS>& W, const DevGraph& G, con
     const int hoRef = uniform(P.hoRef);
     const bool pairs = K == kPairs && pl >= 0;
     const bool fastLeaf = leaf_fast_ok<D>(P, last, pairs);
+    const bool byEntry = D != kDedupSort && uniform(P.hoByEntry) != 0;   // head-object ranks from crank
     const uint64_t below = (1ull << lane) - 1ull;
     int hd[NL], ct[NL];
 #pragma unroll
@@ -1818,7 +1819,7 @@ __device__ __forceinline__ uint64_t ls_run(LsLdsT<NS>& W, const DevGraph& G, con
             }
             int hr = jhr;
             if (pairs && X == pl && ok) {
-                hr = dd.rank_at(e1, term_value(v, hoRef));
+                hr = byEntry ? G.crank[e1] : dd.rank(term_value(v, hoRef));
                 ok = !dd.seen_r(hr);
             }
             nCand += (uint64_t)T;
@@ -1877,7 +1878,9 @@ __device__ __forceinline__ uint64_t ls_run(LsLdsT<NS>& W, const DevGraph& G, con
                 } else if (ground) {
                     mine += dd.insert_r(hr);
                 }
-                for (int cc = X + 1; cc <= last; ++cc) {   // children, appended level by level in lane order
+                // children, appended level by level in lane order (below X + 1 only after a CHECK passed)
+                const int cEnd = __any(has && c > X + 1) ? last : X + 1;
+                for (int cc = X + 1; cc <= cEnd; ++cc) {
                     const uint64_t cm = __ballot(has && c == cc);
                     if (cm == 0) continue;
                     if (has && c == cc) {
