@@ -1482,6 +1482,11 @@ constexpr int kLsFill = 32;
 #define GR_LS_CHUNK 8
 #endif
 constexpr int kLsLeafChunk = GR_LS_CHUNK;   // last-level entries a lane takes from one item per step (chunked leaf steps)
+#ifndef GR_LS_ISECT_U
+#define GR_LS_ISECT_U 2
+#endif
+// ISECT candidates a lane takes per step where every candidate found ends as a grounding
+constexpr int kLsIsectU = GR_LS_ISECT_U;
 
 template <int NS>
 struct LsLdsT {
@@ -1660,6 +1665,11 @@ __device__ __forceinline__ uint64_t ls_run(LsLdsT<NS>& W, const DevGraph& G, con
     const bool pairs = K == kPairs && pl >= 0;
     const bool fastLeaf = leaf_fast_ok<D>(P, last, pairs);
     const bool byEntry = D != kDedupSort && uniform(P.hoByEntry) != 0;   // head-object ranks from crank
+    // an ISECT level right above its conflict-free CHECK as the last level: every
+    // candidate it finds is a grounding, so its steps keep no children (bitmap
+    // dedup or existence: the sort path emits keys wave by wave)
+    const bool isectLeaf = nlev >= 2 && level_kind(P.lev[nlev >= 2 ? last - 1 : 0].flags) == kIsect &&
+                           level_conflicts(P.lev[last].flags) == 0 && (!pairs || D != kDedupSort);
     const uint64_t below = (1ull << lane) - 1ull;
     int hd[NL], ct[NL];
 #pragma unroll
@@ -1710,6 +1720,174 @@ __device__ __forceinline__ uint64_t ls_run(LsLdsT<NS>& W, const DevGraph& G, con
                 ++nStream;
                 GR_PROF_ADD(2, tp);
             }
+        }
+        if (!streamed && isectLeaf && X == last - 1) {
+            // An ISECT level whose candidates end as groundings: with no children to
+            // keep, a step deals up to kLsIsectU candidates per lane (positions lane,
+            // lane + 64, ...), and their row searches run in lockstep, so a lane has
+            // that many independent loads in flight.
+            streamed = true;
+            constexpr int U = kLsIsectU;
+            int ii, ie, ihr, sj;
+            const int n = rget(ct, X);
+            const int rem = ls_items(W, dd, X, rget(hd, X), n, pairs && pl < X, ii, ie, ihr, sj);
+            const int incl = wave_incl_scan(rem);
+            const int total = uniform(__shfl(incl, kWave - 1, kWave));
+            const int T = total < kWave * U ? total : kWave * U;
+            const int excl = incl - rem;
+            int took = T - excl;
+            took = took < 0 ? 0 : (took > rem ? rem : took);
+            const LsLevel<NS> lx = ls_level(W, X);
+            if (lane < n && took > 0) lx.i[sj] = ii + took;
+            const uint64_t doneM = __ballot(lane < n && rem == took);
+            const int pre = doneM == ~0ull ? kWave : (int)__builtin_ctzll(~doneM);
+            const Level& lv = P.lev[X];
+            // the item of each position, window by window (an item starting before a
+            // window carries into it)
+            int jk[U];
+            int carry = -1;
+#pragma unroll
+            for (int k = 0; k < U; ++k) {
+                W.own[lane] = -1;
+                wave_sync();
+                const int rel = excl - kWave * k;
+                if (lane < n && rem > 0 && excl < T && rel >= 0 && rel < kWave) W.own[rel] = lane;
+                wave_sync();
+                int m = wave_incl_max(W.own[lane]);
+                m = m > carry ? m : carry;
+                jk[k] = m < 0 ? 0 : m;
+                carry = uniform(__shfl(m, kWave - 1, kWave));
+                wave_sync();
+            }
+            int x[U], e[U], hrk[U], lo[U], hi[U], vl[U], vh[U], res[U], swpk[U];
+            bool ok[U], go[U];
+#pragma unroll
+            for (int k = 0; k < U; ++k) {
+                const int j = jk[k];
+                const int jx = __shfl(excl, j, kWave);
+                const int ji = __shfl(ii, j, kWave);
+                const int js = __shfl(sj, j, kWave);
+                hrk[k] = __shfl(ihr, j, kWave);
+                const int q = lane + kWave * k;
+                const bool act = q < T;
+                e[k] = act ? ji + (q - jx) : 0;
+                x[k] = G.col[e[k]];
+                int v[NS];
+#pragma unroll
+                for (int t = 0; t < NS; ++t) v[t] = act ? lx.v[t * lx.vs + js] : -1;
+                ok[k] = act && bind_entry_x(lv, G, R, v, e[k], x[k]);
+                int r2i = 0, r2e = 1;
+                swpk[k] = 0;
+                if (act) {
+                    r2i = lx.oi[js];
+                    r2e = lx.oe[js];
+                    swpk[k] = r2e < 0 ? 1 : 0;
+                    r2e = swpk[k] ? ~r2e : r2e;
+                }
+                lo[k] = r2i;
+                hi[k] = r2e - 1;
+            }
+            // the searched rows' end values, then up to three interpolation probes, all in lockstep
+#pragma unroll
+            for (int k = 0; k < U; ++k) {
+                vl[k] = G.col[lo[k]];
+                vh[k] = G.col[hi[k]];
+            }
+#pragma unroll
+            for (int k = 0; k < U; ++k) {
+                res[k] = -1;
+                go[k] = ok[k] && x[k] >= vl[k] && x[k] <= vh[k];
+                if (go[k] && x[k] == vl[k]) {
+                    res[k] = lo[k];
+                    go[k] = false;
+                }
+                if (go[k] && x[k] == vh[k]) {
+                    res[k] = hi[k];
+                    go[k] = false;
+                }
+            }
+#pragma unroll
+            for (int it = 0; it < 3; ++it) {
+                int m[U], y[U];
+#pragma unroll
+                for (int k = 0; k < U; ++k) {
+                    const bool p = go[k] && hi[k] - lo[k] > 8;
+                    int mm = lo[k];
+                    if (p) {
+                        const float f = (float)(x[k] - vl[k]) / (float)(vh[k] - vl[k]);
+                        mm = lo[k] + (int)(f * (float)(hi[k] - lo[k]));
+                        mm = mm <= lo[k] ? lo[k] + 1 : (mm >= hi[k] ? hi[k] - 1 : mm);
+                    }
+                    m[k] = mm;
+                }
+#pragma unroll
+                for (int k = 0; k < U; ++k) y[k] = G.col[m[k]];
+#pragma unroll
+                for (int k = 0; k < U; ++k) {
+                    if (!(go[k] && hi[k] - lo[k] > 8)) continue;
+                    if (y[k] == x[k]) {
+                        res[k] = m[k];
+                        go[k] = false;
+                    } else if (y[k] < x[k]) {
+                        lo[k] = m[k];
+                        vl[k] = y[k];
+                    } else {
+                        hi[k] = m[k];
+                        vh[k] = y[k];
+                    }
+                }
+            }
+            // binary search of (lo, hi), in lockstep
+            int a[U], b[U];
+#pragma unroll
+            for (int k = 0; k < U; ++k) {
+                a[k] = lo[k] + 1;
+                b[k] = hi[k];
+            }
+            for (;;) {
+                bool any = false;
+#pragma unroll
+                for (int k = 0; k < U; ++k) any = any || (go[k] && a[k] < b[k]);
+                if (!__any(any)) break;
+                int m[U], y[U];
+#pragma unroll
+                for (int k = 0; k < U; ++k) m[k] = (go[k] && a[k] < b[k]) ? (a[k] + b[k]) >> 1 : lo[k];
+#pragma unroll
+                for (int k = 0; k < U; ++k) y[k] = G.col[m[k]];
+#pragma unroll
+                for (int k = 0; k < U; ++k) {
+                    if (!(go[k] && a[k] < b[k])) continue;
+                    if (y[k] < x[k]) a[k] = m[k] + 1;
+                    else b[k] = m[k];
+                }
+            }
+            {
+                int y[U];
+#pragma unroll
+                for (int k = 0; k < U; ++k) y[k] = G.col[go[k] && a[k] < hi[k] ? a[k] : lo[k]];
+#pragma unroll
+                for (int k = 0; k < U; ++k)
+                    if (go[k] && a[k] < hi[k] && y[k] == x[k]) res[k] = a[k];
+            }
+            // groundings
+#pragma unroll
+            for (int k = 0; k < U; ++k) {
+                if (res[k] < 0) continue;
+                if (!pairs) {
+                    found = true;
+                } else if (X == pl) {   // the ISECT binds the head object: each match is one
+                    const int e1 = swpk[k] ? res[k] : e[k];
+                    const int h1 = byEntry ? G.crank[e1] : dd.rank(x[k]);
+                    if (!dd.seen_r(h1)) mine += dd.insert_r(h1);
+                } else {                // head object bound above: the grounding settles it
+                    mine += dd.insert_r(hrk[k]);
+                }
+            }
+            nCand += (uint64_t)T;
+            radd(hd, X, pre);
+            radd(ct, X, -pre);
+            ++nBal;
+            GR_PROF_ADD(4, tp);
         }
         if (!streamed && X == last && fastLeaf) {
             // Last level, plain EXT: the items are cut into chunks of kLsLeafChunk
