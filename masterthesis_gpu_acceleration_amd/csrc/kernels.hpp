@@ -56,9 +56,10 @@ struct SortSink {
 // cuts each listed seed into sub-items -- ranges of its level-0 row, or of
 // its level-1 rows when the level-0 row is short, each with the bindings
 // above -- and a sub-item kernel runs them on all waves. Sub-items of one
-// giant count against one HBM bitmap row (PAIRS: marked directly, or, with
-// LDS bitmaps, ORed in from the sub-item's own bitmap when it ends; the bits
-// an atomicOr sets first count each head object once) or share one found
+// giant count against one HBM bitmap row (PAIRS: marked directly, the bits an
+// atomicOr sets first counting each head object once, or, with LDS bitmaps,
+// ORed in from the sub-item's own bitmap when it ends and counted whole by
+// the giant's last sub-item) or share one found
 // flag (SUPPORT); the sub-item that finishes last clears the row. Nothing
 // waits on another wave.
 struct alignas(16) SubItem {

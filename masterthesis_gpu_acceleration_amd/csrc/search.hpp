@@ -1007,19 +1007,18 @@ __device__ __forceinline__ uint64_t seed_finish(Dedup<D, NS>& dd, const DevGraph
         }
     }
     if (D == kDedupLds) {
-        // a giant's sub-item ORs its bitmap into the giant's row: the bits
-        // that OR sets first are the objects it adds (each bit is set first
-        // exactly once over all sub-items)
-        uint64_t fresh = 0;
+        // a giant's sub-item ORs its bitmap into the giant's row (atomics with no
+        // return value, so none is waited for); the giant's last sub-item counts
+        // the row whole, so a sub-item counts nothing itself
         for (int64_t w = lane; w < rowWords; w += kWave) {
             if (dd.merge) {
                 const uint32_t x = dd.bits[w];
-                if (x) fresh += (uint64_t)__popc(x & ~atomicOr(dd.merge + w, x));
+                if (x) (void)__hip_atomic_fetch_or(dd.merge + w, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
             dd.clear_word((uint32_t)w);
         }
         wave_sync();
-        if (dd.merge) mine = fresh;
+        if (dd.merge) mine = 0;
     } else {
         const uint32_t nt = *dd.nTouched;
         if (nt <= (uint32_t)kTouchCap) {
@@ -2533,16 +2532,26 @@ __global__ void __launch_bounds__(kWave) k_wave_sub(WaveArgs a, DevGraph G) {
             dd.merge = nullptr;
         }
         if (lane == 0 && c) atomicAdd(a.counts + 2 * W.plan.rule + K, (unsigned long long)c);
-        // fused support: the giant's row is complete once its last sub-item
-        // is done; that sub-item counts the seed's supported head triples
+        // The giant's row is complete once its last sub-item is done: that
+        // sub-item counts the seed's supported head triples from it (fused
+        // support) and, with LDS bitmaps, the seed's distinct head objects (the
+        // set bits; the sub-items merged their bitmaps without counting).
         const bool fs = K == kPairs && D != kDedupSort && a.fuse && W.plan.supPred != -1;
-        if (fs) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const bool rowCount = K == kPairs && D == kDedupLds;
+        if (fs || rowCount) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         uint32_t last = 0;
         if (lane == 0) last = atomicSub(ga.remaining + g, 1u) == 1u ? 1u : 0u;
         if (bcast(last) && K == kPairs && D != kDedupSort) {
             uint32_t* row = ga.rows + (int64_t)g * a.rowWords;
+            if (fs || rowCount) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            if (rowCount) {
+                uint64_t nb = 0;
+                for (int64_t w = lane; w < a.rowWords; w += kWave)
+                    nb += (uint64_t)__popc(__hip_atomic_load(row + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+                nb = wave_sum(nb);
+                if (lane == 0 && nb) atomicAdd(a.counts + 2 * W.plan.rule + K, (unsigned long long)nb);
+            }
             if (fs) {
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
                 const int supPred = W.plan.supPred;
                 const int32_t* supList = supPred <= -2 ? a.supTab + (-supPred - 2) : nullptr;
                 const int s0 = uniform(S.v[0]);
