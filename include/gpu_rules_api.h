@@ -149,6 +149,27 @@ void* gpu_batch_create(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t f
 int gpu_batch_run(void* batch, GpuRuleResult64* out);
 void gpu_batch_free(void* batch);
 
+/* Support-pruned evaluation for a rule miner's refinement loop. No
+ * counterpart in the reference's ABI: the reference evaluates every rule
+ * fully and leaves pruning to the miner, whose support computation stops a
+ * rule once it cannot reach minSupport (Rewritten_CPP/SupportCounting.cpp:
+ * 66-90 restates RDFRules' computeSupport with minSupport = 1). Here the
+ * miner's thresholds go to the device: every rule's support is counted
+ * exactly, and body pairs only for rules with support >= minSupport and
+ * headCoverage = support / headSize >= minHeadCoverage (the same double
+ * quotient as the result's headCoverage); thresholds below 1 act as 1, the
+ * gate of every gated batch. A rule below them keeps status 0 and its exact
+ * support, headSize, headSupport and headCoverage, with bodyPairs = -1,
+ * bodySize = 0 and confidence = 0. Body pairs shared by rules with equal
+ * bodies are reported only to the rules that meet the thresholds. Such a
+ * batch counts support before body pairs (no support in the body-pair
+ * pass). flags: 0 or GPU_RULES_PER_RULE; GPU_RULES_UNGATED, a NaN, negative
+ * or > 1 minHeadCoverage return -1 (NULL). */
+int gpu_evaluate_rules_min(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags, int64_t minSupport,
+                           double minHeadCoverage, GpuRuleResult64* out);
+void* gpu_batch_create_min(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags, int64_t minSupport,
+                           double minHeadCoverage);
+
 /* Device time of the last gpu_evaluate_rules* call (ms) and its phases. */
 int gpu_last_batch_stats(void* ctx, double* totalMs, double* supportMs, double* pairsMs, int64_t* heavySeeds);
 
@@ -319,6 +340,12 @@ int gpu_host_parse_rule(void* handle, const char* rule, int32_t isJson, int32_t*
  * -1 on a negative argument or unknown `which`. Both counters are 64-bit
  * or bounded on the device, so they cannot wrap. */
 int32_t gpu_host_count_action(int32_t which, int64_t count, int64_t cap, int64_t capMax);
+
+/* The support gate of gpu_evaluate_rules_min for a rule of headSize head
+ * triples: the smallest support that meets minSupport and minHeadCoverage
+ * (at least 1; 2^31 - 1 when none can), or -1 for a threshold the entry
+ * points refuse. A host check with no counterpart in the reference. */
+int64_t gpu_host_support_threshold(int64_t headSize, int64_t minSupport, double minHeadCoverage);
 
 /* Head figures and plan facts of a structured rule: info[0] headSize,
  * info[1] headSupport, info[2] status, info[3] support launched (0/1),

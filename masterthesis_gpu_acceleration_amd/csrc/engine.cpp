@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <array>
 #include <chrono>
+#include <cmath>
 #include <climits>
 #include <cstdio>
 #include <cstdlib>
@@ -341,13 +342,31 @@ bool run_one(Context& C, OneScratch S, const Plan& P, const ConflictRefs& R, con
     }
 }
 
+uint32_t support_threshold(int64_t headSize, const Thresholds& t) {
+    int64_t thr = std::max<int64_t>(1, t.minSupport);
+    if (t.minHeadCoverage > 0.0) {
+        if (headSize <= 0) return kGateMax;   // coverage 0: no support meets a positive minimum
+        // the smallest s with s / headSize >= minHeadCoverage in the same
+        // double division as the reported headCoverage
+        const double h = (double)headSize;
+        int64_t s = (int64_t)std::ceil(t.minHeadCoverage * h);
+        if (s > headSize + 1) s = headSize + 1;   // more than every head triple: never met
+        while (s > 0 && (double)(s - 1) / h >= t.minHeadCoverage) --s;
+        while (s <= headSize && (double)s / h < t.minHeadCoverage) ++s;
+        thr = std::max(thr, s);
+    }
+    return (uint32_t)std::min<int64_t>(thr, kGateMax);
+}
+
+bool Context::fuses(const Batch& B, int mode) const { return fuseActive(mode) && !B.thr.active(); }
+
 void finish_measures(GpuRuleResult64& r) {
     r.headCoverage = r.headSize > 0 ? (double)r.support / (double)r.headSize : 0.0;
     r.confidence = r.bodySize > 0 ? (double)r.support / (double)r.bodySize : 0.0;
 }
 
-bool eval_rule_plain(Context& C, OneScratch S, const RuleIR& r, bool gated, GpuRuleResult64& res, double& supMs,
-                     double& confMs) {
+bool eval_rule_plain(Context& C, OneScratch S, const RuleIR& r, bool gated, const Thresholds& thr,
+                     GpuRuleResult64& res, double& supMs, double& confMs) {
     std::memset(&res, 0, sizeof(res));
     res.bodyPairs = -1;
     supMs = confMs = 0;
@@ -365,7 +384,7 @@ bool eval_rule_plain(Context& C, OneScratch S, const RuleIR& r, bool gated, GpuR
         !run_one(C, S, cr.sup, cr.supRefs, cr.supWave, cr.supWaveRefs, cr.supItems, cut, sup, supMs, heavy))
         return false;
     res.support = (int64_t)sup;
-    if (!gated || sup > 0) {
+    if (!gated || sup >= support_threshold(cr.headSize, thr)) {
         uint64_t pairs = 0;
         if (cr.pairsLaunch &&
             !run_one(C, S, cr.pairs, cr.pairsRefs, cr.pairsWave, cr.pairsWaveRefs, cr.pairItems, cut, pairs, confMs,
@@ -485,6 +504,10 @@ bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const s
         for (auto& e : errs)
             if (e) std::rethrow_exception(e);
     }
+    B.ruleThr.assign(rules.size(), 1u);
+    if (gated)
+        for (size_t i = 0; i < rules.size(); ++i)
+            if (B.status[i] == kRuleOk) B.ruleThr[i] = support_threshold(B.comp[i].headSize, B.thr);
     // rules with the body of an earlier rule take its body-pair count
     B.pairsFrom = C.shareBodies ? shared_pairs(rules, B.comp, B.status) : std::vector<int32_t>(rules.size(), -1);
     std::vector<std::vector<int32_t>> members(rules.size());
@@ -555,7 +578,9 @@ bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const s
             for (const Job& j : jobs) {
                 if (j.ph != ph || (ph == kSupport && (int)j.fus != pass)) continue;
                 // a search whose count other rules share runs whatever its own rule's support
-                const uint32_t gate = (ph == kPairs && gated && !j.shared) ? (j.fus ? 2u : 1u) : 0u;
+                const uint32_t gate = (ph == kPairs && gated && !j.shared)
+                                          ? (B.ruleThr[j.k / 2] | (j.fus ? kGateFused : 0u))
+                                          : 0u;
                 for (int64_t b = 0; b < j.n; b += chunk) {
                     lists[ph].push_back(
                         Chunk{(uint32_t)j.k, (uint32_t)b, (uint32_t)std::min<int64_t>(chunk, j.n - b), gate});
@@ -619,7 +644,7 @@ hipError_t batch_enqueue(Context& C, Batch& B, const ExecTarget& t) {
     hipStream_t st = t.st;
     uint32_t* ctl = B.ctl.as<uint32_t>();
     unsigned long long* counts = B.counts.as<unsigned long long>();
-    const bool fuse = C.fuseActive(t.pairsDedup);
+    const bool fuse = C.fuses(B, t.pairsDedup);
     const uint32_t ncPh[2] = {B.nChunks[0] - (fuse ? B.nFusable : 0u), B.nChunks[1]};
     // ungated: no pairs chunk reads a support count, so the phases may overlap
     const bool fork = t.st2 && t.giants2 && !t.ev && (B.flags & GPU_RULES_UNGATED) && ncPh[0] && ncPh[1];
@@ -730,7 +755,8 @@ void batch_collect(Batch& B, GpuRuleResult64* out) {
         if (r.status != kRuleOk) continue;
         r.support = cr.supportLaunch ? (int64_t)B.hCounts[2 * i] : 0;
         const size_t src = B.pairsFrom.empty() || B.pairsFrom[i] < 0 ? (size_t)i : (size_t)B.pairsFrom[i];
-        if (!gated || r.support > 0) r.bodyPairs = cr.pairsLaunch ? (int64_t)B.hCounts[2 * src + 1] : 0;
+        if (!gated || r.support >= (int64_t)B.ruleThr[i])
+            r.bodyPairs = cr.pairsLaunch ? (int64_t)B.hCounts[2 * src + 1] : 0;
         r.bodySize = r.support > 0 && r.bodyPairs > 0 ? r.bodyPairs : 0;
         finish_measures(r);
     }
@@ -755,7 +781,7 @@ bool batch_run(Context& C, Batch& B, GpuRuleResult64* out) {
                            B.capturedMode != mode || B.capturedGiants != C.giants.rows.p ||
                            B.capturedGiantAfter != C.giantAfter || B.capturedOverlap != (int)C.overlap ||
                            B.capturedEngine != C.waveEngine ||
-                           B.capturedFuse != (int)C.fuseActive(mode))) {
+                           B.capturedFuse != (int)C.fuses(B, mode))) {
                 (void)hipGraphExecDestroy(B.exec);
                 (void)hipGraphDestroy(B.graph);
                 B.exec = nullptr;
@@ -772,7 +798,7 @@ bool batch_run(Context& C, Batch& B, GpuRuleResult64* out) {
                 B.capturedGiantAfter = C.giantAfter;
                 B.capturedEngine = C.waveEngine;
                 B.capturedOverlap = (int)C.overlap;
-                B.capturedFuse = (int)C.fuseActive(mode);
+                B.capturedFuse = (int)C.fuses(B, mode);
                 GR_HIP(hipStreamBeginCapture(C.st, hipStreamCaptureModeThreadLocal));
                 ExecTarget t{C.st, C.dedup.as<uint32_t>(), C.rowWords, C.waveBlocks(), nullptr, sb,
                              C.sortBlocks(), mode, &C.giants, C.giantAfter};
@@ -846,10 +872,10 @@ bool batch_run(Context& C, Batch& B, GpuRuleResult64* out) {
     C.lastHeavy = B.lastHeavy;
     C.lastEmitted = sb ? B.lastEmitted : 0;
     C.lastGiants = (int64_t)B.hGiant[0] + B.hGiant[1];
-    C.lastFused = C.fuseActive(mode) ? B.nFused : 0;
+    C.lastFused = C.fuses(B, mode) ? B.nFused : 0;
     C.lastShared = B.nShared;
     // fused support chunks do not run: those rules' support is counted by the pairs searches
-    C.lastSearches[0] = B.nSearches[0] - (C.fuseActive(mode) ? B.nFused : 0);
+    C.lastSearches[0] = B.nSearches[0] - (C.fuses(B, mode) ? B.nFused : 0);
     C.lastSearches[1] = B.nSearches[1];
     for (int k = 0; k < 2 * kWaveStats; ++k) C.lastWaveStats[k] = B.hWaveStats[k];
     for (int k = 0; k < 4; ++k) C.lastKernelMs[k] = B.kernelMs[k];
@@ -1034,7 +1060,7 @@ void gpu_free_graph(void* ctx) {
 static int eval_one_abi(Context* C, const RuleIR& r, GpuRuleResult* out, OneScratch S) {
     GpuRuleResult64 res;
     double supMs = 0, confMs = 0;
-    if (!eval_rule_plain(*C, S, r, true, res, supMs, confMs)) return -1;
+    if (!eval_rule_plain(*C, S, r, true, Thresholds{}, res, supMs, confMs)) return -1;
     out->support = (int)res.support;
     out->headSize = (int)res.headSize;
     out->headSupport = (int)res.headSupport;
@@ -1095,7 +1121,7 @@ int gpu_evaluate_rule_json(void* ctx, const char* ruleJson, GpuRuleResult* out) 
 }
 
 static int eval_batch(Context* C, std::vector<RuleIR>& rules, std::vector<int32_t>& st, int32_t flags,
-                      GpuRuleResult64* out) {
+                      GpuRuleResult64* out, const Thresholds& thr = Thresholds{}) {
     if (flags & GPU_RULES_PER_RULE) {
         const bool gated = !(flags & GPU_RULES_UNGATED);
         double tot = 0, ts = 0, tp = 0;
@@ -1111,7 +1137,7 @@ static int eval_batch(Context* C, std::vector<RuleIR>& rules, std::vector<int32_
             }
             double a = 0, b = 0;
             const auto t0 = std::chrono::steady_clock::now();
-            if (!eval_rule_plain(*C, context_scratch(*C), rules[i], gated, out[i], a, b)) return -1;
+            if (!eval_rule_plain(*C, context_scratch(*C), rules[i], gated, thr, out[i], a, b)) return -1;
             const auto t1 = std::chrono::steady_clock::now();
             out[i].latencyUs = (int32_t)std::min<int64_t>(
                 std::chrono::duration_cast<std::chrono::microseconds>(t1 - t0).count(), INT32_MAX);
@@ -1127,6 +1153,7 @@ static int eval_batch(Context* C, std::vector<RuleIR>& rules, std::vector<int32_
     C->lastBatch = nullptr;
     auto* B = new Batch();
     B->magic = kMagicBatch;
+    B->thr = thr;
     if (!batch_build(*C, *B, rules, st, flags, C->st) || !batch_run(*C, *B, out)) {
         delete B;
         return -1;
@@ -1171,7 +1198,34 @@ int gpu_evaluate_rules_text(void* ctx, const char* const* rules, int32_t n, int3
     }
 }
 
-void* gpu_batch_create(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags) {
+// Thresholds of the *_min entry points: NaN, a negative or a > 1 head
+// coverage, and an ungated batch are refused.
+static bool thresholds_of(int64_t minSupport, double minHeadCoverage, int32_t flags, Thresholds& t) {
+    if (flags & GPU_RULES_UNGATED) return false;
+    if (!(minHeadCoverage >= 0.0 && minHeadCoverage <= 1.0)) return false;
+    t.minSupport = minSupport;
+    t.minHeadCoverage = minHeadCoverage;
+    return true;
+}
+
+int gpu_evaluate_rules_min(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags, int64_t minSupport,
+                           double minHeadCoverage, GpuRuleResult64* out) {
+    Context* C = as_ctx(ctx);
+    Thresholds t;
+    if (!C || (!rules && n > 0) || (!out && n > 0) || n < 0 || !thresholds_of(minSupport, minHeadCoverage, flags, t))
+        return -1;
+    try {
+        std::vector<RuleIR> rs(n);
+        std::vector<int32_t> st(n, kRuleOk);
+        for (int32_t i = 0; i < n; ++i) rs[i] = rule_from_words(rules[i].words);
+        return eval_batch(C, rs, st, flags, out, t);
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "[gpu_rules] gpu_evaluate_rules_min error: %s\n", e.what());
+        return -1;
+    }
+}
+
+static void* batch_create(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags, const Thresholds& thr) {
     Context* C = as_ctx(ctx);
     if (!C || (!rules && n > 0) || n < 0) return nullptr;
     try {
@@ -1180,6 +1234,7 @@ void* gpu_batch_create(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t f
         for (int32_t i = 0; i < n; ++i) rs[i] = rule_from_words(rules[i].words);
         auto* B = new Batch();
         B->magic = kMagicBatch;
+        B->thr = thr;
         if (!batch_build(*C, *B, rs, st, flags & GPU_RULES_UNGATED, C->st)) {
             delete B;
             return nullptr;
@@ -1189,6 +1244,17 @@ void* gpu_batch_create(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t f
         std::fprintf(stderr, "[gpu_rules] gpu_batch_create error: %s\n", e.what());
         return nullptr;
     }
+}
+
+void* gpu_batch_create(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags) {
+    return batch_create(ctx, rules, n, flags, Thresholds{});
+}
+
+void* gpu_batch_create_min(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags, int64_t minSupport,
+                           double minHeadCoverage) {
+    Thresholds t;
+    if (!thresholds_of(minSupport, minHeadCoverage, flags, t)) return nullptr;
+    return batch_create(ctx, rules, n, flags, t);
 }
 
 int gpu_batch_run(void* batch, GpuRuleResult64* out) {
@@ -1670,6 +1736,12 @@ int32_t gpu_host_count_action(int32_t which, int64_t count, int64_t cap, int64_t
     if (which == 0) return sort_count_action((uint64_t)count, (uint64_t)cap, (uint64_t)capMax);
     if (which == 1) return heavy_count_action((uint64_t)count, (uint64_t)cap);
     return -1;
+}
+
+int64_t gpu_host_support_threshold(int64_t headSize, int64_t minSupport, double minHeadCoverage) {
+    Thresholds t;
+    if (headSize < 0 || !thresholds_of(minSupport, minHeadCoverage, 0, t)) return -1;
+    return (int64_t)support_threshold(headSize, t);
 }
 
 int gpu_host_rules_work(void* handle, const int32_t* words, int32_t n, double* work) {

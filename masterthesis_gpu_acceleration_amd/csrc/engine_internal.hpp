@@ -211,11 +211,28 @@ struct HipFirst {
     bool ok() const { return e == hipSuccess; }
 };
 
+// Support thresholds of a gated batch: body pairs are counted only for rules
+// whose support reaches minSupport and whose head coverage (support /
+// headSize, the IEEE quotient the results report) reaches minHeadCoverage.
+// The defaults (1, 0) are the gate of every gated batch: support > 0.
+struct Thresholds {
+    int64_t minSupport = 1;
+    double minHeadCoverage = 0.0;
+    bool active() const { return minSupport > 1 || minHeadCoverage > 0.0; }
+};
+
+// The smallest support that meets t for a rule of headSize head triples, at
+// least 1, clamped to kGateMax (support never exceeds the triple count).
+constexpr uint32_t kGateMax = 0x7fffffffu;
+uint32_t support_threshold(int64_t headSize, const Thresholds& t);
+
 // A compiled, device-resident batch of rules.
 struct Batch : Handle {
     Context* ctx = nullptr;
     int32_t n = 0;
     int32_t flags = 0;
+    Thresholds thr;                 // gated batches: the gate of the pairs phase
+    std::vector<uint32_t> ruleThr;  // per rule: support_threshold (gated batches)
     std::vector<CompiledRule> comp;
     std::vector<int32_t> status;
     // chunk lists and heavy-queue segments per phase (0 support, 1 pairs)
@@ -325,6 +342,9 @@ struct Context : Handle {
     // one body-pair search.
     bool shareBodies = true;
     bool fuseActive(int mode) const { return fuseSupport && (mode == kDedupRows || mode == kDedupLds); }
+    // A batch with support thresholds above the default gate needs every
+    // rule's support before its pairs phase, so its pairs launch never fuses.
+    bool fuses(const Batch& B, int mode) const;
     int64_t lastGiants = 0;
     int64_t lastFused = 0;   // rules whose support the last batch counted in its pairs pass
     int64_t lastShared = 0;  // rules of the last batch that took another rule's body-pair count

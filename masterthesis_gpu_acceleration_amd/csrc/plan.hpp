@@ -129,12 +129,18 @@ inline int plan_slots(const Plan& P, const ConflictRefs& R) {
 // The slot class whose kernels run a launch of plans needing `slots` (plan_slots).
 constexpr int slot_class(int slots) { return slots <= 4 ? 4 : (slots <= 8 ? 8 : 16); }
 
+// Chunk::gate of a PAIRS chunk: bits 0-30 a support threshold (0: none), the
+// chunk is skipped when the rule's support counter is below it; with
+// kGateFused set, only when the launch does not fuse support (the pairs pass
+// itself then counts that support).
+constexpr uint32_t kGateFused = 0x80000000u;
+
 // One unit of thread-mode work: up to kChunkSeeds consecutive seeds of one plan.
 struct alignas(16) Chunk {
     uint32_t plan;       // index into the plan table
     uint32_t begin;      // first seed (relative to the plan's seed range)
     uint32_t count;      // seeds in this chunk (<= kChunkSeeds)
-    uint32_t gate;       // PAIRS chunks: 1 skip if the rule's support is 0; 2 the same unless the launch fuses support
+    uint32_t gate;       // PAIRS chunks: support threshold | kGateFused, or 0
 };
 
 // A seed escalated from thread mode to wave mode.

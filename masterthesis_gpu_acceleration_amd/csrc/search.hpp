@@ -2212,7 +2212,8 @@ __global__ void __launch_bounds__(kBlock, kThreadMinWaves) k_thread_batch(Thread
         const uint32_t c = wave_take(a.cursor);
         if (c >= a.nChunks) break;
         const Chunk ch = a.chunks[c];
-        if ((ch.gate == 1u || (ch.gate == 2u && !a.fuse)) && a.counts[2 * a.plans[ch.plan].rule + kSupport] == 0)
+        const uint32_t thr = ch.gate & ~kGateFused;
+        if (thr && (!(ch.gate & kGateFused) || !a.fuse) && a.counts[2 * a.plans[ch.plan].rule + kSupport] < thr)
             continue;
         if (ch.plan != loaded) {
             copy_plan(W, a.plans + ch.plan);

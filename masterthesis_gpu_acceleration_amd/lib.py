@@ -61,6 +61,10 @@ _SIGS = {
                                             ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                             ctypes.POINTER(ctypes.c_int64)]),
     "gpu_batch_create": (ctypes.c_void_p, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
+    "gpu_evaluate_rules_min": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                                              ctypes.c_int64, ctypes.c_double, ctypes.c_void_p]),
+    "gpu_batch_create_min": (ctypes.c_void_p, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                                               ctypes.c_int64, ctypes.c_double]),
     "gpu_batch_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "gpu_batch_free": (None, [ctypes.c_void_p]),
     "gpu_copy_probe": (ctypes.c_double, [ctypes.c_void_p, ctypes.c_int64]),
@@ -87,6 +91,7 @@ _SIGS = {
     "gpu_host_plan_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "gpu_host_rules_work": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]),
     "gpu_host_count_action": (ctypes.c_int32, [ctypes.c_int32, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64]),
+    "gpu_host_support_threshold": (ctypes.c_int64, [ctypes.c_int64, ctypes.c_int64, ctypes.c_double]),
     "gpu_host_term_string": (ctypes.c_int64, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_char_p, ctypes.c_int64]),
     "gpu_rules_version": (ctypes.c_char_p, []),
 }
@@ -275,6 +280,20 @@ class Engine:
             raise RuntimeError("gpu_evaluate_rules returned %d" % rc)
         return out
 
+    def evaluate_min(self, rule_words: np.ndarray, min_support: int = 1, min_head_coverage: float = 0.0,
+                     flags: int = 0) -> np.ndarray:
+        """gpu_evaluate_rules_min: exact support of every rule, body pairs only
+        for rules with support >= min_support and head coverage >=
+        min_head_coverage (bodyPairs -1 for the others)."""
+        w = np.ascontiguousarray(rule_words, np.int32)
+        n = int(w.shape[0])
+        out = np.zeros(n, RESULT64_DTYPE)
+        rc = self.lib.gpu_evaluate_rules_min(self.ctx, w.ctypes.data, n, int(flags), int(min_support),
+                                             float(min_head_coverage), out.ctypes.data)
+        if rc != 0:
+            raise RuntimeError("gpu_evaluate_rules_min returned %d" % rc)
+        return out
+
     def evaluate_text(self, rules, is_json: bool = False, flags: int = 0) -> np.ndarray:
         n = len(rules)
         arr = (ctypes.c_char_p * n)(*[r.encode() for r in rules])
@@ -284,8 +303,9 @@ class Engine:
             raise RuntimeError("gpu_evaluate_rules_text returned %d" % rc)
         return out
 
-    def batch(self, rule_words: np.ndarray, flags: int = 0) -> "ResidentBatch":
-        return ResidentBatch(self, rule_words, flags)
+    def batch(self, rule_words: np.ndarray, flags: int = 0, min_support=None,
+              min_head_coverage: float = 0.0) -> "ResidentBatch":
+        return ResidentBatch(self, rule_words, flags, min_support, min_head_coverage)
 
     def last_stats(self):
         t, s, p = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
@@ -359,11 +379,17 @@ class ResidentBatch:
     """gpu_batch_create / gpu_batch_run / gpu_batch_free: rules compiled once,
     kept in HBM, evaluated by replaying the batch's HIP graph."""
 
-    def __init__(self, engine: Engine, rule_words: np.ndarray, flags: int = 0):
+    def __init__(self, engine: Engine, rule_words: np.ndarray, flags: int = 0, min_support=None,
+                 min_head_coverage: float = 0.0):
         self.engine = engine
         w = np.ascontiguousarray(rule_words, np.int32)
         self.n = int(w.shape[0])
-        self.h = engine.lib.gpu_batch_create(engine.ctx, w.ctypes.data, self.n, int(flags))
+        if min_support is None and not min_head_coverage:
+            self.h = engine.lib.gpu_batch_create(engine.ctx, w.ctypes.data, self.n, int(flags))
+        else:   # gpu_batch_create_min: support thresholds gate the body-pair phase
+            self.h = engine.lib.gpu_batch_create_min(engine.ctx, w.ctypes.data, self.n, int(flags),
+                                                     int(1 if min_support is None else min_support),
+                                                     float(min_head_coverage))
         if not self.h:
             raise RuntimeError("gpu_batch_create failed")
         self.out = np.zeros(self.n, RESULT64_DTYPE)

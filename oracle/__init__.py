@@ -41,6 +41,8 @@ def _load():
         lib.oracle_graph_free.argtypes = [ctypes.c_void_p]
         lib.oracle_eval.restype = ctypes.c_int
         lib.oracle_eval.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]
+        lib.oracle_eval_support.restype = ctypes.c_int
+        lib.oracle_eval_support.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]
         _lib = lib
     return _lib
 
@@ -58,6 +60,13 @@ class Oracle:
         w = np.ascontiguousarray(rule_words, np.int32)
         out = np.zeros((w.shape[0], 7), np.int64)
         _load().oracle_eval(self._h, w.ctypes.data, int(w.shape[0]), out.ctypes.data)
+        return out
+
+    def eval_support(self, rule_words: np.ndarray) -> np.ndarray:
+        """eval() without the body-pair search (bodyPairs = bodySize = 0)."""
+        w = np.ascontiguousarray(rule_words, np.int32)
+        out = np.zeros((w.shape[0], 7), np.int64)
+        _load().oracle_eval_support(self._h, w.ctypes.data, int(w.shape[0]), out.ctypes.data)
         return out
 
     def close(self):

@@ -209,7 +209,7 @@ struct Result {
     int64_t support, headSize, headSupport, bodyPairs, bodySize, skipped, nodes;
 };
 
-Result evalRule(const Graph& g, const Rule& r) {
+Result evalRule(const Graph& g, const Rule& r, bool withPairs = true) {
     Result res{};
     const PredIdx* hp = g.get(r.head.p);
     res.headSize = hp ? (int64_t)hp->spo.size() : 0;
@@ -242,7 +242,7 @@ Result evalRule(const Graph& g, const Rule& r) {
     }
 
     // ---- distinct body pairs
-    bool closed = r.head.s.var && r.head.o.var;
+    bool closed = withPairs && r.head.s.var && r.head.o.var;
     if (closed) {
         bool sIn = false, oIn = false;
         for (auto& a : r.body) {
@@ -294,7 +294,7 @@ void oracle_graph_free(void* g) { delete static_cast<OracleGraph*>(g); }
 // rules: n_rules records of 28 int32 words: [nBody, head(s,p,o), body[8](s,p,o)].
 // out: n_rules records of 7 int64: support, headSize, headSupport, bodyPairs,
 // bodySize (gated), skipped, nodes.
-int oracle_eval(void* gp, const int32_t* rules, int64_t n_rules, int64_t* out) {
+static int eval_records(void* gp, const int32_t* rules, int64_t n_rules, int64_t* out, bool withPairs) {
     const Graph& g = static_cast<OracleGraph*>(gp)->g;
     for (int64_t i = 0; i < n_rules; ++i) {
         const int32_t* w = rules + i * 28;
@@ -316,12 +316,22 @@ int oracle_eval(void* gp, const int32_t* rules, int64_t n_rules, int64_t* out) {
         r.head.p = w[2];
         r.nvars = nv;
         if (nb > 8) r.body.resize(9);  // marks "too long" for the skip rule
-        Result res = evalRule(g, r);
+        Result res = evalRule(g, r, withPairs);
         int64_t* o = out + i * 7;
         o[0] = res.support; o[1] = res.headSize; o[2] = res.headSupport;
         o[3] = res.bodyPairs; o[4] = res.bodySize; o[5] = res.skipped; o[6] = res.nodes;
     }
     return 0;
+}
+
+int oracle_eval(void* gp, const int32_t* rules, int64_t n_rules, int64_t* out) {
+    return eval_records(gp, rules, n_rules, out, true);
+}
+
+// The same records without the body-pair search: bodyPairs and bodySize 0
+// (support and the head figures only).
+int oracle_eval_support(void* gp, const int32_t* rules, int64_t n_rules, int64_t* out) {
+    return eval_records(gp, rules, n_rules, out, false);
 }
 
 }  // extern "C"
