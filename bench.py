@@ -19,6 +19,13 @@ Default run (`python bench.py --gpus 1 --steps K --warmup W`), one JSON line:
   work filter) and, beside it, the same stream with paths of estimated search
   work above 1e8 redrawn; p50/p99/max latency from arrival to result, served
   rate, mean batch, requests routed to heavy batches.
+* "mining" -- a level-wise rule miner (masterthesis_gpu_acceleration_amd.mining)
+  over the headline graph: every candidate of one body length in one
+  support-pruned device batch (gpu_evaluate_rules_min: body pairs only for
+  candidates that meet minSupport and minHeadCoverage), per level
+  candidates, survivors and times; beside it the same loop with body pairs
+  counted for every closed candidate with support > 0 (thresholds applied
+  on the host), which must find the same rules.
 
 Every block also reports:
   * per_rule: the same rules through the reference library's per-rule launch
@@ -45,6 +52,7 @@ from fixed seeds.
 Usage: python bench.py [--steps K] [--warmup W] [--gpus 1]
        python bench.py --workload hub_fanout --only      (one configuration)
        python bench.py --serve                           (serving block only)
+       python bench.py --mining                          (mining block only)
 """
 from __future__ import annotations
 
@@ -583,6 +591,68 @@ def serving_block(args, g):
 
 # ------------------------------------------------------------ main
 
+# ------------------------------------------------------------ mining
+
+class _UnprunedEvaluator:
+    """Body pairs for every closed candidate with support > 0 (a gated
+    batch); open candidates for support only -- a miner that applies its
+    thresholds on the host."""
+
+    def __init__(self, eng, g):
+        self.eng, self.g = eng, g
+        self.device_ms = []
+
+    def __call__(self, rules):
+        from masterthesis_gpu_acceleration_amd import mining as M
+        words = L.rule_words_for_graph(rules, self.g)
+        closed = np.array([M.is_closed(r) for r in rules], bool)
+        out = np.zeros(len(rules), L.RESULT64_DTYPE)
+        ms = 0.0
+        if closed.any():
+            out[closed] = self.eng.evaluate(words[closed], 0)
+            ms += self.eng.last_stats()["total_ms"]
+        if (~closed).any():
+            out[~closed] = self.eng.evaluate_min(words[~closed], M.SUPPORT_ONLY, 0.0)
+            ms += self.eng.last_stats()["total_ms"]
+        self.device_ms.append(ms)
+        return out
+
+
+def mining_block(args, g):
+    from masterthesis_gpu_acceleration_amd import mining as M
+    heads = M.frequent_predicates(g, args.mine_heads)
+    bodies = M.frequent_predicates(g, args.mine_bodies)
+    miner = M.Miner(heads, bodies, args.mine_atoms, args.mine_support, args.mine_hc)
+    cfg = {"graph": "%d triples, %d entities, %d predicates" % (len(g.s), g.n_entities, g.n_predicates),
+           "head_predicates": len(heads), "body_predicates": len(bodies), "max_body_atoms": args.mine_atoms,
+           "min_support": args.mine_support, "min_head_coverage": args.mine_hc,
+           "predicates": "the most frequent ones", "rules": "variables only, closed rules reported"}
+    eng = L.Engine.from_graph(g)
+    try:
+        ev = M.GpuEvaluator(eng, g, miner)
+        mined, st = miner.mine(ev)
+        ev0 = _UnprunedEvaluator(eng, g)
+        mined0, st0 = miner.mine(ev0)
+    finally:
+        eng.close()
+    same = M.results_table(mined) == M.results_table(mined0) and st.candidates == st0.candidates
+    levels = [{"body_atoms": k + 1, "candidates": c, "passed": p, "eval_ms": round(1e3 * e, 2),
+               "device_ms": round(d, 3)}
+              for k, (c, p, e, d) in enumerate(zip(st.candidates, st.passed, st.eval_s, ev.device_ms))]
+    ncand = sum(st.candidates)
+    out = {"config": cfg, "levels": levels, "mined_rules": len(mined), "candidates": ncand,
+           "total_s": round(st.total_s, 3), "eval_s": round(sum(st.eval_s), 3),
+           "device_ms": round(sum(ev.device_ms), 3), "host_refinement_s": round(st.total_s - sum(st.eval_s), 3),
+           "candidates_per_s_evaluated": round(ncand / max(sum(st.eval_s), 1e-9), 1),
+           "without_device_thresholds": {"eval_s": round(sum(st0.eval_s), 3),
+                                         "device_ms": round(sum(ev0.device_ms), 3), "same_rules": same}}
+    log("mining: %d candidates over %d levels, %d rules; evaluation %.2f s (device %.1f ms), refinement %.2f s; "
+        "without device thresholds: evaluation %.2f s (device %.1f ms), same rules: %s" % (
+            ncand, len(levels), len(mined), sum(st.eval_s), sum(ev.device_ms), out["host_refinement_s"],
+            sum(st0.eval_s), sum(ev0.device_ms), same))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -595,6 +665,13 @@ def main():
     ap.add_argument("--gated", action="store_true", help="body pairs only for rules with support > 0")
     ap.add_argument("--serve", action="store_true", help="only the serving block")
     ap.add_argument("--no-serve", action="store_true", help="leave out the serving block")
+    ap.add_argument("--mining", action="store_true", help="only the mining block")
+    ap.add_argument("--no-mining", action="store_true", help="leave out the mining block")
+    ap.add_argument("--mine-heads", type=int, default=10, help="mining: head predicates (most frequent)")
+    ap.add_argument("--mine-bodies", type=int, default=20, help="mining: body predicates (most frequent)")
+    ap.add_argument("--mine-atoms", type=int, default=3)
+    ap.add_argument("--mine-support", type=int, default=10)
+    ap.add_argument("--mine-hc", type=float, default=0.01)
     ap.add_argument("--serve-requests", type=int, default=9000)
     ap.add_argument("--serve-rate", type=float, default=150.0)
     ap.add_argument("--serve-capacity-requests", type=int, default=3000,
@@ -647,6 +724,11 @@ def main():
         print(json.dumps({"metric": METRIC, "serving": serving_block(args, g)}), flush=True)
         return
 
+    if args.mining:
+        g = WORKLOADS[args.workload][0]().graph
+        print(json.dumps({"metric": METRIC, "mining": mining_block(args, g)}), flush=True)
+        return
+
     wl = WORKLOADS[args.workload][0]()
     if args.atoms:
         keep = {int(x) for x in args.atoms.split(",")}
@@ -659,6 +741,8 @@ def main():
         "vs_baseline": None, "dtype": "int32", "data": "synthetic",
     }
     out.update(head)
+    if not args.only and not args.no_mining:
+        out["mining"] = mining_block(args, wl.graph)
     if not args.only and args.workload != "hub_fanout":
         t = time.time()
         hub = W.hub_fanout()

@@ -160,11 +160,12 @@ void gpu_batch_free(void* batch);
  * quotient as the result's headCoverage); thresholds below 1 act as 1, the
  * gate of every gated batch. A rule below them keeps status 0 and its exact
  * support, headSize, headSupport and headCoverage, with bodyPairs = -1,
- * bodySize = 0 and confidence = 0. Body pairs shared by rules with equal
- * bodies are reported only to the rules that meet the thresholds. Such a
- * batch counts support before body pairs (no support in the body-pair
- * pass). flags: 0 or GPU_RULES_PER_RULE; GPU_RULES_UNGATED, a NaN, negative
- * or > 1 minHeadCoverage return -1 (NULL). */
+ * bodySize = 0 and confidence = 0. A rule whose support the body-pair pass
+ * counts itself ("fuse_support"), or whose body-pair search other rules
+ * with equal bodies share, has its body pairs searched whatever its support:
+ * they are reported only if it meets the thresholds. flags: 0 or
+ * GPU_RULES_PER_RULE; GPU_RULES_UNGATED, a NaN, negative or > 1
+ * minHeadCoverage return -1 (NULL). */
 int gpu_evaluate_rules_min(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags, int64_t minSupport,
                            double minHeadCoverage, GpuRuleResult64* out);
 void* gpu_batch_create_min(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags, int64_t minSupport,
@@ -249,7 +250,10 @@ int gpu_autotune(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags, 
  * default all its scratch holds), "sort_cap_max" (the
  * largest capacity they may grow to, 4096 .. 2^31, default 2^31: a pass that
  * emits more keys fails -- gpu_evaluate_rules* and gpu_batch_run return -1,
- * served requests complete with status 2). Returns 0, or -1 for
+ * served requests complete with status 2), "phase_seed_cap" (seeds one
+ * phase of a batch may hold, 1 .. 2^32 - 16, default the largest: the
+ * gpu_evaluate_rules* calls evaluate a larger set of rules in halves,
+ * recursively, gpu_batch_create* return NULL). Returns 0, or -1 for
  * an unknown key or value. gpu_get_tuning also reads "num_cus",
  * "wave_slots", "sort_active" (0/1), "sort_cap", "pairs_dedup" (the
  * DedupMode in use: 0 HBM rows, 1 sort, 2 LDS), "fuse_active" (0/1: the

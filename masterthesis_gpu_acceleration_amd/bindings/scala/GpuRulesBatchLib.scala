@@ -77,6 +77,16 @@ trait GpuRulesBatchNative extends Library {
   def gpu_evaluate_rules(ctx: Pointer, rules: Pointer, n: Int, flags: Int, out: GpuRuleResult64): Int
   // void* gpu_batch_create(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags)
   def gpu_batch_create(ctx: Pointer, rules: Pointer, n: Int, flags: Int): Pointer
+  // int gpu_evaluate_rules_min(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags,
+  //                            int64_t minSupport, double minHeadCoverage, GpuRuleResult64* out)
+  def gpu_evaluate_rules_min(ctx: Pointer, rules: Pointer, n: Int, flags: Int, minSupport: Long,
+                             minHeadCoverage: Double, out: GpuRuleResult64): Int
+  // void* gpu_batch_create_min(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags,
+  //                            int64_t minSupport, double minHeadCoverage)
+  def gpu_batch_create_min(ctx: Pointer, rules: Pointer, n: Int, flags: Int, minSupport: Long,
+                           minHeadCoverage: Double): Pointer
+  // int64_t gpu_host_support_threshold(int64_t headSize, int64_t minSupport, double minHeadCoverage)
+  def gpu_host_support_threshold(headSize: Long, minSupport: Long, minHeadCoverage: Double): Long
   // int gpu_batch_run(void* batch, GpuRuleResult64* out)
   def gpu_batch_run(batch: Pointer, out: GpuRuleResult64): Int
   // void gpu_batch_free(void* batch)
@@ -140,6 +150,19 @@ class GpuRulesBatchLib(val ctx: Pointer) {
     out
   }
 
+  /** A refinement level of a miner: support of every rule, confidence only for the rules with
+   *  support >= minSupport and headCoverage >= minHeadCoverage (bodyPairs -1 for the others). */
+  def evaluateRulesMin(rules: Seq[String], minSupport: Long, minHeadCoverage: Double,
+                       json: Boolean = false): Array[GpuRuleResult64] = {
+    if (rules.isEmpty) return Array.empty
+    val out = new GpuRuleResult64().toArray(rules.length).asInstanceOf[Array[GpuRuleResult64]]
+    val rc = lib.gpu_evaluate_rules_min(ctx, ruleDescs(rules, json), rules.length, 0, minSupport, minHeadCoverage,
+                                        out(0))
+    if (rc != 0) throw new RuntimeException(s"gpu_evaluate_rules_min error code $rc")
+    out.foreach(_.read())
+    out
+  }
+
   /** Structured rules (GPU_RULE_WORDS ints each) of rule lines, for the batch and serving calls. */
   def ruleDescs(rules: Seq[String], json: Boolean = false): Memory = {
     val mem = new Memory(math.max(1L, rules.length.toLong * RuleWords * 4))
@@ -153,9 +176,12 @@ class GpuRulesBatchLib(val ctx: Pointer) {
   }
 
   /** A batch kept resident in HBM and replayed as a HIP graph on every run(). */
-  final class ResidentBatch(rules: Seq[String], ungated: Boolean) extends AutoCloseable {
+  final class ResidentBatch(rules: Seq[String], ungated: Boolean, minSupport: Long = 1L,
+                            minHeadCoverage: Double = 0.0) extends AutoCloseable {
     private val descs = ruleDescs(rules)
-    private val h = lib.gpu_batch_create(ctx, descs, rules.length, if (ungated) Ungated else 0)
+    private val h =
+      if (minSupport <= 1L && minHeadCoverage == 0.0) lib.gpu_batch_create(ctx, descs, rules.length, if (ungated) Ungated else 0)
+      else lib.gpu_batch_create_min(ctx, descs, rules.length, 0, minSupport, minHeadCoverage)
     if (h == null) throw new RuntimeException("gpu_batch_create failed")
     private val out = new GpuRuleResult64().toArray(math.max(1, rules.length)).asInstanceOf[Array[GpuRuleResult64]]
     def run(): Array[GpuRuleResult64] = {

@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <exception>
+#include <functional>
 #include <memory>
 #include <string>
 #include <thread>
@@ -358,8 +359,6 @@ uint32_t support_threshold(int64_t headSize, const Thresholds& t) {
     return (uint32_t)std::min<int64_t>(thr, kGateMax);
 }
 
-bool Context::fuses(const Batch& B, int mode) const { return fuseActive(mode) && !B.thr.active(); }
-
 void finish_measures(GpuRuleResult64& r) {
     r.headCoverage = r.headSize > 0 ? (double)r.support / (double)r.headSize : 0.0;
     r.confidence = r.bodySize > 0 ? (double)r.support / (double)r.bodySize : 0.0;
@@ -589,9 +588,10 @@ bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const s
             }
     }
     for (int ph = 0; ph < 2; ++ph)
-        if (items[ph] > kMaxPhaseSeeds) {
+        if (items[ph] > C.phaseSeedCap) {
             std::fprintf(stderr, "[gpu_rules] a batch phase with %llu seeds exceeds the heavy queue's 32-bit range; "
                                  "split the batch\n", (unsigned long long)items[ph]);
+            B.seedsOverflow = true;
             return false;
         }
     std::vector<Chunk> all;
@@ -644,7 +644,7 @@ hipError_t batch_enqueue(Context& C, Batch& B, const ExecTarget& t) {
     hipStream_t st = t.st;
     uint32_t* ctl = B.ctl.as<uint32_t>();
     unsigned long long* counts = B.counts.as<unsigned long long>();
-    const bool fuse = C.fuses(B, t.pairsDedup);
+    const bool fuse = C.fuseActive(t.pairsDedup);
     const uint32_t ncPh[2] = {B.nChunks[0] - (fuse ? B.nFusable : 0u), B.nChunks[1]};
     // ungated: no pairs chunk reads a support count, so the phases may overlap
     const bool fork = t.st2 && t.giants2 && !t.ev && (B.flags & GPU_RULES_UNGATED) && ncPh[0] && ncPh[1];
@@ -781,7 +781,7 @@ bool batch_run(Context& C, Batch& B, GpuRuleResult64* out) {
                            B.capturedMode != mode || B.capturedGiants != C.giants.rows.p ||
                            B.capturedGiantAfter != C.giantAfter || B.capturedOverlap != (int)C.overlap ||
                            B.capturedEngine != C.waveEngine ||
-                           B.capturedFuse != (int)C.fuses(B, mode))) {
+                           B.capturedFuse != (int)C.fuseActive(mode))) {
                 (void)hipGraphExecDestroy(B.exec);
                 (void)hipGraphDestroy(B.graph);
                 B.exec = nullptr;
@@ -798,7 +798,7 @@ bool batch_run(Context& C, Batch& B, GpuRuleResult64* out) {
                 B.capturedGiantAfter = C.giantAfter;
                 B.capturedEngine = C.waveEngine;
                 B.capturedOverlap = (int)C.overlap;
-                B.capturedFuse = (int)C.fuses(B, mode);
+                B.capturedFuse = (int)C.fuseActive(mode);
                 GR_HIP(hipStreamBeginCapture(C.st, hipStreamCaptureModeThreadLocal));
                 ExecTarget t{C.st, C.dedup.as<uint32_t>(), C.rowWords, C.waveBlocks(), nullptr, sb,
                              C.sortBlocks(), mode, &C.giants, C.giantAfter};
@@ -872,10 +872,10 @@ bool batch_run(Context& C, Batch& B, GpuRuleResult64* out) {
     C.lastHeavy = B.lastHeavy;
     C.lastEmitted = sb ? B.lastEmitted : 0;
     C.lastGiants = (int64_t)B.hGiant[0] + B.hGiant[1];
-    C.lastFused = C.fuses(B, mode) ? B.nFused : 0;
+    C.lastFused = C.fuseActive(mode) ? B.nFused : 0;
     C.lastShared = B.nShared;
     // fused support chunks do not run: those rules' support is counted by the pairs searches
-    C.lastSearches[0] = B.nSearches[0] - (C.fuses(B, mode) ? B.nFused : 0);
+    C.lastSearches[0] = B.nSearches[0] - (C.fuseActive(mode) ? B.nFused : 0);
     C.lastSearches[1] = B.nSearches[1];
     for (int k = 0; k < 2 * kWaveStats; ++k) C.lastWaveStats[k] = B.hWaveStats[k];
     for (int k = 0; k < 4; ++k) C.lastKernelMs[k] = B.kernelMs[k];
@@ -1151,14 +1151,42 @@ static int eval_batch(Context* C, std::vector<RuleIR>& rules, std::vector<int32_
     }
     delete C->lastBatch;
     C->lastBatch = nullptr;
-    auto* B = new Batch();
-    B->magic = kMagicBatch;
-    B->thr = thr;
-    if (!batch_build(*C, *B, rules, st, flags, C->st) || !batch_run(*C, *B, out)) {
-        delete B;
-        return -1;
+    // One batch; or, when a phase's seeds exceed the heavy queue's range (a
+    // host check before any device work), its two halves, recursively. The
+    // last-pass figures then add up over the parts.
+    double ms[4] = {0, 0, 0, 0};
+    int64_t heavy = 0, giants = 0, fused = 0, shared = 0, searches[2] = {0, 0};
+    int parts = 0;
+    std::function<bool(size_t, size_t)> part = [&](size_t lo, size_t hi) -> bool {
+        std::vector<RuleIR> rs(rules.begin() + (ptrdiff_t)lo, rules.begin() + (ptrdiff_t)hi);
+        std::vector<int32_t> ss(st.begin() + (ptrdiff_t)lo, st.begin() + (ptrdiff_t)hi);
+        auto* B = new Batch();
+        B->magic = kMagicBatch;
+        B->thr = thr;
+        if (!batch_build(*C, *B, rs, ss, flags, C->st)) {
+            const bool split = B->seedsOverflow && hi - lo > 1;
+            delete B;
+            const size_t mid = lo + (hi - lo) / 2;
+            return split && part(lo, mid) && part(mid, hi);
+        }
+        if (!batch_run(*C, *B, out + lo)) {
+            delete B;
+            return false;
+        }
+        delete C->lastBatch;
+        C->lastBatch = B;
+        ms[0] += C->lastMs; ms[1] += C->lastSupMs; ms[2] += C->lastPairMs; ms[3] += C->lastSortMs;
+        heavy += C->lastHeavy; giants += C->lastGiants; fused += C->lastFused; shared += C->lastShared;
+        searches[0] += C->lastSearches[0]; searches[1] += C->lastSearches[1];
+        ++parts;
+        return true;
+    };
+    if (!part(0, rules.size())) return -1;
+    if (parts > 1) {
+        C->lastMs = ms[0]; C->lastSupMs = ms[1]; C->lastPairMs = ms[2]; C->lastSortMs = ms[3];
+        C->lastHeavy = heavy; C->lastGiants = giants; C->lastFused = fused; C->lastShared = shared;
+        C->lastSearches[0] = searches[0]; C->lastSearches[1] = searches[1];
     }
-    C->lastBatch = B;
     return 0;
 }
 
@@ -1553,6 +1581,11 @@ int gpu_set_tuning(void* ctx, const char* key, int64_t value) {
     } else if (k == "per_rule_giant_work") {
         // per-rule launches cut giant seeds for rules of at least this estimated work (0: every rule)
         C->perRuleGiantWork = (double)std::max<int64_t>(0, value);
+    } else if (k == "phase_seed_cap") {
+        // seeds one phase of a batch may hold (the heavy queue's range at most); one-shot
+        // evaluations split a larger batch in halves, gpu_batch_create refuses it
+        if (value < 1 || (uint64_t)value > kMaxPhaseSeeds) return -1;
+        C->phaseSeedCap = (uint64_t)value;
     } else if (k == "serve_heavy_work") {
         // servers started afterwards: estimated work above which a request runs in a heavy batch (0: off)
         C->serveHeavyWork = (double)std::max<int64_t>(0, value);
@@ -1608,6 +1641,7 @@ int64_t gpu_get_tuning(void* ctx, const char* key) {
     if (k == "wave_engine") return C->waveEngine;
     if (k == "serve_heavy_work") return (int64_t)C->serveHeavyWork;
     if (k == "per_rule_giant_work") return (int64_t)C->perRuleGiantWork;
+    if (k == "phase_seed_cap") return (int64_t)C->phaseSeedCap;
     if (k == "serve_heavy_percent") return C->serveHeavyPercent;
     if (k == "last_giants") return C->lastGiants;
     if (k == "last_fused") return C->lastFused;

@@ -211,14 +211,15 @@ struct HipFirst {
     bool ok() const { return e == hipSuccess; }
 };
 
-// Support thresholds of a gated batch: body pairs are counted only for rules
+// Support thresholds of a gated batch: body pairs are searched only for rules
 // whose support reaches minSupport and whose head coverage (support /
-// headSize, the IEEE quotient the results report) reaches minHeadCoverage.
-// The defaults (1, 0) are the gate of every gated batch: support > 0.
+// headSize, the IEEE quotient the results report) reaches minHeadCoverage --
+// except where the body-pair pass itself counts the support (fused rules),
+// whose body pairs are then reported only if the rule meets them. The
+// defaults (1, 0) are the gate of every gated batch: support > 0.
 struct Thresholds {
     int64_t minSupport = 1;
     double minHeadCoverage = 0.0;
-    bool active() const { return minSupport > 1 || minHeadCoverage > 0.0; }
 };
 
 // The smallest support that meets t for a rule of headSize head triples, at
@@ -232,6 +233,7 @@ struct Batch : Handle {
     int32_t n = 0;
     int32_t flags = 0;
     Thresholds thr;                 // gated batches: the gate of the pairs phase
+    bool seedsOverflow = false;     // batch_build refused: a phase's seeds exceed the context's cap
     std::vector<uint32_t> ruleThr;  // per rule: support_threshold (gated batches)
     std::vector<CompiledRule> comp;
     std::vector<int32_t> status;
@@ -332,6 +334,7 @@ struct Context : Handle {
     // Per-rule launches cut giant seeds into sub-items only for rules of at
     // least this estimated work (0: every rule).
     double perRuleGiantWork = 1e7;
+    uint64_t phaseSeedCap = kMaxPhaseSeeds;   // seeds per batch phase (tuning key phase_seed_cap)
     // Count the support of qualifying rules in the body-pair pass instead of
     // a support pass of their own; needs bitmap dedup (rows or LDS).
     bool fuseSupport = true;
@@ -342,9 +345,6 @@ struct Context : Handle {
     // one body-pair search.
     bool shareBodies = true;
     bool fuseActive(int mode) const { return fuseSupport && (mode == kDedupRows || mode == kDedupLds); }
-    // A batch with support thresholds above the default gate needs every
-    // rule's support before its pairs phase, so its pairs launch never fuses.
-    bool fuses(const Batch& B, int mode) const;
     int64_t lastGiants = 0;
     int64_t lastFused = 0;   // rules whose support the last batch counted in its pairs pass
     int64_t lastShared = 0;  // rules of the last batch that took another rule's body-pair count

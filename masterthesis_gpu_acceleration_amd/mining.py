@@ -9,8 +9,9 @@ grows when an atom is added, so no refinement of a dropped rule can pass
 Rewritten_CPP/SupportCounting.cpp:67-90). `Miner` runs that loop over the
 device: all candidates of one body length form one batch of
 `gpu_evaluate_rules_min`, which counts every candidate's support exactly and
-counts body pairs (for confidence) only for the candidates that meet the
-thresholds.
+reports body pairs (for confidence) only for the candidates that meet the
+thresholds, searching them only for those (a rule whose support the
+body-pair pass counts itself gets both from one search).
 
 Rules are the structured rules of `workloads` -- (head, body), atoms (s, p, o)
 with terms ("v", name) -- over variables only: the head is p(?a, ?b), and a
@@ -199,16 +200,21 @@ class GpuEvaluator:
         self.graph = graph
         self.miner = miner
         self.flags = flags
+        self.device_ms = []   # per call: device time of its batches (gpu_last_batch_stats)
 
     def __call__(self, rules):
         words = self.L.rule_words_for_graph(rules, self.graph)
         closed = np.array([is_closed(r) for r in rules], bool)
         out = np.zeros(len(rules), self.L.RESULT64_DTYPE)
+        ms = 0.0
         if closed.any():
             out[closed] = self.engine.evaluate_min(words[closed], self.miner.min_support, self.miner.min_hc,
                                                    self.flags)
+            ms += self.engine.last_stats()["total_ms"]
         if (~closed).any():
             out[~closed] = self.engine.evaluate_min(words[~closed], SUPPORT_ONLY, 0.0, self.flags)
+            ms += self.engine.last_stats()["total_ms"]
+        self.device_ms.append(ms)
         return out
 
 

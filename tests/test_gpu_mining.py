@@ -146,3 +146,41 @@ def test_miner_gpu_equals_oracle(case):
     assert st.candidates == st0.candidates and st.passed == st0.passed
     assert M.results_table(got) == M.results_table(exp)
     assert len(got) > 0
+
+
+def test_oversized_batches_split_in_halves(engines):
+    """A one-shot evaluation whose phase seeds exceed the context's cap
+    (forced low here with phase_seed_cap; the default is the heavy queue's
+    32-bit range) runs in halves, recursively, with the same results; a
+    resident batch over the cap is refused, and so is a cap below one
+    rule's seeds."""
+    say("split")
+    eng, fx, g, rules = engine_for(engines, "dbpedia_shaped_5k")
+    words = L.rule_words_for_graph(rules, g)
+    info = np.zeros((len(rules), 8), np.int64)
+    for i in range(len(rules)):
+        assert eng.lib.gpu_host_plan_info(eng.ctx, words[i].ctypes.data, info[i].ctypes.data) == 0
+    biggest = int(info[:, 5:7].max())
+    total = int(info[:, 5:7].sum(axis=0).max())
+    assert total > 4 * biggest
+    full = [eng.evaluate(words, f).copy() for f in (L.UNGATED, 0)]
+    pruned = eng.evaluate_min(words, 3, 0.02).copy()
+    cap = eng.get_tuning("phase_seed_cap")
+    try:
+        eng.set_tuning("phase_seed_cap", biggest)
+        for f, exp in zip((L.UNGATED, 0), full):
+            res = eng.evaluate(words, f)
+            check(fx, rules, res, not (f & L.UNGATED), "split/f%d" % f)
+            for k in ("support", "bodyPairs", "bodySize", "headSize"):
+                assert np.array_equal(res[k], exp[k]), k
+        res = eng.evaluate_min(words, 3, 0.02)
+        for k in ("support", "bodyPairs", "bodySize", "confidence"):
+            assert np.array_equal(res[k], pruned[k]), k
+        with pytest.raises(RuntimeError):
+            eng.batch(words, 0)
+        eng.set_tuning("phase_seed_cap", max(1, biggest // 2))
+        with pytest.raises(RuntimeError):
+            eng.evaluate(words, 0)
+    finally:
+        eng.set_tuning("phase_seed_cap", cap)
+    check(fx, rules, eng.evaluate(words, 0), True, "split/after")
