@@ -54,20 +54,17 @@ def is_closed(rule) -> bool:
 
 def canonical(rule):
     """A key equal for rules that differ only in the order of their body
-    atoms and the names of their non-head variables: the smallest encoding
-    over the body orders, each naming the other variables by first
-    appearance."""
+    atoms and the names of their non-head variables: over the numberings of
+    the non-head variables (2, 3, ...), the smallest sorted tuple of body
+    atoms (predicate, subject number, object number)."""
     head, body = rule
+    others = sorted({t[1] for a in body for t in (a[0], a[2])} - {HEAD_S, HEAD_O})
     best = None
-    for perm in itertools.permutations(body):
-        names = {HEAD_S: 0, HEAD_O: 1}
-        enc = []
-        for s, p, o in perm:
-            for t in (s, o):
-                if t[1] not in names:
-                    names[t[1]] = len(names)
-            enc.append((p, names[s[1]], names[o[1]]))
-        enc = tuple(enc)
+    for perm in itertools.permutations(range(2, 2 + len(others))):
+        names = dict(zip(others, perm))
+        names[HEAD_S] = 0
+        names[HEAD_O] = 1
+        enc = tuple(sorted((p, names[s[1]], names[o[1]]) for s, p, o in body))
         if best is None or enc < best:
             best = enc
     return (head[1], best)
