@@ -378,6 +378,20 @@ def measure(args, name, wl, decompose):
         }
         log("%s: parity vs golden: %s" % (name, block["parity_vs_golden"]))
 
+        # pre-dedup groundings: what the reference's body-size kernel would emit for these rules,
+        # counted on the host (this engine never enumerates them); too many to count for high fan-out
+        if name != "hub_fanout":
+            t = time.perf_counter()
+            gr = eng.body_groundings(words)
+            cnt = int(gr[gr >= 0].sum())
+            block["body_groundings"] = {
+                "per_step": cnt, "rules_counted": int((gr >= 0).sum()),
+                "per_s_equivalent": round(cnt / (ms_step / 1000.0), 1),
+                "what": "injective body groundings of the rules (one pair each in the reference's body-size kernel, "
+                        "before dedup), counted on the host; per second of this engine's step",
+                "count_s": round(time.perf_counter() - t, 3)}
+            log("%s: %d body groundings per step (%.3g per second of the step)" % (name, cnt, cnt / (ms_step / 1000.0)))
+
         # end to end: compile + upload + run + readback of a fresh batch (PCIe-inclusive; never `value`)
         t = time.perf_counter()
         eng.evaluate(words, flags)

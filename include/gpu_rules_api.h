@@ -362,6 +362,14 @@ int gpu_host_plan_info(void* handle, const int32_t* words, int64_t* info);
  * device. 0 or -1. No counterpart in the reference: it is for callers that
  * schedule rule batches. */
 int gpu_host_rules_work(void* handle, const int32_t* words, int32_t n, double* work);
+/* Injective body groundings of n structured rules: the (head subject, head
+ * object) pairs the reference's body-size kernel emits before it
+ * deduplicates them (GPU_03/gpu_kernels.cuh:218-317), counted on the host
+ * threads (a search whose last atom's row is counted, not enumerated); -1
+ * for a rule with constants, more than 8 body atoms or a head variable
+ * missing from its body. For measurement: this library never enumerates
+ * them. 0 or -1. */
+int gpu_host_body_groundings(void* handle, const int32_t* words, int32_t n, int64_t* out);
 
 /* Library version string. */
 const char* gpu_rules_version(void);

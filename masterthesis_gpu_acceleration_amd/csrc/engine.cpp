@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <climits>
@@ -1786,6 +1787,36 @@ int gpu_host_rules_work(void* handle, const int32_t* words, int32_t n, double* w
     for (int32_t i = 0; i < n; ++i)
         work[i] = estimate_work(compile_rule(rule_from_words(words + (size_t)i * GPU_RULE_WORDS), *g, t), *g, st);
     return 0;
+}
+
+int gpu_host_body_groundings(void* handle, const int32_t* words, int32_t n, int64_t* out) {
+    const HostGraph* g = as_graph(handle);
+    if (!g || n < 0 || (n > 0 && (!words || !out))) return -1;
+    try {
+        std::atomic<int32_t> next{0};
+        std::atomic<bool> failed{false};
+        auto work = [&]() {
+            try {
+                for (;;) {
+                    const int32_t i = next.fetch_add(1);
+                    if (i >= n) break;
+                    out[i] = body_groundings(rule_from_words(words + (size_t)i * GPU_RULE_WORDS), *g);
+                }
+            } catch (...) {
+                failed = true;
+            }
+        };
+        const unsigned hw = std::thread::hardware_concurrency();
+        const int nt = (int)std::max(1u, std::min(hw ? hw : 1u, 16u));
+        std::vector<std::thread> pool;
+        for (int k = 1; k < nt; ++k) pool.emplace_back(work);
+        work();
+        for (auto& th : pool) th.join();
+        return failed ? -1 : 0;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "[gpu_rules] gpu_host_body_groundings error: %s\n", e.what());
+        return -1;
+    }
 }
 
 int gpu_host_plan_info(void* handle, const int32_t* words, int64_t* info) {

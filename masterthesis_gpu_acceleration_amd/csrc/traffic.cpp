@@ -395,3 +395,143 @@ void plan_traffic_many(const std::vector<TrafficJob>& jobs, const HostGraph& g, 
 }
 
 }  // namespace gr
+
+namespace gr {
+
+namespace {
+
+// Injective groundings of a rule body (variables only) by depth-first search
+// over its atoms, each atom taken when it has a bound term (the one with the
+// fewest triples first); a last atom that binds one fresh variable is
+// counted without enumerating its row: the row's entries minus the bound
+// values it holds (a row holds each value once).
+struct GroundingCounter {
+    const HostGraph& g;
+    const std::vector<AtomIR>& body;
+    std::vector<int32_t> pred;    // dense predicate per atom
+    std::vector<int> order;       // atoms in search order
+    std::vector<int32_t> val;     // per variable: bound value or -1
+    std::vector<int32_t> boundVals;
+    int64_t count = 0;
+
+    int32_t vcol(int dir, int64_t e) const { return dir == 0 ? g.spoVal[(size_t)e] : g.posVal[(size_t)e]; }
+    bool taken(int32_t x) const {
+        for (int32_t y : val)
+            if (y == x) return true;
+        return false;
+    }
+    bool in_row(int dir, std::pair<int32_t, int32_t> r, int32_t x) const {
+        int32_t lo = r.first, hi = r.second;
+        while (lo < hi) {
+            const int32_t m = (lo + hi) >> 1;
+            if (vcol(dir, m) < x) lo = m + 1; else hi = m;
+        }
+        return lo < r.second && vcol(dir, lo) == x;
+    }
+    void dfs(size_t k) {
+        if (k == order.size()) {
+            ++count;
+            return;
+        }
+        const AtomIR& a = body[(size_t)order[k]];
+        const int32_t p = pred[(size_t)order[k]];
+        const int sv = a.s.v, ov = a.o.v;
+        const bool sb = val[(size_t)sv] >= 0, ob = val[(size_t)ov] >= 0;
+        if (sb && ob) {
+            if (g.has(p, val[(size_t)sv], val[(size_t)ov])) dfs(k + 1);
+            return;
+        }
+        if (sb || ob) {
+            const int dir = sb ? 0 : 1;   // key = the bound term
+            const int32_t key = sb ? val[(size_t)sv] : val[(size_t)ov];
+            const int fv = sb ? ov : sv;
+            if (fv == (sb ? sv : ov)) {   // p(x, x) with x bound
+                if (g.has(p, key, key)) dfs(k + 1);
+                return;
+            }
+            const auto r = g.row(dir, p, key);
+            if (k + 1 == order.size()) {   // last atom: count the row
+                int64_t n = r.second - r.first;
+                for (int32_t y : val)
+                    if (y >= 0 && in_row(dir, r, y)) --n;
+                count += n;
+                return;
+            }
+            for (int32_t e = r.first; e < r.second; ++e) {
+                const int32_t x = vcol(dir, e);
+                if (taken(x)) continue;
+                val[(size_t)fv] = x;
+                dfs(k + 1);
+                val[(size_t)fv] = -1;
+            }
+            return;
+        }
+        // neither bound: every triple of the predicate
+        for (int32_t e = g.predOff[(size_t)p]; e < g.predOff[(size_t)p + 1]; ++e) {
+            const int32_t s = g.spoKey[(size_t)e], o = g.spoVal[(size_t)e];
+            if (sv == ov) {
+                if (s != o || taken(s)) continue;
+                val[(size_t)sv] = s;
+                dfs(k + 1);
+                val[(size_t)sv] = -1;
+                continue;
+            }
+            if (s == o || taken(s) || taken(o)) continue;
+            val[(size_t)sv] = s;
+            val[(size_t)ov] = o;
+            dfs(k + 1);
+            val[(size_t)sv] = val[(size_t)ov] = -1;
+        }
+    }
+};
+
+}  // namespace
+
+int64_t body_groundings(const RuleIR& r, const HostGraph& g) {
+    if (r.body.empty() || r.body.size() > (size_t)kMaxBody) return -1;
+    if (!r.head.s.var || !r.head.o.var) return -1;
+    int32_t nv = 0;
+    bool hs = false, ho = false;
+    for (const AtomIR& a : r.body) {
+        if (!a.s.var || !a.o.var) return -1;
+        nv = std::max({nv, a.s.v + 1, a.o.v + 1});
+        hs = hs || a.s.v == r.head.s.v || a.o.v == r.head.s.v;
+        ho = ho || a.s.v == r.head.o.v || a.o.v == r.head.o.v;
+    }
+    if (!hs || !ho) return -1;
+    GroundingCounter c{g, r.body, {}, {}, std::vector<int32_t>((size_t)nv, -1), {}, 0};
+    for (const AtomIR& a : r.body) {
+        const int32_t p = g.dense_pred(a.pterm);
+        if (p < 0) return 0;
+        c.pred.push_back(p);
+    }
+    // order: the atom with the fewest triples, then atoms with a bound term
+    // (both bound first, then the fewest triples)
+    std::vector<char> used(r.body.size(), 0), bound((size_t)nv, 0);
+    for (size_t k = 0; k < r.body.size(); ++k) {
+        int best = -1;
+        int64_t bestKey = 0;
+        for (size_t j = 0; j < r.body.size(); ++j) {
+            if (used[j]) continue;
+            const AtomIR& a = r.body[j];
+            const int nb = (bound[(size_t)a.s.v] ? 1 : 0) + (bound[(size_t)a.o.v] ? 1 : 0);
+            if (k > 0 && nb == 0) continue;
+            const int64_t key = (int64_t)(2 - nb) * (int64_t(1) << 40) + g.nnz(c.pred[j]);
+            if (best < 0 || key < bestKey) {
+                best = (int)j;
+                bestKey = key;
+            }
+        }
+        if (best < 0) {   // a part of the body not joined to the rest: take it as a new start
+            for (size_t j = 0; j < r.body.size(); ++j)
+                if (!used[j] && (best < 0 || g.nnz(c.pred[j]) < g.nnz(c.pred[(size_t)best]))) best = (int)j;
+        }
+        used[(size_t)best] = 1;
+        bound[(size_t)r.body[(size_t)best].s.v] = bound[(size_t)r.body[(size_t)best].o.v] = 1;
+        c.order.push_back(best);
+    }
+    c.dfs(0);
+    return c.count;
+}
+
+}  // namespace gr

@@ -5,6 +5,7 @@
 
 #include "graph_host.hpp"
 #include "plan.hpp"
+#include "rules_host.hpp"
 
 namespace gr {
 
@@ -26,5 +27,12 @@ struct TrafficJob {
 };
 void plan_traffic_many(const std::vector<TrafficJob>& jobs, const HostGraph& g, std::vector<TrafficEstimate>& out,
                        int threads);
+
+// Injective groundings of r's body: the pairs the reference's body-size
+// kernel emits before they are deduplicated (GPU_03/gpu_kernels.cuh:
+// 218-317, which emits one (head subject, head object) pair per grounding).
+// -1 for a body with constants, more than kMaxBody atoms, or without both
+// head variables (no such pairs).
+int64_t body_groundings(const RuleIR& r, const HostGraph& g);
 
 }  // namespace gr

@@ -92,6 +92,7 @@ _SIGS = {
     "gpu_host_rules_work": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]),
     "gpu_host_count_action": (ctypes.c_int32, [ctypes.c_int32, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64]),
     "gpu_host_support_threshold": (ctypes.c_int64, [ctypes.c_int64, ctypes.c_int64, ctypes.c_double]),
+    "gpu_host_body_groundings": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]),
     "gpu_host_term_string": (ctypes.c_int64, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_char_p, ctypes.c_int64]),
     "gpu_rules_version": (ctypes.c_char_p, []),
 }
@@ -157,6 +158,14 @@ class GpuRulesLib:
         self.lib.gpu_free_graph(ctx)
 
 
+def _body_groundings(lib, handle, rule_words):
+    w = np.ascontiguousarray(rule_words, np.int32)
+    out = np.zeros(w.shape[0], np.int64)
+    if lib.gpu_host_body_groundings(handle, w.ctypes.data, int(w.shape[0]), out.ctypes.data) != 0:
+        raise RuntimeError("gpu_host_body_groundings failed")
+    return out
+
+
 class HostGraph:
     """Host-only graph handle (no device): planner and traffic model."""
 
@@ -205,6 +214,11 @@ class HostGraph:
         if self.lib.gpu_plan_traffic(self.h, w.ctypes.data, int(w.shape[0]), out.ctypes.data) != 0:
             raise RuntimeError("gpu_plan_traffic failed")
         return out
+
+    def body_groundings(self, rule_words: np.ndarray) -> np.ndarray:
+        """Per rule: injective body groundings -- the pairs the reference's
+        body-size kernel emits before dedup (-1 where not defined)."""
+        return _body_groundings(self.lib, self.h, rule_words)
 
     def work(self, rule_words: np.ndarray) -> np.ndarray:
         """Per rule: estimated candidates its searches enumerate (the
@@ -293,6 +307,10 @@ class Engine:
         if rc != 0:
             raise RuntimeError("gpu_evaluate_rules_min returned %d" % rc)
         return out
+
+    def body_groundings(self, rule_words: np.ndarray) -> np.ndarray:
+        """gpu_host_body_groundings on this context's graph (host threads)."""
+        return _body_groundings(self.lib, self.ctx, rule_words)
 
     def evaluate_text(self, rules, is_json: bool = False, flags: int = 0) -> np.ndarray:
         n = len(rules)
