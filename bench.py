@@ -616,11 +616,11 @@ class _UnprunedEvaluator:
         self.eng, self.g = eng, g
         self.device_ms = []
 
-    def __call__(self, rules):
+    def __call__(self, records):
         from masterthesis_gpu_acceleration_amd import mining as M
-        words = L.rule_words_for_graph(rules, self.g)
-        closed = np.array([M.is_closed(r) for r in rules], bool)
-        out = np.zeros(len(rules), L.RESULT64_DTYPE)
+        words = M.words_of_keys(records, self.g.n_entities)
+        closed = M.closed_mask(records)
+        out = np.zeros(len(records), L.RESULT64_DTYPE)
         ms = 0.0
         if closed.any():
             out[closed] = self.eng.evaluate(words[closed], 0)

@@ -370,6 +370,23 @@ int gpu_host_rules_work(void* handle, const int32_t* words, int32_t n, double* w
  * missing from its body. For measurement: this library never enumerates
  * them. 0 or -1. */
 int gpu_host_body_groundings(void* handle, const int32_t* words, int32_t n, int64_t* out);
+/* One refinement level of an AMIE-style rule miner (no counterpart in the
+ * reference, which leaves mining to its Scala caller). A rule p(?a, ?b) <=
+ * body over variables is a key record of GPU_RULE_KEY_WORDS int32: head
+ * predicate, atom count n, then n atoms (predicate, subject, object) with
+ * variables numbered 0 (?a), 1 (?b), 2, 3, ..., the rest zero; canonical when
+ * its atoms, over the numberings of the non-head variables, form the
+ * smallest sorted sequence. Writes the canonical keys of every refinement of
+ * the n parents -- one more atom over the nPreds body predicates, closing (two
+ * variables of the rule) or dangling (one variable and a fresh one), in
+ * either orientation, no repeated atom or copy of the head atom, at most
+ * maxAtoms atoms, and no rule left with more variables occurring once than
+ * the remaining atoms can close -- sorted, without repeats, up to cap
+ * records into out. Returns their number (more than cap: call again with
+ * room), or -1 on a bad argument. */
+#define GPU_RULE_KEY_WORDS 26
+int64_t gpu_host_refine_rules(const int32_t* parents, int32_t n, const int32_t* preds, int32_t nPreds,
+                              int32_t maxAtoms, int32_t* out, int64_t cap);
 
 /* Library version string. */
 const char* gpu_rules_version(void);

@@ -34,6 +34,7 @@ HIPFLAGS = ["-x", "hip", "--offload-arch=" + ARCH, "-Xclang", "-target-feature",
             "-fno-gpu-rdc"]
 
 SOURCES = ["graph_host.cpp", "rules_host.cpp", "planner.cpp", "traffic.cpp", "snapshot.cpp", "engine.cpp", "serve.cpp",
+           "refine.cpp",
            "kernels.hip", "search_s4.hip", "search_s8.hip", "search_s16.hip", "sort_unique.hip", "graph_device.hip",
            "noalloc.hip"]
 

@@ -21,6 +21,7 @@
 #include "engine_internal.hpp"
 #include "graph_device.hpp"
 #include "noalloc.hpp"
+#include "refine.hpp"
 #include "snapshot.hpp"
 #include "traffic.hpp"
 
@@ -1815,6 +1816,16 @@ int gpu_host_body_groundings(void* handle, const int32_t* words, int32_t n, int6
         return failed ? -1 : 0;
     } catch (const std::exception& e) {
         std::fprintf(stderr, "[gpu_rules] gpu_host_body_groundings error: %s\n", e.what());
+        return -1;
+    }
+}
+
+int64_t gpu_host_refine_rules(const int32_t* parents, int32_t n, const int32_t* preds, int32_t nPreds,
+                              int32_t maxAtoms, int32_t* out, int64_t cap) {
+    try {
+        return refine_rules(parents, n, preds, nPreds, maxAtoms, out, cap);
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "[gpu_rules] gpu_host_refine_rules error: %s\n", e.what());
         return -1;
     }
 }

@@ -93,6 +93,8 @@ _SIGS = {
     "gpu_host_count_action": (ctypes.c_int32, [ctypes.c_int32, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64]),
     "gpu_host_support_threshold": (ctypes.c_int64, [ctypes.c_int64, ctypes.c_int64, ctypes.c_double]),
     "gpu_host_body_groundings": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]),
+    "gpu_host_refine_rules": (ctypes.c_int64, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32,
+                                               ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64]),
     "gpu_host_term_string": (ctypes.c_int64, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_char_p, ctypes.c_int64]),
     "gpu_rules_version": (ctypes.c_char_p, []),
 }

@@ -23,15 +23,19 @@ class OracleEvaluator:
         self.min_hc = float(min_head_coverage)
         self.calls = 0
 
-    def __call__(self, rules):
-        """Support of every rule; body pairs of the closed rules that meet
-        the thresholds (the miner reads no other rule's)."""
+    def __call__(self, keys):
+        """Support of every rule (key records as the miner passes them, or
+        canonical keys); body pairs of the closed rules that meet the
+        thresholds (the miner reads no other rule's)."""
         self.calls += 1
+        if isinstance(keys, np.ndarray):
+            keys = [M.record_key(r) for r in keys]
+        rules = [M.rule_from_key(k) for k in keys]
         words = W.encode_rules(rules)
         exp = self.oracle.eval_support(words)
         sup, hs = exp[:, 0], exp[:, 1]
         hc0 = np.where(hs > 0, sup.astype(np.float64) / np.maximum(hs, 1).astype(np.float64), 0.0)
-        full = (sup >= self.min_support) & (hc0 >= self.min_hc) & np.array([M.is_closed(r) for r in rules], bool)
+        full = (sup >= self.min_support) & (hc0 >= self.min_hc) & np.array([M.key_closed(k) for k in keys], bool)
         if full.any():
             exp[full] = self.oracle.eval(words[full])
         sup, hs, hsup, pairs = exp[:, 0], exp[:, 1], exp[:, 2], exp[:, 3]
@@ -90,8 +94,7 @@ def all_closed_rules(head_predicates, body_predicates, max_atoms):
 def expected_table(g, head_predicates, body_predicates, max_atoms, min_support, min_hc, min_conf):
     """The miner's expected output by exhaustive enumeration and full counts."""
     keys = sorted(all_closed_rules(head_predicates, body_predicates, max_atoms))
-    rules = [M.rule_from_key(k) for k in keys]
-    res = OracleEvaluator(g, min_support, min_hc)(rules)
+    res = OracleEvaluator(g, min_support, min_hc)(keys)
     table = {}
     for k, r in zip(keys, res):
         sup = int(r["support"])

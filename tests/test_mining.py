@@ -95,3 +95,40 @@ def test_support_threshold_arithmetic():
     assert f(0, 1, 0.1) == 2 ** 31 - 1
     assert f(10, 1, 1.5) == -1 and f(10, 1, -0.1) == -1 and f(10, 1, math.nan) == -1 and f(-1, 1, 0.0) == -1
     assert f(10, 2 ** 40, 0.0) == 2 ** 31 - 1
+
+
+def test_words_of_keys_mean_the_same_rules():
+    """The miner's rule words (variables by canonical number) and the
+    structured rules' words (variables by first appearance) give the same
+    counts."""
+    from oracle import Oracle
+    g = W.tiny_random(4471, n_entities=10, n_predicates=3, n_triples=60)
+    keys = sorted(M.key_refinements((1, ()), [0, 1, 2], 3))
+    keys += sorted(set().union(*(M.key_refinements(k, [0, 2], 3) for k in keys[:6])))
+    o = Oracle(g.s, g.p, g.o)
+    a = o.eval(M.words_of_keys(keys, 0))
+    b = o.eval(W.encode_rules([M.rule_from_key(k) for k in keys]))
+    assert (a[:, :5] == b[:, :5]).all()
+    assert M.words_of_keys(keys, g.n_entities)[:, 2].tolist() == [k[0] + g.n_entities for k in keys]
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_native_refinement_equals_python(seed):
+    """gpu_host_refine_rules (C++, host threads) against key_refinements, the
+    same refinement written in Python, on random parents of 0-3 atoms; also
+    closed_mask against key_closed and words_of_keys over records against
+    the same over keys."""
+    import random
+    rnd = random.Random(seed)
+    preds = [0, 2, 5, 7][: 2 + seed]
+    level = sorted(M.key_refinements((3, ()), preds, 4))
+    parents = [(3, ())] + rnd.sample(level, min(20, len(level)))
+    lvl2 = sorted(set().union(*(M.key_refinements(k, preds, 4) for k in parents[1:6])))
+    parents += rnd.sample(lvl2, min(40, len(lvl2)))
+    for max_atoms in (2, 3, 4):
+        want = sorted(set().union(*(M.key_refinements(k, preds, max_atoms) for k in parents)))
+        got = M.refine(M.key_records(parents), preds, max_atoms)
+        assert sorted(M.record_key(r) for r in got) == want
+        assert got.shape[0] == len(want)   # sorted and without repeats
+        assert M.closed_mask(got).tolist() == [M.key_closed(M.record_key(r)) for r in got]
+        assert (M.words_of_keys(got, 11) == M.words_of_keys([M.record_key(r) for r in got], 11)).all()
