@@ -87,6 +87,10 @@ trait GpuRulesBatchNative extends Library {
                            minHeadCoverage: Double): Pointer
   // int64_t gpu_host_support_threshold(int64_t headSize, int64_t minSupport, double minHeadCoverage)
   def gpu_host_support_threshold(headSize: Long, minSupport: Long, minHeadCoverage: Double): Long
+  // int64_t gpu_host_refine_rules(const int32_t* parents, int32_t n, const int32_t* preds, int32_t nPreds,
+  //                               int32_t maxAtoms, int32_t* out, int64_t cap)
+  def gpu_host_refine_rules(parents: Pointer, n: Int, preds: Array[Int], nPreds: Int, maxAtoms: Int, out: Pointer,
+                            cap: Long): Long
   // int gpu_batch_run(void* batch, GpuRuleResult64* out)
   def gpu_batch_run(batch: Pointer, out: GpuRuleResult64): Int
   // void gpu_batch_free(void* batch)
