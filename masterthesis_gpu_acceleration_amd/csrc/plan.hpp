@@ -81,6 +81,11 @@ struct alignas(16) Plan {
     // hoLo | hoHi << 32, global row number hoRow0 of its first key, hoKeys keys.
     int32_t hoLo, hoHi, hoRow0, hoKeys;
     int32_t hoByEntry;   // 1: the pair level binds the head object as col[e], whose rank is crank[e]
+    // PAIRS, 1: level 0 is an EXT keyed by the seed over the seeds' own
+    // (direction, predicate), so seed i's level-0 row is row seedBegin + i
+    // (rows are numbered in the order of the keys array), read from rowStart
+    // without the key index
+    int32_t seedRow0;
     // PAIRS, supPred >= 0: the plan also counts the rule's support (when the
     // launch's `fuse` is set): a head triple (s, p_head, o) is supported iff
     // (s, o) is a distinct body pair, so each seed s adds the head objects it

@@ -99,6 +99,7 @@ struct Builder {
     // head-triple comparisons (support plans).
     void levels(const std::vector<int>& ord, Plan& P, ConflictRefs& R, bool withHead) {
         P.nlev = (int32_t)ord.size();
+        P.seedRow0 = 0;
         const int64_t T = g.T();
         for (size_t L = 0; L < ord.size(); ++L) {
             const AtomIR& a = r.body[ord[L]];
@@ -349,6 +350,10 @@ CompiledRule compile_rule(const RuleIR& r, const HostGraph& g, const Tuning& t) 
                     }
                 }
             if (Q.pairLevel >= 0) head_object_ranks(Q, g);
+            const Level& l0 = Q.lev[0];
+            const int64_t b0 = (int64_t)(uint32_t)l0.lo | ((int64_t)l0.hi << 32);
+            Q.seedRow0 = Q.nlev > 0 && level_kind(l0.flags) == kExt && l0.key == 0 &&
+                         level_dir(l0.flags) == bestDir && b0 == ((int64_t)bestDir * g.P() + bestP) * g.nBlocks();
         };
         finish(P, B);
         c.pairItems = P.seedEnd - P.seedBegin;

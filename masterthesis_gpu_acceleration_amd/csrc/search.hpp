@@ -563,6 +563,7 @@ __device__ __forceinline__ uint64_t thread_chunk(ThreadLdsT<levels_of(NS)>& W, c
     const int hoRef = uniform(P.hoRef);
     const int budget0 = uniform(P.budget);
     const bool pairs = K == kPairs && pl >= 0;
+    const bool seedRow0 = K == kPairs && uniform(P.seedRow0) != 0;
     // also count the seeds' supported head triples: of this plan's rule, or of
     // each rule in its member list (added to their counters directly)
     const int supPred = uniform(P.supPred);
@@ -588,7 +589,8 @@ __device__ __forceinline__ uint64_t thread_chunk(ThreadLdsT<levels_of(NS)>& W, c
                 item = begin + off;
                 active = bind_seed(P, G, (int)item, v);
                 if (active) {
-                    const Frame f0 = enter(W, P, 0, G, v);
+                    const int r0 = P.seedBegin + (int)item;
+                    const Frame f0 = seedRow0 ? Frame{G.rowStart[r0], G.rowStart[r0 + 1]} : enter(W, P, 0, G, v);
                     W.frame[0][lane] = f0;
                     d = 0;
                     // a distinct-pair search visits every level-0 entry at one

@@ -5,7 +5,8 @@
 // counts the bytes the search must read from the graph arrays:
 //   seed              8 B (SUPPORT: coo + col of the head entry) or 4 B (PAIRS key)
 //   row lookup        16 B key-index block per EXT/CHECK level entered,
-//                     plus 8 B of row bounds when the key has a row
+//                     plus 8 B of row bounds when the key has a row (a PAIRS
+//                     seed's level-0 row, when Plan::seedRow0, 8 B alone)
 //   CHECK probe       4 B per binary-search step (plus the final compare)
 //   EXT candidate     4 B (col)
 //   SCAN candidate    8 B (coo + col)
@@ -167,6 +168,7 @@ struct Walker {
             return;
         }
         key_row(base, tv(lv.key), lo, hi);
+        if (L == 0 && P.kind == kPairs && P.seedRow0) bytes -= 16;   // the seed's row from rowStart alone
         if (kind == kCheck) {
             const int target = tv(lv.val);
             int64_t l = lo, h = hi;
