@@ -727,6 +727,8 @@ __device__ __forceinline__ uint64_t thread_chunk(ThreadLdsT<levels_of(NS)>& W, c
                 }
             } else {
                 W.frame[d][lane].i = fr.i + 1;
+                // this level has nothing left after this candidate unless it descends
+                bool spent = fr.i + 1 >= fr.end;
                 if (--budget < 0) {
                     heavyNow = true;
                     active = false;
@@ -743,17 +745,30 @@ __device__ __forceinline__ uint64_t thread_chunk(ThreadLdsT<levels_of(NS)>& W, c
                                     active = false;
                                 } else {
                                     d = pl;   // head object settled: next candidate at the pair level
+                                    spent = false;
                                 }
                             } else {
                                 mine += 1;
                                 active = false;
                             }
                         } else {
-                            ++d;
-                            W.frame[d][lane] = enter(W, P, d, G, v);
+                            const Frame nf = enter(W, P, d + 1, G, v);
+                            if (nf.i < nf.end) {   // an empty row is not entered
+                                ++d;
+                                W.frame[d][lane] = nf;
+                                spent = false;
+                            }
                         }
                     }
                 }
+                // leave spent levels now rather than one per step (the seed's own
+                // level ends in the step that finds it spent)
+                if (active && spent)
+                    while (d > 0) {
+                        --d;
+                        const Frame up = W.frame[d][lane];
+                        if (up.i < up.end) break;
+                    }
             }
         }
         push_heavy(heavyNow, planIdx, item, hq, hCount, hCap);
