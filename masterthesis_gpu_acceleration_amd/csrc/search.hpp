@@ -591,6 +591,7 @@ __device__ __forceinline__ uint64_t thread_chunk(ThreadLdsT<levels_of(NS)>& W, c
                 if (active) {
                     const int r0 = P.seedBegin + (int)item;
                     const Frame f0 = seedRow0 ? Frame{G.rowStart[r0], G.rowStart[r0 + 1]} : enter(W, P, 0, G, v);
+                    active = f0.i < f0.end;   // an empty level-0 row: nothing to count
                     W.frame[0][lane] = f0;
                     d = 0;
                     // a distinct-pair search visits every level-0 entry at one
@@ -641,6 +642,7 @@ __device__ __forceinline__ uint64_t thread_chunk(ThreadLdsT<levels_of(NS)>& W, c
         }
         if (active) {
             const Frame fr = W.frame[d][lane];
+            bool spent = false;   // level d has nothing left to try after this step
             if (fr.i >= fr.end) {
                 if (d == 0) {
                     active = false;
@@ -661,6 +663,7 @@ __device__ __forceinline__ uint64_t thread_chunk(ThreadLdsT<levels_of(NS)>& W, c
                 // last level, plain EXT: up to kLeafBatchT candidates per step, straight-line
                 const int take = fr.end - fr.i < kLeafBatchT ? fr.end - fr.i : kLeafBatchT;
                 W.frame[d][lane].i = fr.i + take;
+                spent = fr.i + take >= fr.end;
                 budget -= take;
                 if (budget < 0) {
                     heavyNow = true;
@@ -695,6 +698,7 @@ __device__ __forceinline__ uint64_t thread_chunk(ThreadLdsT<levels_of(NS)>& W, c
                 // last level: up to kLeafBatchT candidates per step, loaded together
                 const int take = fr.end - fr.i < kLeafBatchT ? fr.end - fr.i : kLeafBatchT;
                 W.frame[d][lane].i = fr.i + take;
+                spent = fr.i + take >= fr.end;
                 budget -= take;
                 if (budget < 0) {
                     heavyNow = true;
@@ -722,13 +726,13 @@ __device__ __forceinline__ uint64_t thread_chunk(ThreadLdsT<levels_of(NS)>& W, c
                         } else if (d != pl) {
                             d = pl;   // head object settled: next candidate at the pair level
                             stop = true;
+                            spent = false;
                         }
                     }
                 }
             } else {
                 W.frame[d][lane].i = fr.i + 1;
-                // this level has nothing left after this candidate unless it descends
-                bool spent = fr.i + 1 >= fr.end;
+                spent = fr.i + 1 >= fr.end;   // unless the candidate descends
                 if (--budget < 0) {
                     heavyNow = true;
                     active = false;
@@ -761,15 +765,15 @@ __device__ __forceinline__ uint64_t thread_chunk(ThreadLdsT<levels_of(NS)>& W, c
                         }
                     }
                 }
-                // leave spent levels now rather than one per step (the seed's own
-                // level ends in the step that finds it spent)
-                if (active && spent)
-                    while (d > 0) {
-                        --d;
-                        const Frame up = W.frame[d][lane];
-                        if (up.i < up.end) break;
-                    }
             }
+            // leave spent levels now rather than one per step (the seed's own
+            // level ends in the step that finds it spent)
+            if (active && spent)
+                while (d > 0) {
+                    --d;
+                    const Frame up = W.frame[d][lane];
+                    if (up.i < up.end) break;
+                }
         }
         push_heavy(heavyNow, planIdx, item, hq, hCount, hCap);
     }
