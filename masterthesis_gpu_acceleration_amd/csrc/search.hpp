@@ -642,23 +642,11 @@ __device__ __forceinline__ uint64_t thread_chunk(ThreadLdsT<levels_of(NS)>& W, c
         }
         if (active) {
             const Frame fr = W.frame[d][lane];
-            bool spent = false;   // level d has nothing left to try after this step
+            bool spent = false;      // level d has nothing left to try after this step
+            bool finished = false;   // the seed's search is over
             if (fr.i >= fr.end) {
-                if (d == 0) {
-                    active = false;
-                    if (pairs) mine += (uint64_t)set.n;
-                    if (fs && set.n > 0) {
-                        const int nm = supList ? supList[0] : 1;
-                        for (int m = 0; m < nm; ++m) {
-                            const int pred = supList ? supList[2 + 2 * m] : supPred;
-                            const uint32_t h = head_hits(G, (int64_t)pred * G.nBlocks, v[0], set);
-                            if (!supList) sup += h;
-                            else if (h) atomicAdd(counts + 2 * supList[1 + 2 * m] + kSupport, (unsigned long long)h);
-                        }
-                    }
-                } else {
-                    --d;
-                }
+                if (d == 0) finished = true;
+                else --d;
             } else if (d + 1 == nlev && fr.end - fr.i > 1 && leaf_plain(P, d, pairs)) {
                 // last level, plain EXT: up to kLeafBatchT candidates per step, straight-line
                 const int take = fr.end - fr.i < kLeafBatchT ? fr.end - fr.i : kLeafBatchT;
@@ -766,14 +754,30 @@ __device__ __forceinline__ uint64_t thread_chunk(ThreadLdsT<levels_of(NS)>& W, c
                     }
                 }
             }
-            // leave spent levels now rather than one per step (the seed's own
-            // level ends in the step that finds it spent)
-            if (active && spent)
+            // leave spent levels now rather than one per step; a seed whose
+            // level-0 row is spent ends in this step
+            if (active && spent) {
+                finished = d == 0;
                 while (d > 0) {
                     --d;
                     const Frame up = W.frame[d][lane];
-                    if (up.i < up.end) break;
+                    finished = up.i >= up.end;
+                    if (!finished) break;
                 }
+            }
+            if (active && finished) {
+                active = false;
+                if (pairs) mine += (uint64_t)set.n;
+                if (fs && set.n > 0) {
+                    const int nm = supList ? supList[0] : 1;
+                    for (int m = 0; m < nm; ++m) {
+                        const int pred = supList ? supList[2 + 2 * m] : supPred;
+                        const uint32_t h = head_hits(G, (int64_t)pred * G.nBlocks, v[0], set);
+                        if (!supList) sup += h;
+                        else if (h) atomicAdd(counts + 2 * supList[1 + 2 * m] + kSupport, (unsigned long long)h);
+                    }
+                }
+            }
         }
         push_heavy(heavyNow, planIdx, item, hq, hCount, hCap);
     }
