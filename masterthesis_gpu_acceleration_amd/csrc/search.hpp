@@ -2223,10 +2223,11 @@ __device__ __forceinline__ int ls_place_sub(LsLdsT<NS>& W, const Dedup<D, NS>& d
 // Persistent thread mode over a chunk list: each wave repeatedly takes a
 // chunk of seeds of one plan; its lanes take the chunk's seeds one at a time.
 // For the 4-slot class the second launch bound asks for registers that let
-// 8 waves share a SIMD (the launch runs up to 32 waves per CU): its pairs
-// kernel then spills a few bytes per lane yet runs about 5% faster on wide,
-// shallow graphs. The larger slot classes need more registers than that.
-constexpr int kThreadMinWaves = 8;
+// 6 waves share a SIMD: its pairs kernel then keeps 78 registers and spills
+// 12 bytes per lane. With 8 (64 registers) it spilled 72 bytes and ran 6%
+// slower on wide, shallow graphs; with 7, 1% slower. The larger slot
+// classes need more registers than that.
+constexpr int kThreadMinWaves = 6;
 template <int K, int NS>
 __global__ void __launch_bounds__(kBlock, NS == 4 ? kThreadMinWaves : 1) k_thread_batch(ThreadArgs a, DevGraph G) {
     __shared__ ThreadLdsT<levels_of(NS)> lds[kWavesPerBlock];
