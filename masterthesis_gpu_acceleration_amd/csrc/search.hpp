@@ -191,7 +191,10 @@ __device__ __forceinline__ bool repeats(const ConflictRefs* __restrict__ R, cons
 
 // Distinct head objects of one seed, in registers (thread mode).
 struct RegSet {
-    static constexpr int kCap = 8;
+    // 12 entries: on the DBpedia-shaped batch 8 sent seeds with 9-12 head
+    // objects to wave mode (wave pass 0.23 -> 0.14 ms with 12); 16 cost the
+    // thread pass more registers than it saved
+    static constexpr int kCap = 12;
     int e[kCap];
     int n;
     __device__ __forceinline__ bool seen(int x) const {

@@ -22,7 +22,7 @@
 //                     head-row entries (4 B each) and a binary search each
 //   fused support     PAIRS plans with supPred >= 0, per seed with a body pair:
 //                     the seed's head row lookup (as a row lookup), then, as the device
-//                     checks it: up to 8 distinct head objects (a thread-mode
+//                     checks it: up to 12 distinct head objects (a thread-mode
 //                     seed) a binary search of the row (as a CHECK probe) per
 //                     object; more (only wave mode holds them) 20 B per row
 //                     entry (col + the key-index block of its rank)
@@ -318,7 +318,7 @@ struct Walker {
     // Fused support: the seed's head row, then one binary search of it per
     // distinct head object found (thread mode, at most kThreadSet objects),
     // or one pass over it (wave mode).
-    static constexpr int64_t kThreadSet = 8;
+    static constexpr int64_t kThreadSet = 12;
     void head_hits(int32_t pred) {
         int64_t n = 0;
         for (int64_t w : touched) n += __builtin_popcountll(bits[w]);
