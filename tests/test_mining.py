@@ -132,3 +132,26 @@ def test_native_refinement_equals_python(seed):
         assert got.shape[0] == len(want)   # sorted and without repeats
         assert M.closed_mask(got).tolist() == [M.key_closed(M.record_key(r)) for r in got]
         assert (M.words_of_keys(got, 11) == M.words_of_keys([M.record_key(r) for r in got], 11)).all()
+
+
+def test_new_entry_points_refuse_bad_arguments():
+    """The support-pruned and refinement entry points reject bad arguments
+    before any device work (no GPU is touched: a NULL context fails first)."""
+    import ctypes
+    import numpy as np
+    lib = L.load_library()
+    out = np.zeros(4, L.RESULT64_DTYPE)
+    w = np.zeros((1, 28), np.int32)
+    assert lib.gpu_evaluate_rules_min(None, w.ctypes.data, 1, 0, 1, 0.0, out.ctypes.data) == -1
+    assert not lib.gpu_batch_create_min(None, w.ctypes.data, 1, 0, 1, 0.0)
+    rec = np.zeros((2, M.KEY_WORDS), np.int32)
+    preds = np.array([0, 1], np.int32)
+    buf = np.zeros((4, M.KEY_WORDS), np.int32)
+    assert lib.gpu_host_refine_rules(rec.ctypes.data, -1, preds.ctypes.data, 2, 3, buf.ctypes.data, 4) == -1
+    assert lib.gpu_host_refine_rules(rec.ctypes.data, 2, preds.ctypes.data, 2, 0, buf.ctypes.data, 4) == -1
+    assert lib.gpu_host_refine_rules(rec.ctypes.data, 2, preds.ctypes.data, 2, 9, buf.ctypes.data, 4) == -1
+    assert lib.gpu_host_refine_rules(rec.ctypes.data, 2, preds.ctypes.data, 2, 3, None, 4) == -1
+    # room for none: the count alone (the caller then asks again with room)
+    n = lib.gpu_host_refine_rules(rec.ctypes.data, 1, preds.ctypes.data, 2, 3, None, 0)
+    assert n == len(M.key_refinements((0, ()), [0, 1], 3))
+    assert lib.gpu_host_body_groundings(None, w.ctypes.data, 1, ctypes.c_void_p(0)) == -1
