@@ -215,7 +215,8 @@ double gpu_copy_probe(void* ctx, int64_t bytes);
  * waves per CU, head-object bitmaps
  * in HBM rows or LDS (auto dedup only), and the steps before a wave-mode
  * seed is cut into sub-items -- budget and thread-mode waves as a grid,
- * the others one at a time, then budget and waves again -- and keeps the
+ * then thread-mode waves and work units per wave as a grid, the others one
+ * at a time, then budget and waves again -- and keeps the
  * fastest in the context. If `log` is given it receives up to logCap rows of
  * (budget, thread waves/CU, wave waves/CU, ms, escalated seeds, pairs dedup
  * mode, giant_after); *logLen gets the row count. 0 or -1. */

@@ -1390,16 +1390,25 @@ int gpu_autotune(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags, 
                     bestQ = q;
                 }
         C->tune.budget = budgets[bestB];
-        const int bestTw = tw[bestQ];
+        int bestTw = tw[bestQ];
         bestMs = grid[bestB][bestQ];
         C->threadWavesPerCU = bestTw;
-        // thread-mode work units per wave (unit size)
+        // thread-mode waves and work units per wave together: the unit size
+        // follows their product (units of 64 to 1,024 seeds, at least
+        // chunks_per_wave per wave), so one tuned alone fixes the other's
+        // best value -- waves that just fill the resident slots may want
+        // more units per wave than the grid above used
         int bestCpw = C->chunksPerWave;
-        for (int cpw : {2, 4, 8, 16}) {
-            C->chunksPerWave = cpw;
-            if (!trial(ms)) return -1;
-            if (ms < bestMs) { bestMs = ms; bestCpw = cpw; }
-        }
+        const int gridTw = bestTw, gridCpw = bestCpw;
+        for (int q = 0; q < nT; ++q)
+            for (int cpw : {1, 2, 4, 8, 16}) {
+                if (tw[q] == gridTw && cpw == gridCpw) continue;   // timed in the grid
+                C->threadWavesPerCU = tw[q];
+                C->chunksPerWave = cpw;
+                if (!trial(ms)) return -1;
+                if (ms < bestMs) { bestMs = ms; bestTw = tw[q]; bestCpw = cpw; }
+            }
+        C->threadWavesPerCU = bestTw;
         C->chunksPerWave = bestCpw;
         int bestWw = C->waveWavesPerCU;
         for (int w : ww) {

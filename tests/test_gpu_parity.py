@@ -386,7 +386,7 @@ def test_autotune_keeps_counts(name):
         assert len(log) >= 20, len(log)
         assert eng.get_tuning("budget") in (64, 128, 256, 1024, 4096, 16384, 65536)
         assert eng.get_tuning("thread_waves_per_cu") in (8, 16, 24, 32)
-        assert eng.get_tuning("chunks_per_wave") in (2, 4, 8, 16)
+        assert eng.get_tuning("chunks_per_wave") in (1, 2, 4, 8, 16)
         assert eng.get_tuning("wave_waves_per_cu") in (4, 8, 16)
         assert eng.get_tuning("giant_after") in (512, 1024, 2048, 4096, 8192)
         assert eng.get_tuning("use_graph") == 1
