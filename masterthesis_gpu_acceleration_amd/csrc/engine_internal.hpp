@@ -138,7 +138,7 @@ struct SortBufs {
 // sub-item of a giant clears its row). Counters, remaining counts and flags
 // are zeroed by fill kernels before each wave-mode phase.
 // Giants listed per wave-mode phase: up to kGiantCapMax (sub-items 96 B x
-// 256 per giant, 100 MB at the maximum), fewer when their bitmap rows would
+// 512 per giant, 201 MB at the maximum), fewer when their bitmap rows would
 // pass kGiantRowBudget, never fewer than kGiantCapMin.
 // A seed past the step cap when the list is full runs to its end as usual.
 constexpr uint32_t kGiantCapMin = 512;

@@ -73,7 +73,7 @@ struct alignas(16) SubItem {
     int32_t v[kMaxSlots];   // bindings above `level`
 };
 
-constexpr int kSubsPerGiant = 256;   // sub-items one giant is cut into, at most
+constexpr int kSubsPerGiant = 512;   // sub-items one giant is cut into, at most
 constexpr int kSubMinRow = 64;       // a level-0 row at least this long is cut at level 0
 
 struct GiantArgs {
