@@ -522,9 +522,26 @@ bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const s
     // their support chunks): a search's own rule alone (Plan::supPred = its
     // head predicate), or, when rules share the search, every qualifying one
     // through a member list in supTab (supPred <= -2).
+    // A gated rule whose threshold exceeds all its head triples cannot meet it
+    // (its support counts head triples): it needs no body-pair count, and its
+    // support comes from the support pass, an existence search per head
+    // triple, not from a body-pair search. A search other rules share still
+    // runs while one of them may meet its threshold.
+    std::vector<uint8_t> noPairs(rules.size(), 0);
+    if (gated)
+        for (size_t i = 0; i < rules.size(); ++i)
+            if (B.status[i] == kRuleOk &&
+                (int64_t)B.ruleThr[i] > std::max<int64_t>(B.comp[i].headSize, B.comp[i].headSupport))
+                noPairs[i] = 1;
+    auto searched = [&](size_t i) {
+        if (!noPairs[i]) return true;
+        for (int32_t m : members[i])
+            if (!noPairs[(size_t)m]) return true;
+        return false;
+    };
     auto fusable = [&](size_t i) {
         const CompiledRule& cr = B.comp[i];
-        return B.status[i] == kRuleOk && cr.supPred >= 0 && cr.supItems > 0;
+        return B.status[i] == kRuleOk && !noPairs[i] && cr.supPred >= 0 && cr.supItems > 0;
     };
     std::vector<uint8_t> fused(rules.size(), 0);
     std::vector<int32_t> supField(rules.size(), -1);
@@ -532,7 +549,7 @@ bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const s
     for (size_t i = 0; i < rules.size(); ++i) {
         if (B.status[i] != kRuleOk || B.pairsFrom[i] >= 0) continue;
         const CompiledRule& cr = B.comp[i];
-        if (!cr.pairsLaunch || cr.pairItems <= 0) continue;
+        if (!cr.pairsLaunch || cr.pairItems <= 0 || !searched(i)) continue;
         std::vector<int32_t> list;
         if (fusable(i)) list.push_back((int32_t)i);
         for (int32_t m : members[i])
@@ -555,7 +572,7 @@ bool batch_build(Context& C, Batch& B, const std::vector<RuleIR>& rules, const s
         const CompiledRule& cr = B.comp[i];
         if (cr.supportLaunch && cr.supItems > 0)
             add(0, i, cr.sup, cr.supRefs, cr.supWave, cr.supWaveRefs, cr.supItems, -1, fused[i] != 0);
-        if (B.pairsFrom[i] < 0 && cr.pairsLaunch && cr.pairItems > 0) {
+        if (B.pairsFrom[i] < 0 && cr.pairsLaunch && cr.pairItems > 0 && searched(i)) {
             add(1, i, cr.pairs, cr.pairsRefs, cr.pairsWave, cr.pairsWaveRefs, cr.pairItems, supField[i],
                 supField[i] != -1);
             if (!members[i].empty()) jobs.back().shared = true;

@@ -122,6 +122,23 @@ def test_threshold_boundaries(engines):
         eng.evaluate_min(one, 1, float("nan"))
 
 
+def test_support_only_runs_no_pair_search(engines):
+    """Rules whose support threshold exceeds every head triple cannot meet
+    it: a support-only evaluation runs no body-pair search at all, and the
+    supports are those of the full evaluation."""
+    say("support-only")
+    eng, fx, g, rules = engine_for(engines, "dbpedia_shaped_5k")
+    words = L.rule_words_for_graph(rules, g)
+    full = eng.evaluate(words, 0).copy()
+    res = eng.evaluate_min(words, M.SUPPORT_ONLY, 0.0)
+    assert eng.get_tuning("last_pair_searches") == 0
+    assert np.array_equal(res["support"], full["support"])
+    assert np.all(res["bodyPairs"][full["status"] == 0] == -1)
+    res = eng.evaluate_min(words, 1, 0.0)
+    assert eng.get_tuning("last_pair_searches") > 0
+    expect_pruned(full, res, thresholds_of(eng.lib, full["headSize"], 1, 0.0), "support-only/after")
+
+
 @pytest.mark.parametrize("case", [
     ("tiny", 2, 2, 3, 2, 0.05, 0.0),
     ("tiny", 3, 3, 3, 1, 0.0, 0.1),
