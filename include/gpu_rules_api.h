@@ -163,7 +163,11 @@ void gpu_batch_free(void* batch);
  * bodySize = 0 and confidence = 0. A rule whose support the body-pair pass
  * counts itself ("fuse_support"), or whose body-pair search other rules
  * with equal bodies share, has its body pairs searched whatever its support:
- * they are reported only if it meets the thresholds. flags: 0 or
+ * they are reported only if it meets the thresholds. A rule whose support
+ * threshold exceeds all its head triples cannot meet it and gets no
+ * body-pair search (unless a rule sharing that search may meet its own), so
+ * a support-only evaluation (minSupport above every head size) runs the
+ * support pass alone. flags: 0 or
  * GPU_RULES_PER_RULE; GPU_RULES_UNGATED, a NaN, negative or > 1
  * minHeadCoverage return -1 (NULL). */
 int gpu_evaluate_rules_min(void* ctx, const GpuRuleDesc* rules, int32_t n, int32_t flags, int64_t minSupport,
