@@ -171,20 +171,26 @@ def record_key(rec):
     return (int(rec[0]), tuple((int(rec[2 + 3 * j]), int(rec[3 + 3 * j]), int(rec[4 + 3 * j])) for j in range(n)))
 
 
+def _atoms_used(K):
+    """The largest atom count of the key records (0 for none)."""
+    return int(min(_MAX_ATOMS, max(0, int(K[:, 1].max())))) if K.shape[0] else 0
+
+
 def words_of_keys(keys, pred_offset=0):
     """GpuRuleDesc words (RULE_WORDS int32 each) of key records (or of
     canonical keys); predicate p becomes term p + pred_offset, variable k
     becomes -(k + 1)."""
     K = keys if isinstance(keys, np.ndarray) else key_records(keys)
     n = K.shape[0]
-    A = K[:, 2:].reshape(n, _MAX_ATOMS, 3)
-    m = np.arange(_MAX_ATOMS)[None, :] < K[:, 1:2]
+    na = _atoms_used(K)   # atom slots past every record's count stay zero
+    A = K[:, 2:2 + 3 * na].reshape(n, na, 3)
+    m = np.arange(na)[None, :] < K[:, 1:2]
     W = np.zeros((n, 28), np.int32)
     W[:, 0] = K[:, 1]
     W[:, 1] = -1
     W[:, 2] = K[:, 0] + pred_offset
     W[:, 3] = -2
-    B = W[:, 4:].reshape(n, _MAX_ATOMS, 3)
+    B = W[:, 4:4 + 3 * na].reshape(n, na, 3)
     B[:, :, 0] = np.where(m, -(A[:, :, 1] + 1), 0)
     B[:, :, 1] = np.where(m, A[:, :, 0] + pred_offset, 0)
     B[:, :, 2] = np.where(m, -(A[:, :, 2] + 1), 0)
@@ -196,8 +202,9 @@ def closed_mask(K):
     (each row's variable occurrences sorted: none may differ from both
     neighbours)."""
     n = K.shape[0]
-    A = K[:, 2:].reshape(n, _MAX_ATOMS, 3)
-    m = np.arange(_MAX_ATOMS)[None, :] < K[:, 1:2]
+    na = _atoms_used(K)
+    A = K[:, 2:2 + 3 * na].reshape(n, na, 3)
+    m = np.arange(na)[None, :] < K[:, 1:2]
     occ = np.concatenate([np.zeros((n, 1), np.int32), np.ones((n, 1), np.int32),
                           np.where(m, A[:, :, 1], -1), np.where(m, A[:, :, 2], -1)], axis=1)
     occ.sort(axis=1)
